@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: bead-steps/s of the SRK2 Brownian-dynamics step on one MI355X.
+
+Workload: 2D periodic box, 20k chains x 20 beads (400k beads), area fraction
+0.4, LJ with attraction (eps=1, cutoff 2.5 sigma), bending 5, chirality
+U[-0.5, 0.5] per chain, speed 1, T=0.05, dt=5e-4. One step = one SRK2 step
+(predictor + corrector) of every bead. `value` is the mixed form (FP32 pair
+forces, FP64 accumulation and state); the plain FP64 form is timed beside it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line. Inputs are resident in HBM before the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "gliding-collective-rotation_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per bead per launch (FP64 state, 2D): stage 0 reads x
+# and writes x1, stage 1 reads x1, the update reads and writes x -> 80 B per
+# bead-step in total
+ALG_BYTES = {"stage0": 32, "stage1": 16, "update": 32}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=64, help="steps per captured graph")
+    ap.add_argument("--rebuild", type=int, default=8, help="cell-list rebuild period")
+    ap.add_argument("--skin", type=float, default=0.0)
+    ap.add_argument("--chains", type=int, default=20000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU-baseline budget: steps are added in chunks of 40 until this much time")
+    ap.add_argument("--kernel-steps", type=int, default=32)
+    return ap.parse_args()
+
+
+def run_gpu(w, precision, steps, warmup, batch, rebuild, kernel_steps):
+    from gcr_amd import engine as E
+    n = w.n
+    pool = E.Pool(2, precision, bead_capacity=n, cell_capacity=4 * n + 1024,
+                  chain_capacity=w.phys.n_chains, replica_capacity=1)
+    pool.set_schedule(batch, rebuild)
+    rid = pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+    tv = w.time_vals.copy()
+    tv[7] = warmup
+    pool.begin(rid, tv)
+    pool.run()
+    st = pool.status(rid)
+    tv[0] = st["dt"]
+    tv[7] = steps
+    pool.begin(rid, tv)
+    t0 = time.perf_counter()
+    pool.run()  # returns after the device finished the last batch
+    t1 = time.perf_counter()
+    st2 = pool.status(rid)
+    assert st2["steps"] == steps, st2
+    sec = t1 - t0
+    # per-kernel durations with HIP events on the pool's stream
+    tv[0] = st2["dt"]
+    tv[7] = kernel_steps
+    pool.begin(rid, tv)
+    kt = pool.time_kernels(kernel_steps)
+    pos = pool.download(rid, pos1=False, tangent=False, forces=False)["pos"]
+    pool.close()
+    return dict(sec=sec, bead_steps_per_s=n * steps / sec, ms_per_step=1e3 * sec / steps,
+                kernels_ms=kt, replays=st2["replays"], dt=st2["dt"],
+                finite=bool(np.all(np.isfinite(pos))))
+
+
+def cpu_baseline(w, budget_s):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_lib as O
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    sysd = dict(dim=2, n=w.n, n_nearby=w.phys.n_nearby, has_springs=True, has_wca=True,
+                has_propel=True, L=w.phys.L, D=w.phys.D, Dr=w.phys.Dr, radius=w.phys.radius,
+                wca=np.asarray(w.phys.wca), spring=np.asarray(w.phys.spring), v0=w.phys.v0)
+    tv = w.time_vals.copy()
+    pos, tang = w.pos, w.tangent
+    done, sec = 0, 0.0
+    # chunks of 40 steps (one Verlet rebuild each, the reference's cadence)
+    # until about `budget_s` seconds of CPU work
+    while sec < budget_s and done < 100000:
+        tv[7] = 40
+        t0 = time.perf_counter()
+        pos, _, tang, _, st = O.srk2_run(sysd, pos, tang, w.chain_id, w.rest_lengths,
+                                         w.v_tan_angle, tv, xi=None, seed=1, step0=done,
+                                         pair_mode=1, n_threads=threads)
+        sec += time.perf_counter() - t0
+        tv[0] = st[0]
+        done += 40
+    return dict(value=w.n * done / sec, unit="bead-steps/s", cores=threads, kind="port",
+                sample=f"{done} SRK2 steps of the full {w.n}-bead system ({sec:.1f} s), C/OpenMP "
+                       f"FP64 restatement of the reference algorithm (cell grid + Verlet list "
+                       f"rebuilt every 40 steps, Newton-3 inside each thread chunk)")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from gcr_amd import engine as E
+    from gcr_amd import workloads as W
+    from gcr_amd._lib import device_info
+    w = W.config3(seed=1, n_chains=args.chains)
+    if args.skin > 0:
+        w.phys.skin = args.skin
+    dev = device_info()
+    res = {}
+    for name, prec in (("fp64", E.GCR_FP64), ("mixed", E.GCR_MIXED)):
+        res[name] = run_gpu(w, prec, args.steps, args.warmup, args.batch, args.rebuild,
+                            args.kernel_steps)
+    m = res["mixed"]
+    kt = m["kernels_ms"]
+    dom = max(("stage0", "stage1", "update"), key=lambda k: kt[k])
+    achieved = ALG_BYTES[dom] * w.n / (kt[dom] * 1e-3) / 1e9
+    out = {
+        "metric": "bead-steps/sec (400k beads, 1 GPU), mixed vs FP64",
+        "value": m["bead_steps_per_s"],
+        "unit": "bead-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces + f64 state",
+        "data": "synthetic (random non-overlapping chains from a seed)",
+        "config": {"workload": "2D periodic, 20k chains x 20 beads (400k), phi=0.4, LJ eps=1 "
+                               "cutoff 2.5 sigma, kappa=5, chirality U[-0.5,0.5]/chain, v0=1, "
+                               "T=0.05, dtmax=5e-4, SRK2",
+                   "beads": w.n, "box": w.phys.L, "steps_per_graph": args.batch,
+                   "rebuild_every": args.rebuild},
+        "fp64": {"value": res["fp64"]["bead_steps_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "kernels_ms": res["fp64"]["kernels_ms"]},
+        "mixed_over_fp64": m["bead_steps_per_s"] / res["fp64"]["bead_steps_per_s"],
+        "kernels_ms": kt,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "step_achieved": 80 * w.n / (m["ms_per_step"] * 1e-3) / 1e9},
+        "device": dev[0],
+        "replays": {k: v["replays"] for k, v in res.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,972 @@
+// gcr_capi.hip -- host side of libgcr.so: pools of replicas, device memory,
+// graph-captured step batches, replay of a batch when the cell-list skin was
+// exceeded, and the extern "C" entry points declared in include/gcr.h.
+#include "gcr_kernels.hip"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gcr.h"
+
+using namespace gcr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(GCR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+// First-fit allocator of index ranges (beads, cells, chains) inside a pool.
+struct Segs {
+    std::vector<std::pair<int64_t, int64_t>> fr;  // (start, length), sorted
+    void init(int64_t cap) { fr.assign(1, {0, cap}); }
+    int64_t alloc(int64_t n)
+    {
+        for (size_t i = 0; i < fr.size(); ++i)
+            if (fr[i].second >= n) {
+                int64_t s = fr[i].first;
+                fr[i].first += n;
+                fr[i].second -= n;
+                if (fr[i].second == 0) fr.erase(fr.begin() + i);
+                return s;
+            }
+        return -1;
+    }
+    void release(int64_t s, int64_t n)
+    {
+        auto it = std::lower_bound(fr.begin(), fr.end(), std::make_pair(s, (int64_t)0));
+        it = fr.insert(it, {s, n});
+        size_t i = it - fr.begin();
+        if (i + 1 < fr.size() && fr[i].first + fr[i].second == fr[i + 1].first) {
+            fr[i].second += fr[i + 1].second;
+            fr.erase(fr.begin() + i + 1);
+        }
+        if (i > 0 && fr[i - 1].first + fr[i - 1].second == fr[i].first) {
+            fr[i - 1].second += fr[i].second;
+            fr.erase(fr.begin() + i);
+        }
+    }
+};
+
+template <typename T>
+int dalloc(T** p, size_t count)
+{
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc((void**)p, count * sizeof(T)));
+    return GCR_OK;
+}
+
+int64_t cells_per_side(double L, double rc, double skin)
+{
+    int64_t nc = (int64_t)std::floor(L / (rc + skin));
+    return nc < 1 ? 1 : nc;
+}
+
+double default_skin(const gcr_replica_desc* d) { return d->skin > 0 ? d->skin : 0.3 * d->wca[3]; }
+
+}  // namespace
+
+struct gcr_pool {
+    int dim = 2, precision = GCR_FP64;
+    int64_t capB = 0, capC = 0, capCh = 0;
+    int capR = 0;
+    Segs segB, segC, segCh;
+    std::vector<RepParams> P;
+    std::vector<RepState> S;
+    std::vector<int> live;
+    std::vector<int> replays;
+    std::vector<double*> xi_dev;
+    int B = 32, K = 8;
+    bool tables_dirty = true;
+    hipGraphExec_t g_norm = nullptr, g_safe = nullptr;
+    int nblk = 0, nchk = 0;
+    hipStream_t stream = nullptr;
+    PoolArgs a{};
+    // device buffers owned by the pool
+    int32_t *blk_rep = nullptr, *blk_slot0 = nullptr, *chk_rep = nullptr, *chk_cell0 = nullptr,
+            *rep_chk_off = nullptr;
+    RepParams* dP = nullptr;
+    RepState *dS = nullptr, *dSs = nullptr;
+    double *X = nullptr, *X1 = nullptr, *F0 = nullptr, *FT = nullptr, *XI = nullptr, *T = nullptr,
+           *Xn = nullptr, *Tn = nullptr, *Xs = nullptr, *Ts = nullptr;
+    int32_t *META = nullptr, *LINK = nullptr, *CELL = nullptr, *SLOT = nullptr, *KEYOFF = nullptr,
+            *PAIRS = nullptr, *METAn = nullptr, *METAs = nullptr;
+    double *CHAIN_A = nullptr, *CHAIN_CS = nullptr;
+    int32_t *CHAIN_FIRST = nullptr, *CHAIN_LEN = nullptr;
+    int32_t *CCOUNT = nullptr, *CSTART = nullptr, *CHKSUM = nullptr;
+    int64_t capChk = 0, capBlk = 0;
+};
+
+namespace {
+
+void drop_graphs(gcr_pool* p)
+{
+    if (p->g_norm) (void)hipGraphExecDestroy(p->g_norm);
+    if (p->g_safe) (void)hipGraphExecDestroy(p->g_safe);
+    p->g_norm = p->g_safe = nullptr;
+}
+
+int upload_rep(gcr_pool* p, int r)
+{
+    HIPCHK(hipMemcpyAsync(p->dP + r, &p->P[r], sizeof(RepParams), hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->dS + r, &p->S[r], sizeof(RepState), hipMemcpyHostToDevice, p->stream));
+    return GCR_OK;
+}
+
+int pull_states(gcr_pool* p)
+{
+    HIPCHK(hipMemcpyAsync(p->S.data(), p->dS, sizeof(RepState) * p->capR, hipMemcpyDeviceToHost,
+                          p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return GCR_OK;
+}
+
+int push_states(gcr_pool* p)
+{
+    HIPCHK(hipMemcpyAsync(p->dS, p->S.data(), sizeof(RepState) * p->capR, hipMemcpyHostToDevice,
+                          p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return GCR_OK;
+}
+
+// Block and chunk tables of the live replicas.
+int rebuild_tables(gcr_pool* p)
+{
+    std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0);
+    for (int r = 0; r < p->capR; ++r) {
+        rco[r] = (int32_t)cr.size();
+        RepParams& P = p->P[r];
+        if (!p->live[r]) continue;
+        P.blk_off = (int32_t)br.size();
+        P.nblk = (P.n + BS - 1) / BS;
+        for (int b = 0; b < P.nblk; ++b) {
+            br.push_back(r);
+            bs.push_back(P.bead_off + b * BS);
+        }
+        for (int c = 0; c < P.ncell; c += SCAN_CHUNK) {
+            cr.push_back(r);
+            cc.push_back(P.cell_off + c);
+        }
+    }
+    rco[p->capR] = (int32_t)cr.size();
+    if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk)
+        return fail(GCR_ERR_CAPACITY, "block table capacity exceeded");
+    p->nblk = (int)br.size();
+    p->nchk = (int)cr.size();
+    if (p->nblk) {
+        HIPCHK(hipMemcpyAsync(p->blk_rep, br.data(), br.size() * 4, hipMemcpyHostToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(p->blk_slot0, bs.data(), bs.size() * 4, hipMemcpyHostToDevice, p->stream));
+    }
+    if (p->nchk) {
+        HIPCHK(hipMemcpyAsync(p->chk_rep, cr.data(), cr.size() * 4, hipMemcpyHostToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(p->chk_cell0, cc.data(), cc.size() * 4, hipMemcpyHostToDevice, p->stream));
+    }
+    HIPCHK(hipMemcpyAsync(p->rep_chk_off, rco.data(), rco.size() * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->dP, p->P.data(), sizeof(RepParams) * p->capR, hipMemcpyHostToDevice,
+                          p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->a.nblk = p->nblk;
+    p->a.nchk = p->nchk;
+    p->tables_dirty = false;
+    drop_graphs(p);
+    return GCR_OK;
+}
+
+// Kernel sequences (launched directly or captured into a graph).
+void launch_rebuild(gcr_pool* p)
+{
+    const PoolArgs& a = p->a;
+    hipStream_t st = p->stream;
+    if (p->dim == 2)
+        k_bin<2><<<p->nblk, BS, 0, st>>>(a);
+    else
+        k_bin<3><<<p->nblk, BS, 0, st>>>(a);
+    k_scan_chunks<<<p->nchk, BS, 0, st>>>(a);
+    k_scan_reps<<<p->capR, BS, 0, st>>>(a);
+    k_scan_final<<<p->nchk, BS, 0, st>>>(a);
+    k_scatter<<<p->nblk, BS, 0, st>>>(a);
+    k_cellsort<<<p->nchk, BS, 0, st>>>(a);
+    if (p->dim == 2) {
+        k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1);
+        k_links<2><<<p->nblk, BS, 0, st>>>(a, 1);
+    } else {
+        k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1);
+        k_links<3><<<p->nblk, BS, 0, st>>>(a, 1);
+    }
+}
+
+void launch_stage(gcr_pool* p, int which, int j)
+{
+    const PoolArgs& a = p->a;
+    hipStream_t st = p->stream;
+    const bool mixed = p->precision == GCR_MIXED;
+    if (p->dim == 2) {
+        if (which == 0)
+            mixed ? k_stage<2, float, 0><<<p->nblk, BS, 0, st>>>(a, j)
+                  : k_stage<2, double, 0><<<p->nblk, BS, 0, st>>>(a, j);
+        else if (which == 1)
+            mixed ? k_stage<2, float, 1><<<p->nblk, BS, 0, st>>>(a, j)
+                  : k_stage<2, double, 1><<<p->nblk, BS, 0, st>>>(a, j);
+        else
+            k_update<2><<<p->nblk, BS, 0, st>>>(a, j);
+    } else {
+        if (which == 0)
+            mixed ? k_stage<3, float, 0><<<p->nblk, BS, 0, st>>>(a, j)
+                  : k_stage<3, double, 0><<<p->nblk, BS, 0, st>>>(a, j);
+        else if (which == 1)
+            mixed ? k_stage<3, float, 1><<<p->nblk, BS, 0, st>>>(a, j)
+                  : k_stage<3, double, 1><<<p->nblk, BS, 0, st>>>(a, j);
+        else
+            k_update<3><<<p->nblk, BS, 0, st>>>(a, j);
+    }
+}
+
+void launch_begin(gcr_pool* p)
+{
+    if (p->dim == 2)
+        k_batch_begin<2><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
+    else
+        k_batch_begin<3><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
+}
+
+void launch_batch(gcr_pool* p, int K)
+{
+    launch_begin(p);
+    for (int j = 0; j < p->B; ++j) {
+        if (j % K == 0) launch_rebuild(p);
+        launch_stage(p, 0, j);
+        launch_stage(p, 1, j);
+        launch_stage(p, 2, j);
+    }
+}
+
+int capture(gcr_pool* p, int K, hipGraphExec_t* out)
+{
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    launch_batch(p, K);
+    HIPCHK(hipStreamEndCapture(p->stream, &g));
+    HIPCHK(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return GCR_OK;
+}
+
+int ensure_ready(gcr_pool* p)
+{
+    int rc;
+    if (p->tables_dirty && (rc = rebuild_tables(p)) != GCR_OK) return rc;
+    if (!p->g_norm && (rc = capture(p, p->K, &p->g_norm)) != GCR_OK) return rc;
+    if (!p->g_safe && (rc = capture(p, 1, &p->g_safe)) != GCR_OK) return rc;
+    return GCR_OK;
+}
+
+bool running(const RepState& s)
+{
+    if (s.frozen) return false;
+    bool a = s.max_steps < 0 || s.steps < s.max_steps;
+    bool b = s.mdu < 0 || s.elapsed < s.mdu;
+    return a && b;
+}
+
+int count_running(gcr_pool* p)
+{
+    int n = 0;
+    for (int r = 0; r < p->capR; ++r)
+        if (p->live[r] && running(p->S[r])) ++n;
+    return n;
+}
+
+// Run the safe (rebuild every step) batch for the replicas that exceeded
+// their skin, after restoring them to the start of the batch.
+int replay(gcr_pool* p)
+{
+    std::vector<int> which;
+    for (int r = 0; r < p->capR; ++r) {
+        if (!p->live[r]) continue;
+        if (p->S[r].violation) {
+            which.push_back(r);
+            p->S[r].frozen = 2;
+        } else {
+            p->S[r].frozen = 1;
+        }
+    }
+    int rc = push_states(p);
+    if (rc) return rc;
+    if (p->dim == 2)
+        k_restore<2><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
+    else
+        k_restore<3><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
+    k_restore_state<<<(p->capR + 63) / 64, 64, 0, p->stream>>>(p->a);
+    HIPCHK(hipGraphLaunch(p->g_safe, p->stream));
+    if ((rc = pull_states(p))) return rc;
+    bool again = false;
+    for (int r : which) {
+        p->replays[r] += 1;
+        if (p->S[r].violation) again = true;
+    }
+    for (int r = 0; r < p->capR; ++r)
+        if (p->live[r] && p->S[r].frozen == 1) p->S[r].frozen = 0;
+    if ((rc = push_states(p))) return rc;
+    if (again)
+        return fail(GCR_ERR_STATE,
+                    "a bead moved more than skin/2 within one step; increase the skin");
+    return GCR_OK;
+}
+
+int with_only(gcr_pool* p, int rid, bool on)
+{
+    // freeze every other live replica (diagnostics must not re-sort them)
+    int rc = pull_states(p);
+    if (rc) return rc;
+    for (int r = 0; r < p->capR; ++r)
+        if (p->live[r] && r != rid) p->S[r].frozen = on ? 1 : 0;
+    return push_states(p);
+}
+
+int check_rid(gcr_pool* p, int rid)
+{
+    if (!p) return fail(GCR_ERR_ARG, "null pool");
+    if (rid < 0 || rid >= p->capR || !p->live[rid]) return fail(GCR_ERR_ARG, "no such replica");
+    return GCR_OK;
+}
+
+int download_one(gcr_pool* p, int rid, const double* src, double* host, int width)
+{
+    const RepParams& P = p->P[rid];
+    k_unpermute<<<(P.n + BS - 1) / BS, BS, 0, p->stream>>>(src, p->Xn, p->SLOT, P.bead_off, P.n, width);
+    HIPCHK(hipMemcpyAsync(host, p->Xn, sizeof(double) * (size_t)P.n * width, hipMemcpyDeviceToHost,
+                          p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return GCR_OK;
+}
+
+int upload_perm(gcr_pool* p, int rid, const double* host, double* dst)
+{
+    const RepParams& P = p->P[rid];
+    HIPCHK(hipMemcpyAsync(p->Xn, host, sizeof(double) * (size_t)P.n * p->dim, hipMemcpyHostToDevice,
+                          p->stream));
+    k_permute_in<<<(P.n + BS - 1) / BS, BS, 0, p->stream>>>(p->Xn, dst, p->SLOT, P.bead_off, P.n, p->dim);
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return GCR_OK;
+}
+
+}  // namespace
+
+// =================================================================== C-ABI
+
+extern "C" {
+
+const char* gcr_last_error(void) { return g_err.c_str(); }
+
+int gcr_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_bytes)
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+    return GCR_OK;
+}
+
+int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_cells)
+{
+    // per bead: 9 dim-vectors + noise (2 dim) of doubles, 5 int pairs, links,
+    // cell and slot ints; per chain: 4 doubles + 2 ints; per cell: 2 ints
+    return n * (8 * (9 * dim + 2 * dim) + 4 * (5 * 2 + 4 + 2)) + n_chains * (8 * 4 + 8) +
+           (n_cells + 1) * 8;
+}
+
+int64_t gcr_replica_cells(int32_t dim, const gcr_replica_desc* d)
+{
+    if (!d) return -1;
+    int64_t nc = cells_per_side(d->L, d->wca[3], default_skin(d));
+    return dim == 3 ? nc * nc * nc : nc * nc;
+}
+
+int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, int64_t capCh,
+                    int32_t capR, gcr_pool** out)
+{
+    if (!out) return fail(GCR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail(GCR_ERR_ARG, "dim must be 2 or 3");
+    if (precision != GCR_FP64 && precision != GCR_MIXED) return fail(GCR_ERR_ARG, "bad precision");
+    if (capB <= 0 || capC <= 0 || capCh <= 0 || capR <= 0 || capB > (1ll << 30) || capC > (1ll << 30))
+        return fail(GCR_ERR_ARG, "capacities must be positive and below 2^30");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GCR_ERR_HIP, "no HIP device");
+    gcr_pool* p = new gcr_pool();
+    p->dim = dim;
+    p->precision = precision;
+    p->capB = capB;
+    p->capC = capC;
+    p->capCh = capCh;
+    p->capR = capR;
+    p->segB.init(capB);
+    p->segC.init(capC);
+    p->segCh.init(capCh);
+    p->P.assign(capR, RepParams{});
+    p->S.assign(capR, RepState{});
+    for (auto& s : p->S) s.frozen = 1;
+    p->live.assign(capR, 0);
+    p->replays.assign(capR, 0);
+    p->xi_dev.assign(capR, nullptr);
+    p->capBlk = capB / BS + capR + 1;
+    p->capChk = capC / SCAN_CHUNK + capR + 1;
+    int rc = GCR_OK;
+    auto A = [&](int r) {
+        if (r != GCR_OK && rc == GCR_OK) rc = r;
+    };
+    const size_t nb = (size_t)capB;
+    A(dalloc(&p->blk_rep, p->capBlk));
+    A(dalloc(&p->blk_slot0, p->capBlk));
+    A(dalloc(&p->chk_rep, p->capChk));
+    A(dalloc(&p->chk_cell0, p->capChk));
+    A(dalloc(&p->rep_chk_off, capR + 1));
+    A(dalloc(&p->dP, capR));
+    A(dalloc(&p->dS, capR));
+    A(dalloc(&p->dSs, capR));
+    for (double** q : {&p->X, &p->X1, &p->F0, &p->FT, &p->T, &p->Xn, &p->Tn, &p->Xs, &p->Ts})
+        A(dalloc(q, nb * dim));
+    A(dalloc(&p->XI, nb * 2 * dim));
+    for (int32_t** q : {&p->META, &p->KEYOFF, &p->PAIRS, &p->METAn, &p->METAs}) A(dalloc(q, nb * 2));
+    A(dalloc(&p->LINK, nb * 4));
+    A(dalloc(&p->CELL, nb));
+    A(dalloc(&p->SLOT, nb));
+    A(dalloc(&p->CHAIN_A, capCh));
+    A(dalloc(&p->CHAIN_CS, capCh * 3));
+    A(dalloc(&p->CHAIN_FIRST, capCh));
+    A(dalloc(&p->CHAIN_LEN, capCh));
+    A(dalloc(&p->CCOUNT, capC + 1));
+    A(dalloc(&p->CSTART, capC + 1));
+    A(dalloc(&p->CHKSUM, p->capChk));
+    if (rc == GCR_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(GCR_ERR_HIP, "stream creation failed");
+    if (rc == GCR_OK && hipMemset(p->CCOUNT, 0, sizeof(int32_t) * (capC + 1)) != hipSuccess)
+        rc = fail(GCR_ERR_HIP, "memset failed");
+    if (rc == GCR_OK && hipMemset(p->X1, 0, sizeof(double) * nb * dim) != hipSuccess)
+        rc = fail(GCR_ERR_HIP, "memset failed");
+    if (rc == GCR_OK && hipMemset(p->FT, 0, sizeof(double) * nb * dim) != hipSuccess)
+        rc = fail(GCR_ERR_HIP, "memset failed");
+    if (rc == GCR_OK && hipMemcpy(p->dS, p->S.data(), sizeof(RepState) * capR, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(GCR_ERR_HIP, "state upload failed");
+    if (rc != GCR_OK) {
+        gcr_pool_destroy(p);
+        return rc;
+    }
+    PoolArgs& a = p->a;
+    a.blk_rep = p->blk_rep;
+    a.blk_slot0 = p->blk_slot0;
+    a.chk_rep = p->chk_rep;
+    a.chk_cell0 = p->chk_cell0;
+    a.rep_chk_off = p->rep_chk_off;
+    a.nrep = capR;
+    a.P = p->dP;
+    a.S = p->dS;
+    a.X = p->X;
+    a.X1 = p->X1;
+    a.F0 = p->F0;
+    a.FT = p->FT;
+    a.XI = p->XI;
+    a.T = p->T;
+    a.META = p->META;
+    a.LINK = p->LINK;
+    a.CELL = p->CELL;
+    a.SLOT = p->SLOT;
+    a.CHAIN_A = p->CHAIN_A;
+    a.CHAIN_CS = p->CHAIN_CS;
+    a.CHAIN_FIRST = p->CHAIN_FIRST;
+    a.CHAIN_LEN = p->CHAIN_LEN;
+    a.CCOUNT = p->CCOUNT;
+    a.CSTART = p->CSTART;
+    a.CHKSUM = p->CHKSUM;
+    a.KEYOFF = p->KEYOFF;
+    a.PAIRS = p->PAIRS;
+    a.Xn = p->Xn;
+    a.Tn = p->Tn;
+    a.METAn = p->METAn;
+    a.Xs = p->Xs;
+    a.Ts = p->Ts;
+    a.METAs = p->METAs;
+    a.Ss = p->dSs;
+    *out = p;
+    return GCR_OK;
+}
+
+int gcr_pool_set_schedule(gcr_pool* p, int32_t steps_per_batch, int32_t rebuild_every)
+{
+    if (!p) return fail(GCR_ERR_ARG, "null pool");
+    if (steps_per_batch < 1 || rebuild_every < 1 || steps_per_batch > 4096)
+        return fail(GCR_ERR_ARG, "steps_per_batch in [1, 4096] and rebuild_every >= 1 required");
+    if (rebuild_every > steps_per_batch) rebuild_every = steps_per_batch;
+    p->K = rebuild_every;
+    p->B = ((steps_per_batch + rebuild_every - 1) / rebuild_every) * rebuild_every;
+    drop_graphs(p);
+    return GCR_OK;
+}
+
+int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, const double* tangent,
+                    const int64_t* chain_id, const double* rest_lengths, const double* v_tan_angle,
+                    const double* v0_per_chain, int32_t* rid)
+{
+    if (!p || !d || !pos || !chain_id || !rid) return fail(GCR_ERR_ARG, "null argument");
+    const int dim = p->dim;
+    const int64_t n = d->n_beads, nch = d->n_chains;
+    if (n < 1 || nch < 1 || nch > n) return fail(GCR_ERR_ARG, "need 1 <= n_chains <= n_beads");
+    if (!(d->L > 0) || !(d->radius > 0)) return fail(GCR_ERR_ARG, "L and radius must be positive");
+    if (d->spring[3] != 0.0) return fail(GCR_ERR_ARG, "FENE springs are not supported");
+    if (d->has_wca && !(d->wca[3] > 0 && 2 * d->wca[3] < d->L))
+        return fail(GCR_ERR_ARG, "WCA cutoff must be in (0, L/2)");
+    if (d->D < 0 || d->Dr < 0) return fail(GCR_ERR_ARG, "D and Dr must be >= 0 (resolve defaults first)");
+    // chains: runs of equal chain_id with ids 0..nch-1 in order of first appearance
+    std::vector<int32_t> first(nch, -1), len(nch, 0);
+    int64_t prev = -1;
+    for (int64_t b = 0; b < n; ++b) {
+        int64_t c = chain_id[b];
+        if (c < 0 || c >= nch) return fail(GCR_ERR_ARG, "chain_id out of [0, n_chains)");
+        if (c != prev) {
+            if (first[c] >= 0) return fail(GCR_ERR_ARG, "chains must be contiguous runs of chain_id");
+            first[c] = (int32_t)b;
+        }
+        len[c] += 1;
+        prev = c;
+    }
+    for (int64_t c = 0; c < nch; ++c)
+        if (first[c] < 0) return fail(GCR_ERR_ARG, "every chain id needs at least one bead");
+    for (int64_t b = 0; b < n * dim; ++b)
+        if (!(pos[b] >= 0.0 && pos[b] <= d->L)) return fail(GCR_ERR_ARG, "positions must lie in [0, L]");
+    const double skin = default_skin(d);
+    const int64_t nc = d->has_wca ? cells_per_side(d->L, d->wca[3], skin) : 1;
+    const int64_t ncell = dim == 3 ? nc * nc * nc : nc * nc;
+    if (ncell > (1 << 28)) return fail(GCR_ERR_ARG, "cell grid too large");
+    int r = -1;
+    for (int i = 0; i < p->capR; ++i)
+        if (!p->live[i]) {
+            r = i;
+            break;
+        }
+    if (r < 0) return fail(GCR_ERR_CAPACITY, "replica capacity exhausted");
+    int64_t b0 = p->segB.alloc(n);
+    if (b0 < 0) return fail(GCR_ERR_CAPACITY, "bead capacity exhausted");
+    int64_t c0 = p->segC.alloc(ncell + 1);
+    if (c0 < 0) {
+        p->segB.release(b0, n);
+        return fail(GCR_ERR_CAPACITY, "cell capacity exhausted");
+    }
+    int64_t h0 = p->segCh.alloc(nch);
+    if (h0 < 0) {
+        p->segB.release(b0, n);
+        p->segC.release(c0, ncell + 1);
+        return fail(GCR_ERR_CAPACITY, "chain capacity exhausted");
+    }
+    RepParams& P = p->P[r];
+    P = RepParams{};
+    P.n = (int32_t)n;
+    P.bead_off = (int32_t)b0;
+    P.chain_off = (int32_t)h0;
+    P.n_chains = (int32_t)nch;
+    P.cell_off = (int32_t)c0;
+    P.nc = (int32_t)nc;
+    P.ncell = (int32_t)ncell;
+    P.n_nearby = d->n_nearby;
+    P.L = d->L;
+    P.halfL = d->L / 2.0;
+    P.h = d->L / (double)nc;
+    P.inv_h = (double)nc / d->L;
+    P.skin_half = 0.5 * (P.h - (d->has_wca ? d->wca[3] : 0.0));
+    if (!d->has_wca) P.skin_half = 1e300;
+    for (int i = 0; i < 4; ++i) P.wca[i] = d->wca[i];
+    P.rc2 = d->wca[3] * d->wca[3];
+    P.k = d->spring[0];
+    P.kappa = d->spring[1];
+    P.theta0 = d->spring[2];
+    P.D = d->D;
+    P.Dr = d->Dr;
+    P.radius = d->radius;
+    P.seed = d->seed;
+    uint32_t fl = 0;
+    if (d->has_wca) fl |= GCR_F_WCA;
+    if (d->has_springs) fl |= GCR_F_SPRINGS;
+    if (d->has_propel) fl |= GCR_F_PROPEL;
+    bool orient = d->has_springs && d->spring[4] != 0.0;
+    if (orient) fl |= GCR_F_ORIENT;
+    if (d->has_springs && d->spring[1] != 0.0) fl |= GCR_F_BEND;
+    if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
+    P.flags = fl;
+    // host staging of per-bead and per-chain data
+    std::vector<int32_t> meta(2 * n), slot(n);
+    for (int64_t b = 0; b < n; ++b) {
+        meta[2 * b] = (int32_t)(b0 + b);
+        meta[2 * b + 1] = (int32_t)(h0 + chain_id[b]);
+        slot[b] = (int32_t)(b0 + b);
+    }
+    std::vector<double> ca(nch), cs(3 * nch);
+    std::vector<int32_t> cf(nch), cl(nch);
+    for (int64_t c = 0; c < nch; ++c) {
+        ca[c] = rest_lengths ? rest_lengths[c] : 0.0;
+        double ang = v_tan_angle ? v_tan_angle[c] : 0.0;
+        cs[3 * c] = v0_per_chain ? v0_per_chain[c] : d->v0;
+        if (ang == 0.0 || dim != 2) {  // forces.py:214-217: plain v0 * t
+            cs[3 * c + 1] = 1.0;
+            cs[3 * c + 2] = 0.0;
+        } else {
+            cs[3 * c + 1] = std::cos(ang);
+            cs[3 * c + 2] = std::sin(ang);
+        }
+        cf[c] = (int32_t)(b0 + first[c]);
+        cl[c] = len[c];
+    }
+    std::vector<double> tang(n * dim, 0.0);
+    if (tangent)
+        std::memcpy(tang.data(), tangent, sizeof(double) * n * dim);
+    hipStream_t st = p->stream;
+    HIPCHK(hipMemcpyAsync(p->X + b0 * dim, pos, sizeof(double) * n * dim, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->X1 + b0 * dim, pos, sizeof(double) * n * dim, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->T + b0 * dim, tang.data(), sizeof(double) * n * dim, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->META + 2 * b0, meta.data(), 8 * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->SLOT + b0, slot.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->CHAIN_A + h0, ca.data(), 8 * nch, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->CHAIN_CS + 3 * h0, cs.data(), 24 * nch, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->CHAIN_FIRST + h0, cf.data(), 4 * nch, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->CHAIN_LEN + h0, cl.data(), 4 * nch, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(p->FT + b0 * dim, 0, sizeof(double) * n * dim, st));
+    RepState& S = p->S[r];
+    S = RepState{};
+    S.max_steps = 0;  // not armed
+    S.mdu = -1.0;
+    S.frozen = 0;
+    p->live[r] = 1;
+    p->replays[r] = 0;
+    HIPCHK(hipStreamSynchronize(st));
+    int rc = upload_rep(p, r);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    p->tables_dirty = true;
+    // first cell sort of the new replica, so links, cells and the layout are
+    // valid before its first step (other replicas are left untouched)
+    if ((rc = rebuild_tables(p))) return rc;
+    if ((rc = with_only(p, r, true))) return rc;
+    launch_rebuild(p);
+    HIPCHK(hipGetLastError());
+    if ((rc = with_only(p, r, false))) return rc;
+    *rid = r;
+    return GCR_OK;
+}
+
+int gcr_replica_remove(gcr_pool* p, int32_t rid)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    RepParams& P = p->P[rid];
+    p->segB.release(P.bead_off, P.n);
+    p->segC.release(P.cell_off, P.ncell + 1);
+    p->segCh.release(P.chain_off, P.n_chains);
+    if (p->xi_dev[rid]) {
+        (void)hipFree(p->xi_dev[rid]);
+        p->xi_dev[rid] = nullptr;
+    }
+    p->live[rid] = 0;
+    p->S[rid] = RepState{};
+    p->S[rid].frozen = 1;
+    P = RepParams{};
+    rc = upload_rep(p, rid);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->tables_dirty = true;
+    return GCR_OK;
+}
+
+int gcr_replica_upload(gcr_pool* p, int32_t rid, const double* pos, const double* tangent)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    const RepParams& P = p->P[rid];
+    if (pos) {
+        for (int64_t b = 0; b < (int64_t)P.n * p->dim; ++b)
+            if (!(pos[b] >= 0.0 && pos[b] <= P.L)) return fail(GCR_ERR_ARG, "positions must lie in [0, L]");
+        if ((rc = upload_perm(p, rid, pos, p->X))) return rc;
+    }
+    if (tangent && (rc = upload_perm(p, rid, tangent, p->T))) return rc;
+    return GCR_OK;
+}
+
+int gcr_replica_begin(gcr_pool* p, int32_t rid, const double tv[8], const double* xi, int64_t xi_steps)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (!tv) return fail(GCR_ERR_ARG, "null time_vals");
+    RepParams& P = p->P[rid];
+    RepState& S = p->S[rid];
+    if (!(tv[1] > 0) || !(tv[2] > 0) || !(tv[3] > 0)) return fail(GCR_ERR_ARG, "dtmax, max_dist_per_step, RK_cheat_factor must be > 0");
+    if (tv[7] < 0 && tv[4] < 0) return fail(GCR_ERR_ARG, "need max_steps >= 0 or max_dt_usable > 0");
+    P.dtmax = tv[1];
+    P.mdps = tv[2];
+    P.cheat = tv[3];
+    if (p->xi_dev[rid]) {
+        HIPCHK(hipFree(p->xi_dev[rid]));
+        p->xi_dev[rid] = nullptr;
+    }
+    P.flags &= ~GCR_F_INJECT;
+    P.xi_inject = nullptr;
+    P.xi_steps = 0;
+    if (xi) {
+        if (tv[7] < 0 || xi_steps < (int64_t)tv[7])
+            return fail(GCR_ERR_ARG, "injected noise must cover max_steps steps");
+        size_t bytes = sizeof(double) * (size_t)xi_steps * P.n * 2 * p->dim;
+        HIPCHK(hipMalloc((void**)&p->xi_dev[rid], bytes));
+        HIPCHK(hipMemcpy(p->xi_dev[rid], xi, bytes, hipMemcpyHostToDevice));
+        P.flags |= GCR_F_INJECT;
+        P.xi_inject = p->xi_dev[rid];
+        P.xi_steps = xi_steps;
+    }
+    if ((rc = pull_states(p))) return rc;
+    S.dt = tv[0];
+    S.dt_used = 0.0;
+    S.elapsed = 0.0;
+    S.steps = 0;
+    S.batch_base = 0;
+    S.max_steps = tv[7] < 0 ? -1 : (int64_t)tv[7];
+    S.mdu = tv[4];
+    S.fmax2 = 0;
+    S.blocks_done = 0;
+    S.violation = 0;
+    S.frozen = 0;
+    rc = upload_rep(p, rid);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return GCR_OK;
+}
+
+int gcr_pool_run(gcr_pool* p, int32_t max_batches, int32_t* n_running)
+{
+    if (!p) return fail(GCR_ERR_ARG, "null pool");
+    int rc = ensure_ready(p);
+    if (rc) return rc;
+    if ((rc = pull_states(p))) return rc;
+    for (int b = 0; max_batches <= 0 || b < max_batches; ++b) {
+        if (count_running(p) == 0) break;
+        HIPCHK(hipGraphLaunch(p->g_norm, p->stream));
+        if ((rc = pull_states(p))) return rc;
+        bool viol = false;
+        for (int r = 0; r < p->capR; ++r)
+            if (p->live[r] && p->S[r].violation) viol = true;
+        if (viol && (rc = replay(p))) return rc;
+    }
+    if (n_running) *n_running = count_running(p);
+    return GCR_OK;
+}
+
+int gcr_replica_status(gcr_pool* p, int32_t rid, double out[8])
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (!out) return fail(GCR_ERR_ARG, "null output");
+    if ((rc = pull_states(p))) return rc;
+    const RepState& S = p->S[rid];
+    out[0] = S.dt;
+    out[1] = S.elapsed;
+    out[2] = (double)S.steps;
+    out[3] = S.maxm;
+    out[4] = running(S) ? 1.0 : 0.0;
+    out[5] = (double)p->replays[rid];
+    out[6] = (double)S.step_global;
+    out[7] = (double)p->P[rid].n;
+    return GCR_OK;
+}
+
+int gcr_replica_download(gcr_pool* p, int32_t rid, double* pos, double* pos1, double* tangent,
+                         double* forces)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
+    const int dim = p->dim;
+    if (pos && (rc = download_one(p, rid, p->X, pos, dim))) return rc;
+    if (pos1 && (rc = download_one(p, rid, p->X1, pos1, dim))) return rc;
+    if (forces && (rc = download_one(p, rid, p->FT, forces, dim))) return rc;
+    if (tangent) {
+        if (p->P[rid].flags & GCR_F_ORIENT) {
+            // tangents at x1, from the bonds (needs valid links: built by a run)
+            if (dim == 2)
+                k_tangents<2><<<p->nblk, BS, 0, p->stream>>>(p->a, rid);
+            else
+                k_tangents<3><<<p->nblk, BS, 0, p->stream>>>(p->a, rid);
+            // k_tangents writes Tn; move it out of the staging buffer first
+            HIPCHK(hipMemcpyAsync(p->Ts, p->Tn, sizeof(double) * (size_t)p->capB * dim,
+                                  hipMemcpyDeviceToDevice, p->stream));
+            if ((rc = download_one(p, rid, p->Ts, tangent, dim))) return rc;
+        } else if ((rc = download_one(p, rid, p->T, tangent, dim))) {
+            return rc;
+        }
+    }
+    return GCR_OK;
+}
+
+int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* tangent)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
+    if ((rc = with_only(p, rid, true))) return rc;
+    launch_rebuild(p);
+    const bool mixed = p->precision == GCR_MIXED;
+    if (p->dim == 2)
+        mixed ? k_eval<2, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
+              : k_eval<2, double><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid);
+    else
+        mixed ? k_eval<3, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
+              : k_eval<3, double><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p->Ts, p->Tn, sizeof(double) * (size_t)p->capB * p->dim,
+                          hipMemcpyDeviceToDevice, p->stream));
+    if ((rc = with_only(p, rid, false))) return rc;
+    if (forces && (rc = download_one(p, rid, p->FT, forces, p->dim))) return rc;
+    if (tangent && (rc = download_one(p, rid, p->Ts, tangent, p->dim))) return rc;
+    return GCR_OK;
+}
+
+int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
+{
+    if (!p || !ms_out || steps < 1) return fail(GCR_ERR_ARG, "bad arguments");
+    int rc;
+    if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
+    hipEvent_t ev[5];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    double acc[4] = {0, 0, 0, 0};
+    int nreb = 0;
+    launch_begin(p);
+    for (int j = 0; j < steps; ++j) {
+        HIPCHK(hipEventRecord(ev[0], p->stream));
+        if (j % p->K == 0) {
+            launch_rebuild(p);
+            ++nreb;
+        }
+        HIPCHK(hipEventRecord(ev[1], p->stream));
+        launch_stage(p, 0, j);
+        HIPCHK(hipEventRecord(ev[2], p->stream));
+        launch_stage(p, 1, j);
+        HIPCHK(hipEventRecord(ev[3], p->stream));
+        launch_stage(p, 2, j);
+        HIPCHK(hipEventRecord(ev[4], p->stream));
+        HIPCHK(hipEventSynchronize(ev[4]));
+        float t;
+        HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2]));
+        acc[0] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3]));
+        acc[1] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[3], ev[4]));
+        acc[2] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
+        acc[3] += t;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    ms_out[0] = acc[0] / steps;
+    ms_out[1] = acc[1] / steps;
+    ms_out[2] = acc[2] / steps;
+    ms_out[3] = nreb ? acc[3] / nreb : 0.0;
+    if ((rc = pull_states(p))) return rc;
+    bool viol = false;
+    for (int r = 0; r < p->capR; ++r)
+        if (p->live[r] && p->S[r].violation) viol = true;
+    if (viol) return fail(GCR_ERR_STATE, "skin exceeded during kernel timing; results discarded");
+    return GCR_OK;
+}
+
+int gcr_pool_destroy(gcr_pool* p)
+{
+    if (!p) return GCR_OK;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    drop_graphs(p);
+    for (double* q : p->xi_dev)
+        if (q) (void)hipFree(q);
+    void* bufs[] = {p->blk_rep, p->blk_slot0, p->chk_rep, p->chk_cell0, p->rep_chk_off, p->dP, p->dS,
+                    p->dSs, p->X, p->X1, p->F0, p->FT, p->XI, p->T, p->Xn, p->Tn, p->Xs, p->Ts,
+                    p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
+                    p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
+                    p->CHKSUM};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return GCR_OK;
+}
+
+// ------------------------------------------------------ single system API
+
+int gcr_create(const gcr_replica_desc* d, int32_t dim, int32_t precision, const double* pos,
+               const double* tangent, const int64_t* chain_id, const double* rest_lengths,
+               const double* v_tan_angle, gcr_pool** out)
+{
+    if (!d || !out) return fail(GCR_ERR_ARG, "null argument");
+    int64_t ncell = gcr_replica_cells(dim, d);
+    gcr_pool* p = nullptr;
+    int rc = gcr_pool_create(dim, precision, d->n_beads, ncell + 1, d->n_chains, 1, &p);
+    if (rc) return rc;
+    int32_t rid;
+    rc = gcr_replica_add(p, d, pos, tangent, chain_id, rest_lengths, v_tan_angle, nullptr, &rid);
+    if (rc) {
+        std::string m = g_err;
+        gcr_pool_destroy(p);
+        return fail(rc, m);
+    }
+    *out = p;
+    return GCR_OK;
+}
+
+int gcr_upload(gcr_pool* h, const double* pos, const double* tangent)
+{
+    return gcr_replica_upload(h, 0, pos, tangent);
+}
+
+int gcr_run(gcr_pool* h, const double tv[8], double status_out[6], const double* xi, int64_t xi_steps)
+{
+    int rc = gcr_replica_begin(h, 0, tv, xi, xi_steps);
+    if (rc) return rc;
+    int32_t nr = 0;
+    rc = gcr_pool_run(h, 0, &nr);
+    if (rc) return rc;
+    if (status_out) {
+        double st[8];
+        if ((rc = gcr_replica_status(h, 0, st))) return rc;
+        status_out[0] = st[0];
+        status_out[1] = st[1];
+        status_out[2] = st[2];
+        status_out[3] = st[3];
+        status_out[4] = 0.0;
+        status_out[5] = 0.0;
+    }
+    return GCR_OK;
+}
+
+int gcr_download(gcr_pool* h, double* pos, double* pos1, double* tangent, double* forces)
+{
+    return gcr_replica_download(h, 0, pos, pos1, tangent, forces);
+}
+
+int gcr_eval_forces(gcr_pool* h, double* forces, double* tangent)
+{
+    return gcr_replica_eval_forces(h, 0, forces, tangent);
+}
+
+int gcr_destroy(gcr_pool* h) { return gcr_pool_destroy(h); }
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// gcr_device.h -- device-side data layout shared by the kernels and the C-ABI
+// host code. One pool holds many replicas (independent simulations) of the
+// same dimensionality; a single large system is a pool with one replica.
+//
+// Per-bead arrays are indexed by "slot". Slots of replica r occupy
+// [bead_off, bead_off + n). Within a replica, beads are physically ordered by
+// cell (row-major, x fastest) and, inside a cell, by original bead index; the
+// order is rebuilt every `rebuild_every` steps by a counting sort. The
+// original index ("orig", the bead's index in the reference's arrays) travels
+// with each bead and keys the noise, the exclusion rule and the download.
+#pragma once
+#include <stdint.h>
+
+#define GCR_F_WCA 1u        // "WCA" in the force list (forces.py:57-144)
+#define GCR_F_SPRINGS 2u    // "springs" (forces.py:285-402)
+#define GCR_F_PROPEL 4u     // "self-propulsion" (forces.py:206-223)
+#define GCR_F_ORIENT 8u     // spring_params[4] != 0: tangents from bonds
+#define GCR_F_BEND 16u      // kappa != 0
+#define GCR_F_ROTDIFF 32u   // Dr > 0 and not ORIENT (SRK.py:399)
+#define GCR_F_INJECT 64u    // noise read from an injected array
+
+struct RepParams {
+    int32_t n;          // beads
+    int32_t bead_off;   // first slot
+    int32_t chain_off;  // first chain (global chain index)
+    int32_t n_chains;
+    int32_t cell_off;   // first cell
+    int32_t nc;         // cells per side (cubic grid)
+    int32_t ncell;      // nc^dim
+    int32_t n_nearby;   // n_nearby_beads_dont_repel
+    uint32_t flags;
+    int32_t nblk;       // bead blocks of this replica (for last-block finalize)
+    int32_t blk_off;    // first bead block
+    int32_t pad0;
+    double L, halfL;
+    double h, inv_h;    // cell edge and its inverse
+    double skin_half;   // allowed excursion of a bead outside its cell box
+    double wca[4];      // make_WCA_params (populations.py:48-58)
+    double rc2;
+    double k, kappa, theta0;  // spring_params[0..2]
+    double D, Dr, radius;     // pop_consts[0..2]
+    double dtmax, mdps, cheat;
+    uint64_t seed;
+    const double* xi_inject;  // [steps][n][2*dim] in orig order, or null
+    int64_t xi_steps;
+};
+
+// Mutable per-replica step state. Written by the last block of the stage-1
+// kernel (finalize) and by the batch control kernels only.
+struct RepState {
+    double dt;        // most recent dt (the predictor of the next step uses it)
+    double dt_used;   // dt_used of the most recent step (corrector step size)
+    double elapsed;   // sum of dt_used in the current call
+    double maxm;      // max_forces_strength of the most recent step
+    int64_t steps;        // steps done in the current call
+    int64_t step_global;  // steps done since the replica was created (noise key)
+    int64_t batch_base;   // `steps` at the start of the running batch
+    int64_t max_steps;    // < 0: unbounded
+    double mdu;           // max_dt_usable, < 0: unbounded
+    uint64_t fmax2;       // bit pattern of max |F0+F1|^2 (non-negative double)
+    uint32_t blocks_done; // tickets for the last-block finalize
+    int32_t violation;    // a bead left its cell box by more than skin/2
+    int32_t frozen;       // 1: kernels leave this replica untouched
+    int32_t pad1;
+};
+
+struct PoolArgs {
+    // block tables for bead-parallel kernels
+    const int32_t* blk_rep;
+    const int32_t* blk_slot0;
+    int32_t nblk;
+    // chunk tables for the cell scan
+    const int32_t* chk_rep;
+    const int32_t* chk_cell0;
+    int32_t nchk;
+    const int32_t* rep_chk_off;  // first chunk of each replica (+1 entry)
+    int32_t nrep;
+    // replica tables
+    const RepParams* P;
+    RepState* S;
+    // per-slot state
+    double* X;     // [slot][dim] positions
+    double* X1;    // [slot][dim] predictor positions
+    double* F0;    // [slot][dim] stage-0 forces
+    double* FT;    // [slot][dim] F0 + F1
+    double* XI;    // [slot][2*dim] noise of the running step
+    double* T;     // [slot][dim] propulsion direction (ABP / diffusing state)
+    int32_t* META; // [slot][2] (orig_global, chain_global)
+    int32_t* LINK; // [slot][4] (prev2, prev1, next1, next2) slots, -1 if none
+    int32_t* CELL; // [slot] cell (replica-local) the bead was binned into
+    int32_t* SLOT; // [orig_global] -> slot
+    // per-chain constants
+    const double* CHAIN_A;   // rest length
+    const double* CHAIN_CS;  // [chain][3] (v0, cos v_tan_angle, sin v_tan_angle)
+    const int32_t* CHAIN_FIRST;  // first orig_global of the chain
+    const int32_t* CHAIN_LEN;
+    // cells
+    int32_t* CCOUNT;  // [cell] counts during a rebuild
+    int32_t* CSTART;  // [cell + 1] first slot of each cell
+    int32_t* CHKSUM;  // [chunk] partial sums for the scan
+    // rebuild scratch
+    int32_t* KEYOFF;  // [slot][2] (cell, offset within cell)
+    int32_t* PAIRS;   // [slot][2] (orig_global, old slot)
+    double* Xn;
+    double* Tn;
+    int32_t* METAn;
+    // rollback snapshot
+    double* Xs;
+    double* Ts;
+    int32_t* METAs;
+    RepState* Ss;
+};

@@ -1,0 +1,928 @@
+// gcr_kernels.hip -- CDNA4 (gfx950) kernels of the Brownian-dynamics hot path.
+//
+// One step of SRK2 (SRK.py:165-396) is three launches over all replicas of a
+// pool:
+//   k_stage<0>  forces at x   (springs, pair, propulsion) -> x1, F0, noise
+//   k_stage<1>  forces at x1, F = F0 + F1, max |F|^2, last block sets dt
+//   k_update    x <- x + dt_used/(2r) F + sqrt(2 D dt_used) xi, rot. diffusion
+// Every `rebuild_every` steps the beads are re-sorted by cell (counting sort):
+//   k_zero_counts, k_bin, k_scan_{chunks,reps,final}, k_scatter,
+//   k_cellsort, k_gather, k_links
+// Force kernels write one output per bead and use no atomics on forces, so a
+// replica's result does not depend on scheduling or on other replicas.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gcr_device.h"
+
+#define BS 256  // threads per bead block
+
+namespace gcr {
+
+// ------------------------------------------------------------------ helpers
+
+// CPython float_rem (the `tmp %= Lx` at SRK.py:387).
+__device__ __forceinline__ double pymod(double a, double b)
+{
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0) != (m < 0)) m += b;
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+// One-shot periodic correction into [-L/2, L/2) (forces.py:44-54, 103-107).
+__device__ __forceinline__ double wrap1(double s, double L, double halfL)
+{
+    if (s < -halfL)
+        s += L;
+    else if (s >= halfL)
+        s -= L;
+    return s;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11), published multipliers / Weyl constants.
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
+
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& n0, double& n1)
+{
+    double r = sqrt(-2.0 * log(u01(a)));
+    double th = 6.283185307179586 * u01(b);
+    double s, c;
+    sincos(th, &s, &c);
+    n0 = r * c;
+    n1 = r * s;
+}
+
+// Noise of one bead for one step: counter (orig, step lo, step hi, block),
+// key = replica seed. Same definition as the oracle's orc_bead_noise.
+template <int DIM>
+__device__ __forceinline__ void bead_noise(uint64_t seed, uint32_t orig, int64_t step, bool need_rot,
+                                           double* xi)
+{
+    uint32_t c[4] = {orig, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), 0u};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double g0, g1, g2 = 0.0, g3 = 0.0;
+    box_muller(c[0], c[1], g0, g1);
+    if (DIM == 3 || need_rot) box_muller(c[2], c[3], g2, g3);
+    xi[0] = g0;
+    xi[1] = g1;
+    if (DIM == 2) {
+        xi[2] = g2;
+        xi[3] = g3;
+    } else {
+        xi[2] = g2;
+        xi[3] = g3;
+        if (need_rot) {
+            uint32_t d[4] = {orig, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), 1u};
+            philox(d, (uint32_t)seed, (uint32_t)(seed >> 32));
+            double g4, g5;
+            box_muller(d[0], d[1], g4, g5);
+            xi[4] = g4;
+            xi[5] = g5;
+        } else {
+            xi[4] = 0.0;
+            xi[5] = 0.0;
+        }
+    }
+}
+
+// Per-block running condition of a replica for local step j of a batch.
+__device__ __forceinline__ bool step_runs(const RepState* S, int j)
+{
+    if (S->frozen) return false;
+    if (S->steps != S->batch_base + j) return false;
+    bool a = (S->max_steps < 0) || (S->steps < S->max_steps);
+    bool b = (S->mdu < 0) || (S->elapsed < S->mdu);
+    return a && b;
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_vec(const double* A, int s, double* v)
+{
+    if (DIM == 2) {
+        double2 t = *reinterpret_cast<const double2*>(A + 2 * s);
+        v[0] = t.x;
+        v[1] = t.y;
+    } else {
+        v[0] = A[3 * s];
+        v[1] = A[3 * s + 1];
+        v[2] = A[3 * s + 2];
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void store_vec(double* A, int s, const double* v)
+{
+    if (DIM == 2) {
+        *reinterpret_cast<double2*>(A + 2 * s) = make_double2(v[0], v[1]);
+    } else {
+        A[3 * s] = v[0];
+        A[3 * s + 1] = v[1];
+        A[3 * s + 2] = v[2];
+    }
+}
+
+// Cell coordinate of a position along one axis, clamped into [0, nc).
+__device__ __forceinline__ int cell_coord(double x, double inv_h, int nc)
+{
+    int c = (int)(x * inv_h);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+template <int DIM>
+__device__ __forceinline__ int cell_of(const double* x, const RepParams& P)
+{
+    int cx = cell_coord(x[0], P.inv_h, P.nc);
+    int cy = cell_coord(x[1], P.inv_h, P.nc);
+    int cz = DIM == 3 ? cell_coord(x[2], P.inv_h, P.nc) : 0;
+    return (cz * P.nc + cy) * P.nc + cx;
+}
+
+// How far x lies outside the (periodic) box of cell c, max over axes.
+template <int DIM>
+__device__ __forceinline__ double outside_cell(const double* x, int c, const RepParams& P)
+{
+    int cc[3] = {c % P.nc, (c / P.nc) % P.nc, DIM == 3 ? c / (P.nc * P.nc) : 0};
+    double worst = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        double d = wrap1(x[i] - (cc[i] + 0.5) * P.h, P.L, P.halfL);
+        double o = fabs(d) - 0.5 * P.h;
+        worst = o > worst ? o : worst;
+    }
+    return worst;
+}
+
+// ------------------------------------------------------------ bonded forces
+
+// Bond vector of the reference (inbounds_diff(pos[u], pos[v]) = x_u - x_v),
+// its squared length, inverse length and unit vector (unit = raw when the
+// length is zero, as forces.py:326-349 leaves it).
+template <int DIM>
+struct Bond {
+    double t[DIM];
+    double u[DIM];
+    double ds;
+    double inv;
+};
+
+template <int DIM>
+__device__ __forceinline__ void make_bond(const double* xu, const double* xv, const RepParams& P,
+                                          Bond<DIM>& b)
+{
+    b.ds = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        b.t[i] = wrap1(xu[i] - xv[i], P.L, P.halfL);
+        b.ds += b.t[i] * b.t[i];
+    }
+    if (b.ds != 0.0) {
+        b.inv = 1.0 / sqrt(b.ds);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i] * b.inv;
+    } else {
+        b.inv = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i];
+    }
+}
+
+__device__ __forceinline__ double sgn(double x) { return (double)((x > 0) - (x < 0)); }
+
+// Bending prefactor phi of forces.py:364-373 for the angle between ti (bond
+// leaving the vertex forward) and tj (bond arriving from behind). Returns
+// false when the reference skips the term (|dot| == 1).
+template <int DIM>
+__device__ __forceinline__ bool bend_phi(const double* ti, const double* tj, const RepParams& P,
+                                         double& dot, double& phi)
+{
+    dot = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) dot += ti[i] * tj[i];
+    if (!(-1.0 < dot && dot < 1.0)) return false;
+    phi = acos(dot);
+    if (P.theta0 != 0.0) phi += P.theta0 * sgn(-ti[0] * tj[1] + ti[1] * tj[0]);
+    phi *= P.kappa / sqrt(1.0 - dot * dot);
+    return true;
+}
+
+// All bonded terms acting on slot s, added to acc in the reference's order
+// (forces.py:305-402 walks beads in index order: bond p1->s, angle at p1,
+// bond s->n1, angle at s, angle at n1). Writes the tangent at s when
+// tangents follow the bonds.
+template <int DIM>
+__device__ __forceinline__ void bonded(const double* X, int s, const double* xs, int4 lk,
+                                       double a, const RepParams& P, double* acc, double* tang,
+                                       bool& tang_set)
+{
+    const bool bend = (P.flags & GCR_F_BEND) != 0;
+    const bool orient = (P.flags & GCR_F_ORIENT) != 0;
+    const double k = P.k;
+    double xp1[DIM], xn1[DIM], xp2[DIM], xn2[DIM];
+    Bond<DIM> A, B, Pb, Nb;  // (p1,s), (s,n1), (p2,p1), (n1,n2)
+    if (lk.y >= 0) {
+        load_vec<DIM>(X, lk.y, xp1);
+        make_bond<DIM>(xp1, xs, P, A);
+    }
+    if (lk.z >= 0) {
+        load_vec<DIM>(X, lk.z, xn1);
+        make_bond<DIM>(xs, xn1, P, B);
+    }
+    // bond p1 -> s: forces[b1+1] -= f (forces.py:345-348)
+    if (lk.y >= 0 && A.ds != 0.0) {
+        double mag = k * (a * A.inv - 1.0);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) acc[i] -= A.t[i] * mag;
+    }
+    // angle at p1: forces[b1+1] -= f_bend_backward
+    if (bend && lk.x >= 0) {
+        load_vec<DIM>(X, lk.x, xp2);
+        make_bond<DIM>(xp2, xp1, P, Pb);
+        double dot, phi;
+        if (bend_phi<DIM>(A.u, Pb.u, P, dot, phi)) {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) acc[i] -= phi * (Pb.u[i] - dot * A.u[i]);
+        }
+    }
+    // bond s -> n1: forces[b1] += f
+    if (lk.z >= 0 && B.ds != 0.0) {
+        double mag = k * (a * B.inv - 1.0);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) acc[i] += B.t[i] * mag;
+    }
+    // angle at s (interior bead), and the tangent
+    if (lk.y >= 0 && lk.z >= 0) {
+        if (bend || orient) {
+            double dot = 0.0;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) dot += B.u[i] * A.u[i];
+            bool inside = (-1.0 < dot && dot < 1.0);
+            if (inside && bend) {
+                double phi = acos(dot);
+                if (P.theta0 != 0.0) phi += P.theta0 * sgn(-B.u[0] * A.u[1] + B.u[1] * A.u[0]);
+                phi *= P.kappa / sqrt(1.0 - dot * dot);
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    double ff = phi * (dot * A.u[i] - B.u[i]);
+                    double fb = phi * (A.u[i] - dot * B.u[i]);
+                    acc[i] += ff + fb;
+                }
+            }
+            if (orient) {
+                if (inside) {
+                    double c = 1.0 / sqrt(2.0 * (1.0 + dot));
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) tang[i] = c * (B.u[i] + A.u[i]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) tang[i] = B.u[i];
+                }
+                tang_set = true;
+            }
+        }
+    } else if (orient) {
+        if (lk.y < 0 && lk.z >= 0) {  // head: first bond
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) tang[i] = B.u[i];
+            tang_set = true;
+        } else if (lk.y >= 0 && lk.z < 0) {  // tail: last bond
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) tang[i] = A.u[i];
+            tang_set = true;
+        }
+    }
+    // angle at n1: forces[b1-1] -= f_bend_forward
+    if (bend && lk.w >= 0) {
+        load_vec<DIM>(X, lk.w, xn2);
+        make_bond<DIM>(xn1, xn2, P, Nb);
+        double dot, phi;
+        if (bend_phi<DIM>(Nb.u, B.u, P, dot, phi)) {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) acc[i] -= phi * (dot * B.u[i] - Nb.u[i]);
+        }
+    }
+}
+
+// ------------------------------------------------------------- pair forces
+
+// Pair law forces.py:13-15 evaluated in FT; the separation and the cutoff
+// test are FP64, the returned force component is added to an FP64 sum.
+template <typename FT>
+struct PairLaw;
+
+template <>
+struct PairLaw<double> {
+    static __device__ __forceinline__ double coeff(double ds, const RepParams& P)
+    {
+        double inv = 1.0 / ds;
+        double i2 = inv * inv;
+        double i3 = i2 * inv;
+        return (i2 * i2) * (P.wca[0] - P.wca[1] * i3);
+    }
+};
+
+template <>
+struct PairLaw<float> {
+    static __device__ __forceinline__ float coeff(float ds, const RepParams& P)
+    {
+        float inv = __builtin_amdgcn_rcpf(ds);
+        float i2 = inv * inv;
+        float i3 = i2 * inv;
+        return (i2 * i2) * ((float)P.wca[0] - (float)P.wca[1] * i3);
+    }
+};
+
+template <int DIM, typename FT>
+__device__ __forceinline__ void pair_sum(const double* X, const int32_t* META, const int32_t* CSTART,
+                                         const double* xs, int orig, int chain, int cell,
+                                         const RepParams& P, double* acc)
+{
+    const int nc = P.nc;
+    int cc[3] = {cell % nc, (cell / nc) % nc, DIM == 3 ? cell / (nc * nc) : 0};
+    // neighbour stencil per axis: the 3 surrounding cells, or every cell when
+    // the grid has fewer than 3 cells on that axis
+    const int lo = nc >= 3 ? -1 : 0;
+    const int hi = nc >= 3 ? 1 : nc - 1;
+    const int zlo = DIM == 3 ? lo : 0, zhi = DIM == 3 ? hi : 0;
+    const bool shifted = P.wca[2] != 0.0;
+    for (int dz = zlo; dz <= zhi; ++dz) {
+        int z = DIM == 3 ? (nc >= 3 ? (cc[2] + dz + nc) % nc : dz) : 0;
+        for (int dy = lo; dy <= hi; ++dy) {
+            int y = nc >= 3 ? (cc[1] + dy + nc) % nc : dy;
+            for (int dx = lo; dx <= hi; ++dx) {
+                int x = nc >= 3 ? (cc[0] + dx + nc) % nc : dx;
+                int c2 = P.cell_off + (z * nc + y) * nc + x;
+                int t0 = CSTART[c2], t1 = CSTART[c2 + 1];
+                for (int t = t0; t < t1; ++t) {
+                    int2 mj = *reinterpret_cast<const int2*>(META + 2 * t);
+                    if (mj.y == chain) {
+                        if (P.n_nearby < 0) continue;
+                        int dd = mj.x - orig;
+                        if ((dd < 0 ? -dd : dd) <= P.n_nearby) continue;
+                    }
+                    double xj[DIM], sep[DIM];
+                    load_vec<DIM>(X, t, xj);
+                    double ds = 0.0;
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) {
+                        sep[i] = wrap1(xj[i] - xs[i], P.L, P.halfL);
+                        ds += sep[i] * sep[i];
+                    }
+                    if (!(0.0 < ds && ds <= P.rc2)) continue;
+                    if (shifted) {  // WCA_shift, forces.py:123-131
+                        double mag = sqrt(ds);
+                        double nm = mag + P.wca[2];
+                        nm = nm < 0.0 ? 0.0 : nm;
+                        double rs = nm / mag;
+#pragma unroll
+                        for (int i = 0; i < DIM; ++i) sep[i] *= rs;
+                        ds = nm * nm;
+                    }
+                    FT c = PairLaw<FT>::coeff((FT)ds, P);
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) acc[i] += (double)((FT)sep[i] * c);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ stages
+
+template <int DIM>
+__device__ __forceinline__ void block_max_u64(uint64_t v, uint64_t* target)
+{
+    // wave64 max via shuffles, then across the block's waves in LDS
+    __shared__ uint64_t red[BS / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        uint64_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t m = red[0];
+        for (int i = 1; i < BS / 64; ++i) m = red[i] > m ? red[i] : m;
+        atomicMax((unsigned long long*)target, (unsigned long long)m);
+    }
+}
+
+// Stage 0 (STAGE == 0): forces at x, predictor x1 = x + dt/r F0 + sqrt(2 D dt) xi.
+// Stage 1 (STAGE == 1): forces at x1 added to F0 (the reference accumulates
+// into the same array, SRK.py:153-160), max |F|^2, last block finalises dt.
+template <int DIM, typename FT, int STAGE>
+__global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    RepState* S = a.S + r;
+    if (!step_runs(S, j)) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    const bool valid = s < P.bead_off + P.n;
+    const double* Xa = STAGE == 0 ? a.X : a.X1;
+    double fsq = 0.0;
+    if (valid) {
+        double xs[DIM], acc[DIM], tang[DIM];
+        load_vec<DIM>(Xa, s, xs);
+        if (STAGE == 0) {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) acc[i] = 0.0;
+        } else {
+            load_vec<DIM>(a.F0, s, acc);
+        }
+        int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+        bool tang_set = false;
+        if (P.flags & GCR_F_SPRINGS) {
+            int4 lk = *reinterpret_cast<const int4*>(a.LINK + 4 * s);
+            bonded<DIM>(Xa, s, xs, lk, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+        }
+        if (P.flags & GCR_F_WCA)
+            pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
+        if (P.flags & GCR_F_PROPEL) {
+            if (!tang_set) load_vec<DIM>(a.T, s, tang);
+            const double* cs = a.CHAIN_CS + 3 * me.y;
+            if (DIM == 2) {
+                acc[0] += cs[0] * (cs[1] * tang[0] - cs[2] * tang[1]);
+                acc[1] += cs[0] * (cs[2] * tang[0] + cs[1] * tang[1]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) acc[i] += cs[0] * tang[i];
+            }
+        }
+        if (STAGE == 0) {
+            // noise of this step, kept for the corrector (same xi, SRK.py:379-386)
+            double xi[2 * DIM];
+            const int orig_local = me.x - P.bead_off;
+            if (P.flags & GCR_F_INJECT) {
+                const double* src = P.xi_inject + ((size_t)S->steps * P.n + orig_local) * (2 * DIM);
+#pragma unroll
+                for (int i = 0; i < 2 * DIM; ++i) xi[i] = src[i];
+            } else {
+                bead_noise<DIM>(P.seed, (uint32_t)orig_local, S->step_global,
+                                (P.flags & GCR_F_ROTDIFF) != 0, xi);
+            }
+#pragma unroll
+            for (int i = 0; i < 2 * DIM; ++i) a.XI[(size_t)s * 2 * DIM + i] = xi[i];
+            store_vec<DIM>(a.F0, s, acc);
+            const double dt = S->dt;
+            double dt_eff = dt * 1.0;
+            const double s2 = sqrt(2.0 * dt * P.D);
+            dt_eff /= P.radius;
+            double x1[DIM];
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                double tmp = xs[i] + dt_eff * acc[i];
+                if (s2 != 0.0) tmp += s2 * xi[i];
+                x1[i] = pymod(tmp, P.L);
+            }
+            store_vec<DIM>(a.X1, s, x1);
+            if (outside_cell<DIM>(x1, a.CELL[s], P) > P.skin_half) atomicOr(&S->violation, 1);
+        } else {
+            store_vec<DIM>(a.FT, s, acc);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) fsq += acc[i] * acc[i];
+        }
+    }
+    if (STAGE == 1) {
+        block_max_u64<DIM>(__double_as_longlong(fsq), &S->fmax2);
+        __shared__ uint32_t ticket;
+        __threadfence();
+        if (threadIdx.x == 0) ticket = atomicAdd(&S->blocks_done, 1u);
+        __syncthreads();
+        if (threadIdx.x == 0 && ticket == (uint32_t)(P.nblk - 1)) {
+            // finalize: max force and adaptive dt (SRK.py:323-353)
+            __threadfence();
+            uint64_t bits = atomicMax((unsigned long long*)&S->fmax2, 0ull);
+            double mx = __longlong_as_double((long long)bits);
+            double m = sqrt(2.0 * P.D);
+            if (P.n > 0) m += 0.5 * sqrt(mx) / P.radius;
+            double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
+            double dt_used = P.cheat * dt;
+            if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
+            S->dt = dt;
+            S->dt_used = dt_used;
+            S->elapsed += dt_used;
+            S->maxm = m;
+            S->steps += 1;
+            S->step_global += 1;
+            S->fmax2 = 0ull;
+            S->blocks_done = 0u;
+            __threadfence();
+        }
+    }
+}
+
+// Corrector update (SRK.py:355-388) and rotational diffusion (SRK.py:398-423).
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    const RepState* S = a.S + r;
+    if (S->frozen || S->steps != S->batch_base + j + 1) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    const double dt_used = S->dt_used;
+    double dt_eff = dt_used * 0.5;
+    const double s2 = sqrt(2.0 * dt_used * P.D);
+    dt_eff /= P.radius;
+    double x[DIM], f[DIM], xi[2 * DIM];
+    load_vec<DIM>(a.X, s, x);
+    load_vec<DIM>(a.FT, s, f);
+#pragma unroll
+    for (int i = 0; i < 2 * DIM; ++i) xi[i] = a.XI[(size_t)s * 2 * DIM + i];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        double tmp = x[i] + dt_eff * f[i];
+        if (s2 != 0.0) tmp += s2 * xi[i];
+        x[i] = pymod(tmp, P.L);
+    }
+    store_vec<DIM>(a.X, s, x);
+    if (outside_cell<DIM>(x, a.CELL[s], P) > P.skin_half) atomicOr(&a.S[r].violation, 1);
+    if (P.flags & GCR_F_ROTDIFF) {
+        const double sr = sqrt(2.0 * dt_used * P.Dr) * 3.141592653589793;
+        double t[DIM], nr[DIM], proj = 0.0;
+        load_vec<DIM>(a.T, s, t);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) nr[i] = xi[DIM + i];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) proj += nr[i] * t[i];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) nr[i] -= proj * t[i];
+        double mag = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            t[i] = t[i] + sr * nr[i];
+            mag += t[i] * t[i];
+        }
+        mag = sqrt(mag);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) t[i] /= mag;
+        store_vec<DIM>(a.T, s, t);
+    }
+}
+
+// One force evaluation without an update (parity checks of the force path):
+// forces at X (which == 0) or X1 (which == 1) into FT, tangents into Tn.
+template <int DIM, typename FT>
+__global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    if (r != rsel) return;
+    const RepParams& P = a.P[r];
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    const double* Xa = which == 0 ? a.X : a.X1;
+    double xs[DIM], acc[DIM], tang[DIM];
+    load_vec<DIM>(Xa, s, xs);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) acc[i] = 0.0;
+    int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+    bool tang_set = false;
+    if (P.flags & GCR_F_SPRINGS) {
+        int4 lk = *reinterpret_cast<const int4*>(a.LINK + 4 * s);
+        bonded<DIM>(Xa, s, xs, lk, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+    }
+    if (P.flags & GCR_F_WCA)
+        pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
+    if (!tang_set) load_vec<DIM>(a.T, s, tang);
+    if (P.flags & GCR_F_PROPEL) {
+        const double* cs = a.CHAIN_CS + 3 * me.y;
+        if (DIM == 2) {
+            acc[0] += cs[0] * (cs[1] * tang[0] - cs[2] * tang[1]);
+            acc[1] += cs[0] * (cs[2] * tang[0] + cs[1] * tang[1]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) acc[i] += cs[0] * tang[i];
+        }
+    }
+    store_vec<DIM>(a.FT, s, acc);
+    store_vec<DIM>(a.Tn, s, tang);
+}
+
+// ----------------------------------------------------------------- rebuild
+
+__global__ void __launch_bounds__(BS) k_fill_i32(int32_t* p, int n, int32_t v)
+{
+    int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_bin(PoolArgs a)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (a.S[r].frozen) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    double x[DIM];
+    load_vec<DIM>(a.X, s, x);
+    int c = cell_of<DIM>(x, P);
+    int off = atomicAdd(&a.CCOUNT[P.cell_off + c], 1);
+    *reinterpret_cast<int2*>(a.KEYOFF + 2 * s) = make_int2(c, off);
+}
+
+// Inclusive wave64 scan with shuffles, exclusive block scan through LDS.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total)
+{
+    __shared__ int wsum[BS / 64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int o = __shfl_up(inc, off, 64);
+        if (l >= off) inc += o;
+    }
+    if (l == 63) wsum[w] = inc;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) {
+        if (i < w) base += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+#define SCAN_PER_THREAD 8
+#define SCAN_CHUNK (BS * SCAN_PER_THREAD)
+
+// scan phase A: one chunk of SCAN_CHUNK cells of one replica -> its sum
+__global__ void __launch_bounds__(BS) k_scan_chunks(PoolArgs a)
+{
+    const int r = a.chk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (a.S[r].frozen) return;
+    const int c0 = a.chk_cell0[blockIdx.x];
+    const int cend = P.cell_off + P.ncell;
+    int sum = 0;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        int c = c0 + threadIdx.x * SCAN_PER_THREAD + k;
+        if (c < cend) sum += a.CCOUNT[c];
+    }
+    int tot;
+    block_exclusive_scan(sum, &tot);
+    if (threadIdx.x == 0) a.CHKSUM[blockIdx.x] = tot;
+}
+
+// scan phase B: one block per replica scans its chunk sums in place
+__global__ void __launch_bounds__(BS) k_scan_reps(PoolArgs a)
+{
+    const int r = blockIdx.x;
+    if (a.S[r].frozen) return;
+    const int k0 = a.rep_chk_off[r], k1 = a.rep_chk_off[r + 1];
+    int carry = 0;
+    for (int base = k0; base < k1; base += BS) {
+        int k = base + threadIdx.x;
+        int v = k < k1 ? a.CHKSUM[k] : 0;
+        int tot;
+        int ex = block_exclusive_scan(v, &tot);
+        if (k < k1) a.CHKSUM[k] = carry + ex;
+        carry += tot;
+    }
+}
+
+// scan phase C: exclusive scan inside each chunk, offset by the chunk prefix
+// and the replica's first slot -> CSTART; the last chunk also writes the end
+__global__ void __launch_bounds__(BS) k_scan_final(PoolArgs a)
+{
+    const int r = a.chk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (a.S[r].frozen) return;
+    const int c0 = a.chk_cell0[blockIdx.x];
+    const int cend = P.cell_off + P.ncell;
+    int v[SCAN_PER_THREAD];
+    int sum = 0;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        int c = c0 + threadIdx.x * SCAN_PER_THREAD + k;
+        v[k] = c < cend ? a.CCOUNT[c] : 0;
+        if (c < cend) a.CCOUNT[c] = 0;  // counts start from zero at the next rebuild
+        sum += v[k];
+    }
+    int tot;
+    int ex = block_exclusive_scan(sum, &tot);
+    int run = P.bead_off + a.CHKSUM[blockIdx.x] + ex;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        int c = c0 + threadIdx.x * SCAN_PER_THREAD + k;
+        if (c < cend) a.CSTART[c] = run;
+        run += v[k];
+        if (c == cend - 1) a.CSTART[cend] = run;
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_scatter(PoolArgs a)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (a.S[r].frozen) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    int2 ko = *reinterpret_cast<const int2*>(a.KEYOFF + 2 * s);
+    int ns = a.CSTART[P.cell_off + ko.x] + ko.y;
+    *reinterpret_cast<int2*>(a.PAIRS + 2 * ns) = make_int2(a.META[2 * s], s);
+}
+
+// Order each cell's beads by original index so the layout (and with it the
+// pair summation order) is a function of the positions alone.
+__global__ void __launch_bounds__(BS) k_cellsort(PoolArgs a)
+{
+    const int r = a.chk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (a.S[r].frozen) return;
+    const int cend = P.cell_off + P.ncell;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        int c = a.chk_cell0[blockIdx.x] + k * BS + threadIdx.x;
+        if (c >= cend) break;
+        int t0 = a.CSTART[c], t1 = a.CSTART[c + 1];
+        for (int i = t0 + 1; i < t1; ++i) {
+            int2 v = *reinterpret_cast<const int2*>(a.PAIRS + 2 * i);
+            int q = i - 1;
+            while (q >= t0 && a.PAIRS[2 * q] > v.x) {
+                a.PAIRS[2 * (q + 1)] = a.PAIRS[2 * q];
+                a.PAIRS[2 * (q + 1) + 1] = a.PAIRS[2 * q + 1];
+                --q;
+            }
+            a.PAIRS[2 * (q + 1)] = v.x;
+            a.PAIRS[2 * (q + 1) + 1] = v.y;
+        }
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (a.S[r].frozen) return;
+    const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (ns >= P.bead_off + P.n) return;
+    int2 pr = *reinterpret_cast<const int2*>(a.PAIRS + 2 * ns);
+    int so = pr.y;
+    double x[DIM];
+    load_vec<DIM>(a.X, so, x);
+    store_vec<DIM>(a.Xn, ns, x);
+    if (with_t) {
+        double t[DIM];
+        load_vec<DIM>(a.T, so, t);
+        store_vec<DIM>(a.Tn, ns, t);
+    }
+    *reinterpret_cast<int2*>(a.METAn + 2 * ns) = *reinterpret_cast<const int2*>(a.META + 2 * so);
+    a.CELL[ns] = a.KEYOFF[2 * so];
+    a.SLOT[pr.x] = ns;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (a.S[r].frozen) return;
+    const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (ns >= P.bead_off + P.n) return;
+    int2 me = *reinterpret_cast<const int2*>(a.METAn + 2 * ns);
+    int first = a.CHAIN_FIRST[me.y], len = a.CHAIN_LEN[me.y];
+    int kk = me.x - first;
+    int4 lk;
+    lk.x = kk >= 2 ? a.SLOT[me.x - 2] : -1;
+    lk.y = kk >= 1 ? a.SLOT[me.x - 1] : -1;
+    lk.z = kk + 1 < len ? a.SLOT[me.x + 1] : -1;
+    lk.w = kk + 2 < len ? a.SLOT[me.x + 2] : -1;
+    *reinterpret_cast<int4*>(a.LINK + 4 * ns) = lk;
+    double x[DIM];
+    load_vec<DIM>(a.Xn, ns, x);
+    store_vec<DIM>(a.X, ns, x);
+    if (with_t) {
+        double t[DIM];
+        load_vec<DIM>(a.Tn, ns, t);
+        store_vec<DIM>(a.T, ns, t);
+    }
+    *reinterpret_cast<int2*>(a.META + 2 * ns) = me;
+}
+
+// ------------------------------------------------------------ batch control
+
+// Start of a batch: remember where each replica's step count stands, clear
+// the violation flag and snapshot the state for a possible replay.
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_batch_begin(PoolArgs a, int with_t)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    RepState* S = a.S + r;
+    if (S->frozen) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (threadIdx.x == 0 && blockIdx.x == P.blk_off) {
+        S->batch_base = S->steps;
+        S->violation = 0;
+        a.Ss[r] = *S;
+    }
+    if (s >= P.bead_off + P.n) return;
+    double x[DIM];
+    load_vec<DIM>(a.X, s, x);
+    store_vec<DIM>(a.Xs, s, x);
+    if (with_t) {
+        double t[DIM];
+        load_vec<DIM>(a.T, s, t);
+        store_vec<DIM>(a.Ts, s, t);
+    }
+    *reinterpret_cast<int2*>(a.METAs + 2 * s) = *reinterpret_cast<const int2*>(a.META + 2 * s);
+}
+
+// Undo a batch for replicas whose flag `frozen` is 2 (host-marked for replay).
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_restore(PoolArgs a, int with_t)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    RepState* S = a.S + r;
+    if (S->frozen != 2) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s < P.bead_off + P.n) {
+        double x[DIM];
+        load_vec<DIM>(a.Xs, s, x);
+        store_vec<DIM>(a.X, s, x);
+        if (with_t) {
+            double t[DIM];
+            load_vec<DIM>(a.Ts, s, t);
+            store_vec<DIM>(a.T, s, t);
+        }
+        *reinterpret_cast<int2*>(a.META + 2 * s) = *reinterpret_cast<const int2*>(a.METAs + 2 * s);
+    }
+}
+
+__global__ void k_restore_state(PoolArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nrep) return;
+    if (a.S[r].frozen == 2) a.S[r] = a.Ss[r];  // snapshot has frozen == 0
+}
+
+// ---------------------------------------------------------------- download
+
+// Gather one replica's per-slot array into original bead order.
+__global__ void __launch_bounds__(BS) k_unpermute(const double* src, double* dst, const int32_t* SLOT,
+                                                  int orig0, int n, int width)
+{
+    int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    int s = SLOT[orig0 + i];
+    for (int c = 0; c < width; ++c) dst[(size_t)i * width + c] = src[(size_t)s * width + c];
+}
+
+// Scatter an original-order array of one replica into slots.
+__global__ void __launch_bounds__(BS) k_permute_in(const double* src, double* dst, const int32_t* SLOT,
+                                                   int orig0, int n, int width)
+{
+    int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    int s = SLOT[orig0 + i];
+    for (int c = 0; c < width; ++c) dst[(size_t)s * width + c] = src[(size_t)i * width + c];
+}
+
+// Tangents from the bonds at X1 (tangent_list after a step holds the
+// tangents computed during the corrector's force evaluation, SRK.py:228-233).
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_tangents(PoolArgs a, int rsel)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    if (r != rsel) return;
+    const RepParams& P = a.P[r];
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    double xs[DIM], acc[DIM] = {}, tang[DIM];
+    load_vec<DIM>(a.X1, s, xs);
+    int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+    int4 lk = *reinterpret_cast<const int4*>(a.LINK + 4 * s);
+    bool set = false;
+    bonded<DIM>(a.X1, s, xs, lk, a.CHAIN_A[me.y], P, acc, tang, set);
+    if (!set) load_vec<DIM>(a.T, s, tang);
+    store_vec<DIM>(a.Tn, s, tang);
+}
+
+}  // namespace gcr

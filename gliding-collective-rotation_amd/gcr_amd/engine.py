@@ -1,0 +1,232 @@
+"""Python handles over libgcr.so: a pool of replicas and the single-system form.
+
+Device state stays resident between calls; host arrays cross only at
+add/upload/download. Every call goes to the HIP library (no CPU path).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import GCR_FP64, GCR_MIXED, ReplicaDesc, check, f64, ptr
+
+__all__ = ["Physics", "Pool", "System", "GCR_FP64", "GCR_MIXED", "init_chains"]
+
+
+@dataclass
+class Physics:
+    """One population's physics as SRK_jit receives it (gcr_replica_desc)."""
+
+    dim: int
+    n_beads: int
+    n_chains: int
+    L: float
+    D: float
+    Dr: float
+    radius: float = 1.0
+    wca: np.ndarray = field(default_factory=lambda: np.array([12.0, 12.0, 0.0, 1.0]))
+    spring: np.ndarray = field(default_factory=lambda: np.array([100.0, 10.0, 0.0, 0.0, 1.0, 0.1]))
+    v0: float = 1.0
+    n_nearby: int = 4
+    has_springs: bool = True
+    has_wca: bool = True
+    has_propel: bool = True
+    skin: float = 0.0
+    seed: int = 0
+
+    def desc(self):
+        d = ReplicaDesc()
+        d.n_beads = int(self.n_beads)
+        d.n_chains = int(self.n_chains)
+        d.n_nearby = int(self.n_nearby)
+        d.has_springs = int(bool(self.has_springs))
+        d.has_wca = int(bool(self.has_wca))
+        d.has_propel = int(bool(self.has_propel))
+        d.L = float(self.L)
+        d.D = float(self.D)
+        d.Dr = float(self.Dr)
+        d.radius = float(self.radius)
+        for i in range(4):
+            d.wca[i] = float(self.wca[i])
+        for i in range(6):
+            d.spring[i] = float(self.spring[i])
+        d.v0 = float(self.v0)
+        d.skin = float(self.skin)
+        d.seed = int(self.seed) & ((1 << 64) - 1)
+        return d
+
+
+def _time_vals(tv):
+    tv = f64(tv)
+    assert tv.shape == (8,)
+    return tv
+
+
+class Pool:
+    """Many replicas of one dimensionality advanced in the same launches."""
+
+    def __init__(self, dim, precision=GCR_FP64, bead_capacity=1 << 20, cell_capacity=1 << 20,
+                 chain_capacity=1 << 18, replica_capacity=64):
+        self._L = _lib.load()
+        self.dim = dim
+        self.precision = precision
+        h = C.c_void_p()
+        check(self._L.gcr_pool_create(dim, precision, int(bead_capacity), int(cell_capacity),
+                                      int(chain_capacity), int(replica_capacity), C.byref(h)))
+        self.h = h
+        self.n = {}
+
+    def close(self):
+        if self.h:
+            check(self._L.gcr_pool_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_schedule(self, steps_per_batch, rebuild_every):
+        check(self._L.gcr_pool_set_schedule(self.h, int(steps_per_batch), int(rebuild_every)))
+
+    def add(self, phys, pos, tangent, chain_id, rest_lengths, v_tan_angle=None,
+            v0_per_chain=None):
+        pos = f64(pos)
+        tangent = f64(tangent)
+        cid = np.ascontiguousarray(chain_id, dtype=np.int64)
+        rl = f64(rest_lengths)
+        vt = f64(v_tan_angle) if v_tan_angle is not None else np.zeros(phys.n_chains)
+        v0c = f64(v0_per_chain)
+        assert pos.shape == (phys.n_beads, self.dim)
+        d = phys.desc()
+        rid = C.c_int32()
+        check(self._L.gcr_replica_add(self.h, C.byref(d), ptr(pos), ptr(tangent), ptr(cid),
+                                      ptr(rl), ptr(vt), ptr(v0c), C.byref(rid)))
+        self.n[rid.value] = phys.n_beads
+        return rid.value
+
+    def remove(self, rid):
+        check(self._L.gcr_replica_remove(self.h, int(rid)))
+        self.n.pop(rid, None)
+
+    def upload(self, rid, pos=None, tangent=None):
+        pos, tangent = f64(pos), f64(tangent)
+        check(self._L.gcr_replica_upload(self.h, int(rid), ptr(pos), ptr(tangent)))
+
+    def begin(self, rid, time_vals, xi=None):
+        tv = _time_vals(time_vals)
+        xi = f64(xi)
+        steps = 0 if xi is None else xi.shape[0]
+        if xi is not None:
+            assert xi.shape[1:] == (self.n[rid], 2 * self.dim)
+        check(self._L.gcr_replica_begin(self.h, int(rid), ptr(tv), ptr(xi), int(steps)))
+
+    def run(self, max_batches=0):
+        nr = C.c_int32()
+        check(self._L.gcr_pool_run(self.h, int(max_batches), C.byref(nr)))
+        return nr.value
+
+    def status(self, rid):
+        out = np.zeros(8)
+        check(self._L.gcr_replica_status(self.h, int(rid),
+                                         out.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(dt=out[0], elapsed=out[1], steps=int(out[2]), max_forces_strength=out[3],
+                    running=bool(out[4]), replays=int(out[5]), step_global=int(out[6]))
+
+    def download(self, rid, pos=True, pos1=True, tangent=True, forces=True):
+        n = self.n[rid]
+        shape = (n, self.dim)
+        out = {k: (np.empty(shape) if want else None)
+               for k, want in (("pos", pos), ("pos1", pos1), ("tangent", tangent),
+                               ("forces", forces))}
+        check(self._L.gcr_replica_download(self.h, int(rid), ptr(out["pos"]), ptr(out["pos1"]),
+                                           ptr(out["tangent"]), ptr(out["forces"])))
+        return {k: v for k, v in out.items() if v is not None}
+
+    def eval_forces(self, rid):
+        n = self.n[rid]
+        F = np.empty((n, self.dim))
+        T = np.empty((n, self.dim))
+        check(self._L.gcr_replica_eval_forces(self.h, int(rid), ptr(F), ptr(T)))
+        return F, T
+
+    def time_kernels(self, steps):
+        out = np.zeros(4)
+        check(self._L.gcr_pool_time_kernels(self.h, int(steps),
+                                            out.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(stage0=out[0], stage1=out[1], update=out[2], rebuild=out[3])
+
+
+class System:
+    """Single-system form: the SRK_jit replacement (gcr_create / gcr_run ...)."""
+
+    def __init__(self, phys, pos, tangent, chain_id, rest_lengths, v_tan_angle=None,
+                 precision=GCR_FP64):
+        self._L = _lib.load()
+        self.dim = phys.dim
+        self.n = phys.n_beads
+        pos, tangent = f64(pos), f64(tangent)
+        cid = np.ascontiguousarray(chain_id, dtype=np.int64)
+        rl = f64(rest_lengths)
+        vt = f64(v_tan_angle) if v_tan_angle is not None else np.zeros(phys.n_chains)
+        d = phys.desc()
+        h = C.c_void_p()
+        check(self._L.gcr_create(C.byref(d), phys.dim, precision, ptr(pos), ptr(tangent),
+                                 ptr(cid), ptr(rl), ptr(vt), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self._L.gcr_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, pos=None, tangent=None):
+        pos, tangent = f64(pos), f64(tangent)
+        check(self._L.gcr_upload(self.h, ptr(pos), ptr(tangent)))
+
+    def run(self, time_vals, xi=None):
+        tv = _time_vals(time_vals)
+        xi = f64(xi)
+        st = np.zeros(6)
+        steps = 0 if xi is None else xi.shape[0]
+        check(self._L.gcr_run(self.h, ptr(tv), ptr(st), ptr(xi), int(steps)))
+        return st
+
+    def download(self):
+        out = {k: np.empty((self.n, self.dim)) for k in ("pos", "pos1", "tangent", "forces")}
+        check(self._L.gcr_download(self.h, ptr(out["pos"]), ptr(out["pos1"]),
+                                   ptr(out["tangent"]), ptr(out["forces"])))
+        return out
+
+    def eval_forces(self):
+        F = np.empty((self.n, self.dim))
+        T = np.empty((self.n, self.dim))
+        check(self._L.gcr_eval_forces(self.h, ptr(F), ptr(T)))
+        return F, T
+
+
+def init_chains(dim, L, chain_len, bond, min_dist=0.9, seed=0, max_tries=100000):
+    """Random non-overlapping straight chains (gcr_init_chains)."""
+    lib = _lib.load()
+    cl = np.ascontiguousarray(chain_len, dtype=np.int32)
+    n = int(cl.sum())
+    pos = np.empty((n, dim))
+    tang = np.empty((n, dim))
+    f = lib.gcr_init_chains
+    f.restype = C.c_int
+    f.argtypes = [C.c_int32, C.c_double, C.c_int64, C.c_void_p, C.c_double, C.c_double,
+                  C.c_uint64, C.c_int64, C.c_void_p, C.c_void_p]
+    rc = f(dim, float(L), len(cl), ptr(cl), float(bond), float(min_dist),
+           int(seed) & ((1 << 64) - 1), int(max_tries), ptr(pos), ptr(tang))
+    if rc != 0:
+        raise _lib.GcrError(
+            f"gcr_init_chains failed ({rc}): could not place every chain without overlap")
+    return pos, tang

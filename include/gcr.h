@@ -1,0 +1,147 @@
+/*
+ * gcr.h -- C-ABI of the MI355X Brownian-dynamics hot path (libgcr.so).
+ *
+ * Plain C types only: caller-owned host buffers, opaque handles, int return
+ * codes (0 = ok, < 0 = error, message from gcr_last_error()). All arrays are
+ * C-contiguous float64 in the reference's layout: pos[n][dim],
+ * tangent[n][dim], noise[steps][n][2*dim]; chain_id is int64 per bead as
+ * populations.py:606 builds it (chains are runs of equal chain_id).
+ *
+ * Two layers:
+ *  - the single-system API (gcr_create / gcr_upload / gcr_run /
+ *    gcr_download / gcr_destroy) is what updates.SRK2_force_calc would bind
+ *    in place of its SRK_jit call;
+ *  - the pool API runs many replicas (independent simulations of one
+ *    dimensionality) in the same launches; the ensemble and the serving
+ *    loop are built on it.
+ */
+#ifndef GCR_H
+#define GCR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCR_OK 0
+#define GCR_ERR_ARG (-1)      /* bad argument (size, pointer, parameter) */
+#define GCR_ERR_HIP (-2)      /* HIP runtime error, or no device */
+#define GCR_ERR_CAPACITY (-3) /* pool capacity exhausted */
+#define GCR_ERR_STATE (-4)    /* call out of order (e.g. run before begin) */
+
+#define GCR_FP64 0  /* forces and accumulation in FP64 */
+#define GCR_MIXED 1 /* pair forces in FP32, accumulation and state in FP64 */
+
+typedef struct gcr_pool gcr_pool;
+
+/* Physics of one population, as SRK_jit receives it.
+ * Fields map to: pop_consts = [D, Dr, radius, rigid_rod, n_nearby, Lmax]
+ * (updates.py:148-162), force_params rows for "WCA" (populations.py:48-58),
+ * "springs" (populations.py:345-353) and "self-propulsion"
+ * (populations.py:355-358), and config.Lx. */
+typedef struct gcr_replica_desc {
+    int32_t n_beads;
+    int32_t n_chains;
+    int32_t n_nearby;    /* n_nearby_beads_dont_repel */
+    int32_t has_springs; /* "springs" in the force list */
+    int32_t has_wca;     /* "WCA" in the force list */
+    int32_t has_propel;  /* "self-propulsion" and status.b_calc_active_forces */
+    double L;            /* periodic box edge, config.Lx */
+    double D, Dr, radius;
+    double wca[4];       /* [12 eps r^6, 12 eps r^12, WCA_shift, LJ cutoff] */
+    double spring[6];    /* [k, kappa, spontaneous angle, FENE, get_orient, FENE tol] */
+    double v0;           /* active_params[0], used when v0_per_chain is NULL */
+    double skin;         /* cell-list skin; <= 0 picks 0.3 * cutoff */
+    uint64_t seed;       /* key of the in-kernel Philox noise */
+} gcr_replica_desc;
+
+/* ---------------------------------------------------------------- errors */
+const char* gcr_last_error(void);
+/* Device name, CU count and HBM bytes of the current device. */
+int gcr_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_bytes);
+
+/* ------------------------------------------------------ single system API */
+
+/* Replaces the argument marshalling of updates.SRK2_force_calc
+ * (updates.py:132-247) and the state a SRK_jit call reads (SRK.py:14-29). */
+int gcr_create(const gcr_replica_desc* desc, int32_t dim, int32_t precision, const double* pos,
+               const double* tangent, const int64_t* chain_id, const double* rest_lengths,
+               const double* v_tan_angle, gcr_pool** out);
+
+/* Re-upload positions / propulsion directions (NULL leaves one unchanged),
+ * e.g. after the host edited pop.pos between calls. */
+int gcr_upload(gcr_pool* h, const double* pos, const double* tangent);
+
+/* Replaces SRK.SRK_jit (SRK.py:13-484) for SRK_order = 2: runs steps until
+ * time_vals[7] steps (max_steps) or time_vals[4] elapsed time (max_dt_usable)
+ * is reached, as the loop at SRK.py:128-130. time_vals is the array of
+ * updates.py:144-146. status_out = shared_arr[0] of updates.py:249-250:
+ * [dt, elapsed (dt_used sum), steps, max_forces_strength, 0, flag].
+ * xi: NULL -> in-kernel Philox noise; else xi[k][n][2*dim] for the k-th step
+ * of this call (noise injection, k < xi_steps). */
+int gcr_run(gcr_pool* h, const double time_vals[8], double status_out[6], const double* xi,
+            int64_t xi_steps);
+
+/* Copies back pos, pos1 (SRK2 predictor of the last step), tangent_list and
+ * total forces (F0 + F1 of the last step), each [n][dim]; NULL skips one. */
+int gcr_download(gcr_pool* h, double* pos, double* pos1, double* tangent, double* forces);
+
+/* One force evaluation (springs, pair, propulsion; forces.py:57-402) at the
+ * current positions, without integrating: forces[n][dim], tangent[n][dim]. */
+int gcr_eval_forces(gcr_pool* h, double* forces, double* tangent);
+
+int gcr_destroy(gcr_pool* h);
+
+/* -------------------------------------------------------------- pool API */
+
+/* A pool of replicas of one dimensionality; capacities bound HBM use. */
+int gcr_pool_create(int32_t dim, int32_t precision, int64_t bead_capacity, int64_t cell_capacity,
+                    int64_t chain_capacity, int32_t replica_capacity, gcr_pool** out);
+/* Steps per graph-captured batch and cell-list rebuild period (steps);
+ * steps_per_batch is rounded up to a multiple of rebuild_every. */
+int gcr_pool_set_schedule(gcr_pool* p, int32_t steps_per_batch, int32_t rebuild_every);
+/* HBM bytes a replica of n beads, n_chains chains and ncell cells takes. */
+int64_t gcr_replica_bytes(int32_t dim, int64_t n_beads, int64_t n_chains, int64_t n_cells);
+/* Cells a replica would use (needed to size cell_capacity). */
+int64_t gcr_replica_cells(int32_t dim, const gcr_replica_desc* desc);
+
+int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* desc, const double* pos,
+                    const double* tangent, const int64_t* chain_id, const double* rest_lengths,
+                    const double* v_tan_angle, const double* v0_per_chain, int32_t* rid);
+int gcr_replica_remove(gcr_pool* p, int32_t rid);
+int gcr_replica_upload(gcr_pool* p, int32_t rid, const double* pos, const double* tangent);
+/* Arms the next call of a replica (same meaning of time_vals / xi as gcr_run). */
+int gcr_replica_begin(gcr_pool* p, int32_t rid, const double time_vals[8], const double* xi,
+                      int64_t xi_steps);
+/* Advances every armed replica by up to max_batches batches (<= 0: until all
+ * are done). n_running receives how many are still running. */
+int gcr_pool_run(gcr_pool* p, int32_t max_batches, int32_t* n_running);
+/* [dt, elapsed, steps, max_forces_strength, running, replays, step_global, n] */
+int gcr_replica_status(gcr_pool* p, int32_t rid, double out[8]);
+int gcr_replica_download(gcr_pool* p, int32_t rid, double* pos, double* pos1, double* tangent,
+                         double* forces);
+int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* tangent);
+/* Per-kernel timing: runs `steps` steps of every armed replica as individual
+ * launches with HIP events on the pool's stream; ms_out[k] = average duration
+ * of kernel k in [stage0, stage1, update, rebuild(all kernels)]. */
+int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4]);
+int gcr_pool_destroy(gcr_pool* p);
+
+/* ------------------------------------------------- initial configuration */
+
+/* Host-side initial condition (no device work): n_chains straight chains of
+ * chain_len[c] beads at spacing `bond`, centres uniform in the box and
+ * directions uniform on the circle / sphere, placed one after another and
+ * rejected while any bead comes closer than min_dist to a placed bead.
+ * Bead 0 of each chain leads; tangent[b] is the chain direction, pointing
+ * towards bead 0 as the reference's tangents do (forces.py:351-359). Plays
+ * the role of random_initialization + the growth phase
+ * (populations.py:530-557, control.py:153-219). Returns GCR_ERR_STATE when a
+ * chain could not be placed in max_tries attempts. */
+int gcr_init_chains(int32_t dim, double L, int64_t n_chains, const int32_t* chain_len, double bond,
+                    double min_dist, uint64_t seed, int64_t max_tries, double* pos, double* tangent);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

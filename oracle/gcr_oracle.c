@@ -1,0 +1,568 @@
+/*
+ * gcr_oracle.c -- CPU FP64 restatement of the reference hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline). Nothing in the
+ * product library links this file. See gcr_oracle.h for the interface and the
+ * reference lines each function restates.
+ */
+#include "gcr_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+double orc_pymod(double a, double b)
+{
+    /* CPython float_rem: fmod, then move into the divisor's sign. */
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0) != (m < 0)) m += b;
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+void orc_wca_params(double r, double eps, double shift, double cutoff_factor, double out[4])
+{
+    /* populations.py:48-58 */
+    double cut = cutoff_factor * r - shift;
+    double s6 = pow(r, 6.0);
+    double fac = 12.0 * s6 * eps;
+    out[0] = fac;
+    out[1] = fac * s6;
+    out[2] = shift;
+    out[3] = cut;
+}
+
+double orc_wca_coeff(double ds, const double p[4])
+{
+    /* forces.py:13-15 */
+    return pow(ds, -4.0) * (p[0] - p[1] * pow(ds, -3.0));
+}
+
+double orc_inbounds_diff(int dim, const double* a, const double* b, double hb, double L,
+                         double* out)
+{
+    /* forces.py:44-54 */
+    double ds = 0.0;
+    for (int i = 0; i < dim; ++i) {
+        out[i] = a[i] - b[i];
+        if (out[i] < -hb)
+            out[i] += L;
+        else if (out[i] >= hb)
+            out[i] -= L;
+        ds += out[i] * out[i];
+    }
+    return ds;
+}
+
+static double sgn(double x) { return (x > 0) - (x < 0); }
+
+void orc_chain_forces(int dim, int n, const double* pos, const int64_t* chain_id, double L,
+                      const double sp[6], const double* rest_lengths, double* tangent,
+                      double* F)
+{
+    /* forces.py:285-402, Hookean branch. The reference walks beads in index
+     * order, keeps the two most recent normalised bond vectors in a ring
+     * (ti = bond b1->b1+1, tj = bond b1-1->b1) and adds each term to the
+     * beads it touches as soon as it is known. */
+    double half = L / 2.0;
+    int bend = (sp[1] != 0.0);
+    int orient = (sp[4] != 0.0);
+    double tg[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int ti = 0, tj = 1, new_chain = 1;
+    int64_t ci = 0;
+    double a = 0.0;
+    for (int b1 = 0; b1 < n; ++b1) {
+        if (new_chain) {
+            ti = 0;
+            tj = 1;
+            ci = chain_id[b1];
+            a = rest_lengths[ci];
+        }
+        int end = (b1 == n - 1) || (chain_id[b1 + 1] != ci);
+        if (!end) {
+            double ds = orc_inbounds_diff(dim, pos + b1 * dim, pos + (b1 + 1) * dim, half, L,
+                                          tg[ti]);
+            if (ds != 0.0) {
+                double inv = pow(ds, -0.5);
+                double mag = sp[0] * (a * inv - 1.0);
+                for (int i = 0; i < dim; ++i) {
+                    double f = tg[ti][i] * mag;
+                    F[b1 * dim + i] += f;
+                    F[(b1 + 1) * dim + i] -= f;
+                    tg[ti][i] *= inv;
+                }
+            }
+        }
+        if (orient || bend) {
+            if (new_chain) {
+                if (orient)
+                    for (int i = 0; i < dim; ++i) tangent[b1 * dim + i] = tg[ti][i];
+            } else if (end) {
+                if (orient)
+                    for (int i = 0; i < dim; ++i) tangent[b1 * dim + i] = tg[tj][i];
+            } else {
+                double dot = 0.0;
+                for (int i = 0; i < dim; ++i) dot += tg[ti][i] * tg[tj][i];
+                if (-1.0 < dot && dot < 1.0) {
+                    if (bend) {
+                        double phi = acos(dot);
+                        if (sp[2] != 0.0)
+                            phi += sp[2] * sgn(-tg[ti][0] * tg[tj][1] + tg[ti][1] * tg[tj][0]);
+                        phi *= sp[1] / sqrt(1.0 - dot * dot);
+                        for (int i = 0; i < dim; ++i) {
+                            double ff = phi * (dot * tg[tj][i] - tg[ti][i]);
+                            double fb = phi * (tg[tj][i] - dot * tg[ti][i]);
+                            F[b1 * dim + i] += ff + fb;
+                            F[(b1 - 1) * dim + i] -= ff;
+                            F[(b1 + 1) * dim + i] -= fb;
+                        }
+                    }
+                    if (orient) {
+                        double c = pow(2.0 * (1.0 + dot), -0.5);
+                        for (int i = 0; i < dim; ++i)
+                            tangent[b1 * dim + i] = c * (tg[ti][i] + tg[tj][i]);
+                    }
+                } else if (orient) {
+                    for (int i = 0; i < dim; ++i) tangent[b1 * dim + i] = tg[ti][i];
+                }
+            }
+        }
+        tj = ti;
+        ti = 1 - ti;
+        new_chain = end;
+    }
+}
+
+/* The pair term for one ordered pair, forces.py:100-133. Returns 1 and the
+ * force on b1 in f[] when the pair is inside the cutoff. */
+static int pair_term(int dim, const double* x1, const double* x2, double L, double half,
+                     const double p[4], double rc2, double* f)
+{
+    double seps[3];
+    double ds = 0.0;
+    for (int i = 0; i < dim; ++i) {
+        double sep = x2[i] - x1[i];
+        if (sep < -half)
+            sep += L;
+        else if (sep >= half)
+            sep -= L;
+        seps[i] = sep;
+        ds += sep * sep;
+    }
+    if (!(0.0 < ds && ds <= rc2)) return 0;
+    if (p[2] != 0.0) {
+        double mag = sqrt(ds);
+        double nm = mag + p[2];
+        if (nm < 0.0) nm = 0.0;
+        double rescale = nm / mag;
+        for (int i = 0; i < dim; ++i) seps[i] *= rescale;
+        ds = nm * nm;
+    }
+    double c = orc_wca_coeff(ds, p);
+    for (int i = 0; i < dim; ++i) f[i] = seps[i] * c;
+    return 1;
+}
+
+static int excluded(const int64_t* chain_id, int b1, int b2, int n_nearby)
+{
+    /* updates.py:645-650 */
+    if (chain_id[b1] != chain_id[b2]) return 0;
+    if (n_nearby < 0) return 1;
+    return abs(b1 - b2) <= n_nearby;
+}
+
+void orc_pair_forces_brute(int dim, int n, const double* pos, const int64_t* chain_id,
+                           double L, const double p[4], int n_nearby, double* F)
+{
+    double half = L / 2.0, rc2 = p[3] * p[3];
+    for (int b1 = 0; b1 < n; ++b1) {
+        for (int b2 = 0; b2 < n; ++b2) {
+            if (excluded(chain_id, b1, b2, n_nearby)) continue;
+            double f[3];
+            if (pair_term(dim, pos + b1 * dim, pos + b2 * dim, L, half, p, rc2, f))
+                for (int i = 0; i < dim; ++i) F[b1 * dim + i] += f[i];
+        }
+    }
+}
+
+void orc_self_propel(int dim, int n, const int64_t* chain_id, double v0, const double* vtan,
+                     const double* t, double* F)
+{
+    /* forces.py:206-223, angle looked up per chain. */
+    for (int b = 0; b < n; ++b) {
+        double ang = vtan ? vtan[chain_id[b]] : 0.0;
+        if (ang == 0.0 || dim != 2) {
+            for (int i = 0; i < dim; ++i) F[b * dim + i] += v0 * t[b * dim + i];
+        } else {
+            double sn = sin(ang), cs = cos(ang);
+            F[b * dim + 0] += v0 * (cs * t[b * dim + 0] - sn * t[b * dim + 1]);
+            F[b * dim + 1] += v0 * (sn * t[b * dim + 0] + cs * t[b * dim + 1]);
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Verlet list on a cell grid (updates.py:502-669), rebuilt on a cadence,
+ * with the reference's thread chunking (updates.py:323-344) and Newton-3
+ * inside a chunk (forces.py:95-99, 134-143). Used by the CPU baseline. */
+
+typedef struct verlet {
+    int n, dim, n_threads;
+    int* start;     /* CSR row starts, n+1 */
+    int* list;      /* neighbour indices */
+    size_t cap;
+    int* chunk;     /* n_threads+1 chain-aligned chunk bounds */
+    int n_grid, window;
+    double radius2;
+    int* cell_head; /* cell -> first bead (linked list) */
+    int* cell_next;
+} verlet;
+
+static void chunk_chain(int T, int n, const int64_t* chain_id, int* bounds)
+{
+    /* updates.py:323-344: cut after roughly n/T beads, moved forward to the
+     * next chain boundary. */
+    int chunklen = (n + T - 1) / T;
+    bounds[0] = 0;
+    int b = 0;
+    for (int t = 1; t < T; ++t) {
+        int test = (t * chunklen - 1 > b) ? t * chunklen - 1 : b;
+        if (test > n - 1) test = n - 1;
+        int64_t c = chain_id[test];
+        while (test < n && chain_id[test] == c) ++test;
+        b = test;
+        bounds[t] = b;
+    }
+    bounds[T] = n;
+}
+
+static void verlet_build(verlet* v, int dim, int n, const double* pos, const int64_t* chain_id,
+                         double L, int n_nearby)
+{
+    int ng = v->n_grid, w = v->window;
+    int ncell = ng * ng * (dim == 3 ? ng : 1);
+    double h = L / ng, half = L / 2.0;
+    for (int c = 0; c < ncell; ++c) v->cell_head[c] = -1;
+    /* insert in descending order so each cell lists beads ascending, as
+     * update_grid_list appends them (updates.py:572-582). */
+    for (int b = n - 1; b >= 0; --b) {
+        int g[3] = {0, 0, 0};
+        for (int i = 0; i < dim; ++i) {
+            int gi = (int)floor(pos[b * dim + i] / h);
+            if (gi >= ng) gi = ng - 1;
+            if (gi < 0) gi = 0;
+            g[i] = gi;
+        }
+        int c = (g[2] * ng + g[1]) * ng + g[0];
+        v->cell_next[b] = v->cell_head[c];
+        v->cell_head[c] = b;
+    }
+    int dgs = -(w < ng / 2 ? w : ng / 2);
+    int dge = (1 + w < ng + dgs) ? 1 + w : ng + dgs;
+    size_t cnt = 0;
+    v->start[0] = 0;
+    for (int b1 = 0; b1 < n; ++b1) {
+        int g[3] = {0, 0, 0};
+        for (int i = 0; i < dim; ++i) {
+            int gi = (int)floor(pos[b1 * dim + i] / h);
+            if (gi >= ng) gi = ng - 1;
+            if (gi < 0) gi = 0;
+            g[i] = gi;
+        }
+        int zs = (dim == 3) ? dgs : 0, ze = (dim == 3) ? dge : 1;
+        for (int dx = dgs; dx < dge; ++dx)
+            for (int dy = dgs; dy < dge; ++dy)
+                for (int dz = zs; dz < ze; ++dz) {
+                    int gx = ((g[0] + dx) % ng + ng) % ng;
+                    int gy = ((g[1] + dy) % ng + ng) % ng;
+                    int gz = (dim == 3) ? ((g[2] + dz) % ng + ng) % ng : 0;
+                    for (int b2 = v->cell_head[(gz * ng + gy) * ng + gx]; b2 >= 0;
+                         b2 = v->cell_next[b2]) {
+                        if (excluded(chain_id, b1, b2, n_nearby)) continue;
+                        double d2 = 0.0;
+                        for (int i = 0; i < dim; ++i) {
+                            double s = pos[b2 * dim + i] - pos[b1 * dim + i];
+                            if (s < -half)
+                                s += L;
+                            else if (s >= half)
+                                s -= L;
+                            d2 += s * s;
+                        }
+                        if (d2 <= v->radius2) {
+                            if (cnt == v->cap) {
+                                v->cap *= 2;
+                                v->list = (int*)realloc(v->list, v->cap * sizeof(int));
+                            }
+                            v->list[cnt++] = b2;
+                        }
+                    }
+                }
+        v->start[b1 + 1] = (int)cnt;
+    }
+}
+
+static void verlet_forces(const verlet* v, int dim, const double* pos, double L,
+                          const double p[4], double* F)
+{
+    double half = L / 2.0, rc2 = p[3] * p[3];
+#pragma omp parallel for schedule(static, 1) num_threads(v->n_threads)
+    for (int t = 0; t < v->n_threads; ++t) {
+        int s0 = v->chunk[t], s1 = v->chunk[t + 1];
+        for (int b1 = s0; b1 < s1; ++b1) {
+            for (int k = v->start[b1]; k < v->start[b1 + 1]; ++k) {
+                int b2 = v->list[k];
+                if (b1 <= b2 && b2 < s1) continue;
+                int same = (b1 > b2 && b2 >= s0);
+                double f[3];
+                if (!pair_term(dim, pos + b1 * dim, pos + b2 * dim, L, half, p, rc2, f))
+                    continue;
+                for (int i = 0; i < dim; ++i) {
+                    F[b1 * dim + i] += f[i];
+                    if (same) F[b2 * dim + i] -= f[i];
+                }
+            }
+        }
+    }
+}
+
+void orc_force_eval(const orc_sys* s, const double* pos, const int64_t* chain_id,
+                    const double* rest_lengths, const double* vtan, double* tangent,
+                    double* F, int pair_mode, void* vv)
+{
+    int dim = s->dim, n = s->n;
+    if (s->has_springs) {
+        if (pair_mode == 1 && vv) {
+            const verlet* v = (const verlet*)vv;
+#pragma omp parallel for schedule(static, 1) num_threads(v->n_threads)
+            for (int t = 0; t < v->n_threads; ++t) {
+                int s0 = v->chunk[t], s1 = v->chunk[t + 1];
+                if (s1 > s0)
+                    orc_chain_forces(dim, s1 - s0, pos + s0 * dim, chain_id + s0, s->L,
+                                     s->spring, rest_lengths, tangent + s0 * dim,
+                                     F + s0 * dim);
+            }
+        } else {
+            orc_chain_forces(dim, n, pos, chain_id, s->L, s->spring, rest_lengths, tangent, F);
+        }
+    }
+    if (s->has_wca) {
+        if (pair_mode == 1 && vv)
+            verlet_forces((const verlet*)vv, dim, pos, s->L, s->wca, F);
+        else
+            orc_pair_forces_brute(dim, n, pos, chain_id, s->L, s->wca, s->n_nearby, F);
+    }
+    if (s->has_propel) orc_self_propel(dim, n, chain_id, s->v0, vtan, tangent, F);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Philox4x32-10, Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11. Multipliers and Weyl key increments are the
+ * published constants. */
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* uniform in (0,1) with 2^-32 resolution, never 0 or 1 */
+static double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
+
+static void box_muller(uint32_t a, uint32_t b, double* n0, double* n1)
+{
+    double r = sqrt(-2.0 * log(u01(a)));
+    double th = 6.283185307179586 * u01(b);
+    *n0 = r * cos(th);
+    *n1 = r * sin(th);
+}
+
+void orc_bead_noise(int dim, uint64_t seed, uint32_t orig, int64_t step, double* xi)
+{
+    /* Counter = (bead, step lo, step hi, block); key = seed. Block 0 gives
+     * four normals (2D: translational pair then rotational pair). 3D uses
+     * block 0 for x,y,z + rot x and block 1 for rot y,z. */
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {orig, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), 0u};
+    uint32_t o[4];
+    double g[8];
+    orc_philox4x32_10(ctr, key, o);
+    box_muller(o[0], o[1], &g[0], &g[1]);
+    box_muller(o[2], o[3], &g[2], &g[3]);
+    if (dim == 3) {
+        ctr[3] = 1u;
+        orc_philox4x32_10(ctr, key, o);
+        box_muller(o[0], o[1], &g[4], &g[5]);
+        box_muller(o[2], o[3], &g[6], &g[7]);
+    }
+    for (int i = 0; i < 2 * dim; ++i) xi[i] = g[i];
+}
+
+/* ---------------------------------------------------------------------- */
+
+int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangent, double* F,
+                     const int64_t* chain_id, const double* rest_lengths, const double* vtan,
+                     const double tv[8], double status_out[4], const double* xi_in,
+                     uint64_t seed, int64_t step0, int pair_mode, int n_threads,
+                     double grid_factor)
+{
+    int dim = s->dim, n = s->n;
+    double dt = tv[0], dtmax = tv[1], mdps = tv[2], cheat = tv[3], mdu = tv[4];
+    double until = tv[5], n_upd = tv[6], max_steps = tv[7];
+    double L = s->L, D = s->D, Dr = s->Dr, r = s->radius;
+    int orient = s->has_springs && (s->spring[4] != 0.0);
+    int nd = 2 * dim;
+    if (n_threads < 1) n_threads = 1;
+    double* xi_buf = (double*)malloc(sizeof(double) * (size_t)n * nd);
+    double* tmpT = NULL;
+    (void)tmpT;
+
+    verlet vv;
+    memset(&vv, 0, sizeof vv);
+    if (pair_mode == 1 && s->has_wca) {
+        /* grid constants as config.grid_list_setup (config.py:147-193) and
+         * the in-loop window of SRK.py:99-103 */
+        double maxrest = 0.0;
+        if (s->has_springs) {
+            int64_t nc = chain_id[n - 1] + 1;
+            for (int64_t c = 0; c < nc; ++c)
+                if (rest_lengths[c] > maxrest) maxrest = rest_lengths[c];
+        }
+        double r_int = 2.0 * maxrest > s->wca[3] ? 2.0 * maxrest : s->wca[3];
+        int ng = (int)(L / (grid_factor * r_int));
+        if (ng < 1) ng = 1;
+        double h = L / ng;
+        int w = (int)ceil(grid_factor * s->wca[3] / h);
+        if (w < 1) w = 1;
+        vv.n = n;
+        vv.dim = dim;
+        vv.n_threads = n_threads;
+        vv.n_grid = ng;
+        vv.window = w;
+        vv.radius2 = (w * h) * (w * h);
+        vv.start = (int*)malloc(sizeof(int) * (n + 1));
+        vv.cap = (size_t)n * 64 + 64;
+        vv.list = (int*)malloc(sizeof(int) * vv.cap);
+        vv.chunk = (int*)malloc(sizeof(int) * (n_threads + 1));
+        int ncell = ng * ng * (dim == 3 ? ng : 1);
+        vv.cell_head = (int*)malloc(sizeof(int) * ncell);
+        vv.cell_next = (int*)malloc(sizeof(int) * n);
+        chunk_chain(n_threads, n, chain_id, vv.chunk);
+        verlet_build(&vv, dim, n, pos, chain_id, L, s->n_nearby);
+    }
+    void* vp = (pair_mode == 1 && s->has_wca) ? (void*)&vv : NULL;
+
+    int64_t steps = 0;
+    double elapsed = 0.0, m = 0.0, dt_for_update = dt;
+    while ((steps < max_steps || max_steps < 0) && (elapsed < mdu || mdu < 0)) {
+        /* noise for this step (SRK.py:134-142): drawn when D>0 / Dr>0 */
+        const double* xi;
+        if (xi_in) {
+            xi = xi_in + (size_t)steps * n * nd;
+        } else {
+#pragma omp parallel for num_threads(n_threads)
+            for (int b = 0; b < n; ++b)
+                orc_bead_noise(dim, seed, (uint32_t)b, step0 + steps, xi_buf + (size_t)b * nd);
+            xi = xi_buf;
+        }
+        memset(F, 0, sizeof(double) * (size_t)n * dim);
+        for (int stage = 0; stage < 2; ++stage) {
+            double pref = stage == 0 ? 1.0 : 0.5;
+            const double* at = stage == 0 ? pos : pos1;
+            double* out = stage == 0 ? pos1 : pos;
+            orc_force_eval(s, at, chain_id, rest_lengths, vtan, tangent, F, pair_mode, vp);
+            if (stage == 1) {
+                /* SRK.py:324-353 */
+                double mx = 0.0;
+                for (int b = 0; b < n; ++b) {
+                    double q = 0.0;
+                    for (int i = 0; i < dim; ++i) q += F[b * dim + i] * F[b * dim + i];
+                    if (q > mx) mx = q;
+                }
+                m = sqrt(2.0 * D);
+                if (n > 0) m += pref * sqrt(mx) / r;
+                steps += 1;
+                if (m == 0.0)
+                    dt = dtmax;
+                else
+                    dt = fmin(dtmax, mdps / m);
+                double dt_used = cheat * dt;
+                if (mdu > 0) dt_used = fmin(dt_used, mdu - elapsed);
+                elapsed += dt_used;
+                dt_for_update = dt_used;
+            } else {
+                dt_for_update = dt;
+            }
+            /* SRK.py:355-388 */
+            double dt_eff = dt_for_update * pref;
+            double s2 = sqrt(2.0 * dt_for_update * D);
+            dt_eff /= r;
+#pragma omp parallel for num_threads(n_threads)
+            for (int b = 0; b < n; ++b)
+                for (int i = 0; i < dim; ++i) {
+                    double tmp = pos[b * dim + i] + dt_eff * F[b * dim + i];
+                    if (s2 != 0.0) tmp += s2 * xi[(size_t)b * nd + i];
+                    out[b * dim + i] = orc_pymod(tmp, L);
+                }
+        }
+        /* rotational diffusion, SRK.py:398-423 */
+        if (Dr > 0 && !orient) {
+            double sr = sqrt(2.0 * dt_for_update * Dr) * 3.141592653589793;
+            for (int b = 0; b < n; ++b) {
+                double nr[3], proj = 0.0;
+                for (int i = 0; i < dim; ++i) nr[i] = xi[(size_t)b * nd + dim + i];
+                for (int i = 0; i < dim; ++i) proj += nr[i] * tangent[b * dim + i];
+                for (int i = 0; i < dim; ++i) nr[i] -= proj * tangent[b * dim + i];
+                double mag = 0.0;
+                for (int i = 0; i < dim; ++i) {
+                    double t = tangent[b * dim + i] + sr * nr[i];
+                    mag += t * t;
+                    tangent[b * dim + i] = t;
+                }
+                mag = sqrt(mag);
+                for (int i = 0; i < dim; ++i) tangent[b * dim + i] /= mag;
+            }
+        }
+        /* list rebuild cadence, SRK.py:428-484 */
+        if (vp && (double)steps >= until) {
+            verlet_build(&vv, dim, n, pos, chain_id, L, s->n_nearby);
+            until += n_upd;
+        }
+    }
+    status_out[0] = dt;
+    status_out[1] = elapsed;
+    status_out[2] = (double)steps;
+    status_out[3] = m;
+    free(xi_buf);
+    if (vp) {
+        free(vv.start);
+        free(vv.list);
+        free(vv.chunk);
+        free(vv.cell_head);
+        free(vv.cell_next);
+    }
+    return steps;
+}

@@ -1,0 +1,105 @@
+/*
+ * gcr_oracle.h -- CPU FP64 restatement of the reference Brownian-dynamics hot
+ * path (WCA/LJ pair forces, Hookean bond + bending forces, chiral
+ * self-propulsion, SRK2 step with adaptive dt, rotational diffusion).
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: tests/, the
+ * smoke() entry point and bench.py's CPU-baseline leg may load it. The
+ * product path (libgcr.so) never links or calls it.
+ *
+ * Each function cites the reference file:line it restates. Parity status of
+ * each function (what pins it) is listed in oracle/README.md.
+ */
+#ifndef GCR_ORACLE_H
+#define GCR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Physics of one population of chains (or ABPs when every chain has one
+ * bead), as SRK_jit sees it (SRK.py:46-97, updates.py:144-162). */
+typedef struct orc_sys {
+    int32_t dim;              /* 2 or 3 */
+    int32_t n;                /* beads */
+    int32_t n_nearby;         /* n_nearby_beads_dont_repel (pop_consts[4]) */
+    int32_t has_springs;      /* "springs" in the force list, longest chain > 1 */
+    int32_t has_wca;          /* "WCA" in the force list */
+    int32_t has_propel;       /* "self-propulsion" and b_calc_active_forces */
+    double  L;                /* Lx */
+    double  D, Dr, radius;    /* pop_consts[0..2] */
+    double  wca[4];           /* make_WCA_params output (populations.py:48-58) */
+    double  spring[6];        /* spring_params (populations.py:345-353) */
+    double  v0;               /* active_params[0] (populations.py:355-358) */
+} orc_sys;
+
+/* Python float modulo (CPython float_rem semantics, used by `tmp %= Lx`
+ * at SRK.py:387): result has the sign of b, can equal b for tiny negative a. */
+double orc_pymod(double a, double b);
+
+/* populations.py:48-58 */
+void orc_wca_params(double repel_d_max, double eps, double shift, double cutoff_factor,
+                    double out[4]);
+
+/* forces.py:13-15: force coefficient; F_on_b1 = coeff * (x_b2 - x_b1). */
+double orc_wca_coeff(double dist_sq, const double p[4]);
+
+/* forces.py:44-54: out = a - b, each component corrected once into [-L/2, L/2). */
+double orc_inbounds_diff(int dim, const double* a, const double* b, double half_L, double L,
+                         double* out);
+
+/* forces.py:285-402 (Hookean branch; FENE is out of scope and rejected by
+ * the caller). Accumulates into forces[n*dim]; writes tangent[n*dim] when
+ * spring[4] != 0. chain_id is int64 as in populations.py:606. */
+void orc_chain_forces(int dim, int n, const double* pos, const int64_t* chain_id, double L,
+                      const double spring[6], const double* rest_lengths,
+                      double* tangent, double* forces);
+
+/* forces.py:57-144 with the neighbour set of updates.py:586-669: every pair
+ * (b1, b2) with 0 < |d|^2 <= rc^2, skipping same-chain pairs with
+ * |b1-b2| <= n_nearby (all same-chain pairs when n_nearby < 0). Brute force
+ * O(n^2), full list (each pair evaluated from both sides, the cross-chunk
+ * branch forces.py:141-143). Accumulates into forces. */
+void orc_pair_forces_brute(int dim, int n, const double* pos, const int64_t* chain_id, double L,
+                           const double wca[4], int n_nearby, double* forces);
+
+/* forces.py:206-223 with a per-chain chirality angle (reduces to the
+ * reference's scalar v_tan_angle when every entry is equal). */
+void orc_self_propel(int dim, int n, const int64_t* chain_id, double v0,
+                     const double* v_tan_angle_per_chain, const double* tangent, double* forces);
+
+/* One force evaluation in the reference order (SRK.py:228-253):
+ * springs (+tangents) -> pair -> self-propulsion, accumulated into forces. */
+void orc_force_eval(const orc_sys* s, const double* pos, const int64_t* chain_id,
+                    const double* rest_lengths, const double* v_tan_angle_per_chain,
+                    double* tangent, double* forces, int pair_mode, void* verlet);
+
+/* Noise: Philox4x32-10 (Salmon et al., SC'11) and the build's bead-keyed
+ * normal generator shared with the GPU path. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* xi[2*dim] for bead `orig` at global step `step`: translational normals in
+ * xi[0:dim], rotational in xi[dim:2dim]. */
+void orc_bead_noise(int dim, uint64_t seed, uint32_t orig, int64_t step, double* xi);
+
+/* The SRK2 loop of SRK_jit (SRK.py:128-423) for one population with a single
+ * thread, SRK_order = 2. time_vals as built at updates.py:144-146:
+ * [dt, dtmax, max_dist_per_step, RK_cheat_factor, max_dt_usable,
+ *  steps_until_update, n_steps_update, max_steps].
+ * status_out = [dt, elapsed (dt_used sum), steps, max_forces_strength].
+ * xi: NULL -> generated by orc_bead_noise(seed, b, step0 + k); otherwise
+ * xi[k*n*2*dim + b*2*dim + c] for local step k (noise injection).
+ * pair_mode: 0 = brute force, 1 = Verlet list (cell grid, rebuilt every
+ * n_steps_update steps, half list with Newton-3 inside each OpenMP chunk, as
+ * the reference's threads do).
+ * Returns steps done. */
+int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangent,
+                     double* forces, const int64_t* chain_id, const double* rest_lengths,
+                     const double* v_tan_angle_per_chain, const double time_vals[8],
+                     double status_out[4], const double* xi, uint64_t seed, int64_t step0,
+                     int pair_mode, int n_threads, double grid_factor);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,177 @@
+"""ctypes binding of the C oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg use it as the checker. The product path never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+
+class OrcSys(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("n", C.c_int32),
+        ("n_nearby", C.c_int32),
+        ("has_springs", C.c_int32),
+        ("has_wca", C.c_int32),
+        ("has_propel", C.c_int32),
+        ("L", C.c_double),
+        ("D", C.c_double),
+        ("Dr", C.c_double),
+        ("radius", C.c_double),
+        ("wca", C.c_double * 4),
+        ("spring", C.c_double * 6),
+        ("v0", C.c_double),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        L.orc_pymod.restype = C.c_double
+        L.orc_pymod.argtypes = [C.c_double, C.c_double]
+        L.orc_wca_coeff.restype = C.c_double
+        L.orc_wca_coeff.argtypes = [C.c_double, dp]
+        L.orc_wca_params.argtypes = [C.c_double] * 4 + [dp]
+        L.orc_chain_forces.argtypes = [C.c_int, C.c_int, dp, ip, C.c_double, dp, dp, dp, dp]
+        L.orc_pair_forces_brute.argtypes = [C.c_int, C.c_int, dp, ip, C.c_double, dp,
+                                            C.c_int, dp]
+        L.orc_self_propel.argtypes = [C.c_int, C.c_int, ip, C.c_double, dp, dp, dp]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.orc_bead_noise.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_int64, dp]
+        L.orc_srk2_run.restype = C.c_int64
+        L.orc_srk2_run.argtypes = [C.POINTER(OrcSys), dp, dp, dp, dp, ip, dp, dp, dp, dp,
+                                   C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_int,
+                                   C.c_double]
+        L.orc_force_eval.argtypes = [C.POINTER(OrcSys), dp, ip, dp, dp, dp, dp, C.c_int,
+                                     C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def make_sys(sysd):
+    s = OrcSys()
+    s.dim = sysd["dim"]
+    s.n = sysd["n"]
+    s.n_nearby = sysd["n_nearby"]
+    s.has_springs = int(sysd["has_springs"])
+    s.has_wca = int(sysd["has_wca"])
+    s.has_propel = int(sysd["has_propel"])
+    s.L = sysd["L"]
+    s.D = sysd["D"]
+    s.Dr = sysd["Dr"]
+    s.radius = sysd["radius"]
+    for i in range(4):
+        s.wca[i] = sysd["wca"][i]
+    for i in range(6):
+        s.spring[i] = sysd["spring"][i]
+    s.v0 = sysd["v0"]
+    return s
+
+
+def pymod(a, b):
+    return lib().orc_pymod(a, b)
+
+
+def wca_params(r, eps, shift, cf):
+    out = np.zeros(4)
+    lib().orc_wca_params(r, eps, shift, cf, out)
+    return out
+
+
+def wca_coeff(ds, p):
+    return lib().orc_wca_coeff(ds, np.ascontiguousarray(p, dtype=np.float64))
+
+
+def chain_forces(pos, chain_id, L, sp, rest_lengths, tangent=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, dim = pos.shape
+    F = np.zeros_like(pos)
+    T = np.full_like(pos, np.nan) if tangent is None else np.array(tangent, dtype=np.float64)
+    lib().orc_chain_forces(dim, n, pos, np.ascontiguousarray(chain_id, dtype=np.int64), L,
+                           np.ascontiguousarray(sp, dtype=np.float64),
+                           np.ascontiguousarray(rest_lengths, dtype=np.float64), T, F)
+    return F, T
+
+
+def pair_forces(pos, chain_id, L, p, n_nearby):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, dim = pos.shape
+    F = np.zeros_like(pos)
+    lib().orc_pair_forces_brute(dim, n, pos, np.ascontiguousarray(chain_id, dtype=np.int64), L,
+                                np.ascontiguousarray(p, dtype=np.float64), n_nearby, F)
+    return F
+
+
+def self_propel(tangent, chain_id, v0, vtan):
+    t = np.ascontiguousarray(tangent, dtype=np.float64)
+    n, dim = t.shape
+    F = np.zeros_like(t)
+    lib().orc_self_propel(dim, n, np.ascontiguousarray(chain_id, dtype=np.int64), v0,
+                          np.ascontiguousarray(vtan, dtype=np.float64), t, F)
+    return F
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def bead_noise(dim, seed, orig, step):
+    xi = np.zeros(2 * dim)
+    lib().orc_bead_noise(dim, seed, orig, step, xi)
+    return xi
+
+
+def force_eval(sysd, pos, chain_id, rest_lengths, vtan, tangent):
+    s = make_sys(sysd)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    T = np.array(tangent, dtype=np.float64)
+    F = np.zeros_like(pos)
+    lib().orc_force_eval(C.byref(s), pos, np.ascontiguousarray(chain_id, dtype=np.int64),
+                         np.ascontiguousarray(rest_lengths, dtype=np.float64),
+                         np.ascontiguousarray(vtan, dtype=np.float64), T, F, 0, None)
+    return F, T
+
+
+def srk2_run(sysd, pos, tangent, chain_id, rest_lengths, vtan, time_vals, xi=None, seed=0,
+             step0=0, pair_mode=0, n_threads=1, grid_factor=2.0):
+    """Returns (pos, pos1, tangent, forces, status[dt, elapsed, steps, m])."""
+    s = make_sys(sysd)
+    pos = np.array(pos, dtype=np.float64)
+    pos1 = pos.copy()
+    T = np.array(tangent, dtype=np.float64)
+    F = np.zeros_like(pos)
+    st = np.zeros(4)
+    xi_arr = None
+    if xi is not None:
+        xi_arr = np.ascontiguousarray(xi, dtype=np.float64)
+    lib().orc_srk2_run(C.byref(s), pos, pos1, T, F,
+                       np.ascontiguousarray(chain_id, dtype=np.int64),
+                       np.ascontiguousarray(rest_lengths, dtype=np.float64),
+                       np.ascontiguousarray(vtan, dtype=np.float64),
+                       np.ascontiguousarray(time_vals, dtype=np.float64), st,
+                       xi_arr.ctypes.data if xi_arr is not None else None, seed, step0,
+                       pair_mode, n_threads, grid_factor)
+    return pos, pos1, T, F, st

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU FP64 oracle.
+
+Tolerances (FP64 path): forces within 1e-11 relative to the largest force;
+positions after 100 SRK2 steps with injected noise within 1e-10 * L
+(min-image difference), step status within 1e-10 relative. The mixed path
+(FP32 pair forces, FP64 accumulation) is held to 1e-5 relative force error.
+"""
+import numpy as np
+import pytest
+
+import gcr_oracle_np as N
+import oracle_lib as O
+from gcr_amd import engine as E
+from gcr_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL = 1e-10  # times L
+FORCE_TOL = 1e-11  # times max |F|
+
+
+def sysd_of(w):
+    p = w.phys
+    return dict(dim=p.dim, n=p.n_beads, n_nearby=p.n_nearby, has_springs=p.has_springs,
+                has_wca=p.has_wca, has_propel=p.has_propel, L=p.L, D=p.D, Dr=p.Dr,
+                radius=p.radius, wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0)
+
+
+def cases():
+    out = {}
+    out["config1"] = W.config1(seed=1, steps=100)
+    out["config2_cut"] = W.chains("config2_cut", 2, 100, 20, phi=0.3, kappa=10.0, vtan=0.2,
+                                  T=0.01, dtmax=1e-3, steps=100, seed=2)
+    rng = np.random.default_rng(3)
+    out["config3_cut"] = W.chains("config3_cut", 2, 100, 20, phi=0.4,
+                                  cutoff_factor=W.LJ_25_SIGMA, kappa=5.0,
+                                  vtan=rng.uniform(-0.5, 0.5, 100), T=0.05, dtmax=5e-4,
+                                  steps=100, seed=3)
+    out["abp"] = W.chains("abp", 2, 600, 1, phi=0.3, T=0.01, Dr=0.1, steps=100, seed=4)
+    out["chains3d"] = W.chains("chains3d", 3, 60, 8, L=12.0, kappa=4.0, T=0.01, steps=100,
+                               seed=5)
+    out["abp3d"] = W.chains("abp3d", 3, 300, 1, L=10.0, T=0.01, Dr=0.1, steps=100, seed=6)
+    w = W.chains("shift_excl", 2, 40, 10, L=22.0, kappa=3.0, theta0=0.2, shift=0.1,
+                 n_nearby=-1, T=0.01, steps=100, seed=7)
+    out["shift_excl"] = w
+    out["diffusing_chains"] = W.chains("diffusing", 2, 40, 6, L=20.0, T=0.01, Dr=0.05,
+                                       tangent_locked=False, vtan=0.3, steps=100, seed=8)
+    return out
+
+
+CASES = cases()
+
+
+def oracle_run(w, xi, tv, seed=None):
+    return O.srk2_run(sysd_of(w), w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                      tv, xi=xi, seed=w.phys.seed if seed is None else seed)
+
+
+def gpu_system(w, precision=E.GCR_FP64):
+    return E.System(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                    precision=precision)
+
+
+def pos_err(a, b, L):
+    return np.max(np.abs(N.min_image(a - b, L)))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_force_eval_parity(name):
+    w = CASES[name]
+    print(f"[force parity] {name} n={w.n} dim={w.phys.dim}", flush=True)
+    F_o, T_o = O.force_eval(sysd_of(w), w.pos, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                            w.tangent)
+    s = gpu_system(w)
+    F_g, T_g = s.eval_forces()
+    s.close()
+    scale = max(1.0, np.max(np.abs(F_o)))
+    assert np.max(np.abs(F_g - F_o)) <= FORCE_TOL * scale
+    assert np.max(np.abs(T_g - T_o)) <= 1e-13
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_srk2_100_steps_injected_noise(name):
+    w = CASES[name]
+    print(f"[srk2 parity] {name} n={w.n}", flush=True)
+    rng = np.random.default_rng(100)
+    xi = rng.normal(size=(100, w.n, 2 * w.phys.dim))
+    tv = w.time_vals.copy()
+    tv[7] = 100
+    pos_o, pos1_o, T_o, F_o, st_o = oracle_run(w, xi, tv)
+    s = gpu_system(w)
+    st_g = s.run(tv, xi=xi)
+    out = s.download()
+    s.close()
+    L = w.phys.L
+    assert pos_err(out["pos"], pos_o, L) <= POS_TOL * L
+    assert pos_err(out["pos1"], pos1_o, L) <= POS_TOL * L
+    assert np.max(np.abs(out["tangent"] - T_o)) <= 1e-9
+    assert np.max(np.abs(out["forces"] - F_o)) <= 1e-8 * max(1.0, np.max(np.abs(F_o)))
+    assert st_g[2] == st_o[2] == 100
+    assert st_g[0] == pytest.approx(st_o[0], rel=1e-10)
+    assert st_g[1] == pytest.approx(st_o[1], rel=1e-10)
+    assert st_g[3] == pytest.approx(st_o[3], rel=1e-10)
+    assert np.all((out["pos"] >= 0) & (out["pos"] <= L))
+
+
+def test_in_kernel_philox_matches_oracle_generator():
+    """xi = None on both sides: the GPU's in-kernel Philox/Box-Muller noise and
+    the oracle's orc_bead_noise give the same trajectory."""
+    w = CASES["config2_cut"]
+    tv = w.time_vals.copy()
+    tv[7] = 100
+    pos_o, _, _, _, st_o = oracle_run(w, None, tv, seed=w.phys.seed)
+    s = gpu_system(w)
+    s.run(tv)
+    out = s.download()
+    s.close()
+    assert pos_err(out["pos"], pos_o, w.phys.L) <= POS_TOL * w.phys.L
+
+
+def test_two_calls_equal_one_call():
+    """Chunked like the reference driver: the second call starts from the
+    first call's dt (status.dt, updates.py:249-256) and the next noise steps."""
+    w = CASES["config1"]
+    rng = np.random.default_rng(9)
+    xi = rng.normal(size=(100, w.n, 4))
+    tv = w.time_vals.copy()
+    tv[7] = 100
+    pos_o, *_ = oracle_run(w, xi, tv)
+    s = gpu_system(w)
+    tv1 = tv.copy()
+    tv1[7] = 37
+    st = s.run(tv1, xi=xi[:37])
+    tv2 = tv.copy()
+    tv2[0] = st[0]
+    tv2[7] = 63
+    s.run(tv2, xi=xi[37:])
+    out = s.download()
+    s.close()
+    assert pos_err(out["pos"], pos_o, w.phys.L) <= POS_TOL * w.phys.L
+
+
+def test_max_dt_usable_clips_time():
+    w = CASES["config1"]
+    rng = np.random.default_rng(10)
+    xi = rng.normal(size=(50, w.n, 4))
+    tv = w.time_vals.copy()
+    tv[0] = 1e-3
+    tv[7] = 50
+    tv[4] = 0.0123  # stops after 13 steps with elapsed == 0.0123
+    pos_o, _, _, _, st_o = oracle_run(w, xi, tv)
+    s = gpu_system(w)
+    st_g = s.run(tv, xi=xi)
+    out = s.download()
+    s.close()
+    assert st_o[2] == st_g[2] == 13
+    assert st_g[1] == pytest.approx(0.0123, rel=1e-14)
+    assert pos_err(out["pos"], pos_o, w.phys.L) <= POS_TOL * w.phys.L
+
+
+def test_deterministic_bits():
+    w = CASES["config3_cut"]
+    tv = w.time_vals.copy()
+    tv[7] = 60
+    res = []
+    for _ in range(2):
+        s = gpu_system(w)
+        s.run(tv)
+        res.append(s.download())
+        s.close()
+    for k in res[0]:
+        assert np.array_equal(res[0][k], res[1][k]), k
+
+
+def test_rebuild_period_and_replay():
+    """Results do not depend on the cell-list schedule beyond rounding; a skin
+    too thin for the schedule is caught and the batch replayed."""
+    w = CASES["config2_cut"]
+    rng = np.random.default_rng(11)
+    xi = rng.normal(size=(64, w.n, 4))
+    tv = w.time_vals.copy()
+    tv[7] = 64
+    pos_o, *_ = oracle_run(w, xi, tv)
+    outs = []
+    for K, skin in ((1, 0.3), (16, 0.3), (32, 0.02)):
+        phys = W.chains("c", 2, 100, 20, phi=0.3, kappa=10.0, vtan=0.2, T=0.01, dtmax=1e-3,
+                        steps=64, seed=2, skin=skin).phys
+        p = E.Pool(2, E.GCR_FP64, bead_capacity=4096, cell_capacity=1 << 16,
+                   chain_capacity=256, replica_capacity=2)
+        p.set_schedule(32, K)
+        rid = p.add(phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+        p.begin(rid, tv, xi=xi)
+        p.run()
+        st = p.status(rid)
+        outs.append((K, skin, st["replays"], p.download(rid)["pos"]))
+        p.close()
+    for K, skin, replays, pos in outs:
+        assert pos_err(pos, pos_o, w.phys.L) <= POS_TOL * w.phys.L, (K, skin)
+    assert outs[2][2] > 0  # the thin skin forced a replay
+    assert outs[0][2] == 0 and outs[1][2] == 0
+
+
+def test_alone_vs_mixed_pool_bit_identical():
+    """A replica gives the same bits alone and in a pool with others."""
+    ws = [W.chains(f"m{i}", 2, 30 + 20 * i, 5 + 3 * i, phi=0.25, vtan=0.1 * i, T=0.01,
+                   steps=80, seed=20 + i) for i in range(4)]
+
+    def run(which):
+        p = E.Pool(2, E.GCR_MIXED, bead_capacity=1 << 14, cell_capacity=1 << 16,
+                   chain_capacity=1024, replica_capacity=8)
+        p.set_schedule(16, 8)
+        rids = [p.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+                for w in which]
+        for w, r in zip(which, rids):
+            tv = w.time_vals.copy()
+            tv[7] = 80
+            p.begin(r, tv)
+        p.run()
+        res = [p.download(r) for r in rids]
+        p.close()
+        return res
+
+    mix = run(ws)
+    for i, w in enumerate(ws):
+        alone = run([w])[0]
+        for k in alone:
+            assert np.array_equal(alone[k], mix[i][k]), (i, k)
+
+
+def test_mixed_precision_force_error():
+    w = CASES["config3_cut"]
+    F_o, _ = O.force_eval(sysd_of(w), w.pos, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                          w.tangent)
+    s = gpu_system(w, precision=E.GCR_MIXED)
+    F_m, _ = s.eval_forces()
+    s.close()
+    scale = np.max(np.abs(F_o))
+    err = np.max(np.abs(F_m - F_o)) / scale
+    print(f"mixed-precision max force error / max force = {err:.3e}")
+    assert err < 1e-5

@@ -1,0 +1,432 @@
+"""Pins of the oracle (CPU, no GPU): each oracle function is checked against
+what the mathematics fixes -- closed forms, energy gradients by central finite
+differences, exact recurrences, invariants, published known-answer vectors --
+and the C restatement is checked against the independent NumPy restatement.
+The reference ships no golden vectors or tests for this path."""
+import math
+
+import numpy as np
+import pytest
+
+import gcr_oracle_np as N
+import oracle_lib as O
+
+L2D = 20.0
+
+
+def lj_energy(r, p):
+    """U(r) whose -dU/dr gives the pair law of forces.py:13-15:
+    U = eps ((rho/r)^12 - 2 (rho/r)^6) with p0 = 12 eps rho^6, p1 = 12 eps rho^12."""
+    return p[1] / 12.0 * r ** -12 - p[0] / 6.0 * r ** -6
+
+
+def straight_chains(rng, n_chains, length, L, a=0.5, jitter=0.05, dim=2, min_dist=0.9):
+    """Chains along random directions, rejected while a bead comes closer than
+    min_dist to a bead of an earlier chain."""
+    pos = np.zeros((n_chains * length, dim))
+    tang = np.zeros_like(pos)
+    for c in range(n_chains):
+        for _ in range(10000):
+            x0 = rng.uniform(0, L, dim)
+            t = rng.normal(size=dim)
+            t /= np.linalg.norm(t)
+            cand = np.mod(x0 - a * np.arange(length)[:, None] * t
+                          + rng.normal(0, jitter, (length, dim)), L)
+            if c == 0:
+                break
+            d = N.min_image(cand[:, None, :] - pos[None, :c * length, :], L)
+            if np.sqrt((d ** 2).sum(-1)).min() >= min_dist:
+                break
+        pos[c * length:(c + 1) * length] = cand
+        tang[c * length:(c + 1) * length] = t
+    return pos, tang, np.repeat(np.arange(n_chains), length).astype(np.int64)
+
+
+# ------------------------------------------------------------------ pair law
+
+def test_wca_params_closed_form():
+    p = O.wca_params(1.0, 1.0, 0.0, 1.0)
+    assert np.array_equal(p, [12.0, 12.0, 0.0, 1.0])
+    p = O.wca_params(1.3, 0.7, 0.1, 2.5)
+    assert np.allclose(p, [12 * 1.3 ** 6 * 0.7, 12 * 1.3 ** 12 * 0.7, 0.1, 2.5 * 1.3 - 0.1])
+    assert np.allclose(p, N.wca_params(1.3, 0.7, 0.1, 2.5), rtol=1e-15)
+
+
+@pytest.mark.parametrize("r0,eps", [(1.0, 1.0), (1.3, 0.7)])
+def test_wca_coeff_is_minus_gradient_of_lj(r0, eps):
+    p = O.wca_params(r0, eps, 0.0, 2.5)
+    # zero force at the potential minimum r = repel_d_max
+    assert abs(O.wca_coeff(r0 * r0, p)) < 1e-12 * p[0]
+    for r in [0.8 * r0, 0.95 * r0, 1.2 * r0, 2.0 * r0]:
+        h = 1e-6 * r
+        dUdr = (lj_energy(r + h, p) - lj_energy(r - h, p)) / (2 * h)
+        # force on b1 = coeff * (x2 - x1); radial component along (x1 - x2) is -coeff * r
+        f_radial = -O.wca_coeff(r * r, p) * r
+        assert f_radial == pytest.approx(-dUdr, rel=1e-7)
+    # repulsive inside, attractive outside the minimum
+    assert O.wca_coeff(0.9 * r0 * 0.9 * r0, p) < 0 < O.wca_coeff(1.5 * r0 * 1.5 * r0, p)
+
+
+def test_pymod_python_semantics():
+    for a in [-1e-17, -3.5, 0.0, -0.0, 7.25, 20.0, 41.0, -20.0]:
+        assert O.pymod(a, L2D) == a % L2D
+        assert math.copysign(1, O.pymod(a, L2D)) == math.copysign(1, a % L2D)
+    assert O.pymod(-1e-17, 20.0) == 20.0  # the edge the kernels must clamp
+
+
+def pair_energy(pos, chain_id, L, p, n_nearby):
+    n = len(pos)
+    E = 0.0
+    for i in range(n):
+        for j in range(i + 1, n):
+            if chain_id[i] == chain_id[j] and (n_nearby < 0 or abs(i - j) <= n_nearby):
+                continue
+            d = N.min_image(pos[j] - pos[i], L)
+            r = np.linalg.norm(d)
+            if 0 < r <= p[3]:
+                E += lj_energy(r, p)
+    return E
+
+
+@pytest.mark.parametrize("dim,n_nearby", [(2, 4), (2, -1), (3, 2)])
+def test_pair_forces_are_energy_gradient(dim, n_nearby):
+    rng = np.random.default_rng(3)
+    L = 6.0
+    p = O.wca_params(1.0, 1.0, 0.0, 2.5 / 2 ** (1 / 6))
+    chain = np.repeat(np.arange(7), 2).astype(np.int64)
+    pts = []
+    while len(pts) < 14:  # sequential placement: no close pair, none near the cutoff
+        x = rng.uniform(0, L, dim)
+        ok = True
+        for y in pts:
+            r = np.linalg.norm(N.min_image(x - y, L))
+            if r < 0.8 or abs(r - p[3]) < 1e-2:
+                ok = False
+        if ok:
+            pts.append(x)
+    pos = np.array(pts)
+    F = O.pair_forces(pos, chain, L, p, n_nearby)
+    h = 1e-6
+    for b in range(len(pos)):
+        for i in range(dim):
+            pp, pm = pos.copy(), pos.copy()
+            pp[b, i] += h
+            pm[b, i] -= h
+            g = (pair_energy(pp, chain, L, p, n_nearby) - pair_energy(pm, chain, L, p, n_nearby)) / (2 * h)
+            assert F[b, i] == pytest.approx(-g, abs=1e-6 * max(1, abs(g)))
+    assert np.allclose(F.sum(0), 0, atol=1e-10)  # Newton's third law
+    assert np.allclose(F, N.pair_forces(pos, chain, L, p, n_nearby), rtol=1e-12, atol=1e-12)
+
+
+def test_pair_forces_exclusion_and_shift():
+    # two beads on one chain at distance 0.8: excluded when |i-j| <= n_nearby
+    pos = np.array([[5.0, 5.0], [5.8, 5.0]])
+    same = np.array([0, 0], dtype=np.int64)
+    other = np.array([0, 1], dtype=np.int64)
+    p = O.wca_params(1.0, 1.0, 0.0, 1.0)
+    assert np.all(O.pair_forces(pos, same, 10.0, p, 1) == 0)
+    assert np.all(O.pair_forces(pos, same, 10.0, p, -1) == 0)
+    assert np.any(O.pair_forces(pos, same, 10.0, p, 0) != 0)
+    F = O.pair_forces(pos, other, 10.0, p, 4)
+    assert F[0, 0] < 0 and F[1, 0] > 0  # repulsion along x
+    # WCA_shift s: force at distance d equals the unshifted force at d + s
+    ps = O.wca_params(1.0, 1.0, 0.1, 1.0)
+    Fs = O.pair_forces(pos, other, 10.0, ps, 4)
+    f_expect = -O.wca_coeff(0.9 ** 2, p) * 0.9  # radial magnitude at 0.8 + 0.1
+    assert Fs[1, 0] == pytest.approx(f_expect, rel=1e-13)
+    # periodic image: beads across the boundary interact with the minimum image
+    pos = np.array([[0.1, 5.0], [9.6, 5.0]])
+    F = O.pair_forces(pos, other, 10.0, p, 4)
+    assert F[0, 0] > 0 and F[1, 0] < 0
+
+
+def test_pair_cutoff_inclusive():
+    p = O.wca_params(1.0, 1.0, 0.0, 2.0)
+    pos = np.array([[1.0, 1.0], [3.0, 1.0]])  # exactly at rc = 2
+    F = O.pair_forces(pos, np.array([0, 1], dtype=np.int64), 10.0, p, 4)
+    assert F[0, 0] != 0  # 0 < ds <= rc^2 includes rc (forces.py:122)
+
+
+# --------------------------------------------------------------- chain forces
+
+def bond_energy(pos, chain, L, k, a):
+    E = 0.0
+    for b in range(len(pos) - 1):
+        if chain[b] == chain[b + 1]:
+            t = N.min_image(pos[b] - pos[b + 1], L)
+            E += 0.5 * k * (np.linalg.norm(t) - a) ** 2
+    return E
+
+
+def bend_energy_terms(pos, chain, L, kappa, theta0):
+    out = []
+    for m in range(1, len(pos) - 1):
+        if chain[m - 1] == chain[m] == chain[m + 1]:
+            ti = N.min_image(pos[m] - pos[m + 1], L)
+            tj = N.min_image(pos[m - 1] - pos[m], L)
+            ti /= np.linalg.norm(ti)
+            tj /= np.linalg.norm(tj)
+            th = np.arccos(np.clip(ti @ tj, -1, 1))
+            s = np.sign(-ti[0] * tj[1] + ti[1] * tj[0]) if theta0 else 0.0
+            out.append((m, 0.5 * kappa * (th + theta0 * s) ** 2))
+    return out
+
+
+def test_hookean_bonds_are_energy_gradient():
+    rng = np.random.default_rng(5)
+    pos, tang, chain = straight_chains(rng, 3, 6, L2D, jitter=0.08)
+    sp = np.array([37.0, 0.0, 0.0, 0.0, 0.0, 0.1])
+    rl = np.array([0.5, 0.45, 0.6])
+    F, _ = O.chain_forces(pos, chain, L2D, sp, rl)
+    h = 1e-6
+    for b in range(len(pos)):
+        for i in range(2):
+            pp, pm = pos.copy(), pos.copy()
+            pp[b, i] += h
+            pm[b, i] -= h
+            # rest length is per chain (rest_lengths[chain_id])
+            ep = sum(bond_energy(pp[chain == c], chain[chain == c], L2D, 37.0, rl[c]) for c in range(3))
+            em = sum(bond_energy(pm[chain == c], chain[chain == c], L2D, 37.0, rl[c]) for c in range(3))
+            assert F[b, i] == pytest.approx(-(ep - em) / (2 * h), abs=1e-6)
+    assert np.allclose(F.sum(0), 0, atol=1e-12)
+
+
+@pytest.mark.parametrize("ell,theta0", [(1.0, 0.0), (0.5, 0.0), (1.3, 0.0), (0.7, 0.3)])
+def test_bending_is_bond_length_times_energy_gradient(ell, theta0):
+    """forces.py:361-386 applies kappa*theta/sin(theta) times unit-vector
+    combinations with no 1/|bond| factor: for chains whose bonds all have
+    length ell the bending force is ell * (-grad of 1/2 kappa (theta+theta0 s)^2)."""
+    rng = np.random.default_rng(7)
+    n = 7
+    ang = np.cumsum(rng.uniform(-0.6, 0.6, n - 1)) + 0.3
+    pos = np.zeros((n, 2))
+    pos[0] = [10.0, 10.0]
+    for b in range(1, n):
+        pos[b] = pos[b - 1] - ell * np.array([np.cos(ang[b - 1]), np.sin(ang[b - 1])])
+    chain = np.zeros(n, dtype=np.int64)
+    kappa = 3.0
+    sp = np.array([0.0, kappa, theta0, 0.0, 0.0, 0.1])
+    F, _ = O.chain_forces(pos, chain, L2D, sp, np.array([ell]))
+    # bond lengths stay ell only to first order under a displacement: the
+    # gradient of the bending energy alone is what the formula scales
+    h = 1e-6
+    G = np.zeros_like(pos)
+    for b in range(n):
+        for i in range(2):
+            pp, pm = pos.copy(), pos.copy()
+            pp[b, i] += h
+            pm[b, i] -= h
+            ep = sum(e for _, e in bend_energy_terms(pp, chain, L2D, kappa, theta0))
+            em = sum(e for _, e in bend_energy_terms(pm, chain, L2D, kappa, theta0))
+            G[b, i] = (ep - em) / (2 * h)
+    assert np.allclose(F, -ell * G, atol=1e-6 * kappa)
+    assert np.allclose(F.sum(0), 0, atol=1e-12)
+    # net torque vanishes for equal bond lengths
+    tq = np.sum(pos[:, 0] * F[:, 1] - pos[:, 1] * F[:, 0])
+    assert abs(tq) < 1e-9
+
+
+def test_straight_chain_no_bending_and_tangents():
+    n = 6
+    t = np.array([0.6, 0.8])
+    pos = np.array([[10.0, 10.0] - 0.5 * j * t for j in range(n)])
+    chain = np.zeros(n, dtype=np.int64)
+    sp = np.array([100.0, 10.0, 0.0, 0.0, 1.0, 0.1])
+    F, T = O.chain_forces(pos, chain, L2D, sp, np.array([0.5]))
+    assert np.allclose(F, 0, atol=1e-12)  # at rest length and straight
+    assert np.allclose(T, t, atol=1e-15)  # tangent points towards bead 0
+    # bent chain: interior tangent is the bisector (t_i + t_j)/|t_i + t_j|
+    pos[3:] = pos[2] - 0.5 * np.outer(np.arange(1, n - 2), [1.0, 0.0])
+    F, T = O.chain_forces(pos, chain, L2D, sp, np.array([0.5]))
+    ti, tj = np.array([1.0, 0.0]), t
+    bis = (ti + tj) / np.linalg.norm(ti + tj)
+    assert np.allclose(T[2], bis, atol=1e-14)
+    assert np.allclose(T[0], t) and np.allclose(T[-1], [1.0, 0.0])
+    Fn, Tn = N.chain_forces(pos, chain, L2D, sp, np.array([0.5]))
+    assert np.allclose(F, Fn, atol=1e-13) and np.allclose(T, Tn, atol=1e-15)
+
+
+def test_chain_forces_c_vs_numpy_random():
+    rng = np.random.default_rng(11)
+    pos, tang, chain = straight_chains(rng, 5, 9, L2D, jitter=0.1)
+    sp = np.array([100.0, 10.0, 0.2, 0.0, 1.0, 0.1])
+    rl = rng.uniform(0.4, 0.6, 5)
+    F, T = O.chain_forces(pos, chain, L2D, sp, rl)
+    Fn, Tn = N.chain_forces(pos, chain, L2D, sp, rl)
+    assert np.allclose(F, Fn, rtol=1e-12, atol=1e-11)
+    assert np.allclose(T, Tn, atol=1e-14)
+
+
+# ------------------------------------------------------------- self-propulsion
+
+def test_self_propulsion_rotation():
+    rng = np.random.default_rng(2)
+    t = rng.normal(size=(8, 2))
+    t /= np.linalg.norm(t, axis=1)[:, None]
+    chain = np.repeat(np.arange(4), 2).astype(np.int64)
+    ang = np.array([0.0, 0.2, -0.5, np.pi / 2])
+    F = O.self_propel(t, chain, 1.7, ang)
+    for b in range(8):
+        a = ang[chain[b]]
+        assert np.linalg.norm(F[b]) == pytest.approx(1.7, rel=1e-14)
+        # signed angle from t to F equals v_tan_angle
+        cross = t[b, 0] * F[b, 1] - t[b, 1] * F[b, 0]
+        assert math.atan2(cross, t[b] @ F[b]) == pytest.approx(a, abs=1e-14)
+    assert np.array_equal(F[:2], 1.7 * t[:2])  # angle 0: plain v0 t (forces.py:214-217)
+    F3 = O.self_propel(np.array([[0.0, 0.6, 0.8]]), np.array([0], dtype=np.int64), 2.0, [0.3])
+    assert np.allclose(F3, [[0.0, 1.2, 1.6]])  # 3D ignores the chiral angle
+
+
+# ---------------------------------------------------------------------- SRK2
+
+def abp_sys(n, L, D=0.0, Dr=0.0, v0=1.0, dim=2, wca=False):
+    return dict(dim=dim, n=n, n_nearby=4, has_springs=False, has_wca=wca, has_propel=True, L=L,
+                D=D, Dr=Dr, radius=1.0, wca=O.wca_params(1.0, 1.0, 0.0, 1.0),
+                spring=np.zeros(6), v0=v0)
+
+
+def test_srk2_free_abp_exact_drift():
+    """T = 0, one ABP: F0 = F1 = v0 t, m = v0/r, so every step moves v0 dt_used t
+    with dt = min(dtmax, mdps r / v0); the first predictor uses dt0."""
+    L, v0, dtmax, mdps = 50.0, 2.0, 1e-3, 1e-3
+    t = np.array([[0.6, -0.8]])
+    x0 = np.array([[10.0, 10.0]])
+    tv = np.array([1e-7, dtmax, mdps, 1.0, -1.0, 40, 40, 25])
+    pos, pos1, T, F, st = O.srk2_run(abp_sys(1, L, v0=v0), x0, t, np.array([0]), np.array([0.5]),
+                                     np.array([0.0]), tv, xi=np.zeros((25, 1, 4)))
+    dt = min(dtmax, mdps / v0)
+    assert st[0] == dt and st[2] == 25
+    assert st[1] == pytest.approx(25 * dt, rel=1e-14)
+    assert np.allclose(pos, x0 + v0 * 25 * dt * t, atol=1e-13)
+    assert np.allclose(pos1, x0 + v0 * 24 * dt * t + v0 * dt * t, atol=1e-13)
+    assert np.allclose(F, 2 * v0 * t)
+
+
+def test_srk2_spring_dimer_heun_recurrence():
+    """Two beads on one chain, T = 0, no WCA: the separation s obeys
+    ds/dt = 2k(a - s)/r. SRK2 with the reference's sequencing is the scalar
+    map  s1 = s + dt_prev 2k(a-s)/r,  m = |k(2a - s - s1)| / (2r),
+    dt = min(dtmax, mdps/m),  s' = s + dt k((a-s) + (a-s1)) / r."""
+    k, a, r = 50.0, 0.5, 1.3
+    sysd = dict(dim=2, n=2, n_nearby=4, has_springs=True, has_wca=False, has_propel=False,
+                L=20.0, D=0.0, Dr=0.0, radius=r, wca=np.zeros(4),
+                spring=np.array([k, 0.0, 0.0, 0.0, 1.0, 0.1]), v0=0.0)
+    s0, dtmax, mdps, dt0 = 0.9, 2e-3, 4e-3, 1e-6
+    pos = np.array([[10.0 + s0, 7.0], [10.0, 7.0]])
+    nsteps = 30
+    tv = np.array([dt0, dtmax, mdps, 1.0, -1.0, 40, 40, nsteps])
+    out = O.srk2_run(sysd, pos, np.zeros((2, 2)), np.array([0, 0]), np.array([a]),
+                     np.array([0.0]), tv, xi=np.zeros((nsteps, 2, 4)))
+    s, dt, dts = s0, dt0, []
+    for _ in range(nsteps):
+        s1 = s + dt * 2 * k * (a - s) / r
+        m = 0.5 * abs(k * ((a - s) + (a - s1))) / r  # 1/2 max(F0 + F1) / r
+        dt = min(dtmax, mdps / m)
+        dts.append(dt)
+        s = s + dt * k * ((a - s) + (a - s1)) / r
+    sep = out[0][0, 0] - out[0][1, 0]
+    assert sep == pytest.approx(s, rel=1e-12)
+    assert out[4][0] == pytest.approx(dts[-1], rel=1e-12)
+    assert out[4][1] == pytest.approx(sum(dts), rel=1e-12)
+    assert len(set(np.round(dts, 12))) > 2  # the adaptive dt actually moved
+    assert out[0][0, 0] + out[0][1, 0] == pytest.approx(20.0 + s0, rel=1e-14)  # centre fixed
+
+
+def test_srk2_pure_diffusion_and_time_clip():
+    """F = 0: x' = (x + sqrt(2 D dt_used) xi) mod L, dt = min(dtmax, mdps / sqrt(2D));
+    max_dt_usable stops the loop with elapsed == max_dt_usable exactly."""
+    D, L = 0.3, 10.0
+    sysd = abp_sys(3, L, D=D, v0=0.0)
+    rng = np.random.default_rng(0)
+    xi = rng.normal(size=(10, 3, 4))
+    x0 = rng.uniform(0, L, (3, 2))
+    dtmax, mdps = 1e-2, 5e-3
+    dt = min(dtmax, mdps / np.sqrt(2 * D))
+    mdu = 3.5 * dt
+    tv = np.array([1e-6, dtmax, mdps, 1.0, mdu, 40, 40, 10])
+    pos, pos1, T, F, st = O.srk2_run(sysd, x0, np.tile([1.0, 0.0], (3, 1)), np.arange(3),
+                                     np.ones(3), np.zeros(3), tv, xi=xi)
+    assert st[2] == 4 and st[1] == pytest.approx(mdu, rel=1e-15)
+    x = x0.copy()
+    used = [dt, dt, dt, mdu - 3 * dt]
+    for kk, du in enumerate(used):
+        x = np.mod(x + np.sqrt(2 * du * D) * xi[kk, :, :2], L)
+    assert np.allclose(pos, x, atol=1e-14)
+
+
+def test_rotational_diffusion_closed_form():
+    """SRK.py:398-423: t' = normalize(t + pi sqrt(2 Dr dt) (xi_r - (xi_r.t) t))."""
+    Dr = 0.2
+    sysd = abp_sys(2, 30.0, Dr=Dr, v0=0.0)
+    t = np.array([[1.0, 0.0], [0.0, 1.0]])
+    xi = np.zeros((1, 2, 4))
+    xi[0, :, 2:] = [[0.3, -1.1], [2.0, 0.5]]
+    dtmax, mdps = 1e-3, 1.0
+    tv = np.array([1e-3, dtmax, mdps, 1.0, -1.0, 40, 40, 1])
+    _, _, T, _, st = O.srk2_run(sysd, np.full((2, 2), 5.0), t, np.arange(2), np.ones(2),
+                                np.zeros(2), tv, xi=xi)
+    g = np.pi * np.sqrt(2 * Dr * st[0])
+    e0 = np.array([1.0, g * -1.1])
+    e1 = np.array([g * 2.0, 1.0])
+    assert np.allclose(T[0], e0 / np.linalg.norm(e0), atol=1e-15)
+    assert np.allclose(T[1], e1 / np.linalg.norm(e1), atol=1e-15)
+
+
+def test_srk2_c_vs_numpy_100_steps():
+    rng = np.random.default_rng(1)
+    pos, tang, chain = straight_chains(rng, 4, 7, L2D)
+    sysd = dict(dim=2, n=28, n_nearby=4, has_springs=True, has_wca=True, has_propel=True,
+                L=L2D, D=0.01, Dr=0.0075, radius=1.0, wca=O.wca_params(1.0, 1.0, 0.0, 1.0),
+                spring=np.array([100.0, 10.0, 0.0, 0.0, 1.0, 0.1]), v0=1.0)
+    xi = rng.normal(size=(100, 28, 4))
+    tv = np.array([1e-7, 1e-3, 0.0125, 1.0, -1.0, 40, 40, 100])
+    vt = np.full(4, 0.2)
+    a = O.srk2_run(sysd, pos, tang, chain, np.full(4, 0.5), vt, tv, xi=xi)
+    b = N.srk2_run(sysd, pos, tang, chain, np.full(4, 0.5), vt, tv, xi)
+    assert np.max(np.abs(N.min_image(a[0] - b[0], L2D))) < 1e-12
+    assert np.max(np.abs(N.min_image(a[1] - b[1], L2D))) < 1e-12
+    assert np.allclose(a[2], b[2], atol=1e-12)
+    assert np.allclose(a[3], b[3], rtol=1e-10, atol=1e-10)
+    assert np.allclose(a[4], b[4], rtol=1e-12)
+
+
+def test_verlet_mode_matches_brute_force():
+    """The CPU-baseline pair path (cell grid + Verlet list, Newton-3 inside
+    chain-aligned thread chunks) reproduces the all-pairs path."""
+    rng = np.random.default_rng(4)
+    pos, tang, chain = straight_chains(rng, 30, 10, 25.0, jitter=0.02)
+    sysd = dict(dim=2, n=300, n_nearby=4, has_springs=True, has_wca=True, has_propel=True,
+                L=25.0, D=0.01, Dr=0.0075, radius=1.0,
+                wca=O.wca_params(1.0, 1.0, 0.0, 2.5 / 2 ** (1 / 6)),
+                spring=np.array([100.0, 5.0, 0.0, 0.0, 1.0, 0.1]), v0=1.0)
+    xi = rng.normal(size=(60, 300, 4))
+    tv = np.array([1e-7, 5e-4, 0.0278, 1.0, -1.0, 40, 40, 60])
+    vt = rng.uniform(-0.5, 0.5, 30)
+    a = O.srk2_run(sysd, pos, tang, chain, np.full(30, 0.5), vt, tv, xi=xi, pair_mode=0)
+    for thr in (1, 3):
+        b = O.srk2_run(sysd, pos, tang, chain, np.full(30, 0.5), vt, tv, xi=xi, pair_mode=1,
+                       n_threads=thr)
+        assert np.max(np.abs(N.min_image(a[0] - b[0], 25.0))) < 1e-11
+        assert np.allclose(a[4], b[4], rtol=1e-10)
+
+
+# ---------------------------------------------------------------------- noise
+
+def test_philox_known_answers():
+    """Known-answer vectors published with Random123 (Salmon et al., SC'11)."""
+    assert O.philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert O.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                            0x6D5451FD]
+    assert O.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                    [0xA4093822, 0x299F31D0]) == [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_bead_noise_statistics():
+    g = np.array([O.bead_noise(2, 1234, b, s) for b in range(200) for s in range(50)])
+    assert abs(g.mean()) < 0.02
+    assert abs(g.var() - 1.0) < 0.03
+    c = np.corrcoef(g.T)
+    assert np.max(np.abs(c - np.eye(4))) < 0.03
+    # counter-based: the same key gives the same numbers, another bead does not
+    assert np.array_equal(O.bead_noise(3, 9, 5, 77), O.bead_noise(3, 9, 5, 77))
+    assert not np.array_equal(O.bead_noise(3, 9, 5, 77), O.bead_noise(3, 9, 6, 77))
