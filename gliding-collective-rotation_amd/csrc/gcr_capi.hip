@@ -110,6 +110,7 @@ struct gcr_pool {
     double *CHAIN_A = nullptr, *CHAIN_CS = nullptr;
     int32_t *CHAIN_FIRST = nullptr, *CHAIN_LEN = nullptr;
     int32_t *CCOUNT = nullptr, *CSTART = nullptr, *CHKSUM = nullptr;
+    double* BMAX = nullptr;
     int64_t capChk = 0, capBlk = 0;
 };
 
@@ -189,25 +190,25 @@ int rebuild_tables(gcr_pool* p)
 }
 
 // Kernel sequences (launched directly or captured into a graph).
-void launch_rebuild(gcr_pool* p)
+void launch_rebuild(gcr_pool* p, int j)
 {
     const PoolArgs& a = p->a;
     hipStream_t st = p->stream;
     if (p->dim == 2)
-        k_bin<2><<<p->nblk, BS, 0, st>>>(a);
+        k_bin<2><<<p->nblk, BS, 0, st>>>(a, j);
     else
-        k_bin<3><<<p->nblk, BS, 0, st>>>(a);
-    k_scan_chunks<<<p->nchk, BS, 0, st>>>(a);
-    k_scan_reps<<<p->capR, BS, 0, st>>>(a);
-    k_scan_final<<<p->nchk, BS, 0, st>>>(a);
-    k_scatter<<<p->nblk, BS, 0, st>>>(a);
-    k_cellsort<<<p->nchk, BS, 0, st>>>(a);
+        k_bin<3><<<p->nblk, BS, 0, st>>>(a, j);
+    k_scan_chunks<<<p->nchk, BS, 0, st>>>(a, j);
+    k_scan_reps<<<p->capR, BS, 0, st>>>(a, j);
+    k_scan_final<<<p->nchk, BS, 0, st>>>(a, j);
+    k_scatter<<<p->nblk, BS, 0, st>>>(a, j);
+    k_cellsort<<<p->nchk, BS, 0, st>>>(a, j);
     if (p->dim == 2) {
-        k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1);
-        k_links<2><<<p->nblk, BS, 0, st>>>(a, 1);
+        k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
+        k_links<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
     } else {
-        k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1);
-        k_links<3><<<p->nblk, BS, 0, st>>>(a, 1);
+        k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
+        k_links<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
     }
 }
 
@@ -220,19 +221,21 @@ void launch_stage(gcr_pool* p, int which, int j)
         if (which == 0)
             mixed ? k_stage<2, float, 0><<<p->nblk, BS, 0, st>>>(a, j)
                   : k_stage<2, double, 0><<<p->nblk, BS, 0, st>>>(a, j);
-        else if (which == 1)
+        else if (which == 1) {
             mixed ? k_stage<2, float, 1><<<p->nblk, BS, 0, st>>>(a, j)
                   : k_stage<2, double, 1><<<p->nblk, BS, 0, st>>>(a, j);
-        else
+            k_finalize<<<p->capR, BS, 0, st>>>(a, j);
+        } else
             k_update<2><<<p->nblk, BS, 0, st>>>(a, j);
     } else {
         if (which == 0)
             mixed ? k_stage<3, float, 0><<<p->nblk, BS, 0, st>>>(a, j)
                   : k_stage<3, double, 0><<<p->nblk, BS, 0, st>>>(a, j);
-        else if (which == 1)
+        else if (which == 1) {
             mixed ? k_stage<3, float, 1><<<p->nblk, BS, 0, st>>>(a, j)
                   : k_stage<3, double, 1><<<p->nblk, BS, 0, st>>>(a, j);
-        else
+            k_finalize<<<p->capR, BS, 0, st>>>(a, j);
+        } else
             k_update<3><<<p->nblk, BS, 0, st>>>(a, j);
     }
 }
@@ -249,7 +252,7 @@ void launch_batch(gcr_pool* p, int K)
 {
     launch_begin(p);
     for (int j = 0; j < p->B; ++j) {
-        if (j % K == 0) launch_rebuild(p);
+        if (j % K == 0) launch_rebuild(p, j);
         launch_stage(p, 0, j);
         launch_stage(p, 1, j);
         launch_stage(p, 2, j);
@@ -459,6 +462,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CCOUNT, capC + 1));
     A(dalloc(&p->CSTART, capC + 1));
     A(dalloc(&p->CHKSUM, p->capChk));
+    A(dalloc(&p->BMAX, p->capBlk));
     if (rc == GCR_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(GCR_ERR_HIP, "stream creation failed");
     if (rc == GCR_OK && hipMemset(p->CCOUNT, 0, sizeof(int32_t) * (capC + 1)) != hipSuccess)
@@ -499,6 +503,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.CCOUNT = p->CCOUNT;
     a.CSTART = p->CSTART;
     a.CHKSUM = p->CHKSUM;
+    a.BMAX = p->BMAX;
     a.KEYOFF = p->KEYOFF;
     a.PAIRS = p->PAIRS;
     a.Xn = p->Xn;
@@ -665,7 +670,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     // valid before its first step (other replicas are left untouched)
     if ((rc = rebuild_tables(p))) return rc;
     if ((rc = with_only(p, r, true))) return rc;
-    launch_rebuild(p);
+    launch_rebuild(p, -1);
     HIPCHK(hipGetLastError());
     if ((rc = with_only(p, r, false))) return rc;
     *rid = r;
@@ -746,8 +751,6 @@ int gcr_replica_begin(gcr_pool* p, int32_t rid, const double tv[8], const double
     S.batch_base = 0;
     S.max_steps = tv[7] < 0 ? -1 : (int64_t)tv[7];
     S.mdu = tv[4];
-    S.fmax2 = 0;
-    S.blocks_done = 0;
     S.violation = 0;
     S.frozen = 0;
     rc = upload_rep(p, rid);
@@ -827,7 +830,7 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
     if (rc) return rc;
     if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
     if ((rc = with_only(p, rid, true))) return rc;
-    launch_rebuild(p);
+    launch_rebuild(p, -1);
     const bool mixed = p->precision == GCR_MIXED;
     if (p->dim == 2)
         mixed ? k_eval<2, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
@@ -857,7 +860,7 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
     for (int j = 0; j < steps; ++j) {
         HIPCHK(hipEventRecord(ev[0], p->stream));
         if (j % p->K == 0) {
-            launch_rebuild(p);
+            launch_rebuild(p, j);
             ++nreb;
         }
         HIPCHK(hipEventRecord(ev[1], p->stream));
@@ -902,7 +905,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->dSs, p->X, p->X1, p->F0, p->FT, p->XI, p->T, p->Xn, p->Tn, p->Xs, p->Ts,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
-                    p->CHKSUM};
+                    p->CHKSUM, p->BMAX};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

@@ -29,7 +29,7 @@ struct RepParams {
     int32_t ncell;      // nc^dim
     int32_t n_nearby;   // n_nearby_beads_dont_repel
     uint32_t flags;
-    int32_t nblk;       // bead blocks of this replica (for last-block finalize)
+    int32_t nblk;       // bead blocks of this replica
     int32_t blk_off;    // first bead block
     int32_t pad0;
     double L, halfL;
@@ -45,8 +45,8 @@ struct RepParams {
     int64_t xi_steps;
 };
 
-// Mutable per-replica step state. Written by the last block of the stage-1
-// kernel (finalize) and by the batch control kernels only.
+// Mutable per-replica step state. Written by k_finalize (after the stage-1
+// launch) and by the batch control kernels only.
 struct RepState {
     double dt;        // most recent dt (the predictor of the next step uses it)
     double dt_used;   // dt_used of the most recent step (corrector step size)
@@ -57,11 +57,8 @@ struct RepState {
     int64_t batch_base;   // `steps` at the start of the running batch
     int64_t max_steps;    // < 0: unbounded
     double mdu;           // max_dt_usable, < 0: unbounded
-    uint64_t fmax2;       // bit pattern of max |F0+F1|^2 (non-negative double)
-    uint32_t blocks_done; // tickets for the last-block finalize
     int32_t violation;    // a bead left its cell box by more than skin/2
     int32_t frozen;       // 1: kernels leave this replica untouched
-    int32_t pad1;
 };
 
 struct PoolArgs {
@@ -78,6 +75,7 @@ struct PoolArgs {
     // replica tables
     const RepParams* P;
     RepState* S;
+    double* BMAX;  // [bead block] max |F0+F1|^2 of the block (stage 1)
     // per-slot state
     double* X;     // [slot][dim] positions
     double* X1;    // [slot][dim] predictor positions

@@ -35,11 +35,12 @@ double wrapd(double d, double L)
 }  // namespace
 
 extern "C" int gcr_init_chains(int32_t dim, double L, int64_t n_chains, const int32_t* chain_len,
-                               double bond, double min_dist, uint64_t seed, int64_t max_tries,
-                               double* pos, double* tangent)
+                               double bond, double min_dist, double min_dist_floor, uint64_t seed,
+                               int64_t max_tries, double* pos, double* tangent,
+                               double* min_dist_used)
 {
     if ((dim != 2 && dim != 3) || !(L > 0) || n_chains < 1 || !chain_len || !pos || !tangent ||
-        !(min_dist > 0) || !(min_dist < 0.5 * L))
+        !(min_dist > 0) || !(min_dist < 0.5 * L) || !(min_dist_floor <= min_dist))
         return GCR_ERR_ARG;
     int64_t n = 0;
     for (int64_t c = 0; c < n_chains; ++c) {
@@ -52,7 +53,10 @@ extern "C" int gcr_init_chains(int32_t dim, double L, int64_t n_chains, const in
     const int64_t ncell = dim == 3 ? (int64_t)nc * nc * nc : (int64_t)nc * nc;
     std::vector<int64_t> head(ncell, -1), next(n, -1);
     Rng rng{seed ^ 0x6A09E667F3BCC909ull};
-    const double md2 = min_dist * min_dist;
+    // the exclusion distance in force; lowered by 10% steps (not below
+    // min_dist_floor) whenever a chain finds no free place in max_tries
+    double cur = min_dist;
+    double md2 = cur * cur;
     auto cell_of = [&](const double* x) {
         int g[3] = {0, 0, 0};
         for (int i = 0; i < dim; ++i) {
@@ -125,8 +129,15 @@ extern "C" int gcr_init_chains(int32_t dim, double L, int64_t n_chains, const in
                 head[cc] = b;
             }
         }
-        if (!placed) return GCR_ERR_STATE;
+        if (!placed) {
+            if (cur * 0.9 < min_dist_floor) return GCR_ERR_STATE;
+            cur *= 0.9;
+            md2 = cur * cur;
+            --c;  // retry this chain at the smaller distance
+            continue;
+        }
         b0 += len;
     }
+    if (min_dist_used) *min_dist_used = cur;
     return GCR_OK;
 }

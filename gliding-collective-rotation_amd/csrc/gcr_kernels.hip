@@ -3,7 +3,8 @@
 // One step of SRK2 (SRK.py:165-396) is three launches over all replicas of a
 // pool:
 //   k_stage<0>  forces at x   (springs, pair, propulsion) -> x1, F0, noise
-//   k_stage<1>  forces at x1, F = F0 + F1, max |F|^2, last block sets dt
+//   k_stage<1>  forces at x1, F = F0 + F1, per-block max |F|^2
+//   k_finalize  per-replica max force -> dt, dt_used (SRK.py:323-353)
 //   k_update    x <- x + dt_used/(2r) F + sqrt(2 D dt_used) xi, rot. diffusion
 // Every `rebuild_every` steps the beads are re-sorted by cell (counting sort):
 //   k_zero_counts, k_bin, k_scan_{chunks,reps,final}, k_scatter,
@@ -116,6 +117,15 @@ __device__ __forceinline__ bool step_runs(const RepState* S, int j)
     bool a = (S->max_steps < 0) || (S->steps < S->max_steps);
     bool b = (S->mdu < 0) || (S->elapsed < S->mdu);
     return a && b;
+}
+
+// A cell re-sort placed before local step j runs only when that step runs
+// (j < 0: unconditional, for set-up and diagnostics), so the arrays of the
+// last executed step (x1, forces) keep the layout they were computed in.
+__device__ __forceinline__ bool rebuild_runs(const RepState* S, int j)
+{
+    if (S->frozen) return false;
+    return j < 0 || step_runs(S, j);
 }
 
 template <int DIM>
@@ -411,23 +421,22 @@ __device__ __forceinline__ void pair_sum(const double* X, const int32_t* META, c
 
 // ------------------------------------------------------------------ stages
 
-template <int DIM>
-__device__ __forceinline__ void block_max_u64(uint64_t v, uint64_t* target)
+// Max over the block (wave64 shuffles, then the block's waves through LDS);
+// the result is valid in thread 0.
+__device__ __forceinline__ double block_max(double v)
 {
-    // wave64 max via shuffles, then across the block's waves in LDS
-    __shared__ uint64_t red[BS / 64];
+    __shared__ double red[BS / 64];
     for (int off = 32; off > 0; off >>= 1) {
-        uint64_t o = __shfl_xor(v, off, 64);
+        double o = __shfl_xor(v, off, 64);
         v = o > v ? o : v;
     }
-    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     if (l == 0) red[w] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t m = red[0];
-        for (int i = 1; i < BS / 64; ++i) m = red[i] > m ? red[i] : m;
-        atomicMax((unsigned long long*)target, (unsigned long long)m);
-    }
+    double m = red[0];
+    for (int i = 1; i < BS / 64; ++i) m = red[i] > m ? red[i] : m;
+    __syncthreads();
+    return m;
 }
 
 // Stage 0 (STAGE == 0): forces at x, predictor x1 = x + dt/r F0 + sqrt(2 D dt) xi.
@@ -507,31 +516,38 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
         }
     }
     if (STAGE == 1) {
-        block_max_u64<DIM>(__double_as_longlong(fsq), &S->fmax2);
-        __shared__ uint32_t ticket;
-        __threadfence();
-        if (threadIdx.x == 0) ticket = atomicAdd(&S->blocks_done, 1u);
-        __syncthreads();
-        if (threadIdx.x == 0 && ticket == (uint32_t)(P.nblk - 1)) {
-            // finalize: max force and adaptive dt (SRK.py:323-353)
-            __threadfence();
-            uint64_t bits = atomicMax((unsigned long long*)&S->fmax2, 0ull);
-            double mx = __longlong_as_double((long long)bits);
-            double m = sqrt(2.0 * P.D);
-            if (P.n > 0) m += 0.5 * sqrt(mx) / P.radius;
-            double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
-            double dt_used = P.cheat * dt;
-            if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
-            S->dt = dt;
-            S->dt_used = dt_used;
-            S->elapsed += dt_used;
-            S->maxm = m;
-            S->steps += 1;
-            S->step_global += 1;
-            S->fmax2 = 0ull;
-            S->blocks_done = 0u;
-            __threadfence();
-        }
+        // per-block max |F0+F1|^2; k_finalize reduces it per replica
+        const double m = block_max(fsq);
+        if (threadIdx.x == 0) a.BMAX[blockIdx.x] = m;
+    }
+}
+
+// Max force and adaptive dt of each replica (SRK.py:323-353), one block per
+// replica, after the stage-1 launch has written every block's maximum.
+__global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
+{
+    const int r = blockIdx.x;
+    RepState* S = a.S + r;
+    if (!step_runs(S, j)) return;
+    const RepParams& P = a.P[r];
+    double m2 = 0.0;
+    for (int i = threadIdx.x; i < P.nblk; i += BS) {
+        double v = a.BMAX[P.blk_off + i];
+        m2 = v > m2 ? v : m2;
+    }
+    m2 = block_max(m2);
+    if (threadIdx.x == 0) {
+        double m = sqrt(2.0 * P.D);
+        if (P.n > 0) m += 0.5 * sqrt(m2) / P.radius;
+        double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
+        double dt_used = P.cheat * dt;
+        if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
+        S->dt = dt;
+        S->dt_used = dt_used;
+        S->elapsed += dt_used;
+        S->maxm = m;
+        S->steps += 1;
+        S->step_global += 1;
     }
 }
 
@@ -632,11 +648,11 @@ __global__ void __launch_bounds__(BS) k_fill_i32(int32_t* p, int n, int32_t v)
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(BS) k_bin(PoolArgs a)
+__global__ void __launch_bounds__(BS) k_bin(PoolArgs a, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (s >= P.bead_off + P.n) return;
     double x[DIM];
@@ -674,11 +690,11 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total)
 #define SCAN_CHUNK (BS * SCAN_PER_THREAD)
 
 // scan phase A: one chunk of SCAN_CHUNK cells of one replica -> its sum
-__global__ void __launch_bounds__(BS) k_scan_chunks(PoolArgs a)
+__global__ void __launch_bounds__(BS) k_scan_chunks(PoolArgs a, int j)
 {
     const int r = a.chk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int c0 = a.chk_cell0[blockIdx.x];
     const int cend = P.cell_off + P.ncell;
     int sum = 0;
@@ -692,10 +708,10 @@ __global__ void __launch_bounds__(BS) k_scan_chunks(PoolArgs a)
 }
 
 // scan phase B: one block per replica scans its chunk sums in place
-__global__ void __launch_bounds__(BS) k_scan_reps(PoolArgs a)
+__global__ void __launch_bounds__(BS) k_scan_reps(PoolArgs a, int j)
 {
     const int r = blockIdx.x;
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int k0 = a.rep_chk_off[r], k1 = a.rep_chk_off[r + 1];
     int carry = 0;
     for (int base = k0; base < k1; base += BS) {
@@ -710,11 +726,11 @@ __global__ void __launch_bounds__(BS) k_scan_reps(PoolArgs a)
 
 // scan phase C: exclusive scan inside each chunk, offset by the chunk prefix
 // and the replica's first slot -> CSTART; the last chunk also writes the end
-__global__ void __launch_bounds__(BS) k_scan_final(PoolArgs a)
+__global__ void __launch_bounds__(BS) k_scan_final(PoolArgs a, int j)
 {
     const int r = a.chk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int c0 = a.chk_cell0[blockIdx.x];
     const int cend = P.cell_off + P.ncell;
     int v[SCAN_PER_THREAD];
@@ -736,11 +752,11 @@ __global__ void __launch_bounds__(BS) k_scan_final(PoolArgs a)
     }
 }
 
-__global__ void __launch_bounds__(BS) k_scatter(PoolArgs a)
+__global__ void __launch_bounds__(BS) k_scatter(PoolArgs a, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (s >= P.bead_off + P.n) return;
     int2 ko = *reinterpret_cast<const int2*>(a.KEYOFF + 2 * s);
@@ -750,11 +766,11 @@ __global__ void __launch_bounds__(BS) k_scatter(PoolArgs a)
 
 // Order each cell's beads by original index so the layout (and with it the
 // pair summation order) is a function of the positions alone.
-__global__ void __launch_bounds__(BS) k_cellsort(PoolArgs a)
+__global__ void __launch_bounds__(BS) k_cellsort(PoolArgs a, int j)
 {
     const int r = a.chk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int cend = P.cell_off + P.ncell;
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         int c = a.chk_cell0[blockIdx.x] + k * BS + threadIdx.x;
@@ -775,11 +791,11 @@ __global__ void __launch_bounds__(BS) k_cellsort(PoolArgs a)
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t)
+__global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (ns >= P.bead_off + P.n) return;
     int2 pr = *reinterpret_cast<const int2*>(a.PAIRS + 2 * ns);
@@ -798,11 +814,11 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t)
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t)
+__global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (a.S[r].frozen) return;
+    if (!rebuild_runs(a.S + r, j)) return;
     const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (ns >= P.bead_off + P.n) return;
     int2 me = *reinterpret_cast<const int2*>(a.METAn + 2 * ns);

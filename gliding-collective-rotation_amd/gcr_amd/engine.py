@@ -213,20 +213,26 @@ class System:
         return F, T
 
 
-def init_chains(dim, L, chain_len, bond, min_dist=0.9, seed=0, max_tries=100000):
-    """Random non-overlapping straight chains (gcr_init_chains)."""
+def init_chains(dim, L, chain_len, bond, min_dist=0.9, seed=0, max_tries=20000,
+                min_dist_floor=None, return_min_dist=False):
+    """Random straight chains without overlap (gcr_init_chains). With
+    min_dist_floor below min_dist, dense boxes that random sequential
+    placement cannot fill are completed at a smaller exclusion distance."""
     lib = _lib.load()
     cl = np.ascontiguousarray(chain_len, dtype=np.int32)
     n = int(cl.sum())
     pos = np.empty((n, dim))
     tang = np.empty((n, dim))
+    used = C.c_double()
     f = lib.gcr_init_chains
     f.restype = C.c_int
     f.argtypes = [C.c_int32, C.c_double, C.c_int64, C.c_void_p, C.c_double, C.c_double,
-                  C.c_uint64, C.c_int64, C.c_void_p, C.c_void_p]
-    rc = f(dim, float(L), len(cl), ptr(cl), float(bond), float(min_dist),
-           int(seed) & ((1 << 64) - 1), int(max_tries), ptr(pos), ptr(tang))
+                  C.c_double, C.c_uint64, C.c_int64, C.c_void_p, C.c_void_p,
+                  C.POINTER(C.c_double)]
+    floor = min_dist if min_dist_floor is None else min_dist_floor
+    rc = f(dim, float(L), len(cl), ptr(cl), float(bond), float(min_dist), float(floor),
+           int(seed) & ((1 << 64) - 1), int(max_tries), ptr(pos), ptr(tang), C.byref(used))
     if rc != 0:
         raise _lib.GcrError(
             f"gcr_init_chains failed ({rc}): could not place every chain without overlap")
-    return pos, tang
+    return (pos, tang, used.value) if return_min_dist else (pos, tang)

@@ -1,0 +1,168 @@
+"""Population objects with the reference's interface, for driving the GPU path.
+
+Mirrors the attributes and the force-list "plugin" interface that the hot
+path consumes (populations.py:61-611): state arrays pos, pos1, noise_vec,
+where_active, tangent_list, forces; chain layout chain_id, rest_lengths,
+phonebook; parameter vectors WCA_params (make_WCA_params), spring_params(),
+active_params(); and self_interactions() returning [[name, params], ...].
+Constructors take the reference's `sim` dictionary (defaults.set_defaults).
+
+Differences from the reference, by design: random_initialization places
+straight chains without overlap (gcr_init_chains) instead of the dot-like
+chains the growth phase expands (populations.py:530-557, control.py:153-219);
+motors, shells, growth/mitosis and walls are not part of this path. An
+optional per-chain chirality array `v_tan_angle_per_chain` extends the
+scalar v_tan_angle (a per-population constant reduces to the reference).
+"""
+import numpy as np
+
+from . import config
+from .engine import init_chains
+
+
+def make_WCA_params(repel_d_max, WCA_epsilon, WCA_shift, LJ_cutoff_factor):
+    """populations.py:48-58: [12 eps r^6, 12 eps r^12, shift, cutoff]."""
+    s6 = repel_d_max ** 6
+    fac = 12 * s6 * WCA_epsilon
+    return np.array([fac, fac * s6, WCA_shift, LJ_cutoff_factor * repel_d_max - WCA_shift])
+
+
+class Population:
+    """Base population (populations.py:61-141)."""
+
+    def __init__(self, total_pop, sim, name="pop"):
+        self.total_pop = total_pop
+        self.pos = np.empty((total_pop, config.dim))
+        self.pos1 = np.empty_like(self.pos)
+        self.noise_vec = np.empty((total_pop, 2 * config.dim))
+        self.name = name
+        self.special_interactions = {}
+        self.v0 = 0.0
+        self.where_active = np.full(total_pop, 0)
+        self.n_nearby_beads_dont_repel = 0
+        self.b_init_growing = False
+        self.pos_init_noise = sim["pos_init_noise"]
+        self.D = sim["D"]
+        self.Dr = sim["Dr"]
+        self.initialized = False
+        self.fixed = False
+
+    def interactions(self, pop):
+        return []
+
+    def self_interactions(self):
+        return []
+
+
+class WCA_Population(Population):
+    """Beads with a WCA/LJ pair force (populations.py:143-241)."""
+
+    def __init__(self, total_pop, sim, name="beads"):
+        super().__init__(total_pop, sim, name=name)
+        self.forces = np.zeros((total_pop, config.dim))
+        self.repel_d_max = sim["repel_d_max"]
+        self.WCA_epsilon = sim["WCA_epsilon"]
+        self.WCA_shift = sim["WCA_shift"]
+        self.LJ_cutoff_factor = sim["LJ_cutoff_factor"]
+        self.adjust_WCA_params()
+        self.sim = sim
+        if self.D < 0:  # Stokes-Einstein defaults, populations.py:195-203
+            self.D = sim["T"] / self.repel_d_max
+        if self.Dr < 0:
+            self.Dr = 3 * self.D / (4 * self.repel_d_max ** 2)
+
+    def adjust_WCA_params(self):
+        self.WCA_params = make_WCA_params(self.repel_d_max, self.WCA_epsilon, self.WCA_shift,
+                                          self.LJ_cutoff_factor)
+
+    def self_interactions(self):
+        return super().self_interactions() + [["WCA", self.WCA_params]]
+
+
+class Oriented_WCA_Population(WCA_Population):
+    """Beads with a propulsion direction (populations.py:244-301)."""
+
+    def __init__(self, total_pop, sim, name="beads"):
+        super().__init__(total_pop, sim, name=name)
+        self.v0 = sim["v0"]
+        self.tangent_list = np.empty((total_pop, config.dim))
+        self.tangent_list[:] = 0.0
+        self.tangent_list[:, 0] = 1.0
+
+
+class Chain_Collection(Oriented_WCA_Population):
+    """Beads strung into chains (populations.py:305-591)."""
+
+    def __init__(self, num_subpops, total_pop, sim, name="beads", chain_id=None):
+        super().__init__(total_pop, sim, name=name)
+        self.num_subpops = num_subpops
+        if chain_id is None:
+            chain_id = np.floor(np.arange(total_pop) / (total_pop / num_subpops)).astype(np.int64)
+        self.chain_id = np.asarray(chain_id, dtype=np.int64)
+        self.k = sim["k"]
+        self.reference_rest_length = sim["a"]
+        self.rest_lengths = np.full(num_subpops, sim["a"], dtype=np.float64)
+        self.spontaneous_bond_angle = sim["spontaneous_bond_angle"]
+        if sim["kappa"] >= 0:
+            self.rigid_rod_projection_factor = sim["rigid_rod_projection_factor"]
+            self.kappa = sim["kappa"]
+        else:
+            self.rigid_rod_projection_factor = 1
+            self.kappa = 0
+        self.n_nearby_beads_dont_repel = sim["n_nearby_beads_dont_repel"]
+        self.v_tan_angle = sim["v_tan_angle"]
+        self.b_use_tangent_as_propulsion_direction = sim["b_use_tangent_as_propulsion_direction"]
+        self.sliding_partner_maxdist = sim["sliding_partner_maxdist"]
+        self.b_FENE = sim["b_FENE"]
+        self.FENE_tolerance = sim["FENE_tolerance"]
+        self.longest_chain_length = int(np.max(np.bincount(self.chain_id)))
+        self.b_get_orientations = False
+        self.v_tan_angle_per_chain = None
+
+    def spring_params(self):
+        return np.array([self.k, self.kappa, self.spontaneous_bond_angle, float(int(self.b_FENE)),
+                         float(self.b_get_orientations), self.FENE_tolerance])
+
+    def active_params(self):
+        return np.array([self.v0, self.sliding_partner_maxdist, self.v_tan_angle])
+
+    def num_chains(self):
+        return self.num_subpops
+
+    def self_interactions(self):
+        """populations.py:570-591: springs first (they update tangents), WCA,
+        then self-propulsion."""
+        flist = super().self_interactions()
+        flist.append(["self-propulsion", self.active_params()])
+        if self.longest_chain_length > 1:
+            flist.insert(0, ["springs", self.spring_params()])
+        if self.sliding_partner_maxdist > 0 and self.v0 > 0:
+            flist = [f for f in flist if f[0] not in ("WCA", "self-propulsion")]
+            flist.append(["active_sliding_and_WCA",
+                          np.append(self.WCA_params, self.active_params())])
+        return flist
+
+    def random_initialization(self, seed=0, min_dist=0.9):
+        lens = np.bincount(self.chain_id).astype(np.int32)
+        pos, tang = init_chains(config.dim, config.Lx, lens, self.reference_rest_length,
+                                min_dist=min_dist, seed=seed)
+        self.pos[:] = pos
+        self.pos1[:] = pos
+        self.tangent_list[:] = tang
+        self.initialized = True
+
+
+class Equal_Size_Chain_Collection(Chain_Collection):
+    """Chains of equal length (populations.py:593-614)."""
+
+    def __init__(self, sim, num_chains=-1, chain_length=-1, name="beads"):
+        self.num_subpops = sim["n_chains"] if num_chains < 0 else num_chains
+        self.subpop_size = sim["chain_length"] if chain_length < 0 else chain_length
+        total = self.num_subpops * self.subpop_size
+        chain_id = np.floor(np.arange(total) / self.subpop_size).astype(np.int64)
+        super().__init__(self.num_subpops, total, sim, name=name, chain_id=chain_id)
+        self.b_get_orientations = (self.subpop_size > 1
+                                   and self.b_use_tangent_as_propulsion_direction)
+
+    def chain_length(self):
+        return self.subpop_size

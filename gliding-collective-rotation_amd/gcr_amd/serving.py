@@ -1,0 +1,212 @@
+"""Serving: a request loop with continuous batching of simulations.
+
+Requests (a simulation: initial state, physics, number of steps) arrive over
+time. The server admits a request as soon as its pool has room for it under
+the HBM budget (beads, cells, chains, replica slots reserved at start-up),
+keeps its state resident on the device, advances every admitted simulation
+in the same launches batch after batch, and retires a simulation the batch
+it completes, freeing its room for the next arrivals. 2D and 3D requests go
+to two pools driven from two host threads, so their launches overlap on the
+device. Results do not depend on what else is running: noise is keyed by
+(request seed, bead, step), reductions are per replica and order-free, and
+the cell-list schedule follows each replica's own step count.
+"""
+import ctypes as C
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from . import engine as E
+from . import workloads as W
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float      # seconds after the stream starts
+    work: W.Workload    # initial state + physics (the client's payload)
+    steps: int
+    admitted: float = -1.0
+    finished: float = -1.0
+    result: dict = field(default_factory=dict)
+
+    @property
+    def dim(self):
+        return self.work.phys.dim
+
+    @property
+    def n(self):
+        return self.work.n
+
+
+def make_request_workload(rng, dim, n_target, abp, seed, T=0.01, phi=0.25):
+    """ABPs (chain length 1, D_r = 0.1) or chiral chains (length 5..40)."""
+    if abp:
+        ln, nc = 1, max(1, n_target)
+    else:
+        ln = int(rng.integers(5, 41))
+        nc = max(1, n_target // ln)
+    vt = float(rng.uniform(-0.5, 0.5)) if not abp else 0.0
+    if dim == 2:
+        L = W.box_from_area_fraction(nc, ln, 0.5, 1.0, phi)
+    else:  # volume fraction of beads of diameter 1
+        L = (nc * ln * (np.pi / 6) / phi) ** (1.0 / 3.0)
+    return W.chains(f"req{seed}", dim, nc, ln, L=L, vtan=vt, T=T, Dr=0.1 if abp else None,
+                    dtmax=1e-3, seed=seed)
+
+
+def make_stream(seed=0, n_requests=64, rate=20.0, n_min=1000, n_max=200000, steps_min=1000,
+                steps_max=100000, scale_steps=1.0):
+    """Poisson arrivals; bead counts and step counts log-uniform; half ABPs,
+    half chains; half 2D, half 3D. scale_steps shortens every request."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        n = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        steps = int(np.exp(rng.uniform(np.log(steps_min), np.log(steps_max))) * scale_steps)
+        dim = 2 if rng.random() < 0.5 else 3
+        abp = rng.random() < 0.5
+        w = make_request_workload(rng, dim, n, abp, seed=seed * 1000003 + i)
+        reqs.append(Request(i, t, w, max(1, steps)))
+    return reqs
+
+
+def replica_cells(w):
+    lib = _lib.load()
+    d = w.phys.desc()
+    return int(lib.gcr_replica_cells(w.phys.dim, C.byref(d))) + 1
+
+
+class _Lane:
+    """One pool (one dimensionality) and its bookkeeping."""
+
+    def __init__(self, dim, budget_bytes, precision, steps_per_batch, rebuild_every,
+                 max_replicas):
+        # split the byte budget: per bead ~ gcr_replica_bytes(dim, 1, 1, 2)
+        lib = _lib.load()
+        per_bead = int(lib.gcr_replica_bytes(dim, 1, 1, 2))
+        self.cap_beads = max(1024, budget_bytes // per_bead)
+        self.cap_cells = 2 * self.cap_beads + 1024
+        self.cap_chains = self.cap_beads
+        self.cap_reps = max_replicas
+        self.pool = E.Pool(dim, precision, self.cap_beads, self.cap_cells, self.cap_chains,
+                           max_replicas)
+        self.pool.set_schedule(steps_per_batch, rebuild_every)
+        self.used = np.zeros(3, dtype=np.int64)  # beads, cells, chains
+        self.active = {}  # rid -> (request, cells)
+        self.batches = 0
+
+    def fits(self, req, cells):
+        need = np.array([req.n, cells, req.work.phys.n_chains])
+        cap = np.array([self.cap_beads, self.cap_cells, self.cap_chains])
+        return len(self.active) < self.cap_reps and np.all(self.used + need <= cap)
+
+    def admit(self, req, cells, now):
+        w = req.work
+        try:
+            rid = self.pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths,
+                                w.v_tan_angle, w.v0_per_chain)
+        except _lib.GcrError:
+            return False  # fragmented: wait for more room
+        tv = w.time_vals.copy()
+        tv[7] = req.steps
+        self.pool.begin(rid, tv)
+        self.used += [req.n, cells, w.phys.n_chains]
+        self.active[rid] = (req, cells)
+        req.admitted = now
+        return True
+
+    def step_batch(self):
+        if self.active:
+            self.pool.run(max_batches=1)
+            self.batches += 1
+
+    def retire(self, now, keep_results):
+        done = []
+        for rid, (req, cells) in list(self.active.items()):
+            st = self.pool.status(rid)
+            if not st["running"]:
+                if keep_results:
+                    req.result = self.pool.download(rid, pos1=False, forces=False)
+                req.result["status"] = st
+                req.finished = now
+                self.pool.remove(rid)
+                self.used -= [req.n, cells, req.work.phys.n_chains]
+                del self.active[rid]
+                done.append(req)
+        return done
+
+
+class Server:
+    def __init__(self, hbm_budget_bytes=8 << 30, precision=E.GCR_MIXED, steps_per_batch=64,
+                 rebuild_every=8, max_replicas=256, split_2d=0.5):
+        self.lanes = {
+            2: _Lane(2, int(hbm_budget_bytes * split_2d), precision, steps_per_batch,
+                     rebuild_every, max_replicas),
+            3: _Lane(3, int(hbm_budget_bytes * (1 - split_2d)), precision, steps_per_batch,
+                     rebuild_every, max_replicas),
+        }
+
+    def close(self):
+        for lane in self.lanes.values():
+            lane.pool.close()
+
+    def serve(self, requests, keep_results=False, realtime=True):
+        """Run the stream. With realtime, requests become visible at their
+        arrival time; otherwise all are visible at once (offline batch)."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        for r in pending:
+            if not self.lanes[r.dim].fits(r, 0) and r.n > self.lanes[r.dim].cap_beads:
+                raise ValueError(f"request {r.rid} exceeds the HBM budget of its pool")
+        cells = {r.rid: replica_cells(r.work) for r in pending}
+        waiting = []
+        finished = []
+        t0 = time.perf_counter()
+        lock = threading.Lock()
+
+        def now():
+            return time.perf_counter() - t0
+
+        while pending or waiting or any(l.active for l in self.lanes.values()):
+            t = now()
+            while pending and (not realtime or pending[0].arrival <= t):
+                waiting.append(pending.pop(0))
+            # admission in arrival order; a request that does not fit waits,
+            # smaller later ones may pass it (backfill)
+            still = []
+            for r in waiting:
+                lane = self.lanes[r.dim]
+                if lane.fits(r, cells[r.rid]) and lane.admit(r, cells[r.rid], now()):
+                    continue
+                still.append(r)
+            waiting = still
+            busy = [l for l in self.lanes.values() if l.active]
+            if not busy:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                continue
+            threads = [threading.Thread(target=l.step_batch) for l in busy]
+            for th in threads:
+                th.start()
+            for th in threads:
+                th.join()
+            t = now()
+            for l in busy:
+                with lock:
+                    finished += l.retire(t, keep_results)
+        return finished
+
+
+def report(finished):
+    lat = np.array([r.finished - r.arrival for r in finished])
+    work = sum(r.n * r.result["status"]["steps"] for r in finished)
+    span = max(r.finished for r in finished) - min(r.arrival for r in finished)
+    return dict(requests=len(finished), bead_steps=int(work), makespan_s=span,
+                bead_steps_per_s=work / span, p50_latency_s=float(np.percentile(lat, 50)),
+                p99_latency_s=float(np.percentile(lat, 99)),
+                mean_queue_s=float(np.mean([r.admitted - r.arrival for r in finished])))

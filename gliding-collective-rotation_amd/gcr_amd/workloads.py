@@ -58,12 +58,13 @@ class Workload:
 def chains(name, dim, n_chains, chain_len, *, phi=None, L=None, a=0.5, r=1.0, eps=1.0,
            cutoff_factor=1.0, shift=0.0, k=100.0, kappa=10.0, theta0=0.0, v0=1.0, vtan=0.0,
            T=1e-3, Dr=None, dtmax=1e-3, steps=10000, seed=0, n_nearby=4, tangent_locked=True,
-           skin=0.0, min_dist=0.9):
+           skin=0.0, min_dist=0.9, min_dist_floor=0.5):
     """A population of equal chains (Equal_Size_Chain_Collection, populations.py:593-611)."""
     if L is None:
         L = box_from_area_fraction(n_chains, chain_len, a, r, phi, dim)
     lens = np.full(n_chains, chain_len, dtype=np.int32)
-    pos, tang = init_chains(dim, L, lens, a, min_dist=min_dist, seed=seed)
+    pos, tang = init_chains(dim, L, lens, a, min_dist=min_dist, seed=seed,
+                            min_dist_floor=min_dist_floor)
     chain_id = np.repeat(np.arange(n_chains, dtype=np.int64), chain_len)
     wca = wca_params(r, eps, shift, cutoff_factor)
     D = T / r

@@ -132,14 +132,17 @@ int gcr_pool_destroy(gcr_pool* p);
 /* Host-side initial condition (no device work): n_chains straight chains of
  * chain_len[c] beads at spacing `bond`, centres uniform in the box and
  * directions uniform on the circle / sphere, placed one after another and
- * rejected while any bead comes closer than min_dist to a placed bead.
- * Bead 0 of each chain leads; tangent[b] is the chain direction, pointing
- * towards bead 0 as the reference's tangents do (forces.py:351-359). Plays
- * the role of random_initialization + the growth phase
- * (populations.py:530-557, control.py:153-219). Returns GCR_ERR_STATE when a
- * chain could not be placed in max_tries attempts. */
+ * rejected while any bead comes closer than the exclusion distance to a
+ * placed bead. The distance starts at min_dist and drops by 10% (not below
+ * min_dist_floor) when a chain finds no place in max_tries attempts; the
+ * value reached is returned in *min_dist_used. Bead 0 of each chain leads;
+ * tangent[b] is the chain direction, pointing towards bead 0 as the
+ * reference's tangents do (forces.py:351-359). Plays the role of
+ * random_initialization + the growth phase (populations.py:530-557,
+ * control.py:153-219). GCR_ERR_STATE when even the floor distance fails. */
 int gcr_init_chains(int32_t dim, double L, int64_t n_chains, const int32_t* chain_len, double bond,
-                    double min_dist, uint64_t seed, int64_t max_tries, double* pos, double* tangent);
+                    double min_dist, double min_dist_floor, uint64_t seed, int64_t max_tries,
+                    double* pos, double* tangent, double* min_dist_used);
 
 #ifdef __cplusplus
 }

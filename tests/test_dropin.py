@@ -1,0 +1,90 @@
+"""The drop-in SRK2_force_calc (updates.py:132-267 replacement) driven the way
+control.sim_loop drives it (control.py:24-84): population objects with the
+reference's interface, a status object and the sim dictionary."""
+import numpy as np
+import pytest
+
+import gcr_oracle_np as N
+import oracle_lib as O
+from gcr_amd import config, defaults, dropin, populations
+
+
+def make_run(n_chains=6, chain_length=7, Lx=20.0, **over):
+    sim = defaults.set_defaults()
+    sim.update(dict(n_chains=n_chains, chain_length=chain_length, Lx=Lx, kappa=0.0, T=0.01,
+                    dtmax=1e-3, seed=5, t_init_grow=0.0, save_state_every=-1.0))
+    sim.update(over)
+    sim["dt0"] = 1e-4 * sim["dtmax"]
+    config.configure(sim)
+    pop = populations.Equal_Size_Chain_Collection(sim)
+    pop.random_initialization(seed=3)
+    config.grid_list_setup(sim, [pop])
+    status = config.status_class(sim, start_t=0.0)
+    return sim, pop, status
+
+
+def test_dropin_rejects_unported_features():
+    sim, pop, status = make_run()
+    f = dropin.make_SRK2_force_calc(config)
+    with pytest.raises(NotImplementedError, match="one"):
+        f([pop, pop], status, sim)
+    pop.sliding_partner_maxdist = 1.0
+    with pytest.raises(NotImplementedError, match="active_sliding"):
+        f([pop], status, sim)
+    pop.sliding_partner_maxdist = 0.0
+    pop.b_FENE = True
+    with pytest.raises(NotImplementedError, match="FENE"):
+        f([pop], status, sim)
+
+
+def test_reference_shaped_objects():
+    sim, pop, status = make_run()
+    assert pop.WCA_params.tolist() == [12.0, 12.0, 0.0, 1.0]
+    assert [f[0] for f in pop.self_interactions()] == ["springs", "WCA", "self-propulsion"]
+    assert pop.spring_params().tolist() == [100.0, 0.0, 0.0, 0.0, 1.0, 0.1]
+    assert pop.D == 0.01 and pop.Dr == 0.0075
+    assert config.max_dist_per_step == 0.0125 and config.n_steps_update_grid_list == 40
+    assert status.dt == pytest.approx(1e-7)
+
+
+@pytest.mark.gpu
+def test_dropin_chunks_match_oracle():
+    """Three driver chunks through the drop-in equal one oracle run with the
+    same in-kernel noise; status bookkeeping as updates.py:249-256."""
+    sim, pop, status = make_run(v_tan_angle=0.15)
+    x0, t0 = pop.pos.copy(), pop.tangent_list.copy()
+    f = dropin.make_SRK2_force_calc(config)
+    steps_total = 0
+    for chunk in (30, 50, 20):
+        status.next_output_step = status.stepnum + chunk
+        f([pop], status, sim)
+        steps_total += chunk
+        assert status.stepnum == steps_total
+    assert np.all(pop.where_active == -1)
+    seed = dropin._mix(sim["seed"], 1)
+    sysd = dict(dim=2, n=pop.total_pop, n_nearby=4, has_springs=True, has_wca=True,
+                has_propel=True, L=20.0, D=pop.D, Dr=pop.Dr, radius=1.0, wca=pop.WCA_params,
+                spring=pop.spring_params(), v0=1.0)
+    tv = np.array([sim["dt0"], sim["dtmax"], config.max_dist_per_step, 1.0, -1, 40, 40, 100])
+    pos_o, pos1_o, T_o, F_o, st_o = O.srk2_run(sysd, x0, t0, pop.chain_id, pop.rest_lengths,
+                                               np.full(6, 0.15), tv, xi=None, seed=seed)
+    assert np.max(np.abs(N.min_image(pop.pos - pos_o, 20.0))) < 1e-10 * 20.0
+    assert np.max(np.abs(N.min_image(pop.pos1 - pos1_o, 20.0))) < 1e-10 * 20.0
+    assert np.allclose(pop.tangent_list, T_o, atol=1e-9)
+    assert status.dt == pytest.approx(st_o[0], rel=1e-10)
+    # status.t accumulates the elapsed time of every chunk
+    assert status.t == pytest.approx(st_o[1], rel=1e-10)
+
+
+@pytest.mark.gpu
+def test_dropin_sees_host_edits():
+    """Positions edited on the host between calls are uploaded again."""
+    sim, pop, status = make_run()
+    f = dropin.make_SRK2_force_calc(config)
+    status.next_output_step = 10
+    f([pop], status, sim)
+    pop.pos[0] = np.mod(pop.pos[0] + 0.01, 20.0)
+    edited = pop.pos.copy()
+    status.next_output_step = status.stepnum  # zero steps: state must come back unchanged
+    f([pop], status, sim)
+    assert np.array_equal(pop.pos, edited)
