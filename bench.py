@@ -32,8 +32,8 @@ ALG_BYTES = {"stage0": 32, "stage1": 16, "update": 32}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1024)
+    ap.add_argument("--warmup", type=int, default=128)
     ap.add_argument("--batch", type=int, default=64, help="steps per captured graph")
     ap.add_argument("--rebuild", type=int, default=8, help="cell-list rebuild period")
     ap.add_argument("--skin", type=float, default=0.0)
@@ -42,6 +42,16 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline budget: steps are added in chunks of 40 until this much time")
     ap.add_argument("--kernel-steps", type=int, default=32)
+    ap.add_argument("--mode", choices=("single", "ensemble", "serve"), default="single",
+                    help="single: 400k-bead system (the headline line); ensemble: 256 "
+                         "replicas batched vs one at a time; serve: request stream")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--seq-sample", type=int, default=16,
+                    help="ensemble: replicas timed one at a time for the sequential rate")
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--steps-scale", type=float, default=0.25,
+                    help="serve: multiplies every request's step count")
+    ap.add_argument("--budget-gb", type=float, default=16.0)
     return ap.parse_args()
 
 
@@ -106,11 +116,73 @@ def cpu_baseline(w, budget_s):
                        f"rebuilt every 40 steps, Newton-3 inside each thread chunk)")
 
 
+def main_ensemble(args):
+    """256 replicas of uneven size (config 4) in one pool vs one at a time."""
+    from gcr_amd import ensemble as ENS
+    from gcr_amd import workloads as W
+    t0 = time.perf_counter()
+    ws = W.config4_replicas(seed=0, n_replicas=args.replicas, steps=args.steps)
+    t_init = time.perf_counter() - t0
+    sec, bs, pool, rids = ENS.run_batched(ws, args.steps, steps_per_batch=args.batch,
+                                          rebuild_every=args.rebuild, warmup=args.warmup)
+    replays = sum(pool.status(r)["replays"] for r in rids)
+    pool.close()
+    rng = np.random.default_rng(1)
+    sample = [ws[i] for i in rng.choice(len(ws), size=min(args.seq_sample, len(ws)),
+                                       replace=False)]
+    seq_steps = max(64, args.steps // 4)
+    ssec, sbs = ENS.run_sequential(sample, seq_steps, steps_per_batch=args.batch,
+                                   rebuild_every=args.rebuild)
+    out = {"metric": "ensemble aggregate bead-steps/sec (256 replicas, 1 GPU)",
+           "value": bs / sec, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 forces + f64 state", "data": "synthetic",
+           "config": {"workload": "config 4: chain counts log-U[50,2000], lengths U{5..40}, "
+                                  "chirality U[-0.5,0.5], v0 U[0.1,2], phi U[0.1,0.5], WCA, "
+                                  "dt=1e-3", **ENS.summary(ws)},
+           "sequential": {"value": sbs / ssec, "replicas_timed": len(sample),
+                          "steps_each": seq_steps},
+           "batched_over_sequential": (bs / sec) / (sbs / ssec),
+           "replays": replays, "host_init_s": t_init}
+    print(json.dumps(out), flush=True)
+
+
+def main_serve(args):
+    """Poisson request stream (config 5) through the continuous-batching server."""
+    from gcr_amd import serving as SV
+    t0 = time.perf_counter()
+    reqs = SV.make_stream(seed=0, n_requests=args.requests, scale_steps=args.steps_scale)
+    t_init = time.perf_counter() - t0
+    srv = SV.Server(hbm_budget_bytes=int(args.budget_gb * (1 << 30)),
+                    steps_per_batch=args.batch, rebuild_every=args.rebuild)
+    done = srv.serve(reqs)
+    srv.close()
+    rep = SV.report(done)
+    offered = sum(r.n * r.steps for r in reqs) / max(r.arrival for r in reqs)
+    out = {"metric": "serving aggregate bead-steps/sec + p50/p99 completion latency (1 GPU)",
+           "value": rep["bead_steps_per_s"], "unit": "bead-steps/s", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 forces + f64 state", "data": "synthetic",
+           "config": {"workload": "config 5: Poisson 20 req/s, beads log-U[1k,200k], steps "
+                                  "log-U[1e3,1e5] x steps_scale, ABPs (D_r=0.1) and chains "
+                                  "5-40, 2D and 3D", "requests": len(reqs),
+                      "steps_scale": args.steps_scale, "hbm_budget_gb": args.budget_gb},
+           "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
+           "mean_queue_s": rep["mean_queue_s"], "makespan_s": rep["makespan_s"],
+           "offered_bead_steps_per_s": offered, "host_init_s": t_init}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.mode == "ensemble":
+        return main_ensemble(args)
+    if args.mode == "serve":
+        return main_serve(args)
     from gcr_amd import engine as E
     from gcr_amd import workloads as W
     from gcr_amd._lib import device_info

@@ -2,6 +2,9 @@
 // graph-captured step batches, replay of a batch when the cell-list skin was
 // exceeded, and the extern "C" entry points declared in include/gcr.h.
 #include "gcr_kernels.hip"
+#include "gcr_tile2d.hip"
+
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -111,7 +114,10 @@ struct gcr_pool {
     int32_t *CHAIN_FIRST = nullptr, *CHAIN_LEN = nullptr;
     int32_t *CCOUNT = nullptr, *CSTART = nullptr, *CHKSUM = nullptr;
     double* BMAX = nullptr;
-    int64_t capChk = 0, capBlk = 0;
+    int32_t* TILES = nullptr;
+    int64_t capChk = 0, capBlk = 0, capTile = 0;
+    int ntile = 0;
+    int tile_side = 0;  // 0: chosen per replica from the cell occupancy
 };
 
 namespace {
@@ -146,10 +152,23 @@ int push_states(gcr_pool* p)
     return GCR_OK;
 }
 
-// Block and chunk tables of the live replicas.
+// Tile edge (cells) of the 2D LDS force kernel: about BS owned beads per
+// workgroup, at most TILE_MAX_ROWS, and never more than the grid.
+int pick_tile(const RepParams& P, int forced)
+{
+    int side = forced;
+    if (side <= 0) {
+        double occ = (double)P.n / (double)P.ncell;
+        side = (int)std::lround(std::sqrt((double)BS / std::max(occ, 1e-3)));
+    }
+    side = std::max(1, std::min(side, TILE_MAX_ROWS));
+    return std::min(side, P.nc);
+}
+
+// Block, chunk and tile tables of the live replicas.
 int rebuild_tables(gcr_pool* p)
 {
-    std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0);
+    std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0), tl;
     for (int r = 0; r < p->capR; ++r) {
         rco[r] = (int32_t)cr.size();
         RepParams& P = p->P[r];
@@ -164,12 +183,32 @@ int rebuild_tables(gcr_pool* p)
             cr.push_back(r);
             cc.push_back(P.cell_off + c);
         }
+        if (p->dim == 2) {
+            const int side = pick_tile(P, p->tile_side);
+            P.tile_x = P.tile_y = side;
+            P.tile_off = (int32_t)(tl.size() / 4);
+            for (int y = 0; y < P.nc; y += side)
+                for (int x = 0; x < P.nc; x += side) {
+                    tl.push_back(r);
+                    tl.push_back(x);
+                    tl.push_back(y);
+                    tl.push_back(0);
+                }
+            P.ntile = (int32_t)(tl.size() / 4) - P.tile_off;
+            P.red_off = P.tile_off;
+            P.red_n = P.ntile;
+        } else {
+            P.red_off = P.blk_off;
+            P.red_n = P.nblk;
+        }
     }
     rco[p->capR] = (int32_t)cr.size();
-    if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk)
+    if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk ||
+        (int64_t)tl.size() / 4 > p->capTile)
         return fail(GCR_ERR_CAPACITY, "block table capacity exceeded");
     p->nblk = (int)br.size();
     p->nchk = (int)cr.size();
+    p->ntile = (int)(tl.size() / 4);
     if (p->nblk) {
         HIPCHK(hipMemcpyAsync(p->blk_rep, br.data(), br.size() * 4, hipMemcpyHostToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(p->blk_slot0, bs.data(), bs.size() * 4, hipMemcpyHostToDevice, p->stream));
@@ -178,12 +217,15 @@ int rebuild_tables(gcr_pool* p)
         HIPCHK(hipMemcpyAsync(p->chk_rep, cr.data(), cr.size() * 4, hipMemcpyHostToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(p->chk_cell0, cc.data(), cc.size() * 4, hipMemcpyHostToDevice, p->stream));
     }
+    if (p->ntile)
+        HIPCHK(hipMemcpyAsync(p->TILES, tl.data(), tl.size() * 4, hipMemcpyHostToDevice, p->stream));
     HIPCHK(hipMemcpyAsync(p->rep_chk_off, rco.data(), rco.size() * 4, hipMemcpyHostToDevice, p->stream));
     HIPCHK(hipMemcpyAsync(p->dP, p->P.data(), sizeof(RepParams) * p->capR, hipMemcpyHostToDevice,
                           p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     p->a.nblk = p->nblk;
     p->a.nchk = p->nchk;
+    p->a.ntile = p->ntile;
     p->tables_dirty = false;
     drop_graphs(p);
     return GCR_OK;
@@ -202,7 +244,6 @@ void launch_rebuild(gcr_pool* p, int j)
     k_scan_reps<<<p->capR, BS, 0, st>>>(a, j);
     k_scan_final<<<p->nchk, BS, 0, st>>>(a, j);
     k_scatter<<<p->nblk, BS, 0, st>>>(a, j);
-    k_cellsort<<<p->nchk, BS, 0, st>>>(a, j);
     if (p->dim == 2) {
         k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
@@ -212,31 +253,44 @@ void launch_rebuild(gcr_pool* p, int j)
     }
 }
 
-void launch_stage(gcr_pool* p, int which, int j)
+// Launch one kernel of a step; with events (ev0, ev1) the dispatch itself
+// records its start and end (hipExtLaunchKernelGGL), which times the kernel
+// alone on the pool's stream.
+template <typename K>
+void launch_k(K kernel, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, const PoolArgs& a, int j)
+{
+    if (grid <= 0) return;
+    if (ev0)
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(BS), 0, st, ev0, ev1, 0, a, j);
+    else
+        kernel<<<grid, BS, 0, st>>>(a, j);
+}
+
+void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
     const PoolArgs& a = p->a;
     hipStream_t st = p->stream;
     const bool mixed = p->precision == GCR_MIXED;
-    if (p->dim == 2) {
+    if (which == 3) {
+        launch_k(k_finalize, p->capR, st, ev0, ev1, a, j);
+    } else if (p->dim == 2) {
         if (which == 0)
-            mixed ? k_stage<2, float, 0><<<p->nblk, BS, 0, st>>>(a, j)
-                  : k_stage<2, double, 0><<<p->nblk, BS, 0, st>>>(a, j);
-        else if (which == 1) {
-            mixed ? k_stage<2, float, 1><<<p->nblk, BS, 0, st>>>(a, j)
-                  : k_stage<2, double, 1><<<p->nblk, BS, 0, st>>>(a, j);
-            k_finalize<<<p->capR, BS, 0, st>>>(a, j);
-        } else
-            k_update<2><<<p->nblk, BS, 0, st>>>(a, j);
+            mixed ? launch_k(k_stage_tile2d<float, 0>, p->ntile, st, ev0, ev1, a, j)
+                  : launch_k(k_stage_tile2d<double, 0>, p->ntile, st, ev0, ev1, a, j);
+        else if (which == 1)
+            mixed ? launch_k(k_stage_tile2d<float, 1>, p->ntile, st, ev0, ev1, a, j)
+                  : launch_k(k_stage_tile2d<double, 1>, p->ntile, st, ev0, ev1, a, j);
+        else
+            launch_k(k_update<2>, p->nblk, st, ev0, ev1, a, j);
     } else {
         if (which == 0)
-            mixed ? k_stage<3, float, 0><<<p->nblk, BS, 0, st>>>(a, j)
-                  : k_stage<3, double, 0><<<p->nblk, BS, 0, st>>>(a, j);
-        else if (which == 1) {
-            mixed ? k_stage<3, float, 1><<<p->nblk, BS, 0, st>>>(a, j)
-                  : k_stage<3, double, 1><<<p->nblk, BS, 0, st>>>(a, j);
-            k_finalize<<<p->capR, BS, 0, st>>>(a, j);
-        } else
-            k_update<3><<<p->nblk, BS, 0, st>>>(a, j);
+            mixed ? launch_k(k_stage<3, float, 0>, p->nblk, st, ev0, ev1, a, j)
+                  : launch_k(k_stage<3, double, 0>, p->nblk, st, ev0, ev1, a, j);
+        else if (which == 1)
+            mixed ? launch_k(k_stage<3, float, 1>, p->nblk, st, ev0, ev1, a, j)
+                  : launch_k(k_stage<3, double, 1>, p->nblk, st, ev0, ev1, a, j);
+        else
+            launch_k(k_update<3>, p->nblk, st, ev0, ev1, a, j);
     }
 }
 
@@ -255,6 +309,7 @@ void launch_batch(gcr_pool* p, int K)
         if (j % K == 0) launch_rebuild(p, j);
         launch_stage(p, 0, j);
         launch_stage(p, 1, j);
+        launch_stage(p, 3, j);
         launch_stage(p, 2, j);
     }
 }
@@ -435,6 +490,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->xi_dev.assign(capR, nullptr);
     p->capBlk = capB / BS + capR + 1;
     p->capChk = capC / SCAN_CHUNK + capR + 1;
+    p->capTile = dim == 2 ? capC / 2 + 4 * (int64_t)capR + 1 : 1;
     int rc = GCR_OK;
     auto A = [&](int r) {
         if (r != GCR_OK && rc == GCR_OK) rc = r;
@@ -462,7 +518,8 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CCOUNT, capC + 1));
     A(dalloc(&p->CSTART, capC + 1));
     A(dalloc(&p->CHKSUM, p->capChk));
-    A(dalloc(&p->BMAX, p->capBlk));
+    A(dalloc(&p->BMAX, std::max(p->capBlk, p->capTile)));
+    A(dalloc(&p->TILES, p->capTile * 4));
     if (rc == GCR_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(GCR_ERR_HIP, "stream creation failed");
     if (rc == GCR_OK && hipMemset(p->CCOUNT, 0, sizeof(int32_t) * (capC + 1)) != hipSuccess)
@@ -504,6 +561,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.CSTART = p->CSTART;
     a.CHKSUM = p->CHKSUM;
     a.BMAX = p->BMAX;
+    a.TILES = p->TILES;
     a.KEYOFF = p->KEYOFF;
     a.PAIRS = p->PAIRS;
     a.Xn = p->Xn;
@@ -833,8 +891,8 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
     launch_rebuild(p, -1);
     const bool mixed = p->precision == GCR_MIXED;
     if (p->dim == 2)
-        mixed ? k_eval<2, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
-              : k_eval<2, double><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid);
+        mixed ? k_stage_tile2d<float, 2><<<p->ntile, BS, 0, p->stream>>>(p->a, rid)
+              : k_stage_tile2d<double, 2><<<p->ntile, BS, 0, p->stream>>>(p->a, rid);
     else
         mixed ? k_eval<3, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
               : k_eval<3, double><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid);
@@ -852,34 +910,38 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
     if (!p || !ms_out || steps < 1) return fail(GCR_ERR_ARG, "bad arguments");
     int rc;
     if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
-    hipEvent_t ev[5];
+    // The step sequence of the batch graphs, launched directly; the three
+    // per-step kernels record start/end events in their own dispatch packets
+    // (hipExtLaunchKernelGGL) on the pool's stream; the re-sort (several
+    // kernels) is bracketed by stream events.
+    hipEvent_t ev[8];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     double acc[4] = {0, 0, 0, 0};
     int nreb = 0;
+    float t;
     launch_begin(p);
     for (int j = 0; j < steps; ++j) {
-        HIPCHK(hipEventRecord(ev[0], p->stream));
-        if (j % p->K == 0) {
+        const bool withr = (j % p->K == 0);
+        if (withr) {
+            HIPCHK(hipEventRecord(ev[6], p->stream));
             launch_rebuild(p, j);
+            HIPCHK(hipEventRecord(ev[7], p->stream));
+        }
+        launch_stage(p, 0, j, ev[0], ev[1]);
+        launch_stage(p, 1, j, ev[2], ev[3]);
+        launch_stage(p, 3, j);
+        launch_stage(p, 2, j, ev[4], ev[5]);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(p->stream));
+        if (withr) {
+            HIPCHK(hipEventElapsedTime(&t, ev[6], ev[7]));
+            acc[3] += t;
             ++nreb;
         }
-        HIPCHK(hipEventRecord(ev[1], p->stream));
-        launch_stage(p, 0, j);
-        HIPCHK(hipEventRecord(ev[2], p->stream));
-        launch_stage(p, 1, j);
-        HIPCHK(hipEventRecord(ev[3], p->stream));
-        launch_stage(p, 2, j);
-        HIPCHK(hipEventRecord(ev[4], p->stream));
-        HIPCHK(hipEventSynchronize(ev[4]));
-        float t;
-        HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2]));
-        acc[0] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3]));
-        acc[1] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[3], ev[4]));
-        acc[2] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
-        acc[3] += t;
+        for (int k = 0; k < 3; ++k) {
+            HIPCHK(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+            acc[k] += t;
+        }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     ms_out[0] = acc[0] / steps;
@@ -894,6 +956,15 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
     return GCR_OK;
 }
 
+int gcr_pool_set_tile(gcr_pool* p, int32_t side)
+{
+    if (!p) return fail(GCR_ERR_ARG, "null pool");
+    if (side < 0 || side > TILE_MAX_ROWS) return fail(GCR_ERR_ARG, "tile side must be in [0, 16]");
+    p->tile_side = side;
+    p->tables_dirty = true;
+    return GCR_OK;
+}
+
 int gcr_pool_destroy(gcr_pool* p)
 {
     if (!p) return GCR_OK;
@@ -905,7 +976,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->dSs, p->X, p->X1, p->F0, p->FT, p->XI, p->T, p->Xn, p->Tn, p->Xs, p->Ts,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
-                    p->CHKSUM, p->BMAX};
+                    p->CHKSUM, p->BMAX, p->TILES};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

@@ -31,7 +31,10 @@ struct RepParams {
     uint32_t flags;
     int32_t nblk;       // bead blocks of this replica
     int32_t blk_off;    // first bead block
-    int32_t pad0;
+    int32_t red_off;    // first block of the stage-1 max reduction
+    int32_t red_n;      // blocks of the stage-1 max reduction
+    int32_t tile_x, tile_y;  // 2D cell tile of the LDS force kernel
+    int32_t tile_off, ntile; // first tile, tiles of this replica
     double L, halfL;
     double h, inv_h;    // cell edge and its inverse
     double skin_half;   // allowed excursion of a bead outside its cell box
@@ -75,13 +78,15 @@ struct PoolArgs {
     // replica tables
     const RepParams* P;
     RepState* S;
-    double* BMAX;  // [bead block] max |F0+F1|^2 of the block (stage 1)
+    double* BMAX;  // [block] max |F0+F1|^2 of the block (stage 1)
+    const int32_t* TILES;  // [tile][4] (replica, cx0, cy0, 0) for the 2D tile kernels
+    int32_t ntile;
     // per-slot state
     double* X;     // [slot][dim] positions
     double* X1;    // [slot][dim] predictor positions
     double* F0;    // [slot][dim] stage-0 forces
     double* FT;    // [slot][dim] F0 + F1
-    double* XI;    // [slot][2*dim] noise of the running step
+    double* XI;    // [orig_global][2*dim] noise of the next / running step
     double* T;     // [slot][dim] propulsion direction (ABP / diffusing state)
     int32_t* META; // [slot][2] (orig_global, chain_global)
     int32_t* LINK; // [slot][4] (prev2, prev1, next1, next2) slots, -1 if none

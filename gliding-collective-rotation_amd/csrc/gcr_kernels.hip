@@ -7,8 +7,7 @@
 //   k_finalize  per-replica max force -> dt, dt_used (SRK.py:323-353)
 //   k_update    x <- x + dt_used/(2r) F + sqrt(2 D dt_used) xi, rot. diffusion
 // Every `rebuild_every` steps the beads are re-sorted by cell (counting sort):
-//   k_zero_counts, k_bin, k_scan_{chunks,reps,final}, k_scatter,
-//   k_cellsort, k_gather, k_links
+//   k_bin, k_scan_{chunks,reps,final}, k_scatter, k_gather, k_links
 // Force kernels write one output per bead and use no atomics on forces, so a
 // replica's result does not depend on scheduling or on other replicas.
 #include <hip/hip_runtime.h>
@@ -107,6 +106,34 @@ __device__ __forceinline__ void bead_noise(uint64_t seed, uint32_t orig, int64_t
             xi[5] = 0.0;
         }
     }
+}
+
+// Noise of bead `orig_global` for the replica's step with call-local index
+// `steps` and global index `step_global`, into XI[orig_global] (indexed by
+// original bead, so cell re-sorts never move it): drawn in-kernel, or copied
+// from the injected array (zeros past its end: that step never runs).
+template <int DIM>
+__device__ __forceinline__ void make_noise(const PoolArgs& a, const RepParams& P, int orig_global,
+                                           int64_t steps, int64_t step_global)
+{
+    double xi[2 * DIM];
+    const int orig_local = orig_global - P.bead_off;
+    if (P.flags & GCR_F_INJECT) {
+        if (steps < P.xi_steps) {
+            const double* src = P.xi_inject + ((size_t)steps * P.n + orig_local) * (2 * DIM);
+#pragma unroll
+            for (int i = 0; i < 2 * DIM; ++i) xi[i] = src[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2 * DIM; ++i) xi[i] = 0.0;
+        }
+    } else {
+        bead_noise<DIM>(P.seed, (uint32_t)orig_local, step_global, (P.flags & GCR_F_ROTDIFF) != 0,
+                        xi);
+    }
+    double* dst = a.XI + (size_t)orig_global * 2 * DIM;
+#pragma unroll
+    for (int i = 0; i < 2 * DIM; ++i) dst[i] = xi[i];
 }
 
 // Per-block running condition of a replica for local step j of a batch.
@@ -439,9 +466,63 @@ __device__ __forceinline__ double block_max(double v)
     return m;
 }
 
-// Stage 0 (STAGE == 0): forces at x, predictor x1 = x + dt/r F0 + sqrt(2 D dt) xi.
-// Stage 1 (STAGE == 1): forces at x1 added to F0 (the reference accumulates
-// into the same array, SRK.py:153-160), max |F|^2, last block finalises dt.
+// Self-propulsion (forces.py:206-223), per-chain (v0, cos a, sin a).
+template <int DIM>
+__device__ __forceinline__ void propel(const PoolArgs& a, int chain, const double* tang, double* acc)
+{
+    const double* cs = a.CHAIN_CS + 3 * chain;
+    if (DIM == 2) {
+        acc[0] += cs[0] * (cs[1] * tang[0] - cs[2] * tang[1]);
+        acc[1] += cs[0] * (cs[2] * tang[0] + cs[1] * tang[1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) acc[i] += cs[0] * tang[i];
+    }
+}
+
+// What a force stage does with the total force on slot s:
+// STAGE 0: with this step's noise (the corrector reuses the same xi,
+//          SRK.py:379-386), store F0 and the predictor
+//          x1 = (x + dt/r F0 + sqrt(2 D dt) xi) mod L (SRK.py:355-388);
+// STAGE 1: store F = F0 + F1 and return |F|^2 for the max-force reduction;
+// STAGE 2: (force evaluation only) store the force.
+template <int DIM, int STAGE>
+__device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams& P, RepState* S,
+                                              int s, int orig_global, const double* xs,
+                                              const double* acc)
+{
+    if (STAGE == 0) {
+        // this step's noise, drawn by the previous update (or the batch start)
+        double xi[2 * DIM];
+        load_vec<DIM>(a.XI + (size_t)orig_global * 2 * DIM, 0, xi);
+        store_vec<DIM>(a.F0, s, acc);
+        const double dt = S->dt;
+        double dt_eff = dt * 1.0;
+        const double s2 = sqrt(2.0 * dt * P.D);
+        dt_eff /= P.radius;
+        double x1[DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            double tmp = xs[i] + dt_eff * acc[i];
+            if (s2 != 0.0) tmp += s2 * xi[i];
+            x1[i] = pymod(tmp, P.L);
+        }
+        store_vec<DIM>(a.X1, s, x1);
+        if (outside_cell<DIM>(x1, a.CELL[s], P) > P.skin_half) atomicOr(&S->violation, 1);
+        return 0.0;
+    } else {
+        store_vec<DIM>(a.FT, s, acc);
+        double q = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) q += acc[i] * acc[i];
+        return q;
+    }
+}
+
+// Stage kernels over bead blocks (any dimension; the 2D production path is
+// k_stage_tile2d). STAGE 0: forces at x -> x1, F0, noise. STAGE 1: forces at
+// x1 added to F0 (the reference accumulates into the same array,
+// SRK.py:153-160) and the block's max |F|^2 for k_finalize.
 template <int DIM, typename FT, int STAGE>
 __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
 {
@@ -472,48 +553,9 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
             pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
         if (P.flags & GCR_F_PROPEL) {
             if (!tang_set) load_vec<DIM>(a.T, s, tang);
-            const double* cs = a.CHAIN_CS + 3 * me.y;
-            if (DIM == 2) {
-                acc[0] += cs[0] * (cs[1] * tang[0] - cs[2] * tang[1]);
-                acc[1] += cs[0] * (cs[2] * tang[0] + cs[1] * tang[1]);
-            } else {
-#pragma unroll
-                for (int i = 0; i < DIM; ++i) acc[i] += cs[0] * tang[i];
-            }
+            propel<DIM>(a, me.y, tang, acc);
         }
-        if (STAGE == 0) {
-            // noise of this step, kept for the corrector (same xi, SRK.py:379-386)
-            double xi[2 * DIM];
-            const int orig_local = me.x - P.bead_off;
-            if (P.flags & GCR_F_INJECT) {
-                const double* src = P.xi_inject + ((size_t)S->steps * P.n + orig_local) * (2 * DIM);
-#pragma unroll
-                for (int i = 0; i < 2 * DIM; ++i) xi[i] = src[i];
-            } else {
-                bead_noise<DIM>(P.seed, (uint32_t)orig_local, S->step_global,
-                                (P.flags & GCR_F_ROTDIFF) != 0, xi);
-            }
-#pragma unroll
-            for (int i = 0; i < 2 * DIM; ++i) a.XI[(size_t)s * 2 * DIM + i] = xi[i];
-            store_vec<DIM>(a.F0, s, acc);
-            const double dt = S->dt;
-            double dt_eff = dt * 1.0;
-            const double s2 = sqrt(2.0 * dt * P.D);
-            dt_eff /= P.radius;
-            double x1[DIM];
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) {
-                double tmp = xs[i] + dt_eff * acc[i];
-                if (s2 != 0.0) tmp += s2 * xi[i];
-                x1[i] = pymod(tmp, P.L);
-            }
-            store_vec<DIM>(a.X1, s, x1);
-            if (outside_cell<DIM>(x1, a.CELL[s], P) > P.skin_half) atomicOr(&S->violation, 1);
-        } else {
-            store_vec<DIM>(a.FT, s, acc);
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) fsq += acc[i] * acc[i];
-        }
+        fsq = stage_store<DIM, STAGE>(a, P, S, s, me.x, xs, acc);
     }
     if (STAGE == 1) {
         // per-block max |F0+F1|^2; k_finalize reduces it per replica
@@ -531,8 +573,8 @@ __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
     if (!step_runs(S, j)) return;
     const RepParams& P = a.P[r];
     double m2 = 0.0;
-    for (int i = threadIdx.x; i < P.nblk; i += BS) {
-        double v = a.BMAX[P.blk_off + i];
+    for (int i = threadIdx.x; i < P.red_n; i += BS) {
+        double v = a.BMAX[P.red_off + i];
         m2 = v > m2 ? v : m2;
     }
     m2 = block_max(m2);
@@ -568,8 +610,9 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
     double x[DIM], f[DIM], xi[2 * DIM];
     load_vec<DIM>(a.X, s, x);
     load_vec<DIM>(a.FT, s, f);
+    const int orig = a.META[2 * s];
 #pragma unroll
-    for (int i = 0; i < 2 * DIM; ++i) xi[i] = a.XI[(size_t)s * 2 * DIM + i];
+    for (int i = 0; i < 2 * DIM; ++i) xi[i] = a.XI[(size_t)orig * 2 * DIM + i];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         double tmp = x[i] + dt_eff * f[i];
@@ -599,6 +642,9 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
         for (int i = 0; i < DIM; ++i) t[i] /= mag;
         store_vec<DIM>(a.T, s, t);
     }
+    // the next step's noise (finalize already advanced the step counters);
+    // computed here, where the update leaves the ALUs idle
+    make_noise<DIM>(a, P, orig, S->steps, S->step_global);
 }
 
 // One force evaluation without an update (parity checks of the force path):
@@ -625,16 +671,7 @@ __global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
     if (P.flags & GCR_F_WCA)
         pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
     if (!tang_set) load_vec<DIM>(a.T, s, tang);
-    if (P.flags & GCR_F_PROPEL) {
-        const double* cs = a.CHAIN_CS + 3 * me.y;
-        if (DIM == 2) {
-            acc[0] += cs[0] * (cs[1] * tang[0] - cs[2] * tang[1]);
-            acc[1] += cs[0] * (cs[2] * tang[0] + cs[1] * tang[1]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) acc[i] += cs[0] * tang[i];
-        }
-    }
+    if (P.flags & GCR_F_PROPEL) propel<DIM>(a, me.y, tang, acc);
     store_vec<DIM>(a.FT, s, acc);
     store_vec<DIM>(a.Tn, s, tang);
 }
@@ -764,32 +801,10 @@ __global__ void __launch_bounds__(BS) k_scatter(PoolArgs a, int j)
     *reinterpret_cast<int2*>(a.PAIRS + 2 * ns) = make_int2(a.META[2 * s], s);
 }
 
-// Order each cell's beads by original index so the layout (and with it the
-// pair summation order) is a function of the positions alone.
-__global__ void __launch_bounds__(BS) k_cellsort(PoolArgs a, int j)
-{
-    const int r = a.chk_rep[blockIdx.x];
-    const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
-    const int cend = P.cell_off + P.ncell;
-    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-        int c = a.chk_cell0[blockIdx.x] + k * BS + threadIdx.x;
-        if (c >= cend) break;
-        int t0 = a.CSTART[c], t1 = a.CSTART[c + 1];
-        for (int i = t0 + 1; i < t1; ++i) {
-            int2 v = *reinterpret_cast<const int2*>(a.PAIRS + 2 * i);
-            int q = i - 1;
-            while (q >= t0 && a.PAIRS[2 * q] > v.x) {
-                a.PAIRS[2 * (q + 1)] = a.PAIRS[2 * q];
-                a.PAIRS[2 * (q + 1) + 1] = a.PAIRS[2 * q + 1];
-                --q;
-            }
-            a.PAIRS[2 * (q + 1)] = v.x;
-            a.PAIRS[2 * (q + 1) + 1] = v.y;
-        }
-    }
-}
-
+// Move each bead to its new slot. The scatter left a cell's beads in atomic
+// (arbitrary) order; each bead's final place inside its cell is its rank by
+// original index, counted over the cell, so the layout (and with it the pair
+// summation order) depends on the positions alone.
 template <int DIM>
 __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
 {
@@ -798,19 +813,24 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
     if (!rebuild_runs(a.S + r, j)) return;
     const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (ns >= P.bead_off + P.n) return;
-    int2 pr = *reinterpret_cast<const int2*>(a.PAIRS + 2 * ns);
-    int so = pr.y;
+    const int2 pr = *reinterpret_cast<const int2*>(a.PAIRS + 2 * ns);
+    const int so = pr.y;
+    const int key = a.KEYOFF[2 * so];
+    const int t0 = a.CSTART[P.cell_off + key], t1 = a.CSTART[P.cell_off + key + 1];
+    int rank = 0;
+    for (int t = t0; t < t1; ++t) rank += a.PAIRS[2 * t] < pr.x;
+    const int d = t0 + rank;
     double x[DIM];
     load_vec<DIM>(a.X, so, x);
-    store_vec<DIM>(a.Xn, ns, x);
+    store_vec<DIM>(a.Xn, d, x);
     if (with_t) {
         double t[DIM];
         load_vec<DIM>(a.T, so, t);
-        store_vec<DIM>(a.Tn, ns, t);
+        store_vec<DIM>(a.Tn, d, t);
     }
-    *reinterpret_cast<int2*>(a.METAn + 2 * ns) = *reinterpret_cast<const int2*>(a.META + 2 * so);
-    a.CELL[ns] = a.KEYOFF[2 * so];
-    a.SLOT[pr.x] = ns;
+    *reinterpret_cast<int2*>(a.METAn + 2 * d) = *reinterpret_cast<const int2*>(a.META + 2 * so);
+    a.CELL[d] = key;
+    a.SLOT[pr.x] = d;
 }
 
 template <int DIM>
@@ -859,6 +879,9 @@ __global__ void __launch_bounds__(BS) k_batch_begin(PoolArgs a, int with_t)
         a.Ss[r] = *S;
     }
     if (s >= P.bead_off + P.n) return;
+    // the first step of the batch needs its noise (after a begin or a replay
+    // nothing else has drawn it)
+    make_noise<DIM>(a, P, a.META[2 * s], S->steps, S->step_global);
     double x[DIM];
     load_vec<DIM>(a.X, s, x);
     store_vec<DIM>(a.Xs, s, x);

@@ -91,6 +91,10 @@ class Pool:
     def set_schedule(self, steps_per_batch, rebuild_every):
         check(self._L.gcr_pool_set_schedule(self.h, int(steps_per_batch), int(rebuild_every)))
 
+    def set_tile(self, side):
+        """Tile edge of the 2D LDS force kernel (0: automatic)."""
+        check(self._L.gcr_pool_set_tile(self.h, int(side)))
+
     def add(self, phys, pos, tangent, chain_id, rest_lengths, v_tan_angle=None,
             v0_per_chain=None):
         pos = f64(pos)

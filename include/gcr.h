@@ -121,10 +121,15 @@ int gcr_replica_status(gcr_pool* p, int32_t rid, double out[8]);
 int gcr_replica_download(gcr_pool* p, int32_t rid, double* pos, double* pos1, double* tangent,
                          double* forces);
 int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* tangent);
-/* Per-kernel timing: runs `steps` steps of every armed replica as individual
- * launches with HIP events on the pool's stream; ms_out[k] = average duration
- * of kernel k in [stage0, stage1, update, rebuild(all kernels)]. */
+/* Per-kernel timing: runs `steps` steps of every armed replica with the
+ * kernels of the batch graphs launched directly; stage0, stage1 and update
+ * record start/end HIP events in their own dispatch (hipExtLaunchKernelGGL)
+ * on the pool's stream. ms_out = average duration of [stage0, stage1, update,
+ * cell re-sort (all its kernels, per re-sort)]. */
 int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4]);
+/* Tile edge (cells) of the 2D LDS force kernel; 0 picks it per replica from
+ * the cell occupancy (about 256 owned beads per workgroup). At most 16. */
+int gcr_pool_set_tile(gcr_pool* p, int32_t side);
 int gcr_pool_destroy(gcr_pool* p);
 
 /* ------------------------------------------------- initial configuration */

@@ -1,0 +1,55 @@
+"""Ensemble and serving on the GPU: batched runs and the continuous-batching
+server give each simulation exactly the result it gets alone."""
+import numpy as np
+import pytest
+
+from gcr_amd import engine as E
+from gcr_amd import ensemble as ENS
+from gcr_amd import serving as SV
+from gcr_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_ensemble_runs_every_replica():
+    ws = [W.chains(f"e{i}", 2, 20 + 15 * i, 5 + 4 * i, phi=0.1 + 0.08 * i, vtan=0.1 * i - 0.2,
+                   v0=0.5 + 0.3 * i, T=1e-3, steps=130, seed=40 + i) for i in range(5)]
+    sec, bs, pool, rids = ENS.run_batched(ws, 130, steps_per_batch=32, rebuild_every=8)
+    assert bs == sum(w.n for w in ws) * 130
+    for w, r in zip(ws, rids):
+        st = pool.status(r)
+        assert st["steps"] == 130 and not st["running"]
+        pos = pool.download(r, pos1=False, tangent=False, forces=False)["pos"]
+        assert np.all(np.isfinite(pos)) and np.all((pos >= 0) & (pos <= w.phys.L))
+    pool.close()
+
+
+def small_stream():
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(6):
+        dim = 2 if i % 2 == 0 else 3
+        abp = i % 3 == 0
+        w = SV.make_request_workload(rng, dim, int(rng.integers(200, 1500)), abp, seed=900 + i)
+        reqs.append(SV.Request(i, 0.0, w, int(rng.integers(40, 200))))
+    return reqs
+
+
+def test_serving_results_equal_alone():
+    reqs = small_stream()
+    srv = SV.Server(hbm_budget_bytes=64 << 20, steps_per_batch=32, rebuild_every=8,
+                    max_replicas=8)
+    done = srv.serve(reqs, keep_results=True, realtime=False)
+    srv.close()
+    assert len(done) == len(reqs)
+    for r in done:
+        assert r.result["status"]["steps"] == r.steps
+        alone = SV.Request(r.rid, 0.0, r.work, r.steps)
+        srv1 = SV.Server(hbm_budget_bytes=64 << 20, steps_per_batch=32, rebuild_every=8,
+                         max_replicas=2)
+        (a,) = srv1.serve([alone], keep_results=True, realtime=False)
+        srv1.close()
+        assert np.array_equal(a.result["pos"], r.result["pos"]), r.rid
+        assert np.array_equal(a.result["tangent"], r.result["tangent"]), r.rid
+    rep = SV.report(done)
+    assert rep["requests"] == 6 and rep["p99_latency_s"] >= rep["p50_latency_s"] > 0
