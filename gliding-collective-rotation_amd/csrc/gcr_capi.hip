@@ -22,6 +22,10 @@ namespace {
 
 thread_local std::string g_err;
 
+// RepState::frozen values: 0 running, 1 held by the host (replay or
+// diagnostics), 2 marked for restore, FROZEN_FAILED stopped for good
+constexpr int FROZEN_FAILED = 3;
+
 int fail(int code, const std::string& msg)
 {
     g_err = msg;
@@ -373,18 +377,16 @@ int replay(gcr_pool* p)
     k_restore_state<<<(p->capR + 63) / 64, 64, 0, p->stream>>>(p->a);
     HIPCHK(hipGraphLaunch(p->g_safe, p->stream));
     if ((rc = pull_states(p))) return rc;
-    bool again = false;
     for (int r : which) {
         p->replays[r] += 1;
-        if (p->S[r].violation) again = true;
+        // even a re-sort before every step did not keep this replica's beads
+        // within half a skin of their cells: its pair set may be incomplete,
+        // so it stops here, marked failed, and the others go on
+        if (p->S[r].violation) p->S[r].frozen = FROZEN_FAILED;
     }
     for (int r = 0; r < p->capR; ++r)
         if (p->live[r] && p->S[r].frozen == 1) p->S[r].frozen = 0;
-    if ((rc = push_states(p))) return rc;
-    if (again)
-        return fail(GCR_ERR_STATE,
-                    "a bead moved more than skin/2 within one step; increase the skin");
-    return GCR_OK;
+    return push_states(p);
 }
 
 int with_only(gcr_pool* p, int rid, bool on)
@@ -393,7 +395,7 @@ int with_only(gcr_pool* p, int rid, bool on)
     int rc = pull_states(p);
     if (rc) return rc;
     for (int r = 0; r < p->capR; ++r)
-        if (p->live[r] && r != rid) p->S[r].frozen = on ? 1 : 0;
+        if (p->live[r] && r != rid && p->S[r].frozen != FROZEN_FAILED) p->S[r].frozen = on ? 1 : 0;
     return push_states(p);
 }
 
@@ -850,7 +852,7 @@ int gcr_replica_status(gcr_pool* p, int32_t rid, double out[8])
     out[4] = running(S) ? 1.0 : 0.0;
     out[5] = (double)p->replays[rid];
     out[6] = (double)S.step_global;
-    out[7] = (double)p->P[rid].n;
+    out[7] = S.frozen == FROZEN_FAILED ? 1.0 : 0.0;
     return GCR_OK;
 }
 
@@ -1018,6 +1020,10 @@ int gcr_run(gcr_pool* h, const double tv[8], double status_out[6], const double*
     int32_t nr = 0;
     rc = gcr_pool_run(h, 0, &nr);
     if (rc) return rc;
+    if ((rc = pull_states(h))) return rc;
+    if (h->S[0].frozen == FROZEN_FAILED)
+        return fail(GCR_ERR_STATE, "a bead moved more than skin/2 within one step (pair set no "
+                                   "longer complete); create the system with a larger skin");
     if (status_out) {
         double st[8];
         if ((rc = gcr_replica_status(h, 0, st))) return rc;

@@ -371,23 +371,40 @@ struct PairLaw;
 
 template <>
 struct PairLaw<double> {
+    // 1/ds from v_rcp_f64 refined by two Newton steps (error ~1 ulp)
+    static __device__ __forceinline__ double inv(double ds)
+    {
+        double r = __builtin_amdgcn_rcp(ds);
+        double e = fma(-ds, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-ds, r, 1.0);
+        return fma(r, e, r);
+    }
+    static __device__ __forceinline__ double coeff(double ds, double p0, double p1)
+    {
+        double iv = inv(ds);
+        double i2 = iv * iv;
+        double i3 = i2 * iv;
+        return (i2 * i2) * (p0 - p1 * i3);
+    }
     static __device__ __forceinline__ double coeff(double ds, const RepParams& P)
     {
-        double inv = 1.0 / ds;
-        double i2 = inv * inv;
-        double i3 = i2 * inv;
-        return (i2 * i2) * (P.wca[0] - P.wca[1] * i3);
+        return coeff(ds, P.wca[0], P.wca[1]);
     }
 };
 
 template <>
 struct PairLaw<float> {
+    static __device__ __forceinline__ float coeff(float ds, float p0, float p1)
+    {
+        float iv = __builtin_amdgcn_rcpf(ds);
+        float i2 = iv * iv;
+        float i3 = i2 * iv;
+        return (i2 * i2) * (p0 - p1 * i3);
+    }
     static __device__ __forceinline__ float coeff(float ds, const RepParams& P)
     {
-        float inv = __builtin_amdgcn_rcpf(ds);
-        float i2 = inv * inv;
-        float i3 = i2 * inv;
-        return (i2 * i2) * ((float)P.wca[0] - (float)P.wca[1] * i3);
+        return coeff(ds, (float)P.wca[0], (float)P.wca[1]);
     }
 };
 

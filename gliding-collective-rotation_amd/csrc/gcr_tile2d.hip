@@ -12,58 +12,99 @@
 // A tile whose beads do not fit in LDS (or a box of fewer than 3 cells per
 // side) sums its pairs from global memory with pair_sum instead.
 #define TILE_MAX_HALO 400    // (tile_x + 2) * (tile_y + 2) cells
-#define TILE_MAX_BEADS 1024  // beads of a tile with its halo held in LDS
+#define TILE_MAX_BEADS 1024  // beads of a tile with its halo held in LDS (16-24 B each)
 #define TILE_MAX_ROWS 16
 
 namespace gcr {
 
+// LDS record of a staged bead: position relative to the centre of its cell
+// and (original index, chain). FP32: one 16-byte record; FP64: 24 bytes.
 template <typename FT>
-struct LPos {
+struct alignas(16) LBead {
     FT x, y;
+    int orig, chain;
 };
 
-// Pair forces of one bead from the LDS tile: cell c0 = hy * W + hx is its
-// cell in the tile frame, (xi, yi) its position relative to that cell's centre.
+// Scalars of the pair law, held in registers for the whole loop.
 template <typename FT>
-__device__ __forceinline__ void tile_pair_sum(const LPos<FT>* lp, const int2* lm, const int* cst, int W,
-                                              int c0, FT xi, FT yi, int2 me, const RepParams& P,
-                                              double* acc)
+struct PairConsts {
+    FT h, rc2, p0, p1, shift;
+    int nn;
+    bool shifted;
+};
+
+template <typename FT>
+__device__ __forceinline__ PairConsts<FT> pair_consts(const RepParams& P)
 {
-    const FT h = (FT)P.h;
-    const FT rc2 = (FT)P.rc2;
-    const bool shifted = P.wca[2] != 0.0;
-    const FT shift = (FT)P.wca[2];
-    const int nn = P.n_nearby;
+    PairConsts<FT> k;
+    k.h = (FT)P.h;
+    k.rc2 = (FT)P.rc2;
+    k.p0 = (FT)P.wca[0];
+    k.p1 = (FT)P.wca[1];
+    k.shift = (FT)P.wca[2];
+    k.nn = P.n_nearby;
+    k.shifted = P.wca[2] != 0.0;
+    return k;
+}
+
+// Pair forces of one bead from the LDS tile: cell c0 = hy * W + hx is its
+// cell in the tile frame, (xi, yi) its position relative to that cell's
+// centre. Branch-free body: every candidate of the 3 x 3 cells is evaluated
+// with a mask (cutoff, self, same-chain exclusion of updates.py:645-650), so
+// all lanes follow one path and the LDS reads pipeline. The forces of one
+// neighbour cell are summed in FT and added to the FP64 accumulator.
+template <typename FT>
+__device__ __forceinline__ void tile_pair_sum(const LBead<FT>* lb, const int* cst, int W, int c0, FT xi,
+                                              FT yi, int2 me, const PairConsts<FT>& k, double* acc)
+{
     for (int dy = -1; dy <= 1; ++dy) {
-        const FT oy = (FT)dy * h - yi;
+        const FT oy = (FT)dy * k.h - yi;
         for (int dx = -1; dx <= 1; ++dx) {
-            const FT ox = (FT)dx * h - xi;
+            const FT ox = (FT)dx * k.h - xi;
             const int c = c0 + dy * W + dx;
             const int t1 = cst[c + 1];
-            for (int t = cst[c]; t < t1; ++t) {
-                const LPos<FT> q = lp[t];
-                FT sx = q.x + ox, sy = q.y + oy;
-                FT ds = sx * sx + sy * sy;
-                if (!(ds <= rc2 && ds > (FT)0)) continue;
-                const int2 mj = lm[t];
-                if (mj.y == me.y) {  // updates.py:645-650
-                    if (nn < 0) continue;
-                    const int d = mj.x - me.x;
-                    if ((d < 0 ? -d : d) <= nn) continue;
+            FT fx = (FT)0, fy = (FT)0;
+            if (!k.shifted) {
+#pragma unroll 4
+                for (int t = cst[c]; t < t1; ++t) {
+                    const LBead<FT> q = lb[t];
+                    const FT sx = q.x + ox, sy = q.y + oy;
+                    const FT ds = sx * sx + sy * sy;
+                    const int d = q.orig - me.x;
+                    // bitwise, not short-circuit: keeps the body free of branches
+                    const int excl = (q.chain == me.y) & ((k.nn < 0) | ((d < 0 ? -d : d) <= k.nn));
+                    const bool use = (ds <= k.rc2) & (ds > (FT)0) & !excl;
+                    const FT cf = PairLaw<FT>::coeff(use ? ds : (FT)1, k.p0, k.p1);
+                    const FT w = use ? cf : (FT)0;
+                    fx += sx * w;
+                    fy += sy * w;
                 }
-                if (shifted) {  // WCA_shift, forces.py:123-131
+            } else {
+                for (int t = cst[c]; t < t1; ++t) {
+                    const LBead<FT> q = lb[t];
+                    FT sx = q.x + ox, sy = q.y + oy;
+                    FT ds = sx * sx + sy * sy;
+                    if (!(ds <= k.rc2 && ds > (FT)0)) continue;
+                    if (q.chain == me.y) {
+                        if (k.nn < 0) continue;
+                        const int d = q.orig - me.x;
+                        if ((d < 0 ? -d : d) <= k.nn) continue;
+                    }
+                    // WCA_shift, forces.py:123-131
                     FT mag = sqrt(ds);
-                    FT nm = mag + shift;
+                    FT nm = mag + k.shift;
                     nm = nm < (FT)0 ? (FT)0 : nm;
                     FT rs = nm / mag;
                     sx *= rs;
                     sy *= rs;
                     ds = nm * nm;
+                    const FT cf = PairLaw<FT>::coeff(ds, k.p0, k.p1);
+                    fx += sx * cf;
+                    fy += sy * cf;
                 }
-                const FT cf = PairLaw<FT>::coeff(ds, P);
-                acc[0] += (double)(sx * cf);
-                acc[1] += (double)(sy * cf);
             }
+            acc[0] += (double)fx;
+            acc[1] += (double)fy;
         }
     }
 }
@@ -91,8 +132,7 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
     __shared__ int cst[TILE_MAX_HALO + 1];  // LDS offset of each halo cell
     __shared__ int gst[TILE_MAX_HALO];      // first global slot of each halo cell
     __shared__ int own_pref[TILE_MAX_ROWS + 1];
-    __shared__ LPos<FT> lp[TILE_MAX_BEADS];
-    __shared__ int2 lm[TILE_MAX_BEADS];
+    __shared__ LBead<FT> lb[TILE_MAX_BEADS];
 
     // 1. bead counts of the halo cells -> LDS offsets (two cells per thread)
     int cnt[2], g0[2], sum = 0;
@@ -150,16 +190,19 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
             const int gy = (cy0 - 1 + lo / W + nc) % nc;
             const int g = gst[lo] + (i - cst[lo]);
             const double2 x = *reinterpret_cast<const double2*>(Xa + 2 * g);
-            LPos<FT> q;
+            const int2 m = *reinterpret_cast<const int2*>(a.META + 2 * g);
+            LBead<FT> q;
             q.x = (FT)wrap1(x.x - (gx + 0.5) * P.h, P.L, P.halfL);
             q.y = (FT)wrap1(x.y - (gy + 0.5) * P.h, P.L, P.halfL);
-            lp[i] = q;
-            lm[i] = *reinterpret_cast<const int2*>(a.META + 2 * g);
+            q.orig = m.x;
+            q.chain = m.y;
+            lb[i] = q;
         }
     }
     __syncthreads();
 
     // 3. owned beads
+    const PairConsts<FT> kc = pair_consts<FT>(P);
     const int n_own = own_pref[own_y];
     double fsq = 0.0;
     for (int o = threadIdx.x; o < n_own; o += BS) {
@@ -187,8 +230,8 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
         }
         if (P.flags & GCR_F_WCA) {
             if (in_lds) {
-                const LPos<FT> q = lp[li];
-                tile_pair_sum<FT>(lp, lm, cst, W, c0, q.x, q.y, me, P, acc);
+                const LBead<FT> q = lb[li];
+                tile_pair_sum<FT>(lb, cst, W, c0, q.x, q.y, me, kc, acc);
             } else {
                 pair_sum<2, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
             }

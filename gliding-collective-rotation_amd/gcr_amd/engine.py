@@ -137,7 +137,8 @@ class Pool:
         check(self._L.gcr_replica_status(self.h, int(rid),
                                          out.ctypes.data_as(C.POINTER(C.c_double))))
         return dict(dt=out[0], elapsed=out[1], steps=int(out[2]), max_forces_strength=out[3],
-                    running=bool(out[4]), replays=int(out[5]), step_global=int(out[6]))
+                    running=bool(out[4]), replays=int(out[5]), step_global=int(out[6]),
+                    failed=bool(out[7]))
 
     def download(self, rid, pos=True, pos1=True, tangent=True, forces=True):
         n = self.n[rid]

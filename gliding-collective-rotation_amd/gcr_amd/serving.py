@@ -42,8 +42,12 @@ class Request:
         return self.work.n
 
 
-def make_request_workload(rng, dim, n_target, abp, seed, T=0.01, phi=0.25):
-    """ABPs (chain length 1, D_r = 0.1) or chiral chains (length 5..40)."""
+def make_request_workload(rng, dim, n_target, abp, seed, T=0.01, phi=None):
+    """ABPs (chain length 1, D_r = 0.1) or chiral chains (length 5..40).
+    Default packing: area fraction 0.25 in 2D; volume fraction 0.25 for 3D
+    ABPs and 0.15 for 3D chains (random placement of long rods jams early)."""
+    if phi is None:
+        phi = 0.25 if (dim == 2 or abp) else 0.15
     if abp:
         ln, nc = 1, max(1, n_target)
     else:
@@ -55,7 +59,7 @@ def make_request_workload(rng, dim, n_target, abp, seed, T=0.01, phi=0.25):
     else:  # volume fraction of beads of diameter 1
         L = (nc * ln * (np.pi / 6) / phi) ** (1.0 / 3.0)
     return W.chains(f"req{seed}", dim, nc, ln, L=L, vtan=vt, T=T, Dr=0.1 if abp else None,
-                    dtmax=1e-3, seed=seed)
+                    dtmax=1e-3, seed=seed, min_dist_floor=0.7)
 
 
 def make_stream(seed=0, n_requests=64, rate=20.0, n_min=1000, n_max=200000, steps_min=1000,
@@ -100,6 +104,7 @@ class _Lane:
         self.used = np.zeros(3, dtype=np.int64)  # beads, cells, chains
         self.active = {}  # rid -> (request, cells)
         self.batches = 0
+        self.error = None
 
     def fits(self, req, cells):
         need = np.array([req.n, cells, req.work.phys.n_chains])
@@ -122,16 +127,20 @@ class _Lane:
         return True
 
     def step_batch(self):
-        if self.active:
-            self.pool.run(max_batches=1)
-            self.batches += 1
+        self.error = None
+        try:
+            if self.active:
+                self.pool.run(max_batches=1)
+                self.batches += 1
+        except Exception as e:  # re-raised by the serving loop
+            self.error = e
 
     def retire(self, now, keep_results):
         done = []
         for rid, (req, cells) in list(self.active.items()):
             st = self.pool.status(rid)
             if not st["running"]:
-                if keep_results:
+                if keep_results and not st["failed"]:
                     req.result = self.pool.download(rid, pos1=False, forces=False)
                 req.result["status"] = st
                 req.finished = now
@@ -195,6 +204,9 @@ class Server:
                 th.start()
             for th in threads:
                 th.join()
+            for l in busy:
+                if l.error is not None:
+                    raise l.error
             t = now()
             for l in busy:
                 with lock:
@@ -203,10 +215,12 @@ class Server:
 
 
 def report(finished):
-    lat = np.array([r.finished - r.arrival for r in finished])
+    failed = [r for r in finished if r.result["status"]["failed"]]
+    ok = [r for r in finished if not r.result["status"]["failed"]]
+    lat = np.array([r.finished - r.arrival for r in ok])
     work = sum(r.n * r.result["status"]["steps"] for r in finished)
     span = max(r.finished for r in finished) - min(r.arrival for r in finished)
-    return dict(requests=len(finished), bead_steps=int(work), makespan_s=span,
+    return dict(requests=len(finished), failed=len(failed), bead_steps=int(work), makespan_s=span,
                 bead_steps_per_s=work / span, p50_latency_s=float(np.percentile(lat, 50)),
                 p99_latency_s=float(np.percentile(lat, 99)),
-                mean_queue_s=float(np.mean([r.admitted - r.arrival for r in finished])))
+                mean_queue_s=float(np.mean([r.admitted - r.arrival for r in ok])))

@@ -114,9 +114,14 @@ int gcr_replica_upload(gcr_pool* p, int32_t rid, const double* pos, const double
 int gcr_replica_begin(gcr_pool* p, int32_t rid, const double time_vals[8], const double* xi,
                       int64_t xi_steps);
 /* Advances every armed replica by up to max_batches batches (<= 0: until all
- * are done). n_running receives how many are still running. */
+ * are done). n_running receives how many are still running. A batch in which
+ * some replica's beads left their cells by more than half the skin is
+ * replayed for that replica with a re-sort before every step. */
 int gcr_pool_run(gcr_pool* p, int32_t max_batches, int32_t* n_running);
-/* [dt, elapsed, steps, max_forces_strength, running, replays, step_global, n] */
+/* [dt, elapsed, steps, max_forces_strength, running, replays, step_global,
+ *  failed]. failed = 1: even with a cell re-sort before every step a bead
+ * moved more than half the skin in one step; the replica stopped there (its
+ * pair set could be incomplete) while the rest of the pool went on. */
 int gcr_replica_status(gcr_pool* p, int32_t rid, double out[8]);
 int gcr_replica_download(gcr_pool* p, int32_t rid, double* pos, double* pos1, double* tangent,
                          double* forces);
