@@ -216,125 +216,148 @@ __device__ __forceinline__ double outside_cell(const double* x, int c, const Rep
 
 // Bond vector of the reference (inbounds_diff(pos[u], pos[v]) = x_u - x_v),
 // its squared length, inverse length and unit vector (unit = raw when the
-// length is zero, as forces.py:326-349 leaves it).
-template <int DIM>
+// length is zero, as forces.py:326-349 leaves it). BT is the arithmetic
+// type: double for the FP64 path, float for the mixed path (the difference
+// of the FP64 positions is taken in FP64, then rounded).
+template <int DIM, typename BT>
 struct Bond {
-    double t[DIM];
-    double u[DIM];
-    double ds;
-    double inv;
+    BT t[DIM];
+    BT u[DIM];
+    BT ds;
+    BT inv;
 };
 
-template <int DIM>
+__device__ __forceinline__ double inv_sqrt(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float inv_sqrt(float x) { return 1.0f / sqrtf(x); }
+__device__ __forceinline__ double acos_t(double x) { return acos(x); }
+__device__ __forceinline__ float acos_t(float x) { return acosf(x); }
+__device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
+
+template <int DIM, typename BT>
 __device__ __forceinline__ void make_bond(const double* xu, const double* xv, const RepParams& P,
-                                          Bond<DIM>& b)
+                                          Bond<DIM, BT>& b)
 {
-    b.ds = 0.0;
+    b.ds = (BT)0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        b.t[i] = wrap1(xu[i] - xv[i], P.L, P.halfL);
+        b.t[i] = (BT)wrap1(xu[i] - xv[i], P.L, P.halfL);
         b.ds += b.t[i] * b.t[i];
     }
-    if (b.ds != 0.0) {
-        b.inv = 1.0 / sqrt(b.ds);
+    if (b.ds != (BT)0) {
+        b.inv = inv_sqrt(b.ds);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i] * b.inv;
     } else {
-        b.inv = 0.0;
+        b.inv = (BT)0;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i];
     }
 }
 
-__device__ __forceinline__ double sgn(double x) { return (double)((x > 0) - (x < 0)); }
+template <typename BT>
+__device__ __forceinline__ BT sgn(BT x)
+{
+    return (BT)((x > (BT)0) - (x < (BT)0));
+}
 
 // Bending prefactor phi of forces.py:364-373 for the angle between ti (bond
 // leaving the vertex forward) and tj (bond arriving from behind). Returns
 // false when the reference skips the term (|dot| == 1).
-template <int DIM>
-__device__ __forceinline__ bool bend_phi(const double* ti, const double* tj, const RepParams& P,
-                                         double& dot, double& phi)
+template <int DIM, typename BT>
+__device__ __forceinline__ bool bend_phi(const BT* ti, const BT* tj, BT kappa, BT theta0, BT& dot,
+                                         BT& phi)
 {
-    dot = 0.0;
+    dot = (BT)0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) dot += ti[i] * tj[i];
-    if (!(-1.0 < dot && dot < 1.0)) return false;
-    phi = acos(dot);
-    if (P.theta0 != 0.0) phi += P.theta0 * sgn(-ti[0] * tj[1] + ti[1] * tj[0]);
-    phi *= P.kappa / sqrt(1.0 - dot * dot);
+    if (!((BT)-1 < dot && dot < (BT)1)) return false;
+    phi = acos_t(dot);
+    if (theta0 != (BT)0) phi += theta0 * sgn<BT>(-ti[0] * tj[1] + ti[1] * tj[0]);
+    phi *= kappa / sqrt_t((BT)1 - dot * dot);
     return true;
 }
 
-// All bonded terms acting on slot s, added to acc in the reference's order
-// (forces.py:305-402 walks beads in index order: bond p1->s, angle at p1,
-// bond s->n1, angle at s, angle at n1). Writes the tangent at s when
-// tangents follow the bonds.
+// Positions of the chain neighbours of a bead (2 and 1 before, 1 and 2
+// after); loaded before the pair loop so their latency overlaps it.
 template <int DIM>
-__device__ __forceinline__ void bonded(const double* X, int s, const double* xs, int4 lk,
-                                       double a, const RepParams& P, double* acc, double* tang,
-                                       bool& tang_set)
+struct ChainNb {
+    int4 lk;
+    double p2[DIM], p1[DIM], n1[DIM], n2[DIM];
+};
+
+template <int DIM>
+__device__ __forceinline__ void load_chain_nb(const double* X, const int32_t* LINK, int s, ChainNb<DIM>& c)
+{
+    c.lk = *reinterpret_cast<const int4*>(LINK + 4 * s);
+    if (c.lk.x >= 0) load_vec<DIM>(X, c.lk.x, c.p2);
+    if (c.lk.y >= 0) load_vec<DIM>(X, c.lk.y, c.p1);
+    if (c.lk.z >= 0) load_vec<DIM>(X, c.lk.z, c.n1);
+    if (c.lk.w >= 0) load_vec<DIM>(X, c.lk.w, c.n2);
+}
+
+// All bonded terms acting on the bead at xs, added to acc in the
+// reference's order (forces.py:305-402 walks beads in index order: bond
+// p1->s, angle at p1, bond s->n1, angle at s, angle at n1). Writes the
+// tangent at s when tangents follow the bonds.
+template <int DIM, typename BT>
+__device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, double a,
+                                       const RepParams& P, double* acc, double* tang, bool& tang_set)
 {
     const bool bend = (P.flags & GCR_F_BEND) != 0;
     const bool orient = (P.flags & GCR_F_ORIENT) != 0;
-    const double k = P.k;
-    double xp1[DIM], xn1[DIM], xp2[DIM], xn2[DIM];
-    Bond<DIM> A, B, Pb, Nb;  // (p1,s), (s,n1), (p2,p1), (n1,n2)
-    if (lk.y >= 0) {
-        load_vec<DIM>(X, lk.y, xp1);
-        make_bond<DIM>(xp1, xs, P, A);
-    }
-    if (lk.z >= 0) {
-        load_vec<DIM>(X, lk.z, xn1);
-        make_bond<DIM>(xs, xn1, P, B);
-    }
+    const BT k = (BT)P.k, kappa = (BT)P.kappa, theta0 = (BT)P.theta0, ab = (BT)a;
+    const int4 lk = c.lk;
+    Bond<DIM, BT> A, B, Pb, Nb;  // (p1,s), (s,n1), (p2,p1), (n1,n2)
+    if (lk.y >= 0) make_bond<DIM, BT>(c.p1, xs, P, A);
+    if (lk.z >= 0) make_bond<DIM, BT>(xs, c.n1, P, B);
     // bond p1 -> s: forces[b1+1] -= f (forces.py:345-348)
-    if (lk.y >= 0 && A.ds != 0.0) {
-        double mag = k * (a * A.inv - 1.0);
+    if (lk.y >= 0 && A.ds != (BT)0) {
+        const BT mag = k * (ab * A.inv - (BT)1);
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) acc[i] -= A.t[i] * mag;
+        for (int i = 0; i < DIM; ++i) acc[i] -= (double)(A.t[i] * mag);
     }
     // angle at p1: forces[b1+1] -= f_bend_backward
     if (bend && lk.x >= 0) {
-        load_vec<DIM>(X, lk.x, xp2);
-        make_bond<DIM>(xp2, xp1, P, Pb);
-        double dot, phi;
-        if (bend_phi<DIM>(A.u, Pb.u, P, dot, phi)) {
+        make_bond<DIM, BT>(c.p2, c.p1, P, Pb);
+        BT dot, phi;
+        if (bend_phi<DIM, BT>(A.u, Pb.u, kappa, theta0, dot, phi)) {
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) acc[i] -= phi * (Pb.u[i] - dot * A.u[i]);
+            for (int i = 0; i < DIM; ++i) acc[i] -= (double)(phi * (Pb.u[i] - dot * A.u[i]));
         }
     }
     // bond s -> n1: forces[b1] += f
-    if (lk.z >= 0 && B.ds != 0.0) {
-        double mag = k * (a * B.inv - 1.0);
+    if (lk.z >= 0 && B.ds != (BT)0) {
+        const BT mag = k * (ab * B.inv - (BT)1);
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) acc[i] += B.t[i] * mag;
+        for (int i = 0; i < DIM; ++i) acc[i] += (double)(B.t[i] * mag);
     }
     // angle at s (interior bead), and the tangent
     if (lk.y >= 0 && lk.z >= 0) {
         if (bend || orient) {
-            double dot = 0.0;
+            BT dot = (BT)0;
 #pragma unroll
             for (int i = 0; i < DIM; ++i) dot += B.u[i] * A.u[i];
-            bool inside = (-1.0 < dot && dot < 1.0);
+            const bool inside = ((BT)-1 < dot && dot < (BT)1);
             if (inside && bend) {
-                double phi = acos(dot);
-                if (P.theta0 != 0.0) phi += P.theta0 * sgn(-B.u[0] * A.u[1] + B.u[1] * A.u[0]);
-                phi *= P.kappa / sqrt(1.0 - dot * dot);
+                BT phi = acos_t(dot);
+                if (theta0 != (BT)0) phi += theta0 * sgn<BT>(-B.u[0] * A.u[1] + B.u[1] * A.u[0]);
+                phi *= kappa / sqrt_t((BT)1 - dot * dot);
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) {
-                    double ff = phi * (dot * A.u[i] - B.u[i]);
-                    double fb = phi * (A.u[i] - dot * B.u[i]);
-                    acc[i] += ff + fb;
+                    const BT ff = phi * (dot * A.u[i] - B.u[i]);
+                    const BT fb = phi * (A.u[i] - dot * B.u[i]);
+                    acc[i] += (double)(ff + fb);
                 }
             }
             if (orient) {
                 if (inside) {
-                    double c = 1.0 / sqrt(2.0 * (1.0 + dot));
+                    const BT cc = inv_sqrt((BT)2 * ((BT)1 + dot));
 #pragma unroll
-                    for (int i = 0; i < DIM; ++i) tang[i] = c * (B.u[i] + A.u[i]);
+                    for (int i = 0; i < DIM; ++i) tang[i] = (double)(cc * (B.u[i] + A.u[i]));
                 } else {
 #pragma unroll
-                    for (int i = 0; i < DIM; ++i) tang[i] = B.u[i];
+                    for (int i = 0; i < DIM; ++i) tang[i] = (double)B.u[i];
                 }
                 tang_set = true;
             }
@@ -342,22 +365,21 @@ __device__ __forceinline__ void bonded(const double* X, int s, const double* xs,
     } else if (orient) {
         if (lk.y < 0 && lk.z >= 0) {  // head: first bond
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) tang[i] = B.u[i];
+            for (int i = 0; i < DIM; ++i) tang[i] = (double)B.u[i];
             tang_set = true;
         } else if (lk.y >= 0 && lk.z < 0) {  // tail: last bond
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) tang[i] = A.u[i];
+            for (int i = 0; i < DIM; ++i) tang[i] = (double)A.u[i];
             tang_set = true;
         }
     }
     // angle at n1: forces[b1-1] -= f_bend_forward
     if (bend && lk.w >= 0) {
-        load_vec<DIM>(X, lk.w, xn2);
-        make_bond<DIM>(xn1, xn2, P, Nb);
-        double dot, phi;
-        if (bend_phi<DIM>(Nb.u, B.u, P, dot, phi)) {
+        make_bond<DIM, BT>(c.n1, c.n2, P, Nb);
+        BT dot, phi;
+        if (bend_phi<DIM, BT>(Nb.u, B.u, kappa, theta0, dot, phi)) {
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) acc[i] -= phi * (dot * B.u[i] - Nb.u[i]);
+            for (int i = 0; i < DIM; ++i) acc[i] -= (double)(phi * (dot * B.u[i] - Nb.u[i]));
         }
     }
 }
@@ -563,8 +585,9 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
         int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
         bool tang_set = false;
         if (P.flags & GCR_F_SPRINGS) {
-            int4 lk = *reinterpret_cast<const int4*>(a.LINK + 4 * s);
-            bonded<DIM>(Xa, s, xs, lk, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+            ChainNb<DIM> cn;
+            load_chain_nb<DIM>(Xa, a.LINK, s, cn);
+            bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
         }
         if (P.flags & GCR_F_WCA)
             pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
@@ -682,8 +705,9 @@ __global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
     int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
     bool tang_set = false;
     if (P.flags & GCR_F_SPRINGS) {
-        int4 lk = *reinterpret_cast<const int4*>(a.LINK + 4 * s);
-        bonded<DIM>(Xa, s, xs, lk, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+        ChainNb<DIM> cn;
+        load_chain_nb<DIM>(Xa, a.LINK, s, cn);
+        bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
     }
     if (P.flags & GCR_F_WCA)
         pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
@@ -701,6 +725,11 @@ __global__ void __launch_bounds__(BS) k_fill_i32(int32_t* p, int n, int32_t v)
     if (i < n) p[i] = v;
 }
 
+// Beads are stored sorted by cell, so neighbouring lanes mostly fall in the
+// same cell: each run of equal cells in a wave takes one atomic for the whole
+// run (its first lane adds the run length) and the lanes of the run number
+// themselves from the returned base. Offsets only need to be distinct within
+// a cell; the gather orders each cell by original index.
 template <int DIM>
 __global__ void __launch_bounds__(BS) k_bin(PoolArgs a, int j)
 {
@@ -708,12 +737,25 @@ __global__ void __launch_bounds__(BS) k_bin(PoolArgs a, int j)
     const RepParams& P = a.P[r];
     if (!rebuild_runs(a.S + r, j)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
-    if (s >= P.bead_off + P.n) return;
-    double x[DIM];
-    load_vec<DIM>(a.X, s, x);
-    int c = cell_of<DIM>(x, P);
-    int off = atomicAdd(&a.CCOUNT[P.cell_off + c], 1);
-    *reinterpret_cast<int2*>(a.KEYOFF + 2 * s) = make_int2(c, off);
+    const bool live = s < P.bead_off + P.n;
+    int c = -1;
+    if (live) {
+        double x[DIM];
+        load_vec<DIM>(a.X, s, x);
+        c = cell_of<DIM>(x, P);
+    }
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(c, 1, 64);
+    const bool head = lane == 0 || prev != c;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const int hl = 63 - __clzll((long long)(heads & upto));  // head of my run (lane 0 always is one)
+    const unsigned long long after = heads & ~upto;         // heads past my lane
+    const int end = after ? __ffsll(after) - 1 : 64;        // one past my run
+    int base = 0;
+    if (head && live) base = atomicAdd(&a.CCOUNT[P.cell_off + c], end - lane);
+    base = __shfl(base, hl, 64);
+    if (live) *reinterpret_cast<int2*>(a.KEYOFF + 2 * s) = make_int2(c, base + lane - hl);
 }
 
 // Inclusive wave64 scan with shuffles, exclusive block scan through LDS.
@@ -974,9 +1016,10 @@ __global__ void __launch_bounds__(BS) k_tangents(PoolArgs a, int rsel)
     double xs[DIM], acc[DIM] = {}, tang[DIM];
     load_vec<DIM>(a.X1, s, xs);
     int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
-    int4 lk = *reinterpret_cast<const int4*>(a.LINK + 4 * s);
+    ChainNb<DIM> cn;
+    load_chain_nb<DIM>(a.X1, a.LINK, s, cn);
     bool set = false;
-    bonded<DIM>(a.X1, s, xs, lk, a.CHAIN_A[me.y], P, acc, tang, set);
+    bonded<DIM, double>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, set);
     if (!set) load_vec<DIM>(a.T, s, tang);
     store_vec<DIM>(a.Tn, s, tang);
 }

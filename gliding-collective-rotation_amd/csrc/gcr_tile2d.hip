@@ -216,26 +216,30 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
         const int s = gst[c0] + (li - cst[c0]);
         double xs[2], acc[2], tang[2];
         load_vec<2>(Xa, s, xs);
+        const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+        const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
+        ChainNb<2> cn;
+        if (springs) load_chain_nb<2>(Xa, a.LINK, s, cn);  // in flight during the pair loop
+        double pa[2] = {0.0, 0.0};
+        if (P.flags & GCR_F_WCA) {
+            if (in_lds) {
+                const LBead<FT> q = lb[li];
+                tile_pair_sum<FT>(lb, cst, W, c0, q.x, q.y, me, kc, pa);
+            } else {
+                pair_sum<2, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, pa);
+            }
+        }
         if (STAGE == 1) {
             load_vec<2>(a.F0, s, acc);
         } else {
             acc[0] = 0.0;
             acc[1] = 0.0;
         }
-        const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+        // bonded terms first, then the pair sum, then propulsion (SRK.py:228-253)
         bool tang_set = false;
-        if (P.flags & GCR_F_SPRINGS) {
-            const int4 lk = *reinterpret_cast<const int4*>(a.LINK + 4 * s);
-            bonded<2>(Xa, s, xs, lk, a.CHAIN_A[me.y], P, acc, tang, tang_set);
-        }
-        if (P.flags & GCR_F_WCA) {
-            if (in_lds) {
-                const LBead<FT> q = lb[li];
-                tile_pair_sum<FT>(lb, cst, W, c0, q.x, q.y, me, kc, acc);
-            } else {
-                pair_sum<2, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
-            }
-        }
+        if (springs) bonded<2, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+        acc[0] += pa[0];
+        acc[1] += pa[1];
         if (STAGE == 2) {
             if (!tang_set) load_vec<2>(a.T, s, tang);
             if (P.flags & GCR_F_PROPEL) propel<2>(a, me.y, tang, acc);
