@@ -119,6 +119,9 @@ struct gcr_pool {
     int32_t *CCOUNT = nullptr, *CSTART = nullptr, *CHKSUM = nullptr;
     double* BMAX = nullptr;
     int32_t* TILES = nullptr;
+    uint16_t* NL = nullptr;
+    int32_t* NLEN = nullptr;
+    double* XR = nullptr;
     int64_t capChk = 0, capBlk = 0, capTile = 0;
     int ntile = 0;
     int tile_side = 0;  // 0: chosen per replica from the cell occupancy
@@ -251,6 +254,12 @@ void launch_rebuild(gcr_pool* p, int j)
     if (p->dim == 2) {
         k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
+        if (p->ntile) {
+            if (p->precision == GCR_MIXED)
+                k_nlist_tile2d<float><<<p->ntile, BS, 0, st>>>(a, j);
+            else
+                k_nlist_tile2d<double><<<p->ntile, BS, 0, st>>>(a, j);
+        }
     } else {
         k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
@@ -451,8 +460,10 @@ int gcr_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_by
 int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_cells)
 {
     // per bead: 9 dim-vectors + noise (2 dim) of doubles, 5 int pairs, links,
-    // cell and slot ints; per chain: 4 doubles + 2 ints; per cell: 2 ints
-    return n * (8 * (9 * dim + 2 * dim) + 4 * (5 * 2 + 4 + 2)) + n_chains * (8 * 4 + 8) +
+    // cell and slot ints; 2D adds the neighbour list, its length and the
+    // re-sort positions; per chain: 4 doubles + 2 ints; per cell: 2 ints
+    const int64_t nl = dim == 2 ? 2 * NL_MAX + 4 + 8 * dim : 0;
+    return n * (8 * (9 * dim + 2 * dim) + 4 * (5 * 2 + 4 + 2) + nl) + n_chains * (8 * 4 + 8) +
            (n_cells + 1) * 8;
 }
 
@@ -522,6 +533,11 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CHKSUM, p->capChk));
     A(dalloc(&p->BMAX, std::max(p->capBlk, p->capTile)));
     A(dalloc(&p->TILES, p->capTile * 4));
+    if (dim == 2) {
+        A(dalloc(&p->NL, nb * NL_MAX));
+        A(dalloc(&p->NLEN, nb));
+        A(dalloc(&p->XR, nb * 2));
+    }
     if (rc == GCR_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(GCR_ERR_HIP, "stream creation failed");
     if (rc == GCR_OK && hipMemset(p->CCOUNT, 0, sizeof(int32_t) * (capC + 1)) != hipSuccess)
@@ -564,6 +580,10 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.CHKSUM = p->CHKSUM;
     a.BMAX = p->BMAX;
     a.TILES = p->TILES;
+    a.NL = p->NL;
+    a.NLEN = p->NLEN;
+    a.XR = p->XR;
+    a.nl_stride = (int64_t)capB;
     a.KEYOFF = p->KEYOFF;
     a.PAIRS = p->PAIRS;
     a.Xn = p->Xn;
@@ -659,6 +679,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     P.inv_h = (double)nc / d->L;
     P.skin_half = 0.5 * (P.h - (d->has_wca ? d->wca[3] : 0.0));
     if (!d->has_wca) P.skin_half = 1e300;
+    P.rl2 = P.h * P.h * (1.0 + 1e-5);  // list radius rc + 2 skin_half = h
     for (int i = 0; i < 4; ++i) P.wca[i] = d->wca[i];
     P.rc2 = d->wca[3] * d->wca[3];
     P.k = d->spring[0];
@@ -978,7 +999,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->dSs, p->X, p->X1, p->F0, p->FT, p->XI, p->T, p->Xn, p->Tn, p->Xs, p->Ts,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
-                    p->CHKSUM, p->BMAX, p->TILES};
+                    p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

@@ -37,7 +37,9 @@ struct RepParams {
     int32_t tile_off, ntile; // first tile, tiles of this replica
     double L, halfL;
     double h, inv_h;    // cell edge and its inverse
-    double skin_half;   // allowed excursion of a bead outside its cell box
+    double skin_half;   // half the skin: allowed displacement since the last re-sort
+                        // (2D, neighbour lists) or excursion outside the cell box (3D)
+    double rl2;         // squared neighbour-list radius (2D): h^2, slightly padded
     double wca[4];      // make_WCA_params (populations.py:48-58)
     double rc2;
     double k, kappa, theta0;  // spring_params[0..2]
@@ -92,6 +94,11 @@ struct PoolArgs {
     int32_t* LINK; // [slot][4] (prev2, prev1, next1, next2) slots, -1 if none
     int32_t* CELL; // [slot] cell (replica-local) the bead was binned into
     int32_t* SLOT; // [orig_global] -> slot
+    // 2D neighbour lists, rebuilt with every cell re-sort
+    uint16_t* NL;  // [group][slot][4] LDS indices (tile order) of the neighbours
+    int32_t* NLEN; // [slot] list length, -1: no list (pairs from the cell scan)
+    double* XR;    // [slot][2] positions at the last re-sort
+    int64_t nl_stride;  // slots per group (the pool's bead capacity)
     // per-chain constants
     const double* CHAIN_A;   // rest length
     const double* CHAIN_CS;  // [chain][3] (v0, cos v_tan_angle, sin v_tan_angle)

@@ -212,6 +212,22 @@ __device__ __forceinline__ double outside_cell(const double* x, int c, const Rep
     return worst;
 }
 
+// Half-skin check of a new position of the bead in slot s. 2D: displacement
+// since the last re-sort, which keeps the neighbour lists (radius rc + skin)
+// complete for every pair within rc. 3D: excursion outside the cell box,
+// which keeps the 3 x 3 x 3 cell scan complete.
+template <int DIM>
+__device__ __forceinline__ bool moved_too_far(const PoolArgs& a, const RepParams& P, int s, const double* x)
+{
+    if (DIM == 2) {
+        const double2 r = *reinterpret_cast<const double2*>(a.XR + 2 * s);
+        const double dx = wrap1(x[0] - r.x, P.L, P.halfL);
+        const double dy = wrap1(x[1] - r.y, P.L, P.halfL);
+        return dx * dx + dy * dy > P.skin_half * P.skin_half;
+    }
+    return outside_cell<DIM>(x, a.CELL[s], P) > P.skin_half;
+}
+
 // ------------------------------------------------------------ bonded forces
 
 // Bond vector of the reference (inbounds_diff(pos[u], pos[v]) = x_u - x_v),
@@ -547,7 +563,7 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
             x1[i] = pymod(tmp, P.L);
         }
         store_vec<DIM>(a.X1, s, x1);
-        if (outside_cell<DIM>(x1, a.CELL[s], P) > P.skin_half) atomicOr(&S->violation, 1);
+        if (moved_too_far<DIM>(a, P, s, x1)) atomicOr(&S->violation, 1);
         return 0.0;
     } else {
         store_vec<DIM>(a.FT, s, acc);
@@ -660,7 +676,7 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
         x[i] = pymod(tmp, P.L);
     }
     store_vec<DIM>(a.X, s, x);
-    if (outside_cell<DIM>(x, a.CELL[s], P) > P.skin_half) atomicOr(&a.S[r].violation, 1);
+    if (moved_too_far<DIM>(a, P, s, x)) atomicOr(&a.S[r].violation, 1);
     if (P.flags & GCR_F_ROTDIFF) {
         const double sr = sqrt(2.0 * dt_used * P.Dr) * 3.141592653589793;
         double t[DIM], nr[DIM], proj = 0.0;
@@ -912,6 +928,7 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
     double x[DIM];
     load_vec<DIM>(a.Xn, ns, x);
     store_vec<DIM>(a.X, ns, x);
+    if (DIM == 2) store_vec<DIM>(a.XR, ns, x);
     if (with_t) {
         double t[DIM];
         load_vec<DIM>(a.Tn, ns, t);

@@ -1,35 +1,93 @@
-// gcr_tile2d.hip -- 2D force stages over tiles of cells staged in LDS.
+// gcr_tile2d.hip -- 2D force stages over tiles of cells staged in LDS, with
+// per-bead neighbour lists.
 //
 // One workgroup owns a tile of tile_x x tile_y cells of one replica. It stages
-// the beads of the tile and of a one-cell halo into LDS (positions relative
-// to the centre of the cell each bead is binned in, plus (orig, chain)), then
-// every owned bead sums its pair forces over the 3 x 3 surrounding cells out
-// of LDS. Relative coordinates make the periodic image implicit (the
-// neighbour cell's offset is -h, 0 or +h), so the inner loop has no wrap and,
-// in the mixed form, runs entirely in FP32 with separations accurate to the
-// FP32 ulp of a cell edge. Bonded terms, propulsion and the SRK2 stage update
-// are the same device functions as k_stage; outputs are one write per bead.
-// A tile whose beads do not fit in LDS (or a box of fewer than 3 cells per
-// side) sums its pairs from global memory with pair_sum instead.
+// the beads of the tile and of a one-cell halo into LDS in a fixed order
+// (halo cells row-major, a cell's beads in slot order). That order depends on
+// the cell contents only, so it is the same at every step between two cell
+// re-sorts. At each re-sort, k_nlist_tile2d stores for every bead the LDS
+// indices of the beads within rc + skin of it that are not excluded by the
+// same-chain rule of updates.py:645-650 (a Verlet list in tile coordinates).
+// The stage kernels then sum pair forces over the list only: about a third of
+// the candidates of the 3 x 3 cell scan, no exclusion test, and every load
+// from LDS. The list stays complete while no bead has moved more than half
+// the skin since the re-sort; the stages check that and flag the replica for
+// a replay otherwise.
+//
+// LDS coordinates: mixed form, FP32 position relative to the centre of the
+// bead's cell plus the halo coordinates of that cell (small integers), so a
+// separation is (dcell * h) + (dx), both accurate to the FP32 ulp of a cell
+// edge; FP64 form, the position relative to the tile's corner cell.
+// Bonded terms, propulsion and the SRK2 stage update are the same device
+// functions as k_stage; outputs are one write per bead. A bead whose list
+// overflowed, and every bead of a tile that does not fit in LDS (or of a box
+// of fewer than 3 cells per side), sums its pairs from global memory with
+// pair_sum instead.
 #define TILE_MAX_HALO 400    // (tile_x + 2) * (tile_y + 2) cells
-#define TILE_MAX_BEADS 1024  // beads of a tile with its halo held in LDS (16-24 B each)
+#define TILE_MAX_BEADS 1024  // beads of a tile with its halo held in LDS
 #define TILE_MAX_ROWS 16
+#define NL_MAX 64            // neighbour-list entries per bead (16 groups of 4)
 
 namespace gcr {
 
-// LDS record of a staged bead: position relative to the centre of its cell
-// and (original index, chain). FP32: one 16-byte record; FP64: 24 bytes.
+// LDS position record of a staged bead.
 template <typename FT>
-struct alignas(16) LBead {
-    FT x, y;
-    int orig, chain;
+struct LPos;
+
+template <>
+struct alignas(16) LPos<float> {
+    float x, y;    // relative to the centre of the bead's cell
+    float cx, cy;  // halo coordinates of that cell
 };
+
+template <>
+struct alignas(16) LPos<double> {
+    double x, y;  // relative to the centre of the tile's halo cell (0, 0)
+};
+
+template <typename FT>
+__device__ __forceinline__ LPos<FT> make_lpos(double rx, double ry, int hx, int hy, double h);
+
+template <>
+__device__ __forceinline__ LPos<float> make_lpos<float>(double rx, double ry, int hx, int hy, double)
+{
+    LPos<float> q;
+    q.x = (float)rx;
+    q.y = (float)ry;
+    q.cx = (float)hx;
+    q.cy = (float)hy;
+    return q;
+}
+
+template <>
+__device__ __forceinline__ LPos<double> make_lpos<double>(double rx, double ry, int hx, int hy, double h)
+{
+    LPos<double> q;
+    q.x = rx + hx * h;
+    q.y = ry + hy * h;
+    return q;
+}
+
+// Separation q - me. The cell offsets are -1, 0 or +1 for every listed pair,
+// so (dc * h) is exact in FP32.
+__device__ __forceinline__ void separation(const LPos<float>& q, const LPos<float>& me, float h, float& sx,
+                                           float& sy)
+{
+    sx = fmaf(q.cx - me.cx, h, q.x - me.x);
+    sy = fmaf(q.cy - me.cy, h, q.y - me.y);
+}
+
+__device__ __forceinline__ void separation(const LPos<double>& q, const LPos<double>& me, double,
+                                           double& sx, double& sy)
+{
+    sx = q.x - me.x;
+    sy = q.y - me.y;
+}
 
 // Scalars of the pair law, held in registers for the whole loop.
 template <typename FT>
 struct PairConsts {
     FT h, rc2, p0, p1, shift;
-    int nn;
     bool shifted;
 };
 
@@ -42,69 +100,289 @@ __device__ __forceinline__ PairConsts<FT> pair_consts(const RepParams& P)
     k.p0 = (FT)P.wca[0];
     k.p1 = (FT)P.wca[1];
     k.shift = (FT)P.wca[2];
-    k.nn = P.n_nearby;
     k.shifted = P.wca[2] != 0.0;
     return k;
 }
 
-// Pair forces of one bead from the LDS tile: cell c0 = hy * W + hx is its
-// cell in the tile frame, (xi, yi) its position relative to that cell's
-// centre. Branch-free body: every candidate of the 3 x 3 cells is evaluated
-// with a mask (cutoff, self, same-chain exclusion of updates.py:645-650), so
-// all lanes follow one path and the LDS reads pipeline. The forces of one
-// neighbour cell are summed in FT and added to the FP64 accumulator.
+// One listed candidate: branch-free (cutoff and zero-distance mask), so all
+// lanes follow one path and the LDS reads pipeline.
 template <typename FT>
-__device__ __forceinline__ void tile_pair_sum(const LBead<FT>* lb, const int* cst, int W, int c0, FT xi,
-                                              FT yi, int2 me, const PairConsts<FT>& k, double* acc)
+__device__ __forceinline__ void pair_term(const LPos<FT>& q, const LPos<FT>& me, const PairConsts<FT>& k,
+                                          FT& fx, FT& fy)
 {
-    for (int dy = -1; dy <= 1; ++dy) {
-        const FT oy = (FT)dy * k.h - yi;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const FT ox = (FT)dx * k.h - xi;
-            const int c = c0 + dy * W + dx;
-            const int t1 = cst[c + 1];
-            FT fx = (FT)0, fy = (FT)0;
-            if (!k.shifted) {
-#pragma unroll 4
-                for (int t = cst[c]; t < t1; ++t) {
-                    const LBead<FT> q = lb[t];
-                    const FT sx = q.x + ox, sy = q.y + oy;
-                    const FT ds = sx * sx + sy * sy;
-                    const int d = q.orig - me.x;
-                    // bitwise, not short-circuit: keeps the body free of branches
-                    const int excl = (q.chain == me.y) & ((k.nn < 0) | ((d < 0 ? -d : d) <= k.nn));
-                    const bool use = (ds <= k.rc2) & (ds > (FT)0) & !excl;
-                    const FT cf = PairLaw<FT>::coeff(use ? ds : (FT)1, k.p0, k.p1);
-                    const FT w = use ? cf : (FT)0;
-                    fx += sx * w;
-                    fy += sy * w;
-                }
-            } else {
-                for (int t = cst[c]; t < t1; ++t) {
-                    const LBead<FT> q = lb[t];
-                    FT sx = q.x + ox, sy = q.y + oy;
-                    FT ds = sx * sx + sy * sy;
-                    if (!(ds <= k.rc2 && ds > (FT)0)) continue;
-                    if (q.chain == me.y) {
-                        if (k.nn < 0) continue;
-                        const int d = q.orig - me.x;
-                        if ((d < 0 ? -d : d) <= k.nn) continue;
+    FT sx, sy;
+    separation(q, me, k.h, sx, sy);
+    const FT ds = sx * sx + sy * sy;
+    const bool use = (ds <= k.rc2) & (ds > (FT)0);
+    const FT cf = PairLaw<FT>::coeff(use ? ds : (FT)1, k.p0, k.p1);
+    const FT w = use ? cf : (FT)0;
+    fx += sx * w;
+    fy += sy * w;
+}
+
+// WCA_shift (forces.py:123-131): the cutoff applies to the unshifted distance.
+template <typename FT>
+__device__ __forceinline__ void pair_term_shifted(const LPos<FT>& q, const LPos<FT>& me,
+                                                  const PairConsts<FT>& k, FT& fx, FT& fy)
+{
+    FT sx, sy;
+    separation(q, me, k.h, sx, sy);
+    FT ds = sx * sx + sy * sy;
+    if (!(ds <= k.rc2 && ds > (FT)0)) return;
+    const FT mag = sqrt(ds);
+    FT nm = mag + k.shift;
+    nm = nm < (FT)0 ? (FT)0 : nm;
+    const FT rs = nm / mag;
+    sx *= rs;
+    sy *= rs;
+    ds = nm * nm;
+    const FT cf = PairLaw<FT>::coeff(ds, k.p0, k.p1);
+    fx += sx * cf;
+    fy += sy * cf;
+}
+
+// Pair forces of one bead over its list (len entries, groups of 4 LDS
+// indices at NL[g * stride + s]; the last group is padded with the bead's own
+// index, which the zero-distance mask drops). Summed in FT.
+template <typename FT>
+struct FT2 {
+    FT x, y;
+};
+
+template <typename FT>
+__device__ __forceinline__ FT2<FT> list_pair_sum(const LPos<FT>* lp, const uint2* G, int64_t stride, int len,
+                                                 const LPos<FT> me, const PairConsts<FT>& k)
+{
+    FT fx = (FT)0, fy = (FT)0;
+    const int ng = (len + 3) >> 2;
+    if (!k.shifted) {
+        uint32_t e0 = 0u, e1 = 0u;
+        if (ng > 0) {
+            const uint2 e = G[0];
+            e0 = e.x;
+            e1 = e.y;
+        }
+        for (int g = 0; g < ng; ++g) {
+            uint32_t n0 = e0, n1 = e1;
+            if (g + 1 < ng) {  // next group in flight during this one
+                const uint2 en = G[(int64_t)(g + 1) * stride];
+                n0 = en.x;
+                n1 = en.y;
+            }
+            pair_term<FT>(lp[e0 & 0xffffu], me, k, fx, fy);
+            pair_term<FT>(lp[e0 >> 16], me, k, fx, fy);
+            pair_term<FT>(lp[e1 & 0xffffu], me, k, fx, fy);
+            pair_term<FT>(lp[e1 >> 16], me, k, fx, fy);
+            e0 = n0;
+            e1 = n1;
+        }
+    } else {
+        for (int g = 0; g < ng; ++g) {
+            const uint2 e = G[(int64_t)g * stride];
+            pair_term_shifted<FT>(lp[e.x & 0xffffu], me, k, fx, fy);
+            pair_term_shifted<FT>(lp[e.x >> 16], me, k, fx, fy);
+            pair_term_shifted<FT>(lp[e.y & 0xffffu], me, k, fx, fy);
+            pair_term_shifted<FT>(lp[e.y >> 16], me, k, fx, fy);
+        }
+    }
+    return FT2<FT>{fx, fy};
+}
+
+// Geometry of a tile and its halo in LDS, shared by the list builder and the
+// stage kernels so that both see the same LDS order.
+struct TileLayout {
+    int cx0, cy0, W, nhalo, own_x, own_y, total;
+    bool in_lds;
+};
+
+struct TileShared {
+    int cst[TILE_MAX_HALO + 1];  // LDS offset of each halo cell
+    int gst[TILE_MAX_HALO];      // first global slot of each halo cell
+    int own_pref[TILE_MAX_ROWS + 1];
+};
+
+// Halo bead counts -> LDS offsets (two cells per thread), owned-bead row
+// prefix; ends with a barrier.
+__device__ __forceinline__ TileLayout tile_layout(const PoolArgs& a, const RepParams& P, int4 tile,
+                                                  TileShared& sh)
+{
+    TileLayout t;
+    const int nc = P.nc, TX = P.tile_x, TY = P.tile_y;
+    t.W = TX + 2;
+    t.cx0 = tile.y;
+    t.cy0 = tile.z;
+    t.own_x = min(TX, nc - t.cx0);
+    t.own_y = min(TY, nc - t.cy0);
+    t.nhalo = t.W * (TY + 2);
+    int cnt[2], g0[2], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int c = threadIdx.x * 2 + k;
+        cnt[k] = 0;
+        g0[k] = 0;
+        if (c < t.nhalo) {
+            const int gx = (t.cx0 - 1 + c % t.W + nc) % nc;
+            const int gy = (t.cy0 - 1 + c / t.W + nc) % nc;
+            const int gc = P.cell_off + gy * nc + gx;
+            g0[k] = a.CSTART[gc];
+            cnt[k] = a.CSTART[gc + 1] - g0[k];
+        }
+        sum += cnt[k];
+    }
+    int ex = block_exclusive_scan(sum, &t.total);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int c = threadIdx.x * 2 + k;
+        if (c < t.nhalo) {
+            sh.cst[c] = ex;
+            sh.gst[c] = g0[k];
+        }
+        ex += cnt[k];
+    }
+    if (threadIdx.x == 0) sh.cst[t.nhalo] = t.total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // owned beads of tile row ry are the LDS range of cells 1..own_x of halo row ry+1
+        int p = 0;
+        for (int ry = 0; ry < t.own_y; ++ry) {
+            sh.own_pref[ry] = p;
+            const int row = (ry + 1) * t.W;
+            p += sh.cst[row + t.own_x + 1] - sh.cst[row + 1];
+        }
+        sh.own_pref[t.own_y] = p;
+    }
+    t.in_lds = (P.flags & GCR_F_WCA) && t.total <= TILE_MAX_BEADS && nc >= 3;
+    __syncthreads();
+    return t;
+}
+
+// Halo cell and global slot of staged bead i.
+__device__ __forceinline__ void staged_bead(const TileShared& sh, const TileLayout& t, int i, int& hc, int& g)
+{
+    int lo = 0, hi = t.nhalo;  // cst[lo] <= i < cst[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sh.cst[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    hc = lo;
+    g = sh.gst[lo] + (i - sh.cst[lo]);
+}
+
+// Stage the halo's positions (and, for the list builder, (orig, chain)).
+template <typename FT>
+__device__ __forceinline__ void stage_positions(const PoolArgs& a, const RepParams& P, const double* Xa,
+                                                const TileShared& sh, const TileLayout& t, LPos<FT>* lp,
+                                                int2* lm)
+{
+    const int nc = P.nc;
+    for (int i = threadIdx.x; i < t.total; i += BS) {
+        int hc, g;
+        staged_bead(sh, t, i, hc, g);
+        const int hx = hc % t.W, hy = hc / t.W;
+        const int gx = (t.cx0 - 1 + hx + nc) % nc;
+        const int gy = (t.cy0 - 1 + hy + nc) % nc;
+        const double2 x = *reinterpret_cast<const double2*>(Xa + 2 * g);
+        const double rx = wrap1(x.x - (gx + 0.5) * P.h, P.L, P.halfL);
+        const double ry = wrap1(x.y - (gy + 0.5) * P.h, P.L, P.halfL);
+        lp[i] = make_lpos<FT>(rx, ry, hx, hy, P.h);
+        if (lm) lm[i] = *reinterpret_cast<const int2*>(a.META + 2 * g);
+    }
+}
+
+// Owned bead o of the tile: its LDS index li, halo cell c0 and slot s.
+__device__ __forceinline__ void owned_bead(const TileShared& sh, const TileLayout& t, int o, int& li, int& c0,
+                                           int& s)
+{
+    int ry = 0;
+    while (sh.own_pref[ry + 1] <= o) ++ry;
+    const int row = (ry + 1) * t.W;
+    li = sh.cst[row + 1] + (o - sh.own_pref[ry]);
+    int hx = 1;
+    while (sh.cst[row + hx + 1] <= li) ++hx;
+    c0 = row + hx;
+    s = sh.gst[c0] + (li - sh.cst[c0]);
+}
+
+// Neighbour lists of the tile's owned beads, after a cell re-sort (same run
+// condition as the re-sort). Each bead scans the 3 x 3 cells around its own
+// in LDS order and keeps the beads within rc + skin that the same-chain rule
+// does not exclude; the list order is that scan order, so it depends on the
+// positions alone. NLEN = -1 marks a bead without a list (overflow, or a
+// tile outside LDS). The positions at the re-sort are kept in XR for the
+// displacement check of the stages.
+template <typename FT>
+__global__ void __launch_bounds__(BS) k_nlist_tile2d(PoolArgs a, int j)
+{
+    const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
+    const RepParams& P = a.P[tile.x];
+    if (!rebuild_runs(a.S + tile.x, j)) return;
+    if (!(P.flags & GCR_F_WCA)) return;
+    __shared__ TileShared sh;
+    __shared__ LPos<FT> lp[TILE_MAX_BEADS];
+    __shared__ int2 lm[TILE_MAX_BEADS];
+    const TileLayout t = tile_layout(a, P, tile, sh);
+    if (t.in_lds) stage_positions<FT>(a, P, a.X, sh, t, lp, lm);
+    __syncthreads();
+    const FT rl2 = (FT)P.rl2, h = (FT)P.h;
+    const int nn = P.n_nearby;
+    const int n_own = sh.own_pref[t.own_y];
+    uint2* NL = reinterpret_cast<uint2*>(a.NL);
+    for (int o = threadIdx.x; o < n_own; o += BS) {
+        int li, c0, s;
+        owned_bead(sh, t, o, li, c0, s);
+        if (!t.in_lds) {
+            a.NLEN[s] = -1;
+            continue;
+        }
+        const LPos<FT> me = lp[li];
+        const int2 mm = lm[li];
+        int n = 0;
+        uint32_t w0 = 0u, w1 = 0u;  // the group being filled
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int c = c0 + dy * t.W + dx;
+                for (int q = sh.cst[c]; q < sh.cst[c + 1]; ++q) {
+                    FT sx, sy;
+                    separation(lp[q], me, h, sx, sy);
+                    if (q == li || !(sx * sx + sy * sy <= rl2)) continue;
+                    const int2 m = lm[q];
+                    if (m.y == mm.y) {
+                        if (nn < 0) continue;
+                        const int d = m.x - mm.x;
+                        if ((d < 0 ? -d : d) <= nn) continue;
                     }
-                    // WCA_shift, forces.py:123-131
-                    FT mag = sqrt(ds);
-                    FT nm = mag + k.shift;
-                    nm = nm < (FT)0 ? (FT)0 : nm;
-                    FT rs = nm / mag;
-                    sx *= rs;
-                    sy *= rs;
-                    ds = nm * nm;
-                    const FT cf = PairLaw<FT>::coeff(ds, k.p0, k.p1);
-                    fx += sx * cf;
-                    fy += sy * cf;
+                    if (n < NL_MAX) {
+                        const int sl = n & 3;
+                        const uint32_t v = (uint32_t)q << (16 * (sl & 1));
+                        if (sl < 2)
+                            w0 |= v;
+                        else
+                            w1 |= v;
+                        if (sl == 3) {
+                            NL[(int64_t)(n >> 2) * a.nl_stride + s] = make_uint2(w0, w1);
+                            w0 = w1 = 0u;
+                        }
+                    }
+                    ++n;
                 }
             }
-            acc[0] += (double)fx;
-            acc[1] += (double)fy;
+        if (n <= NL_MAX) {
+            if (n & 3) {  // pad the last group with the bead's own index
+                for (int sl = n & 3; sl < 4; ++sl) {
+                    const uint32_t v = (uint32_t)li << (16 * (sl & 1));
+                    if (sl < 2)
+                        w0 |= v;
+                    else
+                        w1 |= v;
+                }
+                NL[(int64_t)(n >> 2) * a.nl_stride + s] = make_uint2(w0, w1);
+            }
+            a.NLEN[s] = n;
+        } else {
+            a.NLEN[s] = -1;
         }
     }
 }
@@ -124,109 +402,38 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
         return;
     }
     const double* Xa = STAGE == 1 ? a.X1 : a.X;
-    const int nc = P.nc, TX = P.tile_x, TY = P.tile_y, W = TX + 2, H = TY + 2;
-    const int cx0 = tile.y, cy0 = tile.z;
-    const int own_x = min(TX, nc - cx0), own_y = min(TY, nc - cy0);
-    const int nhalo = W * H;
-
-    __shared__ int cst[TILE_MAX_HALO + 1];  // LDS offset of each halo cell
-    __shared__ int gst[TILE_MAX_HALO];      // first global slot of each halo cell
-    __shared__ int own_pref[TILE_MAX_ROWS + 1];
-    __shared__ LBead<FT> lb[TILE_MAX_BEADS];
-
-    // 1. bead counts of the halo cells -> LDS offsets (two cells per thread)
-    int cnt[2], g0[2], sum = 0;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int c = threadIdx.x * 2 + k;
-        cnt[k] = 0;
-        g0[k] = 0;
-        if (c < nhalo) {
-            const int gx = (cx0 - 1 + c % W + nc) % nc;
-            const int gy = (cy0 - 1 + c / W + nc) % nc;
-            const int gc = P.cell_off + gy * nc + gx;
-            g0[k] = a.CSTART[gc];
-            cnt[k] = a.CSTART[gc + 1] - g0[k];
-        }
-        sum += cnt[k];
-    }
-    int total;
-    int ex = block_exclusive_scan(sum, &total);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int c = threadIdx.x * 2 + k;
-        if (c < nhalo) {
-            cst[c] = ex;
-            gst[c] = g0[k];
-        }
-        ex += cnt[k];
-    }
-    if (threadIdx.x == 0) cst[nhalo] = total;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // owned beads of tile row ry are the LDS range of cells 1..own_x of halo row ry+1
-        int p = 0;
-        for (int ry = 0; ry < own_y; ++ry) {
-            own_pref[ry] = p;
-            const int row = (ry + 1) * W;
-            p += cst[row + own_x + 1] - cst[row + 1];
-        }
-        own_pref[own_y] = p;
-    }
-    const bool in_lds = (P.flags & GCR_F_WCA) && total <= TILE_MAX_BEADS && nc >= 3;
-
-    // 2. stage the halo: positions relative to the centre of their cell
-    if (in_lds) {
-        for (int i = threadIdx.x; i < total; i += BS) {
-            int lo = 0, hi = nhalo;  // cst[lo] <= i < cst[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (cst[mid] <= i)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const int gx = (cx0 - 1 + lo % W + nc) % nc;
-            const int gy = (cy0 - 1 + lo / W + nc) % nc;
-            const int g = gst[lo] + (i - cst[lo]);
-            const double2 x = *reinterpret_cast<const double2*>(Xa + 2 * g);
-            const int2 m = *reinterpret_cast<const int2*>(a.META + 2 * g);
-            LBead<FT> q;
-            q.x = (FT)wrap1(x.x - (gx + 0.5) * P.h, P.L, P.halfL);
-            q.y = (FT)wrap1(x.y - (gy + 0.5) * P.h, P.L, P.halfL);
-            q.orig = m.x;
-            q.chain = m.y;
-            lb[i] = q;
-        }
-    }
+    __shared__ TileShared sh;
+    __shared__ LPos<FT> lp[TILE_MAX_BEADS];
+    const TileLayout t = tile_layout(a, P, tile, sh);
+    if (t.in_lds) stage_positions<FT>(a, P, Xa, sh, t, lp, nullptr);
     __syncthreads();
 
-    // 3. owned beads
     const PairConsts<FT> kc = pair_consts<FT>(P);
-    const int n_own = own_pref[own_y];
+    const int n_own = sh.own_pref[t.own_y];
+    const uint2* NL = reinterpret_cast<const uint2*>(a.NL);
+    const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
     double fsq = 0.0;
     for (int o = threadIdx.x; o < n_own; o += BS) {
-        int ry = 0;
-        while (own_pref[ry + 1] <= o) ++ry;
-        const int row = (ry + 1) * W;
-        const int li = cst[row + 1] + (o - own_pref[ry]);
-        int hx = 1;
-        while (cst[row + hx + 1] <= li) ++hx;
-        const int c0 = row + hx;
-        const int s = gst[c0] + (li - cst[c0]);
+        int li, c0, s;
+        owned_bead(sh, t, o, li, c0, s);
         double xs[2], acc[2], tang[2];
         load_vec<2>(Xa, s, xs);
         const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
-        const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
         ChainNb<2> cn;
         if (springs) load_chain_nb<2>(Xa, a.LINK, s, cn);  // in flight during the pair loop
-        double pa[2] = {0.0, 0.0};
+        double pa0 = 0.0, pa1 = 0.0;
         if (P.flags & GCR_F_WCA) {
-            if (in_lds) {
-                const LBead<FT> q = lb[li];
-                tile_pair_sum<FT>(lb, cst, W, c0, q.x, q.y, me, kc, pa);
+            const int len = a.NLEN[s];
+            if (len >= 0) {
+                const LPos<FT> mp = lp[li];
+                const FT2<FT> f = list_pair_sum<FT>(lp, NL + s, a.nl_stride, len, mp, kc);
+                pa0 = (double)f.x;
+                pa1 = (double)f.y;
             } else {
-                pair_sum<2, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, pa);
+                double pg[2] = {0.0, 0.0};
+                pair_sum<2, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, pg);
+                pa0 = pg[0];
+                pa1 = pg[1];
             }
         }
         if (STAGE == 1) {
@@ -238,8 +445,8 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
         // bonded terms first, then the pair sum, then propulsion (SRK.py:228-253)
         bool tang_set = false;
         if (springs) bonded<2, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
-        acc[0] += pa[0];
-        acc[1] += pa[1];
+        acc[0] += pa0;
+        acc[1] += pa1;
         if (STAGE == 2) {
             if (!tang_set) load_vec<2>(a.T, s, tang);
             if (P.flags & GCR_F_PROPEL) propel<2>(a, me.y, tang, acc);

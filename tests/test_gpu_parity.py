@@ -55,6 +55,15 @@ def cases():
                                       shift=0.1, n_nearby=-1, T=0.01, steps=100, seed=7)
     out["diffusing_chains"] = W.chains("diffusing", 2, 40, 6, L=20.0, T=0.01, Dr=0.05,
                                        tangent_locked=False, vtan=0.3, steps=100, seed=8)
+    # a wide skin puts about as many beads within rc + skin as a neighbour
+    # list holds: some beads keep a list, the others sum over their cells
+    out["list_overflow"] = W.chains("list_overflow", 2, 900, 1, phi=0.55,
+                                    cutoff_factor=W.LJ_25_SIGMA, T=0.01, Dr=0.1, steps=100,
+                                    seed=9, skin=3.3)
+    # cells so full that a tile with its halo does not fit in LDS
+    out["tile_outside_lds"] = W.chains("tile_outside_lds", 2, 1300, 1, phi=0.55,
+                                       cutoff_factor=W.LJ_25_SIGMA, T=0.01, Dr=0.1,
+                                       steps=100, seed=10, skin=8.0)
     return out
 
 
