@@ -37,6 +37,11 @@ def parse():
     ap.add_argument("--batch", type=int, default=64, help="steps per captured graph")
     ap.add_argument("--rebuild", type=int, default=8, help="cell-list rebuild period")
     ap.add_argument("--skin", type=float, default=0.0)
+    ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0: automatic)")
+    ap.add_argument("--threads", type=int, default=0, help="threads per tile (0: automatic)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="single: take skin / rebuild / tile / threads from the flags instead of "
+                         "timing candidates first")
     ap.add_argument("--chains", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -55,12 +60,14 @@ def parse():
     return ap.parse_args()
 
 
-def run_gpu(w, precision, steps, warmup, batch, rebuild, kernel_steps):
+def run_gpu(w, precision, steps, warmup, batch, rebuild, kernel_steps, tile=0, threads=0):
     from gcr_amd import engine as E
     n = w.n
     pool = E.Pool(2, precision, bead_capacity=n, cell_capacity=4 * n + 1024,
                   chain_capacity=w.phys.n_chains, replica_capacity=1)
     pool.set_schedule(batch, rebuild)
+    pool.set_tile(tile)
+    pool.set_tile_threads(threads)
     rid = pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
     tv = w.time_vals.copy()
     tv[7] = warmup
@@ -187,13 +194,22 @@ def main():
     from gcr_amd import workloads as W
     from gcr_amd._lib import device_info
     w = W.config3(seed=1, n_chains=args.chains)
-    if args.skin > 0:
-        w.phys.skin = args.skin
     dev = device_info()
-    res = {}
+    res, tuned = {}, {}
     for name, prec in (("fp64", E.GCR_FP64), ("mixed", E.GCR_MIXED)):
-        res[name] = run_gpu(w, prec, args.steps, args.warmup, args.batch, args.rebuild,
-                            args.kernel_steps)
+        if args.no_autotune:
+            cfg = dict(skin=args.skin if args.skin > 0 else 0.3 * float(w.phys.wca[3]),
+                       rebuild_every=args.rebuild, tile=args.tile, threads=args.threads)
+        else:
+            from gcr_amd import autotune as AT
+            t0 = time.perf_counter()
+            t = AT.tune(w, prec)
+            cfg = {k: t[k] for k in ("skin", "rebuild_every", "tile", "threads")}
+            cfg["tuning_s"] = time.perf_counter() - t0
+        tuned[name] = cfg
+        w.phys.skin = cfg["skin"]
+        res[name] = run_gpu(w, prec, args.steps, args.warmup, args.batch, cfg["rebuild_every"],
+                            args.kernel_steps, cfg["tile"], cfg["threads"])
     m = res["mixed"]
     kt = m["kernels_ms"]
     dom = max(("stage0", "stage1", "update"), key=lambda k: kt[k])
@@ -215,7 +231,7 @@ def main():
                                "cutoff 2.5 sigma, kappa=5, chirality U[-0.5,0.5]/chain, v0=1, "
                                "T=0.05, dtmax=5e-4, SRK2",
                    "beads": w.n, "box": w.phys.L, "steps_per_graph": args.batch,
-                   "rebuild_every": args.rebuild},
+                   "cell_list": tuned},
         "fp64": {"value": res["fp64"]["bead_steps_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
                  "kernels_ms": res["fp64"]["kernels_ms"]},
         "mixed_over_fp64": m["bead_steps_per_s"] / res["fp64"]["bead_steps_per_s"],

@@ -111,7 +111,7 @@ struct gcr_pool {
     RepParams* dP = nullptr;
     RepState *dS = nullptr, *dSs = nullptr;
     double *X = nullptr, *X1 = nullptr, *F0 = nullptr, *FT = nullptr, *XI = nullptr, *T = nullptr,
-           *Xn = nullptr, *Tn = nullptr, *Xs = nullptr, *Ts = nullptr;
+           *Xn = nullptr, *Tn = nullptr, *Xs = nullptr, *Ts = nullptr, *XIn = nullptr, *X1n = nullptr;
     int32_t *META = nullptr, *LINK = nullptr, *CELL = nullptr, *SLOT = nullptr, *KEYOFF = nullptr,
             *PAIRS = nullptr, *METAn = nullptr, *METAs = nullptr;
     double *CHAIN_A = nullptr, *CHAIN_CS = nullptr;
@@ -124,7 +124,9 @@ struct gcr_pool {
     double* XR = nullptr;
     int64_t capChk = 0, capBlk = 0, capTile = 0;
     int ntile = 0;
-    int tile_side = 0;  // 0: chosen per replica from the cell occupancy
+    int tile_side = 0;     // 0: chosen per replica from the cell occupancy
+    int tile_threads = 0;  // threads per tile workgroup: 256, 512, 0 = from the tile sizes
+    int nt_launch = BS;    // resolved by rebuild_tables
 };
 
 namespace {
@@ -159,17 +161,46 @@ int push_states(gcr_pool* p)
     return GCR_OK;
 }
 
-// Tile edge (cells) of the 2D LDS force kernel: about BS owned beads per
-// workgroup, at most TILE_MAX_ROWS, and never more than the grid.
+// Tile edge (cells) of the 2D LDS force kernel: about 2 BS owned beads per
+// workgroup (fewer, larger tiles stage less halo and leave no partial last
+// wave of workgroups on large grids), at most TILE_MAX_ROWS, the tile with its
+// halo within TILE_MAX_BEADS at the mean occupancy, and never more than the
+// grid.
 int pick_tile(const RepParams& P, int forced)
 {
     int side = forced;
     if (side <= 0) {
-        double occ = (double)P.n / (double)P.ncell;
-        side = (int)std::lround(std::sqrt((double)BS / std::max(occ, 1e-3)));
+        const double occ = std::max((double)P.n / (double)P.ncell, 1e-3);
+        side = (int)std::lround(std::sqrt(2.0 * BS / occ));
+        while (side > 1 && (side + 2) * (side + 2) * occ > 0.7 * TILE_MAX_BEADS) --side;
     }
     side = std::max(1, std::min(side, TILE_MAX_ROWS));
     return std::min(side, P.nc);
+}
+
+// The dispatcher hands a launch's workgroups to the 8 XCDs of the MI355X
+// round-robin by workgroup index, and each XCD has its own L2. order[b] is
+// the tile (index into a row-major list of n tiles) that workgroup
+// first + b runs: each XCD gets one contiguous run of tiles (a band of
+// rows), so neighbouring tiles, which read each other's cells as halo, mostly
+// share an L2.
+std::vector<int> xcd_order(int first, int n)
+{
+    constexpr int NXCD = 8;
+    int cnt[NXCD] = {0}, start[NXCD];
+    for (int b = 0; b < n; ++b) ++cnt[(first + b) % NXCD];
+    int acc = 0;
+    for (int x = 0; x < NXCD; ++x) {
+        start[x] = acc;
+        acc += cnt[x];
+    }
+    std::vector<int> order(n);
+    int seen[NXCD] = {0};
+    for (int b = 0; b < n; ++b) {
+        const int x = (first + b) % NXCD;
+        order[b] = start[x] + seen[x]++;
+    }
+    return order;
 }
 
 // Block, chunk and tile tables of the live replicas.
@@ -194,14 +225,21 @@ int rebuild_tables(gcr_pool* p)
             const int side = pick_tile(P, p->tile_side);
             P.tile_x = P.tile_y = side;
             P.tile_off = (int32_t)(tl.size() / 4);
+            std::vector<int32_t> rowmajor;  // (x, y) of the tiles, row by row
             for (int y = 0; y < P.nc; y += side)
                 for (int x = 0; x < P.nc; x += side) {
-                    tl.push_back(r);
-                    tl.push_back(x);
-                    tl.push_back(y);
-                    tl.push_back(0);
+                    rowmajor.push_back(x);
+                    rowmajor.push_back(y);
                 }
-            P.ntile = (int32_t)(tl.size() / 4) - P.tile_off;
+            const int n = (int)rowmajor.size() / 2;
+            const std::vector<int> order = xcd_order(P.tile_off, n);
+            for (int b = 0; b < n; ++b) {
+                tl.push_back(r);
+                tl.push_back(rowmajor[2 * order[b]]);
+                tl.push_back(rowmajor[2 * order[b] + 1]);
+                tl.push_back(0);
+            }
+            P.ntile = n;
             P.red_off = P.tile_off;
             P.red_n = P.ntile;
         } else {
@@ -210,6 +248,14 @@ int rebuild_tables(gcr_pool* p)
         }
     }
     rco[p->capR] = (int32_t)cr.size();
+    if (p->dim == 2) {
+        // 512 threads when tiles average 384 or more owned beads
+        int64_t beads = 0;
+        for (int r = 0; r < p->capR; ++r)
+            if (p->live[r]) beads += p->P[r].n;
+        const int64_t nt = (int64_t)tl.size() / 4;
+        p->nt_launch = p->tile_threads ? p->tile_threads : (nt && beads >= 384 * nt ? 512 : 256);
+    }
     if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk ||
         (int64_t)tl.size() / 4 > p->capTile)
         return fail(GCR_ERR_CAPACITY, "block table capacity exceeded");
@@ -255,10 +301,13 @@ void launch_rebuild(gcr_pool* p, int j)
         k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         if (p->ntile) {
+            const bool big = p->nt_launch == 512;
             if (p->precision == GCR_MIXED)
-                k_nlist_tile2d<float><<<p->ntile, BS, 0, st>>>(a, j);
+                big ? k_nlist_tile2d<float, 512><<<p->ntile, 512, 0, st>>>(a, j)
+                    : k_nlist_tile2d<float, 256><<<p->ntile, 256, 0, st>>>(a, j);
             else
-                k_nlist_tile2d<double><<<p->ntile, BS, 0, st>>>(a, j);
+                big ? k_nlist_tile2d<double, 512><<<p->ntile, 512, 0, st>>>(a, j)
+                    : k_nlist_tile2d<double, 256><<<p->ntile, 256, 0, st>>>(a, j);
         }
     } else {
         k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
@@ -270,13 +319,24 @@ void launch_rebuild(gcr_pool* p, int j)
 // records its start and end (hipExtLaunchKernelGGL), which times the kernel
 // alone on the pool's stream.
 template <typename K>
-void launch_k(K kernel, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, const PoolArgs& a, int j)
+void launch_k(K kernel, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, const PoolArgs& a, int j,
+              int block = BS)
 {
     if (grid <= 0) return;
     if (ev0)
-        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(BS), 0, st, ev0, ev1, 0, a, j);
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, ev0, ev1, 0, a, j);
     else
-        kernel<<<grid, BS, 0, st>>>(a, j);
+        kernel<<<grid, block, 0, st>>>(a, j);
+}
+
+// 2D tile kernels at the pool's threads per tile.
+template <typename FT, int STAGE>
+void launch_tile(gcr_pool* p, int j, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (p->nt_launch == 512)
+        launch_k(k_stage_tile2d<FT, STAGE, 512>, p->ntile, p->stream, ev0, ev1, p->a, j, 512);
+    else
+        launch_k(k_stage_tile2d<FT, STAGE, 256>, p->ntile, p->stream, ev0, ev1, p->a, j, 256);
 }
 
 void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
@@ -288,11 +348,9 @@ void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEv
         launch_k(k_finalize, p->capR, st, ev0, ev1, a, j);
     } else if (p->dim == 2) {
         if (which == 0)
-            mixed ? launch_k(k_stage_tile2d<float, 0>, p->ntile, st, ev0, ev1, a, j)
-                  : launch_k(k_stage_tile2d<double, 0>, p->ntile, st, ev0, ev1, a, j);
+            mixed ? launch_tile<float, 0>(p, j, ev0, ev1) : launch_tile<double, 0>(p, j, ev0, ev1);
         else if (which == 1)
-            mixed ? launch_k(k_stage_tile2d<float, 1>, p->ntile, st, ev0, ev1, a, j)
-                  : launch_k(k_stage_tile2d<double, 1>, p->ntile, st, ev0, ev1, a, j);
+            mixed ? launch_tile<float, 1>(p, j, ev0, ev1) : launch_tile<double, 1>(p, j, ev0, ev1);
         else
             launch_k(k_update<2>, p->nblk, st, ev0, ev1, a, j);
     } else {
@@ -459,11 +517,11 @@ int gcr_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_by
 
 int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_cells)
 {
-    // per bead: 9 dim-vectors + noise (2 dim) of doubles, 5 int pairs, links,
+    // per bead: 10 dim-vectors + noise (2 x 2 dim) of doubles, 5 int pairs, links,
     // cell and slot ints; 2D adds the neighbour list, its length and the
     // re-sort positions; per chain: 4 doubles + 2 ints; per cell: 2 ints
     const int64_t nl = dim == 2 ? 2 * NL_MAX + 4 + 8 * dim : 0;
-    return n * (8 * (9 * dim + 2 * dim) + 4 * (5 * 2 + 4 + 2) + nl) + n_chains * (8 * 4 + 8) +
+    return n * (8 * (10 * dim + 4 * dim) + 4 * (5 * 2 + 4 + 2) + nl) + n_chains * (8 * 4 + 8) +
            (n_cells + 1) * 8;
 }
 
@@ -503,7 +561,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->xi_dev.assign(capR, nullptr);
     p->capBlk = capB / BS + capR + 1;
     p->capChk = capC / SCAN_CHUNK + capR + 1;
-    p->capTile = dim == 2 ? capC / 2 + 4 * (int64_t)capR + 1 : 1;
+    p->capTile = dim == 2 ? capC + (int64_t)capR + 1 : 1;  // tiles of one cell at worst
     int rc = GCR_OK;
     auto A = [&](int r) {
         if (r != GCR_OK && rc == GCR_OK) rc = r;
@@ -517,9 +575,10 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->dP, capR));
     A(dalloc(&p->dS, capR));
     A(dalloc(&p->dSs, capR));
-    for (double** q : {&p->X, &p->X1, &p->F0, &p->FT, &p->T, &p->Xn, &p->Tn, &p->Xs, &p->Ts})
+    for (double** q : {&p->X, &p->X1, &p->F0, &p->FT, &p->T, &p->Xn, &p->Tn, &p->Xs, &p->Ts, &p->X1n})
         A(dalloc(q, nb * dim));
     A(dalloc(&p->XI, nb * 2 * dim));
+    A(dalloc(&p->XIn, nb * 2 * dim));
     for (int32_t** q : {&p->META, &p->KEYOFF, &p->PAIRS, &p->METAn, &p->METAs}) A(dalloc(q, nb * 2));
     A(dalloc(&p->LINK, nb * 4));
     A(dalloc(&p->CELL, nb));
@@ -589,6 +648,8 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.Xn = p->Xn;
     a.Tn = p->Tn;
     a.METAn = p->METAn;
+    a.XIn = p->XIn;
+    a.X1n = p->X1n;
     a.Xs = p->Xs;
     a.Ts = p->Ts;
     a.METAs = p->METAs;
@@ -914,8 +975,7 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
     launch_rebuild(p, -1);
     const bool mixed = p->precision == GCR_MIXED;
     if (p->dim == 2)
-        mixed ? k_stage_tile2d<float, 2><<<p->ntile, BS, 0, p->stream>>>(p->a, rid)
-              : k_stage_tile2d<double, 2><<<p->ntile, BS, 0, p->stream>>>(p->a, rid);
+        mixed ? launch_tile<float, 2>(p, rid, nullptr, nullptr) : launch_tile<double, 2>(p, rid, nullptr, nullptr);
     else
         mixed ? k_eval<3, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
               : k_eval<3, double><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid);
@@ -988,6 +1048,16 @@ int gcr_pool_set_tile(gcr_pool* p, int32_t side)
     return GCR_OK;
 }
 
+int gcr_pool_set_tile_threads(gcr_pool* p, int32_t threads)
+{
+    if (!p) return fail(GCR_ERR_ARG, "null pool");
+    if (threads != 0 && threads != 256 && threads != 512)
+        return fail(GCR_ERR_ARG, "threads per tile must be 0, 256 or 512");
+    p->tile_threads = threads;
+    p->tables_dirty = true;
+    return GCR_OK;
+}
+
 int gcr_pool_destroy(gcr_pool* p)
 {
     if (!p) return GCR_OK;
@@ -997,6 +1067,7 @@ int gcr_pool_destroy(gcr_pool* p)
         if (q) (void)hipFree(q);
     void* bufs[] = {p->blk_rep, p->blk_slot0, p->chk_rep, p->chk_cell0, p->rep_chk_off, p->dP, p->dS,
                     p->dSs, p->X, p->X1, p->F0, p->FT, p->XI, p->T, p->Xn, p->Tn, p->Xs, p->Ts,
+                    p->XIn, p->X1n,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
                     p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR};

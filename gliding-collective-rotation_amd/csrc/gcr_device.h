@@ -88,7 +88,8 @@ struct PoolArgs {
     double* X1;    // [slot][dim] predictor positions
     double* F0;    // [slot][dim] stage-0 forces
     double* FT;    // [slot][dim] F0 + F1
-    double* XI;    // [orig_global][2*dim] noise of the next / running step
+    double* XI;    // [slot][2][dim] noise of the next / running step
+                   // (translation, rotation); moves with the bead at re-sorts
     double* T;     // [slot][dim] propulsion direction (ABP / diffusing state)
     int32_t* META; // [slot][2] (orig_global, chain_global)
     int32_t* LINK; // [slot][4] (prev2, prev1, next1, next2) slots, -1 if none
@@ -114,6 +115,8 @@ struct PoolArgs {
     double* Xn;
     double* Tn;
     int32_t* METAn;
+    double* XIn;
+    double* X1n;
     // rollback snapshot
     double* Xs;
     double* Ts;

@@ -109,14 +109,13 @@ __device__ __forceinline__ void bead_noise(uint64_t seed, uint32_t orig, int64_t
 }
 
 // Noise of bead `orig_global` for the replica's step with call-local index
-// `steps` and global index `step_global`, into XI[orig_global] (indexed by
-// original bead, so cell re-sorts never move it): drawn in-kernel, or copied
+// `steps` and global index `step_global`: drawn in-kernel (keyed by the
+// original bead index, so it does not depend on the slot layout), or read
 // from the injected array (zeros past its end: that step never runs).
 template <int DIM>
-__device__ __forceinline__ void make_noise(const PoolArgs& a, const RepParams& P, int orig_global,
-                                           int64_t steps, int64_t step_global)
+__device__ __forceinline__ void draw_noise(const RepParams& P, int orig_global, int64_t steps,
+                                           int64_t step_global, double* xi)
 {
-    double xi[2 * DIM];
     const int orig_local = orig_global - P.bead_off;
     if (P.flags & GCR_F_INJECT) {
         if (steps < P.xi_steps) {
@@ -131,9 +130,6 @@ __device__ __forceinline__ void make_noise(const PoolArgs& a, const RepParams& P
         bead_noise<DIM>(P.seed, (uint32_t)orig_local, step_global, (P.flags & GCR_F_ROTDIFF) != 0,
                         xi);
     }
-    double* dst = a.XI + (size_t)orig_global * 2 * DIM;
-#pragma unroll
-    for (int i = 0; i < 2 * DIM; ++i) dst[i] = xi[i];
 }
 
 // Per-block running condition of a replica for local step j of a batch.
@@ -179,6 +175,18 @@ __device__ __forceinline__ void store_vec(double* A, int s, const double* v)
         A[3 * s + 1] = v[1];
         A[3 * s + 2] = v[2];
     }
+}
+
+// Noise of the bead in slot s for the given step into XI[s]: translation,
+// and rotation when the update uses it.
+template <int DIM>
+__device__ __forceinline__ void store_noise(const PoolArgs& a, const RepParams& P, int s, int64_t steps,
+                                            int64_t step_global)
+{
+    double xi[2 * DIM];
+    draw_noise<DIM>(P, a.META[2 * s], steps, step_global, xi);
+    store_vec<DIM>(a.XI, 2 * s, xi);
+    if (P.flags & GCR_F_ROTDIFF) store_vec<DIM>(a.XI, 2 * s + 1, xi + DIM);
 }
 
 // Cell coordinate of a position along one axis, clamped into [0, nc).
@@ -423,7 +431,7 @@ struct PairLaw<double> {
         double iv = inv(ds);
         double i2 = iv * iv;
         double i3 = i2 * iv;
-        return (i2 * i2) * (p0 - p1 * i3);
+        return (i2 * i2) * fma(-p1, i3, p0);
     }
     static __device__ __forceinline__ double coeff(double ds, const RepParams& P)
     {
@@ -438,7 +446,7 @@ struct PairLaw<float> {
         float iv = __builtin_amdgcn_rcpf(ds);
         float i2 = iv * iv;
         float i3 = i2 * iv;
-        return (i2 * i2) * (p0 - p1 * i3);
+        return (i2 * i2) * fmaf(-p1, i3, p0);
     }
     static __device__ __forceinline__ float coeff(float ds, const RepParams& P)
     {
@@ -446,68 +454,158 @@ struct PairLaw<float> {
     }
 };
 
+// Scalars of the pair law, held in registers for a whole pair loop.
+template <typename FT>
+struct PairConsts {
+    FT h, rc2, p0, p1, shift;
+    bool shifted;
+};
+
+template <typename FT>
+__device__ __forceinline__ PairConsts<FT> pair_consts(const RepParams& P)
+{
+    PairConsts<FT> k;
+    k.h = (FT)P.h;
+    k.rc2 = (FT)P.rc2;
+    k.p0 = (FT)P.wca[0];
+    k.p1 = (FT)P.wca[1];
+    k.shift = (FT)P.wca[2];
+    k.shifted = P.wca[2] != 0.0;
+    return k;
+}
+
+// Separation along one axis from positions relative to their cell centres
+// (rounded to FT) and the cell offset dc in {-1, 0, 1}: dc * h is exact, so
+// the result carries one rounding of a cell-sized difference. Every pair path
+// (LDS lists, global cell scan) goes through this, in the same operations.
+template <typename FT>
+__device__ __forceinline__ FT cell_sep(FT dc, FT h, FT rel_j, FT rel_i)
+{
+    return fma(dc, h, rel_j - rel_i);
+}
+
+// Force of one candidate at separation sep (pointing to the partner), added
+// to f. Branch-free: the cutoff and zero-distance mask keep every lane on one
+// path (forces.py:96-118).
+template <int DIM, typename FT>
+__device__ __forceinline__ void pair_term(const FT* sep, const PairConsts<FT>& k, FT* f)
+{
+    FT ds = sep[0] * sep[0];
+#pragma unroll
+    for (int i = 1; i < DIM; ++i) ds = fma(sep[i], sep[i], ds);
+    const bool use = (ds <= k.rc2) & (ds > (FT)0);
+    const FT cf = PairLaw<FT>::coeff(use ? ds : (FT)1, k.p0, k.p1);
+    const FT w = use ? cf : (FT)0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) f[i] = fma(sep[i], w, f[i]);
+}
+
+// WCA_shift (forces.py:123-131): the cutoff applies to the unshifted distance.
+template <int DIM, typename FT>
+__device__ __forceinline__ void pair_term_shifted(const FT* sep_in, const PairConsts<FT>& k, FT* f)
+{
+    FT sep[DIM];
+    FT ds = sep_in[0] * sep_in[0];
+#pragma unroll
+    for (int i = 1; i < DIM; ++i) ds = fma(sep_in[i], sep_in[i], ds);
+    if (!(ds <= k.rc2 && ds > (FT)0)) return;
+    const FT mag = sqrt(ds);
+    FT nm = mag + k.shift;
+    nm = nm < (FT)0 ? (FT)0 : nm;
+    const FT rs = nm / mag;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) sep[i] = sep_in[i] * rs;
+    ds = nm * nm;
+    const FT cf = PairLaw<FT>::coeff(ds, k.p0, k.p1);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) f[i] = fma(sep[i], cf, f[i]);
+}
+
+// Position relative to the centre of cell coordinate c along one axis. The
+// fma is explicit: left to contraction, a centre hoisted out of a loop would
+// be rounded on its own in one path and fused in another.
+__device__ __forceinline__ double cell_rel(double x, int c, const RepParams& P)
+{
+    return wrap1(fma(-((double)c + 0.5), P.h, x), P.L, P.halfL);
+}
+
+// Pair forces of the bead at xs (binned in `cell`) over the surrounding
+// cells, from global memory, summed in FT into f in the order cells (z, y, x),
+// then slots -- the order of the neighbour lists. With 3 or more cells per
+// side the separations are formed as in the LDS tiles (cell_sep), so a bead
+// gets the same sum from either path; smaller grids scan every cell with the
+// plain minimum-image difference.
 template <int DIM, typename FT>
 __device__ __forceinline__ void pair_sum(const double* X, const int32_t* META, const int32_t* CSTART,
                                          const double* xs, int orig, int chain, int cell,
-                                         const RepParams& P, double* acc)
+                                         const RepParams& P, FT* f)
 {
     const int nc = P.nc;
     int cc[3] = {cell % nc, (cell / nc) % nc, DIM == 3 ? cell / (nc * nc) : 0};
-    // neighbour stencil per axis: the 3 surrounding cells, or every cell when
-    // the grid has fewer than 3 cells on that axis
-    const int lo = nc >= 3 ? -1 : 0;
-    const int hi = nc >= 3 ? 1 : nc - 1;
+    const bool rel = nc >= 3;
+    const int lo = rel ? -1 : 0;
+    const int hi = rel ? 1 : nc - 1;
     const int zlo = DIM == 3 ? lo : 0, zhi = DIM == 3 ? hi : 0;
-    const bool shifted = P.wca[2] != 0.0;
+    const PairConsts<FT> k = pair_consts<FT>(P);
+    FT ri[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) ri[i] = (FT)cell_rel(xs[i], cc[i], P);
     for (int dz = zlo; dz <= zhi; ++dz) {
-        int z = DIM == 3 ? (nc >= 3 ? (cc[2] + dz + nc) % nc : dz) : 0;
+        const int z = DIM == 3 ? (rel ? (cc[2] + dz + nc) % nc : dz) : 0;
         for (int dy = lo; dy <= hi; ++dy) {
-            int y = nc >= 3 ? (cc[1] + dy + nc) % nc : dy;
+            const int y = rel ? (cc[1] + dy + nc) % nc : dy;
             for (int dx = lo; dx <= hi; ++dx) {
-                int x = nc >= 3 ? (cc[0] + dx + nc) % nc : dx;
-                int c2 = P.cell_off + (z * nc + y) * nc + x;
-                int t0 = CSTART[c2], t1 = CSTART[c2 + 1];
+                const int x = rel ? (cc[0] + dx + nc) % nc : dx;
+                const int cj[3] = {x, y, z};
+                const FT dc[3] = {(FT)dx, (FT)dy, (FT)dz};
+                const int c2 = P.cell_off + (z * nc + y) * nc + x;
+                const int t0 = CSTART[c2], t1 = CSTART[c2 + 1];
                 for (int t = t0; t < t1; ++t) {
-                    int2 mj = *reinterpret_cast<const int2*>(META + 2 * t);
+                    const int2 mj = *reinterpret_cast<const int2*>(META + 2 * t);
                     if (mj.y == chain) {
                         if (P.n_nearby < 0) continue;
-                        int dd = mj.x - orig;
+                        const int dd = mj.x - orig;
                         if ((dd < 0 ? -dd : dd) <= P.n_nearby) continue;
                     }
-                    double xj[DIM], sep[DIM];
+                    double xj[DIM];
                     load_vec<DIM>(X, t, xj);
-                    double ds = 0.0;
+                    FT sep[DIM];
 #pragma unroll
-                    for (int i = 0; i < DIM; ++i) {
-                        sep[i] = wrap1(xj[i] - xs[i], P.L, P.halfL);
-                        ds += sep[i] * sep[i];
-                    }
-                    if (!(0.0 < ds && ds <= P.rc2)) continue;
-                    if (shifted) {  // WCA_shift, forces.py:123-131
-                        double mag = sqrt(ds);
-                        double nm = mag + P.wca[2];
-                        nm = nm < 0.0 ? 0.0 : nm;
-                        double rs = nm / mag;
-#pragma unroll
-                        for (int i = 0; i < DIM; ++i) sep[i] *= rs;
-                        ds = nm * nm;
-                    }
-                    FT c = PairLaw<FT>::coeff((FT)ds, P);
-#pragma unroll
-                    for (int i = 0; i < DIM; ++i) acc[i] += (double)((FT)sep[i] * c);
+                    for (int i = 0; i < DIM; ++i)
+                        sep[i] = rel ? cell_sep<FT>(dc[i], k.h, (FT)cell_rel(xj[i], cj[i], P), ri[i])
+                                     : (FT)wrap1(xj[i] - xs[i], P.L, P.halfL);
+                    if (k.shifted)
+                        pair_term_shifted<DIM, FT>(sep, k, f);
+                    else
+                        pair_term<DIM, FT>(sep, k, f);
                 }
             }
         }
     }
 }
 
+// Pair forces added to an FP64 accumulator.
+template <int DIM, typename FT>
+__device__ __forceinline__ void add_pair_sum(const double* X, const int32_t* META, const int32_t* CSTART,
+                                             const double* xs, int orig, int chain, int cell,
+                                             const RepParams& P, double* acc)
+{
+    FT f[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) f[i] = (FT)0;
+    pair_sum<DIM, FT>(X, META, CSTART, xs, orig, chain, cell, P, f);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) acc[i] += (double)f[i];
+}
+
 // ------------------------------------------------------------------ stages
 
 // Max over the block (wave64 shuffles, then the block's waves through LDS);
 // the result is valid in thread 0.
+template <int NT = BS>
 __device__ __forceinline__ double block_max(double v)
 {
-    __shared__ double red[BS / 64];
+    __shared__ double red[NT / 64];
     for (int off = 32; off > 0; off >>= 1) {
         double o = __shfl_xor(v, off, 64);
         v = o > v ? o : v;
@@ -516,7 +614,7 @@ __device__ __forceinline__ double block_max(double v)
     if (l == 0) red[w] = v;
     __syncthreads();
     double m = red[0];
-    for (int i = 1; i < BS / 64; ++i) m = red[i] > m ? red[i] : m;
+    for (int i = 1; i < NT / 64; ++i) m = red[i] > m ? red[i] : m;
     __syncthreads();
     return m;
 }
@@ -547,9 +645,10 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
                                               const double* acc)
 {
     if (STAGE == 0) {
-        // this step's noise, drawn by the previous update (or the batch start)
-        double xi[2 * DIM];
-        load_vec<DIM>(a.XI + (size_t)orig_global * 2 * DIM, 0, xi);
+        // this step's translational noise, drawn by the previous update (or
+        // the batch start) and carried through re-sorts
+        double xi[DIM];
+        load_vec<DIM>(a.XI, 2 * s, xi);
         store_vec<DIM>(a.F0, s, acc);
         const double dt = S->dt;
         double dt_eff = dt * 1.0;
@@ -606,7 +705,7 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
             bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
         }
         if (P.flags & GCR_F_WCA)
-            pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
+            add_pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
         if (P.flags & GCR_F_PROPEL) {
             if (!tang_set) load_vec<DIM>(a.T, s, tang);
             propel<DIM>(a, me.y, tang, acc);
@@ -666,9 +765,7 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
     double x[DIM], f[DIM], xi[2 * DIM];
     load_vec<DIM>(a.X, s, x);
     load_vec<DIM>(a.FT, s, f);
-    const int orig = a.META[2 * s];
-#pragma unroll
-    for (int i = 0; i < 2 * DIM; ++i) xi[i] = a.XI[(size_t)orig * 2 * DIM + i];
+    load_vec<DIM>(a.XI, 2 * s, xi);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         double tmp = x[i] + dt_eff * f[i];
@@ -681,6 +778,7 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
         const double sr = sqrt(2.0 * dt_used * P.Dr) * 3.141592653589793;
         double t[DIM], nr[DIM], proj = 0.0;
         load_vec<DIM>(a.T, s, t);
+        load_vec<DIM>(a.XI, 2 * s + 1, xi + DIM);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) nr[i] = xi[DIM + i];
 #pragma unroll
@@ -698,9 +796,9 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
         for (int i = 0; i < DIM; ++i) t[i] /= mag;
         store_vec<DIM>(a.T, s, t);
     }
-    // the next step's noise (finalize already advanced the step counters);
-    // computed here, where the update leaves the ALUs idle
-    make_noise<DIM>(a, P, orig, S->steps, S->step_global);
+    // the next step's noise (finalize already advanced the step counters),
+    // drawn here where the ALUs would otherwise wait on memory
+    store_noise<DIM>(a, P, s, S->steps, S->step_global);
 }
 
 // One force evaluation without an update (parity checks of the force path):
@@ -726,7 +824,7 @@ __global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
         bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
     }
     if (P.flags & GCR_F_WCA)
-        pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
+        add_pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
     if (!tang_set) load_vec<DIM>(a.T, s, tang);
     if (P.flags & GCR_F_PROPEL) propel<DIM>(a, me.y, tang, acc);
     store_vec<DIM>(a.FT, s, acc);
@@ -775,9 +873,10 @@ __global__ void __launch_bounds__(BS) k_bin(PoolArgs a, int j)
 }
 
 // Inclusive wave64 scan with shuffles, exclusive block scan through LDS.
+template <int NT = BS>
 __device__ __forceinline__ int block_exclusive_scan(int v, int* total)
 {
-    __shared__ int wsum[BS / 64];
+    __shared__ int wsum[NT / 64];
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     int inc = v;
 #pragma unroll
@@ -789,7 +888,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total)
     __syncthreads();
     int base = 0, tot = 0;
 #pragma unroll
-    for (int i = 0; i < BS / 64; ++i) {
+    for (int i = 0; i < NT / 64; ++i) {
         if (i < w) base += wsum[i];
         tot += wsum[i];
     }
@@ -906,6 +1005,15 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
     *reinterpret_cast<int2*>(a.METAn + 2 * d) = *reinterpret_cast<const int2*>(a.META + 2 * so);
     a.CELL[d] = key;
     a.SLOT[pr.x] = d;
+    // the pending noise and the last predictor travel with the bead
+    load_vec<DIM>(a.XI, 2 * so, x);
+    store_vec<DIM>(a.XIn, 2 * d, x);
+    if (P.flags & GCR_F_ROTDIFF) {
+        load_vec<DIM>(a.XI, 2 * so + 1, x);
+        store_vec<DIM>(a.XIn, 2 * d + 1, x);
+    }
+    load_vec<DIM>(a.X1, so, x);
+    store_vec<DIM>(a.X1n, d, x);
 }
 
 template <int DIM>
@@ -935,6 +1043,14 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
         store_vec<DIM>(a.T, ns, t);
     }
     *reinterpret_cast<int2*>(a.META + 2 * ns) = me;
+    load_vec<DIM>(a.XIn, 2 * ns, x);
+    store_vec<DIM>(a.XI, 2 * ns, x);
+    if (P.flags & GCR_F_ROTDIFF) {
+        load_vec<DIM>(a.XIn, 2 * ns + 1, x);
+        store_vec<DIM>(a.XI, 2 * ns + 1, x);
+    }
+    load_vec<DIM>(a.X1n, ns, x);
+    store_vec<DIM>(a.X1, ns, x);
 }
 
 // ------------------------------------------------------------ batch control
@@ -957,7 +1073,7 @@ __global__ void __launch_bounds__(BS) k_batch_begin(PoolArgs a, int with_t)
     if (s >= P.bead_off + P.n) return;
     // the first step of the batch needs its noise (after a begin or a replay
     // nothing else has drawn it)
-    make_noise<DIM>(a, P, a.META[2 * s], S->steps, S->step_global);
+    store_noise<DIM>(a, P, s, S->steps, S->step_global);
     double x[DIM];
     load_vec<DIM>(a.X, s, x);
     store_vec<DIM>(a.Xs, s, x);

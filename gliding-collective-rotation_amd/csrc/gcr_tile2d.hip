@@ -14,178 +14,102 @@
 // the skin since the re-sort; the stages check that and flag the replica for
 // a replay otherwise.
 //
-// LDS coordinates: mixed form, FP32 position relative to the centre of the
-// bead's cell plus the halo coordinates of that cell (small integers), so a
-// separation is (dcell * h) + (dx), both accurate to the FP32 ulp of a cell
-// edge; FP64 form, the position relative to the tile's corner cell.
+// LDS coordinates: position relative to the centre of the bead's cell (FP32
+// in the mixed form, FP64 in the plain one) plus the halo coordinates of that
+// cell, so a separation is (dcell * h) + (dx), accurate to the ulp of a cell
+// edge, and identical to the one the global cell scan forms.
 // Bonded terms, propulsion and the SRK2 stage update are the same device
 // functions as k_stage; outputs are one write per bead. A bead whose list
 // overflowed, and every bead of a tile that does not fit in LDS (or of a box
 // of fewer than 3 cells per side), sums its pairs from global memory with
 // pair_sum instead.
 #define TILE_MAX_HALO 400    // (tile_x + 2) * (tile_y + 2) cells
-#define TILE_MAX_BEADS 1024  // beads of a tile with its halo held in LDS
+#define TILE_MAX_BEADS 1536  // beads of a tile with its halo held in LDS
 #define TILE_MAX_ROWS 16
 #define NL_MAX 64            // neighbour-list entries per bead (16 groups of 4)
 
 namespace gcr {
 
-// LDS position record of a staged bead.
+// LDS position record of a staged bead: position relative to the centre of
+// the bead's cell (rounded to FT) and the halo coordinates of that cell
+// (small integers, as floats). Separations are formed by cell_sep from these,
+// as the global cell scan forms them, so the sums do not depend on the tile
+// shape or on which path computed them.
 template <typename FT>
 struct LPos;
 
 template <>
 struct alignas(16) LPos<float> {
-    float x, y;    // relative to the centre of the bead's cell
-    float cx, cy;  // halo coordinates of that cell
+    float x, y;
+    float cx, cy;
 };
 
 template <>
-struct alignas(16) LPos<double> {
-    double x, y;  // relative to the centre of the tile's halo cell (0, 0)
+struct alignas(8) LPos<double> {
+    double x, y;
+    float cx, cy;
 };
 
 template <typename FT>
-__device__ __forceinline__ LPos<FT> make_lpos(double rx, double ry, int hx, int hy, double h);
-
-template <>
-__device__ __forceinline__ LPos<float> make_lpos<float>(double rx, double ry, int hx, int hy, double)
+__device__ __forceinline__ LPos<FT> make_lpos(double rx, double ry, int hx, int hy)
 {
-    LPos<float> q;
-    q.x = (float)rx;
-    q.y = (float)ry;
+    LPos<FT> q;
+    q.x = (FT)rx;
+    q.y = (FT)ry;
     q.cx = (float)hx;
     q.cy = (float)hy;
     return q;
 }
 
-template <>
-__device__ __forceinline__ LPos<double> make_lpos<double>(double rx, double ry, int hx, int hy, double h)
-{
-    LPos<double> q;
-    q.x = rx + hx * h;
-    q.y = ry + hy * h;
-    return q;
-}
-
-// Separation q - me. The cell offsets are -1, 0 or +1 for every listed pair,
-// so (dc * h) is exact in FP32.
-__device__ __forceinline__ void separation(const LPos<float>& q, const LPos<float>& me, float h, float& sx,
-                                           float& sy)
-{
-    sx = fmaf(q.cx - me.cx, h, q.x - me.x);
-    sy = fmaf(q.cy - me.cy, h, q.y - me.y);
-}
-
-__device__ __forceinline__ void separation(const LPos<double>& q, const LPos<double>& me, double,
-                                           double& sx, double& sy)
-{
-    sx = q.x - me.x;
-    sy = q.y - me.y;
-}
-
-// Scalars of the pair law, held in registers for the whole loop.
+// Separation q - me; the cell offsets are -1, 0 or +1 for every listed pair.
 template <typename FT>
-struct PairConsts {
-    FT h, rc2, p0, p1, shift;
-    bool shifted;
-};
-
-template <typename FT>
-__device__ __forceinline__ PairConsts<FT> pair_consts(const RepParams& P)
+__device__ __forceinline__ void separation(const LPos<FT>& q, const LPos<FT>& me, FT h, FT* sep)
 {
-    PairConsts<FT> k;
-    k.h = (FT)P.h;
-    k.rc2 = (FT)P.rc2;
-    k.p0 = (FT)P.wca[0];
-    k.p1 = (FT)P.wca[1];
-    k.shift = (FT)P.wca[2];
-    k.shifted = P.wca[2] != 0.0;
-    return k;
+    sep[0] = cell_sep<FT>((FT)(q.cx - me.cx), h, q.x, me.x);
+    sep[1] = cell_sep<FT>((FT)(q.cy - me.cy), h, q.y, me.y);
 }
 
-// One listed candidate: branch-free (cutoff and zero-distance mask), so all
-// lanes follow one path and the LDS reads pipeline.
-template <typename FT>
-__device__ __forceinline__ void pair_term(const LPos<FT>& q, const LPos<FT>& me, const PairConsts<FT>& k,
-                                          FT& fx, FT& fy)
-{
-    FT sx, sy;
-    separation(q, me, k.h, sx, sy);
-    const FT ds = sx * sx + sy * sy;
-    const bool use = (ds <= k.rc2) & (ds > (FT)0);
-    const FT cf = PairLaw<FT>::coeff(use ? ds : (FT)1, k.p0, k.p1);
-    const FT w = use ? cf : (FT)0;
-    fx += sx * w;
-    fy += sy * w;
-}
-
-// WCA_shift (forces.py:123-131): the cutoff applies to the unshifted distance.
-template <typename FT>
-__device__ __forceinline__ void pair_term_shifted(const LPos<FT>& q, const LPos<FT>& me,
-                                                  const PairConsts<FT>& k, FT& fx, FT& fy)
-{
-    FT sx, sy;
-    separation(q, me, k.h, sx, sy);
-    FT ds = sx * sx + sy * sy;
-    if (!(ds <= k.rc2 && ds > (FT)0)) return;
-    const FT mag = sqrt(ds);
-    FT nm = mag + k.shift;
-    nm = nm < (FT)0 ? (FT)0 : nm;
-    const FT rs = nm / mag;
-    sx *= rs;
-    sy *= rs;
-    ds = nm * nm;
-    const FT cf = PairLaw<FT>::coeff(ds, k.p0, k.p1);
-    fx += sx * cf;
-    fy += sy * cf;
-}
-
-// Pair forces of one bead over its list (len entries, groups of 4 LDS
-// indices at NL[g * stride + s]; the last group is padded with the bead's own
-// index, which the zero-distance mask drops). Summed in FT.
 template <typename FT>
 struct FT2 {
     FT x, y;
 };
 
+// Pair forces of one bead over its list (len entries, groups of 4 LDS
+// indices at NL[g * stride + s]; the last group is padded with the bead's own
+// index, which the zero-distance mask drops). Summed in FT.
 template <typename FT>
 __device__ __forceinline__ FT2<FT> list_pair_sum(const LPos<FT>* lp, const uint2* G, int64_t stride, int len,
                                                  const LPos<FT> me, const PairConsts<FT>& k)
 {
-    FT fx = (FT)0, fy = (FT)0;
+    FT f[2] = {(FT)0, (FT)0};
     const int ng = (len + 3) >> 2;
-    if (!k.shifted) {
-        uint32_t e0 = 0u, e1 = 0u;
-        if (ng > 0) {
-            const uint2 e = G[0];
-            e0 = e.x;
-            e1 = e.y;
-        }
-        for (int g = 0; g < ng; ++g) {
-            uint32_t n0 = e0, n1 = e1;
-            if (g + 1 < ng) {  // next group in flight during this one
-                const uint2 en = G[(int64_t)(g + 1) * stride];
-                n0 = en.x;
-                n1 = en.y;
-            }
-            pair_term<FT>(lp[e0 & 0xffffu], me, k, fx, fy);
-            pair_term<FT>(lp[e0 >> 16], me, k, fx, fy);
-            pair_term<FT>(lp[e1 & 0xffffu], me, k, fx, fy);
-            pair_term<FT>(lp[e1 >> 16], me, k, fx, fy);
-            e0 = n0;
-            e1 = n1;
-        }
-    } else {
-        for (int g = 0; g < ng; ++g) {
-            const uint2 e = G[(int64_t)g * stride];
-            pair_term_shifted<FT>(lp[e.x & 0xffffu], me, k, fx, fy);
-            pair_term_shifted<FT>(lp[e.x >> 16], me, k, fx, fy);
-            pair_term_shifted<FT>(lp[e.y & 0xffffu], me, k, fx, fy);
-            pair_term_shifted<FT>(lp[e.y >> 16], me, k, fx, fy);
-        }
+    uint32_t e0 = 0u, e1 = 0u;
+    if (ng > 0) {
+        const uint2 e = G[0];
+        e0 = e.x;
+        e1 = e.y;
     }
-    return FT2<FT>{fx, fy};
+    for (int g = 0; g < ng; ++g) {
+        uint32_t n0 = e0, n1 = e1;
+        if (g + 1 < ng) {  // next group in flight during this one
+            const uint2 en = G[(int64_t)(g + 1) * stride];
+            n0 = en.x;
+            n1 = en.y;
+        }
+        const uint32_t idx[4] = {e0 & 0xffffu, e0 >> 16, e1 & 0xffffu, e1 >> 16};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            FT sep[2];
+            separation<FT>(lp[idx[u]], me, k.h, sep);
+            if (k.shifted)
+                pair_term_shifted<2, FT>(sep, k, f);
+            else
+                pair_term<2, FT>(sep, k, f);
+        }
+        e0 = n0;
+        e1 = n1;
+    }
+    return FT2<FT>{f[0], f[1]};
 }
 
 // Geometry of a tile and its halo in LDS, shared by the list builder and the
@@ -203,6 +127,7 @@ struct TileShared {
 
 // Halo bead counts -> LDS offsets (two cells per thread), owned-bead row
 // prefix; ends with a barrier.
+template <int NT>
 __device__ __forceinline__ TileLayout tile_layout(const PoolArgs& a, const RepParams& P, int4 tile,
                                                   TileShared& sh)
 {
@@ -229,7 +154,7 @@ __device__ __forceinline__ TileLayout tile_layout(const PoolArgs& a, const RepPa
         }
         sum += cnt[k];
     }
-    int ex = block_exclusive_scan(sum, &t.total);
+    int ex = block_exclusive_scan<NT>(sum, &t.total);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int c = threadIdx.x * 2 + k;
@@ -272,22 +197,20 @@ __device__ __forceinline__ void staged_bead(const TileShared& sh, const TileLayo
 }
 
 // Stage the halo's positions (and, for the list builder, (orig, chain)).
-template <typename FT>
+template <typename FT, int NT>
 __device__ __forceinline__ void stage_positions(const PoolArgs& a, const RepParams& P, const double* Xa,
                                                 const TileShared& sh, const TileLayout& t, LPos<FT>* lp,
                                                 int2* lm)
 {
     const int nc = P.nc;
-    for (int i = threadIdx.x; i < t.total; i += BS) {
+    for (int i = threadIdx.x; i < t.total; i += NT) {
         int hc, g;
         staged_bead(sh, t, i, hc, g);
         const int hx = hc % t.W, hy = hc / t.W;
         const int gx = (t.cx0 - 1 + hx + nc) % nc;
         const int gy = (t.cy0 - 1 + hy + nc) % nc;
         const double2 x = *reinterpret_cast<const double2*>(Xa + 2 * g);
-        const double rx = wrap1(x.x - (gx + 0.5) * P.h, P.L, P.halfL);
-        const double ry = wrap1(x.y - (gy + 0.5) * P.h, P.L, P.halfL);
-        lp[i] = make_lpos<FT>(rx, ry, hx, hy, P.h);
+        lp[i] = make_lpos<FT>(cell_rel(x.x, gx, P), cell_rel(x.y, gy, P), hx, hy);
         if (lm) lm[i] = *reinterpret_cast<const int2*>(a.META + 2 * g);
     }
 }
@@ -308,13 +231,13 @@ __device__ __forceinline__ void owned_bead(const TileShared& sh, const TileLayou
 
 // Neighbour lists of the tile's owned beads, after a cell re-sort (same run
 // condition as the re-sort). Each bead scans the 3 x 3 cells around its own
-// in LDS order and keeps the beads within rc + skin that the same-chain rule
+// in LDS order (row by row; a row's three cells are one LDS range) and keeps the beads within rc + skin that the same-chain rule
 // does not exclude; the list order is that scan order, so it depends on the
 // positions alone. NLEN = -1 marks a bead without a list (overflow, or a
 // tile outside LDS). The positions at the re-sort are kept in XR for the
 // displacement check of the stages.
-template <typename FT>
-__global__ void __launch_bounds__(BS) k_nlist_tile2d(PoolArgs a, int j)
+template <typename FT, int NT>
+__global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
     const RepParams& P = a.P[tile.x];
@@ -323,14 +246,14 @@ __global__ void __launch_bounds__(BS) k_nlist_tile2d(PoolArgs a, int j)
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ int2 lm[TILE_MAX_BEADS];
-    const TileLayout t = tile_layout(a, P, tile, sh);
-    if (t.in_lds) stage_positions<FT>(a, P, a.X, sh, t, lp, lm);
+    const TileLayout t = tile_layout<NT>(a, P, tile, sh);
+    if (t.in_lds) stage_positions<FT, NT>(a, P, a.X, sh, t, lp, lm);
     __syncthreads();
     const FT rl2 = (FT)P.rl2, h = (FT)P.h;
     const int nn = P.n_nearby;
     const int n_own = sh.own_pref[t.own_y];
     uint2* NL = reinterpret_cast<uint2*>(a.NL);
-    for (int o = threadIdx.x; o < n_own; o += BS) {
+    for (int o = threadIdx.x; o < n_own; o += NT) {
         int li, c0, s;
         owned_bead(sh, t, o, li, c0, s);
         if (!t.in_lds) {
@@ -341,34 +264,28 @@ __global__ void __launch_bounds__(BS) k_nlist_tile2d(PoolArgs a, int j)
         const int2 mm = lm[li];
         int n = 0;
         uint32_t w0 = 0u, w1 = 0u;  // the group being filled
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int c = c0 + dy * t.W + dx;
-                for (int q = sh.cst[c]; q < sh.cst[c + 1]; ++q) {
-                    FT sx, sy;
-                    separation(lp[q], me, h, sx, sy);
-                    if (q == li || !(sx * sx + sy * sy <= rl2)) continue;
-                    const int2 m = lm[q];
-                    if (m.y == mm.y) {
-                        if (nn < 0) continue;
-                        const int d = m.x - mm.x;
-                        if ((d < 0 ? -d : d) <= nn) continue;
-                    }
-                    if (n < NL_MAX) {
-                        const int sl = n & 3;
-                        const uint32_t v = (uint32_t)q << (16 * (sl & 1));
-                        if (sl < 2)
-                            w0 |= v;
-                        else
-                            w1 |= v;
-                        if (sl == 3) {
-                            NL[(int64_t)(n >> 2) * a.nl_stride + s] = make_uint2(w0, w1);
-                            w0 = w1 = 0u;
-                        }
-                    }
-                    ++n;
+        // the three cells of a halo row are consecutive in LDS: one range per row
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int c = c0 + dy * t.W;
+            const int q1 = sh.cst[c + 2];
+            for (int q = sh.cst[c - 1]; q < q1; ++q) {
+                FT sep[2];
+                separation<FT>(lp[q], me, h, sep);
+                const int2 m = lm[q];
+                const int d = m.x - mm.x;
+                const bool excl = (m.y == mm.y) & ((nn < 0) | ((d < 0 ? -d : d) <= nn));
+                const bool ok = (q != li) & (fma(sep[1], sep[1], sep[0] * sep[0]) <= rl2) & !excl;
+                const bool put = ok & (n < NL_MAX);
+                const uint32_t v = put ? (uint32_t)q << (16 * (n & 1)) : 0u;
+                w0 |= (n & 2) ? 0u : v;
+                w1 |= (n & 2) ? v : 0u;
+                if (put & ((n & 3) == 3)) {
+                    NL[(int64_t)(n >> 2) * a.nl_stride + s] = make_uint2(w0, w1);
+                    w0 = w1 = 0u;
                 }
+                n += ok ? 1 : 0;
             }
+        }
         if (n <= NL_MAX) {
             if (n & 3) {  // pad the last group with the bead's own index
                 for (int sl = n & 3; sl < 4; ++sl) {
@@ -389,8 +306,9 @@ __global__ void __launch_bounds__(BS) k_nlist_tile2d(PoolArgs a, int j)
 
 // STAGE 0 / 1 as k_stage; STAGE 2: force evaluation of replica `j` only at
 // the positions X (forces into FT, tangents into Tn; no noise, no update).
-template <typename FT, int STAGE>
-__global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
+// NT threads per tile (256 or 512): large tiles keep more waves in flight.
+template <typename FT, int STAGE, int NT>
+__global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
     const int r = tile.x;
@@ -404,8 +322,8 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
     const double* Xa = STAGE == 1 ? a.X1 : a.X;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
-    const TileLayout t = tile_layout(a, P, tile, sh);
-    if (t.in_lds) stage_positions<FT>(a, P, Xa, sh, t, lp, nullptr);
+    const TileLayout t = tile_layout<NT>(a, P, tile, sh);
+    if (t.in_lds) stage_positions<FT, NT>(a, P, Xa, sh, t, lp, nullptr);
     __syncthreads();
 
     const PairConsts<FT> kc = pair_consts<FT>(P);
@@ -413,7 +331,7 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
     const uint2* NL = reinterpret_cast<const uint2*>(a.NL);
     const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
     double fsq = 0.0;
-    for (int o = threadIdx.x; o < n_own; o += BS) {
+    for (int o = threadIdx.x; o < n_own; o += NT) {
         int li, c0, s;
         owned_bead(sh, t, o, li, c0, s);
         double xs[2], acc[2], tang[2];
@@ -430,10 +348,10 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
                 pa0 = (double)f.x;
                 pa1 = (double)f.y;
             } else {
-                double pg[2] = {0.0, 0.0};
+                FT pg[2] = {(FT)0, (FT)0};
                 pair_sum<2, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, pg);
-                pa0 = pg[0];
-                pa1 = pg[1];
+                pa0 = (double)pg[0];
+                pa1 = (double)pg[1];
             }
         }
         if (STAGE == 1) {
@@ -462,7 +380,7 @@ __global__ void __launch_bounds__(BS) k_stage_tile2d(PoolArgs a, int j)
         }
     }
     if (STAGE == 1) {
-        const double m = block_max(fsq);
+        const double m = block_max<NT>(fsq);
         if (threadIdx.x == 0) a.BMAX[blockIdx.x] = m;
     }
 }

@@ -1,0 +1,111 @@
+"""Autotuner of the cell list and the 2D tiles for one workload.
+
+Picks by timing on the device:
+
+- the tile edge and threads per tile of the 2D force kernels, from the
+  force-stage durations (gcr_pool_time_kernels). No result bit depends on
+  them: every pair path forms separations from cell-relative positions and
+  sums in cell-scan order, whatever the tiling;
+- the cell-list skin (cell edge >= cutoff + skin; the neighbour-list radius is
+  the cell edge) and the re-sort period, from whole-step wall time of graph
+  batches, replays included (a skin too thin for the period makes batches
+  replay with a re-sort before every step). These move results at the
+  rounding level only: another cell grid sums the same pairs in another order.
+
+Every candidate runs on a temporary pool holding a copy of the workload's
+initial state; no pool of the caller is touched. The returned settings go to
+Physics.skin, Pool.set_schedule, Pool.set_tile and Pool.set_tile_threads.
+"""
+import copy
+import time
+
+from . import engine as E
+from ._lib import GcrError
+
+
+def _pool_for(w, precision, skin):
+    phys = copy.copy(w.phys)
+    phys.skin = skin
+    n = w.n
+    pool = E.Pool(phys.dim, precision, bead_capacity=n, cell_capacity=4 * n + 1024,
+                  chain_capacity=phys.n_chains, replica_capacity=1)
+    rid = pool.add(phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                   getattr(w, "v0_per_chain", None))
+    return pool, rid
+
+
+def _arm(pool, rid, w, steps):
+    tv = w.time_vals.copy()
+    tv[4] = -1.0  # no time limit: exactly `steps` steps
+    tv[7] = steps
+    pool.begin(rid, tv)
+
+
+def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512), steps=32):
+    """[(tile, threads, stage0 + stage1 ms)] for every candidate that ran
+    without exceeding the skin, fastest first."""
+    pool, rid = _pool_for(w, precision, skin)
+    out = []
+    try:
+        for t in tiles:
+            for nt in threads:
+                pool.set_tile(t)
+                pool.set_tile_threads(nt)
+                _arm(pool, rid, w, 1 << 30)
+                try:
+                    pool.time_kernels(4)  # builds tables and warms up
+                    kt = pool.time_kernels(steps)
+                except GcrError:
+                    continue
+                out.append((t, nt, kt["stage0"] + kt["stage1"]))
+    finally:
+        pool.close()
+    return sorted(out, key=lambda r: r[2])
+
+
+def time_schedule(w, precision, skin, rebuild_every, tile, threads, steps=256, batch=64):
+    """Wall-clock ms per step of graph batches (replays included) and the
+    number of replays, after one untimed batch (graph capture)."""
+    pool, rid = _pool_for(w, precision, skin)
+    try:
+        pool.set_schedule(batch, rebuild_every)
+        pool.set_tile(tile)
+        pool.set_tile_threads(threads)
+        _arm(pool, rid, w, batch)
+        pool.run()
+        _arm(pool, rid, w, steps)
+        t0 = time.perf_counter()
+        pool.run()
+        sec = time.perf_counter() - t0
+        st = pool.status(rid)
+    finally:
+        pool.close()
+    return 1e3 * sec / steps, st["replays"], st["failed"]
+
+
+def tune(w, precision=E.GCR_MIXED, skin_fracs=(0.1, 0.15, 0.2, 0.3), rebuilds=(8, 16, 32),
+         tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512), steps=256):
+    """Best (skin, rebuild_every, tile, threads) for workload w.
+
+    The tiling is tuned first at the default skin (0.3 cutoff), then every
+    (skin, period) pair is timed with that tiling, and the tiling is re-tuned
+    at the chosen skin. skin_fracs are fractions of the pair cutoff."""
+    rc = float(w.phys.wca[3])
+    dim = w.phys.dim
+    tiling = [(0, 0, 0.0)]
+    if dim == 2:
+        tiling = tune_tiling(w, precision, 0.3 * rc, tiles, threads)
+    tile, nt = tiling[0][0], tiling[0][1]
+    trials = []
+    for f in skin_fracs:
+        for k in rebuilds:
+            ms, replays, failed = time_schedule(w, precision, f * rc, k, tile, nt, steps)
+            if not failed:
+                trials.append(dict(skin=f * rc, rebuild_every=k, ms_per_step=ms, replays=replays))
+    best = min(trials, key=lambda r: r["ms_per_step"])
+    if dim == 2:
+        tiling = tune_tiling(w, precision, best["skin"], tiles, threads)
+        tile, nt = tiling[0][0], tiling[0][1]
+    return dict(skin=best["skin"], rebuild_every=best["rebuild_every"], tile=tile, threads=nt,
+                ms_per_step=best["ms_per_step"], schedule_trials=trials,
+                tiling_trials=[dict(tile=t, threads=n, stages_ms=m) for t, n, m in tiling])

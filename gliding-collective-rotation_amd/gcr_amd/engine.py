@@ -95,6 +95,10 @@ class Pool:
         """Tile edge of the 2D LDS force kernel (0: automatic)."""
         check(self._L.gcr_pool_set_tile(self.h, int(side)))
 
+    def set_tile_threads(self, threads):
+        """Threads per tile workgroup of the 2D force kernels (0, 256 or 512)."""
+        check(self._L.gcr_pool_set_tile_threads(self.h, int(threads)))
+
     def add(self, phys, pos, tangent, chain_id, rest_lengths, v_tan_angle=None,
             v0_per_chain=None):
         pos = f64(pos)

@@ -133,8 +133,14 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
  * cell re-sort (all its kernels, per re-sort)]. */
 int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4]);
 /* Tile edge (cells) of the 2D LDS force kernel; 0 picks it per replica from
- * the cell occupancy (about 256 owned beads per workgroup). At most 16. */
+ * the cell occupancy (about 512 owned beads per workgroup, the tile with its
+ * halo within 70% of the LDS capacity at the mean occupancy). At most 16. */
 int gcr_pool_set_tile(gcr_pool* p, int32_t side);
+/* Threads per tile workgroup of the 2D force kernels: 256, 512, or 0 to pick
+ * 512 when tiles average 384 or more owned beads. Neither this nor the tile
+ * edge changes any result bit: the pair sums keep one order and one
+ * arithmetic whatever the tiling. */
+int gcr_pool_set_tile_threads(gcr_pool* p, int32_t threads);
 int gcr_pool_destroy(gcr_pool* p);
 
 /* ------------------------------------------------- initial configuration */

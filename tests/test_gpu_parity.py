@@ -60,10 +60,11 @@ def cases():
     out["list_overflow"] = W.chains("list_overflow", 2, 900, 1, phi=0.55,
                                     cutoff_factor=W.LJ_25_SIGMA, T=0.01, Dr=0.1, steps=100,
                                     seed=9, skin=3.3)
-    # cells so full that a tile with its halo does not fit in LDS
-    out["tile_outside_lds"] = W.chains("tile_outside_lds", 2, 1300, 1, phi=0.55,
-                                       cutoff_factor=W.LJ_25_SIGMA, T=0.01, Dr=0.1,
-                                       steps=100, seed=10, skin=8.0)
+    # about 80 beads per cell: a one-cell tile with its halo fits in LDS, a
+    # three-cell one does not (see test_tile_shape_does_not_change_results)
+    out["dense_cells"] = W.chains("dense_cells", 2, 1300, 1, phi=0.55,
+                                  cutoff_factor=W.LJ_25_SIGMA, T=0.01, Dr=0.1, steps=100,
+                                  seed=10, skin=8.0)
     return out
 
 
@@ -244,6 +245,38 @@ def test_alone_vs_mixed_pool_bit_identical():
         alone = run([w])[0]
         for k in alone:
             assert np.array_equal(alone[k], mix[i][k]), (i, k)
+
+
+@pytest.mark.parametrize("precision", [E.GCR_FP64, E.GCR_MIXED])
+@pytest.mark.parametrize("name,sides", [("config3_cut", ((1, 256), (2, 512), (5, 256), (16, 512))),
+                                        ("dense_cells", ((1, 256), (3, 512))),
+                                        ("list_overflow", ((1, 512), (4, 256)))])
+def test_tile_shape_does_not_change_results(name, sides, precision):
+    """Separations are formed from cell-relative positions in every pair path,
+    and lists keep the cell-scan order, so the tile edge and the threads per
+    tile (and with them whether
+    a bead's pairs come from its LDS list or from the global cell scan: in
+    dense_cells a 3-cell tile does not fit in LDS) leaves every bit of the
+    trajectory unchanged."""
+    w = CASES[name]
+    print(f"[tile shapes] {name} sides={sides} precision={precision}", flush=True)
+    tv = w.time_vals.copy()
+    tv[7] = 50
+    outs = []
+    for side, threads in sides:
+        p = E.Pool(2, precision, bead_capacity=w.n, cell_capacity=4 * w.n + 1024,
+                   chain_capacity=w.phys.n_chains, replica_capacity=1)
+        p.set_schedule(16, 8)
+        p.set_tile(side)
+        p.set_tile_threads(threads)
+        rid = p.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+        p.begin(rid, tv)
+        p.run()
+        outs.append(p.download(rid))
+        p.close()
+    for o in outs[1:]:
+        for k in o:
+            assert np.array_equal(o[k], outs[0][k]), k
 
 
 def test_mixed_precision_force_error():

@@ -53,3 +53,18 @@ def test_serving_results_equal_alone():
         assert np.array_equal(a.result["tangent"], r.result["tangent"]), r.rid
     rep = SV.report(done)
     assert rep["requests"] == 6 and rep["p99_latency_s"] >= rep["p50_latency_s"] > 0
+
+
+def test_autotune_picks_a_measured_setting():
+    from gcr_amd import autotune as AT
+    w = W.chains("tune", 2, 200, 20, phi=0.3, kappa=10.0, vtan=0.2, T=0.01, dtmax=1e-3,
+                 steps=100, seed=2)
+    r = AT.tune(w, E.GCR_MIXED, skin_fracs=(0.2, 0.3), rebuilds=(8, 16), tiles=(0, 2, 4),
+                threads=(256, 512), steps=64)
+    print("autotune:", {k: r[k] for k in ("skin", "rebuild_every", "tile", "threads",
+                                           "ms_per_step")}, flush=True)
+    assert r["threads"] in (256, 512) and r["tile"] in (0, 2, 4)
+    assert r["rebuild_every"] in (8, 16)
+    assert any(abs(r["skin"] - f * w.phys.wca[3]) < 1e-12 for f in (0.2, 0.3))
+    assert len(r["tiling_trials"]) == 6
+    assert r["ms_per_step"] == min(t["ms_per_step"] for t in r["schedule_trials"])
