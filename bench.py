@@ -47,9 +47,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline budget: steps are added in chunks of 40 until this much time")
     ap.add_argument("--kernel-steps", type=int, default=32)
-    ap.add_argument("--mode", choices=("single", "ensemble", "serve"), default="single",
+    ap.add_argument("--mode", choices=("single", "ensemble", "serve", "copy"), default="single",
                     help="single: 400k-bead system (the headline line); ensemble: 256 "
-                         "replicas batched vs one at a time; serve: request stream")
+                         "replicas batched vs one at a time; serve: request stream; copy: "
+                         "a plain 1 GiB device copy kernel, the reference point for HBM "
+                         "byte counters (FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--seq-sample", type=int, default=16,
                     help="ensemble: replicas timed one at a time for the sequential rate")
@@ -181,11 +183,28 @@ def main_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def main_copy():
+    """dst = 1 * src over 1 GiB of float64, three times: one elementwise
+    kernel that reads 1 GiB and writes 1 GiB per launch. Counter readings of
+    this kernel convert FETCH_SIZE / WRITE_SIZE into bytes."""
+    import torch
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy", "bytes_read_per_launch": 8 * n,
+                      "bytes_written_per_launch": 8 * n, "launches": 3}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.mode == "copy":
+        return main_copy()
     if args.mode == "ensemble":
         return main_ensemble(args)
     if args.mode == "serve":

@@ -649,6 +649,8 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.Tn = p->Tn;
     a.METAn = p->METAn;
     a.XIn = p->XIn;
+    a.XIRn = p->XIn + nb * dim;
+    a.XIR = p->XI + nb * dim;
     a.X1n = p->X1n;
     a.Xs = p->Xs;
     a.Ts = p->Ts;

@@ -88,8 +88,9 @@ struct PoolArgs {
     double* X1;    // [slot][dim] predictor positions
     double* F0;    // [slot][dim] stage-0 forces
     double* FT;    // [slot][dim] F0 + F1
-    double* XI;    // [slot][2][dim] noise of the next / running step
-                   // (translation, rotation); moves with the bead at re-sorts
+    double* XI;    // [slot][dim] translational noise of the next / running
+                   // step; moves with the bead at re-sorts
+    double* XIR;   // [slot][dim] rotational noise (diffusing directions)
     double* T;     // [slot][dim] propulsion direction (ABP / diffusing state)
     int32_t* META; // [slot][2] (orig_global, chain_global)
     int32_t* LINK; // [slot][4] (prev2, prev1, next1, next2) slots, -1 if none
@@ -116,6 +117,7 @@ struct PoolArgs {
     double* Tn;
     int32_t* METAn;
     double* XIn;
+    double* XIRn;
     double* X1n;
     // rollback snapshot
     double* Xs;

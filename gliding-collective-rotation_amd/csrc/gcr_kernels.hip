@@ -177,16 +177,16 @@ __device__ __forceinline__ void store_vec(double* A, int s, const double* v)
     }
 }
 
-// Noise of the bead in slot s for the given step into XI[s]: translation,
-// and rotation when the update uses it.
+// Noise of the bead in slot s for the given step: translation into XI[s],
+// rotation into XIR[s] when the update uses it.
 template <int DIM>
 __device__ __forceinline__ void store_noise(const PoolArgs& a, const RepParams& P, int s, int64_t steps,
                                             int64_t step_global)
 {
     double xi[2 * DIM];
     draw_noise<DIM>(P, a.META[2 * s], steps, step_global, xi);
-    store_vec<DIM>(a.XI, 2 * s, xi);
-    if (P.flags & GCR_F_ROTDIFF) store_vec<DIM>(a.XI, 2 * s + 1, xi + DIM);
+    store_vec<DIM>(a.XI, s, xi);
+    if (P.flags & GCR_F_ROTDIFF) store_vec<DIM>(a.XIR, s, xi + DIM);
 }
 
 // Cell coordinate of a position along one axis, clamped into [0, nc).
@@ -648,7 +648,7 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
         // this step's translational noise, drawn by the previous update (or
         // the batch start) and carried through re-sorts
         double xi[DIM];
-        load_vec<DIM>(a.XI, 2 * s, xi);
+        load_vec<DIM>(a.XI, s, xi);
         store_vec<DIM>(a.F0, s, acc);
         const double dt = S->dt;
         double dt_eff = dt * 1.0;
@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
     double x[DIM], f[DIM], xi[2 * DIM];
     load_vec<DIM>(a.X, s, x);
     load_vec<DIM>(a.FT, s, f);
-    load_vec<DIM>(a.XI, 2 * s, xi);
+    load_vec<DIM>(a.XI, s, xi);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         double tmp = x[i] + dt_eff * f[i];
@@ -778,7 +778,7 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
         const double sr = sqrt(2.0 * dt_used * P.Dr) * 3.141592653589793;
         double t[DIM], nr[DIM], proj = 0.0;
         load_vec<DIM>(a.T, s, t);
-        load_vec<DIM>(a.XI, 2 * s + 1, xi + DIM);
+        load_vec<DIM>(a.XIR, s, xi + DIM);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) nr[i] = xi[DIM + i];
 #pragma unroll
@@ -1006,11 +1006,11 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
     a.CELL[d] = key;
     a.SLOT[pr.x] = d;
     // the pending noise and the last predictor travel with the bead
-    load_vec<DIM>(a.XI, 2 * so, x);
-    store_vec<DIM>(a.XIn, 2 * d, x);
+    load_vec<DIM>(a.XI, so, x);
+    store_vec<DIM>(a.XIn, d, x);
     if (P.flags & GCR_F_ROTDIFF) {
-        load_vec<DIM>(a.XI, 2 * so + 1, x);
-        store_vec<DIM>(a.XIn, 2 * d + 1, x);
+        load_vec<DIM>(a.XIR, so, x);
+        store_vec<DIM>(a.XIRn, d, x);
     }
     load_vec<DIM>(a.X1, so, x);
     store_vec<DIM>(a.X1n, d, x);
@@ -1043,11 +1043,11 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
         store_vec<DIM>(a.T, ns, t);
     }
     *reinterpret_cast<int2*>(a.META + 2 * ns) = me;
-    load_vec<DIM>(a.XIn, 2 * ns, x);
-    store_vec<DIM>(a.XI, 2 * ns, x);
+    load_vec<DIM>(a.XIn, ns, x);
+    store_vec<DIM>(a.XI, ns, x);
     if (P.flags & GCR_F_ROTDIFF) {
-        load_vec<DIM>(a.XIn, 2 * ns + 1, x);
-        store_vec<DIM>(a.XI, 2 * ns + 1, x);
+        load_vec<DIM>(a.XIRn, ns, x);
+        store_vec<DIM>(a.XIR, ns, x);
     }
     load_vec<DIM>(a.X1n, ns, x);
     store_vec<DIM>(a.X1, ns, x);

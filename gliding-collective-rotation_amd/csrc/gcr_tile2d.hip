@@ -77,12 +77,10 @@ struct FT2 {
 // Pair forces of one bead over its list (len entries, groups of 4 LDS
 // indices at NL[g * stride + s]; the last group is padded with the bead's own
 // index, which the zero-distance mask drops). Summed in FT.
-template <typename FT>
-__device__ __forceinline__ FT2<FT> list_pair_sum(const LPos<FT>* lp, const uint2* G, int64_t stride, int len,
-                                                 const LPos<FT> me, const PairConsts<FT>& k)
+template <typename FT, bool SHIFTED>
+__device__ __forceinline__ void list_loop(const LPos<FT>* lp, const uint2* G, int64_t stride, int ng,
+                                          const LPos<FT>& me, const PairConsts<FT>& k, FT* f)
 {
-    FT f[2] = {(FT)0, (FT)0};
-    const int ng = (len + 3) >> 2;
     uint32_t e0 = 0u, e1 = 0u;
     if (ng > 0) {
         const uint2 e = G[0];
@@ -96,12 +94,13 @@ __device__ __forceinline__ FT2<FT> list_pair_sum(const LPos<FT>* lp, const uint2
             n0 = en.x;
             n1 = en.y;
         }
-        const uint32_t idx[4] = {e0 & 0xffffu, e0 >> 16, e1 & 0xffffu, e1 >> 16};
+        // the four LDS reads issue together, then the four terms
+        const LPos<FT> q[4] = {lp[e0 & 0xffffu], lp[e0 >> 16], lp[e1 & 0xffffu], lp[e1 >> 16]};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             FT sep[2];
-            separation<FT>(lp[idx[u]], me, k.h, sep);
-            if (k.shifted)
+            separation<FT>(q[u], me, k.h, sep);
+            if (SHIFTED)
                 pair_term_shifted<2, FT>(sep, k, f);
             else
                 pair_term<2, FT>(sep, k, f);
@@ -109,6 +108,18 @@ __device__ __forceinline__ FT2<FT> list_pair_sum(const LPos<FT>* lp, const uint2
         e0 = n0;
         e1 = n1;
     }
+}
+
+template <typename FT>
+__device__ __forceinline__ FT2<FT> list_pair_sum(const LPos<FT>* lp, const uint2* G, int64_t stride, int len,
+                                                 const LPos<FT> me, const PairConsts<FT>& k)
+{
+    FT f[2] = {(FT)0, (FT)0};
+    const int ng = (len + 3) >> 2;
+    if (k.shifted)
+        list_loop<FT, true>(lp, G, stride, ng, me, k, f);
+    else
+        list_loop<FT, false>(lp, G, stride, ng, me, k, f);
     return FT2<FT>{f[0], f[1]};
 }
 
