@@ -225,6 +225,8 @@ def main():
             t = AT.tune(w, prec)
             cfg = {k: t[k] for k in ("skin", "rebuild_every", "tile", "threads")}
             cfg["tuning_s"] = time.perf_counter() - t0
+            cfg["tiling_top"] = [[r["tile"], r["threads"], round(1e3 * r["stages_ms"], 1)]
+                                 for r in t["tiling_trials"][:4]]
         tuned[name] = cfg
         w.phys.skin = cfg["skin"]
         res[name] = run_gpu(w, prec, args.steps, args.warmup, args.batch, cfg["rebuild_every"],

@@ -125,7 +125,7 @@ struct gcr_pool {
     int64_t capChk = 0, capBlk = 0, capTile = 0;
     int ntile = 0;
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
-    int tile_threads = 0;  // threads per tile workgroup: 256, 512, 0 = from the tile sizes
+    int tile_threads = 0;  // threads per tile workgroup: 256, 512, 1024, 0 = from the tile sizes
     int nt_launch = BS;    // resolved by rebuild_tables
 };
 
@@ -249,12 +249,14 @@ int rebuild_tables(gcr_pool* p)
     }
     rco[p->capR] = (int32_t)cr.size();
     if (p->dim == 2) {
-        // 512 threads when tiles average 384 or more owned beads
+        // about one owned bead per thread: the per-bead chain of dependent
+        // loads is long, and only more waves per CU hide it
         int64_t beads = 0;
         for (int r = 0; r < p->capR; ++r)
             if (p->live[r]) beads += p->P[r].n;
         const int64_t nt = (int64_t)tl.size() / 4;
-        p->nt_launch = p->tile_threads ? p->tile_threads : (nt && beads >= 384 * nt ? 512 : 256);
+        const int64_t per = nt ? beads / nt : 0;
+        p->nt_launch = p->tile_threads ? p->tile_threads : (per >= 768 ? 1024 : per >= 384 ? 512 : 256);
     }
     if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk ||
         (int64_t)tl.size() / 4 > p->capTile)
@@ -301,13 +303,22 @@ void launch_rebuild(gcr_pool* p, int j)
         k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         if (p->ntile) {
-            const bool big = p->nt_launch == 512;
-            if (p->precision == GCR_MIXED)
-                big ? k_nlist_tile2d<float, 512><<<p->ntile, 512, 0, st>>>(a, j)
-                    : k_nlist_tile2d<float, 256><<<p->ntile, 256, 0, st>>>(a, j);
-            else
-                big ? k_nlist_tile2d<double, 512><<<p->ntile, 512, 0, st>>>(a, j)
-                    : k_nlist_tile2d<double, 256><<<p->ntile, 256, 0, st>>>(a, j);
+            const int nt = p->nt_launch;
+            if (p->precision == GCR_MIXED) {
+                if (nt == 1024)
+                    k_nlist_tile2d<float, 1024><<<p->ntile, 1024, 0, st>>>(a, j);
+                else if (nt == 512)
+                    k_nlist_tile2d<float, 512><<<p->ntile, 512, 0, st>>>(a, j);
+                else
+                    k_nlist_tile2d<float, 256><<<p->ntile, 256, 0, st>>>(a, j);
+            } else {
+                if (nt == 1024)
+                    k_nlist_tile2d<double, 1024><<<p->ntile, 1024, 0, st>>>(a, j);
+                else if (nt == 512)
+                    k_nlist_tile2d<double, 512><<<p->ntile, 512, 0, st>>>(a, j);
+                else
+                    k_nlist_tile2d<double, 256><<<p->ntile, 256, 0, st>>>(a, j);
+            }
         }
     } else {
         k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
@@ -333,7 +344,9 @@ void launch_k(K kernel, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1
 template <typename FT, int STAGE>
 void launch_tile(gcr_pool* p, int j, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (p->nt_launch == 512)
+    if (p->nt_launch == 1024)
+        launch_k(k_stage_tile2d<FT, STAGE, 1024>, p->ntile, p->stream, ev0, ev1, p->a, j, 1024);
+    else if (p->nt_launch == 512)
         launch_k(k_stage_tile2d<FT, STAGE, 512>, p->ntile, p->stream, ev0, ev1, p->a, j, 512);
     else
         launch_k(k_stage_tile2d<FT, STAGE, 256>, p->ntile, p->stream, ev0, ev1, p->a, j, 256);
@@ -1053,8 +1066,8 @@ int gcr_pool_set_tile(gcr_pool* p, int32_t side)
 int gcr_pool_set_tile_threads(gcr_pool* p, int32_t threads)
 {
     if (!p) return fail(GCR_ERR_ARG, "null pool");
-    if (threads != 0 && threads != 256 && threads != 512)
-        return fail(GCR_ERR_ARG, "threads per tile must be 0, 256 or 512");
+    if (threads != 0 && threads != 256 && threads != 512 && threads != 1024)
+        return fail(GCR_ERR_ARG, "threads per tile must be 0, 256, 512 or 1024");
     p->tile_threads = threads;
     p->tables_dirty = true;
     return GCR_OK;

@@ -21,9 +21,16 @@ namespace gcr {
 
 // ------------------------------------------------------------------ helpers
 
-// CPython float_rem (the `tmp %= Lx` at SRK.py:387).
+// CPython float_rem (the `tmp %= Lx` at SRK.py:387), for b > 0. A position
+// that moved by less than one box lies in (-b, 2b): there fmod is the
+// identity or one subtraction that is exact (Sterbenz), and the sign fix-up
+// is the same addition CPython makes, so the fast cases give CPython's bits
+// without the cost of a general FP64 fmod.
 __device__ __forceinline__ double pymod(double a, double b)
 {
+    if (a >= 0.0 && a < b) return a == 0.0 ? 0.0 : a;  // +0 for a signed zero, as copysign(0, b)
+    if (a >= b && a < 2.0 * b) return a - b;
+    if (a < 0.0 && a > -b) return a + b;
     double m = fmod(a, b);
     if (m != 0.0) {
         if ((b < 0) != (m < 0)) m += b;

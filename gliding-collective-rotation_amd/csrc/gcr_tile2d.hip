@@ -207,16 +207,18 @@ __device__ __forceinline__ void staged_bead(const TileShared& sh, const TileLayo
     g = sh.gst[lo] + (i - sh.cst[lo]);
 }
 
-// Stage the halo's positions (and, for the list builder, (orig, chain)).
+// Stage the halo's positions (and, for the list builder, (orig, chain)), and
+// the halo cell of every staged bead.
 template <typename FT, int NT>
 __device__ __forceinline__ void stage_positions(const PoolArgs& a, const RepParams& P, const double* Xa,
                                                 const TileShared& sh, const TileLayout& t, LPos<FT>* lp,
-                                                int2* lm)
+                                                int2* lm, uint16_t* lcell)
 {
     const int nc = P.nc;
     for (int i = threadIdx.x; i < t.total; i += NT) {
         int hc, g;
         staged_bead(sh, t, i, hc, g);
+        lcell[i] = (uint16_t)hc;
         const int hx = hc % t.W, hy = hc / t.W;
         const int gx = (t.cx0 - 1 + hx + nc) % nc;
         const int gy = (t.cy0 - 1 + hy + nc) % nc;
@@ -226,11 +228,24 @@ __device__ __forceinline__ void stage_positions(const PoolArgs& a, const RepPara
     }
 }
 
-// Owned bead o of the tile: its LDS index li, halo cell c0 and slot s.
-__device__ __forceinline__ void owned_bead(const TileShared& sh, const TileLayout& t, int o, int& li, int& c0,
-                                           int& s)
+// Owned bead o of the tile: its LDS index li, halo cell c0 and slot s. A
+// thread visits its owned beads in increasing o, so the tile row ry carries
+// over from the previous call (start at 0).
+__device__ __forceinline__ void owned_bead(const TileShared& sh, const TileLayout& t, const uint16_t* lcell,
+                                           int o, int& ry, int& li, int& c0, int& s)
 {
-    int ry = 0;
+    while (sh.own_pref[ry + 1] <= o) ++ry;
+    const int row = (ry + 1) * t.W;
+    li = sh.cst[row + 1] + (o - sh.own_pref[ry]);
+    c0 = lcell[li];
+    s = sh.gst[c0] + (li - sh.cst[c0]);
+}
+
+// The same without the staged cell table (tile outside LDS): the cell is
+// found by scanning the row's cell offsets.
+__device__ __forceinline__ void owned_bead_scan(const TileShared& sh, const TileLayout& t, int o, int& ry,
+                                                int& li, int& c0, int& s)
+{
     while (sh.own_pref[ry + 1] <= o) ++ry;
     const int row = (ry + 1) * t.W;
     li = sh.cst[row + 1] + (o - sh.own_pref[ry]);
@@ -257,20 +272,23 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ int2 lm[TILE_MAX_BEADS];
+    __shared__ uint16_t lcell[TILE_MAX_BEADS];
     const TileLayout t = tile_layout<NT>(a, P, tile, sh);
-    if (t.in_lds) stage_positions<FT, NT>(a, P, a.X, sh, t, lp, lm);
+    if (t.in_lds) stage_positions<FT, NT>(a, P, a.X, sh, t, lp, lm, lcell);
     __syncthreads();
     const FT rl2 = (FT)P.rl2, h = (FT)P.h;
     const int nn = P.n_nearby;
     const int n_own = sh.own_pref[t.own_y];
     uint2* NL = reinterpret_cast<uint2*>(a.NL);
+    int ry = 0;
     for (int o = threadIdx.x; o < n_own; o += NT) {
         int li, c0, s;
-        owned_bead(sh, t, o, li, c0, s);
         if (!t.in_lds) {
+            owned_bead_scan(sh, t, o, ry, li, c0, s);
             a.NLEN[s] = -1;
             continue;
         }
+        owned_bead(sh, t, lcell, o, ry, li, c0, s);
         const LPos<FT> me = lp[li];
         const int2 mm = lm[li];
         int n = 0;
@@ -333,8 +351,9 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
     const double* Xa = STAGE == 1 ? a.X1 : a.X;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
+    __shared__ uint16_t lcell[TILE_MAX_BEADS];
     const TileLayout t = tile_layout<NT>(a, P, tile, sh);
-    if (t.in_lds) stage_positions<FT, NT>(a, P, Xa, sh, t, lp, nullptr);
+    if (t.in_lds) stage_positions<FT, NT>(a, P, Xa, sh, t, lp, nullptr, lcell);
     __syncthreads();
 
     const PairConsts<FT> kc = pair_consts<FT>(P);
@@ -342,9 +361,13 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
     const uint2* NL = reinterpret_cast<const uint2*>(a.NL);
     const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
     double fsq = 0.0;
+    int ry = 0;
     for (int o = threadIdx.x; o < n_own; o += NT) {
         int li, c0, s;
-        owned_bead(sh, t, o, li, c0, s);
+        if (t.in_lds)
+            owned_bead(sh, t, lcell, o, ry, li, c0, s);
+        else
+            owned_bead_scan(sh, t, o, ry, li, c0, s);
         double xs[2], acc[2], tang[2];
         load_vec<2>(Xa, s, xs);
         const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);

@@ -41,7 +41,7 @@ def _arm(pool, rid, w, steps):
     pool.begin(rid, tv)
 
 
-def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512), steps=32):
+def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512, 1024), steps=32):
     """[(tile, threads, stage0 + stage1 ms)] for every candidate that ran
     without exceeding the skin, fastest first."""
     pool, rid = _pool_for(w, precision, skin)
@@ -84,7 +84,7 @@ def time_schedule(w, precision, skin, rebuild_every, tile, threads, steps=256, b
 
 
 def tune(w, precision=E.GCR_MIXED, skin_fracs=(0.1, 0.15, 0.2, 0.3), rebuilds=(8, 16, 32),
-         tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512), steps=256):
+         tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512, 1024), steps=256):
     """Best (skin, rebuild_every, tile, threads) for workload w.
 
     The tiling is tuned first at the default skin (0.3 cutoff), then every

@@ -96,7 +96,7 @@ class Pool:
         check(self._L.gcr_pool_set_tile(self.h, int(side)))
 
     def set_tile_threads(self, threads):
-        """Threads per tile workgroup of the 2D force kernels (0, 256 or 512)."""
+        """Threads per tile workgroup of the 2D force kernels (0, 256, 512 or 1024)."""
         check(self._L.gcr_pool_set_tile_threads(self.h, int(threads)))
 
     def add(self, phys, pos, tangent, chain_id, rest_lengths, v_tan_angle=None,

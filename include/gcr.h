@@ -136,8 +136,8 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4]);
  * the cell occupancy (about 512 owned beads per workgroup, the tile with its
  * halo within 70% of the LDS capacity at the mean occupancy). At most 16. */
 int gcr_pool_set_tile(gcr_pool* p, int32_t side);
-/* Threads per tile workgroup of the 2D force kernels: 256, 512, or 0 to pick
- * 512 when tiles average 384 or more owned beads. Neither this nor the tile
+/* Threads per tile workgroup of the 2D force kernels: 256, 512, 1024, or 0 to
+ * pick about one owned bead per thread. Neither this nor the tile
  * edge changes any result bit: the pair sums keep one order and one
  * arithmetic whatever the tiling. */
 int gcr_pool_set_tile_threads(gcr_pool* p, int32_t threads);
