@@ -27,6 +27,11 @@ HBM_PEAK_GBS = 8000.0
 # and writes x1, stage 1 reads x1, the update reads and writes x -> 80 B per
 # bead-step in total
 ALG_BYTES = {"stage0": 32, "stage1": 16, "update": 32}
+# HBM bytes per bead per launch of the mixed kernels on this workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (tile edge 8, 256 threads per
+# tile, skin 0.3 cutoff, re-sort every 32 steps), each counter scaled by its
+# reading for a plain 1 GiB copy kernel (`--mode copy`: reads x 2.0, writes x 1.0)
+TRAFFIC_BYTES = {"stage0": 174.2, "stage1": 136.2, "update": 104.8}
 
 
 def parse():
@@ -258,7 +263,9 @@ def main():
         "mixed_over_fp64": m["bead_steps_per_s"] / res["fp64"]["bead_steps_per_s"],
         "kernels_ms": kt,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES[dom] * w.n,
+                     "traffic_gbs": TRAFFIC_BYTES[dom] * w.n / (kt[dom] * 1e-3) / 1e9,
                      "step_achieved": 80 * w.n / (m["ms_per_step"] * 1e-3) / 1e9},
         "device": dev[0],
         "replays": {k: v["replays"] for k, v in res.items()},
