@@ -249,38 +249,41 @@ __device__ __forceinline__ bool moved_too_far(const PoolArgs& a, const RepParams
 // its squared length, inverse length and unit vector (unit = raw when the
 // length is zero, as forces.py:326-349 leaves it). BT is the arithmetic
 // type: double for the FP64 path, float for the mixed path (the difference
-// of the FP64 positions is taken in FP64, then rounded).
+// of the FP64 positions is taken in FP64, then rounded). inv64 is the inverse
+// length in FP64 in both: the spring factor a/|t| - 1 cancels for bonds near
+// their rest length, so the mixed path forms it in FP64 too.
 template <int DIM, typename BT>
 struct Bond {
     BT t[DIM];
     BT u[DIM];
     BT ds;
-    BT inv;
+    double inv64;
 };
 
 __device__ __forceinline__ double inv_sqrt(double x) { return 1.0 / sqrt(x); }
 __device__ __forceinline__ float inv_sqrt(float x) { return 1.0f / sqrtf(x); }
-__device__ __forceinline__ double acos_t(double x) { return acos(x); }
-__device__ __forceinline__ float acos_t(float x) { return acosf(x); }
-__device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
-__device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 
 template <int DIM, typename BT>
 __device__ __forceinline__ void make_bond(const double* xu, const double* xv, const RepParams& P,
                                           Bond<DIM, BT>& b)
 {
+    double t64[DIM];
+    double ds64 = 0.0;
     b.ds = (BT)0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        b.t[i] = (BT)wrap1(xu[i] - xv[i], P.L, P.halfL);
+        t64[i] = wrap1(xu[i] - xv[i], P.L, P.halfL);
+        b.t[i] = (BT)t64[i];
         b.ds += b.t[i] * b.t[i];
+        ds64 += t64[i] * t64[i];
     }
     if (b.ds != (BT)0) {
-        b.inv = inv_sqrt(b.ds);
+        const BT inv = inv_sqrt(b.ds);
+        b.inv64 = sizeof(BT) == 8 ? (double)inv : inv_sqrt(ds64);
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i] * b.inv;
+        for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i] * inv;
     } else {
-        b.inv = (BT)0;
+        b.inv64 = 0.0;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i];
     }
@@ -293,19 +296,48 @@ __device__ __forceinline__ BT sgn(BT x)
 }
 
 // Bending prefactor phi of forces.py:364-373 for the angle between ti (bond
-// leaving the vertex forward) and tj (bond arriving from behind). Returns
-// false when the reference skips the term (|dot| == 1).
-template <int DIM, typename BT>
-__device__ __forceinline__ bool bend_phi(const BT* ti, const BT* tj, BT kappa, BT theta0, BT& dot,
-                                         BT& phi)
+// leaving the vertex forward) and tj (bond arriving from behind):
+// phi = kappa (theta + theta0 sign(.)) / sin(theta). Returns false when the
+// term is skipped.
+//   FP64: the reference's arithmetic, theta = acos(dot), sin = sqrt(1 - dot^2),
+//   skipped when |dot| == 1.
+//   FP32 (mixed): acos of a dot product near 1 loses most digits on nearly
+//   straight chains, so theta = atan2(|ti x tj|, dot) and sin = |ti x tj|,
+//   skipped when the cross product is exactly zero.
+template <int DIM>
+__device__ __forceinline__ bool bend_phi(const double* ti, const double* tj, double kappa, double theta0,
+                                         double& dot, double& phi)
 {
-    dot = (BT)0;
+    dot = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) dot += ti[i] * tj[i];
-    if (!((BT)-1 < dot && dot < (BT)1)) return false;
-    phi = acos_t(dot);
-    if (theta0 != (BT)0) phi += theta0 * sgn<BT>(-ti[0] * tj[1] + ti[1] * tj[0]);
-    phi *= kappa / sqrt_t((BT)1 - dot * dot);
+    if (!(-1.0 < dot && dot < 1.0)) return false;
+    phi = acos(dot);
+    if (theta0 != 0.0) phi += theta0 * sgn<double>(-ti[0] * tj[1] + ti[1] * tj[0]);
+    phi *= kappa / sqrt(1.0 - dot * dot);
+    return true;
+}
+
+template <int DIM>
+__device__ __forceinline__ bool bend_phi(const float* ti, const float* tj, float kappa, float theta0,
+                                         float& dot, float& phi)
+{
+    dot = 0.0f;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) dot += ti[i] * tj[i];
+    const float cz = ti[0] * tj[1] - ti[1] * tj[0];
+    float cn;
+    if (DIM == 2) {
+        cn = fabsf(cz);
+    } else {
+        const float cx = ti[1] * tj[2] - ti[2] * tj[1];
+        const float cy = ti[2] * tj[0] - ti[0] * tj[2];
+        cn = sqrtf(cx * cx + cy * cy + cz * cz);
+    }
+    if (cn == 0.0f) return false;
+    phi = atan2f(cn, dot);
+    if (theta0 != 0.0f) phi += theta0 * sgn<float>(-cz);
+    phi *= kappa / cn;
     return true;
 }
 
@@ -337,14 +369,14 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
 {
     const bool bend = (P.flags & GCR_F_BEND) != 0;
     const bool orient = (P.flags & GCR_F_ORIENT) != 0;
-    const BT k = (BT)P.k, kappa = (BT)P.kappa, theta0 = (BT)P.theta0, ab = (BT)a;
+    const BT kappa = (BT)P.kappa, theta0 = (BT)P.theta0;
     const int4 lk = c.lk;
     Bond<DIM, BT> A, B, Pb, Nb;  // (p1,s), (s,n1), (p2,p1), (n1,n2)
     if (lk.y >= 0) make_bond<DIM, BT>(c.p1, xs, P, A);
     if (lk.z >= 0) make_bond<DIM, BT>(xs, c.n1, P, B);
     // bond p1 -> s: forces[b1+1] -= f (forces.py:345-348)
     if (lk.y >= 0 && A.ds != (BT)0) {
-        const BT mag = k * (ab * A.inv - (BT)1);
+        const BT mag = (BT)(P.k * (a * A.inv64 - 1.0));
 #pragma unroll
         for (int i = 0; i < DIM; ++i) acc[i] -= (double)(A.t[i] * mag);
     }
@@ -352,14 +384,14 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     if (bend && lk.x >= 0) {
         make_bond<DIM, BT>(c.p2, c.p1, P, Pb);
         BT dot, phi;
-        if (bend_phi<DIM, BT>(A.u, Pb.u, kappa, theta0, dot, phi)) {
+        if (bend_phi<DIM>(A.u, Pb.u, kappa, theta0, dot, phi)) {
 #pragma unroll
             for (int i = 0; i < DIM; ++i) acc[i] -= (double)(phi * (Pb.u[i] - dot * A.u[i]));
         }
     }
     // bond s -> n1: forces[b1] += f
     if (lk.z >= 0 && B.ds != (BT)0) {
-        const BT mag = k * (ab * B.inv - (BT)1);
+        const BT mag = (BT)(P.k * (a * B.inv64 - 1.0));
 #pragma unroll
         for (int i = 0; i < DIM; ++i) acc[i] += (double)(B.t[i] * mag);
     }
@@ -370,10 +402,8 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
 #pragma unroll
             for (int i = 0; i < DIM; ++i) dot += B.u[i] * A.u[i];
             const bool inside = ((BT)-1 < dot && dot < (BT)1);
-            if (inside && bend) {
-                BT phi = acos_t(dot);
-                if (theta0 != (BT)0) phi += theta0 * sgn<BT>(-B.u[0] * A.u[1] + B.u[1] * A.u[0]);
-                phi *= kappa / sqrt_t((BT)1 - dot * dot);
+            BT phi, dot2;
+            if (bend && bend_phi<DIM>(B.u, A.u, kappa, theta0, dot2, phi)) {
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) {
                     const BT ff = phi * (dot * A.u[i] - B.u[i]);
@@ -408,7 +438,7 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     if (bend && lk.w >= 0) {
         make_bond<DIM, BT>(c.n1, c.n2, P, Nb);
         BT dot, phi;
-        if (bend_phi<DIM, BT>(Nb.u, B.u, kappa, theta0, dot, phi)) {
+        if (bend_phi<DIM>(Nb.u, B.u, kappa, theta0, dot, phi)) {
 #pragma unroll
             for (int i = 0; i < DIM; ++i) acc[i] -= (double)(phi * (dot * B.u[i] - Nb.u[i]));
         }
