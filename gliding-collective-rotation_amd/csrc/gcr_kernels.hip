@@ -260,10 +260,18 @@ struct Bond {
     double inv64;
 };
 
+// 1/sqrt: the FP64 path keeps the correctly rounded form of the reference's
+// arithmetic; the mixed path uses the hardware estimate (FP32: 1 ulp) and,
+// for the FP64 spring factor, the FP64 estimate refined by one Newton step.
 __device__ __forceinline__ double inv_sqrt(double x) { return 1.0 / sqrt(x); }
-__device__ __forceinline__ float inv_sqrt(float x) { return 1.0f / sqrtf(x); }
+__device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double inv_sqrt_fast(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    return r * fma(-0.5 * x * r, r, 1.5);
+}
 
-template <int DIM, typename BT>
+template <int DIM, typename BT, bool SPRING = true>
 __device__ __forceinline__ void make_bond(const double* xu, const double* xv, const RepParams& P,
                                           Bond<DIM, BT>& b)
 {
@@ -275,11 +283,11 @@ __device__ __forceinline__ void make_bond(const double* xu, const double* xv, co
         t64[i] = wrap1(xu[i] - xv[i], P.L, P.halfL);
         b.t[i] = (BT)t64[i];
         b.ds += b.t[i] * b.t[i];
-        ds64 += t64[i] * t64[i];
+        if (SPRING && sizeof(BT) == 4) ds64 += t64[i] * t64[i];
     }
     if (b.ds != (BT)0) {
         const BT inv = inv_sqrt(b.ds);
-        b.inv64 = sizeof(BT) == 8 ? (double)inv : inv_sqrt(ds64);
+        b.inv64 = sizeof(BT) == 8 ? (double)inv : (SPRING ? inv_sqrt_fast(ds64) : 0.0);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i] * inv;
     } else {
@@ -337,7 +345,7 @@ __device__ __forceinline__ bool bend_phi(const float* ti, const float* tj, float
     if (cn == 0.0f) return false;
     phi = atan2f(cn, dot);
     if (theta0 != 0.0f) phi += theta0 * sgn<float>(-cz);
-    phi *= kappa / cn;
+    phi *= kappa * __builtin_amdgcn_rcpf(cn);
     return true;
 }
 
@@ -382,7 +390,7 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     }
     // angle at p1: forces[b1+1] -= f_bend_backward
     if (bend && lk.x >= 0) {
-        make_bond<DIM, BT>(c.p2, c.p1, P, Pb);
+        make_bond<DIM, BT, false>(c.p2, c.p1, P, Pb);
         BT dot, phi;
         if (bend_phi<DIM>(A.u, Pb.u, kappa, theta0, dot, phi)) {
 #pragma unroll
@@ -436,7 +444,7 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     }
     // angle at n1: forces[b1-1] -= f_bend_forward
     if (bend && lk.w >= 0) {
-        make_bond<DIM, BT>(c.n1, c.n2, P, Nb);
+        make_bond<DIM, BT, false>(c.n1, c.n2, P, Nb);
         BT dot, phi;
         if (bend_phi<DIM>(Nb.u, B.u, kappa, theta0, dot, phi)) {
 #pragma unroll
