@@ -122,6 +122,7 @@ struct gcr_pool {
     uint16_t* NL = nullptr;
     int32_t* NLEN = nullptr;
     double* XR = nullptr;
+    double* OBS = nullptr;  // order-parameter partial sums (4 per chain block + 4)
     int64_t capChk = 0, capBlk = 0, capTile = 0;
     int ntile = 0;
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
@@ -605,6 +606,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CHKSUM, p->capChk));
     A(dalloc(&p->BMAX, std::max(p->capBlk, p->capTile)));
     A(dalloc(&p->TILES, p->capTile * 4));
+    A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
     if (dim == 2) {
         A(dalloc(&p->NL, nb * NL_MAX));
         A(dalloc(&p->NLEN, nb));
@@ -1003,6 +1005,40 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
     return GCR_OK;
 }
 
+int gcr_replica_order(gcr_pool* p, int32_t rid, double out[4])
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (!out) return fail(GCR_ERR_ARG, "null output");
+    if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
+    const RepParams& P = p->P[rid];
+    const double* tang = p->T;
+    if (P.flags & GCR_F_ORIENT) {
+        // tangents at x1 from the bonds, as the download returns them
+        if (p->dim == 2)
+            k_tangents<2><<<p->nblk, BS, 0, p->stream>>>(p->a, rid);
+        else
+            k_tangents<3><<<p->nblk, BS, 0, p->stream>>>(p->a, rid);
+        tang = p->Tn;
+    }
+    const int nblk = (P.n_chains + BS - 1) / BS;
+    k_chain_order<<<nblk, BS, 0, p->stream>>>(p->a, rid, tang, p->dim, p->OBS);
+    k_order_final<<<1, 64, 0, p->stream>>>(p->OBS, nblk);
+    HIPCHK(hipGetLastError());
+    double sums[4];
+    HIPCHK(hipMemcpyAsync(sums, p->OBS + 4 * nblk, sizeof(sums), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    // analysis.py:294-328: polar = |sum e^{i phi}| / C; S = |(q11, q12)| / C,
+    // director = arctan((|q| - q11) / q12) (0 when q12 = 0)
+    const double C = (double)P.n_chains;
+    const double q11 = sums[2], q12 = sums[3], qf = std::hypot(q11, q12);
+    out[0] = std::hypot(sums[0], sums[1]) / C;
+    out[1] = qf / C;
+    out[2] = q12 != 0.0 ? std::atan((qf - q11) / q12) : 0.0;
+    out[3] = C;
+    return GCR_OK;
+}
+
 int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
 {
     if (!p || !ms_out || steps < 1) return fail(GCR_ERR_ARG, "bad arguments");
@@ -1085,7 +1121,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->XIn, p->X1n,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
-                    p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR};
+                    p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR, p->OBS};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

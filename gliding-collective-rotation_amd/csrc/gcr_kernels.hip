@@ -1202,4 +1202,60 @@ __global__ void __launch_bounds__(BS) k_tangents(PoolArgs a, int rsel)
     store_vec<DIM>(a.Tn, s, tang);
 }
 
+// ------------------------------------------------------------ observables
+
+// Order parameters of one replica from its chains' mean directions
+// (output_methods.calc_avg_phi_chain, output_methods.py:492-510, n-fold
+// symmetry 1): phi_c = atan2(sum_b t_y, sum_b t_x) over the chain's beads in
+// bead order. Each block sums, over its chains, cos phi, sin phi,
+// cos 2phi = 2cos^2 phi - 1 and sin 2phi = 2 cos phi sin phi (the Q-tensor
+// terms of analysis.py:294-313) into OBS[4 * block + k]; k_order_final adds
+// the blocks in order, so the result does not depend on scheduling.
+// `tang` holds the tangents by slot (T, or the bond tangents at x1).
+__global__ void __launch_bounds__(BS) k_chain_order(PoolArgs a, int r, const double* tang, int dim, double* OBS)
+{
+    const RepParams& P = a.P[r];
+    const int c = blockIdx.x * BS + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c < P.n_chains) {
+        const int chain = P.chain_off + c;
+        const int first = a.CHAIN_FIRST[chain], len = a.CHAIN_LEN[chain];
+        double sx = 0.0, sy = 0.0;
+        for (int k = 0; k < len; ++k) {
+            const int s = a.SLOT[first + k];
+            sx += tang[(size_t)s * dim];
+            sy += tang[(size_t)s * dim + 1];
+        }
+        const double phi = atan2(sy, sx);
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        v[0] = cs;
+        v[1] = sn;
+        v[2] = 2.0 * cs * cs - 1.0;
+        v[3] = 2.0 * cs * sn;
+    }
+    __shared__ double red[4][BS];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) OBS[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Sums of the per-block terms, in block order, into OBS[4 * nblk + k].
+__global__ void k_order_final(double* OBS, int nblk)
+{
+    const int k = threadIdx.x;
+    if (k >= 4) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += OBS[4 * b + k];
+    OBS[4 * nblk + k] = acc;
+}
+
 }  // namespace gcr

@@ -161,6 +161,14 @@ class Pool:
         check(self._L.gcr_replica_eval_forces(self.h, int(rid), ptr(F), ptr(T)))
         return F, T
 
+    def order_parameters(self, rid):
+        """Polar order, nematic order and director angle of a replica's chains,
+        computed on the device (no position download)."""
+        out = np.zeros(4)
+        check(self._L.gcr_replica_order(self.h, int(rid),
+                                        out.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(polar=out[0], nematic=out[1], director=out[2], n_chains=int(out[3]))
+
     def time_kernels(self, steps):
         out = np.zeros(4)
         check(self._L.gcr_pool_time_kernels(self.h, int(steps),
@@ -220,6 +228,13 @@ class System:
         T = np.empty((self.n, self.dim))
         check(self._L.gcr_eval_forces(self.h, ptr(F), ptr(T)))
         return F, T
+
+    def order_parameters(self):
+        """Polar order, nematic order, director angle (on the device); the
+        system is replica 0 of its one-replica pool."""
+        out = np.zeros(4)
+        check(self._L.gcr_replica_order(self.h, 0, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(polar=out[0], nematic=out[1], director=out[2], n_chains=int(out[3]))
 
 
 def init_chains(dim, L, chain_len, bond, min_dist=0.9, seed=0, max_tries=20000,

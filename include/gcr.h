@@ -126,6 +126,12 @@ int gcr_replica_status(gcr_pool* p, int32_t rid, double out[8]);
 int gcr_replica_download(gcr_pool* p, int32_t rid, double* pos, double* pos1, double* tangent,
                          double* forces);
 int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* tangent);
+/* Order parameters of a replica, computed on the device from its chains'
+ * mean directions phi_c = atan2(sum t_y, sum t_x) (output_methods.py:492-510,
+ * tangents as gcr_replica_download returns them): out = [polar order
+ * |sum e^{i phi}|/C, nematic order S, director angle, C]
+ * (analysis.py:294-328). */
+int gcr_replica_order(gcr_pool* p, int32_t rid, double out[4]);
 /* Per-kernel timing: runs `steps` steps of every armed replica with the
  * kernels of the batch graphs launched directly; stage0, stage1 and update
  * record start/end HIP events in their own dispatch (hipExtLaunchKernelGGL)

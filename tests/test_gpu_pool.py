@@ -68,3 +68,32 @@ def test_autotune_picks_a_measured_setting():
     assert any(abs(r["skin"] - f * w.phys.wca[3]) < 1e-12 for f in (0.2, 0.3))
     assert len(r["tiling_trials"]) == 6
     assert r["ms_per_step"] == min(t["ms_per_step"] for t in r["schedule_trials"])
+
+
+@pytest.mark.parametrize("kind", ["chains2d", "abp2d", "chains3d"])
+def test_device_order_parameters_match_host(kind):
+    """gcr_replica_order against the host observables on the downloaded
+    tangents (per-chain mean angle, polar and nematic order of
+    analysis.py:294-328)."""
+    from gcr_amd import observables as OB
+    if kind == "chains2d":
+        w = W.chains("o2", 2, 300, 10, phi=0.3, kappa=10.0, vtan=0.3, T=0.01, steps=50, seed=31)
+    elif kind == "abp2d":
+        w = W.chains("oa", 2, 2000, 1, phi=0.3, T=0.01, Dr=0.1, steps=50, seed=32)
+    else:
+        w = W.chains("o3", 3, 100, 6, L=14.0, kappa=4.0, T=0.01, steps=50, seed=33)
+    s = E.System(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                 precision=E.GCR_MIXED)
+    tv = w.time_vals.copy()
+    tv[7] = 50
+    s.run(tv)
+    dev = s.order_parameters()
+    t = s.download()["tangent"]
+    s.close()
+    phi = OB.chain_mean_angle(t, w.chain_id)
+    S, d = OB.nematic_order(phi)
+    print(kind, dev, flush=True)
+    assert dev["n_chains"] == w.phys.n_chains
+    assert abs(dev["polar"] - OB.polar_order(phi)) < 1e-12
+    assert abs(dev["nematic"] - S) < 1e-12
+    assert abs(dev["director"] - d) < 1e-9
