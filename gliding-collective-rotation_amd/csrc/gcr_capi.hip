@@ -696,7 +696,6 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     const int64_t n = d->n_beads, nch = d->n_chains;
     if (n < 1 || nch < 1 || nch > n) return fail(GCR_ERR_ARG, "need 1 <= n_chains <= n_beads");
     if (!(d->L > 0) || !(d->radius > 0)) return fail(GCR_ERR_ARG, "L and radius must be positive");
-    if (d->spring[3] != 0.0) return fail(GCR_ERR_ARG, "FENE springs are not supported");
     if (d->has_wca && !(d->wca[3] > 0 && 2 * d->wca[3] < d->L))
         return fail(GCR_ERR_ARG, "WCA cutoff must be in (0, L/2)");
     if (d->D < 0 || d->Dr < 0) return fail(GCR_ERR_ARG, "D and Dr must be >= 0 (resolve defaults first)");
@@ -763,6 +762,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     P.k = d->spring[0];
     P.kappa = d->spring[1];
     P.theta0 = d->spring[2];
+    P.fene_tol = d->spring[5];
     P.D = d->D;
     P.Dr = d->Dr;
     P.radius = d->radius;
@@ -774,6 +774,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     bool orient = d->has_springs && d->spring[4] != 0.0;
     if (orient) fl |= GCR_F_ORIENT;
     if (d->has_springs && d->spring[1] != 0.0) fl |= GCR_F_BEND;
+    if (d->has_springs && d->spring[3] != 0.0) fl |= GCR_F_FENE;
     if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
     P.flags = fl;
     // host staging of per-bead and per-chain data

@@ -18,6 +18,7 @@
 #define GCR_F_BEND 16u      // kappa != 0
 #define GCR_F_ROTDIFF 32u   // Dr > 0 and not ORIENT (SRK.py:399)
 #define GCR_F_INJECT 64u    // noise read from an injected array
+#define GCR_F_FENE 128u     // spring_params[3]: FENE springs (forces.py:331-341)
 
 struct RepParams {
     int32_t n;          // beads
@@ -43,6 +44,7 @@ struct RepParams {
     double wca[4];      // make_WCA_params (populations.py:48-58)
     double rc2;
     double k, kappa, theta0;  // spring_params[0..2]
+    double fene_tol;          // spring_params[5]: FENE maximal stretch / rest length
     double D, Dr, radius;     // pop_consts[0..2]
     double dtmax, mdps, cheat;
     uint64_t seed;

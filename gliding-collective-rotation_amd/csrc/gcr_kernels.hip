@@ -258,6 +258,7 @@ struct Bond {
     BT u[DIM];
     BT ds;
     double inv64;
+    double ds64;
 };
 
 // 1/sqrt: the FP64 path keeps the correctly rounded form of the reference's
@@ -285,6 +286,7 @@ __device__ __forceinline__ void make_bond(const double* xu, const double* xv, co
         b.ds += b.t[i] * b.t[i];
         if (SPRING && sizeof(BT) == 4) ds64 += t64[i] * t64[i];
     }
+    b.ds64 = sizeof(BT) == 8 ? (double)b.ds : ds64;
     if (b.ds != (BT)0) {
         const BT inv = inv_sqrt(b.ds);
         b.inv64 = sizeof(BT) == 8 ? (double)inv : (SPRING ? inv_sqrt_fast(ds64) : 0.0);
@@ -295,6 +297,23 @@ __device__ __forceinline__ void make_bond(const double* xu, const double* xv, co
 #pragma unroll
         for (int i = 0; i < DIM; ++i) b.u[i] = b.t[i];
     }
+}
+
+// Spring factor f of a bond (the force on its first bead is f * t, t the
+// bond vector, forces.py:326-348), in FP64: Hookean k (a/|t| - 1), or FENE
+// -k st / |st^2/m^2 - 1| / |t| with st = |t| - a and m = tol * a, with the
+// reference's guard (forces.py:336-339), which compares st^2 with m itself
+// and divides by zero when it fails.
+template <int DIM, typename BT>
+__device__ __forceinline__ double spring_factor(const Bond<DIM, BT>& b, double a, const RepParams& P)
+{
+    if (P.flags & GCR_F_FENE) {
+        const double m = P.fene_tol * a;
+        const double st = sqrt(b.ds64) - a;
+        const double den = (st * st != m && m != 0.0) ? fabs(st * st / (m * m) - 1.0) : 0.0;
+        return -P.k * b.inv64 * st * 1.0 / den;
+    }
+    return P.k * (a * b.inv64 - 1.0);
 }
 
 template <typename BT>
@@ -384,7 +403,7 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     if (lk.z >= 0) make_bond<DIM, BT>(xs, c.n1, P, B);
     // bond p1 -> s: forces[b1+1] -= f (forces.py:345-348)
     if (lk.y >= 0 && A.ds != (BT)0) {
-        const BT mag = (BT)(P.k * (a * A.inv64 - 1.0));
+        const BT mag = (BT)spring_factor<DIM, BT>(A, a, P);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) acc[i] -= (double)(A.t[i] * mag);
     }
@@ -399,7 +418,7 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     }
     // bond s -> n1: forces[b1] += f
     if (lk.z >= 0 && B.ds != (BT)0) {
-        const BT mag = (BT)(P.k * (a * B.inv64 - 1.0));
+        const BT mag = (BT)spring_factor<DIM, BT>(B, a, P);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) acc[i] += (double)(B.t[i] * mag);
     }

@@ -81,8 +81,6 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         has_springs = "springs" in forces and longest > 1
         has_wca = "WCA" in forces
         has_propel = "self-propulsion" in forces and bool(status.b_calc_active_forces)
-        if has_springs and forces["springs"][3] != 0:
-            raise NotImplementedError("GPU path: FENE springs are not ported")
         if longest > 2 and getattr(pop, "rigid_rod_projection_factor", 0) != 0:
             raise NotImplementedError("GPU path: rigid-rod projection is not ported")
         dim = int(config.dim)

@@ -66,7 +66,10 @@ void orc_chain_forces(int dim, int n, const double* pos, const int64_t* chain_id
                       const double sp[6], const double* rest_lengths, double* tangent,
                       double* F)
 {
-    /* forces.py:285-402, Hookean branch. The reference walks beads in index
+    /* forces.py:285-402 (Hookean, or FENE when sp[3] == 1: forces.py:331-341
+     * with the maximal stretch sp[5] * rest length, forces.py:311-312, and
+     * the reference's guard, which compares stretch^2 with that length
+     * itself). The reference walks beads in index
      * order, keeps the two most recent normalised bond vectors in a ring
      * (ti = bond b1->b1+1, tj = bond b1-1->b1) and adds each term to the
      * beads it touches as soon as it is known. */
@@ -90,7 +93,15 @@ void orc_chain_forces(int dim, int n, const double* pos, const int64_t* chain_id
                                           tg[ti]);
             if (ds != 0.0) {
                 double inv = pow(ds, -0.5);
-                double mag = sp[0] * (a * inv - 1.0);
+                double mag;
+                if (sp[3] != 0.0) {
+                    double m = sp[5] * a;
+                    double st = sqrt(ds) - a;
+                    double den = (st * st != m && m != 0.0) ? fabs(st * st / (m * m) - 1.0) : 0.0;
+                    mag = -sp[0] * inv * st * 1.0 / den;
+                } else {
+                    mag = sp[0] * (a * inv - 1.0);
+                }
                 for (int i = 0; i < dim; ++i) {
                     double f = tg[ti][i] * mag;
                     F[b1 * dim + i] += f;

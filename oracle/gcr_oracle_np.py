@@ -59,8 +59,9 @@ def chain_bonds(chain_id):
 
 
 def chain_forces(pos, chain_id, L, sp, rest_lengths):
-    """Vectorised restatement of forces.py:285-402 (Hookean branch).
-    Returns (forces, tangents); tangents are NaN where not defined."""
+    """Vectorised restatement of forces.py:285-402 (Hookean, or FENE when
+    sp[3] == 1, forces.py:331-341). Returns (forces, tangents); tangents are
+    NaN where not defined."""
     n, dim = pos.shape
     F = np.zeros_like(pos)
     T = np.full_like(pos, np.nan)
@@ -71,7 +72,14 @@ def chain_forces(pos, chain_id, L, sp, rest_lengths):
     inv = np.zeros_like(ds)
     inv[ok] = ds[ok] ** -0.5
     a = rest_lengths[chain_id[bonds]]
-    mag = np.where(ok, sp[0] * (a * inv - 1.0), 0.0)
+    if sp[3] != 0:
+        m = sp[5] * a
+        st = np.sqrt(ds) - a
+        with np.errstate(divide="ignore", invalid="ignore"):
+            den = np.where((st * st != m) & (m != 0), np.abs(st * st / np.where(m != 0, m * m, 1.0) - 1.0), 0.0)
+            mag = np.where(ok, -sp[0] * inv * st / den, 0.0)
+    else:
+        mag = np.where(ok, sp[0] * (a * inv - 1.0), 0.0)
     f = t * mag[:, None]
     np.add.at(F, bonds, f)
     np.add.at(F, bonds + 1, -f)

@@ -32,8 +32,8 @@ def test_dropin_rejects_unported_features():
     with pytest.raises(NotImplementedError, match="active_sliding"):
         f([pop], status, sim)
     pop.sliding_partner_maxdist = 0.0
-    pop.b_FENE = True
-    with pytest.raises(NotImplementedError, match="FENE"):
+    pop.rigid_rod_projection_factor = 0.5
+    with pytest.raises(NotImplementedError, match="rigid-rod"):
         f([pop], status, sim)
 
 

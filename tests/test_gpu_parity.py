@@ -55,6 +55,14 @@ def cases():
                                       shift=0.1, n_nearby=-1, T=0.01, steps=100, seed=7)
     out["diffusing_chains"] = W.chains("diffusing", 2, 40, 6, L=20.0, T=0.01, Dr=0.05,
                                        tangent_locked=False, vtan=0.3, steps=100, seed=8)
+    # FENE springs (forces.py:331-341): straight chains at rest length, so
+    # every bond starts inside the FENE range (|stretch| < 0.3 a)
+    w = W.chains("fene", 2, 100, 12, phi=0.3, kappa=6.0, vtan=0.2, T=0.01, dtmax=1e-3,
+                 steps=100, seed=12)
+    w.phys.spring = w.phys.spring.copy()
+    w.phys.spring[3] = 1.0
+    w.phys.spring[5] = 0.3
+    out["fene"] = w
     # a wide skin puts about as many beads within rc + skin as a neighbour
     # list holds: some beads keep a list, the others sum over their cells
     out["list_overflow"] = W.chains("list_overflow", 2, 900, 1, phi=0.55,

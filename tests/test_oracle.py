@@ -191,6 +191,49 @@ def test_hookean_bonds_are_energy_gradient():
     assert np.allclose(F.sum(0), 0, atol=1e-12)
 
 
+def fene_energy(pos, chain, L, k, a, tol):
+    """U = -1/2 k m^2 ln(1 - s^2/m^2) per bond, s = |t| - a, m = tol * a: the
+    potential whose gradient the FENE branch of forces.py:331-341 gives while
+    |s| < m."""
+    E = 0.0
+    m = tol * a
+    for b in range(len(pos) - 1):
+        if chain[b] == chain[b + 1]:
+            s = np.linalg.norm(N.min_image(pos[b] - pos[b + 1], L)) - a
+            E += -0.5 * k * m * m * np.log(1.0 - s * s / (m * m))
+    return E
+
+
+def test_fene_bonds_are_energy_gradient():
+    rng = np.random.default_rng(6)
+    pos, tang, chain = straight_chains(rng, 3, 6, L2D, jitter=0.02)
+    tol = 0.3
+    sp = np.array([37.0, 0.0, 0.0, 1.0, 0.0, tol])
+    rl = np.array([0.5, 0.45, 0.6])
+    for c in range(3):  # every bond inside the FENE range
+        d = np.linalg.norm(N.min_image(np.diff(pos[chain == c], axis=0), L2D), axis=1) - rl[c]
+        assert np.all(np.abs(d) < 0.8 * tol * rl[c])
+    F, _ = O.chain_forces(pos, chain, L2D, sp, rl)
+    Fn, _ = N.chain_forces(pos, chain, L2D, sp, rl)
+    assert np.allclose(F, Fn, rtol=1e-12, atol=1e-11)
+    h = 1e-7
+    for b in range(len(pos)):
+        for i in range(2):
+            pp, pm = pos.copy(), pos.copy()
+            pp[b, i] += h
+            pm[b, i] -= h
+            ep = sum(fene_energy(pp[chain == c], chain[chain == c], L2D, 37.0, rl[c], tol) for c in range(3))
+            em = sum(fene_energy(pm[chain == c], chain[chain == c], L2D, 37.0, rl[c], tol) for c in range(3))
+            assert F[b, i] == pytest.approx(-(ep - em) / (2 * h), rel=1e-5, abs=1e-6)
+    # a dimer: the FENE force is the Hookean one divided by 1 - s^2/m^2
+    dim_pos = np.array([[5.0, 5.0], [5.0 + 0.56, 5.0]])
+    two = np.zeros(2, dtype=np.int64)
+    Ff, _ = O.chain_forces(dim_pos, two, L2D, sp, np.array([0.5]))
+    Fh, _ = O.chain_forces(dim_pos, two, L2D, np.array([37.0, 0, 0, 0, 0, tol]), np.array([0.5]))
+    s_, m_ = 0.06, tol * 0.5
+    assert Ff[0, 0] == pytest.approx(Fh[0, 0] / (1 - s_ * s_ / (m_ * m_)), rel=1e-12)
+
+
 @pytest.mark.parametrize("ell,theta0", [(1.0, 0.0), (0.5, 0.0), (1.3, 0.0), (0.7, 0.3)])
 def test_bending_is_bond_length_times_energy_gradient(ell, theta0):
     """forces.py:361-386 applies kappa*theta/sin(theta) times unit-vector
