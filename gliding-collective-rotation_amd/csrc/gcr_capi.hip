@@ -18,6 +18,8 @@
 
 using namespace gcr;
 
+static_assert(NL3_MAX * 4 == NL_MAX * 2, "3D lists (int32 slots) share the storage of 2D lists (u16)");
+
 namespace {
 
 thread_local std::string g_err;
@@ -324,6 +326,10 @@ void launch_rebuild(gcr_pool* p, int j)
     } else {
         k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
+        if (p->precision == GCR_MIXED)
+            k_nlist3d<float><<<p->nblk, BS, 0, st>>>(a, j);
+        else
+            k_nlist3d<double><<<p->nblk, BS, 0, st>>>(a, j);
     }
 }
 
@@ -534,7 +540,7 @@ int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_ce
     // per bead: 10 dim-vectors + noise (2 x 2 dim) of doubles, 5 int pairs, links,
     // cell and slot ints; 2D adds the neighbour list, its length and the
     // re-sort positions; per chain: 4 doubles + 2 ints; per cell: 2 ints
-    const int64_t nl = dim == 2 ? 2 * NL_MAX + 4 + 8 * dim : 0;
+    const int64_t nl = 2 * NL_MAX + 4 + 8 * dim;
     return n * (8 * (10 * dim + 4 * dim) + 4 * (5 * 2 + 4 + 2) + nl) + n_chains * (8 * 4 + 8) +
            (n_cells + 1) * 8;
 }
@@ -607,11 +613,11 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->BMAX, std::max(p->capBlk, p->capTile)));
     A(dalloc(&p->TILES, p->capTile * 4));
     A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
-    if (dim == 2) {
-        A(dalloc(&p->NL, nb * NL_MAX));
-        A(dalloc(&p->NLEN, nb));
-        A(dalloc(&p->XR, nb * 2));
-    }
+    // neighbour lists: 2D u16 LDS indices (NL_MAX), 3D int32 slots (NL3_MAX),
+    // 128 B per bead either way
+    A(dalloc(&p->NL, nb * NL_MAX));
+    A(dalloc(&p->NLEN, nb));
+    A(dalloc(&p->XR, nb * dim));
     if (rc == GCR_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(GCR_ERR_HIP, "stream creation failed");
     if (rc == GCR_OK && hipMemset(p->CCOUNT, 0, sizeof(int32_t) * (capC + 1)) != hipSuccess)

@@ -227,20 +227,21 @@ __device__ __forceinline__ double outside_cell(const double* x, int c, const Rep
     return worst;
 }
 
-// Half-skin check of a new position of the bead in slot s. 2D: displacement
+// Half-skin check of a new position of the bead in slot s: its displacement
 // since the last re-sort, which keeps the neighbour lists (radius rc + skin)
-// complete for every pair within rc. 3D: excursion outside the cell box,
-// which keeps the 3 x 3 x 3 cell scan complete.
+// complete for every pair within rc, and keeps the bead within half a skin
+// of its cell box, which keeps the cell scan complete.
 template <int DIM>
 __device__ __forceinline__ bool moved_too_far(const PoolArgs& a, const RepParams& P, int s, const double* x)
 {
-    if (DIM == 2) {
-        const double2 r = *reinterpret_cast<const double2*>(a.XR + 2 * s);
-        const double dx = wrap1(x[0] - r.x, P.L, P.halfL);
-        const double dy = wrap1(x[1] - r.y, P.L, P.halfL);
-        return dx * dx + dy * dy > P.skin_half * P.skin_half;
+    double r[DIM], d2 = 0.0;
+    load_vec<DIM>(a.XR, s, r);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        const double d = wrap1(x[i] - r[i], P.L, P.halfL);
+        d2 += d * d;
     }
-    return outside_cell<DIM>(x, a.CELL[s], P) > P.skin_half;
+    return d2 > P.skin_half * P.skin_half;
 }
 
 // ------------------------------------------------------------ bonded forces
@@ -662,6 +663,153 @@ __device__ __forceinline__ void add_pair_sum(const double* X, const int32_t* MET
     for (int i = 0; i < DIM; ++i) acc[i] += (double)f[i];
 }
 
+// ------------------------------------------------------------ 3D lists
+
+#define NL3_MAX 32  // 3D neighbour-list entries per bead: 8 groups of 4 slots
+
+// Cell offset in {-1, 0, 1} between two cell coordinates of a grid of nc >= 3
+// cells per side (the stencil offset pair_sum visits the partner's cell at).
+__device__ __forceinline__ int cell_offset(int cj, int ci, int nc)
+{
+    const int d = cj - ci;
+    return d > 1 ? d - nc : (d < -1 ? d + nc : d);
+}
+
+// Pair forces of a 3D bead over its list of global slots (groups of 4 at
+// NL[g * stride + s], the last group padded with s itself). Same order and
+// the same operations per pair as pair_sum, so either gives the same bits.
+template <typename FT>
+__device__ __forceinline__ void list_pair_sum3(const PoolArgs& a, const RepParams& P, const double* X, int s,
+                                               const double* xs, int cell, int len, FT* f)
+{
+    const int nc = P.nc;
+    const int ci[3] = {cell % nc, (cell / nc) % nc, cell / (nc * nc)};
+    const PairConsts<FT> k = pair_consts<FT>(P);
+    FT ri[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ri[i] = (FT)cell_rel(xs[i], ci[i], P);
+    const int4* G = reinterpret_cast<const int4*>(a.NL) + s;
+    const int ng = (len + 3) >> 2;
+    for (int g = 0; g < ng; ++g) {
+        const int4 e = G[(int64_t)g * a.nl_stride];
+        const int ts[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = ts[u];
+            double xj[3];
+            load_vec<3>(X, t, xj);
+            const int cj = a.CELL[t];
+            const int cjc[3] = {cj % nc, (cj / nc) % nc, cj / (nc * nc)};
+            FT sep[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                sep[i] = cell_sep<FT>((FT)cell_offset(cjc[i], ci[i], nc), k.h, (FT)cell_rel(xj[i], cjc[i], P), ri[i]);
+            if (k.shifted)
+                pair_term_shifted<3, FT>(sep, k, f);
+            else
+                pair_term<3, FT>(sep, k, f);
+        }
+    }
+}
+
+// Pair forces of a bead, added to its FP64 force: from its list when it has
+// one (3D), else by the cell scan.
+template <int DIM, typename FT>
+__device__ __forceinline__ void add_pairs(const PoolArgs& a, const RepParams& P, const double* X, int s,
+                                          const double* xs, int2 me, double* acc)
+{
+    const int len = DIM == 3 ? a.NLEN[s] : -1;
+    if (DIM == 3 && len >= 0) {
+        FT f[3] = {(FT)0, (FT)0, (FT)0};
+        list_pair_sum3<FT>(a, P, X, s, xs, a.CELL[s], len, f);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) acc[i] += (double)f[i];
+    } else {
+        add_pair_sum<DIM, FT>(X, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
+    }
+}
+
+// 3D neighbour lists after a re-sort (same run condition): each bead scans
+// its 27 cells in pair_sum's order and keeps the slots within h that the
+// same-chain rule does not exclude. NLEN = -1: no list (more than NL3_MAX
+// entries, or fewer than 3 cells per side).
+template <typename FT>
+__global__ void __launch_bounds__(BS) k_nlist3d(PoolArgs a, int j)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    if (!rebuild_runs(a.S + r, j)) return;
+    if (!(P.flags & GCR_F_WCA)) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    const int nc = P.nc;
+    if (nc < 3) {
+        a.NLEN[s] = -1;
+        return;
+    }
+    double xs[3];
+    load_vec<3>(a.X, s, xs);
+    const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+    const int cell = a.CELL[s];
+    const int ci[3] = {cell % nc, (cell / nc) % nc, cell / (nc * nc)};
+    const FT h = (FT)P.h, rl2 = (FT)P.rl2;
+    FT ri[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ri[i] = (FT)cell_rel(xs[i], ci[i], P);
+    int4* NL = reinterpret_cast<int4*>(a.NL);
+    int n = 0;
+    int4 buf = make_int4(s, s, s, s);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = (ci[2] + dz + nc) % nc;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = (ci[1] + dy + nc) % nc;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = (ci[0] + dx + nc) % nc;
+                const int cjc[3] = {x, y, z};
+                const FT dc[3] = {(FT)dx, (FT)dy, (FT)dz};
+                const int c2 = P.cell_off + (z * nc + y) * nc + x;
+                const int t1 = a.CSTART[c2 + 1];
+                for (int t = a.CSTART[c2]; t < t1; ++t) {
+                    if (t == s) continue;
+                    const int2 mj = *reinterpret_cast<const int2*>(a.META + 2 * t);
+                    if (mj.y == me.y) {
+                        if (P.n_nearby < 0) continue;
+                        const int dd = mj.x - me.x;
+                        if ((dd < 0 ? -dd : dd) <= P.n_nearby) continue;
+                    }
+                    double xj[3];
+                    load_vec<3>(a.X, t, xj);
+                    FT sep[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) sep[i] = cell_sep<FT>(dc[i], h, (FT)cell_rel(xj[i], cjc[i], P), ri[i]);
+                    FT ds = sep[0] * sep[0];
+                    ds = fma(sep[1], sep[1], ds);
+                    ds = fma(sep[2], sep[2], ds);
+                    if (!(ds <= rl2)) continue;
+                    if (n < NL3_MAX) {
+                        const int sl = n & 3;
+                        buf.x = sl == 0 ? t : buf.x;
+                        buf.y = sl == 1 ? t : buf.y;
+                        buf.z = sl == 2 ? t : buf.z;
+                        buf.w = sl == 3 ? t : buf.w;
+                        if (sl == 3) {
+                            NL[(int64_t)(n >> 2) * a.nl_stride + s] = buf;
+                            buf = make_int4(s, s, s, s);
+                        }
+                    }
+                    ++n;
+                }
+            }
+        }
+    }
+    if (n <= NL3_MAX) {
+        if (n & 3) NL[(int64_t)(n >> 2) * a.nl_stride + s] = buf;
+        a.NLEN[s] = n;
+    } else {
+        a.NLEN[s] = -1;
+    }
+}
+
 // ------------------------------------------------------------------ stages
 
 // Max over the block (wave64 shuffles, then the block's waves through LDS);
@@ -768,8 +916,7 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
             load_chain_nb<DIM>(Xa, a.LINK, s, cn);
             bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
         }
-        if (P.flags & GCR_F_WCA)
-            add_pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
+        if (P.flags & GCR_F_WCA) add_pairs<DIM, FT>(a, P, Xa, s, xs, me, acc);
         if (P.flags & GCR_F_PROPEL) {
             if (!tang_set) load_vec<DIM>(a.T, s, tang);
             propel<DIM>(a, me.y, tang, acc);
@@ -887,8 +1034,7 @@ __global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
         load_chain_nb<DIM>(Xa, a.LINK, s, cn);
         bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
     }
-    if (P.flags & GCR_F_WCA)
-        add_pair_sum<DIM, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
+    if (P.flags & GCR_F_WCA) add_pairs<DIM, FT>(a, P, Xa, s, xs, me, acc);
     if (!tang_set) load_vec<DIM>(a.T, s, tang);
     if (P.flags & GCR_F_PROPEL) propel<DIM>(a, me.y, tang, acc);
     store_vec<DIM>(a.FT, s, acc);
@@ -1100,7 +1246,7 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
     double x[DIM];
     load_vec<DIM>(a.Xn, ns, x);
     store_vec<DIM>(a.X, ns, x);
-    if (DIM == 2) store_vec<DIM>(a.XR, ns, x);
+    store_vec<DIM>(a.XR, ns, x);
     if (with_t) {
         double t[DIM];
         load_vec<DIM>(a.Tn, ns, t);
