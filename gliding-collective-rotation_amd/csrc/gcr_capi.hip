@@ -109,7 +109,8 @@ struct gcr_pool {
     PoolArgs a{};
     // device buffers owned by the pool
     int32_t *blk_rep = nullptr, *blk_slot0 = nullptr, *chk_rep = nullptr, *chk_cell0 = nullptr,
-            *rep_chk_off = nullptr;
+            *rep_chk_off = nullptr, *chb_rep = nullptr, *chb_chain0 = nullptr;
+    int nchb = 0;  // chain blocks of the rigid-rod replicas
     RepParams* dP = nullptr;
     RepState *dS = nullptr, *dSs = nullptr;
     double *X = nullptr, *X1 = nullptr, *F0 = nullptr, *FT = nullptr, *XI = nullptr, *T = nullptr,
@@ -125,7 +126,7 @@ struct gcr_pool {
     int32_t* NLEN = nullptr;
     double* XR = nullptr;
     double* OBS = nullptr;  // order-parameter partial sums (4 per chain block + 4)
-    int64_t capChk = 0, capBlk = 0, capTile = 0;
+    int64_t capChk = 0, capBlk = 0, capTile = 0, capChb = 0;
     int ntile = 0;
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
     int tile_threads = 0;  // threads per tile workgroup: 256, 512, 1024, 0 = from the tile sizes
@@ -209,7 +210,7 @@ std::vector<int> xcd_order(int first, int n)
 // Block, chunk and tile tables of the live replicas.
 int rebuild_tables(gcr_pool* p)
 {
-    std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0), tl;
+    std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0), tl, hr, hc;
     for (int r = 0; r < p->capR; ++r) {
         rco[r] = (int32_t)cr.size();
         RepParams& P = p->P[r];
@@ -224,6 +225,11 @@ int rebuild_tables(gcr_pool* p)
             cr.push_back(r);
             cc.push_back(P.cell_off + c);
         }
+        if (P.flags & GCR_F_ROD)
+            for (int c = 0; c < P.n_chains; c += BS) {
+                hr.push_back(r);
+                hc.push_back(P.chain_off + c);
+            }
         if (p->dim == 2) {
             const int side = pick_tile(P, p->tile_side);
             P.tile_x = P.tile_y = side;
@@ -262,8 +268,13 @@ int rebuild_tables(gcr_pool* p)
         p->nt_launch = p->tile_threads ? p->tile_threads : (per >= 768 ? 1024 : per >= 384 ? 512 : 256);
     }
     if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk ||
-        (int64_t)tl.size() / 4 > p->capTile)
+        (int64_t)tl.size() / 4 > p->capTile || (int64_t)hr.size() > p->capChb)
         return fail(GCR_ERR_CAPACITY, "block table capacity exceeded");
+    p->nchb = (int)hr.size();
+    if (p->nchb) {
+        HIPCHK(hipMemcpyAsync(p->chb_rep, hr.data(), hr.size() * 4, hipMemcpyHostToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(p->chb_chain0, hc.data(), hc.size() * 4, hipMemcpyHostToDevice, p->stream));
+    }
     p->nblk = (int)br.size();
     p->nchk = (int)cr.size();
     p->ntile = (int)(tl.size() / 4);
@@ -283,6 +294,7 @@ int rebuild_tables(gcr_pool* p)
     HIPCHK(hipStreamSynchronize(p->stream));
     p->a.nblk = p->nblk;
     p->a.nchk = p->nchk;
+    p->a.nchb = p->nchb;
     p->a.ntile = p->ntile;
     p->tables_dirty = false;
     drop_graphs(p);
@@ -385,6 +397,18 @@ void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEv
     }
 }
 
+// Rigid-rod projection of a stage's update (only pools with rod replicas).
+void launch_rod(gcr_pool* p, int stage, int j)
+{
+    if (!p->nchb) return;
+    if (p->dim == 2)
+        stage == 0 ? k_rod<2, 0><<<p->nchb, BS, 0, p->stream>>>(p->a, j)
+                   : k_rod<2, 1><<<p->nchb, BS, 0, p->stream>>>(p->a, j);
+    else
+        stage == 0 ? k_rod<3, 0><<<p->nchb, BS, 0, p->stream>>>(p->a, j)
+                   : k_rod<3, 1><<<p->nchb, BS, 0, p->stream>>>(p->a, j);
+}
+
 void launch_begin(gcr_pool* p)
 {
     if (p->dim == 2)
@@ -399,8 +423,10 @@ void launch_batch(gcr_pool* p, int K)
     for (int j = 0; j < p->B; ++j) {
         if (j % K == 0) launch_rebuild(p, j);
         launch_stage(p, 0, j);
+        launch_rod(p, 0, j);
         launch_stage(p, 1, j);
         launch_stage(p, 3, j);
+        launch_rod(p, 1, j);
         launch_stage(p, 2, j);
     }
 }
@@ -582,6 +608,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->capBlk = capB / BS + capR + 1;
     p->capChk = capC / SCAN_CHUNK + capR + 1;
     p->capTile = dim == 2 ? capC + (int64_t)capR + 1 : 1;  // tiles of one cell at worst
+    p->capChb = capCh / BS + capR + 1;
     int rc = GCR_OK;
     auto A = [&](int r) {
         if (r != GCR_OK && rc == GCR_OK) rc = r;
@@ -592,6 +619,8 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->chk_rep, p->capChk));
     A(dalloc(&p->chk_cell0, p->capChk));
     A(dalloc(&p->rep_chk_off, capR + 1));
+    A(dalloc(&p->chb_rep, p->capChb));
+    A(dalloc(&p->chb_chain0, p->capChb));
     A(dalloc(&p->dP, capR));
     A(dalloc(&p->dS, capR));
     A(dalloc(&p->dSs, capR));
@@ -638,6 +667,8 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.chk_rep = p->chk_rep;
     a.chk_cell0 = p->chk_cell0;
     a.rep_chk_off = p->rep_chk_off;
+    a.chb_rep = p->chb_rep;
+    a.chb_chain0 = p->chb_chain0;
     a.nrep = capR;
     a.P = p->dP;
     a.S = p->dS;
@@ -769,6 +800,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     P.kappa = d->spring[1];
     P.theta0 = d->spring[2];
     P.fene_tol = d->spring[5];
+    P.rod = d->rod;
     P.D = d->D;
     P.Dr = d->Dr;
     P.radius = d->radius;
@@ -781,6 +813,9 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     if (orient) fl |= GCR_F_ORIENT;
     if (d->has_springs && d->spring[1] != 0.0) fl |= GCR_F_BEND;
     if (d->has_springs && d->spring[3] != 0.0) fl |= GCR_F_FENE;
+    // SRK.py:364: projection when the factor is set and the longest chain has
+    // more than two beads
+    if (d->rod != 0.0 && *std::max_element(len.begin(), len.end()) > 2) fl |= GCR_F_ROD;
     if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
     P.flags = fl;
     // host staging of per-bead and per-chain data
@@ -1069,8 +1104,10 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
             HIPCHK(hipEventRecord(ev[7], p->stream));
         }
         launch_stage(p, 0, j, ev[0], ev[1]);
+        launch_rod(p, 0, j);
         launch_stage(p, 1, j, ev[2], ev[3]);
         launch_stage(p, 3, j);
+        launch_rod(p, 1, j);
         launch_stage(p, 2, j, ev[4], ev[5]);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(p->stream));
@@ -1123,7 +1160,8 @@ int gcr_pool_destroy(gcr_pool* p)
     drop_graphs(p);
     for (double* q : p->xi_dev)
         if (q) (void)hipFree(q);
-    void* bufs[] = {p->blk_rep, p->blk_slot0, p->chk_rep, p->chk_cell0, p->rep_chk_off, p->dP, p->dS,
+    void* bufs[] = {p->blk_rep, p->blk_slot0, p->chk_rep, p->chk_cell0, p->rep_chk_off, p->chb_rep,
+                    p->chb_chain0, p->dP, p->dS,
                     p->dSs, p->X, p->X1, p->F0, p->FT, p->XI, p->T, p->Xn, p->Tn, p->Xs, p->Ts,
                     p->XIn, p->X1n,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,

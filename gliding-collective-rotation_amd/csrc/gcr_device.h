@@ -19,6 +19,7 @@
 #define GCR_F_ROTDIFF 32u   // Dr > 0 and not ORIENT (SRK.py:399)
 #define GCR_F_INJECT 64u    // noise read from an injected array
 #define GCR_F_FENE 128u     // spring_params[3]: FENE springs (forces.py:331-341)
+#define GCR_F_ROD 256u      // rigid-rod projection of the updates (updates.py:347-499)
 
 struct RepParams {
     int32_t n;          // beads
@@ -45,6 +46,7 @@ struct RepParams {
     double rc2;
     double k, kappa, theta0;  // spring_params[0..2]
     double fene_tol;          // spring_params[5]: FENE maximal stretch / rest length
+    double rod;               // rigid_rod_projection_factor
     double D, Dr, radius;     // pop_consts[0..2]
     double dtmax, mdps, cheat;
     uint64_t seed;
@@ -73,6 +75,10 @@ struct PoolArgs {
     const int32_t* blk_rep;
     const int32_t* blk_slot0;
     int32_t nblk;
+    // chain-block tables of the replicas with rigid-rod projection
+    const int32_t* chb_rep;
+    const int32_t* chb_chain0;
+    int32_t nchb;
     // chunk tables for the cell scan
     const int32_t* chk_rep;
     const int32_t* chk_cell0;

@@ -874,7 +874,8 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
             x1[i] = pymod(tmp, P.L);
         }
         store_vec<DIM>(a.X1, s, x1);
-        if (moved_too_far<DIM>(a, P, s, x1)) atomicOr(&S->violation, 1);
+        // a rod replica's x1 is replaced (and checked) by k_rod
+        if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) atomicOr(&S->violation, 1);
         return 0.0;
     } else {
         store_vec<DIM>(a.FT, s, acc);
@@ -974,17 +975,19 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
     const double s2 = sqrt(2.0 * dt_used * P.D);
     dt_eff /= P.radius;
     double x[DIM], f[DIM], xi[2 * DIM];
-    load_vec<DIM>(a.X, s, x);
-    load_vec<DIM>(a.FT, s, f);
-    load_vec<DIM>(a.XI, s, xi);
+    if (!(P.flags & GCR_F_ROD)) {  // rod replicas: the corrector ran in k_rod
+        load_vec<DIM>(a.X, s, x);
+        load_vec<DIM>(a.FT, s, f);
+        load_vec<DIM>(a.XI, s, xi);
 #pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-        double tmp = x[i] + dt_eff * f[i];
-        if (s2 != 0.0) tmp += s2 * xi[i];
-        x[i] = pymod(tmp, P.L);
+        for (int i = 0; i < DIM; ++i) {
+            double tmp = x[i] + dt_eff * f[i];
+            if (s2 != 0.0) tmp += s2 * xi[i];
+            x[i] = pymod(tmp, P.L);
+        }
+        store_vec<DIM>(a.X, s, x);
+        if (moved_too_far<DIM>(a, P, s, x)) atomicOr(&a.S[r].violation, 1);
     }
-    store_vec<DIM>(a.X, s, x);
-    if (moved_too_far<DIM>(a, P, s, x)) atomicOr(&a.S[r].violation, 1);
     if (P.flags & GCR_F_ROTDIFF) {
         const double sr = sqrt(2.0 * dt_used * P.Dr) * 3.141592653589793;
         double t[DIM], nr[DIM], proj = 0.0;
@@ -1365,6 +1368,145 @@ __global__ void __launch_bounds__(BS) k_tangents(PoolArgs a, int rsel)
     bonded<DIM, double>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, set);
     if (!set) load_vec<DIM>(a.T, s, tang);
     store_vec<DIM>(a.Tn, s, tang);
+}
+
+// ------------------------------------------------------- rigid-rod updates
+
+// Rigid-rod projection of one stage's update (updates.py:347-499), one thread
+// per chain, for replicas with GCR_F_ROD; STAGE 0 after k_stage<0> (x1 from
+// x, F0, dt_prev), STAGE 1 after k_finalize (x from x1 -- the position the
+// stage-1 forces were evaluated at, as SRK.py:364-371 passes it -- with
+// F0 + F1 and dt_used / 2). The update of a bead is dt/r F + sqrt(2 D dt) xi
+// with the stage's dt (dt_prev or dt_used / 2). Sums run over the chain's
+// beads in order, as the reference's loops do; the chain is re-read in each
+// pass instead of being held in registers.
+template <int DIM, int STAGE>
+__global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
+{
+    const int r = a.chb_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    const RepState* S = a.S + r;
+    if (STAGE == 0) {
+        if (!step_runs(S, j)) return;
+    } else if (S->frozen || S->steps != S->batch_base + j + 1) {
+        return;
+    }
+    const int c = a.chb_chain0[blockIdx.x] + threadIdx.x;
+    if (c >= P.chain_off + P.n_chains) return;
+    const double* prev = STAGE == 0 ? a.X : a.X1;
+    const double* F = STAGE == 0 ? a.F0 : a.FT;
+    double* out = STAGE == 0 ? a.X1 : a.X;
+    const double dt = STAGE == 0 ? S->dt : S->dt_used * 0.5;
+    const double dt_over_r = dt / P.radius;
+    const double sq = sqrt(2.0 * P.D * dt);
+    const double L = P.L, half = 0.5 * L, fac = P.rod;
+    const int first = a.CHAIN_FIRST[c], cl = a.CHAIN_LEN[c];
+    const double ca = a.CHAIN_A[c];
+    const double half_clm1 = (cl - 1) / 2.0;
+    auto slot = [&](int k) { return a.SLOT[first + k]; };
+    auto update = [&](int sl, double* u) {
+        double f[DIM], xi[DIM];
+        load_vec<DIM>(F, sl, f);
+        load_vec<DIM>(a.XI, sl, xi);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            u[i] = dt_over_r * f[i];
+            if (sq != 0.0) u[i] += sq * xi[i];
+        }
+    };
+    double moi = 0.0;
+    for (int k = 0; k < cl; ++k) {
+        const double d = ca * (k - half_clm1);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) moi += d * d;
+    }
+    // pass 1: mean update, centre of mass and head-to-tail direction
+    double p0[DIM], mean[DIM], com[DIM], rh[DIM], rmag = 0.0;
+    load_vec<DIM>(prev, slot(0), p0);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) mean[i] = com[i] = rh[i] = 0.0;
+    for (int k = 0; k < cl; ++k) {
+        const int sl = slot(k);
+        double u[DIM], x[DIM];
+        update(sl, u);
+        load_vec<DIM>(prev, sl, x);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            mean[i] += u[i];
+            double sep = x[i] - p0[i];
+            if (sep < -half)
+                sep += L;
+            else if (sep > half)
+                sep -= L;
+            com[i] += sep;
+            if (k == cl - 1) {
+                rh[i] = sep;
+                rmag += sep * sep;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        mean[i] /= cl;
+        com[i] = pymod(p0[i] + com[i] / cl, L);
+    }
+    if (rmag > 0) {
+        rmag = sqrt(rmag);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) rh[i] /= rmag;
+    }
+    // pass 2: the torque term
+    double tq[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < cl; ++k) {
+        double u[DIM];
+        update(slot(k), u);
+        if (DIM == 2) {
+            const double d = ca * (k - half_clm1);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) tq[i] += (2 * i - 1) * rh[1 - i] * u[i] * d;
+        } else {
+            tq[0] += rh[1] * u[2] - rh[2] * u[1];
+            tq[1] += rh[2] * u[0] - rh[0] * u[2];
+            tq[2] += rh[0] * u[1] - rh[1] * u[0];
+        }
+    }
+    double rot[3] = {0.0, 0.0, 0.0};  // 3D: (|r| / I) (torque x r_hat), the same for every bead
+    if (DIM == 2) {
+        if (moi != 0)
+            for (int i = 0; i < 2; ++i) tq[i] /= moi;
+    } else {
+        for (int i = 0; i < 3; ++i) tq[i] *= rmag;
+        const double w = rmag / moi;
+        rot[0] = w * (tq[1] * rh[2] - tq[2] * rh[1]);
+        rot[1] = w * (tq[2] * rh[0] - tq[0] * rh[2]);
+        rot[2] = w * (tq[0] * rh[1] - tq[1] * rh[0]);
+    }
+    // pass 3: blend with the rigid-rod update and write
+    bool viol = false;
+    for (int k = 0; k < cl; ++k) {
+        const int sl = slot(k);
+        const double d = ca * (k - half_clm1);
+        double u[DIM], x[DIM], y[DIM];
+        update(sl, u);
+        load_vec<DIM>(prev, sl, x);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            double ool = x[i] - (com[i] + rh[i] * d);
+            if (ool < -half)
+                ool += L;
+            else if (ool >= half)
+                ool -= L;
+            double v = u[i] * (1.0 - fac);
+            if (DIM == 2)
+                v += fac * (-ool + mean[i] + (tq[i] * d * (2 * i - 1) * rh[1 - i]));
+            else
+                v += fac * (-ool + mean[i] + rot[i]);
+            y[i] = pymod(x[i] + v, L);
+        }
+        store_vec<DIM>(out, sl, y);
+        viol |= moved_too_far<DIM>(a, P, sl, y);
+    }
+    if (viol) atomicOr(&a.S[r].violation, 1);
 }
 
 // ------------------------------------------------------------ observables

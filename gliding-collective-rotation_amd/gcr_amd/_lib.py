@@ -38,6 +38,7 @@ class ReplicaDesc(C.Structure):
         ("v0", C.c_double),
         ("skin", C.c_double),
         ("seed", C.c_uint64),
+        ("rod", C.c_double),
     ]
 
 

@@ -81,8 +81,6 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         has_springs = "springs" in forces and longest > 1
         has_wca = "WCA" in forces
         has_propel = "self-propulsion" in forces and bool(status.b_calc_active_forces)
-        if longest > 2 and getattr(pop, "rigid_rod_projection_factor", 0) != 0:
-            raise NotImplementedError("GPU path: rigid-rod projection is not ported")
         dim = int(config.dim)
         L = float(config.Lx)
         n = int(pop.total_pop)
@@ -97,12 +95,18 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
             vtan_chain = np.full(n_chains, vtan)
         n_nearby = int(getattr(pop, "n_nearby_beads_dont_repel", 0))
         rest = np.asarray(pop.rest_lengths, dtype=np.float64)
+        rod = float(getattr(pop, "rigid_rod_projection_factor", 0.0))
+        # cells of at least L/512 (2D) or L/64 (3D) per side, so that the tiny
+        # cutoffs of the growth phase (updates.py:304-310) do not ask for a
+        # grid of millions of cells
+        rc = float(wca[3])
+        skin = max(0.3 * rc, L / (512 if dim == 2 else 64) - rc)
         phys = E.Physics(dim=dim, n_beads=n, n_chains=n_chains, L=L, D=float(pop.D),
                          Dr=float(max(pop.Dr, 0.0)), radius=float(pop.repel_d_max), wca=wca,
                          spring=spring, v0=v0, n_nearby=n_nearby, has_springs=has_springs,
-                         has_wca=has_wca, has_propel=has_propel)
+                         has_wca=has_wca, has_propel=has_propel, skin=skin, rod=rod)
         sig = (n, dim, L, phys.D, phys.Dr, phys.radius, n_nearby, tuple(wca), tuple(spring), v0,
-               has_springs, has_wca, has_propel, rest.tobytes(),
+               has_springs, has_wca, has_propel, rod, rest.tobytes(),
                np.asarray(vtan_chain, dtype=np.float64).tobytes(),
                np.asarray(pop.chain_id).tobytes())
         R = resident.setdefault(id(pop), _Resident())

@@ -34,6 +34,7 @@ class Physics:
     has_propel: bool = True
     skin: float = 0.0
     seed: int = 0
+    rod: float = 0.0  # rigid_rod_projection_factor
 
     def desc(self):
         d = ReplicaDesc()
@@ -54,6 +55,7 @@ class Physics:
         d.v0 = float(self.v0)
         d.skin = float(self.skin)
         d.seed = int(self.seed) & ((1 << 64) - 1)
+        d.rod = float(self.rod)
         return d
 
 

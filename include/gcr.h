@@ -53,6 +53,8 @@ typedef struct gcr_replica_desc {
     double v0;           /* active_params[0], used when v0_per_chain is NULL */
     double skin;         /* cell-list skin; <= 0 picks 0.3 * cutoff */
     uint64_t seed;       /* key of the in-kernel Philox noise */
+    double rod;          /* rigid_rod_projection_factor (pop_consts[3]); used when the
+                            longest chain has more than two beads (SRK.py:364) */
 } gcr_replica_desc;
 
 /* ---------------------------------------------------------------- errors */

@@ -434,6 +434,104 @@ void orc_bead_noise(int dim, uint64_t seed, uint32_t orig, int64_t step, double*
 
 /* ---------------------------------------------------------------------- */
 
+void orc_rod_project(int dim, int n, double* rtn, const double* prev, const double* F,
+                     const double* xi, int nd, const int64_t* chain_id, const double* rest_lengths,
+                     double factor, double L, double dt, double D, double radius)
+{
+    /* updates.py:347-499, chain by chain (chains are runs of chain_id). */
+    double half = 0.5 * L;
+    double dt_over_r = dt / radius;
+    double sq = sqrt(2.0 * D * dt);
+    int b0 = 0;
+    while (b0 < n) {
+        int64_t ci = chain_id[b0];
+        int cl = 0;
+        while (b0 + cl < n && chain_id[b0 + cl] == ci) ++cl;
+        double a = rest_lengths[ci];
+        double half_clm1 = (cl - 1) / 2.0;
+        double* upd = (double*)malloc(sizeof(double) * (size_t)cl * dim);
+        double* dist = (double*)malloc(sizeof(double) * (size_t)cl);
+        double moi = 0.0;
+        for (int k = 0; k < cl; ++k) {
+            dist[k] = a * (k - half_clm1);
+            for (int i = 0; i < dim; ++i) moi += dist[k] * dist[k]; /* summed once per axis */
+        }
+        for (int k = 0; k < cl; ++k)
+            for (int i = 0; i < dim; ++i) {
+                upd[k * dim + i] = dt_over_r * F[(b0 + k) * dim + i];
+                if (sq != 0.0) upd[k * dim + i] += sq * xi[(size_t)(b0 + k) * nd + i];
+            }
+        double mean[3] = {0, 0, 0}, com[3] = {0, 0, 0}, rh[3] = {0, 0, 0}, rmag = 0.0;
+        for (int k = 0; k < cl; ++k)
+            for (int i = 0; i < dim; ++i) mean[i] += upd[k * dim + i];
+        for (int i = 0; i < dim; ++i) mean[i] /= cl;
+        for (int k = 0; k < cl; ++k)
+            for (int i = 0; i < dim; ++i) {
+                double sep = prev[(b0 + k) * dim + i] - prev[b0 * dim + i];
+                if (sep < -half)
+                    sep += L;
+                else if (sep > half)
+                    sep -= L;
+                com[i] += sep;
+                if (k == cl - 1) {
+                    rh[i] = sep;
+                    rmag += sep * sep;
+                }
+            }
+        for (int i = 0; i < dim; ++i) com[i] = orc_pymod(prev[b0 * dim + i] + com[i] / cl, L);
+        if (rmag > 0) {
+            rmag = sqrt(rmag);
+            for (int i = 0; i < dim; ++i) rh[i] /= rmag;
+        }
+        double* ool = (double*)malloc(sizeof(double) * (size_t)cl * dim);
+        for (int k = 0; k < cl; ++k)
+            for (int i = 0; i < dim; ++i) {
+                double sep = prev[(b0 + k) * dim + i] - (com[i] + rh[i] * dist[k]);
+                if (sep < -half)
+                    sep += L;
+                else if (sep >= half)
+                    sep -= L;
+                ool[k * dim + i] = sep;
+            }
+        if (dim == 2) {
+            double tq[2] = {0, 0};
+            for (int k = 0; k < cl; ++k)
+                for (int i = 0; i < 2; ++i) tq[i] += (2 * i - 1) * rh[1 - i] * upd[k * 2 + i] * dist[k];
+            if (moi != 0)
+                for (int i = 0; i < 2; ++i) tq[i] /= moi;
+            for (int k = 0; k < cl; ++k)
+                for (int i = 0; i < 2; ++i) {
+                    double u = upd[k * 2 + i] * (1.0 - factor);
+                    u += factor * (-ool[k * 2 + i] + mean[i] + (tq[i] * dist[k] * (2 * i - 1) * rh[1 - i]));
+                    rtn[(b0 + k) * 2 + i] = orc_pymod(prev[(b0 + k) * 2 + i] + u, L);
+                }
+        } else {
+            /* torque = |r| sum_k r_hat x u_k; update += (|r|/I) torque x r_hat */
+            double tq[3] = {0, 0, 0};
+            for (int k = 0; k < cl; ++k) {
+                const double* u = upd + k * 3;
+                tq[0] += rh[1] * u[2] - rh[2] * u[1];
+                tq[1] += rh[2] * u[0] - rh[0] * u[2];
+                tq[2] += rh[0] * u[1] - rh[1] * u[0];
+            }
+            for (int i = 0; i < 3; ++i) tq[i] *= rmag;
+            double c[3] = {tq[1] * rh[2] - tq[2] * rh[1], tq[2] * rh[0] - tq[0] * rh[2],
+                           tq[0] * rh[1] - tq[1] * rh[0]};
+            double w = rmag / moi;
+            for (int k = 0; k < cl; ++k)
+                for (int i = 0; i < 3; ++i) {
+                    double u = upd[k * 3 + i] * (1.0 - factor);
+                    u += factor * (-ool[k * 3 + i] + mean[i] + w * c[i]);
+                    rtn[(b0 + k) * 3 + i] = orc_pymod(prev[(b0 + k) * 3 + i] + u, L);
+                }
+        }
+        free(upd);
+        free(dist);
+        free(ool);
+        b0 += cl;
+    }
+}
+
 int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangent, double* F,
                      const int64_t* chain_id, const double* rest_lengths, const double* vtan,
                      const double tv[8], double status_out[4], const double* xi_in,
@@ -447,6 +545,14 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
     int orient = s->has_springs && (s->spring[4] != 0.0);
     int nd = 2 * dim;
     if (n_threads < 1) n_threads = 1;
+    /* SRK.py:364: projection when the factor is set and the longest chain
+     * has more than two beads */
+    int longest = 0;
+    for (int b = 0, run = 0; b < n; ++b) {
+        run = (b > 0 && chain_id[b] == chain_id[b - 1]) ? run + 1 : 1;
+        if (run > longest) longest = run;
+    }
+    int rod = s->rod != 0.0 && longest > 2;
     double* xi_buf = (double*)malloc(sizeof(double) * (size_t)n * nd);
     double* tmpT = NULL;
     (void)tmpT;
@@ -530,6 +636,14 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
             /* SRK.py:355-388 */
             double dt_eff = dt_for_update * pref;
             double s2 = sqrt(2.0 * dt_for_update * D);
+            if (rod) {
+                /* SRK.py:364-371: the projected update starts from the
+                 * position the forces were evaluated at (pos1 in stage 1)
+                 * and uses sqrt(2 D dt_eff) as noise amplitude */
+                orc_rod_project(dim, n, out, at, F, xi, nd, chain_id, rest_lengths, s->rod, L,
+                                dt_eff, D, r);
+                continue;
+            }
             dt_eff /= r;
 #pragma omp parallel for num_threads(n_threads)
             for (int b = 0; b < n; ++b)

@@ -32,6 +32,7 @@ typedef struct orc_sys {
     double  wca[4];           /* make_WCA_params output (populations.py:48-58) */
     double  spring[6];        /* spring_params (populations.py:345-353) */
     double  v0;               /* active_params[0] (populations.py:355-358) */
+    double  rod;              /* rigid_rod_projection_factor (pop_consts[3]) */
 } orc_sys;
 
 /* Python float modulo (CPython float_rem semantics, used by `tmp %= Lx`
@@ -93,6 +94,14 @@ void orc_bead_noise(int dim, uint64_t seed, uint32_t orig, int64_t step, double*
  * n_steps_update steps, half list with Newton-3 inside each OpenMP chunk, as
  * the reference's threads do).
  * Returns steps done. */
+/* updates.py:347-499 (project_updates_for_rigid_rod): the update of every
+ * chain blended with that of a rigid rod, rtn = (prev + update) % L, for the
+ * chains of [0, n). dt is the update's time step (dt_eff before the division
+ * by the radius), the noise amplitude sqrt(2 D dt). */
+void orc_rod_project(int dim, int n, double* rtn, const double* prev, const double* F,
+                     const double* xi, int nd, const int64_t* chain_id, const double* rest_lengths,
+                     double factor, double L, double dt, double D, double radius);
+
 int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangent,
                      double* forces, const int64_t* chain_id, const double* rest_lengths,
                      const double* v_tan_angle_per_chain, const double time_vals[8],

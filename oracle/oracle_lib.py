@@ -28,6 +28,7 @@ class OrcSys(C.Structure):
         ("wca", C.c_double * 4),
         ("spring", C.c_double * 6),
         ("v0", C.c_double),
+        ("rod", C.c_double),
     ]
 
 
@@ -84,6 +85,7 @@ def make_sys(sysd):
     for i in range(6):
         s.spring[i] = sysd["spring"][i]
     s.v0 = sysd["v0"]
+    s.rod = sysd.get("rod", 0.0)
     return s
 
 
@@ -153,6 +155,25 @@ def force_eval(sysd, pos, chain_id, rest_lengths, vtan, tangent):
                          np.ascontiguousarray(rest_lengths, dtype=np.float64),
                          np.ascontiguousarray(vtan, dtype=np.float64), T, F, 0, None)
     return F, T
+
+
+def rod_project(prev, F, xi, chain_id, rest_lengths, factor, L, dt, D, radius):
+    """orc_rod_project: positions after one projected update."""
+    prev = np.ascontiguousarray(prev, dtype=np.float64)
+    n, dim = prev.shape
+    rtn = np.zeros_like(prev)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    f = lib().orc_rod_project
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_int, np.ctypeslib.ndpointer(np.float64),
+                  np.ctypeslib.ndpointer(np.float64), np.ctypeslib.ndpointer(np.float64),
+                  np.ctypeslib.ndpointer(np.float64), C.c_int, np.ctypeslib.ndpointer(np.int64),
+                  np.ctypeslib.ndpointer(np.float64), C.c_double, C.c_double, C.c_double,
+                  C.c_double, C.c_double]
+    f(dim, n, rtn, prev, np.ascontiguousarray(F, dtype=np.float64), xi, xi.shape[1],
+      np.ascontiguousarray(chain_id, dtype=np.int64),
+      np.ascontiguousarray(rest_lengths, dtype=np.float64), factor, L, dt, D, radius)
+    return rtn
 
 
 def srk2_run(sysd, pos, tangent, chain_id, rest_lengths, vtan, time_vals, xi=None, seed=0,

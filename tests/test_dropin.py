@@ -32,9 +32,12 @@ def test_dropin_rejects_unported_features():
     with pytest.raises(NotImplementedError, match="active_sliding"):
         f([pop], status, sim)
     pop.sliding_partner_maxdist = 0.0
-    pop.rigid_rod_projection_factor = 0.5
-    with pytest.raises(NotImplementedError, match="rigid-rod"):
-        f([pop], status, sim)
+    config.SRK_order = 4
+    try:
+        with pytest.raises(NotImplementedError, match="SRK4"):
+            f([pop], status, sim)
+    finally:
+        config.SRK_order = 2
 
 
 def test_reference_shaped_objects():

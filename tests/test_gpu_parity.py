@@ -23,7 +23,8 @@ def sysd_of(w):
     p = w.phys
     return dict(dim=p.dim, n=p.n_beads, n_nearby=p.n_nearby, has_springs=p.has_springs,
                 has_wca=p.has_wca, has_propel=p.has_propel, L=p.L, D=p.D, Dr=p.Dr,
-                radius=p.radius, wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0)
+                radius=p.radius, wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0,
+                rod=p.rod)
 
 
 def cases():
@@ -63,6 +64,15 @@ def cases():
     w.phys.spring[3] = 1.0
     w.phys.spring[5] = 0.3
     out["fene"] = w
+    # rigid-rod projection of the updates (updates.py:347-499), partial in
+    # 2D and full in 3D
+    w = W.chains("rod2d", 2, 60, 8, phi=0.25, kappa=4.0, vtan=0.3, T=0.01, dtmax=1e-3,
+                 steps=100, seed=14)
+    w.phys.rod = 0.6
+    out["rod2d"] = w
+    w = W.chains("rod3d", 3, 40, 6, L=12.0, kappa=4.0, T=0.01, steps=100, seed=15)
+    w.phys.rod = 1.0
+    out["rod3d"] = w
     # a wide skin puts about as many beads within rc + skin as a neighbour
     # list holds: some beads keep a list, the others sum over their cells
     out["list_overflow"] = W.chains("list_overflow", 2, 900, 1, phi=0.55,

@@ -473,3 +473,72 @@ def test_bead_noise_statistics():
     # counter-based: the same key gives the same numbers, another bead does not
     assert np.array_equal(O.bead_noise(3, 9, 5, 77), O.bead_noise(3, 9, 5, 77))
     assert not np.array_equal(O.bead_noise(3, 9, 5, 77), O.bead_noise(3, 9, 6, 77))
+
+
+# ------------------------------------------------------ rigid-rod projection
+
+def straight_rod(n, a, dim, head, t):
+    return np.array([head - a * k * t for k in range(n)])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rod_projection_factor_zero_is_plain_update(dim):
+    rng = np.random.default_rng(40)
+    prev = rng.uniform(0, L2D, (12, dim))
+    F = rng.normal(size=(12, dim))
+    xi = rng.normal(size=(12, 2 * dim))
+    cid = np.repeat([0, 1, 2], 4).astype(np.int64)
+    out = O.rod_project(prev, F, xi, cid, np.array([0.5, 0.5, 0.5]), 0.0, L2D, 1e-3, 0.02, 1.3)
+    want = np.mod(prev + 1e-3 / 1.3 * F + np.sqrt(2 * 0.02 * 1e-3) * xi[:, :dim], L2D)
+    assert np.allclose(out, want, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rod_projection_uniform_force_translates_rigidly(dim):
+    """Any uniform force in 2D (the 2D torque weights by the distance from the
+    centre, which sums to zero); in 3D the reference's torque is
+    |r| sum_k r_hat x u_k without that weight, so only a force along the rod
+    gives a pure translation there."""
+    t = np.ones(dim) / np.sqrt(dim)
+    prev = straight_rod(7, 0.5, dim, np.full(dim, 10.0), t)
+    f = np.array([1.0, 2.0]) if dim == 2 else 1.7 * t
+    F = np.tile(f, (7, 1))
+    out = O.rod_project(prev, F, np.zeros((7, 2 * dim)), np.zeros(7, dtype=np.int64),
+                        np.array([0.5]), 1.0, L2D, 2e-3, 0.0, 1.0)
+    assert np.allclose(out - prev, 2e-3 * F, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rod_projection_straightens_a_bent_chain(dim):
+    rng = np.random.default_rng(41)
+    t = rng.normal(size=dim)
+    t /= np.linalg.norm(t)
+    a, n = 0.5, 9
+    prev = straight_rod(n, a, dim, np.full(dim, 10.0), t) + rng.normal(0, 0.02, (n, dim))
+    out = O.rod_project(prev, np.zeros((n, dim)), np.zeros((n, 2 * dim)),
+                        np.zeros(n, dtype=np.int64), np.array([a]), 1.0, L2D, 1e-3, 0.0, 1.0)
+    # a straight rod of spacing a through the centre of mass, along head -> tail
+    com = prev.mean(axis=0)
+    rhat = (prev[-1] - prev[0]) / np.linalg.norm(prev[-1] - prev[0])
+    want = com + rhat * a * (np.arange(n) - (n - 1) / 2)[:, None]
+    assert np.allclose(out, want, atol=1e-12)
+
+
+def test_srk2_rod_corrector_starts_from_predictor():
+    """SRK.py:364-371 hands the projection the position the forces were
+    evaluated at; in the corrector that is pos1, so a rigid rod under a
+    uniform force moves (dt_prev + dt_used) v0 / r per step, twice the plain
+    SRK2 displacement."""
+    n, a, v0, dtmax = 6, 0.5, 1.0, 1e-3
+    t = np.array([1.0, 0.0])
+    pos = straight_rod(n, a, 2, np.array([10.0, 10.0]), t)
+    sysd = dict(dim=2, n=n, n_nearby=4, has_springs=True, has_wca=False, has_propel=True,
+                L=L2D, D=0.0, Dr=0.0, radius=1.0, wca=np.array([12.0, 12.0, 0.0, 1.0]),
+                spring=np.array([100.0, 0.0, 0.0, 0.0, 1.0, 0.1]), v0=v0, rod=1.0)
+    tv = np.array([dtmax, dtmax, 1.0, 1.0, -1.0, 1e9, 40.0, 5.0])
+    cid = np.zeros(n, dtype=np.int64)
+    out, *_ = O.srk2_run(sysd, pos, np.tile(t, (n, 1)), cid, np.array([a]), np.zeros(1), tv)
+    assert np.allclose(out - pos, np.tile(5 * 2 * dtmax * v0 * t, (n, 1)), atol=1e-12)
+    sysd["rod"] = 0.0
+    out0, *_ = O.srk2_run(sysd, pos, np.tile(t, (n, 1)), cid, np.array([a]), np.zeros(1), tv)
+    assert np.allclose(out0 - pos, np.tile(5 * dtmax * v0 * t, (n, 1)), atol=1e-12)
