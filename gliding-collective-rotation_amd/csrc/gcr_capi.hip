@@ -207,6 +207,37 @@ std::vector<int> xcd_order(int first, int n)
     return order;
 }
 
+// The physics fields of a replica's parameters (everything but its place in
+// the pool, its cell grid and its noise key).
+void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
+{
+    P.n_nearby = d->n_nearby;
+    P.skin_half = d->has_wca ? 0.5 * (P.h - d->wca[3]) : 1e300;
+    for (int i = 0; i < 4; ++i) P.wca[i] = d->wca[i];
+    P.rc2 = d->wca[3] * d->wca[3];
+    P.k = d->spring[0];
+    P.kappa = d->spring[1];
+    P.theta0 = d->spring[2];
+    P.fene_tol = d->spring[5];
+    P.rod = d->rod;
+    P.D = d->D;
+    P.Dr = d->Dr;
+    P.radius = d->radius;
+    uint32_t fl = P.flags & GCR_F_INJECT;
+    if (d->has_wca) fl |= GCR_F_WCA;
+    if (d->has_springs) fl |= GCR_F_SPRINGS;
+    if (d->has_propel) fl |= GCR_F_PROPEL;
+    bool orient = d->has_springs && d->spring[4] != 0.0;
+    if (orient) fl |= GCR_F_ORIENT;
+    if (d->has_springs && d->spring[1] != 0.0) fl |= GCR_F_BEND;
+    if (d->has_springs && d->spring[3] != 0.0) fl |= GCR_F_FENE;
+    // SRK.py:364: projection when the factor is set and the longest chain has
+    // more than two beads
+    if (d->rod != 0.0 && longest_chain > 2) fl |= GCR_F_ROD;
+    if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
+    P.flags = fl;
+}
+
 // Block, chunk and tile tables of the live replicas.
 int rebuild_tables(gcr_pool* p)
 {
@@ -302,9 +333,12 @@ int rebuild_tables(gcr_pool* p)
 }
 
 // Kernel sequences (launched directly or captured into a graph).
-void launch_rebuild(gcr_pool* p, int j)
+// Cell re-sort, neighbour lists and XR; on_x1: the mid-step re-sort of the
+// safe batch, keyed on the predictor positions.
+void launch_rebuild(gcr_pool* p, int j, bool on_x1 = false)
 {
-    const PoolArgs& a = p->a;
+    PoolArgs a = p->a;
+    a.on_x1 = on_x1 ? 1 : 0;
     hipStream_t st = p->stream;
     if (p->dim == 2)
         k_bin<2><<<p->nblk, BS, 0, st>>>(a, j);
@@ -417,13 +451,19 @@ void launch_begin(gcr_pool* p)
         k_batch_begin<3><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
 }
 
-void launch_batch(gcr_pool* p, int K)
+// One batch of B steps with a re-sort every K steps. The safe batch (K = 1,
+// the replay of a batch that left the skin) also re-sorts a replica on its
+// predictor positions when those left the skin, so every force evaluation
+// sees its complete pair set whatever a step's displacements.
+void launch_batch(gcr_pool* p, int K, bool safe)
 {
+    p->a.safe = safe ? 1 : 0;
     launch_begin(p);
     for (int j = 0; j < p->B; ++j) {
         if (j % K == 0) launch_rebuild(p, j);
         launch_stage(p, 0, j);
         launch_rod(p, 0, j);
+        if (safe) launch_rebuild(p, j, true);
         launch_stage(p, 1, j);
         launch_stage(p, 3, j);
         launch_rod(p, 1, j);
@@ -431,11 +471,12 @@ void launch_batch(gcr_pool* p, int K)
     }
 }
 
-int capture(gcr_pool* p, int K, hipGraphExec_t* out)
+int capture(gcr_pool* p, int K, bool safe, hipGraphExec_t* out)
 {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-    launch_batch(p, K);
+    launch_batch(p, K, safe);
+    p->a.safe = 0;
     HIPCHK(hipStreamEndCapture(p->stream, &g));
     HIPCHK(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
@@ -446,8 +487,8 @@ int ensure_ready(gcr_pool* p)
 {
     int rc;
     if (p->tables_dirty && (rc = rebuild_tables(p)) != GCR_OK) return rc;
-    if (!p->g_norm && (rc = capture(p, p->K, &p->g_norm)) != GCR_OK) return rc;
-    if (!p->g_safe && (rc = capture(p, 1, &p->g_safe)) != GCR_OK) return rc;
+    if (!p->g_norm && (rc = capture(p, p->K, false, &p->g_norm)) != GCR_OK) return rc;
+    if (!p->g_safe && (rc = capture(p, 1, true, &p->g_safe)) != GCR_OK) return rc;
     return GCR_OK;
 }
 
@@ -492,9 +533,9 @@ int replay(gcr_pool* p)
     if ((rc = pull_states(p))) return rc;
     for (int r : which) {
         p->replays[r] += 1;
-        // even a re-sort before every step did not keep this replica's beads
-        // within half a skin of their cells: its pair set may be incomplete,
-        // so it stops here, marked failed, and the others go on
+        // the safe batch re-sorts wherever a step left the skin, so this is
+        // not expected; if it happens the pair set may have been incomplete,
+        // and the replica stops here, marked failed, while the others go on
         if (p->S[r].violation) p->S[r].frozen = FROZEN_FAILED;
     }
     for (int r = 0; r < p->capR; ++r)
@@ -786,38 +827,13 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     P.cell_off = (int32_t)c0;
     P.nc = (int32_t)nc;
     P.ncell = (int32_t)ncell;
-    P.n_nearby = d->n_nearby;
     P.L = d->L;
     P.halfL = d->L / 2.0;
     P.h = d->L / (double)nc;
     P.inv_h = (double)nc / d->L;
-    P.skin_half = 0.5 * (P.h - (d->has_wca ? d->wca[3] : 0.0));
-    if (!d->has_wca) P.skin_half = 1e300;
     P.rl2 = P.h * P.h * (1.0 + 1e-5);  // list radius rc + 2 skin_half = h
-    for (int i = 0; i < 4; ++i) P.wca[i] = d->wca[i];
-    P.rc2 = d->wca[3] * d->wca[3];
-    P.k = d->spring[0];
-    P.kappa = d->spring[1];
-    P.theta0 = d->spring[2];
-    P.fene_tol = d->spring[5];
-    P.rod = d->rod;
-    P.D = d->D;
-    P.Dr = d->Dr;
-    P.radius = d->radius;
     P.seed = d->seed;
-    uint32_t fl = 0;
-    if (d->has_wca) fl |= GCR_F_WCA;
-    if (d->has_springs) fl |= GCR_F_SPRINGS;
-    if (d->has_propel) fl |= GCR_F_PROPEL;
-    bool orient = d->has_springs && d->spring[4] != 0.0;
-    if (orient) fl |= GCR_F_ORIENT;
-    if (d->has_springs && d->spring[1] != 0.0) fl |= GCR_F_BEND;
-    if (d->has_springs && d->spring[3] != 0.0) fl |= GCR_F_FENE;
-    // SRK.py:364: projection when the factor is set and the longest chain has
-    // more than two beads
-    if (d->rod != 0.0 && *std::max_element(len.begin(), len.end()) > 2) fl |= GCR_F_ROD;
-    if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
-    P.flags = fl;
+    set_physics(P, d, *std::max_element(len.begin(), len.end()));
     // host staging of per-bead and per-chain data
     std::vector<int32_t> meta(2 * n), slot(n);
     for (int64_t b = 0; b < n; ++b) {
@@ -898,6 +914,43 @@ int gcr_replica_remove(gcr_pool* p, int32_t rid)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(p->stream));
     p->tables_dirty = true;
+    return GCR_OK;
+}
+
+int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d, const double* rest_lengths,
+                            const double* v0_per_chain)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (!d) return fail(GCR_ERR_ARG, "null descriptor");
+    RepParams& P = p->P[rid];
+    if (d->n_beads != P.n || d->n_chains != P.n_chains || d->L != P.L)
+        return fail(GCR_ERR_ARG, "beads, chains and box must stay the same");
+    if (d->has_wca && !(d->wca[3] > 0 && d->wca[3] < P.h))
+        return fail(GCR_ERR_ARG, "cutoff does not fit the replica's cell grid (re-create the replica)");
+    // the longest chain, from the chain lengths on the device
+    std::vector<int32_t> cl(P.n_chains);
+    HIPCHK(hipMemcpy(cl.data(), p->CHAIN_LEN + P.chain_off, 4 * cl.size(), hipMemcpyDeviceToHost));
+    const int longest = cl.empty() ? 0 : *std::max_element(cl.begin(), cl.end());
+    const uint32_t old = P.flags;
+    set_physics(P, d, longest);
+    if (rest_lengths)
+        HIPCHK(hipMemcpyAsync(p->CHAIN_A + P.chain_off, rest_lengths, 8 * (size_t)P.n_chains,
+                              hipMemcpyHostToDevice, p->stream));
+    {  // speeds: per chain, or d->v0 for every chain (as gcr_replica_add)
+        std::vector<double> cs(3 * (size_t)P.n_chains);
+        HIPCHK(hipMemcpy(cs.data(), p->CHAIN_CS + 3 * (size_t)P.chain_off, 8 * cs.size(), hipMemcpyDeviceToHost));
+        for (int c = 0; c < P.n_chains; ++c) cs[3 * c] = v0_per_chain ? v0_per_chain[c] : d->v0;
+        HIPCHK(hipMemcpyAsync(p->CHAIN_CS + 3 * (size_t)P.chain_off, cs.data(), 8 * cs.size(),
+                              hipMemcpyHostToDevice, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+    }
+    // the parameters only: the device state (dt, counters) stays as it is
+    HIPCHK(hipMemcpyAsync(p->dP + rid, &P, sizeof(RepParams), hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    // rod and pair flags change the launch tables; lists are rebuilt at the
+    // start of every batch, so a new exclusion rule takes effect there
+    if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA)) p->tables_dirty = true;
     return GCR_OK;
 }
 

@@ -68,6 +68,7 @@ struct RepState {
     double mdu;           // max_dt_usable, < 0: unbounded
     int32_t violation;    // a bead left its cell box by more than skin/2
     int32_t frozen;       // 1: kernels leave this replica untouched
+    int32_t pred_far;     // safe batch: this step's predictor left the skin
 };
 
 struct PoolArgs {
@@ -132,4 +133,13 @@ struct PoolArgs {
     double* Ts;
     int32_t* METAs;
     RepState* Ss;
+    // launch modes, fixed when a batch graph is captured:
+    // safe  - the batch that re-sorts before every step: a corrector's skin
+    //         check is moot there (the next step re-sorts first) and a
+    //         predictor that leaves the skin sets pred_far, which re-sorts the
+    //         replica on the predictor positions before the second stage;
+    // on_x1 - that mid-step re-sort: cells, lists and XR from X1, and the
+    //         stage-0 forces travel with the beads
+    int32_t safe;
+    int32_t on_x1;
 };

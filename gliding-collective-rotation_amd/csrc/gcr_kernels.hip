@@ -151,11 +151,22 @@ __device__ __forceinline__ bool step_runs(const RepState* S, int j)
 
 // A cell re-sort placed before local step j runs only when that step runs
 // (j < 0: unconditional, for set-up and diagnostics), so the arrays of the
-// last executed step (x1, forces) keep the layout they were computed in.
-__device__ __forceinline__ bool rebuild_runs(const RepState* S, int j)
+// last executed step (x1, forces) keep the layout they were computed in. The
+// mid-step re-sort on x1 runs only for a replica whose predictor of this
+// step left the skin.
+__device__ __forceinline__ bool rebuild_runs(const PoolArgs& a, int r, int j)
 {
+    const RepState* S = a.S + r;
     if (S->frozen) return false;
+    if (a.on_x1) return step_runs(S, j) && S->pred_far;
     return j < 0 || step_runs(S, j);
+}
+
+// A predictor position outside the skin: outside the safe batch the batch is
+// replayed; inside it the replica is re-sorted on the predictor positions.
+__device__ __forceinline__ void flag_predictor(const PoolArgs& a, RepState* S)
+{
+    atomicOr(a.safe ? &S->pred_far : &S->violation, 1);
 }
 
 template <int DIM>
@@ -738,7 +749,7 @@ __global__ void __launch_bounds__(BS) k_nlist3d(PoolArgs a, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     if (!(P.flags & GCR_F_WCA)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (s >= P.bead_off + P.n) return;
@@ -747,8 +758,9 @@ __global__ void __launch_bounds__(BS) k_nlist3d(PoolArgs a, int j)
         a.NLEN[s] = -1;
         return;
     }
+    const double* Xk = a.on_x1 ? a.X1 : a.X;
     double xs[3];
-    load_vec<3>(a.X, s, xs);
+    load_vec<3>(Xk, s, xs);
     const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
     const int cell = a.CELL[s];
     const int ci[3] = {cell % nc, (cell / nc) % nc, cell / (nc * nc)};
@@ -875,7 +887,7 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
         }
         store_vec<DIM>(a.X1, s, x1);
         // a rod replica's x1 is replaced (and checked) by k_rod
-        if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) atomicOr(&S->violation, 1);
+        if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) flag_predictor(a, S);
         return 0.0;
     } else {
         store_vec<DIM>(a.FT, s, acc);
@@ -986,8 +998,10 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
             x[i] = pymod(tmp, P.L);
         }
         store_vec<DIM>(a.X, s, x);
-        if (moved_too_far<DIM>(a, P, s, x)) atomicOr(&a.S[r].violation, 1);
+        if (!a.safe && moved_too_far<DIM>(a, P, s, x)) atomicOr(&a.S[r].violation, 1);
     }
+    // the step is over: the next mid-step re-sort waits for a new predictor
+    if (threadIdx.x == 0 && blockIdx.x == P.blk_off) a.S[r].pred_far = 0;
     if (P.flags & GCR_F_ROTDIFF) {
         const double sr = sqrt(2.0 * dt_used * P.Dr) * 3.141592653589793;
         double t[DIM], nr[DIM], proj = 0.0;
@@ -1062,13 +1076,13 @@ __global__ void __launch_bounds__(BS) k_bin(PoolArgs a, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     const bool live = s < P.bead_off + P.n;
     int c = -1;
     if (live) {
         double x[DIM];
-        load_vec<DIM>(a.X, s, x);
+        load_vec<DIM>(a.on_x1 ? a.X1 : a.X, s, x);
         c = cell_of<DIM>(x, P);
     }
     const int lane = threadIdx.x & 63;
@@ -1118,7 +1132,7 @@ __global__ void __launch_bounds__(BS) k_scan_chunks(PoolArgs a, int j)
 {
     const int r = a.chk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     const int c0 = a.chk_cell0[blockIdx.x];
     const int cend = P.cell_off + P.ncell;
     int sum = 0;
@@ -1135,7 +1149,7 @@ __global__ void __launch_bounds__(BS) k_scan_chunks(PoolArgs a, int j)
 __global__ void __launch_bounds__(BS) k_scan_reps(PoolArgs a, int j)
 {
     const int r = blockIdx.x;
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     const int k0 = a.rep_chk_off[r], k1 = a.rep_chk_off[r + 1];
     int carry = 0;
     for (int base = k0; base < k1; base += BS) {
@@ -1154,7 +1168,7 @@ __global__ void __launch_bounds__(BS) k_scan_final(PoolArgs a, int j)
 {
     const int r = a.chk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     const int c0 = a.chk_cell0[blockIdx.x];
     const int cend = P.cell_off + P.ncell;
     int v[SCAN_PER_THREAD];
@@ -1180,7 +1194,7 @@ __global__ void __launch_bounds__(BS) k_scatter(PoolArgs a, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (s >= P.bead_off + P.n) return;
     int2 ko = *reinterpret_cast<const int2*>(a.KEYOFF + 2 * s);
@@ -1197,7 +1211,7 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (ns >= P.bead_off + P.n) return;
     const int2 pr = *reinterpret_cast<const int2*>(a.PAIRS + 2 * ns);
@@ -1227,6 +1241,10 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
     }
     load_vec<DIM>(a.X1, so, x);
     store_vec<DIM>(a.X1n, d, x);
+    if (a.on_x1) {  // mid-step: the stage-0 forces are live (FT is free until stage 1)
+        load_vec<DIM>(a.F0, so, x);
+        store_vec<DIM>(a.FT, d, x);
+    }
 }
 
 template <int DIM>
@@ -1234,7 +1252,7 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
-    if (!rebuild_runs(a.S + r, j)) return;
+    if (!rebuild_runs(a, r, j)) return;
     const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (ns >= P.bead_off + P.n) return;
     int2 me = *reinterpret_cast<const int2*>(a.METAn + 2 * ns);
@@ -1249,7 +1267,7 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
     double x[DIM];
     load_vec<DIM>(a.Xn, ns, x);
     store_vec<DIM>(a.X, ns, x);
-    store_vec<DIM>(a.XR, ns, x);
+    if (!a.on_x1) store_vec<DIM>(a.XR, ns, x);
     if (with_t) {
         double t[DIM];
         load_vec<DIM>(a.Tn, ns, t);
@@ -1264,6 +1282,11 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
     }
     load_vec<DIM>(a.X1n, ns, x);
     store_vec<DIM>(a.X1, ns, x);
+    if (a.on_x1) {
+        store_vec<DIM>(a.XR, ns, x);
+        load_vec<DIM>(a.FT, ns, x);
+        store_vec<DIM>(a.F0, ns, x);
+    }
 }
 
 // ------------------------------------------------------------ batch control
@@ -1281,6 +1304,7 @@ __global__ void __launch_bounds__(BS) k_batch_begin(PoolArgs a, int with_t)
     if (threadIdx.x == 0 && blockIdx.x == P.blk_off) {
         S->batch_base = S->steps;
         S->violation = 0;
+        S->pred_far = 0;
         a.Ss[r] = *S;
     }
     if (s >= P.bead_off + P.n) return;
@@ -1506,7 +1530,12 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
         store_vec<DIM>(out, sl, y);
         viol |= moved_too_far<DIM>(a, P, sl, y);
     }
-    if (viol) atomicOr(&a.S[r].violation, 1);
+    if (viol) {
+        if (STAGE == 0)
+            flag_predictor(a, a.S + r);
+        else if (!a.safe)
+            atomicOr(&a.S[r].violation, 1);
+    }
 }
 
 // ------------------------------------------------------------ observables

@@ -267,14 +267,14 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
     const RepParams& P = a.P[tile.x];
-    if (!rebuild_runs(a.S + tile.x, j)) return;
+    if (!rebuild_runs(a, tile.x, j)) return;
     if (!(P.flags & GCR_F_WCA)) return;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ int2 lm[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
     const TileLayout t = tile_layout<NT>(a, P, tile, sh);
-    if (t.in_lds) stage_positions<FT, NT>(a, P, a.X, sh, t, lp, lm, lcell);
+    if (t.in_lds) stage_positions<FT, NT>(a, P, a.on_x1 ? a.X1 : a.X, sh, t, lp, lm, lcell);
     __syncthreads();
     const FT rl2 = (FT)P.rl2, h = (FT)P.h;
     const int nn = P.n_nearby;

@@ -121,6 +121,13 @@ class Pool:
         check(self._L.gcr_replica_remove(self.h, int(rid)))
         self.n.pop(rid, None)
 
+    def set_physics(self, rid, phys, rest_lengths=None, v0_per_chain=None):
+        """Change a replica's physics in place between calls (same beads,
+        chains and box; the cutoff must fit the cell grid made at add)."""
+        d = phys.desc()
+        rl, v0c = f64(rest_lengths), f64(v0_per_chain)
+        check(self._L.gcr_replica_set_physics(self.h, int(rid), C.byref(d), ptr(rl), ptr(v0c)))
+
     def upload(self, rid, pos=None, tangent=None):
         pos, tangent = f64(pos), f64(tangent)
         check(self._L.gcr_replica_upload(self.h, int(rid), ptr(pos), ptr(tangent)))

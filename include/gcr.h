@@ -112,6 +112,14 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* desc, const double* pos
                     const double* v_tan_angle, const double* v0_per_chain, int32_t* rid);
 int gcr_replica_remove(gcr_pool* p, int32_t rid);
 int gcr_replica_upload(gcr_pool* p, int32_t rid, const double* pos, const double* tangent);
+/* Changes a replica's physics in place between calls (forces, constants,
+ * rigid-rod factor, rest lengths when rest_lengths is not NULL, speeds:
+ * v0_per_chain or desc->v0): the parameter changes control.py's
+ * early_time_procedures makes between chunks (control.py:153-219). Bead and
+ * chain counts, box and noise key stay; the cutoff must fit the cell grid
+ * made at gcr_replica_add (desc->skin there sets it). */
+int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* desc,
+                            const double* rest_lengths, const double* v0_per_chain);
 /* Arms the next call of a replica (same meaning of time_vals / xi as gcr_run). */
 int gcr_replica_begin(gcr_pool* p, int32_t rid, const double time_vals[8], const double* xi,
                       int64_t xi_steps);
