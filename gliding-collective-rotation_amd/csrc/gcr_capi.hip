@@ -131,6 +131,7 @@ struct gcr_pool {
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
     int tile_threads = 0;  // threads per tile workgroup: 256, 512, 1024, 0 = from the tile sizes
     int nt_launch = BS;    // resolved by rebuild_tables
+    bool has_srk4 = false; // a live replica integrates with SRK4: the step has 4 force stages
 };
 
 namespace {
@@ -235,13 +236,20 @@ void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
     // more than two beads
     if (d->rod != 0.0 && longest_chain > 2) fl |= GCR_F_ROD;
     if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
+    if (d->srk_order == 4) fl |= GCR_F_SRK4;
     P.flags = fl;
+}
+
+bool srk_order_ok(const gcr_replica_desc* d)
+{
+    return d->srk_order == 0 || d->srk_order == 2 || d->srk_order == 4;
 }
 
 // Block, chunk and tile tables of the live replicas.
 int rebuild_tables(gcr_pool* p)
 {
     std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0), tl, hr, hc;
+    bool has_srk4 = false;
     for (int r = 0; r < p->capR; ++r) {
         rco[r] = (int32_t)cr.size();
         RepParams& P = p->P[r];
@@ -256,6 +264,7 @@ int rebuild_tables(gcr_pool* p)
             cr.push_back(r);
             cc.push_back(P.cell_off + c);
         }
+        if (P.flags & GCR_F_SRK4) has_srk4 = true;
         if (P.flags & GCR_F_ROD)
             for (int c = 0; c < P.n_chains; c += BS) {
                 hr.push_back(r);
@@ -302,6 +311,7 @@ int rebuild_tables(gcr_pool* p)
         (int64_t)tl.size() / 4 > p->capTile || (int64_t)hr.size() > p->capChb)
         return fail(GCR_ERR_CAPACITY, "block table capacity exceeded");
     p->nchb = (int)hr.size();
+    p->has_srk4 = has_srk4;
     if (p->nchb) {
         HIPCHK(hipMemcpyAsync(p->chb_rep, hr.data(), hr.size() * 4, hipMemcpyHostToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(p->chb_chain0, hc.data(), hc.size() * 4, hipMemcpyHostToDevice, p->stream));
@@ -334,11 +344,12 @@ int rebuild_tables(gcr_pool* p)
 
 // Kernel sequences (launched directly or captured into a graph).
 // Cell re-sort, neighbour lists and XR; on_x1: the mid-step re-sort of the
-// safe batch, keyed on the predictor positions.
-void launch_rebuild(gcr_pool* p, int j, bool on_x1 = false)
+// safe batch, keyed on the virtual positions of SRK stage `sub`.
+void launch_rebuild(gcr_pool* p, int j, bool on_x1 = false, int sub = 0)
 {
     PoolArgs a = p->a;
     a.on_x1 = on_x1 ? 1 : 0;
+    a.sub = sub;
     hipStream_t st = p->stream;
     if (p->dim == 2)
         k_bin<2><<<p->nblk, BS, 0, st>>>(a, j);
@@ -395,28 +406,33 @@ void launch_k(K kernel, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1
 
 // 2D tile kernels at the pool's threads per tile.
 template <typename FT, int STAGE>
-void launch_tile(gcr_pool* p, int j, hipEvent_t ev0, hipEvent_t ev1)
+void launch_tile(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (p->nt_launch == 1024)
-        launch_k(k_stage_tile2d<FT, STAGE, 1024>, p->ntile, p->stream, ev0, ev1, p->a, j, 1024);
+        launch_k(k_stage_tile2d<FT, STAGE, 1024>, p->ntile, p->stream, ev0, ev1, a, j, 1024);
     else if (p->nt_launch == 512)
-        launch_k(k_stage_tile2d<FT, STAGE, 512>, p->ntile, p->stream, ev0, ev1, p->a, j, 512);
+        launch_k(k_stage_tile2d<FT, STAGE, 512>, p->ntile, p->stream, ev0, ev1, a, j, 512);
     else
-        launch_k(k_stage_tile2d<FT, STAGE, 256>, p->ntile, p->stream, ev0, ev1, p->a, j, 256);
+        launch_k(k_stage_tile2d<FT, STAGE, 256>, p->ntile, p->stream, ev0, ev1, a, j, 256);
 }
 
-void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+// One launch of a step: which = 0 the first force stage, 1 a later force
+// stage (SRK stage `sub`: 1 and 2 exist in SRK4 only, 3 is the last), 2 the
+// update, 3 the max-force reduction and dt.
+void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                  int sub = 3)
 {
-    const PoolArgs& a = p->a;
+    PoolArgs a = p->a;
+    a.sub = which == 0 ? 0 : sub;
     hipStream_t st = p->stream;
     const bool mixed = p->precision == GCR_MIXED;
     if (which == 3) {
         launch_k(k_finalize, p->capR, st, ev0, ev1, a, j);
     } else if (p->dim == 2) {
         if (which == 0)
-            mixed ? launch_tile<float, 0>(p, j, ev0, ev1) : launch_tile<double, 0>(p, j, ev0, ev1);
+            mixed ? launch_tile<float, 0>(p, a, j, ev0, ev1) : launch_tile<double, 0>(p, a, j, ev0, ev1);
         else if (which == 1)
-            mixed ? launch_tile<float, 1>(p, j, ev0, ev1) : launch_tile<double, 1>(p, j, ev0, ev1);
+            mixed ? launch_tile<float, 1>(p, a, j, ev0, ev1) : launch_tile<double, 1>(p, a, j, ev0, ev1);
         else
             launch_k(k_update<2>, p->nblk, st, ev0, ev1, a, j);
     } else {
@@ -431,16 +447,31 @@ void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEv
     }
 }
 
-// Rigid-rod projection of a stage's update (only pools with rod replicas).
-void launch_rod(gcr_pool* p, int stage, int j)
+// Rigid-rod projection of a stage's update (only pools with rod replicas):
+// stage 0 a virtual update after the force stage of SRK stage `sub`, stage 1
+// the real update.
+void launch_rod(gcr_pool* p, int stage, int j, int sub = 0)
 {
     if (!p->nchb) return;
+    PoolArgs a = p->a;
+    a.sub = stage == 0 ? sub : 3;
     if (p->dim == 2)
-        stage == 0 ? k_rod<2, 0><<<p->nchb, BS, 0, p->stream>>>(p->a, j)
-                   : k_rod<2, 1><<<p->nchb, BS, 0, p->stream>>>(p->a, j);
+        stage == 0 ? k_rod<2, 0><<<p->nchb, BS, 0, p->stream>>>(a, j)
+                   : k_rod<2, 1><<<p->nchb, BS, 0, p->stream>>>(a, j);
     else
-        stage == 0 ? k_rod<3, 0><<<p->nchb, BS, 0, p->stream>>>(p->a, j)
-                   : k_rod<3, 1><<<p->nchb, BS, 0, p->stream>>>(p->a, j);
+        stage == 0 ? k_rod<3, 0><<<p->nchb, BS, 0, p->stream>>>(a, j)
+                   : k_rod<3, 1><<<p->nchb, BS, 0, p->stream>>>(a, j);
+}
+
+// SRK4 virtual update of stage `sub` (1 or 2) for the replicas without rod.
+void launch_virtual(gcr_pool* p, int j, int sub)
+{
+    PoolArgs a = p->a;
+    a.sub = sub;
+    if (p->dim == 2)
+        k_virtual<2><<<p->nblk, BS, 0, p->stream>>>(a, j);
+    else
+        k_virtual<3><<<p->nblk, BS, 0, p->stream>>>(a, j);
 }
 
 void launch_begin(gcr_pool* p)
@@ -453,8 +484,10 @@ void launch_begin(gcr_pool* p)
 
 // One batch of B steps with a re-sort every K steps. The safe batch (K = 1,
 // the replay of a batch that left the skin) also re-sorts a replica on its
-// predictor positions when those left the skin, so every force evaluation
-// sees its complete pair set whatever a step's displacements.
+// virtual positions when those left the skin, so every force evaluation sees
+// its complete pair set whatever a step's displacements. Pools with SRK4
+// replicas run the two inner SRK4 stages between the first and the last
+// force stage; the SRK2 replicas skip them.
 void launch_batch(gcr_pool* p, int K, bool safe)
 {
     p->a.safe = safe ? 1 : 0;
@@ -462,8 +495,14 @@ void launch_batch(gcr_pool* p, int K, bool safe)
     for (int j = 0; j < p->B; ++j) {
         if (j % K == 0) launch_rebuild(p, j);
         launch_stage(p, 0, j);
-        launch_rod(p, 0, j);
-        if (safe) launch_rebuild(p, j, true);
+        launch_rod(p, 0, j, 0);
+        if (safe) launch_rebuild(p, j, true, 0);
+        for (int sub = 1; p->has_srk4 && sub <= 2; ++sub) {
+            launch_stage(p, 1, j, nullptr, nullptr, sub);
+            launch_virtual(p, j, sub);
+            launch_rod(p, 0, j, sub);
+            if (safe) launch_rebuild(p, j, true, sub);
+        }
         launch_stage(p, 1, j);
         launch_stage(p, 3, j);
         launch_rod(p, 1, j);
@@ -777,6 +816,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     if (d->has_wca && !(d->wca[3] > 0 && 2 * d->wca[3] < d->L))
         return fail(GCR_ERR_ARG, "WCA cutoff must be in (0, L/2)");
     if (d->D < 0 || d->Dr < 0) return fail(GCR_ERR_ARG, "D and Dr must be >= 0 (resolve defaults first)");
+    if (!srk_order_ok(d)) return fail(GCR_ERR_ARG, "srk_order must be 2 or 4");
     // chains: runs of equal chain_id with ids 0..nch-1 in order of first appearance
     std::vector<int32_t> first(nch, -1), len(nch, 0);
     int64_t prev = -1;
@@ -923,6 +963,7 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     int rc = check_rid(p, rid);
     if (rc) return rc;
     if (!d) return fail(GCR_ERR_ARG, "null descriptor");
+    if (!srk_order_ok(d)) return fail(GCR_ERR_ARG, "srk_order must be 2 or 4");
     RepParams& P = p->P[rid];
     if (d->n_beads != P.n || d->n_chains != P.n_chains || d->L != P.L)
         return fail(GCR_ERR_ARG, "beads, chains and box must stay the same");
@@ -948,9 +989,10 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     // the parameters only: the device state (dt, counters) stays as it is
     HIPCHK(hipMemcpyAsync(p->dP + rid, &P, sizeof(RepParams), hipMemcpyHostToDevice, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
-    // rod and pair flags change the launch tables; lists are rebuilt at the
-    // start of every batch, so a new exclusion rule takes effect there
-    if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA)) p->tables_dirty = true;
+    // rod, pair and SRK4 flags change the launch tables or the stages of a
+    // step; lists are rebuilt at the start of every batch, so a new exclusion
+    // rule takes effect there
+    if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA | GCR_F_SRK4)) p->tables_dirty = true;
     return GCR_OK;
 }
 
@@ -1087,7 +1129,8 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
     launch_rebuild(p, -1);
     const bool mixed = p->precision == GCR_MIXED;
     if (p->dim == 2)
-        mixed ? launch_tile<float, 2>(p, rid, nullptr, nullptr) : launch_tile<double, 2>(p, rid, nullptr, nullptr);
+        mixed ? launch_tile<float, 2>(p, p->a, rid, nullptr, nullptr)
+              : launch_tile<double, 2>(p, p->a, rid, nullptr, nullptr);
     else
         mixed ? k_eval<3, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
               : k_eval<3, double><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid);
@@ -1139,6 +1182,7 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
     if (!p || !ms_out || steps < 1) return fail(GCR_ERR_ARG, "bad arguments");
     int rc;
     if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
+    if (p->has_srk4) return fail(GCR_ERR_ARG, "kernel timing covers pools of SRK2 replicas");
     // The step sequence of the batch graphs, launched directly; the three
     // per-step kernels record start/end events in their own dispatch packets
     // (hipExtLaunchKernelGGL) on the pool's stream; the re-sort (several

@@ -20,6 +20,7 @@
 #define GCR_F_INJECT 64u    // noise read from an injected array
 #define GCR_F_FENE 128u     // spring_params[3]: FENE springs (forces.py:331-341)
 #define GCR_F_ROD 256u      // rigid-rod projection of the updates (updates.py:347-499)
+#define GCR_F_SRK4 512u     // SRK_order 4 (SRK.py:186-215, 297-321)
 
 struct RepParams {
     int32_t n;          // beads
@@ -68,7 +69,8 @@ struct RepState {
     double mdu;           // max_dt_usable, < 0: unbounded
     int32_t violation;    // a bead left its cell box by more than skin/2
     int32_t frozen;       // 1: kernels leave this replica untouched
-    int32_t pred_far;     // safe batch: this step's predictor left the skin
+    int32_t pred_far;     // safe batch: bit k, the virtual positions of SRK stage k
+                          // left the skin
 };
 
 struct PoolArgs {
@@ -139,7 +141,12 @@ struct PoolArgs {
     //         predictor that leaves the skin sets pred_far, which re-sorts the
     //         replica on the predictor positions before the second stage;
     // on_x1 - that mid-step re-sort: cells, lists and XR from X1, and the
-    //         stage-0 forces travel with the beads
+    //         live force array (SRK2's stage-0 forces, SRK4's running
+    //         total) travels with the beads
     int32_t safe;
     int32_t on_x1;
+    // SRK stage of this launch: 0 the first force evaluation (at X), 1 and 2
+    // the inner SRK4 evaluations, 3 the last (at X1); the re-sort on x1
+    // keys on the virtual positions written by stage `sub`
+    int32_t sub;
 };

@@ -158,15 +158,38 @@ __device__ __forceinline__ bool rebuild_runs(const PoolArgs& a, int r, int j)
 {
     const RepState* S = a.S + r;
     if (S->frozen) return false;
-    if (a.on_x1) return step_runs(S, j) && S->pred_far;
+    if (a.on_x1) return step_runs(S, j) && ((S->pred_far >> a.sub) & 1);
     return j < 0 || step_runs(S, j);
 }
 
-// A predictor position outside the skin: outside the safe batch the batch is
-// replayed; inside it the replica is re-sorted on the predictor positions.
+// A virtual position outside the skin: outside the safe batch the batch is
+// replayed; inside it the replica is re-sorted on those positions before the
+// next force evaluation.
 __device__ __forceinline__ void flag_predictor(const PoolArgs& a, RepState* S)
 {
-    atomicOr(a.safe ? &S->pred_far : &S->violation, 1);
+    atomicOr(a.safe ? &S->pred_far : &S->violation, a.safe ? 1 << a.sub : 1);
+}
+
+// SRK order handling (SRK.py:143-215, 297-321, 355-360).
+// Force prefactor of the real update and of the max force: the mean of two
+// evaluations in SRK2, 1 in SRK4.
+__device__ __forceinline__ double force_prefactor(const RepParams& P)
+{
+    return (P.flags & GCR_F_SRK4) ? 1.0 : 0.5;
+}
+
+// dt_rescale_factor of a virtual update: SRK4 stages 0 and 1 take half the
+// step, stage 2 the whole step; SRK2's predictor the whole step.
+__device__ __forceinline__ double virtual_rescale(const RepParams& P, int sub)
+{
+    return ((P.flags & GCR_F_SRK4) && sub <= 1) ? 0.5 : 1.0;
+}
+
+// Whether a launch of the later force stages (at X1) for SRK stage `sub`
+// has work for this replica: SRK4 evaluates at stages 1, 2 and 3, SRK2 at 3.
+__device__ __forceinline__ bool later_stage_applies(const RepParams& P, int sub)
+{
+    return (P.flags & GCR_F_SRK4) ? sub >= 1 : sub == 3;
 }
 
 template <int DIM>
@@ -857,45 +880,93 @@ __device__ __forceinline__ void propel(const PoolArgs& a, int chain, const doubl
     }
 }
 
+// A virtual update x1 = (x + dt resc / r F + sqrt(2 D dt) xi) mod L with the
+// previous step's dt and this step's noise xi, which every stage shares
+// (SRK.py:355-388). A rod replica's x1 is replaced (and checked) by k_rod.
+template <int DIM>
+__device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParams& P, RepState* S,
+                                               int s, const double* x, const double* f,
+                                               double resc)
+{
+    double xi[DIM];
+    load_vec<DIM>(a.XI, s, xi);
+    const double dt = S->dt;
+    double dt_eff = dt * resc * 1.0;
+    const double s2 = sqrt(2.0 * dt * P.D);
+    dt_eff /= P.radius;
+    double x1[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        double tmp = x[i] + dt_eff * f[i];
+        if (s2 != 0.0) tmp += s2 * xi[i];
+        x1[i] = pymod(tmp, P.L);
+    }
+    store_vec<DIM>(a.X1, s, x1);
+    if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) flag_predictor(a, S);
+}
+
 // What a force stage does with the total force on slot s:
-// STAGE 0: with this step's noise (the corrector reuses the same xi,
-//          SRK.py:379-386), store F0 and the predictor
-//          x1 = (x + dt/r F0 + sqrt(2 D dt) xi) mod L (SRK.py:355-388);
-// STAGE 1: store F = F0 + F1 and return |F|^2 for the max-force reduction;
+// STAGE 0 (forces at x): store F0; SRK4 also starts the weighted total
+//          FT = F0/6; the virtual update from x (SRK2: the predictor);
+// STAGE 1 (forces at x1), SRK2: store F = F0 + F1 and return |F|^2 for the
+//          max-force reduction; SRK4 stage 1, 2: FT += F/3, F0 = F (the
+//          virtual update runs in k_virtual once every bead's forces are
+//          known); SRK4 stage 3: FT += F/6 and return |FT|^2;
 // STAGE 2: (force evaluation only) store the force.
 template <int DIM, int STAGE>
 __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams& P, RepState* S,
                                               int s, int orig_global, const double* xs,
                                               const double* acc)
 {
+    const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
     if (STAGE == 0) {
-        // this step's translational noise, drawn by the previous update (or
-        // the batch start) and carried through re-sorts
-        double xi[DIM];
-        load_vec<DIM>(a.XI, s, xi);
         store_vec<DIM>(a.F0, s, acc);
-        const double dt = S->dt;
-        double dt_eff = dt * 1.0;
-        const double s2 = sqrt(2.0 * dt * P.D);
-        dt_eff /= P.radius;
-        double x1[DIM];
+        if (srk4) {
+            double ft[DIM];
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-            double tmp = xs[i] + dt_eff * acc[i];
-            if (s2 != 0.0) tmp += s2 * xi[i];
-            x1[i] = pymod(tmp, P.L);
+            for (int i = 0; i < DIM; ++i) ft[i] = acc[i] * (1. / 6.);
+            store_vec<DIM>(a.FT, s, ft);
         }
-        store_vec<DIM>(a.X1, s, x1);
-        // a rod replica's x1 is replaced (and checked) by k_rod
-        if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) flag_predictor(a, S);
+        virtual_update<DIM>(a, P, S, s, xs, acc, virtual_rescale(P, 0));
         return 0.0;
-    } else {
+    } else if (!srk4) {
         store_vec<DIM>(a.FT, s, acc);
         double q = 0.0;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) q += acc[i] * acc[i];
         return q;
+    } else {
+        const double w = a.sub == 3 ? 1. / 6. : 1. / 3.;
+        double ft[DIM];
+        load_vec<DIM>(a.FT, s, ft);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) ft[i] = ft[i] + acc[i] * w;
+        store_vec<DIM>(a.FT, s, ft);
+        if (a.sub < 3) {
+            store_vec<DIM>(a.F0, s, acc);
+            return 0.0;
+        }
+        double q = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) q += ft[i] * ft[i];
+        return q;
     }
+}
+
+// SRK4 virtual update of stages 1 and 2 (forces of the stage in F0), from x.
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_virtual(PoolArgs a, int j)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    RepState* S = a.S + r;
+    if (!(P.flags & GCR_F_SRK4) || (P.flags & GCR_F_ROD) || !step_runs(S, j)) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    double x[DIM], f[DIM];
+    load_vec<DIM>(a.X, s, x);
+    load_vec<DIM>(a.F0, s, f);
+    virtual_update<DIM>(a, P, S, s, x, f, virtual_rescale(P, a.sub));
 }
 
 // Stage kernels over bead blocks (any dimension; the 2D production path is
@@ -909,6 +980,7 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
     const RepParams& P = a.P[r];
     RepState* S = a.S + r;
     if (!step_runs(S, j)) return;
+    if (STAGE == 1 && !later_stage_applies(P, a.sub)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     const bool valid = s < P.bead_off + P.n;
     const double* Xa = STAGE == 0 ? a.X : a.X1;
@@ -916,11 +988,11 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
     if (valid) {
         double xs[DIM], acc[DIM], tang[DIM];
         load_vec<DIM>(Xa, s, xs);
-        if (STAGE == 0) {
+        if (STAGE == 1 && !(P.flags & GCR_F_SRK4)) {
+            load_vec<DIM>(a.F0, s, acc);  // SRK2 accumulates into F0
+        } else {
 #pragma unroll
             for (int i = 0; i < DIM; ++i) acc[i] = 0.0;
-        } else {
-            load_vec<DIM>(a.F0, s, acc);
         }
         int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
         bool tang_set = false;
@@ -936,8 +1008,8 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
         }
         fsq = stage_store<DIM, STAGE>(a, P, S, s, me.x, xs, acc);
     }
-    if (STAGE == 1) {
-        // per-block max |F0+F1|^2; k_finalize reduces it per replica
+    if (STAGE == 1 && a.sub == 3) {
+        // per-block max |F|^2 of the total; k_finalize reduces it per replica
         const double m = block_max(fsq);
         if (threadIdx.x == 0) a.BMAX[blockIdx.x] = m;
     }
@@ -959,7 +1031,7 @@ __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
     m2 = block_max(m2);
     if (threadIdx.x == 0) {
         double m = sqrt(2.0 * P.D);
-        if (P.n > 0) m += 0.5 * sqrt(m2) / P.radius;
+        if (P.n > 0) m += force_prefactor(P) * sqrt(m2) / P.radius;
         double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
         double dt_used = P.cheat * dt;
         if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
@@ -983,7 +1055,7 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (s >= P.bead_off + P.n) return;
     const double dt_used = S->dt_used;
-    double dt_eff = dt_used * 0.5;
+    double dt_eff = dt_used * force_prefactor(P);
     const double s2 = sqrt(2.0 * dt_used * P.D);
     dt_eff /= P.radius;
     double x[DIM], f[DIM], xi[2 * DIM];
@@ -1241,9 +1313,14 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
     }
     load_vec<DIM>(a.X1, so, x);
     store_vec<DIM>(a.X1n, d, x);
-    if (a.on_x1) {  // mid-step: the stage-0 forces are live (FT is free until stage 1)
-        load_vec<DIM>(a.F0, so, x);
-        store_vec<DIM>(a.FT, d, x);
+    if (a.on_x1) {
+        // mid-step: one force array is live and the other free -- SRK2's
+        // stage-0 forces (FT is written by the last stage), SRK4's running
+        // total (F0 holds the forces of a stage already applied); the live
+        // one travels through the free one
+        const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
+        load_vec<DIM>(srk4 ? a.FT : a.F0, so, x);
+        store_vec<DIM>(srk4 ? a.F0 : a.FT, d, x);
     }
 }
 
@@ -1284,8 +1361,9 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
     store_vec<DIM>(a.X1, ns, x);
     if (a.on_x1) {
         store_vec<DIM>(a.XR, ns, x);
-        load_vec<DIM>(a.FT, ns, x);
-        store_vec<DIM>(a.F0, ns, x);
+        const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
+        load_vec<DIM>(srk4 ? a.F0 : a.FT, ns, x);
+        store_vec<DIM>(srk4 ? a.FT : a.F0, ns, x);
     }
 }
 
@@ -1397,13 +1475,17 @@ __global__ void __launch_bounds__(BS) k_tangents(PoolArgs a, int rsel)
 // ------------------------------------------------------- rigid-rod updates
 
 // Rigid-rod projection of one stage's update (updates.py:347-499), one thread
-// per chain, for replicas with GCR_F_ROD; STAGE 0 after k_stage<0> (x1 from
-// x, F0, dt_prev), STAGE 1 after k_finalize (x from x1 -- the position the
-// stage-1 forces were evaluated at, as SRK.py:364-371 passes it -- with
-// F0 + F1 and dt_used / 2). The update of a bead is dt/r F + sqrt(2 D dt) xi
-// with the stage's dt (dt_prev or dt_used / 2). Sums run over the chain's
-// beads in order, as the reference's loops do; the chain is re-read in each
-// pass instead of being held in registers.
+// per chain, for replicas with GCR_F_ROD. STAGE 0: a virtual update, after
+// the force stage of SRK stage a.sub (x1 from x in stage 0 and from x1 in the
+// inner SRK4 stages -- the position the forces were evaluated at, as
+// SRK.py:364-371 passes it -- with that stage's forces in F0 and dt_prev
+// times the stage's rescale factor). STAGE 1: the real update after
+// k_finalize (x from x1, with the total force and dt_used times the force
+// prefactor, 1/2 in SRK2). The update of a bead is dt/r F + sqrt(2 D dt) xi
+// with that dt. Sums run over the chain's beads in order, as the reference's
+// loops do; the chain is re-read in each pass instead of being held in
+// registers. Within a stage each thread reads and writes its own chain only,
+// so x1 may be both the input and the output.
 template <int DIM, int STAGE>
 __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
 {
@@ -1412,15 +1494,19 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
     const RepState* S = a.S + r;
     if (STAGE == 0) {
         if (!step_runs(S, j)) return;
+        if (a.sub > 0 && !(P.flags & GCR_F_SRK4)) return;
     } else if (S->frozen || S->steps != S->batch_base + j + 1) {
         return;
     }
     const int c = a.chb_chain0[blockIdx.x] + threadIdx.x;
     if (c >= P.chain_off + P.n_chains) return;
-    const double* prev = STAGE == 0 ? a.X : a.X1;
+    // the projection starts from the positions the stage's forces were
+    // evaluated at (SRK.py:364-371): x in SRK stage 0, x1 after it
+    const double* prev = (STAGE == 0 && a.sub == 0) ? a.X : a.X1;
     const double* F = STAGE == 0 ? a.F0 : a.FT;
     double* out = STAGE == 0 ? a.X1 : a.X;
-    const double dt = STAGE == 0 ? S->dt : S->dt_used * 0.5;
+    const double dt = STAGE == 0 ? S->dt * virtual_rescale(P, a.sub) * 1.0
+                                 : S->dt_used * force_prefactor(P);
     const double dt_over_r = dt / P.radius;
     const double sq = sqrt(2.0 * P.D * dt);
     const double L = P.L, half = 0.5 * L, fac = P.rod;

@@ -333,9 +333,10 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
     }
 }
 
-// STAGE 0 / 1 as k_stage; STAGE 2: force evaluation of replica `j` only at
-// the positions X (forces into FT, tangents into Tn; no noise, no update).
-// NT threads per tile (256 or 512): large tiles keep more waves in flight.
+// STAGE 0 / 1 as k_stage (STAGE 1 for SRK stage a.sub); STAGE 2: force
+// evaluation of replica `j` only at the positions X (forces into FT,
+// tangents into Tn; no noise, no update). NT threads per tile (256, 512 or
+// 1024): large tiles keep more waves in flight.
 template <typename FT, int STAGE, int NT>
 __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
 {
@@ -348,6 +349,7 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
     } else if (!step_runs(S, j)) {
         return;
     }
+    if (STAGE == 1 && !later_stage_applies(P, a.sub)) return;
     const double* Xa = STAGE == 1 ? a.X1 : a.X;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
@@ -388,8 +390,8 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
                 pa1 = (double)pg[1];
             }
         }
-        if (STAGE == 1) {
-            load_vec<2>(a.F0, s, acc);
+        if (STAGE == 1 && !(P.flags & GCR_F_SRK4)) {
+            load_vec<2>(a.F0, s, acc);  // SRK2 accumulates into F0
         } else {
             acc[0] = 0.0;
             acc[1] = 0.0;
@@ -413,7 +415,7 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
             fsq = q > fsq ? q : fsq;
         }
     }
-    if (STAGE == 1) {
+    if (STAGE == 1 && a.sub == 3) {
         const double m = block_max<NT>(fsq);
         if (threadIdx.x == 0) a.BMAX[blockIdx.x] = m;
     }

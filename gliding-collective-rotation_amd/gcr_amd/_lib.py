@@ -39,6 +39,8 @@ class ReplicaDesc(C.Structure):
         ("skin", C.c_double),
         ("seed", C.c_uint64),
         ("rod", C.c_double),
+        ("srk_order", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -13,7 +13,7 @@ the step loop running on the GPU through libgcr.so:
 Device state stays resident between calls; positions are uploaded again only
 when the host changed them. Scope: one moving population of chains (ABPs are
 chains of one bead) with the "springs", "WCA" and "self-propulsion" forces,
-SRK_order 2. Anything else raises NotImplementedError naming the feature.
+SRK_order 2 or 4. Anything else raises NotImplementedError naming the feature.
 """
 import numpy as np
 
@@ -66,8 +66,9 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
             raise NotImplementedError("GPU path: exactly one (moving) population; "
                                       "multi-population interactions are not ported")
         pop = moving[0]
-        if int(getattr(config, "SRK_order", 2)) != 2:
-            raise NotImplementedError("GPU path: SRK_order 2 only (SRK4 is not ported)")
+        order = int(getattr(config, "SRK_order", 2))
+        if order not in (2, 4):
+            raise NotImplementedError(f"GPU path: SRK_order 2 or 4 (not {order})")
         if np.any(getattr(config, "walls", False)):
             raise NotImplementedError("GPU path: walls are not ported")
         if not hasattr(pop, "chain_id"):
@@ -104,9 +105,10 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         phys = E.Physics(dim=dim, n_beads=n, n_chains=n_chains, L=L, D=float(pop.D),
                          Dr=float(max(pop.Dr, 0.0)), radius=float(pop.repel_d_max), wca=wca,
                          spring=spring, v0=v0, n_nearby=n_nearby, has_springs=has_springs,
-                         has_wca=has_wca, has_propel=has_propel, skin=skin, rod=rod)
+                         has_wca=has_wca, has_propel=has_propel, skin=skin, rod=rod,
+                         srk_order=order)
         sig = (n, dim, L, phys.D, phys.Dr, phys.radius, n_nearby, tuple(wca), tuple(spring), v0,
-               has_springs, has_wca, has_propel, rod, rest.tobytes(),
+               has_springs, has_wca, has_propel, rod, order, rest.tobytes(),
                np.asarray(vtan_chain, dtype=np.float64).tobytes(),
                np.asarray(pop.chain_id).tobytes())
         R = resident.setdefault(id(pop), _Resident())

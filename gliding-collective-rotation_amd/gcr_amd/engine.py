@@ -35,6 +35,7 @@ class Physics:
     skin: float = 0.0
     seed: int = 0
     rod: float = 0.0  # rigid_rod_projection_factor
+    srk_order: int = 2  # config.SRK_order (2 or 4)
 
     def desc(self):
         d = ReplicaDesc()
@@ -56,6 +57,7 @@ class Physics:
         d.skin = float(self.skin)
         d.seed = int(self.seed) & ((1 << 64) - 1)
         d.rod = float(self.rod)
+        d.srk_order = int(self.srk_order)
         return d
 
 

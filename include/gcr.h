@@ -55,6 +55,8 @@ typedef struct gcr_replica_desc {
     uint64_t seed;       /* key of the in-kernel Philox noise */
     double rod;          /* rigid_rod_projection_factor (pop_consts[3]); used when the
                             longest chain has more than two beads (SRK.py:364) */
+    int32_t srk_order;   /* config.SRK_order: 4, or 2 (also for 0) */
+    int32_t reserved;
 } gcr_replica_desc;
 
 /* ---------------------------------------------------------------- errors */

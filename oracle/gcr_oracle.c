@@ -592,6 +592,9 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
     }
     void* vp = (pair_mode == 1 && s->has_wca) ? (void*)&vv : NULL;
 
+    const int order = s->order == 4 ? 4 : 2;
+    double* Faux = order == 4 ? (double*)malloc(sizeof(double) * (size_t)n * dim) : NULL;
+
     int64_t steps = 0;
     double elapsed = 0.0, m = 0.0, dt_for_update = dt;
     while ((steps < max_steps || max_steps < 0) && (elapsed < mdu || mdu < 0)) {
@@ -606,7 +609,64 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
             xi = xi_buf;
         }
         memset(F, 0, sizeof(double) * (size_t)n * dim);
-        for (int stage = 0; stage < 2; ++stage) {
+        for (int stage = 0; order == 4 && stage < 4; ++stage) {
+            /* SRK4 (SRK.py:186-215): forces of each stage into a wiped
+             * auxiliary array, at pos in the first stage and at the last
+             * virtual positions after it; total = aux/6 + aux/3 + aux/3 +
+             * aux/6 with the multipliers as doubles (SRK.py:297-318); virtual
+             * updates into pos1 from pos with dt * 1/2, 1/2, 1 and the plain
+             * sqrt(2 D dt) noise; the real update with the total and
+             * dt_used, force prefactor 1 throughout (SRK.py:145, 355-358). */
+            const double* at = stage == 0 ? pos : pos1;
+            double* out = stage == 3 ? pos : pos1;
+            memset(Faux, 0, sizeof(double) * (size_t)n * dim);
+            orc_force_eval(s, at, chain_id, rest_lengths, vtan, tangent, Faux, pair_mode, vp);
+            const double mult = (stage == 0 || stage == 3) ? 1. / 6. : 1. / 3.;
+            for (int k = 0; k < n * dim; ++k)
+                F[k] = stage == 0 ? Faux[k] * mult : F[k] + Faux[k] * mult;
+            const double* src;
+            double resc;
+            if (stage == 3) {
+                double mx = 0.0;
+                for (int b = 0; b < n; ++b) {
+                    double q = 0.0;
+                    for (int i = 0; i < dim; ++i) q += F[b * dim + i] * F[b * dim + i];
+                    if (q > mx) mx = q;
+                }
+                m = sqrt(2.0 * D);
+                if (n > 0) m += 1.0 * sqrt(mx) / r;
+                steps += 1;
+                if (m == 0.0)
+                    dt = dtmax;
+                else
+                    dt = fmin(dtmax, mdps / m);
+                double dt_used = cheat * dt;
+                if (mdu > 0) dt_used = fmin(dt_used, mdu - elapsed);
+                elapsed += dt_used;
+                dt_for_update = dt_used;
+                resc = 1.0;
+                src = F;
+            } else {
+                dt_for_update = dt;
+                resc = stage <= 1 ? 0.5 : 1.0;
+                src = Faux;
+            }
+            double dt_eff = dt_for_update * resc * 1.0;
+            double s2 = sqrt(2.0 * dt_for_update * D);
+            if (rod) {
+                orc_rod_project(dim, n, out, at, src, xi, nd, chain_id, rest_lengths, s->rod, L,
+                                dt_eff, D, r);
+                continue;
+            }
+            dt_eff /= r;
+            for (int b = 0; b < n; ++b)
+                for (int i = 0; i < dim; ++i) {
+                    double tmp = pos[b * dim + i] + dt_eff * src[b * dim + i];
+                    if (s2 != 0.0) tmp += s2 * xi[(size_t)b * nd + i];
+                    out[b * dim + i] = orc_pymod(tmp, L);
+                }
+        }
+        for (int stage = 0; order == 2 && stage < 2; ++stage) {
             double pref = stage == 0 ? 1.0 : 0.5;
             const double* at = stage == 0 ? pos : pos1;
             double* out = stage == 0 ? pos1 : pos;
@@ -682,6 +742,7 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
     status_out[2] = (double)steps;
     status_out[3] = m;
     free(xi_buf);
+    free(Faux);
     if (vp) {
         free(vv.start);
         free(vv.list);

@@ -7,8 +7,8 @@
  * smoke() entry point and bench.py's CPU-baseline leg may load it. The
  * product path (libgcr.so) never links or calls it.
  *
- * Each function cites the reference file:line it restates. Parity status of
- * each function (what pins it) is listed in oracle/README.md.
+ * Each function cites the reference file:line it restates; tests/test_oracle.py
+ * pins each one against closed forms, energy gradients and invariants.
  */
 #ifndef GCR_ORACLE_H
 #define GCR_ORACLE_H
@@ -33,6 +33,7 @@ typedef struct orc_sys {
     double  spring[6];        /* spring_params (populations.py:345-353) */
     double  v0;               /* active_params[0] (populations.py:355-358) */
     double  rod;              /* rigid_rod_projection_factor (pop_consts[3]) */
+    int32_t order;            /* SRK_order: 4, else 2 (config.SRK_order) */
 } orc_sys;
 
 /* Python float modulo (CPython float_rem semantics, used by `tmp %= Lx`
@@ -50,9 +51,9 @@ double orc_wca_coeff(double dist_sq, const double p[4]);
 double orc_inbounds_diff(int dim, const double* a, const double* b, double half_L, double L,
                          double* out);
 
-/* forces.py:285-402 (Hookean branch; FENE is out of scope and rejected by
- * the caller). Accumulates into forces[n*dim]; writes tangent[n*dim] when
- * spring[4] != 0. chain_id is int64 as in populations.py:606. */
+/* forces.py:285-402 (Hookean, or FENE when spring[3] != 0). Accumulates
+ * into forces[n*dim]; writes tangent[n*dim] when spring[4] != 0. chain_id is
+ * int64 as in populations.py:606. */
 void orc_chain_forces(int dim, int n, const double* pos, const int64_t* chain_id, double L,
                       const double spring[6], const double* rest_lengths,
                       double* tangent, double* forces);
@@ -83,8 +84,9 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * xi[0:dim], rotational in xi[dim:2dim]. */
 void orc_bead_noise(int dim, uint64_t seed, uint32_t orig, int64_t step, double* xi);
 
-/* The SRK2 loop of SRK_jit (SRK.py:128-423) for one population with a single
- * thread, SRK_order = 2. time_vals as built at updates.py:144-146:
+/* The SRK loop of SRK_jit (SRK.py:128-423) for one population with a single
+ * thread, SRK_order = s->order (2, or 4: SRK.py:186-215, 297-321, 355-358).
+ * time_vals as built at updates.py:144-146:
  * [dt, dtmax, max_dist_per_step, RK_cheat_factor, max_dt_usable,
  *  steps_until_update, n_steps_update, max_steps].
  * status_out = [dt, elapsed (dt_used sum), steps, max_forces_strength].

@@ -29,6 +29,7 @@ class OrcSys(C.Structure):
         ("spring", C.c_double * 6),
         ("v0", C.c_double),
         ("rod", C.c_double),
+        ("order", C.c_int32),
     ]
 
 
@@ -86,6 +87,7 @@ def make_sys(sysd):
         s.spring[i] = sysd["spring"][i]
     s.v0 = sysd["v0"]
     s.rod = sysd.get("rod", 0.0)
+    s.order = sysd.get("order", 2)
     return s
 
 

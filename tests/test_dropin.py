@@ -32,9 +32,9 @@ def test_dropin_rejects_unported_features():
     with pytest.raises(NotImplementedError, match="active_sliding"):
         f([pop], status, sim)
     pop.sliding_partner_maxdist = 0.0
-    config.SRK_order = 4
+    config.SRK_order = 1
     try:
-        with pytest.raises(NotImplementedError, match="SRK4"):
+        with pytest.raises(NotImplementedError, match="SRK_order 2 or 4"):
             f([pop], status, sim)
     finally:
         config.SRK_order = 2
@@ -77,6 +77,28 @@ def test_dropin_chunks_match_oracle():
     assert status.dt == pytest.approx(st_o[0], rel=1e-10)
     # status.t accumulates the elapsed time of every chunk
     assert status.t == pytest.approx(st_o[1], rel=1e-10)
+
+
+@pytest.mark.gpu
+def test_dropin_srk4_matches_oracle():
+    """config.SRK_order = 4 through the drop-in equals the oracle's SRK4 run."""
+    sim, pop, status = make_run(v_tan_angle=0.15)
+    x0, t0 = pop.pos.copy(), pop.tangent_list.copy()
+    f = dropin.make_SRK2_force_calc(config)
+    config.SRK_order = 4
+    try:
+        status.next_output_step = 40
+        f([pop], status, sim)
+    finally:
+        config.SRK_order = 2
+    seed = dropin._mix(sim["seed"], 1)
+    sysd = dict(dim=2, n=pop.total_pop, n_nearby=4, has_springs=True, has_wca=True,
+                has_propel=True, L=20.0, D=pop.D, Dr=pop.Dr, radius=1.0, wca=pop.WCA_params,
+                spring=pop.spring_params(), v0=1.0, order=4)
+    tv = np.array([sim["dt0"], sim["dtmax"], config.max_dist_per_step, 1.0, -1, 40, 40, 40])
+    pos_o, *_ = O.srk2_run(sysd, x0, t0, pop.chain_id, pop.rest_lengths, np.full(6, 0.15), tv,
+                           xi=None, seed=seed)
+    assert np.max(np.abs(N.min_image(pop.pos - pos_o, 20.0))) < 1e-10 * 20.0
 
 
 @pytest.mark.gpu

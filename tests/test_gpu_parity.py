@@ -24,7 +24,7 @@ def sysd_of(w):
     return dict(dim=p.dim, n=p.n_beads, n_nearby=p.n_nearby, has_springs=p.has_springs,
                 has_wca=p.has_wca, has_propel=p.has_propel, L=p.L, D=p.D, Dr=p.Dr,
                 radius=p.radius, wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0,
-                rod=p.rod)
+                rod=p.rod, order=p.srk_order)
 
 
 def cases():
@@ -73,6 +73,19 @@ def cases():
     w = W.chains("rod3d", 3, 40, 6, L=12.0, kappa=4.0, T=0.01, steps=100, seed=15)
     w.phys.rod = 1.0
     out["rod3d"] = w
+    # SRK_order 4 (SRK.py:186-321): chains, 3D ABPs, and with the rigid rod
+    w = W.chains("srk4_chains", 2, 100, 20, phi=0.3, kappa=10.0, vtan=0.2, T=0.01, dtmax=1e-3,
+                 steps=100, seed=16)
+    w.phys.srk_order = 4
+    out["srk4_chains"] = w
+    w = W.chains("srk4_abp3d", 3, 300, 1, L=10.0, T=0.01, Dr=0.1, steps=100, seed=17)
+    w.phys.srk_order = 4
+    out["srk4_abp3d"] = w
+    w = W.chains("srk4_rod2d", 2, 60, 8, phi=0.25, kappa=4.0, vtan=0.3, T=0.01, dtmax=1e-3,
+                 steps=100, seed=18)
+    w.phys.rod = 0.6
+    w.phys.srk_order = 4
+    out["srk4_rod2d"] = w
     # a wide skin puts about as many beads within rc + skin as a neighbour
     # list holds: some beads keep a list, the others sum over their cells
     out["list_overflow"] = W.chains("list_overflow", 2, 900, 1, phi=0.55,
@@ -214,19 +227,26 @@ def test_deterministic_bits():
         assert np.array_equal(res[0][k], res[1][k]), k
 
 
-def test_rebuild_period_and_replay():
+@pytest.mark.parametrize("order", [2, 4])
+def test_rebuild_period_and_replay(order):
     """Results do not depend on the cell-list schedule beyond rounding; a skin
-    too thin for the schedule is caught and the batch replayed."""
+    too thin for the schedule is caught and the batch replayed (SRK2, and
+    SRK4 with its three virtual stages)."""
     w = CASES["config2_cut"]
     rng = np.random.default_rng(11)
     xi = rng.normal(size=(64, w.n, 4))
     tv = w.time_vals.copy()
     tv[7] = 64
-    pos_o, *_ = oracle_run(w, xi, tv)
+    sysd = sysd_of(w)
+    sysd["order"] = order
+    pos_o, *_ = O.srk2_run(sysd, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                           tv, xi=xi)
+    print(f"[replay] order={order}", flush=True)
     outs = []
     for K, skin in ((1, 0.3), (16, 0.3), (32, 0.1)):
         phys = W.chains("c", 2, 100, 20, phi=0.3, kappa=10.0, vtan=0.2, T=0.01, dtmax=1e-3,
                         steps=64, seed=2, skin=skin).phys
+        phys.srk_order = order
         p = E.Pool(2, E.GCR_FP64, bead_capacity=4096, cell_capacity=1 << 16,
                    chain_capacity=256, replica_capacity=2)
         p.set_schedule(32, K)
@@ -243,9 +263,12 @@ def test_rebuild_period_and_replay():
 
 
 def test_alone_vs_mixed_pool_bit_identical():
-    """A replica gives the same bits alone and in a pool with others."""
+    """A replica gives the same bits alone and in a pool with others, SRK2
+    and SRK4 replicas mixed (the SRK2 ones skip the inner SRK4 stages)."""
     ws = [W.chains(f"m{i}", 2, 30 + 20 * i, 5 + 3 * i, phi=0.25, vtan=0.1 * i, T=0.01,
                    steps=80, seed=20 + i) for i in range(4)]
+    ws[1].phys.srk_order = 4
+    ws[3].phys.srk_order = 4
 
     def run(which):
         p = E.Pool(2, E.GCR_MIXED, bead_capacity=1 << 14, cell_capacity=1 << 16,

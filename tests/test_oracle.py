@@ -397,6 +397,120 @@ def test_srk2_pure_diffusion_and_time_clip():
     assert np.allclose(pos, x, atol=1e-14)
 
 
+def srk4_dimer(k, a, r, s0, dt0, dtmax, mdps, nsteps):
+    """The SRK4 map of the spring dimer written out as a scalar recurrence
+    (SRK.py:186-215, 297-321, 355-358): forces k_i = k (a - s_i) on the first
+    bead, virtual separations from s with dt_prev / 2, dt_prev / 2, dt_prev,
+    total k1/6 + k2/3 + k3/3 + k4/6, m = |total| / r (force prefactor 1),
+    dt = min(dtmax, mdps / m), s' = s + dt 2 total / r. Returns the final
+    separation, the last virtual separation and the dt sequence."""
+    s, dt, dts, s3 = s0, dt0, [], s0
+    for _ in range(nsteps):
+        k1 = k * (a - s)
+        s1 = s + 0.5 * dt * 2 * k1 / r
+        k2 = k * (a - s1)
+        s2 = s + 0.5 * dt * 2 * k2 / r
+        k3 = k * (a - s2)
+        s3 = s + dt * 2 * k3 / r
+        k4 = k * (a - s3)
+        tot = k1 / 6 + k2 / 3 + k3 / 3 + k4 / 6
+        dt = min(dtmax, mdps / (abs(tot) / r))
+        dts.append(dt)
+        s = s + dt * 2 * tot / r
+    return s, s3, dts
+
+
+def dimer_sys(k, r, order):
+    return dict(dim=2, n=2, n_nearby=4, has_springs=True, has_wca=False, has_propel=False,
+                L=20.0, D=0.0, Dr=0.0, radius=r, wca=np.zeros(4),
+                spring=np.array([k, 0.0, 0.0, 0.0, 1.0, 0.1]), v0=0.0, order=order)
+
+
+def test_srk4_spring_dimer_recurrence():
+    """SRK_order 4 on the spring dimer equals the scalar SRK4 recurrence with
+    the reference's sequencing, the adaptive dt included."""
+    k, a, r = 50.0, 0.5, 1.3
+    # from 0.7 the step moves from the max-distance limit (where every
+    # integrator moves the same 2 mdps) to dt = dtmax
+    s0, dtmax, mdps, dt0, nsteps = 0.7, 2e-3, 4e-3, 1e-6, 30
+    pos = np.array([[10.0 + s0, 7.0], [10.0, 7.0]])
+    tv = np.array([dt0, dtmax, mdps, 1.0, -1.0, 40, 40, nsteps])
+    out = O.srk2_run(dimer_sys(k, r, 4), pos, np.zeros((2, 2)), np.array([0, 0]),
+                     np.array([a]), np.array([0.0]), tv, xi=np.zeros((nsteps, 2, 4)))
+    s, s3, dts = srk4_dimer(k, a, r, s0, dt0, dtmax, mdps, nsteps)
+    assert out[0][0, 0] - out[0][1, 0] == pytest.approx(s, rel=1e-12)
+    assert out[1][0, 0] - out[1][1, 0] == pytest.approx(s3, rel=1e-12)  # pos1: last virtual
+    assert out[4][0] == pytest.approx(dts[-1], rel=1e-12)
+    assert out[4][1] == pytest.approx(sum(dts), rel=1e-12)
+    assert len(set(np.round(dts, 12))) > 2
+    # and it is not the SRK2 trajectory
+    out2 = O.srk2_run(dimer_sys(k, r, 2), pos, np.zeros((2, 2)), np.array([0, 0]),
+                      np.array([a]), np.array([0.0]), tv, xi=np.zeros((nsteps, 2, 4)))
+    assert abs((out2[0][0, 0] - out2[0][1, 0]) - s) > 1e-6
+
+
+def test_srk4_constant_dt_is_the_rk4_polynomial():
+    """With dt pinned at dtmax (a weak spring) every step multiplies the
+    extension by the fourth-order Taylor polynomial of exp(-2 k dt / r)."""
+    k, a, r = 2.0, 0.5, 1.0
+    s0, dtmax, mdps, nsteps = 0.51, 1e-2, 1.0, 40
+    pos = np.array([[10.0 + s0, 7.0], [10.0, 7.0]])
+    tv = np.array([dtmax, dtmax, mdps, 1.0, -1.0, 40, 40, nsteps])
+    out = O.srk2_run(dimer_sys(k, r, 4), pos, np.zeros((2, 2)), np.array([0, 0]),
+                     np.array([a]), np.array([0.0]), tv, xi=np.zeros((nsteps, 2, 4)))
+    assert out[4][0] == dtmax
+    z = -2 * k * dtmax / r
+    p4 = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert out[0][0, 0] - out[0][1, 0] - a == pytest.approx((s0 - a) * p4 ** nsteps, rel=1e-10)
+
+
+def test_srk4_pure_diffusion_noise_amplitudes():
+    """F = 0: the real update adds sqrt(2 D dt_used) xi and each virtual one
+    sqrt(2 D dt_prev) xi from pos, the same xi in every stage (SRK.py:361)."""
+    D, L = 0.3, 10.0
+    sysd = abp_sys(3, L, D=D, v0=0.0)
+    sysd["order"] = 4
+    rng = np.random.default_rng(2)
+    xi = rng.normal(size=(6, 3, 4))
+    x0 = rng.uniform(0, L, (3, 2))
+    dtmax, mdps, dt0 = 1e-2, 5e-3, 1e-6
+    tv = np.array([dt0, dtmax, mdps, 1.0, -1.0, 40, 40, 6])
+    pos, pos1, T, F, st = O.srk2_run(sysd, x0, np.tile([1.0, 0.0], (3, 1)), np.arange(3),
+                                     np.ones(3), np.zeros(3), tv, xi=xi)
+    dt = min(dtmax, mdps / np.sqrt(2 * D))
+    x = x0.copy()
+    for kk in range(5):
+        x = np.mod(x + np.sqrt(2 * dt * D) * xi[kk, :, :2], L)
+    assert np.allclose(pos1, np.mod(x + np.sqrt(2 * dt * D) * xi[5, :, :2], L), atol=1e-14)
+    assert np.allclose(pos, pos1, atol=1e-14)  # dt_prev == dt_used after the first step
+    # the first step's virtual stages use dt0
+    p1, q1, *_ = O.srk2_run(sysd, x0, np.tile([1.0, 0.0], (3, 1)), np.arange(3), np.ones(3),
+                            np.zeros(3), np.array([dt0, dtmax, mdps, 1.0, -1.0, 40, 40, 1]),
+                            xi=xi[:1])
+    assert np.allclose(q1, np.mod(x0 + np.sqrt(2 * dt0 * D) * xi[0, :, :2], L), atol=1e-14)
+    assert np.allclose(p1, np.mod(x0 + np.sqrt(2 * dt * D) * xi[0, :, :2], L), atol=1e-14)
+
+
+def test_srk4_rod_virtual_stages_build_on_each_other():
+    """With the rigid-rod projection every SRK4 stage projects from the
+    position its forces were evaluated at (SRK.py:364-371), so a rigid rod
+    under propulsion moves dt/2 + dt/2 + dt in the virtual stages and then
+    dt_used from there: (2 dt_prev + dt_used) v0 / r per step."""
+    n, a, v0, dtmax = 6, 0.5, 1.0, 1e-3
+    t = np.array([1.0, 0.0])
+    pos = straight_rod(n, a, 2, np.array([10.0, 10.0]), t)
+    sysd = dict(dim=2, n=n, n_nearby=4, has_springs=True, has_wca=False, has_propel=True,
+                L=L2D, D=0.0, Dr=0.0, radius=1.0, wca=np.array([12.0, 12.0, 0.0, 1.0]),
+                spring=np.array([100.0, 0.0, 0.0, 0.0, 1.0, 0.1]), v0=v0, rod=1.0, order=4)
+    tv = np.array([dtmax, dtmax, 1.0, 1.0, -1.0, 1e9, 40.0, 5.0])
+    cid = np.zeros(n, dtype=np.int64)
+    out, *_ = O.srk2_run(sysd, pos, np.tile(t, (n, 1)), cid, np.array([a]), np.zeros(1), tv)
+    assert np.allclose(out - pos, np.tile(5 * 3 * dtmax * v0 * t, (n, 1)), atol=1e-12)
+    sysd["rod"] = 0.0
+    out0, *_ = O.srk2_run(sysd, pos, np.tile(t, (n, 1)), cid, np.array([a]), np.zeros(1), tv)
+    assert np.allclose(out0 - pos, np.tile(5 * dtmax * v0 * t, (n, 1)), atol=1e-12)
+
+
 def test_rotational_diffusion_closed_form():
     """SRK.py:398-423: t' = normalize(t + pi sqrt(2 Dr dt) (xi_r - (xi_r.t) t))."""
     Dr = 0.2
