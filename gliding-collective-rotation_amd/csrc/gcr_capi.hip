@@ -132,6 +132,7 @@ struct gcr_pool {
     int tile_threads = 0;  // threads per tile workgroup: 256, 512, 1024, 0 = from the tile sizes
     int nt_launch = BS;    // resolved by rebuild_tables
     bool has_srk4 = false; // a live replica integrates with SRK4: the step has 4 force stages
+    bool tile_gen = false; // a live replica needs the general tile kernels (SRK4, walls, FENE)
 };
 
 namespace {
@@ -237,6 +238,10 @@ void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
     if (d->rod != 0.0 && longest_chain > 2) fl |= GCR_F_ROD;
     if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
     if (d->srk_order == 4) fl |= GCR_F_SRK4;
+    P.walls = d->walls & 7;  // the kernels read the bits of their dimensions
+    P.wall_zone = d->wall_zone;
+    P.wall_rc = d->wall_cutoff > 0.0 ? d->wall_cutoff : d->wca[3];
+    if (P.walls) fl |= GCR_F_WALLS;
     P.flags = fl;
 }
 
@@ -249,7 +254,7 @@ bool srk_order_ok(const gcr_replica_desc* d)
 int rebuild_tables(gcr_pool* p)
 {
     std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0), tl, hr, hc;
-    bool has_srk4 = false;
+    bool has_srk4 = false, tile_gen = false;
     for (int r = 0; r < p->capR; ++r) {
         rco[r] = (int32_t)cr.size();
         RepParams& P = p->P[r];
@@ -265,6 +270,7 @@ int rebuild_tables(gcr_pool* p)
             cc.push_back(P.cell_off + c);
         }
         if (P.flags & GCR_F_SRK4) has_srk4 = true;
+        if (P.flags & (GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE)) tile_gen = true;
         if (P.flags & GCR_F_ROD)
             for (int c = 0; c < P.n_chains; c += BS) {
                 hr.push_back(r);
@@ -312,6 +318,7 @@ int rebuild_tables(gcr_pool* p)
         return fail(GCR_ERR_CAPACITY, "block table capacity exceeded");
     p->nchb = (int)hr.size();
     p->has_srk4 = has_srk4;
+    p->tile_gen = tile_gen;
     if (p->nchb) {
         HIPCHK(hipMemcpyAsync(p->chb_rep, hr.data(), hr.size() * 4, hipMemcpyHostToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(p->chb_chain0, hc.data(), hc.size() * 4, hipMemcpyHostToDevice, p->stream));
@@ -404,16 +411,26 @@ void launch_k(K kernel, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1
         kernel<<<grid, block, 0, st>>>(a, j);
 }
 
-// 2D tile kernels at the pool's threads per tile.
+// 2D tile kernels at the pool's threads per tile, in their lean variant
+// unless a replica needs the general one.
+template <typename FT, int STAGE, bool GEN>
+void launch_tile_v(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (p->nt_launch == 1024)
+        launch_k(k_stage_tile2d<FT, STAGE, 1024, GEN>, p->ntile, p->stream, ev0, ev1, a, j, 1024);
+    else if (p->nt_launch == 512)
+        launch_k(k_stage_tile2d<FT, STAGE, 512, GEN>, p->ntile, p->stream, ev0, ev1, a, j, 512);
+    else
+        launch_k(k_stage_tile2d<FT, STAGE, 256, GEN>, p->ntile, p->stream, ev0, ev1, a, j, 256);
+}
+
 template <typename FT, int STAGE>
 void launch_tile(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (p->nt_launch == 1024)
-        launch_k(k_stage_tile2d<FT, STAGE, 1024>, p->ntile, p->stream, ev0, ev1, a, j, 1024);
-    else if (p->nt_launch == 512)
-        launch_k(k_stage_tile2d<FT, STAGE, 512>, p->ntile, p->stream, ev0, ev1, a, j, 512);
+    if (p->tile_gen)
+        launch_tile_v<FT, STAGE, true>(p, a, j, ev0, ev1);
     else
-        launch_k(k_stage_tile2d<FT, STAGE, 256>, p->ntile, p->stream, ev0, ev1, a, j, 256);
+        launch_tile_v<FT, STAGE, false>(p, a, j, ev0, ev1);
 }
 
 // One launch of a step: which = 0 the first force stage, 1 a later force
@@ -992,7 +1009,8 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     // rod, pair and SRK4 flags change the launch tables or the stages of a
     // step; lists are rebuilt at the start of every batch, so a new exclusion
     // rule takes effect there
-    if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA | GCR_F_SRK4)) p->tables_dirty = true;
+    if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA | GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE))
+        p->tables_dirty = true;
     return GCR_OK;
 }
 

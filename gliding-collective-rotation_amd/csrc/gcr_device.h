@@ -21,6 +21,7 @@
 #define GCR_F_FENE 128u     // spring_params[3]: FENE springs (forces.py:331-341)
 #define GCR_F_ROD 256u      // rigid-rod projection of the updates (updates.py:347-499)
 #define GCR_F_SRK4 512u     // SRK_order 4 (SRK.py:186-215, 297-321)
+#define GCR_F_WALLS 1024u   // WCA walls (forces.py:405-435)
 
 struct RepParams {
     int32_t n;          // beads
@@ -48,6 +49,10 @@ struct RepParams {
     double k, kappa, theta0;  // spring_params[0..2]
     double fene_tol;          // spring_params[5]: FENE maximal stretch / rest length
     double rod;               // rigid_rod_projection_factor
+    double wall_zone;         // distance of the walls from the box faces
+    double wall_rc;           // the walls' range parameter (wall_params[3])
+    int32_t walls;            // bit i: walls normal to axis i
+    int32_t pad0;
     double D, Dr, radius;     // pop_consts[0..2]
     double dtmax, mdps, cheat;
     uint64_t seed;

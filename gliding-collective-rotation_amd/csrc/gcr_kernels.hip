@@ -165,9 +165,9 @@ __device__ __forceinline__ bool rebuild_runs(const PoolArgs& a, int r, int j)
 // A virtual position outside the skin: outside the safe batch the batch is
 // replayed; inside it the replica is re-sorted on those positions before the
 // next force evaluation.
-__device__ __forceinline__ void flag_predictor(const PoolArgs& a, RepState* S)
+__device__ __forceinline__ void flag_predictor(const PoolArgs& a, RepState* S, int sub)
 {
-    atomicOr(a.safe ? &S->pred_far : &S->violation, a.safe ? 1 << a.sub : 1);
+    atomicOr(a.safe ? &S->pred_far : &S->violation, a.safe ? 1 << sub : 1);
 }
 
 // SRK order handling (SRK.py:143-215, 297-321, 355-360).
@@ -338,11 +338,12 @@ __device__ __forceinline__ void make_bond(const double* xu, const double* xv, co
 // bond vector, forces.py:326-348), in FP64: Hookean k (a/|t| - 1), or FENE
 // -k st / |st^2/m^2 - 1| / |t| with st = |t| - a and m = tol * a, with the
 // reference's guard (forces.py:336-339), which compares st^2 with m itself
-// and divides by zero when it fails.
-template <int DIM, typename BT>
+// and divides by zero when it fails. FENE = false compiles Hookean springs
+// only (the lean kernels).
+template <int DIM, typename BT, bool FENE = true>
 __device__ __forceinline__ double spring_factor(const Bond<DIM, BT>& b, double a, const RepParams& P)
 {
-    if (P.flags & GCR_F_FENE) {
+    if (FENE && (P.flags & GCR_F_FENE)) {
         const double m = P.fene_tol * a;
         const double st = sqrt(b.ds64) - a;
         const double den = (st * st != m && m != 0.0) ? fabs(st * st / (m * m) - 1.0) : 0.0;
@@ -424,8 +425,8 @@ __device__ __forceinline__ void load_chain_nb(const double* X, const int32_t* LI
 // All bonded terms acting on the bead at xs, added to acc in the
 // reference's order (forces.py:305-402 walks beads in index order: bond
 // p1->s, angle at p1, bond s->n1, angle at s, angle at n1). Writes the
-// tangent at s when tangents follow the bonds.
-template <int DIM, typename BT>
+// tangent at s when tangents follow the bonds. FENE = false: Hookean only.
+template <int DIM, typename BT, bool FENE = true>
 __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, double a,
                                        const RepParams& P, double* acc, double* tang, bool& tang_set)
 {
@@ -438,7 +439,7 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     if (lk.z >= 0) make_bond<DIM, BT>(xs, c.n1, P, B);
     // bond p1 -> s: forces[b1+1] -= f (forces.py:345-348)
     if (lk.y >= 0 && A.ds != (BT)0) {
-        const BT mag = (BT)spring_factor<DIM, BT>(A, a, P);
+        const BT mag = (BT)spring_factor<DIM, BT, FENE>(A, a, P);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) acc[i] -= (double)(A.t[i] * mag);
     }
@@ -453,7 +454,7 @@ __device__ __forceinline__ void bonded(const ChainNb<DIM>& c, const double* xs, 
     }
     // bond s -> n1: forces[b1] += f
     if (lk.z >= 0 && B.ds != (BT)0) {
-        const BT mag = (BT)spring_factor<DIM, BT>(B, a, P);
+        const BT mag = (BT)spring_factor<DIM, BT, FENE>(B, a, P);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) acc[i] += (double)(B.t[i] * mag);
     }
@@ -886,7 +887,7 @@ __device__ __forceinline__ void propel(const PoolArgs& a, int chain, const doubl
 template <int DIM>
 __device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParams& P, RepState* S,
                                                int s, const double* x, const double* f,
-                                               double resc)
+                                               double resc, int sub)
 {
     double xi[DIM];
     load_vec<DIM>(a.XI, s, xi);
@@ -902,7 +903,41 @@ __device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParam
         x1[i] = pymod(tmp, P.L);
     }
     store_vec<DIM>(a.X1, s, x1);
-    if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) flag_predictor(a, S);
+    if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) flag_predictor(a, S, sub);
+}
+
+// WCA walls normal to the axes in P.walls (forces.py:405-435), `wall_zone`
+// inside the faces, evaluated at x = the bead's position at the start of the
+// step in every stage (SRK.py:270 passes pos): sep from the nearer wall,
+// reduced by WCA_shift (forces.py:425-429, the opposite sign convention to
+// the pair law's), and the force -sep coeff(sep^2) while 0 < sep^2 <= the
+// walls' cutoff -- the squared distance against the cutoff itself, as
+// forces.py:432 has it.
+// The reference visits only beads whose last grid cell lies within
+// 2 grid_window cells of a face; all beads in the walls' range do while beads
+// move less than grid_window cells between grid updates.
+template <int DIM>
+__device__ __forceinline__ void add_walls(const RepParams& P, const double* x, double* acc)
+{
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        if (!((P.walls >> i) & 1)) continue;
+        double sep = x[i];
+        if (sep >= P.halfL)
+            sep -= (P.L - P.wall_zone);
+        else
+            sep -= P.wall_zone;
+        double ds;
+        if (P.wca[2] != 0.0) {
+            double m = fabs(sep) - P.wca[2];
+            m = m > 0.0 ? m : 0.0;
+            sep = sep > 0.0 ? m : (sep < 0.0 ? -m : 0.0);
+            ds = m * m;
+        } else {
+            ds = sep * sep;
+        }
+        if (ds > 0.0 && ds <= P.wall_rc) acc[i] += -sep * PairLaw<double>::coeff(ds, P);
+    }
 }
 
 // What a force stage does with the total force on slot s:
@@ -913,12 +948,13 @@ __device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParam
 //          virtual update runs in k_virtual once every bead's forces are
 //          known); SRK4 stage 3: FT += F/6 and return |FT|^2;
 // STAGE 2: (force evaluation only) store the force.
-template <int DIM, int STAGE>
+// GEN = false compiles the SRK2 path only (the kernels' lean variant).
+template <int DIM, int STAGE, bool GEN = true>
 __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams& P, RepState* S,
                                               int s, int orig_global, const double* xs,
                                               const double* acc)
 {
-    const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
+    const bool srk4 = GEN && (P.flags & GCR_F_SRK4) != 0;
     if (STAGE == 0) {
         store_vec<DIM>(a.F0, s, acc);
         if (srk4) {
@@ -927,7 +963,7 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
             for (int i = 0; i < DIM; ++i) ft[i] = acc[i] * (1. / 6.);
             store_vec<DIM>(a.FT, s, ft);
         }
-        virtual_update<DIM>(a, P, S, s, xs, acc, virtual_rescale(P, 0));
+        virtual_update<DIM>(a, P, S, s, xs, acc, GEN ? virtual_rescale(P, 0) : 1.0, 0);
         return 0.0;
     } else if (!srk4) {
         store_vec<DIM>(a.FT, s, acc);
@@ -966,7 +1002,7 @@ __global__ void __launch_bounds__(BS) k_virtual(PoolArgs a, int j)
     double x[DIM], f[DIM];
     load_vec<DIM>(a.X, s, x);
     load_vec<DIM>(a.F0, s, f);
-    virtual_update<DIM>(a, P, S, s, x, f, virtual_rescale(P, a.sub));
+    virtual_update<DIM>(a, P, S, s, x, f, virtual_rescale(P, a.sub), a.sub);
 }
 
 // Stage kernels over bead blocks (any dimension; the 2D production path is
@@ -1005,6 +1041,11 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
         if (P.flags & GCR_F_PROPEL) {
             if (!tang_set) load_vec<DIM>(a.T, s, tang);
             propel<DIM>(a, me.y, tang, acc);
+        }
+        if (P.flags & GCR_F_WALLS) {
+            double x0[DIM];
+            load_vec<DIM>(a.X, s, x0);
+            add_walls<DIM>(P, x0, acc);
         }
         fsq = stage_store<DIM, STAGE>(a, P, S, s, me.x, xs, acc);
     }
@@ -1126,6 +1167,11 @@ __global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
     if (P.flags & GCR_F_WCA) add_pairs<DIM, FT>(a, P, Xa, s, xs, me, acc);
     if (!tang_set) load_vec<DIM>(a.T, s, tang);
     if (P.flags & GCR_F_PROPEL) propel<DIM>(a, me.y, tang, acc);
+    if (P.flags & GCR_F_WALLS) {
+        double x0[DIM];
+        load_vec<DIM>(a.X, s, x0);
+        add_walls<DIM>(P, x0, acc);
+    }
     store_vec<DIM>(a.FT, s, acc);
     store_vec<DIM>(a.Tn, s, tang);
 }
@@ -1618,7 +1664,7 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
     }
     if (viol) {
         if (STAGE == 0)
-            flag_predictor(a, a.S + r);
+            flag_predictor(a, a.S + r, a.sub);
         else if (!a.safe)
             atomicOr(&a.S[r].violation, 1);
     }

@@ -336,8 +336,11 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
 // STAGE 0 / 1 as k_stage (STAGE 1 for SRK stage a.sub); STAGE 2: force
 // evaluation of replica `j` only at the positions X (forces into FT,
 // tangents into Tn; no noise, no update). NT threads per tile (256, 512 or
-// 1024): large tiles keep more waves in flight.
-template <typename FT, int STAGE, int NT>
+// 1024): large tiles keep more waves in flight. GEN = false is the lean
+// variant for pools without SRK4, walls and FENE springs: their code stays
+// out of the kernel, and with it the registers it would hold (the stage-0
+// kernel then stays within 96 VGPRs, five waves per SIMD).
+template <typename FT, int STAGE, int NT, bool GEN>
 __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
@@ -390,7 +393,7 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
                 pa1 = (double)pg[1];
             }
         }
-        if (STAGE == 1 && !(P.flags & GCR_F_SRK4)) {
+        if (STAGE == 1 && !(GEN && (P.flags & GCR_F_SRK4))) {
             load_vec<2>(a.F0, s, acc);  // SRK2 accumulates into F0
         } else {
             acc[0] = 0.0;
@@ -398,12 +401,13 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
         }
         // bonded terms first, then the pair sum, then propulsion (SRK.py:228-253)
         bool tang_set = false;
-        if (springs) bonded<2, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+        if (springs) bonded<2, FT, GEN>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
         acc[0] += pa0;
         acc[1] += pa1;
         if (STAGE == 2) {
             if (!tang_set) load_vec<2>(a.T, s, tang);
             if (P.flags & GCR_F_PROPEL) propel<2>(a, me.y, tang, acc);
+            if (GEN && (P.flags & GCR_F_WALLS)) add_walls<2>(P, xs, acc);
             store_vec<2>(a.FT, s, acc);
             store_vec<2>(a.Tn, s, tang);
         } else {
@@ -411,7 +415,12 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
                 if (!tang_set) load_vec<2>(a.T, s, tang);
                 propel<2>(a, me.y, tang, acc);
             }
-            const double q = stage_store<2, STAGE>(a, P, S, s, me.x, xs, acc);
+            if (GEN && (P.flags & GCR_F_WALLS)) {  // at the step's start position in every stage
+                double x0[2];
+                load_vec<2>(a.X, s, x0);
+                add_walls<2>(P, x0, acc);
+            }
+            const double q = stage_store<2, STAGE, GEN>(a, P, S, s, me.x, xs, acc);
             fsq = q > fsq ? q : fsq;
         }
     }

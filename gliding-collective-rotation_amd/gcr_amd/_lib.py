@@ -40,7 +40,9 @@ class ReplicaDesc(C.Structure):
         ("seed", C.c_uint64),
         ("rod", C.c_double),
         ("srk_order", C.c_int32),
-        ("reserved", C.c_int32),
+        ("walls", C.c_int32),
+        ("wall_zone", C.c_double),
+        ("wall_cutoff", C.c_double),
     ]
 
 

@@ -12,14 +12,15 @@ the step loop running on the GPU through libgcr.so:
 
 Device state stays resident between calls; positions are uploaded again only
 when the host changed them. Scope: one moving population of chains (ABPs are
-chains of one bead) with the "springs", "WCA" and "self-propulsion" forces,
-SRK_order 2 or 4. Anything else raises NotImplementedError naming the feature.
+chains of one bead) with the "springs", "WCA", "self-propulsion" and "walls"
+forces, SRK_order 2 or 4. Anything else raises NotImplementedError naming the
+feature.
 """
 import numpy as np
 
 from . import engine as E
 
-SUPPORTED = ("springs", "WCA", "self-propulsion")
+SUPPORTED = ("springs", "WCA", "self-propulsion", "walls")
 
 
 def _force_list(pop, repulsive_only):
@@ -69,8 +70,6 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         order = int(getattr(config, "SRK_order", 2))
         if order not in (2, 4):
             raise NotImplementedError(f"GPU path: SRK_order 2 or 4 (not {order})")
-        if np.any(getattr(config, "walls", False)):
-            raise NotImplementedError("GPU path: walls are not ported")
         if not hasattr(pop, "chain_id"):
             raise NotImplementedError("GPU path: populations with chain_id (chains / ABPs) only")
         repulsive_only = bool(getattr(pop, "b_init_growing", False))
@@ -97,6 +96,18 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         n_nearby = int(getattr(pop, "n_nearby_beads_dont_repel", 0))
         rest = np.asarray(pop.rest_lengths, dtype=np.float64)
         rod = float(getattr(pop, "rigid_rod_projection_factor", 0.0))
+        walls, wall_zone, wall_cutoff = 0, 0.0, 0.0
+        if "walls" in forces:
+            # wall_params = WCA params + config.walls (populations.py:222-223); the
+            # walls sit grid_spacing * grid_window / 2 inside the faces, with the
+            # grid window of SRK.py:99-103 (forces.py:417)
+            wp = forces["walls"]
+            walls = sum(1 << i for i in range(int(config.dim)) if wp[4 + i] != 0)
+            spacing = float(config.Lx) / int(config.n_grid)
+            nbor = float(forces["WCA"][3]) if "WCA" in forces else 0.0
+            window = max(1, int(np.ceil(float(config.grid_factor) * nbor / spacing)))
+            wall_zone = spacing * window / 2
+            wall_cutoff = float(wp[3])
         # cells of at least L/512 (2D) or L/64 (3D) per side, so that the tiny
         # cutoffs of the growth phase (updates.py:304-310) do not ask for a
         # grid of millions of cells
@@ -106,9 +117,11 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
                          Dr=float(max(pop.Dr, 0.0)), radius=float(pop.repel_d_max), wca=wca,
                          spring=spring, v0=v0, n_nearby=n_nearby, has_springs=has_springs,
                          has_wca=has_wca, has_propel=has_propel, skin=skin, rod=rod,
-                         srk_order=order)
+                         srk_order=order, walls=walls, wall_zone=wall_zone,
+                         wall_cutoff=wall_cutoff)
         sig = (n, dim, L, phys.D, phys.Dr, phys.radius, n_nearby, tuple(wca), tuple(spring), v0,
-               has_springs, has_wca, has_propel, rod, order, rest.tobytes(),
+               has_springs, has_wca, has_propel, rod, order, walls, wall_zone, wall_cutoff,
+               rest.tobytes(),
                np.asarray(vtan_chain, dtype=np.float64).tobytes(),
                np.asarray(pop.chain_id).tobytes())
         R = resident.setdefault(id(pop), _Resident())

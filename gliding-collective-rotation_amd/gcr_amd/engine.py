@@ -36,6 +36,9 @@ class Physics:
     seed: int = 0
     rod: float = 0.0  # rigid_rod_projection_factor
     srk_order: int = 2  # config.SRK_order (2 or 4)
+    walls: int = 0  # config.walls as bits: x 1, y 2, z 4
+    wall_zone: float = 0.0  # forbidden_zone_width (forces.py:417)
+    wall_cutoff: float = 0.0  # wall_params[3]; <= 0: wca[3]
 
     def desc(self):
         d = ReplicaDesc()
@@ -58,6 +61,9 @@ class Physics:
         d.seed = int(self.seed) & ((1 << 64) - 1)
         d.rod = float(self.rod)
         d.srk_order = int(self.srk_order)
+        d.walls = int(self.walls)
+        d.wall_zone = float(self.wall_zone)
+        d.wall_cutoff = float(self.wall_cutoff)
         return d
 
 

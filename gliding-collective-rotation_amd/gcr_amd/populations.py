@@ -10,7 +10,7 @@ Constructors take the reference's `sim` dictionary (defaults.set_defaults).
 Differences from the reference, by design: random_initialization places
 straight chains without overlap (gcr_init_chains) instead of the dot-like
 chains the growth phase expands (populations.py:530-557, control.py:153-219);
-motors, shells, growth/mitosis and walls are not part of this path. An
+motors, shells and growth/mitosis are not part of this path. An
 optional per-chain chirality array `v_tan_angle_per_chain` extends the
 scalar v_tan_angle (a per-population constant reduces to the reference).
 """
@@ -76,7 +76,11 @@ class WCA_Population(Population):
                                           self.LJ_cutoff_factor)
 
     def self_interactions(self):
-        return super().self_interactions() + [["WCA", self.WCA_params]]
+        """populations.py:219-224: WCA, and walls when config.walls has any axis."""
+        flist = super().self_interactions() + [["WCA", self.WCA_params]]
+        if np.any(config.walls):
+            flist.append(["walls", np.append(self.WCA_params, config.walls)])
+        return flist
 
 
 class Oriented_WCA_Population(WCA_Population):

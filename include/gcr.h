@@ -56,7 +56,13 @@ typedef struct gcr_replica_desc {
     double rod;          /* rigid_rod_projection_factor (pop_consts[3]); used when the
                             longest chain has more than two beads (SRK.py:364) */
     int32_t srk_order;   /* config.SRK_order: 4, or 2 (also for 0) */
-    int32_t reserved;
+    int32_t walls;       /* config.walls as bits (x 1, y 2, z 4): WCA walls replace the
+                            periodic faces (forces.py:405-435) */
+    double wall_zone;    /* forbidden_zone_width of forces.py:417, grid_spacing *
+                            grid_window / 2: the walls sit this far inside the faces */
+    double wall_cutoff;  /* wall_params[3], the walls' range parameter; <= 0: wca[3] (they
+                            differ while growing, when only the pair law is made purely
+                            repulsive, updates.py:304-310) */
 } gcr_replica_desc;
 
 /* ---------------------------------------------------------------- errors */

@@ -342,9 +342,36 @@ static void verlet_forces(const verlet* v, int dim, const double* pos, double L,
     }
 }
 
+void orc_walls(int dim, int n, const double* pos, double L, const double wca[4], int walls,
+               double zone, double* F)
+{
+    double half = L / 2.0;
+    for (int b = 0; b < n; ++b)
+        for (int i = 0; i < dim; ++i) {
+            if (!((walls >> i) & 1)) continue;
+            double sep = pos[b * dim + i];
+            if (sep >= half)
+                sep -= (L - zone);
+            else
+                sep -= zone;
+            double ds;
+            if (wca[2] != 0) {
+                /* forces.py:425-429: |sep| - shift, clipped at 0, sign kept */
+                double m = fabs(sep) - wca[2];
+                if (m < 0) m = 0;
+                sep = (sep > 0) ? m : (sep < 0 ? -m : 0.0);
+                ds = m * m;
+            } else {
+                ds = sep * sep;
+            }
+            /* forces.py:432: the squared distance against wca[3] itself */
+            if (0 < ds && ds <= wca[3]) F[b * dim + i] += -sep * orc_wca_coeff(ds, wca);
+        }
+}
+
 void orc_force_eval(const orc_sys* s, const double* pos, const int64_t* chain_id,
                     const double* rest_lengths, const double* vtan, double* tangent,
-                    double* F, int pair_mode, void* vv)
+                    double* F, int pair_mode, void* vv, const double* wall_pos)
 {
     int dim = s->dim, n = s->n;
     if (s->has_springs) {
@@ -369,6 +396,11 @@ void orc_force_eval(const orc_sys* s, const double* pos, const int64_t* chain_id
             orc_pair_forces_brute(dim, n, pos, chain_id, s->L, s->wca, s->n_nearby, F);
     }
     if (s->has_propel) orc_self_propel(dim, n, chain_id, s->v0, vtan, tangent, F);
+    if (s->walls) {
+        double ww[4] = {s->wca[0], s->wca[1], s->wca[2],
+                        s->wall_cutoff > 0 ? s->wall_cutoff : s->wca[3]};
+        orc_walls(dim, n, wall_pos ? wall_pos : pos, s->L, ww, s->walls, s->wall_zone, F);
+    }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -620,7 +652,7 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
             const double* at = stage == 0 ? pos : pos1;
             double* out = stage == 3 ? pos : pos1;
             memset(Faux, 0, sizeof(double) * (size_t)n * dim);
-            orc_force_eval(s, at, chain_id, rest_lengths, vtan, tangent, Faux, pair_mode, vp);
+            orc_force_eval(s, at, chain_id, rest_lengths, vtan, tangent, Faux, pair_mode, vp, pos);
             const double mult = (stage == 0 || stage == 3) ? 1. / 6. : 1. / 3.;
             for (int k = 0; k < n * dim; ++k)
                 F[k] = stage == 0 ? Faux[k] * mult : F[k] + Faux[k] * mult;
@@ -670,7 +702,7 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
             double pref = stage == 0 ? 1.0 : 0.5;
             const double* at = stage == 0 ? pos : pos1;
             double* out = stage == 0 ? pos1 : pos;
-            orc_force_eval(s, at, chain_id, rest_lengths, vtan, tangent, F, pair_mode, vp);
+            orc_force_eval(s, at, chain_id, rest_lengths, vtan, tangent, F, pair_mode, vp, pos);
             if (stage == 1) {
                 /* SRK.py:324-353 */
                 double mx = 0.0;

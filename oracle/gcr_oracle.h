@@ -34,6 +34,9 @@ typedef struct orc_sys {
     double  v0;               /* active_params[0] (populations.py:355-358) */
     double  rod;              /* rigid_rod_projection_factor (pop_consts[3]) */
     int32_t order;            /* SRK_order: 4, else 2 (config.SRK_order) */
+    int32_t walls;            /* config.walls as bits: x 1, y 2, z 4 */
+    double  wall_zone;        /* forbidden_zone_width (forces.py:417) */
+    double  wall_cutoff;      /* wall_params[3]; <= 0: wca[3] */
 } orc_sys;
 
 /* Python float modulo (CPython float_rem semantics, used by `tmp %= Lx`
@@ -71,11 +74,21 @@ void orc_pair_forces_brute(int dim, int n, const double* pos, const int64_t* cha
 void orc_self_propel(int dim, int n, const int64_t* chain_id, double v0,
                      const double* v_tan_angle_per_chain, const double* tangent, double* forces);
 
-/* One force evaluation in the reference order (SRK.py:228-253):
- * springs (+tangents) -> pair -> self-propulsion, accumulated into forces. */
+/* forces.py:405-435: WCA walls normal to the axes in `walls`, at `zone`
+ * inside the faces, from the positions pos; accumulates into forces. Every
+ * bead is visited (the reference skips beads whose last grid cell is more
+ * than 2 grid_window cells from a face, none of which is within range). */
+void orc_walls(int dim, int n, const double* pos, double L, const double wca[4], int walls,
+               double zone, double* forces);
+
+/* One force evaluation in the reference order (SRK.py:228-275):
+ * springs (+tangents) -> pair -> self-propulsion -> walls, accumulated into
+ * forces; the walls take wall_pos (SRK.py:270 passes pos in every stage),
+ * NULL meaning pos. */
 void orc_force_eval(const orc_sys* s, const double* pos, const int64_t* chain_id,
                     const double* rest_lengths, const double* v_tan_angle_per_chain,
-                    double* tangent, double* forces, int pair_mode, void* verlet);
+                    double* tangent, double* forces, int pair_mode, void* verlet,
+                    const double* wall_pos);
 
 /* Noise: Philox4x32-10 (Salmon et al., SC'11) and the build's bead-keyed
  * normal generator shared with the GPU path. */

@@ -30,6 +30,9 @@ class OrcSys(C.Structure):
         ("v0", C.c_double),
         ("rod", C.c_double),
         ("order", C.c_int32),
+        ("walls", C.c_int32),
+        ("wall_zone", C.c_double),
+        ("wall_cutoff", C.c_double),
     ]
 
 
@@ -63,8 +66,9 @@ def lib():
         L.orc_srk2_run.argtypes = [C.POINTER(OrcSys), dp, dp, dp, dp, ip, dp, dp, dp, dp,
                                    C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_int,
                                    C.c_double]
+        L.orc_walls.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, C.c_int, C.c_double, dp]
         L.orc_force_eval.argtypes = [C.POINTER(OrcSys), dp, ip, dp, dp, dp, dp, C.c_int,
-                                     C.c_void_p]
+                                     C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -88,6 +92,9 @@ def make_sys(sysd):
     s.v0 = sysd["v0"]
     s.rod = sysd.get("rod", 0.0)
     s.order = sysd.get("order", 2)
+    s.walls = sysd.get("walls", 0)
+    s.wall_zone = sysd.get("wall_zone", 0.0)
+    s.wall_cutoff = sysd.get("wall_cutoff", 0.0)
     return s
 
 
@@ -148,6 +155,16 @@ def bead_noise(dim, seed, orig, step):
     return xi
 
 
+def walls(pos, L, wca, mask, zone):
+    """orc_walls: the wall forces on beads at pos."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, dim = pos.shape
+    F = np.zeros_like(pos)
+    lib().orc_walls(dim, n, pos, L, np.ascontiguousarray(wca, dtype=np.float64), int(mask),
+                    float(zone), F)
+    return F
+
+
 def force_eval(sysd, pos, chain_id, rest_lengths, vtan, tangent):
     s = make_sys(sysd)
     pos = np.ascontiguousarray(pos, dtype=np.float64)
@@ -155,7 +172,7 @@ def force_eval(sysd, pos, chain_id, rest_lengths, vtan, tangent):
     F = np.zeros_like(pos)
     lib().orc_force_eval(C.byref(s), pos, np.ascontiguousarray(chain_id, dtype=np.int64),
                          np.ascontiguousarray(rest_lengths, dtype=np.float64),
-                         np.ascontiguousarray(vtan, dtype=np.float64), T, F, 0, None)
+                         np.ascontiguousarray(vtan, dtype=np.float64), T, F, 0, None, None)
     return F, T
 
 

@@ -101,6 +101,47 @@ def test_dropin_srk4_matches_oracle():
     assert np.max(np.abs(N.min_image(pop.pos - pos_o, 20.0))) < 1e-10 * 20.0
 
 
+def test_dropin_wall_geometry():
+    """With sim["walls"] = "xy" the population lists the reference's walls
+    force (populations.py:222-223) and the walls sit grid_spacing *
+    grid_window / 2 inside the faces (forces.py:417, SRK.py:99-103)."""
+    sim, pop, status = make_run(walls="xy")
+    try:
+        names = [f[0] for f in pop.self_interactions()]
+        assert "walls" in names
+        wp = dict(pop.self_interactions())["walls"]
+        assert wp.tolist() == [12.0, 12.0, 0.0, 1.0, 1.0, 1.0, 0.0]
+        assert config.n_grid == 10  # int(20 / (2 * max(2 a, rc)))
+    finally:
+        config.configure(dict(sim, walls="none"))
+
+
+@pytest.mark.gpu
+def test_dropin_walls_match_oracle():
+    """Walls in x and y through the drop-in equal the oracle with the same
+    wall placement; beads start inside the walls."""
+    sim, pop, status = make_run(walls="xy", v_tan_angle=0.1)
+    try:
+        zone = 20.0 / config.n_grid * max(1, int(np.ceil(2.0 * 1.0 / (20.0 / config.n_grid)))) / 2
+        pop.pos[:] = zone + 1.0 + pop.pos * (20.0 - 2 * zone - 2.0) / 20.0
+        x0, t0 = pop.pos.copy(), pop.tangent_list.copy()
+        f = dropin.make_SRK2_force_calc(config)
+        status.next_output_step = 60
+        f([pop], status, sim)
+        seed = dropin._mix(sim["seed"], 1)
+        sysd = dict(dim=2, n=pop.total_pop, n_nearby=4, has_springs=True, has_wca=True,
+                    has_propel=True, L=20.0, D=pop.D, Dr=pop.Dr, radius=1.0,
+                    wca=pop.WCA_params, spring=pop.spring_params(), v0=1.0, walls=3,
+                    wall_zone=zone)
+        tv = np.array([sim["dt0"], sim["dtmax"], config.max_dist_per_step, 1.0, -1, 40, 40, 60])
+        pos_o, *_ = O.srk2_run(sysd, x0, t0, pop.chain_id, pop.rest_lengths, np.full(6, 0.1),
+                               tv, xi=None, seed=seed)
+        assert np.max(np.abs(N.min_image(pop.pos - pos_o, 20.0))) < 1e-10 * 20.0
+        assert np.all((pop.pos > zone) & (pop.pos < 20.0 - zone))
+    finally:
+        config.configure(dict(sim, walls="none"))
+
+
 @pytest.mark.gpu
 def test_dropin_sees_host_edits():
     """Positions edited on the host between calls are uploaded again."""

@@ -24,7 +24,35 @@ def sysd_of(w):
     return dict(dim=p.dim, n=p.n_beads, n_nearby=p.n_nearby, has_springs=p.has_springs,
                 has_wca=p.has_wca, has_propel=p.has_propel, L=p.L, D=p.D, Dr=p.Dr,
                 radius=p.radius, wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0,
-                rod=p.rod, order=p.srk_order)
+                rod=p.rod, order=p.srk_order, walls=p.walls, wall_zone=p.wall_zone)
+
+
+def walled(w, walls, zone, margin=0.8):
+    """The workload moved inside WCA walls (forces.py:405-435): the chains that
+    do not cross the periodic faces are kept, unchanged, and the box grows by
+    zone + margin on each side, so the beads nearest the faces start `margin`
+    from a wall, inside its range."""
+    import copy
+    L = w.phys.L
+    m = w.n // w.phys.n_chains
+    pos = w.pos.reshape(w.phys.n_chains, m, -1)
+    keep = [c for c in range(w.phys.n_chains)
+            if np.all(np.abs(np.diff(pos[c], axis=0)) < L / 2)]
+    pad = zone + margin
+    out = copy.deepcopy(w)
+    p = out.phys
+    p.L = L + 2 * pad
+    p.n_chains = len(keep)
+    p.n_beads = len(keep) * m
+    p.walls = walls
+    p.wall_zone = zone
+    out.pos = (pos[keep] + pad).reshape(-1, pos.shape[2])
+    out.tangent = w.tangent.reshape(w.phys.n_chains, m, -1)[keep].reshape(-1, pos.shape[2])
+    out.chain_id = np.repeat(np.arange(len(keep), dtype=np.int64), m)
+    out.rest_lengths = w.rest_lengths[keep]
+    out.v_tan_angle = w.v_tan_angle[keep]
+    out.v0_per_chain = w.v0_per_chain[keep]
+    return out
 
 
 def cases():
@@ -86,6 +114,21 @@ def cases():
     w.phys.rod = 0.6
     w.phys.srk_order = 4
     out["srk4_rod2d"] = w
+    # WCA walls (forces.py:405-435): ABPs in a 2D box walled in x and y, chains
+    # walled in x with WCA_shift, 3D chains walled in z, and SRK4 with walls
+    out["walls_abp"] = walled(W.chains("walls_abp", 2, 300, 1, L=20.0, T=0.01, Dr=0.1,
+                                       steps=100, seed=19), 3, 0.8)
+    out["walls_shift"] = walled(W.chains("walls_shift", 2, 300, 1, L=20.0, shift=0.1,
+                                         T=0.01, Dr=0.1, steps=100, seed=20), 1, 0.7)
+    out["walls_chains"] = walled(W.chains("walls_chains", 2, 80, 6, L=20.0, kappa=3.0,
+                                          vtan=0.2, T=0.01, steps=100, seed=23), 3, 0.7,
+                                 margin=0.7)
+    out["walls3d"] = walled(W.chains("walls3d", 3, 60, 5, L=10.0, kappa=4.0, T=0.01,
+                                     steps=100, seed=21), 4, 0.9, margin=0.7)
+    w = walled(W.chains("walls_srk4", 2, 200, 1, L=16.0, T=0.01, Dr=0.1, steps=100,
+                        seed=22), 3, 0.8)
+    w.phys.srk_order = 4
+    out["walls_srk4"] = w
     # a wide skin puts about as many beads within rc + skin as a neighbour
     # list holds: some beads keep a list, the others sum over their cells
     out["list_overflow"] = W.chains("list_overflow", 2, 900, 1, phi=0.55,

@@ -511,6 +511,66 @@ def test_srk4_rod_virtual_stages_build_on_each_other():
     assert np.allclose(out0 - pos, np.tile(5 * dtmax * v0 * t, (n, 1)), atol=1e-12)
 
 
+WALL_WCA = np.array([12.0, 12.0, 0.0, 2 ** (1 / 6)])  # eps 1, bead size 1, WCA cutoff
+
+
+def test_walls_are_the_lj_gradient_of_the_distance_to_the_wall():
+    """forces.py:405-435: inside its range a wall pushes with -dU/dx of the pair
+    energy at the distance from the wall, which sits `zone` inside each face;
+    the far face mirrors the near one."""
+    L, zone = 20.0, 0.75
+    d = np.array([0.9, 0.95, 1.0, 1.02])  # distances from the wall
+    lo = np.column_stack((zone + d, np.full(4, 10.0)))
+    hi = np.column_stack((L - zone - d, np.full(4, 10.0)))
+    F_lo = O.walls(lo, L, WALL_WCA, 1, zone)
+    F_hi = O.walls(hi, L, WALL_WCA, 1, zone)
+    h = 1e-6
+    want = -(lj_energy(d + h, WALL_WCA) - lj_energy(d - h, WALL_WCA)) / (2 * h)
+    assert np.allclose(F_lo[:, 0], want, rtol=1e-6)
+    assert np.allclose(F_hi[:, 0], -want, rtol=1e-6)
+    assert np.all(F_lo[:, 1] == 0) and np.all(F_hi[:, 1] == 0)  # no wall along y
+    both = O.walls(np.array([[zone + 0.95, L - zone - 0.95]]), L, WALL_WCA, 3, zone)
+    assert both[0, 0] == pytest.approx(-both[0, 1], rel=1e-15) and both[0, 0] > 0
+
+
+def test_walls_range_and_shift_quirks():
+    """The range test compares the squared distance with wca[3] itself
+    (forces.py:432), so a wall stops at sqrt(rc), not rc; WCA_shift is
+    subtracted from the distance (the pair law adds it, forces.py:124-127)."""
+    L, zone = 20.0, 0.75
+    rc = WALL_WCA[3]
+    inside, outside = np.sqrt(rc) - 1e-9, np.sqrt(rc) + 1e-9
+    assert outside < rc  # within the pair cutoff, yet beyond the wall's range
+    F = O.walls(np.array([[zone + inside, 5.0], [zone + outside, 5.0]]), L, WALL_WCA, 1, zone)
+    assert F[0, 0] != 0 and F[1, 0] == 0
+    shifted = WALL_WCA.copy()
+    shifted[2] = 0.1
+    F_s = O.walls(np.array([[zone + 1.05, 5.0]]), L, shifted, 1, zone)
+    F_0 = O.walls(np.array([[zone + 0.95, 5.0]]), L, WALL_WCA, 1, zone)
+    assert F_s[0, 0] == pytest.approx(F_0[0, 0], rel=1e-13)
+
+
+def test_srk2_walls_use_the_step_start_position():
+    """SRK.py:270 hands the walls pos in both stages: a lone bead at a wall
+    (T = 0, no other force) follows x' = x + dt_used F(x) / r, with m = |F(x)|/r
+    (1/2 (F + F)) setting dt."""
+    L, zone, r = 20.0, 0.75, 1.0
+    sysd = dict(dim=2, n=1, n_nearby=4, has_springs=False, has_wca=False, has_propel=False,
+                L=L, D=0.0, Dr=0.0, radius=r, wca=WALL_WCA, spring=np.zeros(6), v0=0.0,
+                walls=1, wall_zone=zone)
+    x0 = np.array([[zone + 0.92, 4.0]])
+    dtmax, mdps, n = 1e-3, 2e-3, 12
+    tv = np.array([1e-6, dtmax, mdps, 1.0, -1.0, 40, 40, n])
+    pos, *_ = O.srk2_run(sysd, x0, np.array([[1.0, 0.0]]), np.array([0]), np.ones(1),
+                         np.zeros(1), tv, xi=np.zeros((n, 1, 4)))
+    x = x0[0, 0]
+    for _ in range(n):
+        f = O.walls(np.array([[x, 4.0]]), L, WALL_WCA, 1, zone)[0, 0]
+        dt = min(dtmax, mdps / (abs(f) / r))
+        x = x + dt * f / r
+    assert pos[0, 0] == pytest.approx(x, rel=1e-13) and pos[0, 1] == 4.0
+
+
 def test_rotational_diffusion_closed_form():
     """SRK.py:398-423: t' = normalize(t + pi sqrt(2 Dr dt) (xi_r - (xi_r.t) t))."""
     Dr = 0.2
