@@ -133,6 +133,19 @@ struct gcr_pool {
     int nt_launch = BS;    // resolved by rebuild_tables
     bool has_srk4 = false; // a live replica integrates with SRK4: the step has 4 force stages
     bool tile_gen = false; // a live replica needs the general tile kernels (SRK4, walls, FENE)
+    // populations acting on each other: replicas in population order, the law
+    // of b on a at cross[(a * n + b) * 5] (flag, p0, p1, shift, cutoff), fixed
+    // populations
+    struct Group {
+        std::vector<int> rids;
+        std::vector<double> cross;
+        std::vector<int> fixed;
+    };
+    std::vector<Group> groups;
+    XPair* XPAIRS = nullptr;
+    int32_t* GRP = nullptr;
+    int64_t capX = 0;
+    bool has_group = false;
 };
 
 namespace {
@@ -225,7 +238,7 @@ void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
     P.D = d->D;
     P.Dr = d->Dr;
     P.radius = d->radius;
-    uint32_t fl = P.flags & GCR_F_INJECT;
+    uint32_t fl = P.flags & (GCR_F_INJECT | GCR_F_GROUP);
     if (d->has_wca) fl |= GCR_F_WCA;
     if (d->has_springs) fl |= GCR_F_SPRINGS;
     if (d->has_propel) fl |= GCR_F_PROPEL;
@@ -255,6 +268,44 @@ int rebuild_tables(gcr_pool* p)
 {
     std::vector<int32_t> br, bs, cr, cc, rco(p->capR + 1, 0), tl, hr, hc;
     bool has_srk4 = false, tile_gen = false;
+    std::vector<XPair> xp;
+    std::vector<int32_t> grp;
+    for (auto& P : p->P) P.group = -1;
+    for (int g = 0; g < (int)p->groups.size(); ++g) {
+        // members in population order; each moving member takes the laws of
+        // the others in the order (Q - P) mod n and the group's moving list
+        const gcr_pool::Group& G = p->groups[g];
+        const int n = (int)G.rids.size();
+        const int32_t first = (int32_t)grp.size();
+        for (int k = 0; k < n; ++k)
+            if (!G.fixed[k]) grp.push_back(G.rids[k]);
+        for (int k = 0; k < n; ++k) {
+            RepParams& P = p->P[G.rids[k]];
+            P.group = g;
+            P.grp_first = first;
+            P.grp_n = (int32_t)grp.size() - first;
+            P.xin_off = (int32_t)xp.size();
+            P.xin_n = 0;
+            if (G.fixed[k]) continue;
+            for (int ii = 1; ii < n; ++ii) {
+                const int o = (k + ii) % n;
+                const double* c = &G.cross[((size_t)k * n + o) * 5];
+                if (c[0] == 0.0) continue;
+                const RepParams& Q = p->P[G.rids[o]];
+                XPair e;
+                e.src = G.rids[o];
+                const double reach = c[4] + (Q.skin_half < Q.L ? Q.skin_half : Q.L);
+                e.window = std::max(1, (int)std::ceil(reach / Q.h));
+                e.p0 = c[1];
+                e.p1 = c[2];
+                e.shift = c[3];
+                e.rc = c[4];
+                xp.push_back(e);
+                ++P.xin_n;
+            }
+        }
+    }
+    if ((int64_t)xp.size() > p->capX) return fail(GCR_ERR_CAPACITY, "population law table full");
     for (int r = 0; r < p->capR; ++r) {
         rco[r] = (int32_t)cr.size();
         RepParams& P = p->P[r];
@@ -270,7 +321,7 @@ int rebuild_tables(gcr_pool* p)
             cc.push_back(P.cell_off + c);
         }
         if (P.flags & GCR_F_SRK4) has_srk4 = true;
-        if (P.flags & (GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE)) tile_gen = true;
+        if (P.flags & (GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE | GCR_F_GROUP)) tile_gen = true;
         if (P.flags & GCR_F_ROD)
             for (int c = 0; c < P.n_chains; c += BS) {
                 hr.push_back(r);
@@ -297,6 +348,10 @@ int rebuild_tables(gcr_pool* p)
             P.ntile = n;
             P.red_off = P.tile_off;
             P.red_n = P.ntile;
+            if (P.flags & GCR_F_GROUP) {  // maxima per bead block, k_group_stage
+                P.red_off = (int32_t)p->capTile + P.blk_off;
+                P.red_n = P.nblk;
+            }
         } else {
             P.red_off = P.blk_off;
             P.red_n = P.nblk;
@@ -319,6 +374,12 @@ int rebuild_tables(gcr_pool* p)
     p->nchb = (int)hr.size();
     p->has_srk4 = has_srk4;
     p->tile_gen = tile_gen;
+    p->has_group = !p->groups.empty();
+    if (!xp.empty())
+        HIPCHK(hipMemcpyAsync(p->XPAIRS, xp.data(), xp.size() * sizeof(XPair), hipMemcpyHostToDevice,
+                              p->stream));
+    if (!grp.empty())
+        HIPCHK(hipMemcpyAsync(p->GRP, grp.data(), grp.size() * 4, hipMemcpyHostToDevice, p->stream));
     if (p->nchb) {
         HIPCHK(hipMemcpyAsync(p->chb_rep, hr.data(), hr.size() * 4, hipMemcpyHostToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(p->chb_chain0, hc.data(), hc.size() * 4, hipMemcpyHostToDevice, p->stream));
@@ -480,6 +541,17 @@ void launch_rod(gcr_pool* p, int stage, int j, int sub = 0)
                    : k_rod<3, 1><<<p->nchb, BS, 0, p->stream>>>(a, j);
 }
 
+// Forces between the populations of groups, after a force stage.
+void launch_group(gcr_pool* p, int stage, int j)
+{
+    if (p->dim == 2)
+        stage == 0 ? k_group_stage<2, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
+                   : k_group_stage<2, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
+    else
+        stage == 0 ? k_group_stage<3, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
+                   : k_group_stage<3, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
+}
+
 // SRK4 virtual update of stage `sub` (1 or 2) for the replicas without rod.
 void launch_virtual(gcr_pool* p, int j, int sub)
 {
@@ -512,6 +584,7 @@ void launch_batch(gcr_pool* p, int K, bool safe)
     for (int j = 0; j < p->B; ++j) {
         if (j % K == 0) launch_rebuild(p, j);
         launch_stage(p, 0, j);
+        if (p->has_group) launch_group(p, 0, j);
         launch_rod(p, 0, j, 0);
         if (safe) launch_rebuild(p, j, true, 0);
         for (int sub = 1; p->has_srk4 && sub <= 2; ++sub) {
@@ -521,7 +594,9 @@ void launch_batch(gcr_pool* p, int K, bool safe)
             if (safe) launch_rebuild(p, j, true, sub);
         }
         launch_stage(p, 1, j);
+        if (p->has_group) launch_group(p, 1, j);
         launch_stage(p, 3, j);
+        if (p->has_group) k_group_dt<<<(p->capR + 63) / 64, 64, 0, p->stream>>>(p->a, j);
         launch_rod(p, 1, j);
         launch_stage(p, 2, j);
     }
@@ -550,7 +625,7 @@ int ensure_ready(gcr_pool* p)
 
 bool running(const RepState& s)
 {
-    if (s.frozen) return false;
+    if (s.frozen || s.fixed) return false;
     bool a = s.max_steps < 0 || s.steps < s.max_steps;
     bool b = s.mdu < 0 || s.elapsed < s.mdu;
     return a && b;
@@ -569,8 +644,18 @@ int count_running(gcr_pool* p)
 int replay(gcr_pool* p)
 {
     std::vector<int> which;
+    // populations of a group move together: one member's violation replays
+    // the whole group
+    for (const auto& G : p->groups) {
+        bool any = false;
+        for (int r : G.rids) any = any || p->S[r].violation;
+        if (any)
+            for (size_t k = 0; k < G.rids.size(); ++k)
+                if (!G.fixed[k]) p->S[G.rids[k]].violation = 1;
+    }
     for (int r = 0; r < p->capR; ++r) {
         if (!p->live[r]) continue;
+        if (p->S[r].fixed) continue;
         if (p->S[r].violation) {
             which.push_back(r);
             p->S[r].frozen = 2;
@@ -736,7 +821,12 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CCOUNT, capC + 1));
     A(dalloc(&p->CSTART, capC + 1));
     A(dalloc(&p->CHKSUM, p->capChk));
-    A(dalloc(&p->BMAX, std::max(p->capBlk, p->capTile)));
+    // tile or block maxima, and the bead-block maxima of 2D group members
+    // after the tile range
+    A(dalloc(&p->BMAX, p->capBlk + p->capTile));
+    p->capX = (int64_t)capR * GCR_MAX_POPS;
+    A(dalloc(&p->XPAIRS, p->capX));
+    A(dalloc(&p->GRP, capR));
     A(dalloc(&p->TILES, p->capTile * 4));
     A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
     // neighbour lists: 2D u16 LDS indices (NL_MAX), 3D int32 slots (NL3_MAX),
@@ -805,6 +895,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.Ts = p->Ts;
     a.METAs = p->METAs;
     a.Ss = p->dSs;
+    a.XPAIRS = p->XPAIRS;
+    a.GRP = p->GRP;
+    a.red_base = dim == 2 ? (int32_t)p->capTile : 0;
     *out = p;
     return GCR_OK;
 }
@@ -951,10 +1044,78 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     return GCR_OK;
 }
 
+// Members of group g become independent replicas again.
+void dissolve_group(gcr_pool* p, int g)
+{
+    for (int r : p->groups[g].rids) {
+        p->P[r].flags &= ~GCR_F_GROUP;
+        p->P[r].group = -1;
+        p->S[r].fixed = 0;
+    }
+    p->groups.erase(p->groups.begin() + g);
+    p->tables_dirty = true;
+}
+
+int group_of(const gcr_pool* p, int rid)
+{
+    for (int g = 0; g < (int)p->groups.size(); ++g)
+        for (int r : p->groups[g].rids)
+            if (r == rid) return g;
+    return -1;
+}
+
+int gcr_pool_group(gcr_pool* p, const int32_t* rids, int32_t n, const double* cross,
+                   const int32_t* fixed)
+{
+    if (!p || !rids) return fail(GCR_ERR_ARG, "null argument");
+    if (n < 2 || n > GCR_MAX_POPS) return fail(GCR_ERR_ARG, "a group has 2 to GCR_MAX_POPS populations");
+    gcr_pool::Group G;
+    int moving = 0;
+    for (int k = 0; k < n; ++k) {
+        int rc = check_rid(p, rids[k]);
+        if (rc) return rc;
+        for (int q = 0; q < k; ++q)
+            if (rids[q] == rids[k]) return fail(GCR_ERR_ARG, "a replica appears twice in the group");
+        if (group_of(p, rids[k]) >= 0) return fail(GCR_ERR_ARG, "replica already in a group");
+        const RepParams& P = p->P[rids[k]];
+        if (P.L != p->P[rids[0]].L) return fail(GCR_ERR_ARG, "the populations of a group share one box");
+        if (P.flags & (GCR_F_SRK4 | GCR_F_ROD))
+            return fail(GCR_ERR_ARG, "groups run SRK_order 2 without the rigid rod");
+        G.rids.push_back(rids[k]);
+        G.fixed.push_back(fixed && fixed[k] ? 1 : 0);
+        moving += G.fixed.back() ? 0 : 1;
+    }
+    if (!moving) return fail(GCR_ERR_ARG, "a group needs a moving population");
+    G.cross.assign((size_t)n * n * 5, 0.0);
+    if (cross) std::memcpy(G.cross.data(), cross, sizeof(double) * G.cross.size());
+    int rc = pull_states(p);
+    if (rc) return rc;
+    for (int k = 0; k < n; ++k) {
+        RepParams& P = p->P[rids[k]];
+        if (G.fixed[k]) {
+            P.flags &= ~GCR_F_GROUP;
+            p->S[rids[k]].fixed = 1;
+        } else {
+            P.flags |= GCR_F_GROUP;
+            p->S[rids[k]].fixed = 0;
+        }
+    }
+    p->groups.push_back(G);
+    p->tables_dirty = true;
+    if ((rc = push_states(p))) return rc;
+    return rebuild_tables(p);
+}
+
 int gcr_replica_remove(gcr_pool* p, int32_t rid)
 {
     int rc = check_rid(p, rid);
     if (rc) return rc;
+    const int g = group_of(p, rid);
+    if (g >= 0) {
+        if ((rc = pull_states(p))) return rc;
+        dissolve_group(p, g);
+        if ((rc = push_states(p))) return rc;
+    }
     RepParams& P = p->P[rid];
     p->segB.release(P.bead_off, P.n);
     p->segC.release(P.cell_off, P.ncell + 1);
@@ -981,6 +1142,9 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     if (rc) return rc;
     if (!d) return fail(GCR_ERR_ARG, "null descriptor");
     if (!srk_order_ok(d)) return fail(GCR_ERR_ARG, "srk_order must be 2 or 4");
+    const int grp = group_of(p, rid);
+    if (grp >= 0 && (d->srk_order == 4 || (d->rod != 0.0)))
+        return fail(GCR_ERR_ARG, "groups run SRK_order 2 without the rigid rod");
     RepParams& P = p->P[rid];
     if (d->n_beads != P.n || d->n_chains != P.n_chains || d->L != P.L)
         return fail(GCR_ERR_ARG, "beads, chains and box must stay the same");
@@ -1011,6 +1175,7 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     // rule takes effect there
     if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA | GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE))
         p->tables_dirty = true;
+    if (grp >= 0) p->tables_dirty = true;  // the scan windows follow the skin
     return GCR_OK;
 }
 
@@ -1200,7 +1365,8 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
     if (!p || !ms_out || steps < 1) return fail(GCR_ERR_ARG, "bad arguments");
     int rc;
     if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
-    if (p->has_srk4) return fail(GCR_ERR_ARG, "kernel timing covers pools of SRK2 replicas");
+    if (p->has_srk4 || p->has_group)
+        return fail(GCR_ERR_ARG, "kernel timing covers pools of independent SRK2 replicas");
     // The step sequence of the batch graphs, launched directly; the three
     // per-step kernels record start/end events in their own dispatch packets
     // (hipExtLaunchKernelGGL) on the pool's stream; the re-sort (several
@@ -1281,7 +1447,8 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->XIn, p->X1n,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
-                    p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR, p->OBS};
+                    p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR, p->OBS,
+                    p->XPAIRS, p->GRP};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

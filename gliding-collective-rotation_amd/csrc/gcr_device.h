@@ -22,6 +22,18 @@
 #define GCR_F_ROD 256u      // rigid-rod projection of the updates (updates.py:347-499)
 #define GCR_F_SRK4 512u     // SRK_order 4 (SRK.py:186-215, 297-321)
 #define GCR_F_WALLS 1024u   // WCA walls (forces.py:405-435)
+#define GCR_F_GROUP 2048u   // a moving population of a group: other populations act on it
+                            // (updates.py:132-267, SRK.py:287-293)
+
+// The law of one population acting on another (interpop_forces,
+// updates.py:274-312): a WCA/LJ pair law and the source replica, with the
+// half-width in the source's cells of the scan that finds every source bead
+// within range.
+struct XPair {
+    int32_t src;
+    int32_t window;
+    double p0, p1, shift, rc;
+};
 
 struct RepParams {
     int32_t n;          // beads
@@ -49,6 +61,10 @@ struct RepParams {
     double k, kappa, theta0;  // spring_params[0..2]
     double fene_tol;          // spring_params[5]: FENE maximal stretch / rest length
     double rod;               // rigid_rod_projection_factor
+    int32_t group;            // group index, -1: none
+    int32_t grp_first, grp_n; // the group's moving replicas in GRP (the dt is theirs)
+    int32_t xin_off, xin_n;   // laws of the other populations acting on this one (XPAIRS)
+    int32_t pad1;
     double wall_zone;         // distance of the walls from the box faces
     double wall_rc;           // the walls' range parameter (wall_params[3])
     int32_t walls;            // bit i: walls normal to axis i
@@ -67,6 +83,7 @@ struct RepState {
     double dt_used;   // dt_used of the most recent step (corrector step size)
     double elapsed;   // sum of dt_used in the current call
     double maxm;      // max_forces_strength of the most recent step
+    double mpart;     // a group member's own sqrt(2D) + max force term (the group takes the max)
     int64_t steps;        // steps done in the current call
     int64_t step_global;  // steps done since the replica was created (noise key)
     int64_t batch_base;   // `steps` at the start of the running batch
@@ -76,6 +93,7 @@ struct RepState {
     int32_t frozen;       // 1: kernels leave this replica untouched
     int32_t pred_far;     // safe batch: bit k, the virtual positions of SRK stage k
                           // left the skin
+    int32_t fixed;        // a fixed population of a group: never moves (pop.fixed)
 };
 
 struct PoolArgs {
@@ -148,6 +166,10 @@ struct PoolArgs {
     // on_x1 - that mid-step re-sort: cells, lists and XR from X1, and the
     //         live force array (SRK2's stage-0 forces, SRK4's running
     //         total) travels with the beads
+    // populations acting on each other
+    const XPair* XPAIRS;
+    const int32_t* GRP;
+    int32_t red_base;  // first BMAX entry of the bead-block reductions of 2D group members
     int32_t safe;
     int32_t on_x1;
     // SRK stage of this launch: 0 the first force evaluation (at X), 1 and 2

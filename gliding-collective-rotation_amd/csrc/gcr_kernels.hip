@@ -142,7 +142,7 @@ __device__ __forceinline__ void draw_noise(const RepParams& P, int orig_global, 
 // Per-block running condition of a replica for local step j of a batch.
 __device__ __forceinline__ bool step_runs(const RepState* S, int j)
 {
-    if (S->frozen) return false;
+    if (S->frozen || S->fixed) return false;
     if (S->steps != S->batch_base + j) return false;
     bool a = (S->max_steps < 0) || (S->steps < S->max_steps);
     bool b = (S->mdu < 0) || (S->elapsed < S->mdu);
@@ -158,6 +158,7 @@ __device__ __forceinline__ bool rebuild_runs(const PoolArgs& a, int r, int j)
 {
     const RepState* S = a.S + r;
     if (S->frozen) return false;
+    if (S->fixed) return j < 0 && !a.on_x1;  // a fixed population is sorted once
     if (a.on_x1) return step_runs(S, j) && ((S->pred_far >> a.sub) & 1);
     return j < 0 || step_runs(S, j);
 }
@@ -955,6 +956,12 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
                                               const double* acc)
 {
     const bool srk4 = GEN && (P.flags & GCR_F_SRK4) != 0;
+    if (GEN && STAGE != 2 && (P.flags & GCR_F_GROUP)) {
+        // a group member's own forces; the other populations' forces, the
+        // virtual update and the max follow in k_group_stage
+        store_vec<DIM>(STAGE == 0 ? a.F0 : a.FT, s, acc);
+        return 0.0;
+    }
     if (STAGE == 0) {
         store_vec<DIM>(a.F0, s, acc);
         if (srk4) {
@@ -1073,6 +1080,10 @@ __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
     if (threadIdx.x == 0) {
         double m = sqrt(2.0 * P.D);
         if (P.n > 0) m += force_prefactor(P) * sqrt(m2) / P.radius;
+        if (P.flags & GCR_F_GROUP) {  // k_group_dt takes the group's max
+            S->mpart = m;
+            return;
+        }
         double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
         double dt_used = P.cheat * dt;
         if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
@@ -1083,6 +1094,129 @@ __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
         S->steps += 1;
         S->step_global += 1;
     }
+}
+
+// ------------------------------------------------------------ populations
+
+// Forces on a bead at x from the beads of another population Q
+// (forces.py:57-144 with same_pop False: every Q bead within the law's
+// cutoff, no exclusions, the force coeff * (x_q - x), WCA_shift as in the
+// pair law), in FP64, found by scanning Q's cells within e.window of the
+// cell holding x in Q's grid (the window covers the cutoff plus the half skin
+// by which Q's beads may have left their cells).
+template <int DIM>
+__device__ __forceinline__ void cross_sum(const PoolArgs& a, const RepParams& Q, const double* XQ,
+                                          const double* x, const XPair& e, double* f)
+{
+    const int nc = Q.nc;
+    int c[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) c[i] = cell_coord(x[i], Q.inv_h, nc);
+    const bool all = 2 * e.window + 1 >= nc;
+    const int lo = all ? 0 : -e.window, hi = all ? nc - 1 : e.window;
+    const double rc2 = e.rc * e.rc;
+    for (int dz = (DIM == 3 ? lo : 0); dz <= (DIM == 3 ? hi : 0); ++dz) {
+        const int z = DIM == 3 ? (all ? dz : (c[2] + dz + nc) % nc) : 0;
+        for (int dy = lo; dy <= hi; ++dy) {
+            const int y = all ? dy : (c[1] + dy + nc) % nc;
+            for (int dx = lo; dx <= hi; ++dx) {
+                const int xc = all ? dx : (c[0] + dx + nc) % nc;
+                const int cell = Q.cell_off + (z * nc + y) * nc + xc;
+                const int t1 = a.CSTART[cell + 1];
+                for (int t = a.CSTART[cell]; t < t1; ++t) {
+                    double q[DIM], sep[DIM], ds = 0.0;
+                    load_vec<DIM>(XQ, t, q);
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) {
+                        double d = q[i] - x[i];
+                        if (d < -Q.halfL)
+                            d += Q.L;
+                        else if (d >= Q.halfL)
+                            d -= Q.L;
+                        sep[i] = d;
+                        ds += d * d;
+                    }
+                    if (!(ds > 0.0 && ds <= rc2)) continue;
+                    if (e.shift != 0.0) {
+                        const double mag = sqrt(ds);
+                        double nm = mag + e.shift;
+                        nm = nm > 0.0 ? nm : 0.0;
+                        const double sc = nm / mag;
+#pragma unroll
+                        for (int i = 0; i < DIM; ++i) sep[i] *= sc;
+                        ds = nm * nm;
+                    }
+                    const double cf = PairLaw<double>::coeff(ds, e.p0, e.p1);
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) f[i] += sep[i] * cf;
+                }
+            }
+        }
+    }
+}
+
+// Second half of a force stage for the moving populations of a group, one
+// thread per bead: the forces of the other populations, in the order
+// (Q - P) mod n (SRK.py:287-293), at their positions of this stage (x1 in
+// stage 1 unless fixed), added to the bead's own forces; then STAGE 0 the
+// predictor, STAGE 1 the block's max |F|^2 for k_finalize.
+template <int DIM, int STAGE>
+__global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    const RepParams& P = a.P[r];
+    RepState* S = a.S + r;
+    if (!(P.flags & GCR_F_GROUP) || !step_runs(S, j)) return;
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    double fsq = 0.0;
+    if (s < P.bead_off + P.n) {
+        double x[DIM], f[DIM];
+        load_vec<DIM>(STAGE == 0 ? a.X : a.X1, s, x);
+        load_vec<DIM>(STAGE == 0 ? a.F0 : a.FT, s, f);
+        for (int k = P.xin_off; k < P.xin_off + P.xin_n; ++k) {
+            const XPair e = a.XPAIRS[k];
+            const double* XQ = (STAGE == 0 || a.S[e.src].fixed) ? a.X : a.X1;
+            cross_sum<DIM>(a, a.P[e.src], XQ, x, e, f);
+        }
+        if (STAGE == 0) {
+            store_vec<DIM>(a.F0, s, f);
+            virtual_update<DIM>(a, P, S, s, x, f, 1.0, 0);
+        } else {
+            store_vec<DIM>(a.FT, s, f);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) fsq += f[i] * f[i];
+        }
+    }
+    if (STAGE == 1) {
+        const double m = block_max(fsq);
+        if (threadIdx.x == 0) a.BMAX[P.red_off + (blockIdx.x - P.blk_off)] = m;
+    }
+}
+
+// One dt for a group: the largest member part (SRK.py:336-337 takes the max
+// over every thread of every population), then the step bookkeeping of
+// k_finalize for each moving member.
+__global__ void k_group_dt(PoolArgs a, int j)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nrep) return;
+    const RepParams& P = a.P[r];
+    RepState* S = a.S + r;
+    if (!(P.flags & GCR_F_GROUP) || !step_runs(S, j)) return;
+    double m = 0.0;
+    for (int k = P.grp_first; k < P.grp_first + P.grp_n; ++k) {
+        const double v = a.S[a.GRP[k]].mpart;
+        m = v > m ? v : m;
+    }
+    double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
+    double dt_used = P.cheat * dt;
+    if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
+    S->dt = dt;
+    S->dt_used = dt_used;
+    S->elapsed += dt_used;
+    S->maxm = m;
+    S->steps += 1;
+    S->step_global += 1;
 }
 
 // Corrector update (SRK.py:355-388) and rotational diffusion (SRK.py:398-423).

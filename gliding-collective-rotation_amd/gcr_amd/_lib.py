@@ -53,7 +53,7 @@ EXPORTS = [
     "gcr_replica_upload", "gcr_replica_begin", "gcr_pool_run", "gcr_replica_status",
     "gcr_replica_download", "gcr_replica_eval_forces", "gcr_pool_time_kernels",
     "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_threads", "gcr_init_chains",
-    "gcr_replica_order", "gcr_replica_set_physics",
+    "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group",
 ]
 
 _lib = None
@@ -97,6 +97,7 @@ def load():
     L.gcr_pool_set_tile_threads.argtypes = [vp, i32]
     L.gcr_replica_order.argtypes = [vp, i32, dp]
     L.gcr_replica_set_physics.argtypes = [vp, i32, C.POINTER(ReplicaDesc), vp, vp]
+    L.gcr_pool_group.argtypes = [vp, vp, i32, vp, vp]
     L.gcr_create.argtypes = [C.POINTER(ReplicaDesc), i32, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]
     L.gcr_upload.argtypes = [vp, vp, vp]
     L.gcr_run.argtypes = [vp, vp, vp, vp, i64]

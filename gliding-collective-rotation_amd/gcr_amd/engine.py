@@ -186,6 +186,21 @@ class Pool:
                                         out.ctypes.data_as(C.POINTER(C.c_double))))
         return dict(polar=out[0], nematic=out[1], director=out[2], n_chains=int(out[3]))
 
+    def group(self, rids, cross=None, fixed=None):
+        """Make replicas `rids` (in population order) the populations of one
+        system: cross[(a, b)] = the WCA law 4-vector of population b acting on
+        a (interpop_forces); fixed = the indices of fixed populations."""
+        n = len(rids)
+        r = np.ascontiguousarray(rids, dtype=np.int32)
+        c = np.zeros((n, n, 5))
+        for (a, b), law in (cross or {}).items():
+            c[a, b, 0] = 1.0
+            c[a, b, 1:] = law
+        fx = np.zeros(n, dtype=np.int32)
+        for k in fixed or ():
+            fx[k] = 1
+        check(self._L.gcr_pool_group(self.h, ptr(r), n, ptr(c), ptr(fx)))
+
     def time_kernels(self, steps):
         out = np.zeros(4)
         check(self._L.gcr_pool_time_kernels(self.h, int(steps),

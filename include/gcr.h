@@ -29,6 +29,8 @@ extern "C" {
 #define GCR_ERR_CAPACITY (-3) /* pool capacity exhausted */
 #define GCR_ERR_STATE (-4)    /* call out of order (e.g. run before begin) */
 
+#define GCR_MAX_POPS 16 /* populations in one group (gcr_pool_group) */
+
 #define GCR_FP64 0  /* forces and accumulation in FP64 */
 #define GCR_MIXED 1 /* pair forces in FP32, accumulation and state in FP64 */
 
@@ -128,6 +130,21 @@ int gcr_replica_upload(gcr_pool* p, int32_t rid, const double* pos, const double
  * made at gcr_replica_add (desc->skin there sets it). */
 int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* desc,
                             const double* rest_lengths, const double* v0_per_chain);
+/* Makes replicas rids[0..n) of the pool (same box, n <= GCR_MAX_POPS,
+ * SRK_order 2, no rigid rod) the populations of one system, in the order of
+ * config.pop_name_list: the pops list SRK2_force_calc receives
+ * (updates.py:132-267). cross[(a * n + b) * 5] nonzero: population b acts on
+ * population a with the WCA/LJ law cross[(a * n + b) * 5 + 1 .. 4] = {12 eps
+ * r^6, 12 eps r^12, shift, cutoff}, as interpop_forces(pop_a, pop_b) gives it
+ * (updates.py:274-312: interactions() averages the two WCA vectors,
+ * populations.py:226-232; special_interactions override); each replica's own
+ * desc->wca is its self law. fixed[a] nonzero (NULL: none): population a never
+ * moves (pop.fixed) and only acts on the others. Every moving member then
+ * steps with one dt, the largest sqrt(2 D) + max|F| / (2 r) of them
+ * (SRK.py:323-353); begin each moving member with the same time_vals. Removing
+ * a member dissolves the group. */
+int gcr_pool_group(gcr_pool* p, const int32_t* rids, int32_t n, const double* cross,
+                   const int32_t* fixed);
 /* Arms the next call of a replica (same meaning of time_vals / xi as gcr_run). */
 int gcr_replica_begin(gcr_pool* p, int32_t rid, const double time_vals[8], const double* xi,
                       int64_t xi_steps);

@@ -784,3 +784,132 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
     }
     return steps;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Several populations (updates.py:132-267, SRK.py:44, 287-293). */
+
+void orc_cross_pair_forces(int dim, int nA, const double* posA, int nB, const double* posB,
+                           double L, const double p[4], double* FA)
+{
+    double half = L / 2.0, rc2 = p[3] * p[3];
+    for (int a = 0; a < nA; ++a)
+        for (int b = 0; b < nB; ++b) {
+            double f[3];
+            if (pair_term(dim, posA + a * dim, posB + b * dim, L, half, p, rc2, f))
+                for (int i = 0; i < dim; ++i) FA[a * dim + i] += f[i];
+        }
+}
+
+int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const double tv[8],
+                           double status_out[4], int64_t step0)
+{
+    double dt = tv[0], dtmax = tv[1], mdps = tv[2], cheat = tv[3], mdu = tv[4];
+    double max_steps = tv[7];
+    int dim = pops[0].sys.dim;
+    double L = pops[0].sys.L;
+    int nd = 2 * dim;
+    double** xi_buf = (double**)calloc((size_t)npop, sizeof(double*));
+    for (int q = 0; q < npop; ++q)
+        xi_buf[q] = (double*)malloc(sizeof(double) * (size_t)(pops[q].sys.n > 0 ? pops[q].sys.n : 1) * nd);
+    int64_t steps = 0;
+    double elapsed = 0.0, m = 0.0, dt_for_update = dt;
+    while ((steps < max_steps || max_steps < 0) && (elapsed < mdu || mdu < 0)) {
+        const double* xi[16];
+        for (int q = 0; q < npop && q < 16; ++q) {
+            const orc_pop* P = &pops[q];
+            if (P->xi) {
+                xi[q] = P->xi + (size_t)steps * P->sys.n * nd;
+            } else {
+                for (int b = 0; b < P->sys.n; ++b)
+                    orc_bead_noise(dim, P->seed, (uint32_t)b, step0 + steps,
+                                   xi_buf[q] + (size_t)b * nd);
+                xi[q] = xi_buf[q];
+            }
+        }
+        for (int stage = 0; stage < 2; ++stage) {
+            /* forces of every moving population, then (stage 1) one dt */
+            for (int q = 0; q < npop; ++q) {
+                orc_pop* P = &pops[q];
+                if (P->fixed) continue;
+                int n = P->sys.n;
+                const double* at = stage == 0 ? P->pos : P->pos1;
+                if (stage == 0) memset(P->forces, 0, sizeof(double) * (size_t)n * dim);
+                orc_force_eval(&P->sys, at, P->chain_id, P->rest_lengths, P->vtan, P->tangent,
+                               P->forces, 0, NULL, P->pos);
+                for (int ii = 1; ii < npop; ++ii) {
+                    int o = (q + ii) % npop;
+                    const double* c = cross + ((size_t)q * npop + o) * 5;
+                    if (c[0] == 0.0) continue;
+                    const orc_pop* Q = &pops[o];
+                    const double* qpos = (stage == 0 || Q->fixed) ? Q->pos : Q->pos1;
+                    orc_cross_pair_forces(dim, n, at, Q->sys.n, qpos, L, c + 1, P->forces);
+                }
+            }
+            if (stage == 1) {
+                m = 0.0;
+                for (int q = 0; q < npop; ++q) {
+                    const orc_pop* P = &pops[q];
+                    if (P->fixed) continue;
+                    double mx = 0.0;
+                    for (int b = 0; b < P->sys.n; ++b) {
+                        double s2 = 0.0;
+                        for (int i = 0; i < dim; ++i)
+                            s2 += P->forces[b * dim + i] * P->forces[b * dim + i];
+                        if (s2 > mx) mx = s2;
+                    }
+                    double mp = sqrt(2.0 * P->sys.D);
+                    if (P->sys.n > 0) mp += 0.5 * sqrt(mx) / P->sys.radius;
+                    if (mp > m) m = mp;
+                }
+                steps += 1;
+                dt = (m == 0.0) ? dtmax : fmin(dtmax, mdps / m);
+                double dt_used = cheat * dt;
+                if (mdu > 0) dt_used = fmin(dt_used, mdu - elapsed);
+                elapsed += dt_used;
+                dt_for_update = dt_used;
+            } else {
+                dt_for_update = dt;
+            }
+            double pref = stage == 0 ? 1.0 : 0.5;
+            for (int q = 0; q < npop; ++q) {
+                orc_pop* P = &pops[q];
+                if (P->fixed) continue;
+                double* out = stage == 0 ? P->pos1 : P->pos;
+                double dt_eff = dt_for_update * pref / P->sys.radius;
+                double s2 = sqrt(2.0 * dt_for_update * P->sys.D);
+                for (int b = 0; b < P->sys.n; ++b)
+                    for (int i = 0; i < dim; ++i) {
+                        double tmp = P->pos[b * dim + i] + dt_eff * P->forces[b * dim + i];
+                        if (s2 != 0.0) tmp += s2 * xi[q][(size_t)b * nd + i];
+                        out[b * dim + i] = orc_pymod(tmp, L);
+                    }
+            }
+        }
+        for (int q = 0; q < npop; ++q) {  /* SRK.py:398-423 per population */
+            orc_pop* P = &pops[q];
+            int orient = P->sys.has_springs && P->sys.spring[4] != 0.0;
+            if (P->fixed || !(P->sys.Dr > 0) || orient) continue;
+            double sr = sqrt(2.0 * dt_for_update * P->sys.Dr) * 3.141592653589793;
+            for (int b = 0; b < P->sys.n; ++b) {
+                double nr[3], proj = 0.0, mag = 0.0;
+                double* t = P->tangent + b * dim;
+                for (int i = 0; i < dim; ++i) nr[i] = xi[q][(size_t)b * nd + dim + i];
+                for (int i = 0; i < dim; ++i) proj += nr[i] * t[i];
+                for (int i = 0; i < dim; ++i) nr[i] -= proj * t[i];
+                for (int i = 0; i < dim; ++i) {
+                    t[i] = t[i] + sr * nr[i];
+                    mag += t[i] * t[i];
+                }
+                mag = sqrt(mag);
+                for (int i = 0; i < dim; ++i) t[i] /= mag;
+            }
+        }
+    }
+    status_out[0] = dt;
+    status_out[1] = elapsed;
+    status_out[2] = (double)steps;
+    status_out[3] = m;
+    for (int q = 0; q < npop; ++q) free(xi_buf[q]);
+    free(xi_buf);
+    return steps;
+}

@@ -123,6 +123,41 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
                      double status_out[4], const double* xi, uint64_t seed, int64_t step0,
                      int pair_mode, int n_threads, double grid_factor);
 
+/* forces.py:57-144 between two populations (a thread with same_pop False):
+ * every pair (a of A, b of B) with 0 < |x_b - x_a|^2 <= rc^2, no exclusions;
+ * the force on a is coeff * (x_b - x_a), WCA_shift as in the pair law.
+ * Accumulates into FA. */
+void orc_cross_pair_forces(int dim, int nA, const double* posA, int nB, const double* posB,
+                           double L, const double p[4], double* FA);
+
+/* One population of a multi-population run. */
+typedef struct orc_pop {
+    orc_sys sys;              /* its physics; sys.wca is its self interaction */
+    int32_t fixed;            /* pop.fixed: never moves and only pushes the others */
+    int32_t pad;
+    double* pos;
+    double* pos1;
+    double* tangent;
+    double* forces;
+    const int64_t* chain_id;
+    const double* rest_lengths;
+    const double* vtan;
+    const double* xi;         /* [steps][n][2 dim] or NULL: orc_bead_noise(seed, ...) */
+    uint64_t seed;
+} orc_pop;
+
+/* The SRK2 loop over several populations (updates.py:132-267 with the
+ * same_pop False threads of SRK.py): each moving population P evaluates its
+ * own forces (orc_force_eval), then adds the forces from every other
+ * population Q in the order (Q - P) mod npop = 1, 2, ... (SRK.py:287-293)
+ * where cross[(P * npop + Q) * 5] != 0, with the law cross[... + 1 .. 4]
+ * (interpop_forces, updates.py:274-312), Q's positions being pos1 in the
+ * second stage unless Q is fixed; m is the largest sqrt(2 D_P) + 1/2 max|F_P|
+ * / r_P over the moving populations, one dt for all; each population updates
+ * with its own D, radius and noise. Returns steps done. */
+int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const double time_vals[8],
+                           double status_out[4], int64_t step0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,23 @@ class OrcSys(C.Structure):
     ]
 
 
+class OrcPop(C.Structure):
+    _fields_ = [
+        ("sys", OrcSys),
+        ("fixed", C.c_int32),
+        ("pad", C.c_int32),
+        ("pos", C.c_void_p),
+        ("pos1", C.c_void_p),
+        ("tangent", C.c_void_p),
+        ("forces", C.c_void_p),
+        ("chain_id", C.c_void_p),
+        ("rest_lengths", C.c_void_p),
+        ("vtan", C.c_void_p),
+        ("xi", C.c_void_p),
+        ("seed", C.c_uint64),
+    ]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -66,6 +83,11 @@ def lib():
         L.orc_srk2_run.argtypes = [C.POINTER(OrcSys), dp, dp, dp, dp, ip, dp, dp, dp, dp,
                                    C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_int,
                                    C.c_double]
+        L.orc_cross_pair_forces.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                            C.c_double, C.c_void_p, C.c_void_p]
+        L.orc_multi_srk2_run.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_int64]
+        L.orc_multi_srk2_run.restype = C.c_int64
         L.orc_walls.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, C.c_int, C.c_double, dp]
         L.orc_force_eval.argtypes = [C.POINTER(OrcSys), dp, ip, dp, dp, dp, dp, C.c_int,
                                      C.c_void_p, C.c_void_p]
@@ -174,6 +196,57 @@ def force_eval(sysd, pos, chain_id, rest_lengths, vtan, tangent):
                          np.ascontiguousarray(rest_lengths, dtype=np.float64),
                          np.ascontiguousarray(vtan, dtype=np.float64), T, F, 0, None, None)
     return F, T
+
+
+def cross_pair_forces(posA, posB, L, p):
+    posA = np.ascontiguousarray(posA, dtype=np.float64)
+    posB = np.ascontiguousarray(posB, dtype=np.float64)
+    F = np.zeros_like(posA)
+    lib().orc_cross_pair_forces(posA.shape[1], len(posA), posA.ctypes.data, len(posB),
+                                posB.ctypes.data, float(L),
+                                np.ascontiguousarray(p, dtype=np.float64).ctypes.data,
+                                F.ctypes.data)
+    return F
+
+
+def multi_srk2_run(pops, cross, time_vals, step0=0):
+    """orc_multi_srk2_run. pops: list of dicts with sysd, pos, tangent,
+    chain_id, rest_lengths, vtan, and optionally xi, seed, fixed; cross:
+    {(P, Q): wca 4-vector} for the forces of Q on P. Returns a list of
+    (pos, pos1, tangent, forces) per population and the status."""
+    n = len(pops)
+    assert 1 <= n <= 16
+    arr = (OrcPop * n)()
+    keep = []
+    out = []
+    for k, d in enumerate(pops):
+        pos = np.array(d["pos"], dtype=np.float64)
+        pos1 = pos.copy()
+        T = np.array(d["tangent"], dtype=np.float64)
+        F = np.zeros_like(pos)
+        cid = np.ascontiguousarray(d["chain_id"], dtype=np.int64)
+        rl = np.ascontiguousarray(d["rest_lengths"], dtype=np.float64)
+        vt = np.ascontiguousarray(d["vtan"], dtype=np.float64)
+        xi = d.get("xi")
+        xi = None if xi is None else np.ascontiguousarray(xi, dtype=np.float64)
+        keep += [cid, rl, vt, xi]
+        arr[k].sys = make_sys(d["sysd"])
+        arr[k].fixed = int(bool(d.get("fixed", False)))
+        arr[k].pos, arr[k].pos1 = pos.ctypes.data, pos1.ctypes.data
+        arr[k].tangent, arr[k].forces = T.ctypes.data, F.ctypes.data
+        arr[k].chain_id, arr[k].rest_lengths = cid.ctypes.data, rl.ctypes.data
+        arr[k].vtan = vt.ctypes.data
+        arr[k].xi = xi.ctypes.data if xi is not None else None
+        arr[k].seed = int(d.get("seed", 0)) & ((1 << 64) - 1)
+        out.append((pos, pos1, T, F))
+    cr = np.zeros((n, n, 5))
+    for (a, b), p in cross.items():
+        cr[a, b, 0] = 1.0
+        cr[a, b, 1:] = p
+    st = np.zeros(4)
+    tv = np.ascontiguousarray(time_vals, dtype=np.float64)
+    lib().orc_multi_srk2_run(n, arr, cr.ctypes.data, tv.ctypes.data, st.ctypes.data, int(step0))
+    return out, st
 
 
 def rod_project(prev, F, xi, chain_id, rest_lengths, factor, L, dt, D, radius):

@@ -571,6 +571,94 @@ def test_srk2_walls_use_the_step_start_position():
     assert pos[0, 0] == pytest.approx(x, rel=1e-13) and pos[0, 1] == 4.0
 
 
+def test_multi_population_equals_the_union():
+    """Two populations with the same physics whose mutual law is their own
+    WCA law move exactly as the one population holding both (to rounding):
+    cross forces, the shared dt and the per-population updates agree with
+    the single-population loop."""
+    rng = np.random.default_rng(60)
+    L = 7.0
+    pos, tang, chain = straight_chains(rng, 6, 5, L)
+    sysd = dict(dim=2, n=30, n_nearby=4, has_springs=True, has_wca=True, has_propel=True,
+                L=L, D=0.02, Dr=0.0, radius=1.0,
+                wca=O.wca_params(1.0, 1.0, 0.0, 2.5 / 2 ** (1 / 6)),  # LJ to 2.5 sigma
+                spring=np.array([100.0, 4.0, 0.0, 0.0, 1.0, 0.1]), v0=1.0)
+    rest, vt = np.full(6, 0.5), np.linspace(-0.3, 0.3, 6)
+    xi = rng.normal(size=(40, 30, 4))
+    tv = np.array([1e-4, 1e-3, 0.0125, 1.0, -1.0, 40, 40, 40])
+    whole = O.srk2_run(sysd, pos, tang, chain, rest, vt, tv, xi=xi)
+    parts = []
+    for lo, hi in ((0, 3), (3, 6)):
+        sl = slice(5 * lo, 5 * hi)
+        parts.append(dict(sysd=dict(sysd, n=5 * (hi - lo)), pos=pos[sl], tangent=tang[sl],
+                          chain_id=chain[sl] - lo, rest_lengths=rest[lo:hi], vtan=vt[lo:hi],
+                          xi=np.ascontiguousarray(xi[:, sl])))
+    out, st = O.multi_srk2_run(parts, {(0, 1): sysd["wca"], (1, 0): sysd["wca"]}, tv)
+    got = np.vstack([o[0] for o in out])
+    assert np.max(np.abs(N.min_image(got - whole[0], L))) < 1e-11
+    assert st[0] == pytest.approx(whole[4][0], rel=1e-12)
+    assert st[1] == pytest.approx(whole[4][1], rel=1e-12)
+    # without the cross law the populations do not feel each other
+    alone, _ = O.multi_srk2_run(parts, {}, tv)
+    assert np.max(np.abs(N.min_image(alone[0][0] - got[:15], L))) > 1e-4
+
+
+def test_multi_population_fixed_obstacle():
+    """A bead pushed off a fixed bead (T = 0) by the averaged law of
+    populations.py:226-232 follows the scalar Heun map; the fixed bead stays."""
+    r_a, r_b = 1.0, 0.6
+    wa, wb = O.wca_params(r_a, 1.0, 0.0, 1.0), O.wca_params(r_b, 1.0, 0.0, 1.0)
+    w_ab = 0.5 * (wa + wb)
+    mover = dict(sysd=dict(dim=2, n=1, n_nearby=4, has_springs=False, has_wca=True,
+                           has_propel=False, L=L2D, D=0.0, Dr=0.0, radius=r_a, wca=wa,
+                           spring=np.zeros(6), v0=0.0),
+                 pos=np.array([[10.7, 5.0]]), tangent=np.array([[1.0, 0.0]]),
+                 chain_id=np.zeros(1, dtype=np.int64), rest_lengths=np.ones(1),
+                 vtan=np.zeros(1), xi=np.zeros((20, 1, 4)))
+    wall = dict(sysd=dict(mover["sysd"], radius=r_b, wca=wb), pos=np.array([[10.0, 5.0]]),
+                tangent=np.array([[1.0, 0.0]]), chain_id=np.zeros(1, dtype=np.int64),
+                rest_lengths=np.ones(1), vtan=np.zeros(1), fixed=True)
+    dtmax, mdps, n = 1e-3, 2e-3, 20
+    tv = np.array([1e-6, dtmax, mdps, 1.0, -1.0, 40, 40, n])
+    out, st = O.multi_srk2_run([mover, wall], {(0, 1): w_ab, (1, 0): w_ab}, tv)
+
+    def f(s):  # force on the mover at separation s (the obstacle at -s)
+        ds = s * s
+        return 0.0 if ds > w_ab[3] ** 2 else -s * ds ** -4 * (w_ab[0] - w_ab[1] * ds ** -3)
+
+    s, dt = 0.7, 1e-6
+    for _ in range(n):
+        f0 = f(s)
+        s1 = s + dt * f0 / r_a
+        f1 = f0 + f(s1)
+        dt = min(dtmax, mdps / (0.5 * abs(f1) / r_a))
+        s = s + dt * 0.5 * f1 / r_a
+    assert out[0][0][0, 0] - 10.0 == pytest.approx(s, rel=1e-12)
+    assert np.array_equal(out[1][0], wall["pos"])  # the fixed population never moves
+    assert st[0] == pytest.approx(dt, rel=1e-12)
+
+
+def test_multi_population_diffusion_per_population():
+    """F = 0: each population diffuses with its own D; one dt for all from the
+    largest sqrt(2 D)."""
+    rng = np.random.default_rng(61)
+    pops, D = [], (0.05, 0.4)
+    for k in range(2):
+        pops.append(dict(sysd=abp_sys(4, 10.0, D=D[k], v0=0.0), pos=rng.uniform(0, 10, (4, 2)),
+                         tangent=np.tile([1.0, 0.0], (4, 1)), chain_id=np.arange(4),
+                         rest_lengths=np.ones(4), vtan=np.zeros(4),
+                         xi=rng.normal(size=(6, 4, 4))))
+    dtmax, mdps = 1e-2, 5e-3
+    out, st = O.multi_srk2_run(pops, {}, np.array([1e-6, dtmax, mdps, 1.0, -1.0, 40, 40, 6]))
+    dt = min(dtmax, mdps / np.sqrt(2 * max(D)))
+    for k in range(2):
+        x = pops[k]["pos"].copy()
+        for j in range(6):
+            x = np.mod(x + np.sqrt(2 * dt * D[k]) * pops[k]["xi"][j, :, :2], 10.0)
+        assert np.allclose(out[k][0], x, atol=1e-13)
+    assert st[0] == dt
+
+
 def test_rotational_diffusion_closed_form():
     """SRK.py:398-423: t' = normalize(t + pi sqrt(2 Dr dt) (xi_r - (xi_r.t) t))."""
     Dr = 0.2
