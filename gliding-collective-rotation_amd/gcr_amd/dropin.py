@@ -11,10 +11,12 @@ the step loop running on the GPU through libgcr.so:
     updates.SRK2_force_calc = make_SRK2_force_calc(config)
 
 Device state stays resident between calls; positions are uploaded again only
-when the host changed them. Scope: one moving population of chains (ABPs are
-chains of one bead) with the "springs", "WCA", "self-propulsion" and "walls"
-forces, SRK_order 2 or 4. Anything else raises NotImplementedError naming the
-feature.
+when the host changed them. Scope: populations of chains (ABPs are chains of
+one bead) with the "springs", "WCA", "self-propulsion" and "walls" forces,
+SRK_order 2 or 4 for one population; several populations (moving or fixed,
+interacting through WCA laws: interactions() or special_interactions, as
+interpop_forces gives them, updates.py:274-312) with SRK_order 2 and no rigid
+rod. Anything else raises NotImplementedError naming the feature.
 """
 import numpy as np
 
@@ -23,12 +25,16 @@ from . import engine as E
 SUPPORTED = ("springs", "WCA", "self-propulsion", "walls")
 
 
-def _force_list(pop, repulsive_only):
-    """updates.interpop_forces (updates.py:274-312) for a population with itself."""
-    if pop.name in getattr(pop, "special_interactions", {}):
-        flist = pop.special_interactions[pop.name]
-    else:
+def _force_list(pop, repulsive_only, other=None):
+    """updates.interpop_forces (updates.py:274-312) for a population with
+    itself, or with `other`."""
+    other = pop if other is None else other
+    if other.name in getattr(pop, "special_interactions", {}):
+        flist = pop.special_interactions[other.name]
+    elif other is pop:
         flist = pop.self_interactions()
+    else:
+        flist = pop.interactions(other)
     out = {}
     for name, params in flist:
         params = np.array(params, dtype=np.float64)
@@ -55,75 +61,143 @@ class _Resident:
         self.tangent = None
 
 
+def _time_vals(config, status, sim):
+    """time_vals exactly as updates.py:138-146."""
+    if sim["save_state_every"] > 0:
+        max_dt_usable = status.next_save_time - status.t
+    else:
+        max_dt_usable = -1
+    n_upd = config.n_steps_update_grid_list
+    return np.array([status.dt, status.dtmax, config.max_dist_per_step, config.RK_cheat_factor,
+                     max_dt_usable, n_upd - status.update_grid_list_counter, n_upd,
+                     status.next_output_step - status.stepnum], dtype=np.float64)
+
+
+def _write_status(config, status, st):
+    """Status write-back, updates.py:249-256."""
+    n_upd = config.n_steps_update_grid_list
+    status.dt, status.dt_used, steps_done, status.max_forces_strength = st[:4]
+    status.stepnum += int(steps_done)
+    status.t += status.dt_used
+    status.update_grid_list_counter = max(
+        1, (int(steps_done) + status.update_grid_list_counter) % n_upd)
+
+
+def _population(pop, config, status, order, fixed=False):
+    """Physics, per-chain chirality, rest lengths, chain ids and the
+    signature of one population, from its force list (SRK.py:46-97)."""
+    dim = int(config.dim)
+    L = float(config.Lx)
+    n = int(pop.total_pop)
+    chain_id = np.asarray(getattr(pop, "chain_id", np.arange(n)), dtype=np.int64)
+    n_chains = int(np.max(chain_id)) + 1 if n else 0
+    radius = float(getattr(pop, "repel_d_max", 1.0))
+    if fixed:
+        # a fixed population only pushes the others; its cells follow its own
+        # WCA cutoff when it has one
+        wca = np.asarray(getattr(pop, "WCA_params", [0.0, 0.0, 0.0, 1.0]), dtype=np.float64)[:4]
+        phys = E.Physics(dim=dim, n_beads=n, n_chains=n_chains, L=L, D=0.0, Dr=0.0,
+                         radius=radius, wca=wca, spring=np.zeros(6), v0=0.0, n_nearby=-1,
+                         has_springs=False, has_wca=True, has_propel=False,
+                         skin=max(0.3 * float(wca[3]), L / (512 if dim == 2 else 64) - wca[3]))
+        rest = np.ones(n_chains)
+        vt = np.zeros(n_chains)
+        sig = ("fixed", n, dim, L, tuple(wca), chain_id.tobytes())
+        return phys, vt, rest, chain_id, sig
+    if not hasattr(pop, "chain_id"):
+        raise NotImplementedError("GPU path: populations with chain_id (chains / ABPs) only")
+    repulsive_only = bool(getattr(pop, "b_init_growing", False))
+    forces = _force_list(pop, repulsive_only)
+    unknown = [f for f in forces if f not in SUPPORTED]
+    if unknown:
+        raise NotImplementedError(f"GPU path: forces {unknown} are not ported")
+    longest = int(getattr(pop, "longest_chain_length", 1))
+    has_springs = "springs" in forces and longest > 1
+    has_wca = "WCA" in forces
+    has_propel = "self-propulsion" in forces and bool(status.b_calc_active_forces)
+    spring = forces["springs"][:6] if has_springs else np.zeros(6)
+    spring = np.concatenate([spring, np.zeros(6 - len(spring))])
+    wca = forces["WCA"][:4] if has_wca else np.array([0.0, 0.0, 0.0, 1.0])
+    act = forces.get("self-propulsion", np.zeros(3))
+    v0, vtan = float(act[0]), float(act[2]) if len(act) > 2 else 0.0
+    vtan_chain = getattr(pop, "v_tan_angle_per_chain", None)
+    if vtan_chain is None:
+        vtan_chain = np.full(n_chains, vtan)
+    n_nearby = int(getattr(pop, "n_nearby_beads_dont_repel", 0))
+    rest = np.asarray(pop.rest_lengths, dtype=np.float64)
+    rod = float(getattr(pop, "rigid_rod_projection_factor", 0.0))
+    walls, wall_zone, wall_cutoff = 0, 0.0, 0.0
+    if "walls" in forces:
+        # wall_params = WCA params + config.walls (populations.py:222-223); the
+        # walls sit grid_spacing * grid_window / 2 inside the faces, with the
+        # grid window of SRK.py:99-103 (forces.py:417)
+        wp = forces["walls"]
+        walls = sum(1 << i for i in range(dim) if wp[4 + i] != 0)
+        spacing = L / int(config.n_grid)
+        nbor = float(forces["WCA"][3]) if "WCA" in forces else 0.0
+        window = max(1, int(np.ceil(float(config.grid_factor) * nbor / spacing)))
+        wall_zone = spacing * window / 2
+        wall_cutoff = float(wp[3])
+    # cells of at least L/512 (2D) or L/64 (3D) per side, so that the tiny
+    # cutoffs of the growth phase (updates.py:304-310) do not ask for a
+    # grid of millions of cells
+    rc = float(wca[3])
+    skin = max(0.3 * rc, L / (512 if dim == 2 else 64) - rc)
+    phys = E.Physics(dim=dim, n_beads=n, n_chains=n_chains, L=L, D=float(pop.D),
+                     Dr=float(max(pop.Dr, 0.0)), radius=radius, wca=wca,
+                     spring=spring, v0=v0, n_nearby=n_nearby, has_springs=has_springs,
+                     has_wca=has_wca, has_propel=has_propel, skin=skin, rod=rod,
+                     srk_order=order, walls=walls, wall_zone=wall_zone,
+                     wall_cutoff=wall_cutoff)
+    sig = (n, dim, L, phys.D, phys.Dr, phys.radius, n_nearby, tuple(wca), tuple(spring), v0,
+           has_springs, has_wca, has_propel, rod, order, walls, wall_zone, wall_cutoff,
+           rest.tobytes(), np.asarray(vtan_chain, dtype=np.float64).tobytes(),
+           chain_id.tobytes())
+    return phys, vtan_chain, rest, chain_id, sig
+
+
+def _cross_laws(pops, moving):
+    """interpop_forces(P, Q) for every moving P and every other Q: the WCA law
+    of Q acting on P (updates.py:274-312), or nothing."""
+    laws = {}
+    for a, P in enumerate(pops):
+        if not moving[a]:
+            continue
+        growing = bool(getattr(P, "b_init_growing", False))
+        for b, Q in enumerate(pops):
+            if b == a:
+                continue
+            fl = _force_list(P, growing or bool(getattr(Q, "b_init_growing", False)), Q)
+            unknown = [f for f in fl if f != "WCA"]
+            if unknown:
+                raise NotImplementedError(
+                    f"GPU path: forces {unknown} between populations are not ported")
+            if "WCA" in fl:
+                laws[(a, b)] = np.asarray(fl["WCA"][:4], dtype=np.float64)
+    return laws
+
+
+class _Group:
+    """Device pool holding several populations as one group."""
+
+    def __init__(self):
+        self.pool = None
+        self.rids = None
+        self.sig = None
+        self.generation = 0
+        self.pos = None
+        self.tangent = None
+
+
 def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
     """Return a SRK2_force_calc(pops, status, sim) bound to the reference's
     `config` module (its globals Lx, dim, SRK_order, max_dist_per_step,
-    RK_cheat_factor, n_steps_update_grid_list, walls are read each call)."""
+    RK_cheat_factor, n_steps_update_grid_list, walls, n_grid, grid_factor are
+    read each call)."""
     resident = {}
 
-    def SRK2_force_calc(pops, status, sim):
-        moving = [p for p in pops if not p.fixed]
-        if len(pops) != 1 or len(moving) != 1:
-            raise NotImplementedError("GPU path: exactly one (moving) population; "
-                                      "multi-population interactions are not ported")
-        pop = moving[0]
-        order = int(getattr(config, "SRK_order", 2))
-        if order not in (2, 4):
-            raise NotImplementedError(f"GPU path: SRK_order 2 or 4 (not {order})")
-        if not hasattr(pop, "chain_id"):
-            raise NotImplementedError("GPU path: populations with chain_id (chains / ABPs) only")
-        repulsive_only = bool(getattr(pop, "b_init_growing", False))
-        forces = _force_list(pop, repulsive_only)
-        unknown = [f for f in forces if f not in SUPPORTED]
-        if unknown:
-            raise NotImplementedError(f"GPU path: forces {unknown} are not ported")
-        longest = int(getattr(pop, "longest_chain_length", 1))
-        has_springs = "springs" in forces and longest > 1
-        has_wca = "WCA" in forces
-        has_propel = "self-propulsion" in forces and bool(status.b_calc_active_forces)
-        dim = int(config.dim)
-        L = float(config.Lx)
-        n = int(pop.total_pop)
-        n_chains = int(np.max(pop.chain_id)) + 1
-        spring = forces["springs"][:6] if has_springs else np.zeros(6)
-        spring = np.concatenate([spring, np.zeros(6 - len(spring))])
-        wca = forces["WCA"][:4] if has_wca else np.array([0.0, 0.0, 0.0, 1.0])
-        act = forces.get("self-propulsion", np.zeros(3))
-        v0, vtan = float(act[0]), float(act[2]) if len(act) > 2 else 0.0
-        vtan_chain = getattr(pop, "v_tan_angle_per_chain", None)
-        if vtan_chain is None:
-            vtan_chain = np.full(n_chains, vtan)
-        n_nearby = int(getattr(pop, "n_nearby_beads_dont_repel", 0))
-        rest = np.asarray(pop.rest_lengths, dtype=np.float64)
-        rod = float(getattr(pop, "rigid_rod_projection_factor", 0.0))
-        walls, wall_zone, wall_cutoff = 0, 0.0, 0.0
-        if "walls" in forces:
-            # wall_params = WCA params + config.walls (populations.py:222-223); the
-            # walls sit grid_spacing * grid_window / 2 inside the faces, with the
-            # grid window of SRK.py:99-103 (forces.py:417)
-            wp = forces["walls"]
-            walls = sum(1 << i for i in range(int(config.dim)) if wp[4 + i] != 0)
-            spacing = float(config.Lx) / int(config.n_grid)
-            nbor = float(forces["WCA"][3]) if "WCA" in forces else 0.0
-            window = max(1, int(np.ceil(float(config.grid_factor) * nbor / spacing)))
-            wall_zone = spacing * window / 2
-            wall_cutoff = float(wp[3])
-        # cells of at least L/512 (2D) or L/64 (3D) per side, so that the tiny
-        # cutoffs of the growth phase (updates.py:304-310) do not ask for a
-        # grid of millions of cells
-        rc = float(wca[3])
-        skin = max(0.3 * rc, L / (512 if dim == 2 else 64) - rc)
-        phys = E.Physics(dim=dim, n_beads=n, n_chains=n_chains, L=L, D=float(pop.D),
-                         Dr=float(max(pop.Dr, 0.0)), radius=float(pop.repel_d_max), wca=wca,
-                         spring=spring, v0=v0, n_nearby=n_nearby, has_springs=has_springs,
-                         has_wca=has_wca, has_propel=has_propel, skin=skin, rod=rod,
-                         srk_order=order, walls=walls, wall_zone=wall_zone,
-                         wall_cutoff=wall_cutoff)
-        sig = (n, dim, L, phys.D, phys.Dr, phys.radius, n_nearby, tuple(wca), tuple(spring), v0,
-               has_springs, has_wca, has_propel, rod, order, walls, wall_zone, wall_cutoff,
-               rest.tobytes(),
-               np.asarray(vtan_chain, dtype=np.float64).tobytes(),
-               np.asarray(pop.chain_id).tobytes())
+    def one_population(pop, status, sim, order):
+        phys, vtan_chain, rest, chain_id, sig = _population(pop, config, status, order)
         R = resident.setdefault(id(pop), _Resident())
         if R.sig != sig:
             if R.system is not None:
@@ -139,17 +213,7 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
                 R.system.upload(pos=pop.pos)
             if not np.array_equal(pop.tangent_list, R.tangent):
                 R.system.upload(tangent=pop.tangent_list)
-        # time_vals exactly as updates.py:138-146
-        if sim["save_state_every"] > 0:
-            max_dt_usable = status.next_save_time - status.t
-        else:
-            max_dt_usable = -1
-        n_upd = config.n_steps_update_grid_list
-        time_vals = np.array([status.dt, status.dtmax, config.max_dist_per_step,
-                              config.RK_cheat_factor, max_dt_usable,
-                              n_upd - status.update_grid_list_counter, n_upd,
-                              status.next_output_step - status.stepnum], dtype=np.float64)
-        st = R.system.run(time_vals)
+        st = R.system.run(_time_vals(config, status, sim))
         out = R.system.download()
         pop.pos[:] = out["pos"]
         pop.pos1[:] = out["pos1"]
@@ -158,12 +222,84 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         pop.where_active[:] = -1  # SRK.py:132
         R.pos = pop.pos.copy()
         R.tangent = pop.tangent_list.copy()
-        # status write-back, updates.py:249-256
-        status.dt, status.dt_used, steps_done, status.max_forces_strength = st[:4]
-        status.stepnum += int(steps_done)
-        status.t += status.dt_used
-        status.update_grid_list_counter = max(
-            1, (int(steps_done) + status.update_grid_list_counter) % n_upd)
+        _write_status(config, status, st)
+
+    def several_populations(pops, status, sim, order):
+        if order != 2:
+            raise NotImplementedError("GPU path: several populations with SRK_order 2 only")
+        moving = [not p.fixed and hasattr(p, "forces") for p in pops]
+        parts = [_population(p, config, status, order, fixed=not moving[k])
+                 for k, p in enumerate(pops)]
+        if any(ph.rod != 0.0 and moving[k] for k, (ph, *_rest) in enumerate(parts)):
+            raise NotImplementedError("GPU path: the rigid rod with several populations")
+        laws = _cross_laws(pops, moving)
+        sig = (tuple(pt[4] for pt in parts), tuple(moving),
+               tuple(sorted((k, tuple(v)) for k, v in laws.items())))
+        key = tuple(id(p) for p in pops)
+        G = resident.setdefault(key, _Group())
+        tangents = [getattr(p, "tangent_list", None) for p in pops]
+        if G.sig != sig:
+            if G.pool is not None:
+                G.pool.close()
+            G.generation += 1
+            base = sim.get("seed", 0) if seed is None else seed
+            dim = int(config.dim)
+            nb = sum(int(p.total_pop) for p in pops)
+            nch = sum(pt[0].n_chains for pt in parts)
+            cells = sum(E.replica_cells(pt[0]) for pt in parts)
+            G.pool = E.Pool(dim, precision, bead_capacity=nb, cell_capacity=cells + len(pops),
+                            chain_capacity=nch, replica_capacity=len(pops))
+            G.rids = []
+            for k, (p, (phys, vt, rest, cid, _s)) in enumerate(zip(pops, parts)):
+                phys.seed = _mix(base, G.generation * 64 + k)
+                t = tangents[k] if tangents[k] is not None else np.tile(
+                    np.eye(dim)[0], (int(p.total_pop), 1))
+                G.rids.append(G.pool.add(phys, p.pos, t, cid, rest, vt))
+            G.pool.group(G.rids, laws, [k for k in range(len(pops)) if not moving[k]])
+            G.sig = sig
+        else:
+            for k, p in enumerate(pops):
+                if not np.array_equal(p.pos, G.pos[k]):
+                    G.pool.upload(G.rids[k], pos=p.pos)
+                if tangents[k] is not None and not np.array_equal(tangents[k], G.tangent[k]):
+                    G.pool.upload(G.rids[k], tangent=tangents[k])
+        tv = _time_vals(config, status, sim)
+        for k in range(len(pops)):
+            if moving[k]:
+                G.pool.begin(G.rids[k], tv)
+        G.pool.run()
+        st = None
+        for k, p in enumerate(pops):
+            if not moving[k]:
+                continue
+            out = G.pool.download(G.rids[k])
+            p.pos[:] = out["pos"]
+            p.pos1[:] = out["pos1"]
+            if tangents[k] is not None:
+                p.tangent_list[:] = out["tangent"]
+            p.forces[:] = out["forces"]
+            p.where_active[:] = -1  # SRK.py:132
+            if st is None:
+                s_ = G.pool.status(G.rids[k])
+                st = np.array([s_["dt"], s_["elapsed"], s_["steps"], s_["max_forces_strength"]])
+        G.pos = [p.pos.copy() for p in pops]
+        G.tangent = [None if t is None else t.copy() for t in tangents]
+        _write_status(config, status, st)
+
+    def SRK2_force_calc(pops, status, sim):
+        moving = [p for p in pops if not p.fixed]
+        if not moving:
+            raise NotImplementedError("GPU path: no moving population")
+        for p in moving:
+            if getattr(p, "sliding_partner_maxdist", 0.0) > 0 and getattr(p, "v0", 0.0) > 0:
+                raise NotImplementedError("GPU path: active_sliding_and_WCA is not ported")
+        order = int(getattr(config, "SRK_order", 2))
+        if order not in (2, 4):
+            raise NotImplementedError(f"GPU path: SRK_order 2 or 4 (not {order})")
+        if len(pops) == 1:
+            one_population(pops[0], status, sim, order)
+        else:
+            several_populations(pops, status, sim, order)
 
     SRK2_force_calc.resident = resident
     return SRK2_force_calc

@@ -269,6 +269,13 @@ class System:
         return dict(polar=out[0], nematic=out[1], director=out[2], n_chains=int(out[3]))
 
 
+def replica_cells(phys):
+    """Cells (plus the end offset) a replica of this physics takes in a pool
+    (gcr_replica_cells)."""
+    d = phys.desc()
+    return int(_lib.load().gcr_replica_cells(int(phys.dim), C.byref(d))) + 1
+
+
 def init_chains(dim, L, chain_len, bond, min_dist=0.9, seed=0, max_tries=20000,
                 min_dist_floor=None, return_min_dist=False):
     """Random straight chains without overlap (gcr_init_chains). With

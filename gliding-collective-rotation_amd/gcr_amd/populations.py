@@ -27,6 +27,16 @@ def make_WCA_params(repel_d_max, WCA_epsilon, WCA_shift, LJ_cutoff_factor):
     return np.array([fac, fac * s6, WCA_shift, LJ_cutoff_factor * repel_d_max - WCA_shift])
 
 
+def make_WCA_replacements(pop, kwargs):
+    """populations.py:34-45: the population's WCA vector with some of
+    repel_d_max, WCA_epsilon, WCA_shift, LJ_cutoff_factor replaced."""
+    vals = dict(repel_d_max=pop.repel_d_max, WCA_epsilon=pop.WCA_epsilon,
+                WCA_shift=pop.WCA_shift, LJ_cutoff_factor=pop.LJ_cutoff_factor)
+    vals.update({k: v for k, v in kwargs.items() if k in vals})
+    return make_WCA_params(vals["repel_d_max"], vals["WCA_epsilon"], vals["WCA_shift"],
+                           vals["LJ_cutoff_factor"])
+
+
 class Population:
     """Base population (populations.py:61-141)."""
 
@@ -53,6 +63,13 @@ class Population:
     def self_interactions(self):
         return []
 
+    def add_special_interaction(self, pop_name, force_name, **replacements):
+        """populations.py:120-125 (only "WCA" has rules, populations.py:23-31)."""
+        if force_name != "WCA":
+            raise ValueError("invalid force_name for a special interaction: " + force_name)
+        self.special_interactions.setdefault(pop_name, []).append(
+            [force_name, make_WCA_replacements(self, replacements)])
+
 
 class WCA_Population(Population):
     """Beads with a WCA/LJ pair force (populations.py:143-241)."""
@@ -74,6 +91,14 @@ class WCA_Population(Population):
     def adjust_WCA_params(self):
         self.WCA_params = make_WCA_params(self.repel_d_max, self.WCA_epsilon, self.WCA_shift,
                                           self.LJ_cutoff_factor)
+
+    def interactions(self, other_pop):
+        """populations.py:226-232: with another WCA population, the mean of the
+        two WCA vectors."""
+        flist = super().interactions(other_pop)
+        if isinstance(other_pop, WCA_Population):
+            flist = flist + [["WCA", 0.5 * (self.WCA_params + other_pop.WCA_params)]]
+        return flist
 
     def self_interactions(self):
         """populations.py:219-224: WCA, and walls when config.walls has any axis."""

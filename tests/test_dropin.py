@@ -7,6 +7,7 @@ import pytest
 import gcr_oracle_np as N
 import oracle_lib as O
 from gcr_amd import config, defaults, dropin, populations
+from gcr_amd.engine import init_chains
 
 
 def make_run(n_chains=6, chain_length=7, Lx=20.0, **over):
@@ -26,8 +27,11 @@ def make_run(n_chains=6, chain_length=7, Lx=20.0, **over):
 def test_dropin_rejects_unported_features():
     sim, pop, status = make_run()
     f = dropin.make_SRK2_force_calc(config)
-    with pytest.raises(NotImplementedError, match="one"):
-        f([pop, pop], status, sim)
+    moving_pop_missing = [pop]
+    pop.fixed = True
+    with pytest.raises(NotImplementedError, match="no moving"):
+        f(moving_pop_missing, status, sim)
+    pop.fixed = False
     pop.sliding_partner_maxdist = 1.0
     with pytest.raises(NotImplementedError, match="active_sliding"):
         f([pop], status, sim)
@@ -140,6 +144,71 @@ def test_dropin_walls_match_oracle():
         assert np.all((pop.pos > zone) & (pop.pos < 20.0 - zone))
     finally:
         config.configure(dict(sim, walls="none"))
+
+
+def two_populations():
+    """examples/active_and_passive_pops.py: self-propelled chains of 10
+    ("Kimberly") and passive chains of 2 ("Steve") sticking to each other
+    through an LJ law cut at 2 r (a special interaction both ways)."""
+    sim = defaults.set_defaults()
+    sim.update(dict(n_chains=12, chain_length=10, Lx=20.0, kappa=0.0, T=0.01, dtmax=1e-3,
+                    seed=5, t_init_grow=0.0, save_state_every=-1.0))
+    sim["dt0"] = 1e-4 * sim["dtmax"]
+    config.configure(sim)
+    kim = populations.Equal_Size_Chain_Collection(sim, chain_length=10, num_chains=12,
+                                                  name="Kimberly")
+    steve = populations.Equal_Size_Chain_Collection(sim, chain_length=2, num_chains=40,
+                                                    name="Steve")
+    steve.v0 = 0
+    kim.add_special_interaction("Steve", "WCA", LJ_cutoff_factor=2)
+    steve.special_interactions["Kimberly"] = kim.special_interactions["Steve"]
+    pos, tang = init_chains(2, 20.0, np.array([10] * 12 + [2] * 40), 0.5, seed=3)
+    for pop, sl in ((kim, slice(0, 120)), (steve, slice(120, 200))):
+        pop.pos[:] = pos[sl]
+        pop.pos1[:] = pos[sl]
+        pop.tangent_list[:] = tang[sl]
+    config.grid_list_setup(sim, [kim, steve])
+    status = config.status_class(sim, start_t=0.0)
+    return sim, [kim, steve], status
+
+
+def test_dropin_population_laws():
+    """interpop_forces as the reference forms it: the special law both ways,
+    the plain WCA law within each population, nothing else."""
+    sim, pops, status = two_populations()
+    laws = dropin._cross_laws(pops, [True, True])
+    want = populations.make_WCA_params(1.0, 1.0, 0.0, 2.0)
+    assert set(laws) == {(0, 1), (1, 0)}
+    assert np.allclose(laws[(0, 1)], want) and np.allclose(laws[(1, 0)], want)
+    plain = populations.make_WCA_params(1.0, 1.0, 0.0, 1.0)
+    assert np.allclose(pops[0].interactions(pops[1])[0][1], plain)  # averaged: equal sizes
+
+
+@pytest.mark.gpu
+def test_dropin_two_populations_match_oracle():
+    sim, pops, status = two_populations()
+    x0 = [p.pos.copy() for p in pops]
+    t0 = [p.tangent_list.copy() for p in pops]
+    f = dropin.make_SRK2_force_calc(config)
+    status.next_output_step = 50
+    f(pops, status, sim)
+    assert status.stepnum == 50
+    law = populations.make_WCA_params(1.0, 1.0, 0.0, 2.0)
+    orc = []
+    for k, p in enumerate(pops):
+        sysd = dict(dim=2, n=p.total_pop, n_nearby=p.n_nearby_beads_dont_repel,
+                    has_springs=True, has_wca=True, has_propel=True, L=20.0, D=p.D, Dr=p.Dr,
+                    radius=1.0, wca=p.WCA_params, spring=p.spring_params(), v0=float(p.v0))
+        orc.append(dict(sysd=sysd, pos=x0[k], tangent=t0[k], chain_id=p.chain_id,
+                        rest_lengths=p.rest_lengths, vtan=np.zeros(p.num_chains()),
+                        seed=dropin._mix(sim["seed"], 64 + k)))
+    tv = np.array([sim["dt0"], sim["dtmax"], config.max_dist_per_step, 1.0, -1,
+                   config.n_steps_update_grid_list, config.n_steps_update_grid_list, 50])
+    out, st = O.multi_srk2_run(orc, {(0, 1): law, (1, 0): law}, tv)
+    for k, p in enumerate(pops):
+        assert np.max(np.abs(N.min_image(p.pos - out[k][0], 20.0))) < 1e-10 * 20.0, k
+    assert status.dt == pytest.approx(st[0], rel=1e-10)
+    assert status.t == pytest.approx(st[1], rel=1e-10)
 
 
 @pytest.mark.gpu
