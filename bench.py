@@ -31,7 +31,7 @@ ALG_BYTES = {"stage0": 32, "stage1": 16, "update": 32}
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (tile edge 8, 256 threads per
 # tile, skin 0.3 cutoff, re-sort every 32 steps), each counter scaled by its
 # reading for a plain 1 GiB copy kernel (`--mode copy`: reads x 2.0, writes x 1.0)
-TRAFFIC_BYTES = {"stage0": 174.2, "stage1": 136.2, "update": 104.8}
+TRAFFIC_BYTES = {"stage0": 174.5, "stage1": 136.2, "update": 104.9}
 
 
 def parse():
