@@ -144,8 +144,10 @@ struct gcr_pool {
     std::vector<Group> groups;
     XPair* XPAIRS = nullptr;
     int32_t* GRP = nullptr;
+    int32_t *CHAIN_CHUNK = nullptr, *WA = nullptr, *WAn = nullptr;
     int64_t capX = 0;
-    bool has_group = false;
+    bool has_group = false;  // a group exists or a replica has active sliding: the
+                             // second half of each force stage runs (k_group_stage)
 };
 
 namespace {
@@ -252,6 +254,9 @@ void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
     if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
     if (d->srk_order == 4) fl |= GCR_F_SRK4;
     P.walls = d->walls & 7;  // the kernels read the bits of their dimensions
+    P.slide_d2 = d->sliding_maxdist * d->sliding_maxdist;
+    P.slide_v0 = d->sliding_v0;
+    if (d->sliding_maxdist > 0.0) fl |= GCR_F_SLIDE;
     P.wall_zone = d->wall_zone;
     P.wall_rc = d->wall_cutoff > 0.0 ? d->wall_cutoff : d->wca[3];
     if (P.walls) fl |= GCR_F_WALLS;
@@ -260,6 +265,7 @@ void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
 
 bool srk_order_ok(const gcr_replica_desc* d)
 {
+    if (d->srk_order == 4 && d->sliding_maxdist > 0.0) return false;  // not combined
     return d->srk_order == 0 || d->srk_order == 2 || d->srk_order == 4;
 }
 
@@ -321,7 +327,8 @@ int rebuild_tables(gcr_pool* p)
             cc.push_back(P.cell_off + c);
         }
         if (P.flags & GCR_F_SRK4) has_srk4 = true;
-        if (P.flags & (GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE | GCR_F_GROUP)) tile_gen = true;
+        if (P.flags & (GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE | GCR_F_GROUP | GCR_F_SLIDE))
+            tile_gen = true;
         if (P.flags & GCR_F_ROD)
             for (int c = 0; c < P.n_chains; c += BS) {
                 hr.push_back(r);
@@ -348,7 +355,7 @@ int rebuild_tables(gcr_pool* p)
             P.ntile = n;
             P.red_off = P.tile_off;
             P.red_n = P.ntile;
-            if (P.flags & GCR_F_GROUP) {  // maxima per bead block, k_group_stage
+            if (P.flags & (GCR_F_GROUP | GCR_F_SLIDE)) {  // maxima per bead block, k_group_stage
                 P.red_off = (int32_t)p->capTile + P.blk_off;
                 P.red_n = P.nblk;
             }
@@ -375,6 +382,8 @@ int rebuild_tables(gcr_pool* p)
     p->has_srk4 = has_srk4;
     p->tile_gen = tile_gen;
     p->has_group = !p->groups.empty();
+    for (int r = 0; r < p->capR; ++r)
+        if (p->live[r] && (p->P[r].flags & GCR_F_SLIDE)) p->has_group = true;
     if (!xp.empty())
         HIPCHK(hipMemcpyAsync(p->XPAIRS, xp.data(), xp.size() * sizeof(XPair), hipMemcpyHostToDevice,
                               p->stream));
@@ -746,10 +755,10 @@ int gcr_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_by
 int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_cells)
 {
     // per bead: 10 dim-vectors + noise (2 x 2 dim) of doubles, 5 int pairs, links,
-    // cell and slot ints; 2D adds the neighbour list, its length and the
-    // re-sort positions; per chain: 4 doubles + 2 ints; per cell: 2 ints
+    // cell, slot and two where_active ints; the neighbour list, its length and
+    // the re-sort positions; per chain: 4 doubles + 3 ints; per cell: 2 ints
     const int64_t nl = 2 * NL_MAX + 4 + 8 * dim;
-    return n * (8 * (10 * dim + 4 * dim) + 4 * (5 * 2 + 4 + 2) + nl) + n_chains * (8 * 4 + 8) +
+    return n * (8 * (10 * dim + 4 * dim) + 4 * (5 * 2 + 4 + 2 + 2) + nl) + n_chains * (8 * 4 + 12) +
            (n_cells + 1) * 8;
 }
 
@@ -827,6 +836,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->capX = (int64_t)capR * GCR_MAX_POPS;
     A(dalloc(&p->XPAIRS, p->capX));
     A(dalloc(&p->GRP, capR));
+    A(dalloc(&p->CHAIN_CHUNK, capCh));
+    A(dalloc(&p->WA, nb));
+    A(dalloc(&p->WAn, nb));
     A(dalloc(&p->TILES, p->capTile * 4));
     A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
     // neighbour lists: 2D u16 LDS indices (NL_MAX), 3D int32 slots (NL3_MAX),
@@ -897,6 +909,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.Ss = p->dSs;
     a.XPAIRS = p->XPAIRS;
     a.GRP = p->GRP;
+    a.CHAIN_CHUNK = p->CHAIN_CHUNK;
+    a.WA = p->WA;
+    a.WAn = p->WAn;
     a.red_base = dim == 2 ? (int32_t)p->capTile : 0;
     *out = p;
     return GCR_OK;
@@ -926,7 +941,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     if (d->has_wca && !(d->wca[3] > 0 && 2 * d->wca[3] < d->L))
         return fail(GCR_ERR_ARG, "WCA cutoff must be in (0, L/2)");
     if (d->D < 0 || d->Dr < 0) return fail(GCR_ERR_ARG, "D and Dr must be >= 0 (resolve defaults first)");
-    if (!srk_order_ok(d)) return fail(GCR_ERR_ARG, "srk_order must be 2 or 4");
+    if (!srk_order_ok(d)) return fail(GCR_ERR_ARG, "srk_order must be 2 or 4 (2 with active sliding)");
     // chains: runs of equal chain_id with ids 0..nch-1 in order of first appearance
     std::vector<int32_t> first(nch, -1), len(nch, 0);
     int64_t prev = -1;
@@ -1021,6 +1036,8 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     HIPCHK(hipMemcpyAsync(p->CHAIN_FIRST + h0, cf.data(), 4 * nch, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(p->CHAIN_LEN + h0, cl.data(), 4 * nch, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(p->FT + b0 * dim, 0, sizeof(double) * n * dim, st));
+    HIPCHK(hipMemsetAsync(p->CHAIN_CHUNK + h0, 0, 4 * nch, st));
+    k_fill_i32<<<(int)((n + BS - 1) / BS), BS, 0, st>>>(p->WA + b0, (int)n, -1);
     RepState& S = p->S[r];
     S = RepState{};
     S.max_steps = 0;  // not armed
@@ -1141,7 +1158,7 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     int rc = check_rid(p, rid);
     if (rc) return rc;
     if (!d) return fail(GCR_ERR_ARG, "null descriptor");
-    if (!srk_order_ok(d)) return fail(GCR_ERR_ARG, "srk_order must be 2 or 4");
+    if (!srk_order_ok(d)) return fail(GCR_ERR_ARG, "srk_order must be 2 or 4 (2 with active sliding)");
     const int grp = group_of(p, rid);
     if (grp >= 0 && (d->srk_order == 4 || (d->rod != 0.0)))
         return fail(GCR_ERR_ARG, "groups run SRK_order 2 without the rigid rod");
@@ -1173,9 +1190,36 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     // rod, pair and SRK4 flags change the launch tables or the stages of a
     // step; lists are rebuilt at the start of every batch, so a new exclusion
     // rule takes effect there
-    if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA | GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE))
+    if ((old ^ P.flags) & (GCR_F_ROD | GCR_F_WCA | GCR_F_SRK4 | GCR_F_WALLS | GCR_F_FENE | GCR_F_SLIDE))
         p->tables_dirty = true;
     if (grp >= 0) p->tables_dirty = true;  // the scan windows follow the skin
+    return GCR_OK;
+}
+
+int gcr_replica_set_chunks(gcr_pool* p, int32_t rid, const int32_t* chain_chunk)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    const RepParams& P = p->P[rid];
+    if (chain_chunk)
+        HIPCHK(hipMemcpyAsync(p->CHAIN_CHUNK + P.chain_off, chain_chunk, 4 * (size_t)P.n_chains,
+                              hipMemcpyHostToDevice, p->stream));
+    else
+        HIPCHK(hipMemsetAsync(p->CHAIN_CHUNK + P.chain_off, 0, 4 * (size_t)P.n_chains, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return GCR_OK;
+}
+
+int gcr_replica_where_active(gcr_pool* p, int32_t rid, int32_t* out)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (!out) return fail(GCR_ERR_ARG, "null output");
+    const RepParams& P = p->P[rid];
+    int32_t* tmp = reinterpret_cast<int32_t*>(p->Xn);  // staging, free between calls
+    k_unpermute_i32<<<(P.n + BS - 1) / BS, BS, 0, p->stream>>>(p->WA, tmp, p->SLOT, P.bead_off, P.n);
+    HIPCHK(hipMemcpyAsync(out, tmp, 4 * (size_t)P.n, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
     return GCR_OK;
 }
 
@@ -1448,7 +1492,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
                     p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR, p->OBS,
-                    p->XPAIRS, p->GRP};
+                    p->XPAIRS, p->GRP, p->CHAIN_CHUNK, p->WA, p->WAn};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

@@ -24,6 +24,7 @@
 #define GCR_F_WALLS 1024u   // WCA walls (forces.py:405-435)
 #define GCR_F_GROUP 2048u   // a moving population of a group: other populations act on it
                             // (updates.py:132-267, SRK.py:287-293)
+#define GCR_F_SLIDE 4096u   // active sliding (forces.py:79-83, 114-120)
 
 // The law of one population acting on another (interpop_forces,
 // updates.py:274-312): a WCA/LJ pair law and the source replica, with the
@@ -67,6 +68,7 @@ struct RepParams {
     int32_t pad1;
     double wall_zone;         // distance of the walls from the box faces
     double wall_rc;           // the walls' range parameter (wall_params[3])
+    double slide_d2, slide_v0; // active sliding: squared partner range, push per partner
     int32_t walls;            // bit i: walls normal to axis i
     int32_t pad0;
     double D, Dr, radius;     // pop_consts[0..2]
@@ -166,6 +168,10 @@ struct PoolArgs {
     // on_x1 - that mid-step re-sort: cells, lists and XR from X1, and the
     //         live force array (SRK2's stage-0 forces, SRK4's running
     //         total) travels with the beads
+    // active sliding: thread chunk per chain, where_active per slot (+ staging)
+    const int32_t* CHAIN_CHUNK;
+    int32_t* WA;
+    int32_t* WAn;
     // populations acting on each other
     const XPair* XPAIRS;
     const int32_t* GRP;

@@ -956,9 +956,10 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
                                               const double* acc)
 {
     const bool srk4 = GEN && (P.flags & GCR_F_SRK4) != 0;
-    if (GEN && STAGE != 2 && (P.flags & GCR_F_GROUP)) {
-        // a group member's own forces; the other populations' forces, the
-        // virtual update and the max follow in k_group_stage
+    if (GEN && STAGE != 2 && (P.flags & (GCR_F_GROUP | GCR_F_SLIDE))) {
+        // a group member's or a sliding replica's own forces; the sliding
+        // pushes, the other populations' forces, the virtual update and the
+        // max follow in k_group_stage, once every bead's tangent is known
         store_vec<DIM>(STAGE == 0 ? a.F0 : a.FT, s, acc);
         return 0.0;
     }
@@ -1053,6 +1054,10 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
             double x0[DIM];
             load_vec<DIM>(a.X, s, x0);
             add_walls<DIM>(P, x0, acc);
+        }
+        if (P.flags & GCR_F_SLIDE) {  // tangents at this stage, for k_group_stage
+            if (!tang_set) load_vec<DIM>(a.T, s, tang);
+            store_vec<DIM>(a.Tn, s, tang);
         }
         fsq = stage_store<DIM, STAGE>(a, P, S, s, me.x, xs, acc);
     }
@@ -1155,8 +1160,66 @@ __device__ __forceinline__ void cross_sum(const PoolArgs& a, const RepParams& Q,
     }
 }
 
-// Second half of a force stage for the moving populations of a group, one
-// thread per bead: the forces of the other populations, in the order
+// Active sliding partners of the bead in slot s (forces.py:79-83, 114-120):
+// beads of its own replica closer than the sliding range (strictly) whose
+// tangent points against the bead's, the same-chain rule applying, and a
+// partner in the bead's own thread chunk only when its index is lower (the
+// reference skips b2 >= b1 within a chunk, a pair whose pair force it
+// applies to both beads but whose push it gives only to the later one).
+// Found by scanning the replica's cells within the range plus the half skin.
+template <int DIM>
+__device__ __forceinline__ int slide_partners(const PoolArgs& a, const RepParams& P, const double* XA,
+                                              int s, const double* x, const double* t1)
+{
+    const int nc = P.nc;
+    const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+    const int ch1 = a.CHAIN_CHUNK[me.y];
+    const int nn = P.n_nearby;
+    const double wd = ceil((sqrt(P.slide_d2) + fmin(P.skin_half, P.L)) * P.inv_h);
+    const int w = wd >= nc ? nc : max(1, (int)wd);
+    const bool all = 2 * w + 1 >= nc;
+    const int lo = all ? 0 : -w, hi = all ? nc - 1 : w;
+    int c[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) c[i] = cell_coord(x[i], P.inv_h, nc);
+    int cnt = 0;
+    for (int dz = (DIM == 3 ? lo : 0); dz <= (DIM == 3 ? hi : 0); ++dz) {
+        const int z = DIM == 3 ? (all ? dz : (c[2] + dz + nc) % nc) : 0;
+        for (int dy = lo; dy <= hi; ++dy) {
+            const int y = all ? dy : (c[1] + dy + nc) % nc;
+            for (int dx = lo; dx <= hi; ++dx) {
+                const int xc = all ? dx : (c[0] + dx + nc) % nc;
+                const int cell = P.cell_off + (z * nc + y) * nc + xc;
+                const int t1e = a.CSTART[cell + 1];
+                for (int t = a.CSTART[cell]; t < t1e; ++t) {
+                    const int2 m = *reinterpret_cast<const int2*>(a.META + 2 * t);
+                    const int d = m.x - me.x;
+                    if (m.y == me.y && (nn < 0 || (d < 0 ? -d : d) <= nn)) continue;
+                    if (a.CHAIN_CHUNK[m.y] == ch1 && m.x >= me.x) continue;
+                    double q[DIM], t2[DIM], ds = 0.0, dot = 0.0;
+                    load_vec<DIM>(XA, t, q);
+                    load_vec<DIM>(a.Tn, t, t2);
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) {
+                        double sep = q[i] - x[i];
+                        if (sep < -P.halfL)
+                            sep += P.L;
+                        else if (sep >= P.halfL)
+                            sep -= P.L;
+                        ds += sep * sep;
+                        dot += t1[i] * t2[i];
+                    }
+                    if (ds < P.slide_d2 && dot < 0.0) ++cnt;
+                }
+            }
+        }
+    }
+    return cnt;
+}
+
+// Second half of a force stage for the moving populations of a group and for
+// replicas with active sliding, one thread per bead: the sliding pushes, then
+// the forces of the other populations, in the order
 // (Q - P) mod n (SRK.py:287-293), at their positions of this stage (x1 in
 // stage 1 unless fixed), added to the bead's own forces; then STAGE 0 the
 // predictor, STAGE 1 the block's max |F|^2 for k_finalize.
@@ -1166,13 +1229,23 @@ __global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
     const int r = a.blk_rep[blockIdx.x];
     const RepParams& P = a.P[r];
     RepState* S = a.S + r;
-    if (!(P.flags & GCR_F_GROUP) || !step_runs(S, j)) return;
+    if (!(P.flags & (GCR_F_GROUP | GCR_F_SLIDE)) || !step_runs(S, j)) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     double fsq = 0.0;
     if (s < P.bead_off + P.n) {
+        const double* XA = STAGE == 0 ? a.X : a.X1;
         double x[DIM], f[DIM];
-        load_vec<DIM>(STAGE == 0 ? a.X : a.X1, s, x);
+        load_vec<DIM>(XA, s, x);
         load_vec<DIM>(STAGE == 0 ? a.F0 : a.FT, s, f);
+        if (P.flags & GCR_F_SLIDE) {
+            double t1[DIM];
+            load_vec<DIM>(a.Tn, s, t1);
+            const int cnt = slide_partners<DIM>(a, P, XA, s, x, t1);
+            for (int k = 0; k < cnt; ++k)
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) f[i] += P.slide_v0 * t1[i];
+            a.WA[s] = (STAGE == 0 ? -1 : a.WA[s]) + cnt;  // SRK.py:132, forces.py:119
+        }
         for (int k = P.xin_off; k < P.xin_off + P.xin_n; ++k) {
             const XPair e = a.XPAIRS[k];
             const double* XQ = (STAGE == 0 || a.S[e.src].fixed) ? a.X : a.X1;
@@ -1501,6 +1574,7 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
         const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
         load_vec<DIM>(srk4 ? a.FT : a.F0, so, x);
         store_vec<DIM>(srk4 ? a.F0 : a.FT, d, x);
+        if (P.flags & GCR_F_SLIDE) a.WAn[d] = a.WA[so];  // stage-0 partner counts
     }
 }
 
@@ -1544,6 +1618,7 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
         const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
         load_vec<DIM>(srk4 ? a.F0 : a.FT, ns, x);
         store_vec<DIM>(srk4 ? a.FT : a.F0, ns, x);
+        if (P.flags & GCR_F_SLIDE) a.WA[ns] = a.WAn[ns];
     }
 }
 
@@ -1610,6 +1685,14 @@ __global__ void k_restore_state(PoolArgs a)
 }
 
 // ---------------------------------------------------------------- download
+
+// Gather one replica's per-slot int32 array into original bead order.
+__global__ void __launch_bounds__(BS) k_unpermute_i32(const int32_t* src, int32_t* dst, const int32_t* SLOT,
+                                                      int orig0, int n)
+{
+    const int b = blockIdx.x * BS + threadIdx.x;
+    if (b < n) dst[b] = src[SLOT[orig0 + b]];
+}
 
 // Gather one replica's per-slot array into original bead order.
 __global__ void __launch_bounds__(BS) k_unpermute(const double* src, double* dst, const int32_t* SLOT,

@@ -420,6 +420,10 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
                 load_vec<2>(a.X, s, x0);
                 add_walls<2>(P, x0, acc);
             }
+            if (GEN && (P.flags & GCR_F_SLIDE)) {  // tangents at this stage, for k_group_stage
+                if (!tang_set) load_vec<2>(a.T, s, tang);
+                store_vec<2>(a.Tn, s, tang);
+            }
             const double q = stage_store<2, STAGE, GEN>(a, P, S, s, me.x, xs, acc);
             fsq = q > fsq ? q : fsq;
         }

@@ -43,6 +43,8 @@ class ReplicaDesc(C.Structure):
         ("walls", C.c_int32),
         ("wall_zone", C.c_double),
         ("wall_cutoff", C.c_double),
+        ("sliding_maxdist", C.c_double),
+        ("sliding_v0", C.c_double),
     ]
 
 
@@ -53,7 +55,8 @@ EXPORTS = [
     "gcr_replica_upload", "gcr_replica_begin", "gcr_pool_run", "gcr_replica_status",
     "gcr_replica_download", "gcr_replica_eval_forces", "gcr_pool_time_kernels",
     "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_threads", "gcr_init_chains",
-    "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group",
+    "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group", "gcr_replica_set_chunks",
+    "gcr_replica_where_active",
 ]
 
 _lib = None
@@ -98,6 +101,8 @@ def load():
     L.gcr_replica_order.argtypes = [vp, i32, dp]
     L.gcr_replica_set_physics.argtypes = [vp, i32, C.POINTER(ReplicaDesc), vp, vp]
     L.gcr_pool_group.argtypes = [vp, vp, i32, vp, vp]
+    L.gcr_replica_set_chunks.argtypes = [vp, i32, vp]
+    L.gcr_replica_where_active.argtypes = [vp, i32, vp]
     L.gcr_create.argtypes = [C.POINTER(ReplicaDesc), i32, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]
     L.gcr_upload.argtypes = [vp, vp, vp]
     L.gcr_run.argtypes = [vp, vp, vp, vp, i64]

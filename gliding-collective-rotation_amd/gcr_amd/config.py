@@ -15,6 +15,7 @@ max_dist_per_step = 1
 grid_factor = 2.0
 walls = np.array([False, False, False])
 n_grid = 1
+n_threads = 1
 pop_name_list = []
 
 
@@ -38,7 +39,7 @@ class status_class:
 
 def configure(sim):
     """config.py:75-119 (derived counts, globals, seed)."""
-    global dim, SRK_order, Lx, walls
+    global dim, SRK_order, Lx, walls, n_threads
     sim["chain_density"] = sim["bead_density"] / (
         sim["repel_d_max"] * (sim["a"] * (sim["chain_length"] - 1) + sim["repel_d_max"]))
     if sim["n_chains"] < 0:
@@ -48,6 +49,7 @@ def configure(sim):
     SRK_order = sim["SRK_order"]
     Lx = sim["Lx"]
     walls = np.array([(s in sim["walls"] or sim["walls"] in ["all", "True"]) for s in "xyz"])
+    n_threads = sim["n_threads"]  # cline.py:72
     np.random.seed(sim["seed"] % (2 ** 32))
 
 

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import engine as E
 
-SUPPORTED = ("springs", "WCA", "self-propulsion", "walls")
+SUPPORTED = ("springs", "WCA", "self-propulsion", "walls", "active_sliding_and_WCA")
 
 
 def _force_list(pop, repulsive_only, other=None):
@@ -38,10 +38,47 @@ def _force_list(pop, repulsive_only, other=None):
     out = {}
     for name, params in flist:
         params = np.array(params, dtype=np.float64)
-        if name == "WCA" and repulsive_only and params[0] != 0:
+        if name in ("WCA", "active_sliding_and_WCA") and repulsive_only and params[0] != 0:
             params[3] = (params[1] / params[0]) ** (1 / 6)  # updates.py:304-310
         out[name] = params
     return out
+
+
+def chain_chunks(n_threads, chain_id):
+    """The thread chunk of every chain when updates.chunk_chain
+    (updates.py:323-344) splits a population among n_threads: a cut about
+    every ceil(n / n_threads) beads, moved on to the next chain boundary."""
+    chain_id = np.asarray(chain_id)
+    n = len(chain_id)
+    chunklen = int(np.ceil(n / n_threads))
+    splits, first, sii = [], 0, 0
+    while first < n and sii < n_threads - 1:
+        if first > 0:
+            splits.append(first)
+            sii += 1
+        test = min(max(first, (sii + 1) * chunklen - 1), n - 1)
+        last = chain_id[test]
+        while test < n and chain_id[test] == last:
+            test += 1
+        first = test
+    bead_chunk = np.zeros(n, dtype=np.int32)
+    for s_ in splits:
+        bead_chunk[s_:] += 1
+    n_chains = int(chain_id.max()) + 1 if n else 0
+    out = np.zeros(n_chains, dtype=np.int32)
+    out[chain_id[::-1]] = bead_chunk[::-1]  # the chunk of each chain's first bead
+    return out
+
+
+def _thread_counts(config, sim, n_pairs):
+    """updates.py:172-182: threads of the first population pair and of the others."""
+    n = int(getattr(config, "n_threads", 1))
+    if n < n_pairs:
+        n = n_pairs
+    n = min(n, max(n_pairs, int(sim.get("n_threads", 1))))
+    config.n_threads = n
+    per = max(1, n // n_pairs)
+    return n - (n_pairs - 1) * per, per
 
 
 def _mix(seed, k):
@@ -120,6 +157,19 @@ def _population(pop, config, status, order, fixed=False):
     wca = forces["WCA"][:4] if has_wca else np.array([0.0, 0.0, 0.0, 1.0])
     act = forces.get("self-propulsion", np.zeros(3))
     v0, vtan = float(act[0]), float(act[2]) if len(act) > 2 else 0.0
+    sliding_maxdist = sliding_v0 = 0.0
+    if "active_sliding_and_WCA" in forces:
+        # WCA params + active_params (populations.py:580-591): the pair law,
+        # and with active forces on, the push v0 per antiparallel partner
+        # closer than sliding_partner_maxdist (SRK.py:73-80)
+        sp = forces["active_sliding_and_WCA"]
+        wca = np.asarray(sp[:4], dtype=np.float64)
+        has_wca = True
+        vtan = float(sp[6]) if len(sp) > 6 else 0.0
+        if bool(status.b_calc_active_forces):
+            sliding_v0, sliding_maxdist = float(sp[4]), float(sp[5])
+            if order == 4:
+                raise NotImplementedError("GPU path: active sliding with SRK_order 4")
     vtan_chain = getattr(pop, "v_tan_angle_per_chain", None)
     if vtan_chain is None:
         vtan_chain = np.full(n_chains, vtan)
@@ -148,9 +198,11 @@ def _population(pop, config, status, order, fixed=False):
                      spring=spring, v0=v0, n_nearby=n_nearby, has_springs=has_springs,
                      has_wca=has_wca, has_propel=has_propel, skin=skin, rod=rod,
                      srk_order=order, walls=walls, wall_zone=wall_zone,
-                     wall_cutoff=wall_cutoff)
+                     wall_cutoff=wall_cutoff, sliding_maxdist=sliding_maxdist,
+                     sliding_v0=sliding_v0)
     sig = (n, dim, L, phys.D, phys.Dr, phys.radius, n_nearby, tuple(wca), tuple(spring), v0,
            has_springs, has_wca, has_propel, rod, order, walls, wall_zone, wall_cutoff,
+           sliding_maxdist, sliding_v0,
            rest.tobytes(), np.asarray(vtan_chain, dtype=np.float64).tobytes(),
            chain_id.tobytes())
     return phys, vtan_chain, rest, chain_id, sig
@@ -208,6 +260,9 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
             R.system = E.System(phys, pop.pos, pop.tangent_list, pop.chain_id, rest,
                                 vtan_chain, precision=precision)
             R.sig = sig
+        if phys.sliding_maxdist > 0:
+            first, _ = _thread_counts(config, sim, 1)
+            R.system.set_chunks(chain_chunks(first, chain_id))
         else:
             if not np.array_equal(pop.pos, R.pos):
                 R.system.upload(pos=pop.pos)
@@ -219,7 +274,8 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         pop.pos1[:] = out["pos1"]
         pop.tangent_list[:] = out["tangent"]
         pop.forces[:] = out["forces"]
-        pop.where_active[:] = -1  # SRK.py:132
+        # SRK.py:132, forces.py:119: -1, plus the sliding partners of the last step
+        pop.where_active[:] = R.system.where_active() if phys.sliding_maxdist > 0 else -1
         R.pos = pop.pos.copy()
         R.tangent = pop.tangent_list.copy()
         _write_status(config, status, st)
@@ -257,6 +313,15 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
                 G.rids.append(G.pool.add(phys, p.pos, t, cid, rest, vt))
             G.pool.group(G.rids, laws, [k for k in range(len(pops)) if not moving[k]])
             G.sig = sig
+            # thread chunks for active sliding: updates.py:172-200 gives the first
+            # moving population's own pair the leftover threads
+            n_pairs = sum(len(pops) for k in range(len(pops)) if moving[k])
+            first, per = _thread_counts(config, sim, n_pairs)
+            first_moving = moving.index(True)
+            for k, (phys, _v, _r, cid, _s) in enumerate(parts):
+                if moving[k] and phys.sliding_maxdist > 0:
+                    G.pool.set_chunks(G.rids[k],
+                                      chain_chunks(first if k == first_moving else per, cid))
         else:
             for k, p in enumerate(pops):
                 if not np.array_equal(p.pos, G.pos[k]):
@@ -278,7 +343,8 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
             if tangents[k] is not None:
                 p.tangent_list[:] = out["tangent"]
             p.forces[:] = out["forces"]
-            p.where_active[:] = -1  # SRK.py:132
+            p.where_active[:] = (G.pool.where_active(G.rids[k]) if parts[k][0].sliding_maxdist > 0
+                                 else -1)  # SRK.py:132, forces.py:119
             if st is None:
                 s_ = G.pool.status(G.rids[k])
                 st = np.array([s_["dt"], s_["elapsed"], s_["steps"], s_["max_forces_strength"]])
@@ -290,9 +356,6 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         moving = [p for p in pops if not p.fixed]
         if not moving:
             raise NotImplementedError("GPU path: no moving population")
-        for p in moving:
-            if getattr(p, "sliding_partner_maxdist", 0.0) > 0 and getattr(p, "v0", 0.0) > 0:
-                raise NotImplementedError("GPU path: active_sliding_and_WCA is not ported")
         order = int(getattr(config, "SRK_order", 2))
         if order not in (2, 4):
             raise NotImplementedError(f"GPU path: SRK_order 2 or 4 (not {order})")

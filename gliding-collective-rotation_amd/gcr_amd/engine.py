@@ -39,6 +39,8 @@ class Physics:
     walls: int = 0  # config.walls as bits: x 1, y 2, z 4
     wall_zone: float = 0.0  # forbidden_zone_width (forces.py:417)
     wall_cutoff: float = 0.0  # wall_params[3]; <= 0: wca[3]
+    sliding_maxdist: float = 0.0  # active sliding partner range (0: off)
+    sliding_v0: float = 0.0  # active sliding push per partner
 
     def desc(self):
         d = ReplicaDesc()
@@ -64,6 +66,8 @@ class Physics:
         d.walls = int(self.walls)
         d.wall_zone = float(self.wall_zone)
         d.wall_cutoff = float(self.wall_cutoff)
+        d.sliding_maxdist = float(self.sliding_maxdist)
+        d.sliding_v0 = float(self.sliding_v0)
         return d
 
 
@@ -201,6 +205,16 @@ class Pool:
             fx[k] = 1
         check(self._L.gcr_pool_group(self.h, ptr(r), n, ptr(c), ptr(fx)))
 
+    def set_chunks(self, rid, chain_chunk):
+        """The reference's thread chunk of each chain (active sliding)."""
+        c = np.ascontiguousarray(chain_chunk, dtype=np.int32)
+        check(self._L.gcr_replica_set_chunks(self.h, int(rid), ptr(c)))
+
+    def where_active(self, rid):
+        out = np.empty(self.n[rid], dtype=np.int32)
+        check(self._L.gcr_replica_where_active(self.h, int(rid), ptr(out)))
+        return out
+
     def time_kernels(self, steps):
         out = np.zeros(4)
         check(self._L.gcr_pool_time_kernels(self.h, int(steps),
@@ -260,6 +274,15 @@ class System:
         T = np.empty((self.n, self.dim))
         check(self._L.gcr_eval_forces(self.h, ptr(F), ptr(T)))
         return F, T
+
+    def set_chunks(self, chain_chunk):
+        c = np.ascontiguousarray(chain_chunk, dtype=np.int32)
+        check(self._L.gcr_replica_set_chunks(self.h, 0, ptr(c)))
+
+    def where_active(self):
+        out = np.empty(self.n, dtype=np.int32)
+        check(self._L.gcr_replica_where_active(self.h, 0, ptr(out)))
+        return out
 
     def order_parameters(self):
         """Polar order, nematic order, director angle (on the device); the

@@ -65,6 +65,9 @@ typedef struct gcr_replica_desc {
     double wall_cutoff;  /* wall_params[3], the walls' range parameter; <= 0: wca[3] (they
                             differ while growing, when only the pair law is made purely
                             repulsive, updates.py:304-310) */
+    double sliding_maxdist; /* active sliding (the "active_sliding_and_WCA" force,
+                            forces.py:79-83, 114-120): partner range, 0: off */
+    double sliding_v0;   /* push v0 t per antiparallel partner */
 } gcr_replica_desc;
 
 /* ---------------------------------------------------------------- errors */
@@ -145,6 +148,15 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* de
  * a member dissolves the group. */
 int gcr_pool_group(gcr_pool* p, const int32_t* rids, int32_t n, const double* cross,
                    const int32_t* fixed);
+/* Thread chunk of each chain (chain_chunk[n_chains], NULL: all 0): the
+ * reference's split of a population among its threads (updates.py:323-344),
+ * which active sliding depends on: a partner in the bead's own chunk counts
+ * only when its index is lower (forces.py:98-102, 114-120). */
+int gcr_replica_set_chunks(gcr_pool* p, int32_t rid, const int32_t* chain_chunk);
+/* where_active after the last step (int32[n], reference bead order): -1 plus
+ * one per sliding partner in each stage of the step (SRK.py:132,
+ * forces.py:119); -1 without active sliding. */
+int gcr_replica_where_active(gcr_pool* p, int32_t rid, int32_t* where_active);
 /* Arms the next call of a replica (same meaning of time_vals / xi as gcr_run). */
 int gcr_replica_begin(gcr_pool* p, int32_t rid, const double time_vals[8], const double* xi,
                       int64_t xi_steps);

@@ -202,6 +202,32 @@ void orc_pair_forces_brute(int dim, int n, const double* pos, const int64_t* cha
     }
 }
 
+void orc_sliding(int dim, int n, const double* pos, const int64_t* chain_id, double L,
+                 const double* t, int n_nearby, double maxdist, double v0, const int32_t* chunk,
+                 double* F, int32_t* wa)
+{
+    double half = L / 2.0, md2 = maxdist * maxdist;
+    for (int b1 = 0; b1 < n; ++b1)
+        for (int b2 = 0; b2 < n; ++b2) {
+            if (excluded(chain_id, b1, b2, n_nearby)) continue;
+            if ((chunk ? chunk[b1] == chunk[b2] : 1) && b2 >= b1) continue;
+            double ds = 0.0, dot = 0.0;
+            for (int i = 0; i < dim; ++i) {
+                double sep = pos[b2 * dim + i] - pos[b1 * dim + i];
+                if (sep < -half)
+                    sep += L;
+                else if (sep >= half)
+                    sep -= L;
+                ds += sep * sep;
+                dot += t[b1 * dim + i] * t[b2 * dim + i];
+            }
+            if (ds < md2 && dot < 0) {
+                for (int i = 0; i < dim; ++i) F[b1 * dim + i] += v0 * t[b1 * dim + i];
+                if (wa) wa[b1] += 1;
+            }
+        }
+}
+
 void orc_self_propel(int dim, int n, const int64_t* chain_id, double v0, const double* vtan,
                      const double* t, double* F)
 {
@@ -389,6 +415,9 @@ void orc_force_eval(const orc_sys* s, const double* pos, const int64_t* chain_id
             orc_chain_forces(dim, n, pos, chain_id, s->L, s->spring, rest_lengths, tangent, F);
         }
     }
+    if (s->sliding_maxdist > 0)
+        orc_sliding(dim, n, pos, chain_id, s->L, tangent, s->n_nearby, s->sliding_maxdist,
+                    s->sliding_v0, s->chunk, F, s->where_active);
     if (s->has_wca) {
         if (pair_mode == 1 && vv)
             verlet_forces((const verlet*)vv, dim, pos, s->L, s->wca, F);
@@ -630,6 +659,8 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
     int64_t steps = 0;
     double elapsed = 0.0, m = 0.0, dt_for_update = dt;
     while ((steps < max_steps || max_steps < 0) && (elapsed < mdu || mdu < 0)) {
+        if (s->where_active)  /* SRK.py:132 */
+            for (int b = 0; b < n; ++b) s->where_active[b] = -1;
         /* noise for this step (SRK.py:134-142): drawn when D>0 / Dr>0 */
         const double* xi;
         if (xi_in) {

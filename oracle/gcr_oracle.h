@@ -37,6 +37,10 @@ typedef struct orc_sys {
     int32_t walls;            /* config.walls as bits: x 1, y 2, z 4 */
     double  wall_zone;        /* forbidden_zone_width (forces.py:417) */
     double  wall_cutoff;      /* wall_params[3]; <= 0: wca[3] */
+    double  sliding_maxdist;  /* active sliding (active_sliding_and_WCA): partner range, 0: off */
+    double  sliding_v0;       /* its push per antiparallel partner */
+    const int32_t* chunk;     /* thread chunk of each bead (NULL: one chunk) */
+    int32_t* where_active;    /* per bead: -1 at each step start, +1 per partner and stage */
 } orc_sys;
 
 /* Python float modulo (CPython float_rem semantics, used by `tmp %= Lx`
@@ -73,6 +77,17 @@ void orc_pair_forces_brute(int dim, int n, const double* pos, const int64_t* cha
  * reference's scalar v_tan_angle when every entry is equal). */
 void orc_self_propel(int dim, int n, const int64_t* chain_id, double v0,
                      const double* v_tan_angle_per_chain, const double* tangent, double* forces);
+
+/* forces.py:57-144 with b_calc_active_sliding (forces.py:79-83, 114-120): for
+ * every listed partner b2 of b1 (the same-chain rule applies) closer than
+ * maxdist whose tangent points against b1's (t1 . t2 < 0), b1 gets v0 t1 and
+ * where_active[b1] one more. A partner in b1's own thread chunk counts only
+ * when its index is lower (the reference skips b2 >= b1 there, a pair it
+ * handles once with Newton's third law for the pair force but not for the
+ * sliding push). Accumulates into forces. */
+void orc_sliding(int dim, int n, const double* pos, const int64_t* chain_id, double L,
+                 const double* tangent, int n_nearby, double maxdist, double v0,
+                 const int32_t* chunk, double* forces, int32_t* where_active);
 
 /* forces.py:405-435: WCA walls normal to the axes in `walls`, at `zone`
  * inside the faces, from the positions pos; accumulates into forces. Every

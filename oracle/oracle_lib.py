@@ -33,6 +33,10 @@ class OrcSys(C.Structure):
         ("walls", C.c_int32),
         ("wall_zone", C.c_double),
         ("wall_cutoff", C.c_double),
+        ("sliding_maxdist", C.c_double),
+        ("sliding_v0", C.c_double),
+        ("chunk", C.c_void_p),
+        ("where_active", C.c_void_p),
     ]
 
 
@@ -117,6 +121,12 @@ def make_sys(sysd):
     s.walls = sysd.get("walls", 0)
     s.wall_zone = sysd.get("wall_zone", 0.0)
     s.wall_cutoff = sysd.get("wall_cutoff", 0.0)
+    s.sliding_maxdist = sysd.get("sliding_maxdist", 0.0)
+    s.sliding_v0 = sysd.get("sliding_v0", 0.0)
+    ch = sysd.get("chunk")
+    s.chunk = None if ch is None else ch.ctypes.data
+    wa = sysd.get("where_active")
+    s.where_active = None if wa is None else wa.ctypes.data
     return s
 
 

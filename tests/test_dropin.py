@@ -33,9 +33,18 @@ def test_dropin_rejects_unported_features():
         f(moving_pop_missing, status, sim)
     pop.fixed = False
     pop.sliding_partner_maxdist = 1.0
-    with pytest.raises(NotImplementedError, match="active_sliding"):
-        f([pop], status, sim)
+    config.SRK_order = 4
+    try:
+        with pytest.raises(NotImplementedError, match="active sliding with SRK_order 4"):
+            f([pop], status, sim)
+    finally:
+        config.SRK_order = 2
     pop.sliding_partner_maxdist = 0.0
+    own = pop.self_interactions
+    pop.self_interactions = lambda: own() + [["Efield", np.zeros(2)]]
+    with pytest.raises(NotImplementedError, match="Efield"):
+        f([pop], status, sim)
+    pop.self_interactions = own
     config.SRK_order = 1
     try:
         with pytest.raises(NotImplementedError, match="SRK_order 2 or 4"):
@@ -209,6 +218,33 @@ def test_dropin_two_populations_match_oracle():
         assert np.max(np.abs(N.min_image(p.pos - out[k][0], 20.0))) < 1e-10 * 20.0, k
     assert status.dt == pytest.approx(st[0], rel=1e-10)
     assert status.t == pytest.approx(st[1], rel=1e-10)
+
+
+@pytest.mark.gpu
+def test_dropin_active_sliding_matches_oracle():
+    """sliding_partner_maxdist > 0: the population's force list holds
+    active_sliding_and_WCA instead of WCA and self-propulsion; the push per
+    antiparallel partner follows the reference's split into n_threads chunks,
+    and where_active comes back counting the partners."""
+    sim, pop, status = make_run(sliding_partner_maxdist=1.5, n_chains=12, chain_length=7)
+    assert [f[0] for f in pop.self_interactions()] == ["springs", "active_sliding_and_WCA"]
+    x0, t0 = pop.pos.copy(), pop.tangent_list.copy()
+    f = dropin.make_SRK2_force_calc(config)
+    status.next_output_step = 40
+    f([pop], status, sim)
+    assert config.n_threads == 3
+    seed = dropin._mix(sim["seed"], 1)
+    wa = np.zeros(pop.total_pop, dtype=np.int32)
+    chunk = dropin.chain_chunks(3, pop.chain_id)[pop.chain_id].astype(np.int32)
+    sysd = dict(dim=2, n=pop.total_pop, n_nearby=4, has_springs=True, has_wca=True,
+                has_propel=False, L=20.0, D=pop.D, Dr=pop.Dr, radius=1.0, wca=pop.WCA_params,
+                spring=pop.spring_params(), v0=0.0, sliding_maxdist=1.5, sliding_v0=1.0,
+                chunk=chunk, where_active=wa)
+    tv = np.array([sim["dt0"], sim["dtmax"], config.max_dist_per_step, 1.0, -1, 40, 40, 40])
+    pos_o, *_ = O.srk2_run(sysd, x0, t0, pop.chain_id, pop.rest_lengths, np.zeros(12), tv,
+                           xi=None, seed=seed)
+    assert np.max(np.abs(N.min_image(pop.pos - pos_o, 20.0))) < 1e-10 * 20.0
+    assert np.array_equal(pop.where_active, wa)
 
 
 @pytest.mark.gpu

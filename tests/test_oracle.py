@@ -659,6 +659,56 @@ def test_multi_population_diffusion_per_population():
     assert st[0] == dt
 
 
+def sliding_sys(n, L=20.0, chunk=None, wa=None, maxdist=1.5, v0=1.0, n_nearby=4):
+    return dict(dim=2, n=n, n_nearby=n_nearby, has_springs=False, has_wca=False,
+                has_propel=False, L=L, D=0.0, Dr=0.0, radius=1.0,
+                wca=O.wca_params(1.0, 1.0, 0.0, 1.0), spring=np.zeros(6), v0=0.0,
+                sliding_maxdist=maxdist, sliding_v0=v0, chunk=chunk, where_active=wa)
+
+
+def test_sliding_pushes_antiparallel_partners_by_thread_chunk():
+    """forces.py:114-120: v0 t per antiparallel partner closer than the
+    sliding range; in one thread chunk only the bead with the higher index
+    gets it (the reference skips b2 >= b1 there), across chunks both do;
+    parallel or distant partners give nothing."""
+    pos = np.array([[5.0, 5.0], [5.0, 6.2], [12.0, 5.0], [12.0, 6.2], [3.0, 15.0],
+                    [3.0, 16.6]])
+    t = np.array([[1.0, 0.0], [-1.0, 0.0], [1.0, 0.0], [1.0, 0.0], [0.0, 1.0], [0.0, -1.0]])
+    cid = np.arange(6)
+    F, _ = O.force_eval(sliding_sys(6), pos, cid, np.ones(6), np.zeros(6), t)
+    assert np.allclose(F[0], 0) and np.allclose(F[1], t[1])  # one chunk: the later bead
+    assert np.allclose(F[2:], 0)  # parallel pair, and a pair 1.6 apart
+    chunk = np.array([0, 1, 0, 1, 0, 1], dtype=np.int32)
+    F2, _ = O.force_eval(sliding_sys(6, chunk=chunk), pos, cid, np.ones(6), np.zeros(6), t)
+    assert np.allclose(F2[0], t[0]) and np.allclose(F2[1], t[1])
+    # two partners push twice; a same-chain neighbour does not count
+    pos3 = np.array([[5.0, 5.0], [5.0, 5.8], [5.6, 5.4]])
+    t3 = np.array([[-1.0, 0.0], [-1.0, 0.0], [1.0, 0.0]])
+    F3, _ = O.force_eval(sliding_sys(3), pos3, np.array([0, 1, 2]), np.ones(3), np.zeros(3), t3)
+    assert np.allclose(F3[2], 2 * t3[2]) and np.allclose(F3[:2], 0)
+    F4, _ = O.force_eval(sliding_sys(3), pos3, np.array([0, 1, 1]), np.ones(2), np.zeros(2), t3)
+    assert np.allclose(F4[2], t3[2])
+
+
+def test_sliding_pair_motion_and_where_active():
+    """Two antiparallel ABPs in different chunks, T = 0: each moves v0 dt_used
+    along its tangent per step, and where_active counts the partner in both
+    stages of the step (-1 + 2); in one chunk only the second moves."""
+    pos = np.array([[5.0, 5.0], [5.0, 6.0]])
+    t = np.array([[1.0, 0.0], [-1.0, 0.0]])
+    tv = np.array([1e-3, 1e-3, 1.0, 1.0, -1.0, 40, 40, 10])
+    wa = np.zeros(2, dtype=np.int32)
+    out = O.srk2_run(sliding_sys(2, chunk=np.array([0, 1], dtype=np.int32), wa=wa), pos, t,
+                     np.arange(2), np.ones(2), np.zeros(2), tv, xi=np.zeros((10, 2, 4)))
+    assert np.allclose(out[0], pos + 10 * 1e-3 * t, atol=1e-13)
+    assert wa.tolist() == [1, 1]
+    wa1 = np.zeros(2, dtype=np.int32)
+    out1 = O.srk2_run(sliding_sys(2, wa=wa1), pos, t, np.arange(2), np.ones(2), np.zeros(2), tv,
+                      xi=np.zeros((10, 2, 4)))
+    assert np.allclose(out1[0][0], pos[0]) and np.allclose(out1[0][1], pos[1] + 1e-2 * t[1])
+    assert wa1.tolist() == [-1, 1]
+
+
 def test_rotational_diffusion_closed_form():
     """SRK.py:398-423: t' = normalize(t + pi sqrt(2 Dr dt) (xi_r - (xi_r.t) t))."""
     Dr = 0.2
