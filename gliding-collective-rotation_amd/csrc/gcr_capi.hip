@@ -144,10 +144,11 @@ struct gcr_pool {
     std::vector<Group> groups;
     XPair* XPAIRS = nullptr;
     int32_t* GRP = nullptr;
-    int32_t *CHAIN_CHUNK = nullptr, *WA = nullptr, *WAn = nullptr;
+    int32_t *CHAIN_CHUNK = nullptr, *WA = nullptr, *WAn = nullptr, *MC = nullptr;
     int64_t capX = 0;
     bool has_group = false;  // a group exists or a replica has active sliding: the
                              // second half of each force stage runs (k_group_stage)
+    bool has_motor = false;  // a group has motors: k_motor runs in each force stage
 };
 
 namespace {
@@ -257,6 +258,13 @@ void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
     P.slide_d2 = d->sliding_maxdist * d->sliding_maxdist;
     P.slide_v0 = d->sliding_v0;
     if (d->sliding_maxdist > 0.0) fl |= GCR_F_SLIDE;
+    P.motor_d2 = d->motor_max_dist * d->motor_max_dist;
+    P.motor_imparts = d->motor_imparts;
+    P.k_on = d->k_on;
+    P.k_off = d->k_off;
+    P.motor_reach = d->motor_reach > 0.0 ? d->motor_reach : d->motor_max_dist;
+    if (d->motor) fl |= GCR_F_MOTOR;
+    if (d->motor && d->motor_still) fl |= GCR_F_MSTILL;
     P.wall_zone = d->wall_zone;
     P.wall_rc = d->wall_cutoff > 0.0 ? d->wall_cutoff : d->wca[3];
     if (P.walls) fl |= GCR_F_WALLS;
@@ -276,7 +284,11 @@ int rebuild_tables(gcr_pool* p)
     bool has_srk4 = false, tile_gen = false;
     std::vector<XPair> xp;
     std::vector<int32_t> grp;
-    for (auto& P : p->P) P.group = -1;
+    bool has_motor = false;
+    for (auto& P : p->P) {
+        P.group = -1;
+        P.motor_target = -1;
+    }
     for (int g = 0; g < (int)p->groups.size(); ++g) {
         // members in population order; each moving member takes the laws of
         // the others in the order (Q - P) mod n and the group's moving list
@@ -297,9 +309,15 @@ int rebuild_tables(gcr_pool* p)
                 const int o = (k + ii) % n;
                 const double* c = &G.cross[((size_t)k * n + o) * 5];
                 if (c[0] == 0.0) continue;
-                const RepParams& Q = p->P[G.rids[o]];
+                RepParams& Q = p->P[G.rids[o]];
                 XPair e;
                 e.src = G.rids[o];
+                e.kind = c[0] == 2.0 ? 2 : 1;
+                e.pad = 0;
+                if (e.kind == 2) {  // motors of Q push this population
+                    Q.motor_target = G.rids[k];
+                    has_motor = true;
+                }
                 const double reach = c[4] + (Q.skin_half < Q.L ? Q.skin_half : Q.L);
                 e.window = std::max(1, (int)std::ceil(reach / Q.h));
                 e.p0 = c[1];
@@ -382,6 +400,7 @@ int rebuild_tables(gcr_pool* p)
     p->has_srk4 = has_srk4;
     p->tile_gen = tile_gen;
     p->has_group = !p->groups.empty();
+    p->has_motor = has_motor;
     for (int r = 0; r < p->capR; ++r)
         if (p->live[r] && (p->P[r].flags & GCR_F_SLIDE)) p->has_group = true;
     if (!xp.empty())
@@ -561,6 +580,17 @@ void launch_group(gcr_pool* p, int stage, int j)
                    : k_group_stage<3, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
 }
 
+// Motor bindings of a force stage (before the stage's second half).
+void launch_motor(gcr_pool* p, int stage, int j)
+{
+    if (p->dim == 2)
+        stage == 0 ? k_motor<2, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
+                   : k_motor<2, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
+    else
+        stage == 0 ? k_motor<3, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
+                   : k_motor<3, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
+}
+
 // SRK4 virtual update of stage `sub` (1 or 2) for the replicas without rod.
 void launch_virtual(gcr_pool* p, int j, int sub)
 {
@@ -593,6 +623,7 @@ void launch_batch(gcr_pool* p, int K, bool safe)
     for (int j = 0; j < p->B; ++j) {
         if (j % K == 0) launch_rebuild(p, j);
         launch_stage(p, 0, j);
+        if (p->has_motor) launch_motor(p, 0, j);
         if (p->has_group) launch_group(p, 0, j);
         launch_rod(p, 0, j, 0);
         if (safe) launch_rebuild(p, j, true, 0);
@@ -603,6 +634,7 @@ void launch_batch(gcr_pool* p, int K, bool safe)
             if (safe) launch_rebuild(p, j, true, sub);
         }
         launch_stage(p, 1, j);
+        if (p->has_motor) launch_motor(p, 1, j);
         if (p->has_group) launch_group(p, 1, j);
         launch_stage(p, 3, j);
         if (p->has_group) k_group_dt<<<(p->capR + 63) / 64, 64, 0, p->stream>>>(p->a, j);
@@ -839,6 +871,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CHAIN_CHUNK, capCh));
     A(dalloc(&p->WA, nb));
     A(dalloc(&p->WAn, nb));
+    A(dalloc(&p->MC, nb));
     A(dalloc(&p->TILES, p->capTile * 4));
     A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
     // neighbour lists: 2D u16 LDS indices (NL_MAX), 3D int32 slots (NL3_MAX),
@@ -912,6 +945,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.CHAIN_CHUNK = p->CHAIN_CHUNK;
     a.WA = p->WA;
     a.WAn = p->WAn;
+    a.MC = p->MC;
     a.red_base = dim == 2 ? (int32_t)p->capTile : 0;
     *out = p;
     return GCR_OK;
@@ -1105,6 +1139,21 @@ int gcr_pool_group(gcr_pool* p, const int32_t* rids, int32_t n, const double* cr
     if (!moving) return fail(GCR_ERR_ARG, "a group needs a moving population");
     G.cross.assign((size_t)n * n * 5, 0.0);
     if (cross) std::memcpy(G.cross.data(), cross, sizeof(double) * G.cross.size());
+    for (int b = 0; b < n; ++b) {  // motors: a motor population, one pushed population
+        int pushed = 0;
+        for (int a2 = 0; a2 < n; ++a2) {
+            const double kind = G.cross[((size_t)a2 * n + b) * 5];
+            if (kind == 2.0) {
+                if (!(p->P[rids[b]].flags & GCR_F_MOTOR))
+                    return fail(GCR_ERR_ARG, "kind 2 needs a motor population as the source");
+                if (G.fixed[a2]) return fail(GCR_ERR_ARG, "motors push a moving population");
+                ++pushed;
+            } else if (kind != 0.0 && kind != 1.0) {
+                return fail(GCR_ERR_ARG, "cross kinds are 0, 1 (law) or 2 (motors)");
+            }
+        }
+        if (pushed > 1) return fail(GCR_ERR_ARG, "a motor population pushes one population");
+    }
     int rc = pull_states(p);
     if (rc) return rc;
     for (int k = 0; k < n; ++k) {
@@ -1492,7 +1541,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
                     p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR, p->OBS,
-                    p->XPAIRS, p->GRP, p->CHAIN_CHUNK, p->WA, p->WAn};
+                    p->XPAIRS, p->GRP, p->CHAIN_CHUNK, p->WA, p->WAn, p->MC};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

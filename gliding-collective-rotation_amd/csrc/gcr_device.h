@@ -25,6 +25,8 @@
 #define GCR_F_GROUP 2048u   // a moving population of a group: other populations act on it
                             // (updates.py:132-267, SRK.py:287-293)
 #define GCR_F_SLIDE 4096u   // active sliding (forces.py:79-83, 114-120)
+#define GCR_F_MOTOR 8192u   // a motor population (populations.py:617-660)
+#define GCR_F_MSTILL 16384u // its bound motors do not diffuse (SRK.py:389-396)
 
 // The law of one population acting on another (interpop_forces,
 // updates.py:274-312): a WCA/LJ pair law and the source replica, with the
@@ -33,6 +35,8 @@
 struct XPair {
     int32_t src;
     int32_t window;
+    int32_t kind;  // 1: WCA/LJ law, 2: the motors of src push
+    int32_t pad;
     double p0, p1, shift, rc;
 };
 
@@ -69,6 +73,9 @@ struct RepParams {
     double wall_zone;         // distance of the walls from the box faces
     double wall_rc;           // the walls' range parameter (wall_params[3])
     double slide_d2, slide_v0; // active sliding: squared partner range, push per partner
+    double motor_d2, motor_imparts, k_on, k_off, motor_reach;  // a motor population
+    int32_t motor_target;     // the replica its motors push (set by the group)
+    int32_t pad2;
     int32_t walls;            // bit i: walls normal to axis i
     int32_t pad0;
     double D, Dr, radius;     // pop_consts[0..2]
@@ -170,8 +177,9 @@ struct PoolArgs {
     //         total) travels with the beads
     // active sliding: thread chunk per chain, where_active per slot (+ staging)
     const int32_t* CHAIN_CHUNK;
-    int32_t* WA;
+    int32_t* WA;      // where_active; for a motor population the bound bead, -1 free
     int32_t* WAn;
+    int32_t* MC;      // bindings a bead received in the running stage
     // populations acting on each other
     const XPair* XPAIRS;
     const int32_t* GRP;

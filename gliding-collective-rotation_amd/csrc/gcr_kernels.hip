@@ -961,6 +961,10 @@ __device__ __forceinline__ double stage_store(const PoolArgs& a, const RepParams
         // pushes, the other populations' forces, the virtual update and the
         // max follow in k_group_stage, once every bead's tangent is known
         store_vec<DIM>(STAGE == 0 ? a.F0 : a.FT, s, acc);
+        if (!(P.flags & GCR_F_MOTOR)) {
+            if (STAGE == 0) a.WA[s] = -1;  // SRK.py:132
+            a.MC[s] = 0;
+        }
         return 0.0;
     }
     if (STAGE == 0) {
@@ -1055,7 +1059,7 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
             load_vec<DIM>(a.X, s, x0);
             add_walls<DIM>(P, x0, acc);
         }
-        if (P.flags & GCR_F_SLIDE) {  // tangents at this stage, for k_group_stage
+        if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP)) {  // tangents at this stage, for k_group_stage
             if (!tang_set) load_vec<DIM>(a.T, s, tang);
             store_vec<DIM>(a.Tn, s, tang);
         }
@@ -1248,12 +1252,23 @@ __global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
         }
         for (int k = P.xin_off; k < P.xin_off + P.xin_n; ++k) {
             const XPair e = a.XPAIRS[k];
+            if (e.kind == 2) {  // the bindings k_motor counted, imparts * t each
+                double t1[DIM];
+                load_vec<DIM>(a.Tn, s, t1);
+                const double imp = a.P[e.src].motor_imparts;
+                for (int c = a.MC[s]; c > 0; --c)
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) f[i] += imp * t1[i];
+                continue;
+            }
             const double* XQ = (STAGE == 0 || a.S[e.src].fixed) ? a.X : a.X1;
             cross_sum<DIM>(a, a.P[e.src], XQ, x, e, f);
         }
         if (STAGE == 0) {
             store_vec<DIM>(a.F0, s, f);
-            virtual_update<DIM>(a, P, S, s, x, f, 1.0, 0);
+            // a bound motor that does not diffuse keeps its last x1
+            // (SRK.py:389-396 writes only the movers)
+            if (!((P.flags & GCR_F_MSTILL) && a.WA[s] >= 0)) virtual_update<DIM>(a, P, S, s, x, f, 1.0, 0);
         } else {
             store_vec<DIM>(a.FT, s, f);
 #pragma unroll
@@ -1264,6 +1279,110 @@ __global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
         const double m = block_max(fsq);
         if (threadIdx.x == 0) a.BMAX[P.red_off + (blockIdx.x - P.blk_off)] = m;
     }
+}
+
+// The motor rule's uniform draw: Philox4x32-10 keyed by the motor
+// population's seed, counter (motor, bead, step lo, step hi << 8 | 0x80 |
+// stage << 1 | kind), 53 bits in [0, 1). Same definition as the oracle's
+// orc_motor_uniform.
+__device__ __forceinline__ double motor_uniform(uint64_t seed, uint32_t motor, uint32_t bead,
+                                                int64_t step, int stage, int kind)
+{
+    uint32_t c[4] = {motor, bead, (uint32_t)(uint64_t)step,
+                     ((uint32_t)((uint64_t)step >> 32) << 8) | 0x80u | ((uint32_t)stage << 1) |
+                         (uint32_t)kind};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return ((double)(c[0] >> 5) * 67108864.0 + (double)(c[1] >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Motors (forces.py:248-283), one thread per motor of a motor population of a
+// group: the beads of the pushed population within its reach, in index
+// order (each motor's history depends on the beads in order and on nothing
+// else, so the reference's bead-major loop is the same as this motor-major
+// one): a bound motor comes free when its draw is <= k_off (the visiting
+// bead's where_active drops by one), and a free motor binds to the visiting
+// bead when its draw is < k_on and the bead is within the motor range
+// (where_active and the bead's binding count rise by one; k_group_stage adds
+// imparts * t per binding). A moving motor population starts each step free
+// (SRK.py:132); a fixed one keeps its bindings.
+template <int DIM, int STAGE>
+__global__ void __launch_bounds__(BS) k_motor(PoolArgs a, int j)
+{
+    const int q = a.blk_rep[blockIdx.x];
+    const RepParams& Q = a.P[q];
+    if (!(Q.flags & GCR_F_MOTOR) || Q.motor_target < 0) return;
+    const int tr = Q.motor_target;
+    const RepParams& T = a.P[tr];
+    const RepState* ST = a.S + tr;
+    if (!step_runs(ST, j)) return;
+    const int m = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (m >= Q.bead_off + Q.n) return;
+    const bool qfixed = a.S[q].fixed != 0;
+    double xm[DIM];
+    load_vec<DIM>((STAGE == 0 || qfixed) ? a.X : a.X1, m, xm);
+    const double* XT = STAGE == 0 ? a.X : a.X1;
+    int state = a.WA[m];
+    if (!qfixed && STAGE == 0) state = -1;
+    const uint32_t mloc = (uint32_t)(a.META[2 * m] - Q.bead_off);
+    const int64_t step = ST->step_global;
+    const int nc = T.nc;
+    const double reach2 = Q.motor_reach * Q.motor_reach;
+    const double wd = ceil((Q.motor_reach + fmin(T.skin_half, T.L)) * T.inv_h);
+    const int w = wd >= nc ? nc : max(1, (int)wd);
+    const bool all = 2 * w + 1 >= nc;
+    const int lo = all ? 0 : -w, hi = all ? nc - 1 : w;
+    int c[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) c[i] = cell_coord(xm[i], T.inv_h, nc);
+    int last = -1;  // the beads within reach, visited in increasing index
+    for (;;) {
+        int best = 0x7fffffff, bslot = -1;
+        double bd2 = 0.0;
+        for (int dz = (DIM == 3 ? lo : 0); dz <= (DIM == 3 ? hi : 0); ++dz) {
+            const int z = DIM == 3 ? (all ? dz : (c[2] + dz + nc) % nc) : 0;
+            for (int dy = lo; dy <= hi; ++dy) {
+                const int y = all ? dy : (c[1] + dy + nc) % nc;
+                for (int dx = lo; dx <= hi; ++dx) {
+                    const int xc = all ? dx : (c[0] + dx + nc) % nc;
+                    const int cell = T.cell_off + (z * nc + y) * nc + xc;
+                    const int t1 = a.CSTART[cell + 1];
+                    for (int t = a.CSTART[cell]; t < t1; ++t) {
+                        const int o = a.META[2 * t];
+                        if (o <= last || o >= best) continue;
+                        double xb[DIM], d2 = 0.0;
+                        load_vec<DIM>(XT, t, xb);
+#pragma unroll
+                        for (int i = 0; i < DIM; ++i) {
+                            double sep = xm[i] - xb[i];
+                            if (sep < -T.halfL)
+                                sep += T.L;
+                            else if (sep >= T.halfL)
+                                sep -= T.L;
+                            d2 += sep * sep;
+                        }
+                        if (d2 > reach2) continue;
+                        best = o;
+                        bslot = t;
+                        bd2 = d2;
+                    }
+                }
+            }
+        }
+        if (bslot < 0) break;
+        last = best;
+        const uint32_t bloc = (uint32_t)(best - T.bead_off);
+        if (state >= 0) {
+            if (motor_uniform(Q.seed, mloc, bloc, step, STAGE, 0) > Q.k_off) continue;
+            state = -1;
+            atomicAdd(&a.WA[bslot], -1);
+        }
+        if (motor_uniform(Q.seed, mloc, bloc, step, STAGE, 1) < Q.k_on && bd2 <= Q.motor_d2) {
+            atomicAdd(&a.WA[bslot], 1);
+            atomicAdd(&a.MC[bslot], 1);
+            state = (int)bloc;
+        }
+    }
+    a.WA[m] = state;
 }
 
 // One dt for a group: the largest member part (SRK.py:336-337 takes the max
@@ -1307,7 +1426,8 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
     const double s2 = sqrt(2.0 * dt_used * P.D);
     dt_eff /= P.radius;
     double x[DIM], f[DIM], xi[2 * DIM];
-    if (!(P.flags & GCR_F_ROD)) {  // rod replicas: the corrector ran in k_rod
+    const bool still = (P.flags & GCR_F_MSTILL) && a.WA[s] >= 0;  // a bound motor
+    if (!(P.flags & GCR_F_ROD) && !still) {  // rod replicas: the corrector ran in k_rod
         load_vec<DIM>(a.X, s, x);
         load_vec<DIM>(a.FT, s, f);
         load_vec<DIM>(a.XI, s, xi);
@@ -1574,7 +1694,8 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
         const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
         load_vec<DIM>(srk4 ? a.FT : a.F0, so, x);
         store_vec<DIM>(srk4 ? a.F0 : a.FT, d, x);
-        if (P.flags & GCR_F_SLIDE) a.WAn[d] = a.WA[so];  // stage-0 partner counts
+        if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP | GCR_F_MOTOR))
+            a.WAn[d] = a.WA[so];  // where_active so far, or a motor's bound bead
     }
 }
 
@@ -1618,7 +1739,7 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
         const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
         load_vec<DIM>(srk4 ? a.F0 : a.FT, ns, x);
         store_vec<DIM>(srk4 ? a.FT : a.F0, ns, x);
-        if (P.flags & GCR_F_SLIDE) a.WA[ns] = a.WAn[ns];
+        if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP | GCR_F_MOTOR)) a.WA[ns] = a.WAn[ns];
     }
 }
 

@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
                 load_vec<2>(a.X, s, x0);
                 add_walls<2>(P, x0, acc);
             }
-            if (GEN && (P.flags & GCR_F_SLIDE)) {  // tangents at this stage, for k_group_stage
+            if (GEN && (P.flags & (GCR_F_SLIDE | GCR_F_GROUP))) {  // tangents at this stage, for k_group_stage
                 if (!tang_set) load_vec<2>(a.T, s, tang);
                 store_vec<2>(a.Tn, s, tang);
             }

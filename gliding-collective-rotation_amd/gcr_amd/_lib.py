@@ -45,6 +45,13 @@ class ReplicaDesc(C.Structure):
         ("wall_cutoff", C.c_double),
         ("sliding_maxdist", C.c_double),
         ("sliding_v0", C.c_double),
+        ("motor", C.c_int32),
+        ("motor_still", C.c_int32),
+        ("motor_max_dist", C.c_double),
+        ("motor_imparts", C.c_double),
+        ("k_on", C.c_double),
+        ("k_off", C.c_double),
+        ("motor_reach", C.c_double),
     ]
 
 

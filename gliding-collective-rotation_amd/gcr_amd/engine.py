@@ -41,6 +41,13 @@ class Physics:
     wall_cutoff: float = 0.0  # wall_params[3]; <= 0: wca[3]
     sliding_maxdist: float = 0.0  # active sliding partner range (0: off)
     sliding_v0: float = 0.0  # active sliding push per partner
+    motor: bool = False  # a motor population
+    motor_still: bool = True  # b_motors_dont_diffuse_while_pushing
+    motor_max_dist: float = 0.0
+    motor_imparts: float = 0.0  # motor_imparts_force
+    k_on: float = 0.0
+    k_off: float = 0.0
+    motor_reach: float = 0.0  # <= 0: motor_max_dist
 
     def desc(self):
         d = ReplicaDesc()
@@ -68,6 +75,13 @@ class Physics:
         d.wall_cutoff = float(self.wall_cutoff)
         d.sliding_maxdist = float(self.sliding_maxdist)
         d.sliding_v0 = float(self.sliding_v0)
+        d.motor = int(bool(self.motor))
+        d.motor_still = int(bool(self.motor_still))
+        d.motor_max_dist = float(self.motor_max_dist)
+        d.motor_imparts = float(self.motor_imparts)
+        d.k_on = float(self.k_on)
+        d.k_off = float(self.k_off)
+        d.motor_reach = float(self.motor_reach)
         return d
 
 
@@ -193,11 +207,15 @@ class Pool:
     def group(self, rids, cross=None, fixed=None):
         """Make replicas `rids` (in population order) the populations of one
         system: cross[(a, b)] = the WCA law 4-vector of population b acting on
-        a (interpop_forces); fixed = the indices of fixed populations."""
+        a (interpop_forces), or "motor" when b is a motor population pushing
+        a's beads; fixed = the indices of fixed populations."""
         n = len(rids)
         r = np.ascontiguousarray(rids, dtype=np.int32)
         c = np.zeros((n, n, 5))
         for (a, b), law in (cross or {}).items():
+            if isinstance(law, str) and law == "motor":
+                c[a, b, 0] = 2.0
+                continue
             c[a, b, 0] = 1.0
             c[a, b, 1:] = law
         fx = np.zeros(n, dtype=np.int32)

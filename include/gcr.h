@@ -68,6 +68,15 @@ typedef struct gcr_replica_desc {
     double sliding_maxdist; /* active sliding (the "active_sliding_and_WCA" force,
                             forces.py:79-83, 114-120): partner range, 0: off */
     double sliding_v0;   /* push v0 t per antiparallel partner */
+    int32_t motor;       /* a motor population (populations.py:617-660): its beads are motors
+                            that bind to and push the beads of the population the group's
+                            cross table names (kind 2, gcr_pool_group) */
+    int32_t motor_still; /* b_motors_dont_diffuse_while_pushing */
+    double motor_max_dist; /* motor_max_dist: range of a binding */
+    double motor_imparts;  /* motor_imparts_force: push imparts * t of a binding */
+    double k_on, k_off;  /* binding and unbinding probabilities per visit */
+    double motor_reach;  /* beads within this of a motor are visited by it (its neighbour
+                            list); <= 0: motor_max_dist */
 } gcr_replica_desc;
 
 /* ---------------------------------------------------------------- errors */
@@ -136,9 +145,11 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* de
 /* Makes replicas rids[0..n) of the pool (same box, n <= GCR_MAX_POPS,
  * SRK_order 2, no rigid rod) the populations of one system, in the order of
  * config.pop_name_list: the pops list SRK2_force_calc receives
- * (updates.py:132-267). cross[(a * n + b) * 5] nonzero: population b acts on
+ * (updates.py:132-267). cross[(a * n + b) * 5] == 1: population b acts on
  * population a with the WCA/LJ law cross[(a * n + b) * 5 + 1 .. 4] = {12 eps
- * r^6, 12 eps r^12, shift, cutoff}, as interpop_forces(pop_a, pop_b) gives it
+ * r^6, 12 eps r^12, shift, cutoff}; == 2: the motors of motor population b
+ * push the beads of a ("motor_push", forces.py:248-283; one pushed population
+ * per motor population); as interpop_forces(pop_a, pop_b) gives them
  * (updates.py:274-312: interactions() averages the two WCA vectors,
  * populations.py:226-232; special_interactions override); each replica's own
  * desc->wca is its self law. fixed[a] nonzero (NULL: none): population a never

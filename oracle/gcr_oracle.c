@@ -831,6 +831,51 @@ void orc_cross_pair_forces(int dim, int nA, const double* posA, int nB, const do
         }
 }
 
+double orc_motor_uniform(uint64_t seed, uint32_t motor, uint32_t bead, int64_t step, int stage,
+                         int kind)
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {motor, bead, (uint32_t)(uint64_t)step,
+                       ((uint32_t)((uint64_t)step >> 32) << 8) | 0x80u | ((uint32_t)stage << 1) |
+                           (uint32_t)kind};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    return ((double)(o[0] >> 5) * 67108864.0 + (double)(o[1] >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+void orc_motor_push(int dim, int nb, const double* posb, const double* tb, int nm,
+                    const double* posm, double L, const orc_pop* M, int64_t step, int stage,
+                    double* F, int32_t* wab)
+{
+    double half = L / 2.0, reach2 = M->motor_reach * M->motor_reach;
+    for (int b = 0; b < nb; ++b)
+        for (int m = 0; m < nm; ++m) {
+            double d2 = 0.0;
+            for (int i = 0; i < dim; ++i) {
+                double sep = posm[m * dim + i] - posb[b * dim + i];
+                if (sep < -half)
+                    sep += L;
+                else if (sep >= half)
+                    sep -= L;
+                d2 += sep * sep;
+            }
+            if (d2 > reach2) continue;
+            int32_t* mb = &M->motor_bead[m];
+            if (*mb >= 0) {
+                if (orc_motor_uniform(M->seed, (uint32_t)m, (uint32_t)b, step, stage, 0) > M->k_off)
+                    continue;
+                *mb = -1;
+                if (wab) wab[b] -= 1;
+            }
+            if (*mb < 0 && orc_motor_uniform(M->seed, (uint32_t)m, (uint32_t)b, step, stage, 1) < M->k_on &&
+                d2 <= M->motor_max_dist_sq) {
+                if (wab) wab[b] += 1;
+                *mb = b;
+                for (int i = 0; i < dim; ++i) F[b * dim + i] += M->motor_imparts * tb[b * dim + i];
+            }
+        }
+}
+
 int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const double tv[8],
                            double status_out[4], int64_t step0)
 {
@@ -845,6 +890,14 @@ int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const d
     int64_t steps = 0;
     double elapsed = 0.0, m = 0.0, dt_for_update = dt;
     while ((steps < max_steps || max_steps < 0) && (elapsed < mdu || mdu < 0)) {
+        for (int q = 0; q < npop; ++q) {  /* SRK.py:132, by every moving population */
+            orc_pop* P = &pops[q];
+            if (P->fixed) continue;
+            if (P->sys.where_active)
+                for (int b = 0; b < P->sys.n; ++b) P->sys.where_active[b] = -1;
+            if (P->motor && P->motor_bead)
+                for (int b = 0; b < P->sys.n; ++b) P->motor_bead[b] = -1;
+        }
         const double* xi[16];
         for (int q = 0; q < npop && q < 16; ++q) {
             const orc_pop* P = &pops[q];
@@ -873,7 +926,11 @@ int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const d
                     if (c[0] == 0.0) continue;
                     const orc_pop* Q = &pops[o];
                     const double* qpos = (stage == 0 || Q->fixed) ? Q->pos : Q->pos1;
-                    orc_cross_pair_forces(dim, n, at, Q->sys.n, qpos, L, c + 1, P->forces);
+                    if (c[0] == 2.0)
+                        orc_motor_push(dim, n, at, P->tangent, Q->sys.n, qpos, L, Q,
+                                       step0 + steps, stage, P->forces, P->sys.where_active);
+                    else
+                        orc_cross_pair_forces(dim, n, at, Q->sys.n, qpos, L, c + 1, P->forces);
                 }
             }
             if (stage == 1) {
@@ -910,6 +967,9 @@ int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const d
                 double s2 = sqrt(2.0 * dt_for_update * P->sys.D);
                 for (int b = 0; b < P->sys.n; ++b)
                     for (int i = 0; i < dim; ++i) {
+                        /* SRK.py:389-396: a pushing motor stays where it is
+                         * (its pos1 keeps the last value written) */
+                        if (P->motor && P->motor_still && P->motor_bead[b] >= 0) break;
                         double tmp = P->pos[b * dim + i] + dt_eff * P->forces[b * dim + i];
                         if (s2 != 0.0) tmp += s2 * xi[q][(size_t)b * nd + i];
                         out[b * dim + i] = orc_pymod(tmp, L);

@@ -159,19 +159,46 @@ typedef struct orc_pop {
     const double* vtan;
     const double* xi;         /* [steps][n][2 dim] or NULL: orc_bead_noise(seed, ...) */
     uint64_t seed;
+    /* a motor population (populations.py:617-660): its motors push beads of
+     * the populations whose cross entry names it with kind 2 */
+    int32_t motor;
+    int32_t motor_still;      /* b_motors_dont_diffuse_while_pushing */
+    double motor_max_dist_sq, motor_imparts, k_on, k_off;
+    double motor_reach;       /* beads farther than this are not the motor's candidates */
+    int32_t* motor_bead;      /* per motor: the bead it pushes, -1: free (where_active) */
 } orc_pop;
+
+/* The uniform draw of the motor rules: Philox4x32-10 keyed by the motor
+ * population's seed, counter (motor, bead, step lo, step hi << 8 | 0x80 |
+ * stage << 1 | kind), kind 0 the unbinding draw, 1 the binding draw; 53 bits
+ * in [0, 1). Shared with the GPU path. */
+double orc_motor_uniform(uint64_t seed, uint32_t motor, uint32_t bead, int64_t step, int stage,
+                         int kind);
 
 /* The SRK2 loop over several populations (updates.py:132-267 with the
  * same_pop False threads of SRK.py): each moving population P evaluates its
  * own forces (orc_force_eval), then adds the forces from every other
- * population Q in the order (Q - P) mod npop = 1, 2, ... (SRK.py:287-293)
- * where cross[(P * npop + Q) * 5] != 0, with the law cross[... + 1 .. 4]
+ * population Q in the order (Q - P) mod npop = 1, 2, ... (SRK.py:287-293):
+ * cross[(P * npop + Q) * 5] == 1, the WCA law cross[... + 1 .. 4]; == 2, the
+ * motor pushes of motor population Q (forces.py:248-283, below)
  * (interpop_forces, updates.py:274-312), Q's positions being pos1 in the
  * second stage unless Q is fixed; m is the largest sqrt(2 D_P) + 1/2 max|F_P|
  * / r_P over the moving populations, one dt for all; each population updates
  * with its own D, radius and noise. Returns steps done. */
 int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const double time_vals[8],
                            double status_out[4], int64_t step0);
+
+/* forces.py:248-283 in one stage: for every bead b (index order) and every
+ * motor m within motor_reach of it (index order): a bound motor stays bound
+ * unless its draw is <= k_off, in which case it comes free and b's
+ * where_active drops by one (whichever bead it pushed); a free motor binds to
+ * b when its draw is < k_on and b lies within the motor range, adding
+ * imparts * t_b to b's force and one to b's where_active. Draws from
+ * orc_motor_uniform(seed of M, motor, bead, step, stage, kind). Motors are
+ * independent of each other, so each motor's history is its beads in order. */
+void orc_motor_push(int dim, int nb, const double* posb, const double* tb, int nm,
+                    const double* posm, double L, const orc_pop* M, int64_t step, int stage,
+                    double* F, int32_t* where_active_b);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,14 @@ class OrcPop(C.Structure):
         ("vtan", C.c_void_p),
         ("xi", C.c_void_p),
         ("seed", C.c_uint64),
+        ("motor", C.c_int32),
+        ("motor_still", C.c_int32),
+        ("motor_max_dist_sq", C.c_double),
+        ("motor_imparts", C.c_double),
+        ("k_on", C.c_double),
+        ("k_off", C.c_double),
+        ("motor_reach", C.c_double),
+        ("motor_bead", C.c_void_p),
     ]
 
 
@@ -92,6 +100,9 @@ def lib():
         L.orc_multi_srk2_run.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_int64]
         L.orc_multi_srk2_run.restype = C.c_int64
+        L.orc_motor_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int,
+                                        C.c_int]
+        L.orc_motor_uniform.restype = C.c_double
         L.orc_walls.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, C.c_int, C.c_double, dp]
         L.orc_force_eval.argtypes = [C.POINTER(OrcSys), dp, ip, dp, dp, dp, dp, C.c_int,
                                      C.c_void_p, C.c_void_p]
@@ -221,8 +232,10 @@ def cross_pair_forces(posA, posB, L, p):
 
 def multi_srk2_run(pops, cross, time_vals, step0=0):
     """orc_multi_srk2_run. pops: list of dicts with sysd, pos, tangent,
-    chain_id, rest_lengths, vtan, and optionally xi, seed, fixed; cross:
-    {(P, Q): wca 4-vector} for the forces of Q on P. Returns a list of
+    chain_id, rest_lengths, vtan, and optionally xi, seed, fixed, motor (a
+    dict max_dist, imparts, k_on, k_off, reach, still; the per-motor bound
+    bead is returned in d["motor_bead"]); cross: {(P, Q): wca 4-vector, or
+    "motor" for the pushes of motor population Q on P}. Returns a list of
     (pos, pos1, tangent, forces) per population and the status."""
     n = len(pops)
     assert 1 <= n <= 16
@@ -248,9 +261,23 @@ def multi_srk2_run(pops, cross, time_vals, step0=0):
         arr[k].vtan = vt.ctypes.data
         arr[k].xi = xi.ctypes.data if xi is not None else None
         arr[k].seed = int(d.get("seed", 0)) & ((1 << 64) - 1)
+        mo = d.get("motor")
+        if mo is not None:
+            mb = d.setdefault("motor_bead", np.full(len(pos), -1, dtype=np.int32))
+            keep.append(mb)
+            arr[k].motor = 1
+            arr[k].motor_still = int(bool(mo.get("still", True)))
+            arr[k].motor_max_dist_sq = float(mo["max_dist"]) ** 2
+            arr[k].motor_imparts = float(mo["imparts"])
+            arr[k].k_on, arr[k].k_off = float(mo["k_on"]), float(mo["k_off"])
+            arr[k].motor_reach = float(mo.get("reach", mo["max_dist"]))
+            arr[k].motor_bead = mb.ctypes.data
         out.append((pos, pos1, T, F))
     cr = np.zeros((n, n, 5))
     for (a, b), p in cross.items():
+        if isinstance(p, str) and p == "motor":
+            cr[a, b, 0] = 2.0
+            continue
         cr[a, b, 0] = 1.0
         cr[a, b, 1:] = p
     st = np.zeros(4)

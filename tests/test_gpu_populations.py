@@ -189,3 +189,80 @@ def test_mixed_precision_group_close_to_fp64():
     L = pops[0]["phys"].L
     for k in range(2):
         assert np.max(np.abs(N.min_image(got[k]["pos"] - out_o[k][0], L))) < 1e-5
+
+
+def motor_population(dim, L, n, seed, D=0.0, still=True, k_on=0.9, k_off=0.2, imparts=1.0,
+                     max_dist=0.5):
+    """Motor_Population (populations.py:617-660) as a population of single
+    beads with no forces: fixed when D = 0, else diffusing."""
+    rng = np.random.default_rng(seed)
+    phys = E.Physics(dim=dim, n_beads=n, n_chains=n, L=L, D=D, Dr=0.0, radius=1.0,
+                     wca=W.wca_params(1.0, 1.0, 0.0, 1.0), spring=np.zeros(6), v0=0.0,
+                     n_nearby=-1, has_springs=False, has_wca=False, has_propel=False,
+                     seed=seed * 7 + 1, motor=True, motor_still=still, motor_max_dist=max_dist,
+                     motor_imparts=imparts, k_on=k_on, k_off=k_off)
+    t = np.zeros((n, dim))
+    t[:, 0] = 1.0
+    return dict(phys=phys, pos=rng.uniform(0, L, (n, dim)), tangent=t,
+                chain_id=np.arange(n, dtype=np.int64), rest_lengths=np.ones(n),
+                v_tan_angle=np.zeros(n), fixed=D == 0.0)
+
+
+def run_motors(pops, laws, steps=100):
+    """Beads (population 0) and motors (population 1): the GPU against the
+    oracle, positions, where_active of the beads and the motors' bindings."""
+    dim = pops[0]["phys"].dim
+    rng = np.random.default_rng(501)
+    xis = [rng.normal(size=(steps, p["phys"].n_beads, 2 * dim)) for p in pops]
+    mdps, n_upd = W.grid_constants(0.5, 10, 1.0)
+    tv = np.array([1e-7, 1e-3, mdps, 1.0, -1.0, n_upd, n_upd, steps])
+    orc = []
+    for k, p in enumerate(pops):
+        d = dict(sysd=sysd_of(p), pos=p["pos"], tangent=p["tangent"], chain_id=p["chain_id"],
+                 rest_lengths=p["rest_lengths"], vtan=p["v_tan_angle"], xi=xis[k],
+                 fixed=p["fixed"], seed=p["phys"].seed)
+        if p["phys"].motor:
+            ph = p["phys"]
+            d["motor"] = dict(max_dist=ph.motor_max_dist, imparts=ph.motor_imparts, k_on=ph.k_on,
+                              k_off=ph.k_off, still=ph.motor_still)
+        else:
+            d["sysd"]["where_active"] = np.zeros(p["phys"].n_beads, dtype=np.int32)
+        orc.append(d)
+    out_o, st_o = O.multi_srk2_run(orc, laws, tv)
+    nb = sum(p["phys"].n_beads for p in pops)
+    pool = E.Pool(dim, E.GCR_FP64, bead_capacity=nb, cell_capacity=1 << 17, chain_capacity=nb,
+                  replica_capacity=len(pops))
+    try:
+        pool.set_schedule(32, 8)
+        rids = [pool.add(p["phys"], p["pos"], p["tangent"], p["chain_id"], p["rest_lengths"],
+                         p["v_tan_angle"]) for p in pops]
+        pool.group(rids, laws, [k for k, p in enumerate(pops) if p["fixed"]])
+        for k, r in enumerate(rids):
+            if not pops[k]["fixed"]:
+                pool.begin(r, tv, xi=xis[k])
+        pool.run()
+        got = [pool.download(r) for r in rids]
+        sts = [pool.status(r) for r in rids]
+        was = [pool.where_active(r) for r in rids]
+    finally:
+        pool.close()
+    check(pops, out_o, st_o, got, sts)
+    assert np.array_equal(was[0], orc[0]["sysd"]["where_active"])
+    assert np.array_equal(was[1], orc[1]["motor_bead"])
+    return was
+
+
+def test_fixed_motors_push_chains():
+    """Chains with v0 handed to the motors (config.py:95-98) among fixed
+    motors: bindings, unbindings and pushes as the oracle's rule."""
+    chains = make_pops(2, 14.0, [dict(n_chains=25, chain_len=6, v0=0.0, vtan=0.0)], seed=11)[0]
+    motors = motor_population(2, 14.0, 150, seed=12)
+    was = run_motors([chains, motors], {(0, 1): "motor"})
+    assert np.sum(was[1] >= 0) > 5  # motors are bound at the end
+
+
+def test_diffusing_motors_3d():
+    chains = make_pops(3, 8.0, [dict(n_chains=20, chain_len=4, v0=0.0)], seed=13)[0]
+    motors = motor_population(3, 8.0, 200, seed=14, D=0.05, max_dist=0.6)
+    was = run_motors([chains, motors], {(0, 1): "motor"})
+    assert np.sum(was[1] >= 0) > 3

@@ -709,6 +709,80 @@ def test_sliding_pair_motion_and_where_active():
     assert wa1.tolist() == [-1, 1]
 
 
+def motor_pops(bead_pos, bead_t, motor_pos, k_on, k_off, D_m=0.0, still=True, imparts=1.0,
+               max_dist=0.5, L=L2D):
+    nb, nm = len(bead_pos), len(motor_pos)
+    beads = dict(sysd=dict(dim=2, n=nb, n_nearby=4, has_springs=False, has_wca=False,
+                           has_propel=False, L=L, D=0.0, Dr=0.0, radius=1.0,
+                           wca=O.wca_params(1.0, 1.0, 0.0, 1.0), spring=np.zeros(6), v0=0.0,
+                           where_active=np.zeros(nb, dtype=np.int32)),
+                 pos=bead_pos, tangent=bead_t, chain_id=np.arange(nb), rest_lengths=np.ones(nb),
+                 vtan=np.zeros(nb), xi=np.zeros((1, nb, 4)))
+    motors = dict(sysd=dict(beads["sysd"], n=nm, D=D_m, where_active=None), pos=motor_pos,
+                  tangent=np.tile([1.0, 0.0], (nm, 1)), chain_id=np.arange(nm),
+                  rest_lengths=np.ones(nm), vtan=np.zeros(nm), fixed=D_m == 0.0, seed=11,
+                  xi=np.ones((1, nm, 4)),
+                  motor=dict(max_dist=max_dist, imparts=imparts, k_on=k_on, k_off=k_off,
+                             still=still))
+    return beads, motors
+
+
+def test_motors_bind_in_bead_order_and_push_along_the_tangent():
+    """forces.py:248-283 with k_on = 1, k_off = 0: a free motor binds to the
+    first bead (index order) within its range and pushes it by imparts * t once,
+    when it binds (a motor that stays bound gives no push: forces.py:268-269);
+    a bead out of range or later in line gets nothing."""
+    bp = np.array([[5.0, 5.0], [5.3, 5.0], [9.0, 5.0]])
+    bt = np.array([[0.0, 1.0], [1.0, 0.0], [1.0, 0.0]])
+    beads, motors = motor_pops(bp, bt, np.array([[5.15, 5.2], [9.0, 5.8]]), 1.0, 0.0,
+                               imparts=2.0)
+    tv = np.array([1e-3, 1e-3, 1.0, 1.0, -1.0, 40, 40, 1])
+    out, st = O.multi_srk2_run([beads, motors], {(0, 1): "motor"}, tv)
+    # SRK2: x + dt (F0 + F1) / 2 with F0 = 2 t (binding) and F1 = 0 (stays bound)
+    assert np.allclose(out[0][0], bp + 1e-3 * 1.0 * np.array([[0.0, 1.0], [0, 0], [0, 0]]))
+    assert motors["motor_bead"].tolist() == [0, -1]  # the far motor (0.8 away) stays free
+    assert beads["sysd"]["where_active"].tolist() == [0, -1, -1]  # -1 + one binding
+    assert np.array_equal(out[1][0], motors["pos"])  # fixed motors
+
+
+def test_motors_unbinding_moves_them_on_and_counts_against_the_bead_that_frees_them():
+    """k_off = 1: a bound motor seen by the next bead in range comes free (that
+    bead's where_active drops, whichever bead it pushed) and, with k_on = 1,
+    binds to it: both beads are pushed once per stage."""
+    bp = np.array([[5.0, 5.0], [5.3, 5.0]])
+    bt = np.array([[0.0, 1.0], [0.0, -1.0]])
+    beads, motors = motor_pops(bp, bt, np.array([[5.15, 5.0]]), 1.0, 1.0)
+    tv = np.array([1e-3, 1e-3, 1.0, 1.0, -1.0, 40, 40, 1])
+    out, _ = O.multi_srk2_run([beads, motors], {(0, 1): "motor"}, tv)
+    assert np.allclose(out[0][0], bp + 1e-3 * bt)
+    # from -1 each: stage 0, bead 0 binds (0); bead 1 frees it (-2) and binds it
+    # (-1); stage 1, bead 0 frees (-1), binds (0); bead 1 frees (-2), binds (-1)
+    assert beads["sysd"]["where_active"].tolist() == [0, -1]
+    assert motors["motor_bead"].tolist() == [1]
+
+
+def test_motor_draw_rates_and_still_rule():
+    """The binding draw is uniform: with k_on = 0.3 about 30% of free motors
+    next to a bead bind in the first stage; bound motors of a diffusing
+    population stay put (b_motors_dont_diffuse_while_pushing), free ones move
+    by sqrt(2 D dt) xi."""
+    # 2500 beads 2 apart, each with one motor 0.3 away (and 1.7 from the next)
+    g = np.arange(50) * 2.0 + 1.0
+    bp = np.array([[x, y] for y in g for x in g])
+    mp = bp + np.array([0.3, 0.0])
+    bt = np.tile([0.0, 1.0], (len(bp), 1))
+    beads, motors = motor_pops(bp, bt, mp, 0.3, 0.0, D_m=0.5, imparts=1e-3, L=100.0)
+    tv = np.array([1e-3, 1e-3, 1.0, 1.0, -1.0, 40, 40, 1])
+    out, _ = O.multi_srk2_run([beads, motors], {(0, 1): "motor"}, tv)
+    bound = motors["motor_bead"] >= 0
+    frac = bound.mean()  # bound in stage 0 or in stage 1: 1 - 0.7^2 = 0.51
+    assert abs(frac - 0.51) < 0.04
+    moved = np.abs(out[1][0] - mp).max(axis=1)
+    assert np.all(moved[bound] == 0)
+    free_step = np.sqrt(2 * 0.5 * 1e-3) * 1.0  # xi = 1 in every component
+    assert np.allclose(out[1][0][~bound], mp[~bound] + free_step)
+
+
 def test_rotational_diffusion_closed_form():
     """SRK.py:398-423: t' = normalize(t + pi sqrt(2 Dr dt) (xi_r - (xi_r.t) t))."""
     Dr = 0.2
