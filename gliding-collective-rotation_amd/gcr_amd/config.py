@@ -50,6 +50,14 @@ def configure(sim):
     Lx = sim["Lx"]
     walls = np.array([(s in sim["walls"] or sim["walls"] in ["all", "True"]) for s in "xyz"])
     n_threads = sim["n_threads"]  # cline.py:72
+    # config.py:88-100: with motors, the beads' propulsion speed is the motors' push
+    if sim.get("motor_density", 0) > 0:
+        sim["n_motors"] = int(round(sim["Lx"] * sim["Lx"] * sim["motor_density"]))
+    if sim.get("n_motors", 0) > 0:
+        sim["motor_imparts_force"] = sim["v0"]
+        sim["v0"] = 0
+    else:
+        sim["motor_imparts_force"] = 0
     np.random.seed(sim["seed"] % (2 ** 32))
 
 
@@ -64,6 +72,8 @@ def grid_list_setup(sim, pops):
             d = max(d, 2 * max(pop.rest_lengths))
         if hasattr(pop, "WCA_params"):
             d = max(d, pop.WCA_params[-1], getattr(pop, "sliding_partner_maxdist", 0.0))
+        if hasattr(pop, "motor_max_dist"):
+            d = max(d, pop.motor_max_dist)
         dists.append(d)
     sim["biggest_interaction_distance"] = max(dists)
     sim["n_grid"] = max(sim["n_grid_min"],

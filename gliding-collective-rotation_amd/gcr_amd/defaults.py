@@ -42,6 +42,13 @@ def set_defaults():
         v_tan_angle=0.0,
         b_use_tangent_as_propulsion_direction=True,
         sliding_partner_maxdist=0.0,
+        n_motors=0,
+        motor_density=0.0,
+        D_motors=0.1,
+        b_motors_dont_diffuse_while_pushing=True,
+        b_one_motor_per_bead=False,
+        k_on=0.9,
+        k_off=0.2,
         t_wait_v0=-1.0,
         seed=int(time.time()),
         D=-1.0,
@@ -56,6 +63,7 @@ def set_defaults():
         plot_every=500,
     )
     sim["dt0"] = 0.0001 * sim["dtmax"]
+    sim["motor_max_dist"] = sim["a"] / 2
     sim["spontaneous_bond_angle"] = 0.0 * np.pi
     sim["v_tan_angle"] = 0.0 * np.pi
     return sim

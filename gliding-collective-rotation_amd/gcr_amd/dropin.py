@@ -120,9 +120,32 @@ def _write_status(config, status, st):
         1, (int(steps_done) + status.update_grid_list_counter) % n_upd)
 
 
+def _motor_population(pop, config, fixed):
+    """A Motor_Population (populations.py:617-660): single beads with no
+    forces of their own, diffusing ("diffuse", SRK.py:389-396) unless fixed."""
+    dim = int(config.dim)
+    L = float(config.Lx)
+    n = int(pop.total_pop)
+    diffuse = dict(pop.self_interactions()).get("diffuse")
+    still = bool(diffuse[0]) if diffuse is not None else True
+    phys = E.Physics(dim=dim, n_beads=n, n_chains=n, L=L, D=0.0 if fixed else float(pop.D),
+                     Dr=0.0, radius=1.0, wca=np.array([0.0, 0.0, 0.0, 1.0]), spring=np.zeros(6),
+                     v0=0.0, n_nearby=-1, has_springs=False, has_wca=False, has_propel=False,
+                     skin=max(0.3, L / (512 if dim == 2 else 64) - 1.0), motor=True,
+                     motor_still=still, motor_max_dist=float(pop.motor_max_dist),
+                     motor_imparts=float(pop.imparts_force), k_on=float(pop.k_on),
+                     k_off=float(pop.k_off))
+    chain_id = np.arange(n, dtype=np.int64)
+    sig = ("motor", n, dim, L, phys.D, still, phys.motor_max_dist, phys.motor_imparts,
+           phys.k_on, phys.k_off, fixed)
+    return phys, np.zeros(n), np.ones(n), chain_id, sig
+
+
 def _population(pop, config, status, order, fixed=False):
     """Physics, per-chain chirality, rest lengths, chain ids and the
     signature of one population, from its force list (SRK.py:46-97)."""
+    if hasattr(pop, "k_on") and hasattr(pop, "motor_max_dist"):
+        return _motor_population(pop, config, fixed)
     dim = int(config.dim)
     L = float(config.Lx)
     n = int(pop.total_pop)
@@ -210,22 +233,26 @@ def _population(pop, config, status, order, fixed=False):
 
 def _cross_laws(pops, moving):
     """interpop_forces(P, Q) for every moving P and every other Q: the WCA law
-    of Q acting on P (updates.py:274-312), or nothing."""
+    of Q acting on P, or "motor" when Q's motors push P (updates.py:274-312)."""
     laws = {}
     for a, P in enumerate(pops):
-        if not moving[a]:
-            continue
+        if not moving[a] or not hasattr(P, "forces"):
+            continue  # SRK2_force_calc gives populations without forces only themselves
         growing = bool(getattr(P, "b_init_growing", False))
         for b, Q in enumerate(pops):
             if b == a:
                 continue
             fl = _force_list(P, growing or bool(getattr(Q, "b_init_growing", False)), Q)
-            unknown = [f for f in fl if f != "WCA"]
+            unknown = [f for f in fl if f not in ("WCA", "motor_push")]
             if unknown:
                 raise NotImplementedError(
                     f"GPU path: forces {unknown} between populations are not ported")
+            if "WCA" in fl and "motor_push" in fl:
+                raise NotImplementedError("GPU path: WCA and motor_push between one pair")
             if "WCA" in fl:
                 laws[(a, b)] = np.asarray(fl["WCA"][:4], dtype=np.float64)
+            if "motor_push" in fl:
+                laws[(a, b)] = "motor"
     return laws
 
 
@@ -283,14 +310,14 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
     def several_populations(pops, status, sim, order):
         if order != 2:
             raise NotImplementedError("GPU path: several populations with SRK_order 2 only")
-        moving = [not p.fixed and hasattr(p, "forces") for p in pops]
+        moving = [not p.fixed for p in pops]
         parts = [_population(p, config, status, order, fixed=not moving[k])
                  for k, p in enumerate(pops)]
         if any(ph.rod != 0.0 and moving[k] for k, (ph, *_rest) in enumerate(parts)):
             raise NotImplementedError("GPU path: the rigid rod with several populations")
         laws = _cross_laws(pops, moving)
         sig = (tuple(pt[4] for pt in parts), tuple(moving),
-               tuple(sorted((k, tuple(v)) for k, v in laws.items())))
+               tuple(sorted((k, v if isinstance(v, str) else tuple(v)) for k, v in laws.items())))
         key = tuple(id(p) for p in pops)
         G = resident.setdefault(key, _Group())
         tangents = [getattr(p, "tangent_list", None) for p in pops]
@@ -336,15 +363,19 @@ def make_SRK2_force_calc(config, precision=E.GCR_FP64, seed=None):
         st = None
         for k, p in enumerate(pops):
             if not moving[k]:
+                if parts[k][0].motor:  # a fixed motor population's bindings persist
+                    p.where_active[:] = G.pool.where_active(G.rids[k])
                 continue
             out = G.pool.download(G.rids[k])
             p.pos[:] = out["pos"]
             p.pos1[:] = out["pos1"]
             if tangents[k] is not None:
                 p.tangent_list[:] = out["tangent"]
-            p.forces[:] = out["forces"]
-            p.where_active[:] = (G.pool.where_active(G.rids[k]) if parts[k][0].sliding_maxdist > 0
-                                 else -1)  # SRK.py:132, forces.py:119
+            if hasattr(p, "forces"):
+                p.forces[:] = out["forces"]
+            # SRK.py:132, forces.py:119, 262-281: -1 plus sliding partners and
+            # motor bindings; for motors, the bead each one pushes
+            p.where_active[:] = G.pool.where_active(G.rids[k])
             if st is None:
                 s_ = G.pool.status(G.rids[k])
                 st = np.array([s_["dt"], s_["elapsed"], s_["steps"], s_["max_forces_strength"]])

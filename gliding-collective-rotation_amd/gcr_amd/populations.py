@@ -71,6 +71,43 @@ class Population:
             [force_name, make_WCA_replacements(self, replacements)])
 
 
+class Motor_Population(Population):
+    """Motors (populations.py:617-660): points that bind to and push the beads
+    of the other populations (their interactions() name "motor_push")."""
+
+    def __init__(self, sim, num_motors=-1, name="motors"):
+        n = sim["n_motors"] if num_motors < 0 else num_motors
+        super().__init__(n, sim, name=name)
+        self.D = sim["D_motors"]
+        self.motor_max_dist = sim["motor_max_dist"]
+        self.motor_max_dist_sq = self.motor_max_dist ** 2
+        self.b_one_motor_per_bead = sim["b_one_motor_per_bead"]
+        self.b_motors_dont_diffuse_while_pushing = sim["b_motors_dont_diffuse_while_pushing"]
+        self.imparts_force = sim["motor_imparts_force"]
+        self.k_on = sim["k_on"]
+        self.k_off = sim["k_off"]
+        self.where_active = np.full(n, -1)
+        if self.D == 0:
+            self.fixed = True
+
+    def self_interactions(self):
+        if self.D > 0:
+            return [["diffuse", np.array([float(self.b_motors_dont_diffuse_while_pushing)])]]
+        return []
+
+    def size_scale(self):
+        return self.motor_max_dist
+
+    def random_initialization(self, seed=0):
+        """populations.py:130-132 (uniform about the box centre), from a seed."""
+        rng = np.random.RandomState(seed)
+        self.pos[:] = np.mod(config.Lx / 2 * (1.0 + self.pos_init_noise
+                                              * (2 * rng.rand(self.total_pop, config.dim) - 1)),
+                             config.Lx)
+        self.pos1[:] = self.pos
+        self.initialized = True
+
+
 class WCA_Population(Population):
     """Beads with a WCA/LJ pair force (populations.py:143-241)."""
 
@@ -117,6 +154,15 @@ class Oriented_WCA_Population(WCA_Population):
         self.tangent_list = np.empty((total_pop, config.dim))
         self.tangent_list[:] = 0.0
         self.tangent_list[:, 0] = 1.0
+
+    def interactions(self, pop):
+        """populations.py:276-287: motors push these beads."""
+        flist = super().interactions(pop)
+        if isinstance(pop, Motor_Population):
+            flist = flist + [["motor_push", np.array([
+                pop.motor_max_dist_sq, float(pop.b_one_motor_per_bead), pop.imparts_force,
+                pop.k_on, pop.k_off])]]
+        return flist
 
 
 class Chain_Collection(Oriented_WCA_Population):

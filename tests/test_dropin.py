@@ -247,6 +247,66 @@ def test_dropin_active_sliding_matches_oracle():
     assert np.array_equal(pop.where_active, wa)
 
 
+def chains_and_motors(D_motors):
+    """BDsim.py:24-25: chains plus a Motor_Population when n_motors > 0; the
+    beads' v0 is handed to the motors (config.py:95-98)."""
+    sim = defaults.set_defaults()
+    sim.update(dict(n_chains=20, chain_length=6, Lx=14.0, kappa=0.0, T=0.01, dtmax=1e-3,
+                    seed=6, t_init_grow=0.0, save_state_every=-1.0, n_motors=150,
+                    D_motors=D_motors, motor_max_dist=0.5))
+    sim["dt0"] = 1e-4 * sim["dtmax"]
+    config.configure(sim)
+    chains = populations.Equal_Size_Chain_Collection(sim)
+    chains.random_initialization(seed=3)
+    motors = populations.Motor_Population(sim)
+    motors.random_initialization(seed=4)
+    config.grid_list_setup(sim, [chains, motors])
+    status = config.status_class(sim, start_t=0.0)
+    return sim, [chains, motors], status
+
+
+def test_dropin_motor_laws():
+    sim, pops, status = chains_and_motors(0.0)
+    assert pops[0].v0 == 0 and sim["motor_imparts_force"] == 1.0
+    assert pops[1].fixed and pops[1].self_interactions() == []
+    assert dropin._cross_laws(pops, [True, False]) == {(0, 1): "motor"}
+    push = dict(pops[0].interactions(pops[1]))["motor_push"]
+    assert push.tolist() == [0.25, 0.0, 1.0, 0.9, 0.2]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D_motors", [0.0, 0.05])
+def test_dropin_motors_match_oracle(D_motors):
+    sim, pops, status = chains_and_motors(D_motors)
+    chains, motors = pops
+    x0, t0, m0 = chains.pos.copy(), chains.tangent_list.copy(), motors.pos.copy()
+    f = dropin.make_SRK2_force_calc(config)
+    status.next_output_step = 40
+    f(pops, status, sim)
+    assert status.stepnum == 40
+    wa = np.zeros(chains.total_pop, dtype=np.int32)
+    beads = dict(sysd=dict(dim=2, n=chains.total_pop, n_nearby=4, has_springs=True,
+                           has_wca=True, has_propel=True, L=14.0, D=chains.D, Dr=chains.Dr,
+                           radius=1.0, wca=chains.WCA_params, spring=chains.spring_params(),
+                           v0=0.0, where_active=wa),
+                 pos=x0, tangent=t0, chain_id=chains.chain_id, rest_lengths=chains.rest_lengths,
+                 vtan=np.zeros(20), seed=dropin._mix(sim["seed"], 64))
+    mot = dict(sysd=dict(beads["sysd"], n=150, has_springs=False, has_wca=False,
+                         has_propel=False, D=D_motors, Dr=0.0, where_active=None),
+               pos=m0, tangent=np.tile([1.0, 0.0], (150, 1)), chain_id=np.arange(150),
+               rest_lengths=np.ones(150), vtan=np.zeros(150), fixed=D_motors == 0.0,
+               seed=dropin._mix(sim["seed"], 65),
+               motor=dict(max_dist=0.5, imparts=1.0, k_on=0.9, k_off=0.2, still=True))
+    tv = np.array([sim["dt0"], sim["dtmax"], config.max_dist_per_step, 1.0, -1,
+                   config.n_steps_update_grid_list, config.n_steps_update_grid_list, 40])
+    out, st = O.multi_srk2_run([beads, mot], {(0, 1): "motor"}, tv)
+    assert np.max(np.abs(N.min_image(chains.pos - out[0][0], 14.0))) < 1e-10 * 14.0
+    assert np.max(np.abs(N.min_image(motors.pos - out[1][0], 14.0))) < 1e-10 * 14.0
+    assert np.array_equal(chains.where_active, wa)
+    assert np.array_equal(motors.where_active, mot["motor_bead"])
+    assert np.sum(motors.where_active >= 0) > 3
+
+
 @pytest.mark.gpu
 def test_dropin_sees_host_edits():
     """Positions edited on the host between calls are uploaded again."""
