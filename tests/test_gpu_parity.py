@@ -378,3 +378,24 @@ def test_mixed_precision_force_error():
     err = np.max(np.abs(F_m - F_o)) / scale
     print(f"mixed-precision max force error / max force = {err:.3e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("name", ["fene", "srk4_chains", "walls_chains", "walls_srk4", "rod2d",
+                                  "srk4_abp3d", "walls3d"])
+def test_mixed_precision_trajectories_of_each_feature(name):
+    """The mixed form of the general kernels (SRK4, walls, FENE, rod; 2D tiles
+    and 3D) follows the FP64 oracle over 20 steps with injected noise to the
+    accuracy of FP32 pair forces."""
+    w = CASES[name]
+    print(f"[mixed trajectory] {name}", flush=True)
+    rng = np.random.default_rng(101)
+    xi = rng.normal(size=(20, w.n, 2 * w.phys.dim))
+    tv = w.time_vals.copy()
+    tv[7] = 20
+    pos_o, *_ = oracle_run(w, xi, tv)
+    s = gpu_system(w, precision=E.GCR_MIXED)
+    s.run(tv, xi=xi)
+    out = s.download()
+    s.close()
+    L = w.phys.L
+    assert pos_err(out["pos"], pos_o, L) <= 1e-6 * L

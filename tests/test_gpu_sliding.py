@@ -99,3 +99,26 @@ def test_sliding_through_replays():
                          seed=33, skin=0.05))
     print(f"[sliding] replay n={w.n}", flush=True)
     compare(w, steps=64, schedule=(32, 32))
+
+
+def test_sliding_mixed_precision_follows_fp64():
+    w = sliding(W.chains("slide_mixed", 2, 120, 8, phi=0.35, kappa=6.0, T=0.01, steps=20,
+                         seed=34))
+    rng = np.random.default_rng(702)
+    xi = rng.normal(size=(20, w.n, 4))
+    tv = w.time_vals.copy()
+    tv[7] = 20
+    sysd, _ = oracle_sys(w)
+    pos_o, *_ = O.srk2_run(sysd, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                           tv, xi=xi)
+    pool = E.Pool(2, E.GCR_MIXED, bead_capacity=w.n, cell_capacity=1 << 17,
+                  chain_capacity=w.phys.n_chains, replica_capacity=1)
+    try:
+        rid = pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+        pool.set_chunks(rid, w.chunks)
+        pool.begin(rid, tv, xi=xi)
+        pool.run()
+        out = pool.download(rid)
+    finally:
+        pool.close()
+    assert np.max(np.abs(N.min_image(out["pos"] - pos_o, w.phys.L))) <= 1e-6 * w.phys.L
