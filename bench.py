@@ -52,7 +52,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline budget: steps are added in chunks of 40 until this much time")
     ap.add_argument("--kernel-steps", type=int, default=32)
-    ap.add_argument("--mode", choices=("single", "ensemble", "serve", "copy"), default="single",
+    ap.add_argument("--mode", choices=("single", "ensemble", "serve", "copy", "features"),
+                    default="single",
                     help="single: 400k-bead system (the headline line); ensemble: 256 "
                          "replicas batched vs one at a time; serve: request stream; copy: "
                          "a plain 1 GiB device copy kernel, the reference point for HBM "
@@ -188,6 +189,138 @@ def main_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def time_pool(reps, steps, warmup, precision, dim=2, group=None, chunks=None, batch=64,
+              rebuild=32, tile=8, threads=256):
+    """Bead-steps/s of the replicas `reps` (Workloads) advanced together in one
+    pool (a group when `group` = (laws, fixed indices) is given), timed over
+    `steps` steps after `warmup`; the replays the run took."""
+    from gcr_amd import engine as E
+    nb = sum(w.n for w in reps)
+    cells = sum(E.replica_cells(w.phys) for w in reps)
+    pool = E.Pool(dim, precision, bead_capacity=nb, cell_capacity=cells + len(reps),
+                  chain_capacity=sum(w.phys.n_chains for w in reps),
+                  replica_capacity=len(reps))
+    try:
+        pool.set_schedule(batch, rebuild)
+        pool.set_tile(tile)
+        pool.set_tile_threads(threads)
+        rids = [pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+                for w in reps]
+        fixed = set()
+        if group is not None:
+            laws, fixed = group
+            pool.group(rids, laws, sorted(fixed))
+        for k, c in (chunks or {}).items():
+            pool.set_chunks(rids[k], c)
+        moving = [k for k in range(len(reps)) if k not in fixed]
+        tv = reps[moving[0]].time_vals.copy()
+        tv[7] = warmup
+        for k in moving:
+            pool.begin(rids[k], tv)
+        pool.run()
+        tv[0] = pool.status(rids[moving[0]])["dt"]
+        tv[7] = steps
+        for k in moving:
+            pool.begin(rids[k], tv)
+        t0 = time.perf_counter()
+        pool.run()
+        sec = time.perf_counter() - t0
+        sts = [pool.status(rids[k]) for k in moving]
+        assert all(s["steps"] == steps for s in sts), sts
+        moved = sum(reps[k].n for k in moving)
+        return dict(bead_steps_per_s=moved * steps / sec, ms_per_step=1e3 * sec / steps,
+                    beads=moved, replays=int(sum(s["replays"] for s in sts)))
+    finally:
+        pool.close()
+
+
+def main_features(args):
+    """Throughput of each option of the step on the 400k-bead system (mixed
+    precision, the tiling the autotuner picks for the plain step): SRK4,
+    FENE springs, walls, the rigid rod, active sliding, two interacting
+    populations, chains with motors, and the growth phase on 40k beads."""
+    import copy
+    from gcr_amd import dropin
+    from gcr_amd import engine as E
+    from gcr_amd import grow as G
+    from gcr_amd import workloads as W
+    prec = E.GCR_MIXED
+    base = W.config3(seed=1, n_chains=args.chains)
+    base.phys.skin = 0.3 * float(base.phys.wca[3])
+    steps, warm = args.steps, args.warmup
+    out = {}
+
+    def variant(**over):
+        w = copy.deepcopy(base)
+        for k, v in over.items():
+            setattr(w.phys, k, v)
+        return w
+
+    t0 = time.perf_counter()
+    out["srk2 (reference step)"] = time_pool([base], steps, warm, prec)
+    out["srk4"] = time_pool([variant(srk_order=4)], steps, warm, prec)
+    w = variant()
+    w.phys.spring = w.phys.spring.copy()
+    w.phys.spring[3], w.phys.spring[5] = 1.0, 0.3
+    out["fene"] = time_pool([w], steps, warm, prec)
+    out["walls (x and y)"] = time_pool([W.walled(base, 3, 1.25)], steps, warm, prec)
+    out["rigid rod 0.6"] = time_pool([variant(rod=0.6)], steps, warm, prec)
+    w = variant(has_propel=False, sliding_maxdist=1.5, sliding_v0=1.0)
+    out["active sliding"] = time_pool([w], steps, warm, prec,
+                                      chunks={0: dropin.chain_chunks(3, w.chain_id)})
+    # the same beads as two populations of 10k chains acting on each other
+    # with their own law (the union of the reference step)
+    half = args.chains // 2
+    parts = []
+    for lo, hi in ((0, half), (half, args.chains)):
+        w = copy.deepcopy(base)
+        sl = slice(lo * 20, hi * 20)
+        w.pos, w.tangent = base.pos[sl].copy(), base.tangent[sl].copy()
+        w.chain_id = np.repeat(np.arange(hi - lo, dtype=np.int64), 20)
+        w.rest_lengths = base.rest_lengths[lo:hi].copy()
+        w.v_tan_angle = base.v_tan_angle[lo:hi].copy()
+        w.v0_per_chain = base.v0_per_chain[lo:hi].copy()
+        w.phys.n_chains, w.phys.n_beads = hi - lo, (hi - lo) * 20
+        w.phys.seed = 100 + lo
+        parts.append(w)
+    law = np.asarray(base.phys.wca)
+    out["two populations"] = time_pool(parts, steps, warm, prec,
+                                       group=({(0, 1): law, (1, 0): law}, set()))
+    # chains whose v0 is the motors' push (config.py:95-98) among 100k fixed motors
+    chains = variant(v0=0.0)
+    rng = np.random.default_rng(5)
+    nm = 100000
+    mot = copy.deepcopy(base)
+    mot.pos = rng.uniform(0, base.phys.L, (nm, 2))
+    mot.tangent = np.tile([1.0, 0.0], (nm, 1))
+    mot.chain_id = np.arange(nm, dtype=np.int64)
+    mot.rest_lengths, mot.v_tan_angle, mot.v0_per_chain = np.ones(nm), np.zeros(nm), np.zeros(nm)
+    mot.phys = E.Physics(dim=2, n_beads=nm, n_chains=nm, L=base.phys.L, D=0.0, Dr=0.0,
+                         radius=1.0, wca=np.array([0.0, 0.0, 0.0, 1.0]), spring=np.zeros(6),
+                         v0=0.0, n_nearby=-1, has_springs=False, has_wca=False,
+                         has_propel=False, skin=0.3, motor=True, motor_max_dist=0.25,
+                         motor_imparts=1.0, k_on=0.9, k_off=0.2, seed=9)
+    out["chains + 100k fixed motors"] = time_pool([chains, mot], steps, warm, prec,
+                                                  group=({(0, 1): "motor"}, {1}))
+    # the growth schedule (control.py:153-219) for 2000 chains of 20 from points
+    g = W.config3(seed=2, n_chains=2000)
+    pos0, t0_, cid = G.random_initialization(2000, 20, 0.5, g.phys.L, 2, seed=3)
+    tg = time.perf_counter()
+    _, _, info = G.grow(g.phys, pos0, t0_, cid, 0.5, 20, g.time_vals, t_init_grow=0.2,
+                        growing_stepskip=10)
+    sec = time.perf_counter() - tg
+    out["growth, 2000 chains x 20 (FP64)"] = dict(
+        bead_steps_per_s=g.n * info["steps"] / sec, ms_per_step=1e3 * sec / info["steps"],
+        beads=g.n, steps=info["steps"], chunks=info["chunks"])
+    res = {"metric": "bead-steps/sec of each step option (1 GPU)", "unit": "bead-steps/s",
+           "dtype": "f32 forces + f64 state", "data": "synthetic", "steps": steps,
+           "warmup": warm, "config": {"workload": "config 3 (400k beads) and its variants",
+                                      "skin": base.phys.skin, "rebuild_every": 32, "tile": 8,
+                                      "threads": 256},
+           "features": out, "wall_s": time.perf_counter() - t0}
+    print(json.dumps(res), flush=True)
+
+
 def main_copy():
     """dst = 1 * src over 1 GiB of float64, three times: one elementwise
     kernel that reads 1 GiB and writes 1 GiB per launch. Counter readings of
@@ -214,6 +347,8 @@ def main():
         return main_ensemble(args)
     if args.mode == "serve":
         return main_serve(args)
+    if args.mode == "features":
+        return main_features(args)
     from gcr_amd import engine as E
     from gcr_amd import workloads as W
     from gcr_amd._lib import device_info

@@ -83,6 +83,34 @@ def chains(name, dim, n_chains, chain_len, *, phi=None, L=None, a=0.5, r=1.0, ep
                     np.full(n_chains, v0), tv, steps)
 
 
+def walled(w, walls, zone, margin=0.8):
+    """The workload moved inside WCA walls (forces.py:405-435): the chains that
+    do not cross the periodic faces are kept, unchanged, and the box grows by
+    zone + margin on each side, so the beads nearest the faces start `margin`
+    from a wall."""
+    import copy
+    L = w.phys.L
+    m = w.n // w.phys.n_chains
+    pos = w.pos.reshape(w.phys.n_chains, m, -1)
+    keep = np.all(np.abs(np.diff(pos, axis=1)) < L / 2, axis=(1, 2))
+    keep = np.nonzero(keep)[0]
+    pad = zone + margin
+    out = copy.deepcopy(w)
+    p = out.phys
+    p.L = L + 2 * pad
+    p.n_chains = len(keep)
+    p.n_beads = len(keep) * m
+    p.walls = walls
+    p.wall_zone = zone
+    out.pos = (pos[keep] + pad).reshape(-1, pos.shape[2])
+    out.tangent = w.tangent.reshape(w.phys.n_chains, m, -1)[keep].reshape(-1, pos.shape[2])
+    out.chain_id = np.repeat(np.arange(len(keep), dtype=np.int64), m)
+    out.rest_lengths = w.rest_lengths[keep]
+    out.v_tan_angle = w.v_tan_angle[keep]
+    out.v0_per_chain = w.v0_per_chain[keep]
+    return out
+
+
 def config1(seed=0, steps=10000):
     """2D box 20, 4 chains x 7 beads, WCA, k=100, no bending, v0=1, T=0.01, dt=1e-3."""
     return chains("config1", 2, 4, 7, L=20.0, kappa=0.0, T=0.01, dtmax=1e-3, steps=steps,

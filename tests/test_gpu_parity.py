@@ -27,32 +27,7 @@ def sysd_of(w):
                 rod=p.rod, order=p.srk_order, walls=p.walls, wall_zone=p.wall_zone)
 
 
-def walled(w, walls, zone, margin=0.8):
-    """The workload moved inside WCA walls (forces.py:405-435): the chains that
-    do not cross the periodic faces are kept, unchanged, and the box grows by
-    zone + margin on each side, so the beads nearest the faces start `margin`
-    from a wall, inside its range."""
-    import copy
-    L = w.phys.L
-    m = w.n // w.phys.n_chains
-    pos = w.pos.reshape(w.phys.n_chains, m, -1)
-    keep = [c for c in range(w.phys.n_chains)
-            if np.all(np.abs(np.diff(pos[c], axis=0)) < L / 2)]
-    pad = zone + margin
-    out = copy.deepcopy(w)
-    p = out.phys
-    p.L = L + 2 * pad
-    p.n_chains = len(keep)
-    p.n_beads = len(keep) * m
-    p.walls = walls
-    p.wall_zone = zone
-    out.pos = (pos[keep] + pad).reshape(-1, pos.shape[2])
-    out.tangent = w.tangent.reshape(w.phys.n_chains, m, -1)[keep].reshape(-1, pos.shape[2])
-    out.chain_id = np.repeat(np.arange(len(keep), dtype=np.int64), m)
-    out.rest_lengths = w.rest_lengths[keep]
-    out.v_tan_angle = w.v_tan_angle[keep]
-    out.v0_per_chain = w.v0_per_chain[keep]
-    return out
+walled = W.walled
 
 
 def cases():
