@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--steps-scale", type=float, default=0.25,
                     help="serve: multiplies every request's step count")
     ap.add_argument("--budget-gb", type=float, default=16.0)
+    ap.add_argument("--only", default="",
+                    help="features mode: run only the options whose name contains this")
     return ap.parse_args()
 
 
@@ -256,62 +258,84 @@ def main_features(args):
             setattr(w.phys, k, v)
         return w
 
+    def walls():
+        return time_pool([W.walled(base, 3, 1.25)], steps, warm, prec)
+
+    def fene():
+        w = variant()
+        w.phys.spring = w.phys.spring.copy()
+        w.phys.spring[3], w.phys.spring[5] = 1.0, 0.3
+        return time_pool([w], steps, warm, prec)
+
+    def sliding():
+        w = variant(has_propel=False, sliding_maxdist=1.5, sliding_v0=1.0)
+        return time_pool([w], steps, warm, prec, chunks={0: dropin.chain_chunks(3, w.chain_id)})
+
+    def two_pops():
+        # the same beads as two populations of 10k chains acting on each
+        # other with their own law (the union of the reference step)
+        half = args.chains // 2
+        parts = []
+        for lo, hi in ((0, half), (half, args.chains)):
+            w = copy.deepcopy(base)
+            sl = slice(lo * 20, hi * 20)
+            w.pos, w.tangent = base.pos[sl].copy(), base.tangent[sl].copy()
+            w.chain_id = np.repeat(np.arange(hi - lo, dtype=np.int64), 20)
+            w.rest_lengths = base.rest_lengths[lo:hi].copy()
+            w.v_tan_angle = base.v_tan_angle[lo:hi].copy()
+            w.v0_per_chain = base.v0_per_chain[lo:hi].copy()
+            w.phys.n_chains, w.phys.n_beads = hi - lo, (hi - lo) * 20
+            w.phys.seed = 100 + lo
+            parts.append(w)
+        law = np.asarray(base.phys.wca)
+        return time_pool(parts, steps, warm, prec, group=({(0, 1): law, (1, 0): law}, set()))
+
+    def motors():
+        # chains whose v0 is the motors' push (config.py:95-98) among 100k fixed motors
+        chains = variant(v0=0.0)
+        rng = np.random.default_rng(5)
+        nm = 100000
+        mot = copy.deepcopy(base)
+        mot.pos = rng.uniform(0, base.phys.L, (nm, 2))
+        mot.tangent = np.tile([1.0, 0.0], (nm, 1))
+        mot.chain_id = np.arange(nm, dtype=np.int64)
+        mot.rest_lengths = np.ones(nm)
+        mot.v_tan_angle, mot.v0_per_chain = np.zeros(nm), np.zeros(nm)
+        mot.phys = E.Physics(dim=2, n_beads=nm, n_chains=nm, L=base.phys.L, D=0.0, Dr=0.0,
+                             radius=1.0, wca=np.array([0.0, 0.0, 0.0, 1.0]),
+                             spring=np.zeros(6), v0=0.0, n_nearby=-1, has_springs=False,
+                             has_wca=False, has_propel=False, skin=0.3, motor=True,
+                             motor_max_dist=0.25, motor_imparts=1.0, k_on=0.9, k_off=0.2,
+                             seed=9)
+        return time_pool([chains, mot], steps, warm, prec, group=({(0, 1): "motor"}, {1}))
+
+    def growth():
+        # the growth schedule (control.py:153-219) for 2000 chains of 20 from points
+        g = W.config3(seed=2, n_chains=2000)
+        pos0, t0_, cid = G.random_initialization(2000, 20, 0.5, g.phys.L, 2, seed=3)
+        tg = time.perf_counter()
+        _, _, info = G.grow(g.phys, pos0, t0_, cid, 0.5, 20, g.time_vals, t_init_grow=0.2,
+                            growing_stepskip=10)
+        sec = time.perf_counter() - tg
+        return dict(bead_steps_per_s=g.n * info["steps"] / sec,
+                    ms_per_step=1e3 * sec / info["steps"], beads=g.n, steps=info["steps"],
+                    chunks=info["chunks"])
+
+    options = {
+        "srk2 (reference step)": lambda: time_pool([base], steps, warm, prec),
+        "srk4": lambda: time_pool([variant(srk_order=4)], steps, warm, prec),
+        "fene": fene,
+        "walls (x and y)": walls,
+        "rigid rod 0.6": lambda: time_pool([variant(rod=0.6)], steps, warm, prec),
+        "active sliding": sliding,
+        "two populations": two_pops,
+        "chains + 100k fixed motors": motors,
+        "growth, 2000 chains x 20 (FP64)": growth,
+    }
     t0 = time.perf_counter()
-    out["srk2 (reference step)"] = time_pool([base], steps, warm, prec)
-    out["srk4"] = time_pool([variant(srk_order=4)], steps, warm, prec)
-    w = variant()
-    w.phys.spring = w.phys.spring.copy()
-    w.phys.spring[3], w.phys.spring[5] = 1.0, 0.3
-    out["fene"] = time_pool([w], steps, warm, prec)
-    out["walls (x and y)"] = time_pool([W.walled(base, 3, 1.25)], steps, warm, prec)
-    out["rigid rod 0.6"] = time_pool([variant(rod=0.6)], steps, warm, prec)
-    w = variant(has_propel=False, sliding_maxdist=1.5, sliding_v0=1.0)
-    out["active sliding"] = time_pool([w], steps, warm, prec,
-                                      chunks={0: dropin.chain_chunks(3, w.chain_id)})
-    # the same beads as two populations of 10k chains acting on each other
-    # with their own law (the union of the reference step)
-    half = args.chains // 2
-    parts = []
-    for lo, hi in ((0, half), (half, args.chains)):
-        w = copy.deepcopy(base)
-        sl = slice(lo * 20, hi * 20)
-        w.pos, w.tangent = base.pos[sl].copy(), base.tangent[sl].copy()
-        w.chain_id = np.repeat(np.arange(hi - lo, dtype=np.int64), 20)
-        w.rest_lengths = base.rest_lengths[lo:hi].copy()
-        w.v_tan_angle = base.v_tan_angle[lo:hi].copy()
-        w.v0_per_chain = base.v0_per_chain[lo:hi].copy()
-        w.phys.n_chains, w.phys.n_beads = hi - lo, (hi - lo) * 20
-        w.phys.seed = 100 + lo
-        parts.append(w)
-    law = np.asarray(base.phys.wca)
-    out["two populations"] = time_pool(parts, steps, warm, prec,
-                                       group=({(0, 1): law, (1, 0): law}, set()))
-    # chains whose v0 is the motors' push (config.py:95-98) among 100k fixed motors
-    chains = variant(v0=0.0)
-    rng = np.random.default_rng(5)
-    nm = 100000
-    mot = copy.deepcopy(base)
-    mot.pos = rng.uniform(0, base.phys.L, (nm, 2))
-    mot.tangent = np.tile([1.0, 0.0], (nm, 1))
-    mot.chain_id = np.arange(nm, dtype=np.int64)
-    mot.rest_lengths, mot.v_tan_angle, mot.v0_per_chain = np.ones(nm), np.zeros(nm), np.zeros(nm)
-    mot.phys = E.Physics(dim=2, n_beads=nm, n_chains=nm, L=base.phys.L, D=0.0, Dr=0.0,
-                         radius=1.0, wca=np.array([0.0, 0.0, 0.0, 1.0]), spring=np.zeros(6),
-                         v0=0.0, n_nearby=-1, has_springs=False, has_wca=False,
-                         has_propel=False, skin=0.3, motor=True, motor_max_dist=0.25,
-                         motor_imparts=1.0, k_on=0.9, k_off=0.2, seed=9)
-    out["chains + 100k fixed motors"] = time_pool([chains, mot], steps, warm, prec,
-                                                  group=({(0, 1): "motor"}, {1}))
-    # the growth schedule (control.py:153-219) for 2000 chains of 20 from points
-    g = W.config3(seed=2, n_chains=2000)
-    pos0, t0_, cid = G.random_initialization(2000, 20, 0.5, g.phys.L, 2, seed=3)
-    tg = time.perf_counter()
-    _, _, info = G.grow(g.phys, pos0, t0_, cid, 0.5, 20, g.time_vals, t_init_grow=0.2,
-                        growing_stepskip=10)
-    sec = time.perf_counter() - tg
-    out["growth, 2000 chains x 20 (FP64)"] = dict(
-        bead_steps_per_s=g.n * info["steps"] / sec, ms_per_step=1e3 * sec / info["steps"],
-        beads=g.n, steps=info["steps"], chunks=info["chunks"])
+    for name, run in options.items():
+        if args.only in name:
+            out[name] = run()
     res = {"metric": "bead-steps/sec of each step option (1 GPU)", "unit": "bead-steps/s",
            "dtype": "f32 forces + f64 state", "data": "synthetic", "steps": steps,
            "warmup": warm, "config": {"workload": "config 3 (400k beads) and its variants",

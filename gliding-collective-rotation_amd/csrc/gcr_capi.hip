@@ -89,6 +89,21 @@ int64_t cells_per_side(double L, double rc, double skin)
 
 double default_skin(const gcr_replica_desc* d) { return d->skin > 0 ? d->skin : 0.3 * d->wca[3]; }
 
+// Cells per side of a replica's grid. With a pair force the edge is at least
+// the cutoff plus the skin. Without one the replica's own forces scan no
+// cells, and the scans of other populations (cross forces, motors, which add
+// their reach to the edge) and the re-sort's ranking within a cell want few
+// beads per cell: an edge of about four beads per cell on average.
+int64_t grid_side(int32_t dim, const gcr_replica_desc* d)
+{
+    if (d->has_wca) return cells_per_side(d->L, d->wca[3], default_skin(d));
+    if (d->n_beads <= 8 || !(d->L > 0)) return 1;
+    const double h = std::pow(4.0 * std::pow(d->L, dim) / (double)d->n_beads, 1.0 / dim);
+    const int64_t cap = dim == 3 ? 512 : 16384;
+    const int64_t nc = (int64_t)std::floor(d->L / h);
+    return nc < 1 ? 1 : (nc > cap ? cap : nc);
+}
+
 }  // namespace
 
 struct gcr_pool {
@@ -149,6 +164,7 @@ struct gcr_pool {
     bool has_group = false;  // a group exists or a replica has active sliding: the
                              // second half of each force stage runs (k_group_stage)
     bool has_motor = false;  // a group has motors: k_motor runs in each force stage
+    std::vector<int> motor_reps;  // the motor populations k_motor runs over
 };
 
 namespace {
@@ -230,7 +246,9 @@ std::vector<int> xcd_order(int first, int n)
 void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
 {
     P.n_nearby = d->n_nearby;
-    P.skin_half = d->has_wca ? 0.5 * (P.h - d->wca[3]) : 1e300;
+    // without a pair force the half skin is half the cell edge: what the
+    // scans of other populations add to their reach (none on a single cell)
+    P.skin_half = d->has_wca ? 0.5 * (P.h - d->wca[3]) : (P.nc > 1 ? 0.5 * P.h : 1e300);
     for (int i = 0; i < 4; ++i) P.wca[i] = d->wca[i];
     P.rc2 = d->wca[3] * d->wca[3];
     P.k = d->spring[0];
@@ -285,6 +303,7 @@ int rebuild_tables(gcr_pool* p)
     std::vector<XPair> xp;
     std::vector<int32_t> grp;
     bool has_motor = false;
+    std::vector<int> motor_reps;
     for (auto& P : p->P) {
         P.group = -1;
         P.motor_target = -1;
@@ -317,6 +336,8 @@ int rebuild_tables(gcr_pool* p)
                 if (e.kind == 2) {  // motors of Q push this population
                     Q.motor_target = G.rids[k];
                     has_motor = true;
+                    if (std::find(motor_reps.begin(), motor_reps.end(), G.rids[o]) == motor_reps.end())
+                        motor_reps.push_back(G.rids[o]);
                 }
                 const double reach = c[4] + (Q.skin_half < Q.L ? Q.skin_half : Q.L);
                 e.window = std::max(1, (int)std::ceil(reach / Q.h));
@@ -401,6 +422,7 @@ int rebuild_tables(gcr_pool* p)
     p->tile_gen = tile_gen;
     p->has_group = !p->groups.empty();
     p->has_motor = has_motor;
+    p->motor_reps = motor_reps;
     for (int r = 0; r < p->capR; ++r)
         if (p->live[r] && (p->P[r].flags & GCR_F_SLIDE)) p->has_group = true;
     if (!xp.empty())
@@ -561,12 +583,15 @@ void launch_rod(gcr_pool* p, int stage, int j, int sub = 0)
     if (!p->nchb) return;
     PoolArgs a = p->a;
     a.sub = stage == 0 ? sub : 3;
+    // one wavefront per chain: an entry of BS chains takes 64 blocks of 4 waves
+    static_assert(BS == 256, "k_rod maps BS chains onto 64 blocks of 4 wavefronts");
+    const int nb = p->nchb * 64;
     if (p->dim == 2)
-        stage == 0 ? k_rod<2, 0><<<p->nchb, BS, 0, p->stream>>>(a, j)
-                   : k_rod<2, 1><<<p->nchb, BS, 0, p->stream>>>(a, j);
+        stage == 0 ? k_rod<2, 0><<<nb, BS, 0, p->stream>>>(a, j)
+                   : k_rod<2, 1><<<nb, BS, 0, p->stream>>>(a, j);
     else
-        stage == 0 ? k_rod<3, 0><<<p->nchb, BS, 0, p->stream>>>(a, j)
-                   : k_rod<3, 1><<<p->nchb, BS, 0, p->stream>>>(a, j);
+        stage == 0 ? k_rod<3, 0><<<nb, BS, 0, p->stream>>>(a, j)
+                   : k_rod<3, 1><<<nb, BS, 0, p->stream>>>(a, j);
 }
 
 // Forces between the populations of groups, after a force stage.
@@ -581,14 +606,21 @@ void launch_group(gcr_pool* p, int stage, int j)
 }
 
 // Motor bindings of a force stage (before the stage's second half).
+// The motor rule, over the bead blocks of each motor population (MOTOR_LANES
+// blocks per bead block).
 void launch_motor(gcr_pool* p, int stage, int j)
 {
-    if (p->dim == 2)
-        stage == 0 ? k_motor<2, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
-                   : k_motor<2, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
-    else
-        stage == 0 ? k_motor<3, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
-                   : k_motor<3, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
+    for (int r : p->motor_reps) {
+        const RepParams& P = p->P[r];
+        if (!p->live[r] || P.nblk == 0) continue;
+        const int nb = P.nblk * MOTOR_LANES;
+        if (p->dim == 2)
+            stage == 0 ? k_motor<2, 0><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off)
+                       : k_motor<2, 1><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off);
+        else
+            stage == 0 ? k_motor<3, 0><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off)
+                       : k_motor<3, 1><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off);
+    }
 }
 
 // SRK4 virtual update of stage `sub` (1 or 2) for the replicas without rod.
@@ -797,7 +829,7 @@ int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_ce
 int64_t gcr_replica_cells(int32_t dim, const gcr_replica_desc* d)
 {
     if (!d) return -1;
-    int64_t nc = cells_per_side(d->L, d->wca[3], default_skin(d));
+    const int64_t nc = grid_side(dim, d);
     return dim == 3 ? nc * nc * nc : nc * nc;
 }
 
@@ -993,8 +1025,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
         if (first[c] < 0) return fail(GCR_ERR_ARG, "every chain id needs at least one bead");
     for (int64_t b = 0; b < n * dim; ++b)
         if (!(pos[b] >= 0.0 && pos[b] <= d->L)) return fail(GCR_ERR_ARG, "positions must lie in [0, L]");
-    const double skin = default_skin(d);
-    const int64_t nc = d->has_wca ? cells_per_side(d->L, d->wca[3], skin) : 1;
+    const int64_t nc = grid_side(dim, d);
     const int64_t ncell = dim == 3 ? nc * nc * nc : nc * nc;
     if (ncell > (1 << 28)) return fail(GCR_ERR_ARG, "cell grid too large");
     int r = -1;

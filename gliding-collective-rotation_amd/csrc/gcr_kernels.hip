@@ -16,6 +16,7 @@
 #include "gcr_device.h"
 
 #define BS 256  // threads per bead block
+#define MOTOR_LANES 4  // lanes that scan the window of one motor together
 
 namespace gcr {
 
@@ -1107,6 +1108,43 @@ __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
 
 // ------------------------------------------------------------ populations
 
+// Calls fn(t) for every slot t of the cells within w cells (per axis) of the
+// cell holding x in a grid of nc cells per side starting at cell_off (every
+// stride-th slot of each range from its first + `first`, for lanes that split
+// a scan). Slots are sorted by cell, so the cells of one row of the window
+// are one slot range, or two where the window wraps round the box; the whole
+// grid when 2w + 1 >= nc.
+template <int DIM, typename Fn>
+__device__ __forceinline__ void for_window(const PoolArgs& a, int cell_off, int nc, double inv_h,
+                                           const double* x, int w, Fn&& fn, int first = 0,
+                                           int stride = 1)
+{
+    int c[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) c[i] = cell_coord(x[i], inv_h, nc);
+    const bool all = 2 * w + 1 >= nc;
+    const int lo = all ? 0 : -w, hi = all ? nc - 1 : w;
+    for (int dz = (DIM == 3 ? lo : 0); dz <= (DIM == 3 ? hi : 0); ++dz) {
+        const int z = DIM == 3 ? (all ? dz : (c[2] + dz + nc) % nc) : 0;
+        for (int dy = lo; dy <= hi; ++dy) {
+            const int y = all ? dy : (c[1] + dy + nc) % nc;
+            const int row = cell_off + (z * nc + y) * nc;
+            int x0 = all ? 0 : c[0] - w, x1 = all ? nc - 1 : c[0] + w;  // inclusive
+            if (x0 < 0) {  // the part beyond the lower face
+                const int t1 = a.CSTART[row + nc];
+                for (int t = a.CSTART[row + nc + x0] + first; t < t1; t += stride) fn(t);
+                x0 = 0;
+            } else if (x1 >= nc) {  // the part beyond the upper face
+                const int t1 = a.CSTART[row + x1 - nc + 1];
+                for (int t = a.CSTART[row] + first; t < t1; t += stride) fn(t);
+                x1 = nc - 1;
+            }
+            const int t1 = a.CSTART[row + x1 + 1];
+            for (int t = a.CSTART[row + x0] + first; t < t1; t += stride) fn(t);
+        }
+    }
+}
+
 // Forces on a bead at x from the beads of another population Q
 // (forces.py:57-144 with same_pop False: every Q bead within the law's
 // cutoff, no exclusions, the force coeff * (x_q - x), WCA_shift as in the
@@ -1117,51 +1155,34 @@ template <int DIM>
 __device__ __forceinline__ void cross_sum(const PoolArgs& a, const RepParams& Q, const double* XQ,
                                           const double* x, const XPair& e, double* f)
 {
-    const int nc = Q.nc;
-    int c[3] = {0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) c[i] = cell_coord(x[i], Q.inv_h, nc);
-    const bool all = 2 * e.window + 1 >= nc;
-    const int lo = all ? 0 : -e.window, hi = all ? nc - 1 : e.window;
     const double rc2 = e.rc * e.rc;
-    for (int dz = (DIM == 3 ? lo : 0); dz <= (DIM == 3 ? hi : 0); ++dz) {
-        const int z = DIM == 3 ? (all ? dz : (c[2] + dz + nc) % nc) : 0;
-        for (int dy = lo; dy <= hi; ++dy) {
-            const int y = all ? dy : (c[1] + dy + nc) % nc;
-            for (int dx = lo; dx <= hi; ++dx) {
-                const int xc = all ? dx : (c[0] + dx + nc) % nc;
-                const int cell = Q.cell_off + (z * nc + y) * nc + xc;
-                const int t1 = a.CSTART[cell + 1];
-                for (int t = a.CSTART[cell]; t < t1; ++t) {
-                    double q[DIM], sep[DIM], ds = 0.0;
-                    load_vec<DIM>(XQ, t, q);
+    for_window<DIM>(a, Q.cell_off, Q.nc, Q.inv_h, x, e.window, [&](int t) {
+        double q[DIM], sep[DIM], ds = 0.0;
+        load_vec<DIM>(XQ, t, q);
 #pragma unroll
-                    for (int i = 0; i < DIM; ++i) {
-                        double d = q[i] - x[i];
-                        if (d < -Q.halfL)
-                            d += Q.L;
-                        else if (d >= Q.halfL)
-                            d -= Q.L;
-                        sep[i] = d;
-                        ds += d * d;
-                    }
-                    if (!(ds > 0.0 && ds <= rc2)) continue;
-                    if (e.shift != 0.0) {
-                        const double mag = sqrt(ds);
-                        double nm = mag + e.shift;
-                        nm = nm > 0.0 ? nm : 0.0;
-                        const double sc = nm / mag;
-#pragma unroll
-                        for (int i = 0; i < DIM; ++i) sep[i] *= sc;
-                        ds = nm * nm;
-                    }
-                    const double cf = PairLaw<double>::coeff(ds, e.p0, e.p1);
-#pragma unroll
-                    for (int i = 0; i < DIM; ++i) f[i] += sep[i] * cf;
-                }
-            }
+        for (int i = 0; i < DIM; ++i) {
+            double d = q[i] - x[i];
+            if (d < -Q.halfL)
+                d += Q.L;
+            else if (d >= Q.halfL)
+                d -= Q.L;
+            sep[i] = d;
+            ds += d * d;
         }
-    }
+        if (!(ds > 0.0 && ds <= rc2)) return;
+        if (e.shift != 0.0) {
+            const double mag = sqrt(ds);
+            double nm = mag + e.shift;
+            nm = nm > 0.0 ? nm : 0.0;
+            const double sc = nm / mag;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) sep[i] *= sc;
+            ds = nm * nm;
+        }
+        const double cf = PairLaw<double>::coeff(ds, e.p0, e.p1);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) f[i] += sep[i] * cf;
+    });
 }
 
 // Active sliding partners of the bead in slot s (forces.py:79-83, 114-120):
@@ -1181,43 +1202,31 @@ __device__ __forceinline__ int slide_partners(const PoolArgs& a, const RepParams
     const int nn = P.n_nearby;
     const double wd = ceil((sqrt(P.slide_d2) + fmin(P.skin_half, P.L)) * P.inv_h);
     const int w = wd >= nc ? nc : max(1, (int)wd);
-    const bool all = 2 * w + 1 >= nc;
-    const int lo = all ? 0 : -w, hi = all ? nc - 1 : w;
-    int c[3] = {0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) c[i] = cell_coord(x[i], P.inv_h, nc);
     int cnt = 0;
-    for (int dz = (DIM == 3 ? lo : 0); dz <= (DIM == 3 ? hi : 0); ++dz) {
-        const int z = DIM == 3 ? (all ? dz : (c[2] + dz + nc) % nc) : 0;
-        for (int dy = lo; dy <= hi; ++dy) {
-            const int y = all ? dy : (c[1] + dy + nc) % nc;
-            for (int dx = lo; dx <= hi; ++dx) {
-                const int xc = all ? dx : (c[0] + dx + nc) % nc;
-                const int cell = P.cell_off + (z * nc + y) * nc + xc;
-                const int t1e = a.CSTART[cell + 1];
-                for (int t = a.CSTART[cell]; t < t1e; ++t) {
-                    const int2 m = *reinterpret_cast<const int2*>(a.META + 2 * t);
-                    const int d = m.x - me.x;
-                    if (m.y == me.y && (nn < 0 || (d < 0 ? -d : d) <= nn)) continue;
-                    if (a.CHAIN_CHUNK[m.y] == ch1 && m.x >= me.x) continue;
-                    double q[DIM], t2[DIM], ds = 0.0, dot = 0.0;
-                    load_vec<DIM>(XA, t, q);
-                    load_vec<DIM>(a.Tn, t, t2);
+    for_window<DIM>(a, P.cell_off, nc, P.inv_h, x, w, [&](int t) {
+        // range first: the rules below load more, for few beads
+        double q[DIM], ds = 0.0;
+        load_vec<DIM>(XA, t, q);
 #pragma unroll
-                    for (int i = 0; i < DIM; ++i) {
-                        double sep = q[i] - x[i];
-                        if (sep < -P.halfL)
-                            sep += P.L;
-                        else if (sep >= P.halfL)
-                            sep -= P.L;
-                        ds += sep * sep;
-                        dot += t1[i] * t2[i];
-                    }
-                    if (ds < P.slide_d2 && dot < 0.0) ++cnt;
-                }
-            }
+        for (int i = 0; i < DIM; ++i) {
+            double sep = q[i] - x[i];
+            if (sep < -P.halfL)
+                sep += P.L;
+            else if (sep >= P.halfL)
+                sep -= P.L;
+            ds += sep * sep;
         }
-    }
+        if (!(ds < P.slide_d2)) return;
+        const int2 m = *reinterpret_cast<const int2*>(a.META + 2 * t);
+        const int d = m.x - me.x;
+        if (m.y == me.y && (nn < 0 || (d < 0 ? -d : d) <= nn)) return;
+        if (a.CHAIN_CHUNK[m.y] == ch1 && m.x >= me.x) return;
+        double t2[DIM], dot = 0.0;
+        load_vec<DIM>(a.Tn, t, t2);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) dot += t1[i] * t2[i];
+        if (dot < 0.0) ++cnt;
+    });
     return cnt;
 }
 
@@ -1306,17 +1315,23 @@ __device__ __forceinline__ double motor_uniform(uint64_t seed, uint32_t motor, u
 // imparts * t per binding). A moving motor population starts each step free
 // (SRK.py:132); a fixed one keeps its bindings.
 template <int DIM, int STAGE>
-__global__ void __launch_bounds__(BS) k_motor(PoolArgs a, int j)
+__global__ void __launch_bounds__(BS) k_motor(PoolArgs a, int j, int blk0)
 {
-    const int q = a.blk_rep[blockIdx.x];
+    // MOTOR_LANES consecutive lanes per motor: they split the window's slots
+    // and agree on the next bead through a minimum over the group, so every
+    // decision below is the same on the group's lanes
+    const int e = blk0 + (int)blockIdx.x / MOTOR_LANES;
+    const int q = a.blk_rep[e];
     const RepParams& Q = a.P[q];
     if (!(Q.flags & GCR_F_MOTOR) || Q.motor_target < 0) return;
     const int tr = Q.motor_target;
     const RepParams& T = a.P[tr];
     const RepState* ST = a.S + tr;
     if (!step_runs(ST, j)) return;
-    const int m = a.blk_slot0[blockIdx.x] + threadIdx.x;
-    if (m >= Q.bead_off + Q.n) return;
+    const int g = threadIdx.x % MOTOR_LANES;
+    const int m = a.blk_slot0[e] + ((int)blockIdx.x % MOTOR_LANES) * (BS / MOTOR_LANES) +
+                  (int)threadIdx.x / MOTOR_LANES;
+    if (m >= Q.bead_off + Q.n) return;  // the whole group
     const bool qfixed = a.S[q].fixed != 0;
     double xm[DIM];
     load_vec<DIM>((STAGE == 0 || qfixed) ? a.X : a.X1, m, xm);
@@ -1329,43 +1344,41 @@ __global__ void __launch_bounds__(BS) k_motor(PoolArgs a, int j)
     const double reach2 = Q.motor_reach * Q.motor_reach;
     const double wd = ceil((Q.motor_reach + fmin(T.skin_half, T.L)) * T.inv_h);
     const int w = wd >= nc ? nc : max(1, (int)wd);
-    const bool all = 2 * w + 1 >= nc;
-    const int lo = all ? 0 : -w, hi = all ? nc - 1 : w;
-    int c[3] = {0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) c[i] = cell_coord(xm[i], T.inv_h, nc);
     int last = -1;  // the beads within reach, visited in increasing index
     for (;;) {
         int best = 0x7fffffff, bslot = -1;
         double bd2 = 0.0;
-        for (int dz = (DIM == 3 ? lo : 0); dz <= (DIM == 3 ? hi : 0); ++dz) {
-            const int z = DIM == 3 ? (all ? dz : (c[2] + dz + nc) % nc) : 0;
-            for (int dy = lo; dy <= hi; ++dy) {
-                const int y = all ? dy : (c[1] + dy + nc) % nc;
-                for (int dx = lo; dx <= hi; ++dx) {
-                    const int xc = all ? dx : (c[0] + dx + nc) % nc;
-                    const int cell = T.cell_off + (z * nc + y) * nc + xc;
-                    const int t1 = a.CSTART[cell + 1];
-                    for (int t = a.CSTART[cell]; t < t1; ++t) {
-                        const int o = a.META[2 * t];
-                        if (o <= last || o >= best) continue;
-                        double xb[DIM], d2 = 0.0;
-                        load_vec<DIM>(XT, t, xb);
+        for_window<DIM>(
+            a, T.cell_off, nc, T.inv_h, xm, w,
+            [&](int t) {
+                const int o = a.META[2 * t];
+                if (o <= last || o >= best) return;
+                double xb[DIM], d2 = 0.0;
+                load_vec<DIM>(XT, t, xb);
 #pragma unroll
-                        for (int i = 0; i < DIM; ++i) {
-                            double sep = xm[i] - xb[i];
-                            if (sep < -T.halfL)
-                                sep += T.L;
-                            else if (sep >= T.halfL)
-                                sep -= T.L;
-                            d2 += sep * sep;
-                        }
-                        if (d2 > reach2) continue;
-                        best = o;
-                        bslot = t;
-                        bd2 = d2;
-                    }
+                for (int i = 0; i < DIM; ++i) {
+                    double sep = xm[i] - xb[i];
+                    if (sep < -T.halfL)
+                        sep += T.L;
+                    else if (sep >= T.halfL)
+                        sep -= T.L;
+                    d2 += sep * sep;
                 }
+                if (d2 > reach2) return;
+                best = o;
+                bslot = t;
+                bd2 = d2;
+            },
+            g, MOTOR_LANES);
+#pragma unroll
+        for (int k = 1; k < MOTOR_LANES; k <<= 1) {
+            const int ob = __shfl_xor(best, k, 64);
+            const int os = __shfl_xor(bslot, k, 64);
+            const double od = __shfl_xor(bd2, k, 64);
+            if (ob < best) {
+                best = ob;
+                bslot = os;
+                bd2 = od;
             }
         }
         if (bslot < 0) break;
@@ -1374,15 +1387,17 @@ __global__ void __launch_bounds__(BS) k_motor(PoolArgs a, int j)
         if (state >= 0) {
             if (motor_uniform(Q.seed, mloc, bloc, step, STAGE, 0) > Q.k_off) continue;
             state = -1;
-            atomicAdd(&a.WA[bslot], -1);
+            if (g == 0) atomicAdd(&a.WA[bslot], -1);
         }
         if (motor_uniform(Q.seed, mloc, bloc, step, STAGE, 1) < Q.k_on && bd2 <= Q.motor_d2) {
-            atomicAdd(&a.WA[bslot], 1);
-            atomicAdd(&a.MC[bslot], 1);
+            if (g == 0) {
+                atomicAdd(&a.WA[bslot], 1);
+                atomicAdd(&a.MC[bslot], 1);
+            }
             state = (int)bloc;
         }
     }
-    a.WA[m] = state;
+    if (g == 0) a.WA[m] = state;
 }
 
 // One dt for a group: the largest member part (SRK.py:336-337 takes the max
@@ -1858,22 +1873,35 @@ __global__ void __launch_bounds__(BS) k_tangents(PoolArgs a, int rsel)
 
 // ------------------------------------------------------- rigid-rod updates
 
-// Rigid-rod projection of one stage's update (updates.py:347-499), one thread
-// per chain, for replicas with GCR_F_ROD. STAGE 0: a virtual update, after
-// the force stage of SRK stage a.sub (x1 from x in stage 0 and from x1 in the
-// inner SRK4 stages -- the position the forces were evaluated at, as
+// Rigid-rod projection of one stage's update (updates.py:347-499), one
+// wavefront per chain (lane k holds bead k; chains longer than 64 beads in
+// segments of 64), for replicas with GCR_F_ROD. STAGE 0: a virtual update,
+// after the force stage of SRK stage a.sub (x1 from x in stage 0 and from x1
+// in the inner SRK4 stages -- the position the forces were evaluated at, as
 // SRK.py:364-371 passes it -- with that stage's forces in F0 and dt_prev
 // times the stage's rescale factor). STAGE 1: the real update after
 // k_finalize (x from x1, with the total force and dt_used times the force
 // prefactor, 1/2 in SRK2). The update of a bead is dt/r F + sqrt(2 D dt) xi
-// with that dt. Sums run over the chain's beads in order, as the reference's
-// loops do; the chain is re-read in each pass instead of being held in
-// registers. Within a stage each thread reads and writes its own chain only,
-// so x1 may be both the input and the output.
-template <int DIM, int STAGE>
-__global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
+// with that dt. The chain sums run over the beads in order, as the
+// reference's loops do: every lane adds the lanes' terms one by one, read
+// from lane to lane, so each sum rounds as the sequential loop does. Within a stage a
+// wave reads and writes its own chain only, so x1 may be both the input and
+// the output. Each entry of the chain-block tables covers 64 blocks of four
+// waves.
+// Lane q's value of v, for q the same on every lane of the wave (a scalar
+// read of the lane's register; cheaper than a cross-lane permute).
+__device__ __forceinline__ double lane_value(double v, int q)
 {
-    const int r = a.chb_rep[blockIdx.x];
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), q);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), q);
+    return __hiloint2double(hi, lo);
+}
+
+template <int DIM, int STAGE>
+__global__ void __launch_bounds__(256) k_rod(PoolArgs a, int j)
+{
+    const int e = blockIdx.x >> 6;
+    const int r = a.chb_rep[e];
     const RepParams& P = a.P[r];
     const RepState* S = a.S + r;
     if (STAGE == 0) {
@@ -1882,8 +1910,9 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
     } else if (S->frozen || S->steps != S->batch_base + j + 1) {
         return;
     }
-    const int c = a.chb_chain0[blockIdx.x] + threadIdx.x;
-    if (c >= P.chain_off + P.n_chains) return;
+    const int lane = threadIdx.x & 63;
+    const int c = a.chb_chain0[e] + (blockIdx.x & 63) * 4 + (threadIdx.x >> 6);
+    if (c >= P.chain_off + P.n_chains) return;  // the whole wave
     // the projection starts from the positions the stage's forces were
     // evaluated at (SRK.py:364-371): x in SRK stage 0, x1 after it
     const double* prev = (STAGE == 0 && a.sub == 0) ? a.X : a.X1;
@@ -1897,11 +1926,17 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
     const int first = a.CHAIN_FIRST[c], cl = a.CHAIN_LEN[c];
     const double ca = a.CHAIN_A[c];
     const double half_clm1 = (cl - 1) / 2.0;
-    auto slot = [&](int k) { return a.SLOT[first + k]; };
-    auto update = [&](int sl, double* u) {
+    // bead k's update u and position x (k >= cl: zeros)
+    auto bead = [&](int k, double* u, double* x, int& sl) {
+        sl = -1;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) u[i] = x[i] = 0.0;
+        if (k >= cl) return;
+        sl = a.SLOT[first + k];
         double f[DIM], xi[DIM];
         load_vec<DIM>(F, sl, f);
         load_vec<DIM>(a.XI, sl, xi);
+        load_vec<DIM>(prev, sl, x);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
             u[i] = dt_over_r * f[i];
@@ -1916,26 +1951,34 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
     }
     // pass 1: mean update, centre of mass and head-to-tail direction
     double p0[DIM], mean[DIM], com[DIM], rh[DIM], rmag = 0.0;
-    load_vec<DIM>(prev, slot(0), p0);
+    load_vec<DIM>(prev, a.SLOT[first], p0);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) mean[i] = com[i] = rh[i] = 0.0;
-    for (int k = 0; k < cl; ++k) {
-        const int sl = slot(k);
-        double u[DIM], x[DIM];
-        update(sl, u);
-        load_vec<DIM>(prev, sl, x);
+    for (int base = 0; base < cl; base += 64) {
+        double u[DIM], x[DIM], sep[DIM];
+        int sl;
+        bead(base + lane, u, x, sl);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
-            mean[i] += u[i];
-            double sep = x[i] - p0[i];
-            if (sep < -half)
-                sep += L;
-            else if (sep > half)
-                sep -= L;
-            com[i] += sep;
-            if (k == cl - 1) {
-                rh[i] = sep;
-                rmag += sep * sep;
+            double d = x[i] - p0[i];
+            if (d < -half)
+                d += L;
+            else if (d > half)
+                d -= L;
+            sep[i] = d;
+        }
+        const int nk = min(64, cl - base);
+        for (int q = 0; q < nk; ++q)
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                mean[i] += lane_value(u[i], q);
+                com[i] += lane_value(sep[i], q);
+            }
+        if (base + 64 >= cl) {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                rh[i] = lane_value(sep[i], cl - 1 - base);
+                rmag += rh[i] * rh[i];
             }
         }
     }
@@ -1951,18 +1994,23 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
     }
     // pass 2: the torque term
     double tq[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < cl; ++k) {
-        double u[DIM];
-        update(slot(k), u);
+    for (int base = 0; base < cl; base += 64) {
+        double u[DIM], x[DIM], term[3] = {0.0, 0.0, 0.0};
+        int sl;
+        bead(base + lane, u, x, sl);
         if (DIM == 2) {
-            const double d = ca * (k - half_clm1);
+            const double d = ca * (base + lane - half_clm1);
 #pragma unroll
-            for (int i = 0; i < 2; ++i) tq[i] += (2 * i - 1) * rh[1 - i] * u[i] * d;
+            for (int i = 0; i < 2; ++i) term[i] = (2 * i - 1) * rh[1 - i] * u[i] * d;
         } else {
-            tq[0] += rh[1] * u[2] - rh[2] * u[1];
-            tq[1] += rh[2] * u[0] - rh[0] * u[2];
-            tq[2] += rh[0] * u[1] - rh[1] * u[0];
+            term[0] = rh[1] * u[2] - rh[2] * u[1];
+            term[1] = rh[2] * u[0] - rh[0] * u[2];
+            term[2] = rh[0] * u[1] - rh[1] * u[0];
         }
+        const int nk = min(64, cl - base);
+        for (int q = 0; q < nk; ++q)
+#pragma unroll
+            for (int i = 0; i < (DIM == 2 ? 2 : 3); ++i) tq[i] += lane_value(term[i], q);
     }
     double rot[3] = {0.0, 0.0, 0.0};  // 3D: (|r| / I) (torque x r_hat), the same for every bead
     if (DIM == 2) {
@@ -1975,14 +2023,14 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
         rot[1] = w * (tq[2] * rh[0] - tq[0] * rh[2]);
         rot[2] = w * (tq[0] * rh[1] - tq[1] * rh[0]);
     }
-    // pass 3: blend with the rigid-rod update and write
+    // pass 3: blend with the rigid-rod update and write, one bead per lane
     bool viol = false;
-    for (int k = 0; k < cl; ++k) {
-        const int sl = slot(k);
-        const double d = ca * (k - half_clm1);
+    for (int base = 0; base < cl; base += 64) {
         double u[DIM], x[DIM], y[DIM];
-        update(sl, u);
-        load_vec<DIM>(prev, sl, x);
+        int sl;
+        bead(base + lane, u, x, sl);
+        if (sl < 0) continue;
+        const double d = ca * (base + lane - half_clm1);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
             double ool = x[i] - (com[i] + rh[i] * d);
@@ -2000,7 +2048,7 @@ __global__ void __launch_bounds__(BS) k_rod(PoolArgs a, int j)
         store_vec<DIM>(out, sl, y);
         viol |= moved_too_far<DIM>(a, P, sl, y);
     }
-    if (viol) {
+    if (__any(viol) && lane == 0) {
         if (STAGE == 0)
             flag_predictor(a, a.S + r, a.sub);
         else if (!a.safe)
