@@ -270,6 +270,10 @@ void set_physics(RepParams& P, const gcr_replica_desc* d, int longest_chain)
     // SRK.py:364: projection when the factor is set and the longest chain has
     // more than two beads
     if (d->rod != 0.0 && longest_chain > 2) fl |= GCR_F_ROD;
+    {  // as many chains per wavefront of k_rod as fit (longer ones in segments)
+        const int cpw = longest_chain >= 1 && longest_chain <= 64 ? 64 / longest_chain : 1;
+        P.rod_lanes = 64 / std::min(cpw, ROD_MAX_CPW);
+    }
     if (d->Dr > 0 && !orient) fl |= GCR_F_ROTDIFF;
     if (d->srk_order == 4) fl |= GCR_F_SRK4;
     P.walls = d->walls & 7;  // the kernels read the bits of their dimensions

@@ -69,7 +69,7 @@ struct RepParams {
     int32_t group;            // group index, -1: none
     int32_t grp_first, grp_n; // the group's moving replicas in GRP (the dt is theirs)
     int32_t xin_off, xin_n;   // laws of the other populations acting on this one (XPAIRS)
-    int32_t pad1;
+    int32_t rod_lanes;        // lanes per chain of the rigid-rod kernel (64 / chains per wave)
     double wall_zone;         // distance of the walls from the box faces
     double wall_rc;           // the walls' range parameter (wall_params[3])
     double slide_d2, slide_v0; // active sliding: squared partner range, push per partner

@@ -17,6 +17,7 @@
 
 #define BS 256  // threads per bead block
 #define MOTOR_LANES 4  // lanes that scan the window of one motor together
+#define ROD_MAX_CPW 21  // chains per wavefront of k_rod at most (chains of 3 beads)
 
 namespace gcr {
 
@@ -1873,33 +1874,36 @@ __global__ void __launch_bounds__(BS) k_tangents(PoolArgs a, int rsel)
 
 // ------------------------------------------------------- rigid-rod updates
 
-// Rigid-rod projection of one stage's update (updates.py:347-499), one
-// wavefront per chain (lane k holds bead k; chains longer than 64 beads in
-// segments of 64), for replicas with GCR_F_ROD. STAGE 0: a virtual update,
-// after the force stage of SRK stage a.sub (x1 from x in stage 0 and from x1
-// in the inner SRK4 stages -- the position the forces were evaluated at, as
+// Makes this wavefront's LDS writes visible to its own later LDS reads of
+// other lanes' entries (the LDS serves a wavefront's operations in order; the
+// fence keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Rigid-rod projection of one stage's update (updates.py:347-499) for
+// replicas with GCR_F_ROD. P.rod_lanes lanes per chain, as many chains per
+// wavefront as fit (lane k of a chain's group holds bead k; chains longer
+// than 64 beads in segments of 64). STAGE 0: a virtual update, after the
+// force stage of SRK stage a.sub (x1 from x in stage 0 and from x1 in the
+// inner SRK4 stages -- the position the forces were evaluated at, as
 // SRK.py:364-371 passes it -- with that stage's forces in F0 and dt_prev
 // times the stage's rescale factor). STAGE 1: the real update after
 // k_finalize (x from x1, with the total force and dt_used times the force
 // prefactor, 1/2 in SRK2). The update of a bead is dt/r F + sqrt(2 D dt) xi
 // with that dt. The chain sums run over the beads in order, as the
-// reference's loops do: every lane adds the lanes' terms one by one, read
-// from lane to lane, so each sum rounds as the sequential loop does. Within a stage a
-// wave reads and writes its own chain only, so x1 may be both the input and
-// the output. Each entry of the chain-block tables covers 64 blocks of four
-// waves.
-// Lane q's value of v, for q the same on every lane of the wave (a scalar
-// read of the lane's register; cheaper than a cross-lane permute).
-__device__ __forceinline__ double lane_value(double v, int q)
-{
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), q);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), q);
-    return __hiloint2double(hi, lo);
-}
-
+// reference's loops do: the lanes put their terms in LDS and the chain's
+// first lane adds them one by one, so each sum rounds as the sequential loop
+// does. Within a stage a chain's lanes read and write that chain only, so x1
+// may be both the input and the output. Each entry of the chain-block tables
+// (BS chains) is covered by the first of its 64 blocks of four wavefronts.
 template <int DIM, int STAGE>
 __global__ void __launch_bounds__(256) k_rod(PoolArgs a, int j)
 {
+    __shared__ double sh[4][64][2 * DIM];  // per lane: (u, sep), then the torque term
+    __shared__ double res[4][ROD_MAX_CPW][3 * DIM + 1];  // per chain: mean, com, r_hat, |r|
     const int e = blockIdx.x >> 6;
     const int r = a.chb_rep[e];
     const RepParams& P = a.P[r];
@@ -1910,9 +1914,14 @@ __global__ void __launch_bounds__(256) k_rod(PoolArgs a, int j)
     } else if (S->frozen || S->steps != S->batch_base + j + 1) {
         return;
     }
-    const int lane = threadIdx.x & 63;
-    const int c = a.chb_chain0[e] + (blockIdx.x & 63) * 4 + (threadIdx.x >> 6);
-    if (c >= P.chain_off + P.n_chains) return;  // the whole wave
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int G = P.rod_lanes, cpw = 64 / G;
+    const int grp = lane / G, k = lane - grp * G, lead = grp * G;
+    const int c0 = a.chb_chain0[e] + ((blockIdx.x & 63) * 4 + wv) * cpw;
+    const int cend = min(a.chb_chain0[e] + BS, P.chain_off + P.n_chains);
+    if (c0 >= cend) return;  // the whole wavefront
+    const int c = c0 + grp;
+    const bool on = grp < cpw && c < cend;  // the lane belongs to a chain
     // the projection starts from the positions the stage's forces were
     // evaluated at (SRK.py:364-371): x in SRK stage 0, x1 after it
     const double* prev = (STAGE == 0 && a.sub == 0) ? a.X : a.X1;
@@ -1923,16 +1932,16 @@ __global__ void __launch_bounds__(256) k_rod(PoolArgs a, int j)
     const double dt_over_r = dt / P.radius;
     const double sq = sqrt(2.0 * P.D * dt);
     const double L = P.L, half = 0.5 * L, fac = P.rod;
-    const int first = a.CHAIN_FIRST[c], cl = a.CHAIN_LEN[c];
-    const double ca = a.CHAIN_A[c];
+    const int first = on ? a.CHAIN_FIRST[c] : 0, cl = on ? a.CHAIN_LEN[c] : 0;
+    const double ca = on ? a.CHAIN_A[c] : 0.0;
     const double half_clm1 = (cl - 1) / 2.0;
-    // bead k's update u and position x (k >= cl: zeros)
-    auto bead = [&](int k, double* u, double* x, int& sl) {
+    // bead kk's update u and position x (kk >= cl: zeros, slot -1)
+    auto bead = [&](int kk, double* u, double* x, int& sl) {
         sl = -1;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) u[i] = x[i] = 0.0;
-        if (k >= cl) return;
-        sl = a.SLOT[first + k];
+        if (kk >= cl) return;
+        sl = a.SLOT[first + kk];
         double f[DIM], xi[DIM];
         load_vec<DIM>(F, sl, f);
         load_vec<DIM>(a.XI, sl, xi);
@@ -1943,94 +1952,160 @@ __global__ void __launch_bounds__(256) k_rod(PoolArgs a, int j)
             if (sq != 0.0) u[i] += sq * xi[i];
         }
     };
-    double moi = 0.0;
-    for (int k = 0; k < cl; ++k) {
-        const double d = ca * (k - half_clm1);
+    // the lane's bead of the first segment stays in registers
+    double u0[DIM], x0[DIM];
+    int sl0;
+    bead(k, u0, x0, sl0);
+    auto seg_bead = [&](int base, double* u, double* x, int& sl) {
+        if (base == 0) {
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) moi += d * d;
-    }
-    // pass 1: mean update, centre of mass and head-to-tail direction
-    double p0[DIM], mean[DIM], com[DIM], rh[DIM], rmag = 0.0;
-    load_vec<DIM>(prev, a.SLOT[first], p0);
+            for (int i = 0; i < DIM; ++i) {
+                u[i] = u0[i];
+                x[i] = x0[i];
+            }
+            sl = sl0;
+        } else {
+            bead(base + k, u, x, sl);
+        }
+    };
+    double p0[DIM];
+    if (on) load_vec<DIM>(prev, a.SLOT[first], p0);
+    // pass 1: mean update, centre of mass and head-to-tail direction, summed
+    // by the chain's first lane
+    double mean[DIM], com[DIM], rh[DIM], rmag = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) mean[i] = com[i] = rh[i] = 0.0;
-    for (int base = 0; base < cl; base += 64) {
-        double u[DIM], x[DIM], sep[DIM];
+    for (int base = 0; base < cl; base += G) {
+        double u[DIM], x[DIM];
         int sl;
-        bead(base + lane, u, x, sl);
+        seg_bead(base, u, x, sl);
+        if (sl >= 0) {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                double d = x[i] - p0[i];
+                if (d < -half)
+                    d += L;
+                else if (d > half)
+                    d -= L;
+                sh[wv][lane][i] = u[i];
+                sh[wv][lane][DIM + i] = d;
+            }
+        }
+        wave_lds_sync();
+        if (k == 0) {
+            const int nk = min(G, cl - base);
+#pragma unroll 2
+            for (int q = 0; q < nk; ++q)
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    mean[i] += sh[wv][lead + q][i];
+                    com[i] += sh[wv][lead + q][DIM + i];
+                }
+            if (base + G >= cl)
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    rh[i] = sh[wv][lead + cl - 1 - base][DIM + i];
+                    rmag += rh[i] * rh[i];
+                }
+        }
+        wave_lds_sync();
+    }
+    if (k == 0 && on) {
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
-            double d = x[i] - p0[i];
-            if (d < -half)
-                d += L;
-            else if (d > half)
-                d -= L;
-            sep[i] = d;
+            mean[i] /= cl;
+            com[i] = pymod(p0[i] + com[i] / cl, L);
         }
-        const int nk = min(64, cl - base);
-        for (int q = 0; q < nk; ++q)
+        if (rmag > 0) {
+            rmag = sqrt(rmag);
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) {
-                mean[i] += lane_value(u[i], q);
-                com[i] += lane_value(sep[i], q);
-            }
-        if (base + 64 >= cl) {
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) {
-                rh[i] = lane_value(sep[i], cl - 1 - base);
-                rmag += rh[i] * rh[i];
-            }
+            for (int i = 0; i < DIM; ++i) rh[i] /= rmag;
         }
-    }
 #pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-        mean[i] /= cl;
-        com[i] = pymod(p0[i] + com[i] / cl, L);
+        for (int i = 0; i < DIM; ++i) {
+            res[wv][grp][i] = mean[i];
+            res[wv][grp][DIM + i] = com[i];
+            res[wv][grp][2 * DIM + i] = rh[i];
+        }
+        res[wv][grp][3 * DIM] = rmag;
     }
-    if (rmag > 0) {
-        rmag = sqrt(rmag);
+    wave_lds_sync();
+    if (on) {
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) rh[i] /= rmag;
+        for (int i = 0; i < DIM; ++i) {
+            mean[i] = res[wv][grp][i];
+            com[i] = res[wv][grp][DIM + i];
+            rh[i] = res[wv][grp][2 * DIM + i];
+        }
+        rmag = res[wv][grp][3 * DIM];
     }
-    // pass 2: the torque term
+    // pass 2: the torque term, summed by the chain's first lane
     double tq[3] = {0.0, 0.0, 0.0};
-    for (int base = 0; base < cl; base += 64) {
-        double u[DIM], x[DIM], term[3] = {0.0, 0.0, 0.0};
+    for (int base = 0; base < cl; base += G) {
+        double u[DIM], x[DIM];
         int sl;
-        bead(base + lane, u, x, sl);
-        if (DIM == 2) {
-            const double d = ca * (base + lane - half_clm1);
+        seg_bead(base, u, x, sl);
+        if (sl >= 0) {
+            if (DIM == 2) {
+                const double d = ca * (base + k - half_clm1);
 #pragma unroll
-            for (int i = 0; i < 2; ++i) term[i] = (2 * i - 1) * rh[1 - i] * u[i] * d;
-        } else {
-            term[0] = rh[1] * u[2] - rh[2] * u[1];
-            term[1] = rh[2] * u[0] - rh[0] * u[2];
-            term[2] = rh[0] * u[1] - rh[1] * u[0];
+                for (int i = 0; i < 2; ++i) sh[wv][lane][i] = (2 * i - 1) * rh[1 - i] * u[i] * d;
+            } else {
+                sh[wv][lane][0] = rh[1] * u[2] - rh[2] * u[1];
+                sh[wv][lane][1] = rh[2] * u[0] - rh[0] * u[2];
+                sh[wv][lane][2] = rh[0] * u[1] - rh[1] * u[0];
+            }
         }
-        const int nk = min(64, cl - base);
-        for (int q = 0; q < nk; ++q)
+        wave_lds_sync();
+        if (k == 0) {
+            const int nk = min(G, cl - base);
+#pragma unroll 2
+            for (int q = 0; q < nk; ++q)
 #pragma unroll
-            for (int i = 0; i < (DIM == 2 ? 2 : 3); ++i) tq[i] += lane_value(term[i], q);
+                for (int i = 0; i < (DIM == 2 ? 2 : 3); ++i) tq[i] += sh[wv][lead + q][i];
+        }
+        wave_lds_sync();
     }
     double rot[3] = {0.0, 0.0, 0.0};  // 3D: (|r| / I) (torque x r_hat), the same for every bead
-    if (DIM == 2) {
-        if (moi != 0)
-            for (int i = 0; i < 2; ++i) tq[i] /= moi;
-    } else {
-        for (int i = 0; i < 3; ++i) tq[i] *= rmag;
-        const double w = rmag / moi;
-        rot[0] = w * (tq[1] * rh[2] - tq[2] * rh[1]);
-        rot[1] = w * (tq[2] * rh[0] - tq[0] * rh[2]);
-        rot[2] = w * (tq[0] * rh[1] - tq[1] * rh[0]);
+    if (k == 0 && on) {
+        double moi = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < cl; ++q) {
+            const double d = ca * (q - half_clm1);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) moi += d * d;
+        }
+        if (DIM == 2) {
+            if (moi != 0)
+                for (int i = 0; i < 2; ++i) tq[i] /= moi;
+        } else {
+            for (int i = 0; i < 3; ++i) tq[i] *= rmag;
+            const double w = rmag / moi;
+            rot[0] = w * (tq[1] * rh[2] - tq[2] * rh[1]);
+            rot[1] = w * (tq[2] * rh[0] - tq[0] * rh[2]);
+            rot[2] = w * (tq[0] * rh[1] - tq[1] * rh[0]);
+        }
+        // 2D: the scaled torque; 3D: the rotation term
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) sh[wv][lane][i] = DIM == 2 ? tq[i] : rot[i];
     }
+    wave_lds_sync();
+    if (on)
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            if (DIM == 2)
+                tq[i] = sh[wv][lead][i];
+            else
+                rot[i] = sh[wv][lead][i];
+        }
     // pass 3: blend with the rigid-rod update and write, one bead per lane
     bool viol = false;
-    for (int base = 0; base < cl; base += 64) {
+    for (int base = 0; base < cl; base += G) {
         double u[DIM], x[DIM], y[DIM];
         int sl;
-        bead(base + lane, u, x, sl);
+        seg_bead(base, u, x, sl);
         if (sl < 0) continue;
-        const double d = ca * (base + lane - half_clm1);
+        const double d = ca * (base + k - half_clm1);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
             double ool = x[i] - (com[i] + rh[i] * d);
