@@ -136,6 +136,7 @@ struct gcr_pool {
     int32_t *CHAIN_FIRST = nullptr, *CHAIN_LEN = nullptr;
     int32_t *CCOUNT = nullptr, *CSTART = nullptr, *CHKSUM = nullptr;
     double* BMAX = nullptr;
+    int32_t* SCNT = nullptr;  // per slot: active sliding partners of the running stage
     int32_t* TILES = nullptr;
     uint16_t* NL = nullptr;
     int32_t* NLEN = nullptr;
@@ -165,6 +166,7 @@ struct gcr_pool {
                              // second half of each force stage runs (k_group_stage)
     bool has_motor = false;  // a group has motors: k_motor runs in each force stage
     std::vector<int> motor_reps;  // the motor populations k_motor runs over
+    std::vector<int> slide_reps;  // the replicas with active sliding
 };
 
 namespace {
@@ -427,8 +429,12 @@ int rebuild_tables(gcr_pool* p)
     p->has_group = !p->groups.empty();
     p->has_motor = has_motor;
     p->motor_reps = motor_reps;
+    p->slide_reps.clear();
     for (int r = 0; r < p->capR; ++r)
-        if (p->live[r] && (p->P[r].flags & GCR_F_SLIDE)) p->has_group = true;
+        if (p->live[r] && (p->P[r].flags & GCR_F_SLIDE)) {
+            p->has_group = true;
+            p->slide_reps.push_back(r);
+        }
     if (!xp.empty())
         HIPCHK(hipMemcpyAsync(p->XPAIRS, xp.data(), xp.size() * sizeof(XPair), hipMemcpyHostToDevice,
                               p->stream));
@@ -601,6 +607,16 @@ void launch_rod(gcr_pool* p, int stage, int j, int sub = 0)
 // Forces between the populations of groups, after a force stage.
 void launch_group(gcr_pool* p, int stage, int j)
 {
+    for (int r : p->slide_reps) {  // the sliding partners, SLIDE_LANES blocks per bead block
+        const RepParams& P = p->P[r];
+        const int nb = P.nblk * SLIDE_LANES;
+        if (p->dim == 2)
+            stage == 0 ? k_slide_count<2, 0><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off)
+                       : k_slide_count<2, 1><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off);
+        else
+            stage == 0 ? k_slide_count<3, 0><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off)
+                       : k_slide_count<3, 1><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off);
+    }
     if (p->dim == 2)
         stage == 0 ? k_group_stage<2, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
                    : k_group_stage<2, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
@@ -908,6 +924,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->WA, nb));
     A(dalloc(&p->WAn, nb));
     A(dalloc(&p->MC, nb));
+    A(dalloc(&p->SCNT, nb));
     A(dalloc(&p->TILES, p->capTile * 4));
     A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
     // neighbour lists: 2D u16 LDS indices (NL_MAX), 3D int32 slots (NL3_MAX),
@@ -958,6 +975,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.CSTART = p->CSTART;
     a.CHKSUM = p->CHKSUM;
     a.BMAX = p->BMAX;
+    a.SCNT = p->SCNT;
     a.TILES = p->TILES;
     a.NL = p->NL;
     a.NLEN = p->NLEN;
@@ -1576,7 +1594,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
                     p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR, p->OBS,
-                    p->XPAIRS, p->GRP, p->CHAIN_CHUNK, p->WA, p->WAn, p->MC};
+                    p->XPAIRS, p->GRP, p->CHAIN_CHUNK, p->WA, p->WAn, p->MC, p->SCNT};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);

@@ -124,6 +124,7 @@ struct PoolArgs {
     const RepParams* P;
     RepState* S;
     double* BMAX;  // [block] max |F0+F1|^2 of the block (stage 1)
+    int32_t* SCNT; // [slot] active sliding partners of the running force stage
     const int32_t* TILES;  // [tile][4] (replica, cx0, cy0, 0) for the 2D tile kernels
     int32_t ntile;
     // per-slot state

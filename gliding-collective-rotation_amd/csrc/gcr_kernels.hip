@@ -17,6 +17,7 @@
 
 #define BS 256  // threads per bead block
 #define MOTOR_LANES 4  // lanes that scan the window of one motor together
+#define SLIDE_LANES 4  // lanes that scan the window of one bead for sliding partners
 #define ROD_MAX_CPW 21  // chains per wavefront of k_rod at most (chains of 3 beads)
 
 namespace gcr {
@@ -1074,8 +1075,31 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
     }
 }
 
-// Max force and adaptive dt of each replica (SRK.py:323-353), one block per
-// replica, after the stage-1 launch has written every block's maximum.
+// Step bookkeeping of a replica from the largest |F|^2 over its blocks
+// (SRK.py:323-353): the adaptive dt, dt_used clipped to max_dt_usable, the
+// clock and the step counters. A group member only reports its part
+// (k_group_dt takes the group's max). One thread.
+__device__ __forceinline__ void finish_step(const RepParams& P, RepState* S, double m2)
+{
+    double m = sqrt(2.0 * P.D);
+    if (P.n > 0) m += force_prefactor(P) * sqrt(m2) / P.radius;
+    if (P.flags & GCR_F_GROUP) {
+        S->mpart = m;
+        return;
+    }
+    double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
+    double dt_used = P.cheat * dt;
+    if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
+    S->dt = dt;
+    S->dt_used = dt_used;
+    S->elapsed += dt_used;
+    S->maxm = m;
+    S->steps += 1;
+    S->step_global += 1;
+}
+
+// Max force and adaptive dt of each replica, one block per replica, after
+// the stage-1 launch has written every block's maximum.
 __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
 {
     const int r = blockIdx.x;
@@ -1088,23 +1112,7 @@ __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
         m2 = v > m2 ? v : m2;
     }
     m2 = block_max(m2);
-    if (threadIdx.x == 0) {
-        double m = sqrt(2.0 * P.D);
-        if (P.n > 0) m += force_prefactor(P) * sqrt(m2) / P.radius;
-        if (P.flags & GCR_F_GROUP) {  // k_group_dt takes the group's max
-            S->mpart = m;
-            return;
-        }
-        double dt = (m == 0.0) ? P.dtmax : fmin(P.dtmax, P.mdps / m);
-        double dt_used = P.cheat * dt;
-        if (S->mdu > 0) dt_used = fmin(dt_used, S->mdu - S->elapsed);
-        S->dt = dt;
-        S->dt_used = dt_used;
-        S->elapsed += dt_used;
-        S->maxm = m;
-        S->steps += 1;
-        S->step_global += 1;
-    }
+    if (threadIdx.x == 0) finish_step(P, S, m2);
 }
 
 // ------------------------------------------------------------ populations
@@ -1195,7 +1203,8 @@ __device__ __forceinline__ void cross_sum(const PoolArgs& a, const RepParams& Q,
 // Found by scanning the replica's cells within the range plus the half skin.
 template <int DIM>
 __device__ __forceinline__ int slide_partners(const PoolArgs& a, const RepParams& P, const double* XA,
-                                              int s, const double* x, const double* t1)
+                                              int s, const double* x, const double* t1, int first = 0,
+                                              int stride = 1)
 {
     const int nc = P.nc;
     const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
@@ -1204,7 +1213,9 @@ __device__ __forceinline__ int slide_partners(const PoolArgs& a, const RepParams
     const double wd = ceil((sqrt(P.slide_d2) + fmin(P.skin_half, P.L)) * P.inv_h);
     const int w = wd >= nc ? nc : max(1, (int)wd);
     int cnt = 0;
-    for_window<DIM>(a, P.cell_off, nc, P.inv_h, x, w, [&](int t) {
+    for_window<DIM>(
+        a, P.cell_off, nc, P.inv_h, x, w,
+        [&](int t) {
         // range first: the rules below load more, for few beads
         double q[DIM], ds = 0.0;
         load_vec<DIM>(XA, t, q);
@@ -1221,14 +1232,40 @@ __device__ __forceinline__ int slide_partners(const PoolArgs& a, const RepParams
         const int2 m = *reinterpret_cast<const int2*>(a.META + 2 * t);
         const int d = m.x - me.x;
         if (m.y == me.y && (nn < 0 || (d < 0 ? -d : d) <= nn)) return;
-        if (a.CHAIN_CHUNK[m.y] == ch1 && m.x >= me.x) return;
+        if (m.x >= me.x && a.CHAIN_CHUNK[m.y] == ch1) return;
         double t2[DIM], dot = 0.0;
         load_vec<DIM>(a.Tn, t, t2);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) dot += t1[i] * t2[i];
         if (dot < 0.0) ++cnt;
-    });
+        },
+        first, stride);
     return cnt;
+}
+
+// Active sliding partners of every bead of a replica with GCR_F_SLIDE at
+// this force stage (x in stage 0, x1 in stage 1; tangents of the stage from
+// Tn), into SCNT for k_group_stage. SLIDE_LANES lanes per bead split the
+// window's slots and add their counts.
+template <int DIM, int STAGE>
+__global__ void __launch_bounds__(BS) k_slide_count(PoolArgs a, int j, int blk0)
+{
+    const int e = blk0 + (int)blockIdx.x / SLIDE_LANES;
+    const int r = a.blk_rep[e];
+    const RepParams& P = a.P[r];
+    if (!(P.flags & GCR_F_SLIDE) || !step_runs(a.S + r, j)) return;
+    const int g = threadIdx.x % SLIDE_LANES;
+    const int s = a.blk_slot0[e] + ((int)blockIdx.x % SLIDE_LANES) * (BS / SLIDE_LANES) +
+                  (int)threadIdx.x / SLIDE_LANES;
+    if (s >= P.bead_off + P.n) return;  // the whole group
+    const double* XA = STAGE == 0 ? a.X : a.X1;
+    double x[DIM], t1[DIM];
+    load_vec<DIM>(XA, s, x);
+    load_vec<DIM>(a.Tn, s, t1);
+    int cnt = slide_partners<DIM>(a, P, XA, s, x, t1, g, SLIDE_LANES);
+#pragma unroll
+    for (int k = 1; k < SLIDE_LANES; k <<= 1) cnt += __shfl_xor(cnt, k, 64);
+    if (g == 0) a.SCNT[s] = cnt;
 }
 
 // Second half of a force stage for the moving populations of a group and for
@@ -1254,7 +1291,7 @@ __global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
         if (P.flags & GCR_F_SLIDE) {
             double t1[DIM];
             load_vec<DIM>(a.Tn, s, t1);
-            const int cnt = slide_partners<DIM>(a, P, XA, s, x, t1);
+            const int cnt = a.SCNT[s];  // k_slide_count
             for (int k = 0; k < cnt; ++k)
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) f[i] += P.slide_v0 * t1[i];
