@@ -400,13 +400,15 @@ int rebuild_tables(gcr_pool* p)
             P.ntile = n;
             P.red_off = P.tile_off;
             P.red_n = P.ntile;
-            if (P.flags & (GCR_F_GROUP | GCR_F_SLIDE)) {  // maxima per bead block, k_group_stage
-                P.red_off = (int32_t)p->capTile + P.blk_off;
-                P.red_n = P.nblk;
-            }
         } else {
             P.red_off = P.blk_off;
             P.red_n = P.nblk;
+        }
+        // k_group_stage writes the maxima of its blocks (GROUP_LANES per bead
+        // block) after the tile and bead-block maxima
+        if (P.flags & (GCR_F_GROUP | GCR_F_SLIDE)) {
+            P.red_off = (int32_t)(p->capBlk + p->capTile) + GROUP_LANES * P.blk_off;
+            P.red_n = GROUP_LANES * P.nblk;
         }
     }
     rco[p->capR] = (int32_t)cr.size();
@@ -617,12 +619,13 @@ void launch_group(gcr_pool* p, int stage, int j)
             stage == 0 ? k_slide_count<3, 0><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off)
                        : k_slide_count<3, 1><<<nb, BS, 0, p->stream>>>(p->a, j, P.blk_off);
     }
+    const int nb = p->nblk * GROUP_LANES;
     if (p->dim == 2)
-        stage == 0 ? k_group_stage<2, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
-                   : k_group_stage<2, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
+        stage == 0 ? k_group_stage<2, 0><<<nb, BS, 0, p->stream>>>(p->a, j)
+                   : k_group_stage<2, 1><<<nb, BS, 0, p->stream>>>(p->a, j);
     else
-        stage == 0 ? k_group_stage<3, 0><<<p->nblk, BS, 0, p->stream>>>(p->a, j)
-                   : k_group_stage<3, 1><<<p->nblk, BS, 0, p->stream>>>(p->a, j);
+        stage == 0 ? k_group_stage<3, 0><<<nb, BS, 0, p->stream>>>(p->a, j)
+                   : k_group_stage<3, 1><<<nb, BS, 0, p->stream>>>(p->a, j);
 }
 
 // Motor bindings of a force stage (before the stage's second half).
@@ -914,9 +917,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CCOUNT, capC + 1));
     A(dalloc(&p->CSTART, capC + 1));
     A(dalloc(&p->CHKSUM, p->capChk));
-    // tile or block maxima, and the bead-block maxima of 2D group members
-    // after the tile range
-    A(dalloc(&p->BMAX, p->capBlk + p->capTile));
+    // tile or bead-block maxima of the force stages, then those of
+    // k_group_stage's blocks (GROUP_LANES per bead block)
+    A(dalloc(&p->BMAX, p->capBlk + p->capTile + GROUP_LANES * p->capBlk));
     p->capX = (int64_t)capR * GCR_MAX_POPS;
     A(dalloc(&p->XPAIRS, p->capX));
     A(dalloc(&p->GRP, capR));

@@ -18,6 +18,12 @@
 #define BS 256  // threads per bead block
 #define MOTOR_LANES 4  // lanes that scan the window of one motor together
 #define SLIDE_LANES 4  // lanes that scan the window of one bead for sliding partners
+// Lanes that scan the other populations' cells for one bead. One: the cross
+// forces are FP64 pair arithmetic for most candidates (unlike the sliding and
+// motor scans, which reject most of theirs on the range), and four lanes per
+// bead measured slower on 400k beads in two populations (47 against 45 us per
+// stage).
+#define GROUP_LANES 1
 #define ROD_MAX_CPW 21  // chains per wavefront of k_rod at most (chains of 3 beads)
 
 namespace gcr {
@@ -1162,10 +1168,13 @@ __device__ __forceinline__ void for_window(const PoolArgs& a, int cell_off, int 
 // by which Q's beads may have left their cells).
 template <int DIM>
 __device__ __forceinline__ void cross_sum(const PoolArgs& a, const RepParams& Q, const double* XQ,
-                                          const double* x, const XPair& e, double* f)
+                                          const double* x, const XPair& e, double* f, int first = 0,
+                                          int stride = 1)
 {
     const double rc2 = e.rc * e.rc;
-    for_window<DIM>(a, Q.cell_off, Q.nc, Q.inv_h, x, e.window, [&](int t) {
+    for_window<DIM>(
+        a, Q.cell_off, Q.nc, Q.inv_h, x, e.window,
+        [&](int t) {
         double q[DIM], sep[DIM], ds = 0.0;
         load_vec<DIM>(XQ, t, q);
 #pragma unroll
@@ -1191,7 +1200,8 @@ __device__ __forceinline__ void cross_sum(const PoolArgs& a, const RepParams& Q,
         const double cf = PairLaw<double>::coeff(ds, e.p0, e.p1);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) f[i] += sep[i] * cf;
-    });
+        },
+        first, stride);
 }
 
 // Active sliding partners of the bead in slot s (forces.py:79-83, 114-120):
@@ -1269,21 +1279,26 @@ __global__ void __launch_bounds__(BS) k_slide_count(PoolArgs a, int j, int blk0)
 }
 
 // Second half of a force stage for the moving populations of a group and for
-// replicas with active sliding, one thread per bead: the sliding pushes, then
-// the forces of the other populations, in the order
+// replicas with active sliding: the sliding pushes (partners from
+// k_slide_count), then the forces of the other populations, in the order
 // (Q - P) mod n (SRK.py:287-293), at their positions of this stage (x1 in
 // stage 1 unless fixed), added to the bead's own forces; then STAGE 0 the
-// predictor, STAGE 1 the block's max |F|^2 for k_finalize.
+// predictor, STAGE 1 the block's max |F|^2 for k_finalize. GROUP_LANES lanes
+// per bead split the cell windows of the other populations and add their
+// partial sums; the first lane writes.
 template <int DIM, int STAGE>
 __global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
 {
-    const int r = a.blk_rep[blockIdx.x];
+    const int eb = (int)blockIdx.x / GROUP_LANES;
+    const int r = a.blk_rep[eb];
     const RepParams& P = a.P[r];
     RepState* S = a.S + r;
     if (!(P.flags & (GCR_F_GROUP | GCR_F_SLIDE)) || !step_runs(S, j)) return;
-    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    const int g = threadIdx.x % GROUP_LANES;
+    const int s = a.blk_slot0[eb] + ((int)blockIdx.x % GROUP_LANES) * (BS / GROUP_LANES) +
+                  (int)threadIdx.x / GROUP_LANES;
     double fsq = 0.0;
-    if (s < P.bead_off + P.n) {
+    if (s < P.bead_off + P.n) {  // the whole group of lanes
         const double* XA = STAGE == 0 ? a.X : a.X1;
         double x[DIM], f[DIM];
         load_vec<DIM>(XA, s, x);
@@ -1295,7 +1310,7 @@ __global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
             for (int k = 0; k < cnt; ++k)
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) f[i] += P.slide_v0 * t1[i];
-            a.WA[s] = (STAGE == 0 ? -1 : a.WA[s]) + cnt;  // SRK.py:132, forces.py:119
+            if (g == 0) a.WA[s] = (STAGE == 0 ? -1 : a.WA[s]) + cnt;  // SRK.py:132, forces.py:119
         }
         for (int k = P.xin_off; k < P.xin_off + P.xin_n; ++k) {
             const XPair e = a.XPAIRS[k];
@@ -1309,22 +1324,34 @@ __global__ void __launch_bounds__(BS) k_group_stage(PoolArgs a, int j)
                 continue;
             }
             const double* XQ = (STAGE == 0 || a.S[e.src].fixed) ? a.X : a.X1;
-            cross_sum<DIM>(a, a.P[e.src], XQ, x, e, f);
+            double fc[DIM];
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) fc[i] = 0.0;
+            cross_sum<DIM>(a, a.P[e.src], XQ, x, e, fc, g, GROUP_LANES);
+            // butterfly: every lane ends with the same sum (each add commutes)
+#pragma unroll
+            for (int m = 1; m < GROUP_LANES; m <<= 1)
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) fc[i] += __shfl_xor(fc[i], m, 64);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) f[i] += fc[i];
         }
         if (STAGE == 0) {
-            store_vec<DIM>(a.F0, s, f);
-            // a bound motor that does not diffuse keeps its last x1
-            // (SRK.py:389-396 writes only the movers)
-            if (!((P.flags & GCR_F_MSTILL) && a.WA[s] >= 0)) virtual_update<DIM>(a, P, S, s, x, f, 1.0, 0);
+            if (g == 0) {
+                store_vec<DIM>(a.F0, s, f);
+                // a bound motor that does not diffuse keeps its last x1
+                // (SRK.py:389-396 writes only the movers)
+                if (!((P.flags & GCR_F_MSTILL) && a.WA[s] >= 0)) virtual_update<DIM>(a, P, S, s, x, f, 1.0, 0);
+            }
         } else {
-            store_vec<DIM>(a.FT, s, f);
+            if (g == 0) store_vec<DIM>(a.FT, s, f);
 #pragma unroll
             for (int i = 0; i < DIM; ++i) fsq += f[i] * f[i];
         }
     }
     if (STAGE == 1) {
         const double m = block_max(fsq);
-        if (threadIdx.x == 0) a.BMAX[P.red_off + (blockIdx.x - P.blk_off)] = m;
+        if (threadIdx.x == 0) a.BMAX[P.red_off + ((int)blockIdx.x - GROUP_LANES * P.blk_off)] = m;
     }
 }
 
@@ -1389,8 +1416,7 @@ __global__ void __launch_bounds__(BS) k_motor(PoolArgs a, int j, int blk0)
         for_window<DIM>(
             a, T.cell_off, nc, T.inv_h, xm, w,
             [&](int t) {
-                const int o = a.META[2 * t];
-                if (o <= last || o >= best) return;
+                // range first: most candidates are out of reach
                 double xb[DIM], d2 = 0.0;
                 load_vec<DIM>(XT, t, xb);
 #pragma unroll
@@ -1403,6 +1429,8 @@ __global__ void __launch_bounds__(BS) k_motor(PoolArgs a, int j, int blk0)
                     d2 += sep * sep;
                 }
                 if (d2 > reach2) return;
+                const int o = a.META[2 * t];
+                if (o <= last || o >= best) return;
                 best = o;
                 bslot = t;
                 bd2 = d2;
