@@ -134,11 +134,11 @@ def cpu_baseline(w, budget_s):
 
 
 def main_ensemble(args):
-    """256 replicas of uneven size (config 4) in one pool vs one at a time."""
+    """256 replicas of uneven size in one pool vs one at a time."""
     from gcr_amd import ensemble as ENS
     from gcr_amd import workloads as W
     t0 = time.perf_counter()
-    ws = W.config4_replicas(seed=0, n_replicas=args.replicas, steps=args.steps)
+    ws = W.uneven_replicas(seed=0, n_replicas=args.replicas, steps=args.steps)
     t_init = time.perf_counter() - t0
     sec, bs, pool, rids = ENS.run_batched(ws, args.steps, steps_per_batch=args.batch,
                                           rebuild_every=args.rebuild, warmup=args.warmup)
@@ -155,7 +155,7 @@ def main_ensemble(args):
            "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 forces + f64 state", "data": "synthetic",
-           "config": {"workload": "config 4: chain counts log-U[50,2000], lengths U{5..40}, "
+           "config": {"workload": "256 replicas: chain counts log-U[50,2000], lengths U{5..40}, "
                                   "chirality U[-0.5,0.5], v0 U[0.1,2], phi U[0.1,0.5], WCA, "
                                   "dt=1e-3", **ENS.summary(ws)},
            "sequential": {"value": sbs / ssec, "replicas_timed": len(sample),
@@ -166,7 +166,7 @@ def main_ensemble(args):
 
 
 def main_serve(args):
-    """Poisson request stream (config 5) through the continuous-batching server."""
+    """Poisson request stream through the continuous-batching server."""
     from gcr_amd import serving as SV
     t0 = time.perf_counter()
     reqs = SV.make_stream(seed=0, n_requests=args.requests, scale_steps=args.steps_scale)
@@ -181,7 +181,7 @@ def main_serve(args):
            "value": rep["bead_steps_per_s"], "unit": "bead-steps/s", "n_gpus": 1,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 forces + f64 state", "data": "synthetic",
-           "config": {"workload": "config 5: Poisson 20 req/s, beads log-U[1k,200k], steps "
+           "config": {"workload": "request stream: Poisson 20 req/s, beads log-U[1k,200k], steps "
                                   "log-U[1e3,1e5] x steps_scale, ABPs (D_r=0.1) and chains "
                                   "5-40, 2D and 3D", "requests": len(reqs),
                       "steps_scale": args.steps_scale, "hbm_budget_gb": args.budget_gb},
@@ -247,7 +247,7 @@ def main_features(args):
     from gcr_amd import grow as G
     from gcr_amd import workloads as W
     prec = E.GCR_MIXED
-    base = W.config3(seed=1, n_chains=args.chains)
+    base = W.chains_20kx20(seed=1, n_chains=args.chains)
     base.phys.skin = 0.3 * float(base.phys.wca[3])
     steps, warm = args.steps, args.warmup
     out = {}
@@ -311,7 +311,7 @@ def main_features(args):
 
     def growth():
         # the growth schedule (control.py:153-219) for 2000 chains of 20 from points
-        g = W.config3(seed=2, n_chains=2000)
+        g = W.chains_20kx20(seed=2, n_chains=2000)
         pos0, t0_, cid = G.random_initialization(2000, 20, 0.5, g.phys.L, 2, seed=3)
         tg = time.perf_counter()
         _, _, info = G.grow(g.phys, pos0, t0_, cid, 0.5, 20, g.time_vals, t_init_grow=0.2,
@@ -338,7 +338,7 @@ def main_features(args):
             out[name] = run()
     res = {"metric": "bead-steps/sec of each step option (1 GPU)", "unit": "bead-steps/s",
            "dtype": "f32 forces + f64 state", "data": "synthetic", "steps": steps,
-           "warmup": warm, "config": {"workload": "config 3 (400k beads) and its variants",
+           "warmup": warm, "config": {"workload": "20k chains x 20 beads (400k) and its variants",
                                       "skin": base.phys.skin, "rebuild_every": 32, "tile": 8,
                                       "threads": 256},
            "features": out, "wall_s": time.perf_counter() - t0}
@@ -376,7 +376,7 @@ def main():
     from gcr_amd import engine as E
     from gcr_amd import workloads as W
     from gcr_amd._lib import device_info
-    w = W.config3(seed=1, n_chains=args.chains)
+    w = W.chains_20kx20(seed=1, n_chains=args.chains)
     dev = device_info()
     res, tuned = {}, {}
     for name, prec in (("fp64", E.GCR_FP64), ("mixed", E.GCR_MIXED)):

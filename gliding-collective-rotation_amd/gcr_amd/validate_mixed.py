@@ -71,7 +71,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--every", type=int, default=100)
     args = ap.parse_args()
-    w = W.config3(seed=1, n_chains=args.chains)
+    w = W.chains_20kx20(seed=1, n_chains=args.chains)
     L = w.phys.L
     pts = [1, 10, 100, 1000, args.steps]
     t0 = time.perf_counter()

@@ -111,28 +111,28 @@ def walled(w, walls, zone, margin=0.8):
     return out
 
 
-def config1(seed=0, steps=10000):
+def chains_4x7(seed=0, steps=10000):
     """2D box 20, 4 chains x 7 beads, WCA, k=100, no bending, v0=1, T=0.01, dt=1e-3."""
-    return chains("config1", 2, 4, 7, L=20.0, kappa=0.0, T=0.01, dtmax=1e-3, steps=steps,
+    return chains("chains_4x7", 2, 4, 7, L=20.0, kappa=0.0, T=0.01, dtmax=1e-3, steps=steps,
                   seed=seed)
 
 
-def config2(seed=0, steps=100000, n_chains=1000):
+def chains_1000x20(seed=0, steps=100000, n_chains=1000):
     """2D, 1000 chains x 20 (20k beads), phi=0.3, WCA, kappa=10, chirality 0.2, T=0.01."""
-    return chains("config2", 2, n_chains, 20, phi=0.3, kappa=10.0, vtan=0.2, T=0.01,
+    return chains("chains_1000x20", 2, n_chains, 20, phi=0.3, kappa=10.0, vtan=0.2, T=0.01,
                   dtmax=1e-3, steps=steps, seed=seed)
 
 
-def config3(seed=0, steps=10000, n_chains=20000):
+def chains_20kx20(seed=0, steps=10000, n_chains=20000):
     """2D, 20k chains x 20 (400k beads), phi=0.4, LJ 2.5 sigma, kappa=5, chirality
     U[-0.5, 0.5] per chain, v0=1, T=0.05, dt=5e-4."""
     rng = np.random.default_rng(seed + 12345)
     vt = rng.uniform(-0.5, 0.5, n_chains)
-    return chains("config3", 2, n_chains, 20, phi=0.4, cutoff_factor=LJ_25_SIGMA, kappa=5.0,
+    return chains("chains_20kx20", 2, n_chains, 20, phi=0.4, cutoff_factor=LJ_25_SIGMA, kappa=5.0,
                   vtan=vt, T=0.05, dtmax=5e-4, steps=steps, seed=seed)
 
 
-def config4_replicas(seed=0, n_replicas=256, steps=10000):
+def uneven_replicas(seed=0, n_replicas=256, steps=10000):
     """Ensemble: chain counts log-U[50, 2000], lengths U{5..40}, chirality U[-0.5, 0.5],
     speed U[0.1, 2], area fraction U[0.1, 0.5], WCA, dt=1e-3."""
     rng = np.random.default_rng(seed)

@@ -83,7 +83,7 @@ def test_workload_constants():
     assert mdps == pytest.approx(0.5 * W.LJ_25_SIGMA / 40)
     p = W.wca_params(1.0, 1.0, 0.0, W.LJ_25_SIGMA)
     assert p[3] == pytest.approx(2.5 / 2 ** (1 / 6)) and p[0] == 12.0
-    w = W.config1(seed=1)
+    w = W.chains_4x7(seed=1)
     assert w.n == 28 and w.phys.D == 0.01 and w.phys.Dr == 0.0075
     assert w.time_vals[0] == pytest.approx(1e-7) and w.time_vals[1] == 1e-3
     assert w.phys.spring[4] == 1.0 and w.phys.spring[1] == 0.0

@@ -32,11 +32,11 @@ walled = W.walled
 
 def cases():
     out = {}
-    out["config1"] = W.config1(seed=1, steps=100)
-    out["config2_cut"] = W.chains("config2_cut", 2, 100, 20, phi=0.3, kappa=10.0, vtan=0.2,
+    out["chains_4x7"] = W.chains_4x7(seed=1, steps=100)
+    out["chains_wca_cut"] = W.chains("chains_wca_cut", 2, 100, 20, phi=0.3, kappa=10.0, vtan=0.2,
                                   T=0.01, dtmax=1e-3, steps=100, seed=2)
     rng = np.random.default_rng(3)
-    out["config3_cut"] = W.chains("config3_cut", 2, 100, 20, phi=0.4,
+    out["chains_lj_cut"] = W.chains("chains_lj_cut", 2, 100, 20, phi=0.4,
                                   cutoff_factor=W.LJ_25_SIGMA, kappa=5.0,
                                   vtan=rng.uniform(-0.5, 0.5, 100), T=0.05, dtmax=5e-4,
                                   steps=100, seed=3)
@@ -180,7 +180,7 @@ def test_srk2_100_steps_injected_noise(name):
 def test_in_kernel_philox_matches_oracle_generator():
     """xi = None on both sides: the GPU's in-kernel Philox/Box-Muller noise and
     the oracle's orc_bead_noise give the same trajectory."""
-    w = CASES["config2_cut"]
+    w = CASES["chains_wca_cut"]
     tv = w.time_vals.copy()
     tv[7] = 100
     pos_o, _, _, _, st_o = oracle_run(w, None, tv, seed=w.phys.seed)
@@ -194,7 +194,7 @@ def test_in_kernel_philox_matches_oracle_generator():
 def test_two_calls_equal_one_call():
     """Chunked like the reference driver: the second call starts from the
     first call's dt (status.dt, updates.py:249-256) and the next noise steps."""
-    w = CASES["config1"]
+    w = CASES["chains_4x7"]
     rng = np.random.default_rng(9)
     xi = rng.normal(size=(100, w.n, 4))
     tv = w.time_vals.copy()
@@ -214,7 +214,7 @@ def test_two_calls_equal_one_call():
 
 
 def test_max_dt_usable_clips_time():
-    w = CASES["config1"]
+    w = CASES["chains_4x7"]
     rng = np.random.default_rng(10)
     xi = rng.normal(size=(50, w.n, 4))
     tv = w.time_vals.copy()
@@ -232,7 +232,7 @@ def test_max_dt_usable_clips_time():
 
 
 def test_deterministic_bits():
-    w = CASES["config3_cut"]
+    w = CASES["chains_lj_cut"]
     tv = w.time_vals.copy()
     tv[7] = 60
     res = []
@@ -250,7 +250,7 @@ def test_rebuild_period_and_replay(order):
     """Results do not depend on the cell-list schedule beyond rounding; a skin
     too thin for the schedule is caught and the batch replayed (SRK2, and
     SRK4 with its three virtual stages)."""
-    w = CASES["config2_cut"]
+    w = CASES["chains_wca_cut"]
     rng = np.random.default_rng(11)
     xi = rng.normal(size=(64, w.n, 4))
     tv = w.time_vals.copy()
@@ -311,7 +311,7 @@ def test_alone_vs_mixed_pool_bit_identical():
 
 
 @pytest.mark.parametrize("precision", [E.GCR_FP64, E.GCR_MIXED])
-@pytest.mark.parametrize("name,sides", [("config3_cut", ((1, 256), (2, 512), (5, 1024), (16, 512))),
+@pytest.mark.parametrize("name,sides", [("chains_lj_cut", ((1, 256), (2, 512), (5, 1024), (16, 512))),
                                         ("dense_cells", ((1, 256), (3, 512))),
                                         ("list_overflow", ((1, 512), (4, 256)))])
 def test_tile_shape_does_not_change_results(name, sides, precision):
@@ -343,7 +343,7 @@ def test_tile_shape_does_not_change_results(name, sides, precision):
 
 
 def test_mixed_precision_force_error():
-    w = CASES["config3_cut"]
+    w = CASES["chains_lj_cut"]
     F_o, _ = O.force_eval(sysd_of(w), w.pos, w.chain_id, w.rest_lengths, w.v_tan_angle,
                           w.tangent)
     s = gpu_system(w, precision=E.GCR_MIXED)
