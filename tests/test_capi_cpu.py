@@ -87,3 +87,30 @@ def test_workload_constants():
     assert w.n == 28 and w.phys.D == 0.01 and w.phys.Dr == 0.0075
     assert w.time_vals[0] == pytest.approx(1e-7) and w.time_vals[1] == 1e-3
     assert w.phys.spring[4] == 1.0 and w.phys.spring[1] == 0.0
+
+
+def test_cell_grid_sizes():
+    """gcr_replica_cells: with a pair force the cell edge is at least the
+    cutoff plus the skin; without one (a motor population, say) the grid holds
+    about four beads per cell, and tiny populations keep a single cell."""
+    from gcr_amd import engine as E
+    lib = _lib.load()
+
+    def cells(**kw):
+        p = E.Physics(D=0.0, Dr=0.0, **kw)
+        return int(lib.gcr_replica_cells(p.dim, C.byref(p.desc())))
+
+    L = 100.0
+    wca = np.array([12.0, 12.0, 0.0, 2.5])
+    nc = int(np.floor(L / (2.5 + 0.3)))
+    assert cells(dim=2, n_beads=1000, n_chains=1000, L=L, wca=wca, skin=0.3) == nc * nc
+    assert cells(dim=3, n_beads=1000, n_chains=1000, L=L, wca=wca, skin=0.3) == nc ** 3
+    free = dict(n_chains=1, wca=np.array([0.0, 0.0, 0.0, 1.0]), has_wca=False, has_springs=False,
+                has_propel=False, skin=0.3)
+    n = 40000
+    side = int(np.floor(L / np.sqrt(4.0 * L * L / n)))
+    assert cells(dim=2, n_beads=n, L=L, **free) == side * side
+    assert 3.0 <= n / side ** 2 <= 5.0
+    side3 = int(np.floor(L / (4.0 * L ** 3 / n) ** (1 / 3)))
+    assert cells(dim=3, n_beads=n, L=L, **free) == side3 ** 3
+    assert cells(dim=2, n_beads=8, L=L, **free) == 1
