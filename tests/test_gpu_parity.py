@@ -76,6 +76,15 @@ def cases():
     w = W.chains("rod3d", 3, 40, 6, L=12.0, kappa=4.0, T=0.01, steps=100, seed=15)
     w.phys.rod = 1.0
     out["rod3d"] = w
+    # the rod kernel's two layouts at their ends: chains of 80 beads (segments
+    # of 64 lanes) and of 3 beads (21 chains per wavefront)
+    w = W.chains("rod_long2d", 2, 8, 80, L=200.0, kappa=4.0, vtan=0.1, T=0.01, dtmax=1e-3,
+                 steps=100, seed=24)
+    w.phys.rod = 0.5
+    out["rod_long2d"] = w
+    w = W.chains("rod_short3d", 3, 200, 3, L=12.0, kappa=2.0, T=0.01, steps=100, seed=25)
+    w.phys.rod = 1.0
+    out["rod_short3d"] = w
     # SRK_order 4 (SRK.py:186-321): chains, 3D ABPs, and with the rigid rod
     w = W.chains("srk4_chains", 2, 100, 20, phi=0.3, kappa=10.0, vtan=0.2, T=0.01, dtmax=1e-3,
                  steps=100, seed=16)
