@@ -192,8 +192,8 @@ __device__ __forceinline__ TileLayout tile_layout(const PoolArgs& a, const RepPa
     return t;
 }
 
-// Halo cell and global slot of staged bead i.
-__device__ __forceinline__ void staged_bead(const TileShared& sh, const TileLayout& t, int i, int& hc, int& g)
+// Halo cell of staged bead i: a binary search over the cells' LDS offsets.
+__device__ __forceinline__ int staged_cell(const TileShared& sh, const TileLayout& t, int i)
 {
     int lo = 0, hi = t.nhalo;  // cst[lo] <= i < cst[hi]
     while (hi - lo > 1) {
@@ -203,22 +203,33 @@ __device__ __forceinline__ void staged_bead(const TileShared& sh, const TileLayo
         else
             hi = mid;
     }
-    hc = lo;
-    g = sh.gst[lo] + (i - sh.cst[lo]);
+    return lo;
 }
 
 // Stage the halo's positions (and, for the list builder, (orig, chain)), and
-// the halo cell of every staged bead.
+// the halo cell of every staged bead. FP32 positions: each thread first marks
+// the LDS range of its cells and every staged bead reads its cell from there
+// (one LDS read instead of a search). FP64 positions: the search, which keeps
+// the FP64 stage kernels within 128 VGPRs (four waves per SIMD at 512 threads;
+// the marking pass costs them 14 registers). Block-uniform call.
 template <typename FT, int NT>
 __device__ __forceinline__ void stage_positions(const PoolArgs& a, const RepParams& P, const double* Xa,
                                                 const TileShared& sh, const TileLayout& t, LPos<FT>* lp,
                                                 int2* lm, uint16_t* lcell)
 {
     const int nc = P.nc;
+    constexpr bool mark = sizeof(FT) == 4;
+    if (mark) {
+        for (int c = threadIdx.x; c < t.nhalo; c += NT) {
+            const int i1 = sh.cst[c + 1];
+            for (int i = sh.cst[c]; i < i1; ++i) lcell[i] = (uint16_t)c;
+        }
+        __syncthreads();
+    }
     for (int i = threadIdx.x; i < t.total; i += NT) {
-        int hc, g;
-        staged_bead(sh, t, i, hc, g);
-        lcell[i] = (uint16_t)hc;
+        const int hc = mark ? (int)lcell[i] : staged_cell(sh, t, i);
+        if (!mark) lcell[i] = (uint16_t)hc;
+        const int g = sh.gst[hc] + (i - sh.cst[hc]);
         const int hx = hc % t.W, hy = hc / t.W;
         const int gx = (t.cx0 - 1 + hx + nc) % nc;
         const int gy = (t.cy0 - 1 + hy + nc) % nc;
