@@ -266,3 +266,24 @@ def test_diffusing_motors_3d():
     motors = motor_population(3, 8.0, 200, seed=14, D=0.05, max_dist=0.6)
     was = run_motors([chains, motors], {(0, 1): "motor"})
     assert np.sum(was[1] >= 0) > 3
+
+
+def test_moving_population_without_pair_force():
+    """A population with no forces among its own beads (no "WCA" in its list)
+    that the chains act on through their cross law: its grid holds about four
+    beads per cell, its beads move farther than half a cell edge within the
+    run, so the chains' scans of its cells stay complete only through its
+    re-sorts; positions as the oracle's."""
+    import ctypes as C
+    from gcr_amd import _lib
+    pops = make_pops(2, 20.0, [dict(n_chains=20, chain_len=6),
+                               dict(n_chains=150, chain_len=1, T=3.0, v0=8.0)], seed=41)
+    pops[1]["phys"].has_wca = False
+    laws = averaged_laws(pops)
+    assert cross_contacts(pops, laws) > 10
+    out_o, st_o, got, sts = run_both(pops, laws, steps=300)
+    check(pops, out_o, st_o, got, sts)
+    nc = round(np.sqrt(_lib.load().gcr_replica_cells(2, C.byref(pops[1]["phys"].desc()))))
+    assert nc > 3
+    moved = np.sqrt((N.min_image(out_o[1][0] - pops[1]["pos"], 20.0) ** 2).sum(axis=1))
+    assert moved.max() > 0.5 * 20.0 / nc
