@@ -191,6 +191,40 @@ def main_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_option(w, budget_s):
+    """The C/OpenMP FP64 oracle's SRK loop (Verlet list rebuilt every 40
+    steps, Newton-3 inside each thread chunk) on the single-population
+    workload w with all of its physics options (SRK order, FENE, walls, rigid
+    rod, active sliding over the Verlet list with w.chunks per chain), in
+    chunks of 40 steps until about budget_s seconds of CPU work."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_lib as O
+    threads = min(16, os.cpu_count() or 1)
+    p = w.phys
+    sysd = dict(dim=p.dim, n=w.n, n_nearby=p.n_nearby, has_springs=p.has_springs,
+                has_wca=p.has_wca, has_propel=p.has_propel, L=p.L, D=p.D, Dr=p.Dr,
+                radius=p.radius, wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0,
+                rod=p.rod, order=p.srk_order, walls=p.walls, wall_zone=p.wall_zone,
+                wall_cutoff=p.wall_cutoff)
+    if p.sliding_maxdist > 0:  # the chain chunks per bead, where_active
+        sysd.update(sliding_maxdist=p.sliding_maxdist, sliding_v0=p.sliding_v0,
+                    chunk=np.ascontiguousarray(w.chunks[w.chain_id], dtype=np.int32),
+                    where_active=np.zeros(w.n, dtype=np.int32))
+    tv = w.time_vals.copy()
+    pos, tang = w.pos, w.tangent
+    done, sec = 0, 0.0
+    while sec < budget_s and done < 100000:
+        tv[7] = 40
+        t0 = time.perf_counter()
+        pos, _, tang, _, st = O.srk2_run(sysd, pos, tang, w.chain_id, w.rest_lengths,
+                                         w.v_tan_angle, tv, xi=None, seed=1, step0=done,
+                                         pair_mode=1, n_threads=threads)
+        sec += time.perf_counter() - t0
+        tv[0] = st[0]
+        done += 40
+    return dict(bead_steps_per_s=w.n * done / sec, steps=done, seconds=sec, threads=threads)
+
+
 def time_pool(reps, steps, warmup, precision, dim=2, group=None, chunks=None, batch=64,
               rebuild=32, tile=8, threads=256):
     """Bead-steps/s of the replicas `reps` (Workloads) advanced together in one
@@ -258,18 +292,28 @@ def main_features(args):
             setattr(w.phys, k, v)
         return w
 
-    def walls():
-        return time_pool([W.walled(base, 3, 1.25)], steps, warm, prec)
-
-    def fene():
+    def fene_w():
         w = variant()
         w.phys.spring = w.phys.spring.copy()
         w.phys.spring[3], w.phys.spring[5] = 1.0, 0.3
-        return time_pool([w], steps, warm, prec)
+        return w
 
-    def sliding():
+    # single-population options: (workload, algorithmic HBM bytes per
+    # bead-step: x read, x1 written, x1 read, x read, x written, 8 B per
+    # coordinate; SRK4 moves positions through twice as many passes)
+    single = {
+        "srk2 (reference step)": (lambda: base, 80),
+        "srk4": (lambda: variant(srk_order=4), 160),
+        "fene": (fene_w, 80),
+        "walls (x and y)": (lambda: W.walled(base, 3, 1.25), 80),
+        "rigid rod 0.6": (lambda: variant(rod=0.6), 80),
+        "active sliding": (sliding_w, 80),
+    }
+
+    def sliding_w():
         w = variant(has_propel=False, sliding_maxdist=1.5, sliding_v0=1.0)
-        return time_pool([w], steps, warm, prec, chunks={0: dropin.chain_chunks(3, w.chain_id)})
+        w.chunks = dropin.chain_chunks(3, w.chain_id)  # three threads' chunks
+        return w
 
     def two_pops():
         # the same beads as two populations of 10k chains acting on each
@@ -321,21 +365,43 @@ def main_features(args):
                     ms_per_step=1e3 * sec / info["steps"], beads=g.n, steps=info["steps"],
                     chunks=info["chunks"])
 
-    options = {
-        "srk2 (reference step)": lambda: time_pool([base], steps, warm, prec),
-        "srk4": lambda: time_pool([variant(srk_order=4)], steps, warm, prec),
-        "fene": fene,
-        "walls (x and y)": walls,
-        "rigid rod 0.6": lambda: time_pool([variant(rod=0.6)], steps, warm, prec),
-        "active sliding": sliding,
+    def single_run(mk):
+        w = mk()
+        chunks = {0: w.chunks} if getattr(w, "chunks", None) is not None else None
+        return time_pool([w], steps, warm, prec, chunks=chunks)
+
+    options = {name: (lambda mk=mk: single_run(mk)) for name, (mk, _) in single.items()}
+    options.update({
         "two populations": two_pops,
         "chains + 100k fixed motors": motors,
         "growth, 2000 chains x 20 (FP64)": growth,
-    }
+    })
+    no_cpu = ("not timed: the oracle restates the populations' scans by brute force over "
+              "all bead pairs, which is no baseline at 400k beads")
     t0 = time.perf_counter()
     for name, run in options.items():
-        if args.only in name:
-            out[name] = run()
+        if args.only not in name:
+            continue
+        r = run()
+        if name in single:
+            # the whole step against HBM's 8 TB/s at the algorithmic bytes
+            bps = single[name][1]
+            gbs = bps * r["bead_steps_per_s"] / 1e9
+            r["step_roofline"] = dict(bound="hbm", bytes_per_bead_step=bps,
+                                      achieved=gbs, peak=8000.0, unit="GB/s", frac=gbs / 8000.0)
+            if not args.no_cpu:
+                c = cpu_option(single[name][0](), args.cpu_seconds / 2)
+                r["cpu_baseline"] = dict(value=c["bead_steps_per_s"], unit="bead-steps/s",
+                                         cores=c["threads"], kind="port",
+                                         sample=f"{c['steps']} steps of the same system "
+                                                f"({c['seconds']:.1f} s)")
+                r["gpu_over_cpu"] = r["bead_steps_per_s"] / c["bead_steps_per_s"]
+        elif name != "growth, 2000 chains x 20 (FP64)":
+            gbs = 80 * r["bead_steps_per_s"] / 1e9
+            r["step_roofline"] = dict(bound="hbm", bytes_per_bead_step=80, achieved=gbs,
+                                      peak=8000.0, unit="GB/s", frac=gbs / 8000.0)
+            r["cpu_baseline"] = no_cpu
+        out[name] = r
     res = {"metric": "bead-steps/sec of each step option (1 GPU)", "unit": "bead-steps/s",
            "dtype": "f32 forces + f64 state", "data": "synthetic", "steps": steps,
            "warmup": warm, "config": {"workload": "20k chains x 20 beads (400k) and its variants",

@@ -368,6 +368,38 @@ static void verlet_forces(const verlet* v, int dim, const double* pos, double L,
     }
 }
 
+/* Active sliding over the Verlet list, where the reference computes it
+ * (inside central_pair_force's neighbour loop, forces.py:79-83, 114-120):
+ * the rules of orc_sliding, whose count it equals while the sliding range
+ * lies within the list radius. Each thread writes its own beads only. */
+static void verlet_sliding(const verlet* v, int dim, const double* pos, double L, const double* t,
+                           double maxdist, double v0, const int32_t* chunk, double* F,
+                           int32_t* wa)
+{
+    double half = L / 2.0, md2 = maxdist * maxdist;
+#pragma omp parallel for schedule(static, 1) num_threads(v->n_threads)
+    for (int th = 0; th < v->n_threads; ++th)
+        for (int b1 = v->chunk[th]; b1 < v->chunk[th + 1]; ++b1)
+            for (int k = v->start[b1]; k < v->start[b1 + 1]; ++k) {
+                int b2 = v->list[k];
+                if ((chunk ? chunk[b1] == chunk[b2] : 1) && b2 >= b1) continue;
+                double ds = 0.0, dot = 0.0;
+                for (int i = 0; i < dim; ++i) {
+                    double sep = pos[b2 * dim + i] - pos[b1 * dim + i];
+                    if (sep < -half)
+                        sep += L;
+                    else if (sep >= half)
+                        sep -= L;
+                    ds += sep * sep;
+                    dot += t[b1 * dim + i] * t[b2 * dim + i];
+                }
+                if (ds < md2 && dot < 0) {
+                    for (int i = 0; i < dim; ++i) F[b1 * dim + i] += v0 * t[b1 * dim + i];
+                    if (wa) wa[b1] += 1;
+                }
+            }
+}
+
 void orc_walls(int dim, int n, const double* pos, double L, const double wca[4], int walls,
                double zone, double* F)
 {
@@ -415,9 +447,14 @@ void orc_force_eval(const orc_sys* s, const double* pos, const int64_t* chain_id
             orc_chain_forces(dim, n, pos, chain_id, s->L, s->spring, rest_lengths, tangent, F);
         }
     }
-    if (s->sliding_maxdist > 0)
-        orc_sliding(dim, n, pos, chain_id, s->L, tangent, s->n_nearby, s->sliding_maxdist,
-                    s->sliding_v0, s->chunk, F, s->where_active);
+    if (s->sliding_maxdist > 0) {
+        if (pair_mode == 1 && vv)
+            verlet_sliding((const verlet*)vv, dim, pos, s->L, tangent, s->sliding_maxdist,
+                           s->sliding_v0, s->chunk, F, s->where_active);
+        else
+            orc_sliding(dim, n, pos, chain_id, s->L, tangent, s->n_nearby, s->sliding_maxdist,
+                        s->sliding_v0, s->chunk, F, s->where_active);
+    }
     if (s->has_wca) {
         if (pair_mode == 1 && vv)
             verlet_forces((const verlet*)vv, dim, pos, s->L, s->wca, F);

@@ -709,6 +709,35 @@ def test_sliding_pair_motion_and_where_active():
     assert wa1.tolist() == [-1, 1]
 
 
+def test_sliding_over_verlet_list_equals_all_pairs():
+    """The CPU baseline's sliding over the Verlet list (where the reference
+    computes it) counts the partners the all-pairs loop counts, on chains with
+    the LJ 2.5 sigma cutoff (the sliding range lies inside the list radius):
+    where_active equal, positions equal to rounding (the pair sums' order
+    differs between the two loops)."""
+    from gcr_amd import dropin
+    from gcr_amd import workloads as W
+    w = W.chains("slide_lj", 2, 150, 10, phi=0.35, cutoff_factor=W.LJ_25_SIGMA, kappa=6.0,
+                 T=0.01, steps=80, seed=61)
+    chunk = np.ascontiguousarray(dropin.chain_chunks(3, w.chain_id)[w.chain_id], dtype=np.int32)
+    p = w.phys
+    runs = []
+    for mode in (0, 1):
+        wa = np.zeros(w.n, dtype=np.int32)
+        sysd = dict(dim=2, n=w.n, n_nearby=p.n_nearby, has_springs=True, has_wca=True,
+                    has_propel=False, L=p.L, D=p.D, Dr=p.Dr, radius=p.radius,
+                    wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0,
+                    sliding_maxdist=1.5, sliding_v0=1.0, chunk=chunk, where_active=wa)
+        tv = w.time_vals.copy()
+        tv[7] = 80
+        out = O.srk2_run(sysd, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle, tv,
+                         seed=3, pair_mode=mode, n_threads=4)
+        runs.append((out, wa))
+    (o0, wa0), (o1, wa1) = runs
+    assert np.array_equal(wa0, wa1) and np.sum(wa0 > 0) > 20
+    assert np.max(np.abs(N.min_image(o0[0] - o1[0], p.L))) < 1e-11 * p.L
+
+
 def motor_pops(bead_pos, bead_t, motor_pos, k_on, k_off, D_m=0.0, still=True, imparts=1.0,
                max_dist=0.5, L=L2D):
     nb, nm = len(bead_pos), len(motor_pos)
