@@ -225,6 +225,46 @@ def cpu_option(w, budget_s):
     return dict(bead_steps_per_s=w.n * done / sec, steps=done, seconds=sec, threads=threads)
 
 
+def cpu_groups(reps, laws, fixed, budget_s):
+    """The C/OpenMP FP64 oracle's multi-population SRK2 loop on the
+    populations `reps` (Workloads) with the laws {(P, Q): WCA vector or
+    "motor"} and the fixed indices: each population's own pairs over its
+    Verlet list, the other populations and the motors through cell grids,
+    in chunks of 40 steps until about budget_s seconds."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_lib as O
+    threads = min(16, os.cpu_count() or 1)
+    pops = []
+    for k, w in enumerate(reps):
+        p = w.phys
+        sysd = dict(dim=p.dim, n=w.n, n_nearby=p.n_nearby, has_springs=p.has_springs,
+                    has_wca=p.has_wca, has_propel=p.has_propel, L=p.L, D=p.D, Dr=p.Dr,
+                    radius=p.radius, wca=np.asarray(p.wca), spring=np.asarray(p.spring),
+                    v0=p.v0, where_active=np.zeros(w.n, dtype=np.int32))
+        d = dict(sysd=sysd, pos=w.pos, tangent=w.tangent, chain_id=w.chain_id,
+                 rest_lengths=w.rest_lengths, vtan=w.v_tan_angle, seed=p.seed,
+                 fixed=k in fixed)
+        if p.motor:
+            d["motor"] = dict(max_dist=p.motor_max_dist, imparts=p.motor_imparts, k_on=p.k_on,
+                              k_off=p.k_off, reach=p.motor_reach or p.motor_max_dist,
+                              still=p.motor_still)
+        pops.append(d)
+    moving = [k for k in range(len(reps)) if k not in fixed]
+    tv = reps[moving[0]].time_vals.copy()
+    done, sec = 0, 0.0
+    while sec < budget_s and done < 100000:
+        tv[7] = 40
+        t0 = time.perf_counter()
+        out, st = O.multi_srk2_run(pops, laws, tv, step0=done, pair_mode=1, n_threads=threads)
+        sec += time.perf_counter() - t0
+        for k in moving:
+            pops[k]["pos"], pops[k]["tangent"] = out[k][0], out[k][2]
+        tv[0] = st[0]
+        done += 40
+    moved = sum(reps[k].n for k in moving)
+    return dict(bead_steps_per_s=moved * done / sec, steps=done, seconds=sec, threads=threads)
+
+
 def time_pool(reps, steps, warmup, precision, dim=2, group=None, chunks=None, batch=64,
               rebuild=32, tile=8, threads=256):
     """Bead-steps/s of the replicas `reps` (Workloads) advanced together in one
@@ -315,7 +355,7 @@ def main_features(args):
         w.chunks = dropin.chain_chunks(3, w.chain_id)  # three threads' chunks
         return w
 
-    def two_pops():
+    def two_pops_set():
         # the same beads as two populations of 10k chains acting on each
         # other with their own law (the union of the reference step)
         half = args.chains // 2
@@ -332,9 +372,9 @@ def main_features(args):
             w.phys.seed = 100 + lo
             parts.append(w)
         law = np.asarray(base.phys.wca)
-        return time_pool(parts, steps, warm, prec, group=({(0, 1): law, (1, 0): law}, set()))
+        return parts, {(0, 1): law, (1, 0): law}, set()
 
-    def motors():
+    def motors_set():
         # chains whose v0 is the motors' push (config.py:95-98) among 100k fixed motors
         chains = variant(v0=0.0)
         rng = np.random.default_rng(5)
@@ -351,7 +391,9 @@ def main_features(args):
                              has_wca=False, has_propel=False, skin=0.3, motor=True,
                              motor_max_dist=0.25, motor_imparts=1.0, k_on=0.9, k_off=0.2,
                              seed=9)
-        return time_pool([chains, mot], steps, warm, prec, group=({(0, 1): "motor"}, {1}))
+        return [chains, mot], {(0, 1): "motor"}, {1}
+
+    groups = {"two populations": two_pops_set, "chains + 100k fixed motors": motors_set}
 
     def growth():
         # the growth schedule (control.py:153-219) for 2000 chains of 20 from points
@@ -371,13 +413,13 @@ def main_features(args):
         return time_pool([w], steps, warm, prec, chunks=chunks)
 
     options = {name: (lambda mk=mk: single_run(mk)) for name, (mk, _) in single.items()}
-    options.update({
-        "two populations": two_pops,
-        "chains + 100k fixed motors": motors,
-        "growth, 2000 chains x 20 (FP64)": growth,
-    })
-    no_cpu = ("not timed: the oracle restates the populations' scans by brute force over "
-              "all bead pairs, which is no baseline at 400k beads")
+    def group_run(mk):
+        reps, laws, fixed = mk()
+        return time_pool(reps, steps, warm, prec, group=(laws, fixed))
+
+    for name, mk in groups.items():
+        options[name] = (lambda mk=mk: group_run(mk))
+    options["growth, 2000 chains x 20 (FP64)"] = growth
     t0 = time.perf_counter()
     for name, run in options.items():
         if args.only not in name:
@@ -396,11 +438,18 @@ def main_features(args):
                                          sample=f"{c['steps']} steps of the same system "
                                                 f"({c['seconds']:.1f} s)")
                 r["gpu_over_cpu"] = r["bead_steps_per_s"] / c["bead_steps_per_s"]
-        elif name != "growth, 2000 chains x 20 (FP64)":
+        elif name in groups:
             gbs = 80 * r["bead_steps_per_s"] / 1e9
             r["step_roofline"] = dict(bound="hbm", bytes_per_bead_step=80, achieved=gbs,
                                       peak=8000.0, unit="GB/s", frac=gbs / 8000.0)
-            r["cpu_baseline"] = no_cpu
+            if not args.no_cpu:
+                reps, laws, fixed = groups[name]()
+                c = cpu_groups(reps, laws, fixed, args.cpu_seconds / 2)
+                r["cpu_baseline"] = dict(value=c["bead_steps_per_s"], unit="bead-steps/s",
+                                         cores=c["threads"], kind="port",
+                                         sample=f"{c['steps']} steps of the same populations "
+                                                f"({c['seconds']:.1f} s)")
+                r["gpu_over_cpu"] = r["bead_steps_per_s"] / c["bead_steps_per_s"]
         out[name] = r
     res = {"metric": "bead-steps/sec of each step option (1 GPU)", "unit": "bead-steps/s",
            "dtype": "f32 forces + f64 state", "data": "synthetic", "steps": steps,

@@ -630,6 +630,53 @@ void orc_rod_project(int dim, int n, double* rtn, const double* prev, const doub
     }
 }
 
+/* The Verlet list of one population with the reference's grid constants
+ * (config.grid_list_setup, config.py:147-193, and the in-loop window of
+ * SRK.py:99-103), chunked for n_threads threads, built at pos. */
+static void verlet_setup(verlet* vv, const orc_sys* s, const double* pos, const int64_t* chain_id,
+                         const double* rest_lengths, int n_threads, double grid_factor)
+{
+    int n = s->n, dim = s->dim;
+    double L = s->L;
+    double maxrest = 0.0;
+    if (s->has_springs) {
+        int64_t nc = chain_id[n - 1] + 1;
+        for (int64_t c = 0; c < nc; ++c)
+            if (rest_lengths[c] > maxrest) maxrest = rest_lengths[c];
+    }
+    double r_int = 2.0 * maxrest > s->wca[3] ? 2.0 * maxrest : s->wca[3];
+    int ng = (int)(L / (grid_factor * r_int));
+    if (ng < 1) ng = 1;
+    double h = L / ng;
+    int w = (int)ceil(grid_factor * s->wca[3] / h);
+    if (w < 1) w = 1;
+    vv->n = n;
+    vv->dim = dim;
+    vv->n_threads = n_threads;
+    vv->n_grid = ng;
+    vv->window = w;
+    vv->radius2 = (w * h) * (w * h);
+    vv->start = (int*)malloc(sizeof(int) * (n + 1));
+    vv->cap = (size_t)n * 64 + 64;
+    vv->list = (int*)malloc(sizeof(int) * vv->cap);
+    vv->chunk = (int*)malloc(sizeof(int) * (n_threads + 1));
+    int ncell = ng * ng * (dim == 3 ? ng : 1);
+    vv->cell_head = (int*)malloc(sizeof(int) * ncell);
+    vv->cell_next = (int*)malloc(sizeof(int) * n);
+    chunk_chain(n_threads, n, chain_id, vv->chunk);
+    verlet_build(vv, dim, n, pos, chain_id, L, s->n_nearby);
+}
+
+static void verlet_free(verlet* vv)
+{
+    free(vv->start);
+    free(vv->list);
+    free(vv->chunk);
+    free(vv->cell_head);
+    free(vv->cell_next);
+    memset(vv, 0, sizeof *vv);
+}
+
 int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangent, double* F,
                      const int64_t* chain_id, const double* rest_lengths, const double* vtan,
                      const double tv[8], double status_out[4], const double* xi_in,
@@ -657,37 +704,8 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
 
     verlet vv;
     memset(&vv, 0, sizeof vv);
-    if (pair_mode == 1 && s->has_wca) {
-        /* grid constants as config.grid_list_setup (config.py:147-193) and
-         * the in-loop window of SRK.py:99-103 */
-        double maxrest = 0.0;
-        if (s->has_springs) {
-            int64_t nc = chain_id[n - 1] + 1;
-            for (int64_t c = 0; c < nc; ++c)
-                if (rest_lengths[c] > maxrest) maxrest = rest_lengths[c];
-        }
-        double r_int = 2.0 * maxrest > s->wca[3] ? 2.0 * maxrest : s->wca[3];
-        int ng = (int)(L / (grid_factor * r_int));
-        if (ng < 1) ng = 1;
-        double h = L / ng;
-        int w = (int)ceil(grid_factor * s->wca[3] / h);
-        if (w < 1) w = 1;
-        vv.n = n;
-        vv.dim = dim;
-        vv.n_threads = n_threads;
-        vv.n_grid = ng;
-        vv.window = w;
-        vv.radius2 = (w * h) * (w * h);
-        vv.start = (int*)malloc(sizeof(int) * (n + 1));
-        vv.cap = (size_t)n * 64 + 64;
-        vv.list = (int*)malloc(sizeof(int) * vv.cap);
-        vv.chunk = (int*)malloc(sizeof(int) * (n_threads + 1));
-        int ncell = ng * ng * (dim == 3 ? ng : 1);
-        vv.cell_head = (int*)malloc(sizeof(int) * ncell);
-        vv.cell_next = (int*)malloc(sizeof(int) * n);
-        chunk_chain(n_threads, n, chain_id, vv.chunk);
-        verlet_build(&vv, dim, n, pos, chain_id, L, s->n_nearby);
-    }
+    if (pair_mode == 1 && s->has_wca)
+        verlet_setup(&vv, s, pos, chain_id, rest_lengths, n_threads, grid_factor);
     void* vp = (pair_mode == 1 && s->has_wca) ? (void*)&vv : NULL;
 
     const int order = s->order == 4 ? 4 : 2;
@@ -844,11 +862,7 @@ int64_t orc_srk2_run(const orc_sys* s, double* pos, double* pos1, double* tangen
     free(xi_buf);
     free(Faux);
     if (vp) {
-        free(vv.start);
-        free(vv.list);
-        free(vv.chunk);
-        free(vv.cell_head);
-        free(vv.cell_next);
+        verlet_free(&vv);
     }
     return steps;
 }
@@ -913,11 +927,164 @@ void orc_motor_push(int dim, int nb, const double* posb, const double* tb, int n
         }
 }
 
+/* The beads of one population on a cell grid of edge >= reach (about four
+ * beads per cell when that is larger), each cell listing its beads in
+ * ascending index: the cell-list scans of the CPU
+ * baseline between populations (the reference scans nbor_list[other],
+ * updates.py:586-734, built the same way). */
+typedef struct cgrid {
+    int ng, dim, cap_cells, cap_beads;
+    double L, h;
+    int* head;
+    int* next;
+} cgrid;
+
+static void cgrid_build(cgrid* g, int dim, int n, const double* pos, double L, double reach)
+{
+    /* at least the reach, and about four beads per cell when that is more */
+    double edge = n > 0 ? pow(4.0 * pow(L, dim) / n, 1.0 / dim) : L;
+    if (reach > edge) edge = reach;
+    int ng = edge > 0 ? (int)(L / edge) : 1;
+    int lim = dim == 3 ? 256 : 4096;
+    if (ng < 1) ng = 1;
+    if (ng > lim) ng = lim;
+    int ncell = ng * ng * (dim == 3 ? ng : 1);
+    if (ncell > g->cap_cells) {
+        g->head = (int*)realloc(g->head, sizeof(int) * ncell);
+        g->cap_cells = ncell;
+    }
+    if (n > g->cap_beads) {
+        g->next = (int*)realloc(g->next, sizeof(int) * n);
+        g->cap_beads = n;
+    }
+    g->ng = ng;
+    g->dim = dim;
+    g->L = L;
+    g->h = L / ng;
+    for (int c = 0; c < ncell; ++c) g->head[c] = -1;
+    for (int b = n - 1; b >= 0; --b) {
+        int c = 0;
+        for (int i = dim - 1; i >= 0; --i) {
+            int gi = (int)floor(pos[b * dim + i] / g->h);
+            if (gi >= ng) gi = ng - 1;
+            if (gi < 0) gi = 0;
+            c = c * ng + gi;
+        }
+        g->next[b] = g->head[c];
+        g->head[c] = b;
+    }
+}
+
+/* The cells within one cell of x's (every cell when the grid has fewer
+ * than three per side), into cells[]; returns their number. */
+static int cgrid_near(const cgrid* g, const double* x, int* cells)
+{
+    int ng = g->ng, dim = g->dim, k = 0;
+    if (ng < 3) {
+        int ncell = ng * ng * (dim == 3 ? ng : 1);
+        for (int c = 0; c < ncell; ++c) cells[k++] = c;
+        return k;
+    }
+    int gc[3] = {0, 0, 0};
+    for (int i = 0; i < dim; ++i) {
+        int gi = (int)floor(x[i] / g->h);
+        if (gi >= ng) gi = ng - 1;
+        if (gi < 0) gi = 0;
+        gc[i] = gi;
+    }
+    for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int cx = (gc[0] + dx + ng) % ng, cy = (gc[1] + dy + ng) % ng;
+                int cz = dim == 3 ? (gc[2] + dz + ng) % ng : 0;
+                cells[k++] = (cz * ng + cy) * ng + cx;
+            }
+    return k;
+}
+
+/* orc_cross_pair_forces over the cells near each bead of A (grid of B with
+ * edge >= the law's cutoff), beads of A split over n_threads threads. */
+static void cross_pair_forces_cells(int dim, int nA, const double* posA, const cgrid* g,
+                                    const double* posB, double L, const double p[4], double* FA,
+                                    int n_threads)
+{
+    double half = L / 2.0, rc2 = p[3] * p[3];
+#pragma omp parallel for schedule(static) num_threads(n_threads)
+    for (int a = 0; a < nA; ++a) {
+        int cells[27];
+        int nc = g->ng < 3 ? 0 : cgrid_near(g, posA + a * dim, cells);
+        int all = g->ng < 3;
+        int ncell = g->ng * g->ng * (dim == 3 ? g->ng : 1);
+        for (int k = 0; k < (all ? ncell : nc); ++k)
+            for (int b = g->head[all ? k : cells[k]]; b >= 0; b = g->next[b]) {
+                double f[3];
+                if (pair_term(dim, posA + a * dim, posB + b * dim, L, half, p, rc2, f))
+                    for (int i = 0; i < dim; ++i) FA[a * dim + i] += f[i];
+            }
+    }
+}
+
+/* orc_motor_push over the motors in the cells near each bead (grid of the
+ * motors with edge >= their reach). Beads in ascending index as there; each
+ * motor then meets its beads in the same order, and a bead's pushes are
+ * equal terms, so the result is the same. */
+static void motor_push_cells(int dim, int nb, const double* posb, const double* tb,
+                             const cgrid* g, const double* posm, double L, const orc_pop* M,
+                             int64_t step, int stage, double* F, int32_t* wab)
+{
+    double half = L / 2.0, reach2 = M->motor_reach * M->motor_reach;
+    int ncell = g->ng * g->ng * (dim == 3 ? g->ng : 1);
+    for (int b = 0; b < nb; ++b) {
+        int cells[27];
+        int all = g->ng < 3;
+        int nc = all ? ncell : cgrid_near(g, posb + b * dim, cells);
+        for (int k = 0; k < nc; ++k)
+            for (int m = g->head[all ? k : cells[k]]; m >= 0; m = g->next[m]) {
+                double d2 = 0.0;
+                for (int i = 0; i < dim; ++i) {
+                    double sep = posm[m * dim + i] - posb[b * dim + i];
+                    if (sep < -half)
+                        sep += L;
+                    else if (sep >= half)
+                        sep -= L;
+                    d2 += sep * sep;
+                }
+                if (d2 > reach2) continue;
+                int32_t* mb = &M->motor_bead[m];
+                if (*mb >= 0) {
+                    if (orc_motor_uniform(M->seed, (uint32_t)m, (uint32_t)b, step, stage, 0) > M->k_off)
+                        continue;
+                    *mb = -1;
+                    if (wab) wab[b] -= 1;
+                }
+                if (*mb < 0 && orc_motor_uniform(M->seed, (uint32_t)m, (uint32_t)b, step, stage, 1) < M->k_on &&
+                    d2 <= M->motor_max_dist_sq) {
+                    if (wab) wab[b] += 1;
+                    *mb = b;
+                    for (int i = 0; i < dim; ++i) F[b * dim + i] += M->motor_imparts * tb[b * dim + i];
+                }
+            }
+    }
+}
+
 int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const double tv[8],
-                           double status_out[4], int64_t step0)
+                           double status_out[4], int64_t step0, int pair_mode, int n_threads,
+                           double grid_factor)
 {
     double dt = tv[0], dtmax = tv[1], mdps = tv[2], cheat = tv[3], mdu = tv[4];
-    double max_steps = tv[7];
+    double until = tv[5], n_upd = tv[6], max_steps = tv[7];
+    if (n_threads < 1) n_threads = 1;
+    /* pair_mode 1 (the CPU baseline): each moving population's own pairs
+     * over its Verlet list, rebuilt on the reference's cadence; the other
+     * populations through a cell grid of their beads */
+    verlet vv[16];
+    memset(vv, 0, sizeof vv);
+    cgrid cg;
+    memset(&cg, 0, sizeof cg);
+    for (int q = 0; q < npop && q < 16; ++q)
+        if (pair_mode == 1 && !pops[q].fixed && pops[q].sys.has_wca)
+            verlet_setup(&vv[q], &pops[q].sys, pops[q].pos, pops[q].chain_id,
+                         pops[q].rest_lengths, n_threads, grid_factor);
     int dim = pops[0].sys.dim;
     double L = pops[0].sys.L;
     int nd = 2 * dim;
@@ -955,19 +1122,30 @@ int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const d
                 int n = P->sys.n;
                 const double* at = stage == 0 ? P->pos : P->pos1;
                 if (stage == 0) memset(P->forces, 0, sizeof(double) * (size_t)n * dim);
+                void* vq = (q < 16 && vv[q].start) ? (void*)&vv[q] : NULL;
                 orc_force_eval(&P->sys, at, P->chain_id, P->rest_lengths, P->vtan, P->tangent,
-                               P->forces, 0, NULL, P->pos);
+                               P->forces, vq ? 1 : 0, vq, P->pos);
                 for (int ii = 1; ii < npop; ++ii) {
                     int o = (q + ii) % npop;
                     const double* c = cross + ((size_t)q * npop + o) * 5;
                     if (c[0] == 0.0) continue;
                     const orc_pop* Q = &pops[o];
                     const double* qpos = (stage == 0 || Q->fixed) ? Q->pos : Q->pos1;
-                    if (c[0] == 2.0)
+                    if (pair_mode == 1) {
+                        cgrid_build(&cg, dim, Q->sys.n, qpos, L,
+                                    c[0] == 2.0 ? Q->motor_reach : c[4]);
+                        if (c[0] == 2.0)
+                            motor_push_cells(dim, n, at, P->tangent, &cg, qpos, L, Q,
+                                             step0 + steps, stage, P->forces, P->sys.where_active);
+                        else
+                            cross_pair_forces_cells(dim, n, at, &cg, qpos, L, c + 1, P->forces,
+                                                    n_threads);
+                    } else if (c[0] == 2.0) {
                         orc_motor_push(dim, n, at, P->tangent, Q->sys.n, qpos, L, Q,
                                        step0 + steps, stage, P->forces, P->sys.where_active);
-                    else
+                    } else {
                         orc_cross_pair_forces(dim, n, at, Q->sys.n, qpos, L, c + 1, P->forces);
+                    }
                 }
             }
             if (stage == 1) {
@@ -1032,6 +1210,13 @@ int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const d
                 for (int i = 0; i < dim; ++i) t[i] /= mag;
             }
         }
+        if (pair_mode == 1 && (double)steps >= until) {  /* the lists' cadence */
+            for (int q = 0; q < npop && q < 16; ++q)
+                if (vv[q].start)
+                    verlet_build(&vv[q], dim, pops[q].sys.n, pops[q].pos, pops[q].chain_id, L,
+                                 pops[q].sys.n_nearby);
+            until += n_upd;
+        }
     }
     status_out[0] = dt;
     status_out[1] = elapsed;
@@ -1039,5 +1224,9 @@ int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const d
     status_out[3] = m;
     for (int q = 0; q < npop; ++q) free(xi_buf[q]);
     free(xi_buf);
+    for (int q = 0; q < npop && q < 16; ++q)
+        if (vv[q].start) verlet_free(&vv[q]);
+    free(cg.head);
+    free(cg.next);
     return steps;
 }

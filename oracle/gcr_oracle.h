@@ -184,9 +184,14 @@ double orc_motor_uniform(uint64_t seed, uint32_t motor, uint32_t bead, int64_t s
  * (interpop_forces, updates.py:274-312), Q's positions being pos1 in the
  * second stage unless Q is fixed; m is the largest sqrt(2 D_P) + 1/2 max|F_P|
  * / r_P over the moving populations, one dt for all; each population updates
- * with its own D, radius and noise. Returns steps done. */
+ * with its own D, radius and noise. pair_mode 0: every pair of beads;
+ * pair_mode 1 (the CPU baseline): each population's own pairs over its
+ * Verlet list (grid_factor, rebuilt on time_vals' cadence) on n_threads
+ * threads, the other populations through a cell grid of their beads.
+ * Returns steps done. */
 int64_t orc_multi_srk2_run(int npop, orc_pop* pops, const double* cross, const double time_vals[8],
-                           double status_out[4], int64_t step0);
+                           double status_out[4], int64_t step0, int pair_mode, int n_threads,
+                           double grid_factor);
 
 /* forces.py:248-283 in one stage: for every bead b (index order) and every
  * motor m within motor_reach of it (index order): a bound motor stays bound

@@ -98,7 +98,7 @@ def lib():
         L.orc_cross_pair_forces.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                             C.c_double, C.c_void_p, C.c_void_p]
         L.orc_multi_srk2_run.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
-                                         C.c_void_p, C.c_int64]
+                                         C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_double]
         L.orc_multi_srk2_run.restype = C.c_int64
         L.orc_motor_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int,
                                         C.c_int]
@@ -230,13 +230,15 @@ def cross_pair_forces(posA, posB, L, p):
     return F
 
 
-def multi_srk2_run(pops, cross, time_vals, step0=0):
+def multi_srk2_run(pops, cross, time_vals, step0=0, pair_mode=0, n_threads=1, grid_factor=2.0):
     """orc_multi_srk2_run. pops: list of dicts with sysd, pos, tangent,
     chain_id, rest_lengths, vtan, and optionally xi, seed, fixed, motor (a
     dict max_dist, imparts, k_on, k_off, reach, still; the per-motor bound
     bead is returned in d["motor_bead"]); cross: {(P, Q): wca 4-vector, or
-    "motor" for the pushes of motor population Q on P}. Returns a list of
-    (pos, pos1, tangent, forces) per population and the status."""
+    "motor" for the pushes of motor population Q on P}. pair_mode 1 scans
+    each population's own pairs over Verlet lists and the other populations
+    through cell grids, on n_threads threads (the CPU baseline). Returns a
+    list of (pos, pos1, tangent, forces) per population and the status."""
     n = len(pops)
     assert 1 <= n <= 16
     arr = (OrcPop * n)()
@@ -282,7 +284,8 @@ def multi_srk2_run(pops, cross, time_vals, step0=0):
         cr[a, b, 1:] = p
     st = np.zeros(4)
     tv = np.ascontiguousarray(time_vals, dtype=np.float64)
-    lib().orc_multi_srk2_run(n, arr, cr.ctypes.data, tv.ctypes.data, st.ctypes.data, int(step0))
+    lib().orc_multi_srk2_run(n, arr, cr.ctypes.data, tv.ctypes.data, st.ctypes.data, int(step0),
+                             int(pair_mode), int(n_threads), float(grid_factor))
     return out, st
 
 

@@ -738,6 +738,59 @@ def test_sliding_over_verlet_list_equals_all_pairs():
     assert np.max(np.abs(N.min_image(o0[0] - o1[0], p.L))) < 1e-11 * p.L
 
 
+def test_multi_population_cell_lists_equal_all_pairs():
+    """orc_multi_srk2_run with pair_mode 1 (each population's own pairs over
+    Verlet lists, the others through cell grids: the CPU baseline) against
+    pair_mode 0 (every pair): two chain populations acting on each other with
+    the LJ 2.5 sigma law, and fixed motors pushing chains. Positions to
+    rounding (the pair sums' order differs), where_active and every motor's
+    bound bead exactly."""
+    from gcr_amd import workloads as W
+    w = W.chains("two", 2, 60, 8, L=30.0, cutoff_factor=W.LJ_25_SIGMA, kappa=4.0, T=0.01,
+                 steps=50, seed=71)
+    p = w.phys
+    tv = w.time_vals.copy()
+    tv[7] = 50
+
+    def chains_pop(lo, hi, seed, wa=False):
+        sl = slice(lo * 8, hi * 8)
+        sysd = dict(dim=2, n=(hi - lo) * 8, n_nearby=p.n_nearby, has_springs=True,
+                    has_wca=True, has_propel=True, L=p.L, D=p.D, Dr=p.Dr, radius=p.radius,
+                    wca=np.asarray(p.wca), spring=np.asarray(p.spring), v0=p.v0)
+        if wa:
+            sysd["where_active"] = np.zeros(sysd["n"], dtype=np.int32)
+        return dict(sysd=sysd, pos=w.pos[sl], tangent=w.tangent[sl],
+                    chain_id=np.repeat(np.arange(hi - lo), 8), rest_lengths=w.rest_lengths[lo:hi],
+                    vtan=w.v_tan_angle[lo:hi], seed=seed)
+
+    law = np.asarray(p.wca)
+    runs = [O.multi_srk2_run([chains_pop(0, 30, 1), chains_pop(30, 60, 2)],
+                             {(0, 1): law, (1, 0): law}, tv, pair_mode=m, n_threads=4)
+            for m in (0, 1)]
+    for k in range(2):
+        assert np.max(np.abs(N.min_image(runs[0][0][k][0] - runs[1][0][k][0], p.L))) < 1e-10 * p.L
+    rng = np.random.default_rng(72)
+    nm = 400
+    motors_sysd = dict(dim=2, n=nm, n_nearby=-1, has_springs=False, has_wca=False,
+                       has_propel=False, L=p.L, D=0.0, Dr=0.0, radius=1.0,
+                       wca=O.wca_params(1.0, 1.0, 0.0, 1.0), spring=np.zeros(6), v0=0.0)
+    got = []
+    for m in (0, 1):
+        ch = chains_pop(0, 60, 3, wa=True)
+        mo = dict(sysd=motors_sysd, pos=rng.uniform(0, p.L, (nm, 2)) if m == 0 else got[0][3],
+                  tangent=np.tile([1.0, 0.0], (nm, 1)), chain_id=np.arange(nm),
+                  rest_lengths=np.ones(nm), vtan=np.zeros(nm), fixed=True, seed=9,
+                  motor=dict(max_dist=0.5, imparts=1.0, k_on=0.9, k_off=0.2))
+        out, _ = O.multi_srk2_run([ch, mo], {(0, 1): "motor"}, tv, pair_mode=m, n_threads=4)
+        got.append((out[0][0], ch["sysd"]["where_active"].copy(), mo["motor_bead"].copy(),
+                    mo["pos"]))
+    assert np.max(np.abs(N.min_image(got[0][0] - got[1][0], p.L))) < 1e-10 * p.L
+    # fixed motors keep their bindings: a bead's count this step is -1 plus
+    # the bindings made minus those released
+    assert np.array_equal(got[0][1], got[1][1]) and np.sum(got[0][1] != -1) > 5
+    assert np.array_equal(got[0][2], got[1][2]) and np.sum(got[0][2] >= 0) > 50
+
+
 def motor_pops(bead_pos, bead_t, motor_pos, k_on, k_off, D_m=0.0, still=True, imparts=1.0,
                max_dist=0.5, L=L2D):
     nb, nm = len(bead_pos), len(motor_pos)
