@@ -52,7 +52,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline budget: steps are added in chunks of 40 until this much time")
     ap.add_argument("--kernel-steps", type=int, default=32)
-    ap.add_argument("--mode", choices=("single", "ensemble", "serve", "copy", "features"),
+    ap.add_argument("--mode", choices=("single", "ensemble", "serve", "copy", "features",
+                                       "observables"),
                     default="single",
                     help="single: 400k-bead system (the headline line); ensemble: 256 "
                          "replicas batched vs one at a time; serve: request stream; copy: "
@@ -464,6 +465,72 @@ def main_features(args):
     print(json.dumps(res), flush=True)
 
 
+def main_observables(args):
+    """The observables after the step on the 400k-bead system: polar and
+    nematic order with the director (analysis.py:294-328 over the per-chain
+    mean angles, output_methods.py:492-510) on the device, against the host
+    formulas on host arrays (the reference's arrays live on the host); and
+    one .dat snapshot (output_methods.py:420-485), download included."""
+    import tempfile
+    from gcr_amd import engine as E
+    from gcr_amd import observables as OB
+    from gcr_amd import snapshot as SN
+    from gcr_amd import workloads as W
+    w = W.chains_20kx20(seed=1, n_chains=args.chains)
+    w.phys.skin = 0.3 * float(w.phys.wca[3])
+    pool = E.Pool(2, E.GCR_MIXED, bead_capacity=w.n, cell_capacity=E.replica_cells(w.phys) + 1,
+                  chain_capacity=w.phys.n_chains, replica_capacity=1)
+    try:
+        rid = pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+        tv = w.time_vals.copy()
+        tv[7] = 64
+        pool.begin(rid, tv)
+        pool.run()
+        calls = 200
+        for _ in range(10):
+            dev = pool.order_parameters(rid)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            dev = pool.order_parameters(rid)
+        dev_s = (time.perf_counter() - t0) / calls
+        t0 = time.perf_counter()
+        out = pool.download(rid, pos1=False, forces=False)
+        dl_s = time.perf_counter() - t0
+    finally:
+        pool.close()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        phi = OB.chain_mean_angle(out["tangent"], w.chain_id, w.phys.n_chains)
+        host = (OB.polar_order(phi),) + OB.nematic_order(phi)
+    host_s = (time.perf_counter() - t0) / reps
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        fn = SN.save_state(os.path.join(d, "run"), "chains", 1, 64, 0.1, out["pos"],
+                           out["tangent"], w.chain_id)
+        snap_s = time.perf_counter() - t0
+        size = os.path.getsize(fn)
+    # the order parameters read one tangent per bead (16 B) and write a
+    # few numbers: algorithmic bytes 16 per bead
+    gbs = 16 * w.n / dev_s / 1e9
+    res = {"metric": "observables of the 400k-bead system (1 GPU)", "unit": "s per call",
+           "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": "20k chains x 20 beads (400k) after 64 mixed steps"},
+           "order_parameters": {
+               "device_s": dev_s, "host_numpy_s": host_s, "device_over_host": host_s / dev_s,
+               "polar": dev["polar"], "nematic": dev["nematic"],
+               "host_polar": host[0], "host_nematic": host[1],
+               "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                            "frac": gbs / 8000.0,
+                            "note": "whole call, launch and host synchronisation included"},
+               "cpu_baseline": {"value": host_s, "unit": "s per call", "cores": 1,
+                                "kind": "port", "sample": f"{reps} calls of numpy on host "
+                                                         "arrays"}},
+           "snapshot": {"download_s": dl_s, "write_s": snap_s, "bytes": size,
+                        "write_MB_per_s": size / snap_s / 1e6}}
+    print(json.dumps(res), flush=True)
+
+
 def main_copy():
     """dst = 1 * src over 1 GiB of float64, three times: one elementwise
     kernel that reads 1 GiB and writes 1 GiB per launch. Counter readings of
@@ -492,6 +559,8 @@ def main():
         return main_serve(args)
     if args.mode == "features":
         return main_features(args)
+    if args.mode == "observables":
+        return main_observables(args)
     from gcr_amd import engine as E
     from gcr_amd import workloads as W
     from gcr_amd._lib import device_info

@@ -63,7 +63,7 @@ EXPORTS = [
     "gcr_replica_download", "gcr_replica_eval_forces", "gcr_pool_time_kernels",
     "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_threads", "gcr_init_chains",
     "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group", "gcr_replica_set_chunks",
-    "gcr_replica_where_active",
+    "gcr_replica_where_active", "gcr_write_table",
 ]
 
 _lib = None
@@ -110,6 +110,7 @@ def load():
     L.gcr_pool_group.argtypes = [vp, vp, i32, vp, vp]
     L.gcr_replica_set_chunks.argtypes = [vp, i32, vp]
     L.gcr_replica_where_active.argtypes = [vp, i32, vp]
+    L.gcr_write_table.argtypes = [C.c_char_p, C.c_char_p, i64, i32, vp, vp, i32]
     L.gcr_create.argtypes = [C.POINTER(ReplicaDesc), i32, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]
     L.gcr_upload.argtypes = [vp, vp, vp]
     L.gcr_run.argtypes = [vp, vp, vp, vp, i64]

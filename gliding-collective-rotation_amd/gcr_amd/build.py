@@ -29,11 +29,13 @@ FLAGS = [
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-command-line-argument",
+    "-pthread",
 ]
 
 
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, "gcr_capi.hip"), os.path.join(CSRC, "gcr_init.cpp")]
+    srcs = [os.path.join(CSRC, "gcr_capi.hip"), os.path.join(CSRC, "gcr_init.cpp"),
+            os.path.join(CSRC, "gcr_snapshot.cpp")]
     cmd = [HIPCC, *FLAGS, "-I", CSRC, "-I", INCLUDE, *srcs, "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -224,6 +224,15 @@ int gcr_init_chains(int32_t dim, double L, int64_t n_chains, const int32_t* chai
                     double min_dist, double min_dist_floor, uint64_t seed, int64_t max_tries,
                     double* pos, double* tangent, double* min_dist_used);
 
+/* The text table of a .dat snapshot, as the numpy.savetxt call of the
+ * reference's writer (output_methods.py:420-485) leaves it: the header's
+ * lines prefixed "# ", then one line per row, columns joined by a space;
+ * column k (cols[k * n_rows + r]) printed "%d" when is_int[k] (its values
+ * integral), else "%.8e". Rows are formatted on n_threads host threads.
+ * GCR_ERR_ARG for bad arguments or a file that cannot be written. */
+int gcr_write_table(const char* path, const char* header, int64_t n_rows, int32_t n_cols,
+                    const double* cols, const int32_t* is_int, int32_t n_threads);
+
 #ifdef __cplusplus
 }
 #endif
