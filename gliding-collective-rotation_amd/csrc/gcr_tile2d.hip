@@ -388,7 +388,10 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
         load_vec<2>(Xa, s, xs);
         const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
         ChainNb<2> cn;
-        if (springs) load_chain_nb<2>(Xa, a.LINK, s, cn);  // in flight during the pair loop
+        // the lean variant keeps the bonded neighbours' loads in flight
+        // during the pair loop; the general one loads them after it, which
+        // holds it at 94 VGPRs (five waves per SIMD instead of four)
+        if (springs && !GEN) load_chain_nb<2>(Xa, a.LINK, s, cn);
         double pa0 = 0.0, pa1 = 0.0;
         if (P.flags & GCR_F_WCA) {
             const int len = a.NLEN[s];
@@ -412,6 +415,7 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
         }
         // bonded terms first, then the pair sum, then propulsion (SRK.py:228-253)
         bool tang_set = false;
+        if (springs && GEN) load_chain_nb<2>(Xa, a.LINK, s, cn);
         if (springs) bonded<2, FT, GEN>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
         acc[0] += pa0;
         acc[1] += pa1;
