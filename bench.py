@@ -226,6 +226,41 @@ def cpu_option(w, budget_s):
     return dict(bead_steps_per_s=w.n * done / sec, steps=done, seconds=sec, threads=threads)
 
 
+def cpu_growth(final, pos, tangent, chain_id, chain_len, rest_length, time_vals, t_init_grow,
+               stepskip, budget_s):
+    """The growth schedule (control.py:153-219) driven through the C/OpenMP
+    FP64 oracle chunk by chunk (Verlet lists, rebuilt every chunk) until the
+    schedule ends or about budget_s seconds of CPU work."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_lib as O
+    from gcr_amd import grow as G
+    threads = min(16, os.cpu_count() or 1)
+    tv = np.array(time_vals, dtype=np.float64).copy()
+    tv[3], tv[4] = 1.0, -1.0
+    tv[5] = tv[6] = tv[7] = stepskip
+    vt = np.zeros(final.n_chains)
+    sched = G.growth_schedule(final, chain_len, rest_length, t_init_grow)
+    done, sec, step = 0, 0.0, next(sched, None)
+    while step is not None and sec < budget_s:
+        ph, rest = step
+        sysd = dict(dim=ph.dim, n=ph.n_beads, n_nearby=ph.n_nearby, has_springs=ph.has_springs,
+                    has_wca=ph.has_wca, has_propel=ph.has_propel, L=ph.L, D=ph.D, Dr=ph.Dr,
+                    radius=ph.radius, wca=np.asarray(ph.wca), spring=np.asarray(ph.spring),
+                    v0=ph.v0, rod=ph.rod)
+        t0 = time.perf_counter()
+        pos, _, tangent, _, st = O.srk2_run(sysd, pos, tangent, chain_id, rest, vt, tv, seed=1,
+                                            step0=done, pair_mode=1, n_threads=threads)
+        sec += time.perf_counter() - t0
+        tv[0] = st[0]
+        done += int(st[2])
+        try:
+            step = sched.send(st[1])
+        except StopIteration:
+            step = None
+    return dict(bead_steps_per_s=final.n_beads * done / sec, steps=done, seconds=sec,
+                threads=threads)
+
+
 def cpu_groups(reps, laws, fixed, budget_s):
     """The C/OpenMP FP64 oracle's multi-population SRK2 loop on the
     populations `reps` (Workloads) with the laws {(P, Q): WCA vector or
@@ -396,10 +431,14 @@ def main_features(args):
 
     groups = {"two populations": two_pops_set, "chains + 100k fixed motors": motors_set}
 
-    def growth():
+    def growth_set():
         # the growth schedule (control.py:153-219) for 2000 chains of 20 from points
         g = W.chains_20kx20(seed=2, n_chains=2000)
         pos0, t0_, cid = G.random_initialization(2000, 20, 0.5, g.phys.L, 2, seed=3)
+        return g, pos0, t0_, cid
+
+    def growth():
+        g, pos0, t0_, cid = growth_set()
         tg = time.perf_counter()
         _, _, info = G.grow(g.phys, pos0, t0_, cid, 0.5, 20, g.time_vals, t_init_grow=0.2,
                             growing_stepskip=10)
@@ -451,10 +490,15 @@ def main_features(args):
                                          sample=f"{c['steps']} steps of the same populations "
                                                 f"({c['seconds']:.1f} s)")
                 r["gpu_over_cpu"] = r["bead_steps_per_s"] / c["bead_steps_per_s"]
-        else:
-            r["cpu_baseline"] = ("not timed: the first chunks grow beads from near-zero size, "
-                                 "for which the oracle's Verlet grid rule (cells of twice the "
-                                 "interaction range) has no useful size")
+        elif not args.no_cpu:
+            g, pos0, t0_, cid = growth_set()
+            c = cpu_growth(g.phys, pos0, t0_, cid, 20, 0.5, g.time_vals, 0.2, 10,
+                           args.cpu_seconds / 2)
+            r["cpu_baseline"] = dict(value=c["bead_steps_per_s"], unit="bead-steps/s",
+                                     cores=c["threads"], kind="port",
+                                     sample=f"the first {c['steps']} steps of the same schedule "
+                                            f"({c['seconds']:.1f} s)")
+            r["gpu_over_cpu"] = r["bead_steps_per_s"] / c["bead_steps_per_s"]
         out[name] = r
     res = {"metric": "bead-steps/sec of each step option (1 GPU)", "unit": "bead-steps/s",
            "dtype": "f32 forces + f64 state", "data": "synthetic", "steps": steps,

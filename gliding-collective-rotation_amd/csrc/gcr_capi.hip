@@ -27,6 +27,9 @@ thread_local std::string g_err;
 // RepState::frozen values: 0 running, 1 held by the host (replay or
 // diagnostics), 2 marked for restore, FROZEN_FAILED stopped for good
 constexpr int FROZEN_FAILED = 3;
+// after a replay, the replica's next batches start in the safe form (while
+// every running replica of the pool is in that state)
+constexpr int SAFE_STICKY = 8;
 
 int fail(int code, const std::string& msg)
 {
@@ -115,6 +118,7 @@ struct gcr_pool {
     std::vector<RepState> S;
     std::vector<int> live;
     std::vector<int> replays;
+    std::vector<int> safe_left;  // per replica: batches still started in the safe form
     std::vector<double*> xi_dev;
     int B = 32, K = 8;
     bool tables_dirty = true;
@@ -770,6 +774,7 @@ int replay(gcr_pool* p)
     if ((rc = pull_states(p))) return rc;
     for (int r : which) {
         p->replays[r] += 1;
+        p->safe_left[r] = SAFE_STICKY;
         // the safe batch re-sorts wherever a step left the skin, so this is
         // not expected; if it happens the pair set may have been incomplete,
         // and the replica stops here, marked failed, while the others go on
@@ -882,6 +887,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     for (auto& s : p->S) s.frozen = 1;
     p->live.assign(capR, 0);
     p->replays.assign(capR, 0);
+    p->safe_left.assign(capR, 0);
     p->xi_dev.assign(capR, nullptr);
     p->capBlk = capB / BS + capR + 1;
     p->capChk = capC / SCAN_CHUNK + capR + 1;
@@ -1135,6 +1141,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     S.frozen = 0;
     p->live[r] = 1;
     p->replays[r] = 0;
+    p->safe_left[r] = 0;
     HIPCHK(hipStreamSynchronize(st));
     int rc = upload_rep(p, r);
     if (rc) return rc;
@@ -1395,6 +1402,32 @@ int gcr_pool_run(gcr_pool* p, int32_t max_batches, int32_t* n_running)
     if ((rc = pull_states(p))) return rc;
     for (int b = 0; max_batches <= 0 || b < max_batches; ++b) {
         if (count_running(p) == 0) break;
+        // every running replica had a batch replayed lately (a growth phase
+        // does, chunk after chunk): start in the safe form at once, which
+        // the normal batch would most likely need again. Its snapshot and
+        // per-step re-sorts are those of a replay; a violation there stops
+        // the replica as after a replay.
+        int nrun = 0, nsafe = 0;
+        for (int r = 0; r < p->capR; ++r)
+            if (p->live[r] && running(p->S[r])) {
+                ++nrun;
+                if (p->safe_left[r] > 0) ++nsafe;
+            }
+        if (nrun > 0 && nsafe == nrun) {
+            HIPCHK(hipGraphLaunch(p->g_safe, p->stream));
+            if ((rc = pull_states(p))) return rc;
+            bool failed = false;
+            for (int r = 0; r < p->capR; ++r) {
+                if (!p->live[r]) continue;
+                if (p->safe_left[r] > 0) --p->safe_left[r];
+                if (p->S[r].violation && !p->S[r].fixed) {
+                    p->S[r].frozen = FROZEN_FAILED;
+                    failed = true;
+                }
+            }
+            if (failed && (rc = push_states(p))) return rc;
+            continue;
+        }
         HIPCHK(hipGraphLaunch(p->g_norm, p->stream));
         if ((rc = pull_states(p))) return rc;
         bool viol = false;

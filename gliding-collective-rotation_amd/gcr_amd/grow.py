@@ -99,13 +99,28 @@ def growth_physics(final, progress, chain_len, rest_length, a_init_factor=1e-8, 
     return ph, rest
 
 
+def growth_schedule(final, chain_len, rest_length, t_init_grow=1.0, a_init_factor=1e-8, eps=1.0,
+                    lj_cutoff_factor=1.0, max_chunks=1_000_000):
+    """The chunks of the growth phase (control.py:153-219): yields the
+    (physics, rest lengths) of each chunk while t <= 0, from t = -t_init_grow;
+    the caller sends back the time each chunk advanced."""
+    t_start = -t_init_grow
+    t = t_start
+    chunks = 0
+    while t <= 0 and chunks < max_chunks:
+        progress = max((t - t_start) / t_init_grow, EPSILON)
+        elapsed = yield growth_physics(final, progress, chain_len, rest_length, a_init_factor,
+                                       eps, lj_cutoff_factor)
+        t += elapsed
+        chunks += 1
+
+
 def grow(final, pos, tangent, chain_id, rest_length, chain_len, time_vals, t_init_grow=1.0,
          growing_stepskip=10, a_init_factor=1e-8, precision=E.GCR_FP64, eps=1.0,
          lj_cutoff_factor=1.0, max_chunks=1_000_000):
     """Run the growth phase on the device from the chains of
     random_initialization. time_vals: the run's [dt0, dtmax, max_dist_per_step,
     ...] (RK_cheat_factor is 1 while growing). Returns (pos, tangent, info)."""
-    L = final.L
     n = final.n_beads
     # cell grid for the largest cutoff of the schedule (the bead size itself)
     ph0, rest0 = growth_physics(final, EPSILON, chain_len, rest_length, a_init_factor, eps,
@@ -116,17 +131,17 @@ def grow(final, pos, tangent, chain_id, rest_length, chain_len, time_vals, t_ini
                   chain_capacity=final.n_chains, replica_capacity=1)
     pool.set_schedule(growing_stepskip, growing_stepskip)
     rid = pool.add(ph0, pos, tangent, chain_id, rest0, np.zeros(final.n_chains))
-    t_start = -t_init_grow
-    t = t_start
+    t = -t_init_grow
     tv = np.array(time_vals, dtype=np.float64).copy()
     tv[3] = 1.0    # RK_cheat_factor while growing
     tv[4] = -1.0   # no save-time clipping
     chunks = steps = 0
+    sched = growth_schedule(final, chain_len, rest_length, t_init_grow, a_init_factor, eps,
+                            lj_cutoff_factor, max_chunks)
     try:
-        while t <= 0 and chunks < max_chunks:
-            progress = max((t - t_start) / t_init_grow, EPSILON)
-            ph, rest = growth_physics(final, progress, chain_len, rest_length, a_init_factor, eps,
-                                      lj_cutoff_factor)
+        step = next(sched, None)
+        while step is not None:
+            ph, rest = step
             pool.set_physics(rid, ph, rest_lengths=rest)
             tv[7] = growing_stepskip
             pool.begin(rid, tv)
@@ -138,6 +153,10 @@ def grow(final, pos, tangent, chain_id, rest_length, chain_len, time_vals, t_ini
             t += st["elapsed"]
             steps += st["steps"]
             chunks += 1
+            try:
+                step = sched.send(st["elapsed"])
+            except StopIteration:
+                step = None
         out = pool.download(rid, pos1=False, forces=False)
     finally:
         pool.close()

@@ -645,7 +645,12 @@ static void verlet_setup(verlet* vv, const orc_sys* s, const double* pos, const 
             if (rest_lengths[c] > maxrest) maxrest = rest_lengths[c];
     }
     double r_int = 2.0 * maxrest > s->wca[3] ? 2.0 * maxrest : s->wca[3];
-    int ng = (int)(L / (grid_factor * r_int));
+    double ngd = L / (grid_factor * r_int);
+    /* at most about four cells per bead: beads grown from points have a
+     * vanishing range (larger cells only lengthen the lists) */
+    double ng_cap = floor(pow(4.0 * (n > 0 ? n : 1), 1.0 / dim));
+    if (!(ngd <= ng_cap)) ngd = ng_cap;
+    int ng = (int)ngd;
     if (ng < 1) ng = 1;
     double h = L / ng;
     int w = (int)ceil(grid_factor * s->wca[3] / h);
