@@ -103,6 +103,7 @@ struct RepState {
     int32_t pred_far;     // safe batch: bit k, the virtual positions of SRK stage k
                           // left the skin
     int32_t fixed;        // a fixed population of a group: never moves (pop.fixed)
+    int32_t sort_due;     // safe batch: the last update left the skin, re-sort first
 };
 
 struct PoolArgs {

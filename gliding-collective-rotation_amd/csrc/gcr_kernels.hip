@@ -169,6 +169,9 @@ __device__ __forceinline__ bool rebuild_runs(const PoolArgs& a, int r, int j)
     if (S->frozen) return false;
     if (S->fixed) return j < 0 && !a.on_x1;  // a fixed population is sorted once
     if (a.on_x1) return step_runs(S, j) && ((S->pred_far >> a.sub) & 1);
+    // the safe batch re-sorts at its start and then wherever an update left
+    // the skin (its lists are otherwise still complete)
+    if (a.safe && j > 0) return step_runs(S, j) && S->sort_due;
     return j < 0 || step_runs(S, j);
 }
 
@@ -1087,6 +1090,7 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
 // (k_group_dt takes the group's max). One thread.
 __device__ __forceinline__ void finish_step(const RepParams& P, RepState* S, double m2)
 {
+    S->sort_due = 0;  // this step's re-sort, if due, has run
     double m = sqrt(2.0 * P.D);
     if (P.n > 0) m += force_prefactor(P) * sqrt(m2) / P.radius;
     if (P.flags & GCR_F_GROUP) {
@@ -1519,7 +1523,8 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
             x[i] = pymod(tmp, P.L);
         }
         store_vec<DIM>(a.X, s, x);
-        if (!a.safe && moved_too_far<DIM>(a, P, s, x)) atomicOr(&a.S[r].violation, 1);
+        // outside the safe batch a replay, inside it a re-sort before the next step
+        if (moved_too_far<DIM>(a, P, s, x)) atomicOr(a.safe ? &a.S[r].sort_due : &a.S[r].violation, 1);
     }
     // the step is over: the next mid-step re-sort waits for a new predictor
     if (threadIdx.x == 0 && blockIdx.x == P.blk_off) a.S[r].pred_far = 0;
@@ -2191,8 +2196,8 @@ __global__ void __launch_bounds__(256) k_rod(PoolArgs a, int j)
     if (__any(viol) && lane == 0) {
         if (STAGE == 0)
             flag_predictor(a, a.S + r, a.sub);
-        else if (!a.safe)
-            atomicOr(&a.S[r].violation, 1);
+        else
+            atomicOr(a.safe ? &a.S[r].sort_due : &a.S[r].violation, 1);
     }
 }
 
