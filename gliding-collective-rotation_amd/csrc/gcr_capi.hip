@@ -1288,10 +1288,14 @@ int gcr_replica_set_physics(gcr_pool* p, int32_t rid, const gcr_replica_desc* d,
     if (rest_lengths)
         HIPCHK(hipMemcpyAsync(p->CHAIN_A + P.chain_off, rest_lengths, 8 * (size_t)P.n_chains,
                               hipMemcpyHostToDevice, p->stream));
-    {  // speeds: per chain, or d->v0 for every chain (as gcr_replica_add)
+    if (!v0_per_chain) {  // d->v0 for every chain (as gcr_replica_add), set on the device
+        if (P.n_chains > 0)
+            k_set_speed<<<(P.n_chains + BS - 1) / BS, BS, 0, p->stream>>>(
+                p->CHAIN_CS + 3 * (size_t)P.chain_off, P.n_chains, d->v0);
+    } else {  // speeds per chain, beside the chains' angle terms
         std::vector<double> cs(3 * (size_t)P.n_chains);
         HIPCHK(hipMemcpy(cs.data(), p->CHAIN_CS + 3 * (size_t)P.chain_off, 8 * cs.size(), hipMemcpyDeviceToHost));
-        for (int c = 0; c < P.n_chains; ++c) cs[3 * c] = v0_per_chain ? v0_per_chain[c] : d->v0;
+        for (int c = 0; c < P.n_chains; ++c) cs[3 * c] = v0_per_chain[c];
         HIPCHK(hipMemcpyAsync(p->CHAIN_CS + 3 * (size_t)P.chain_off, cs.data(), 8 * cs.size(),
                               hipMemcpyHostToDevice, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));

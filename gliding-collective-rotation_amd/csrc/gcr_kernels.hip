@@ -1591,6 +1591,13 @@ __global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
 
 // ----------------------------------------------------------------- rebuild
 
+// Every chain's speed (the first of its three CHAIN_CS entries) set to v0.
+__global__ void __launch_bounds__(BS) k_set_speed(double* cs, int n_chains, double v0)
+{
+    const int c = blockIdx.x * BS + threadIdx.x;
+    if (c < n_chains) cs[3 * c] = v0;
+}
+
 __global__ void __launch_bounds__(BS) k_fill_i32(int32_t* p, int n, int32_t v)
 {
     int i = blockIdx.x * BS + threadIdx.x;
