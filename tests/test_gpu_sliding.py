@@ -77,6 +77,16 @@ def test_sliding_chains_2d():
     assert np.sum(wa > 0) > 20  # the pushes happen
 
 
+def test_sliding_within_the_pair_cutoff():
+    """LJ 2.5 sigma pairs: the sliding range lies within the cutoff, so the
+    2D partners come from the neighbour lists (the other cases scan cells)."""
+    w = sliding(W.chains("slide_lj", 2, 150, 8, phi=0.35, cutoff_factor=W.LJ_25_SIGMA,
+                         kappa=6.0, T=0.01, steps=100, seed=35))
+    print(f"[sliding] LJ chains n={w.n}", flush=True)
+    wa, _ = compare(w)
+    assert np.sum(wa > 0) > 20
+
+
 def test_sliding_abps_one_chunk():
     w = sliding(W.chains("slide_abp", 2, 500, 1, phi=0.4, T=0.01, Dr=0.2, steps=100, seed=31),
                 maxdist=1.3, threads=1)
