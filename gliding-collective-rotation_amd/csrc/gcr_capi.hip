@@ -37,6 +37,16 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
+}  // namespace
+
+// the message gcr_last_error returns, for the library's host-only sources
+__attribute__((visibility("hidden"))) int gcr_internal_fail(int code, const char* msg)
+{
+    return fail(code, msg ? msg : "");
+}
+
+namespace {
+
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
         hipError_t e_ = (x);                                                               \

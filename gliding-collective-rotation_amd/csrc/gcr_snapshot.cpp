@@ -46,10 +46,14 @@ void format_rows(std::string& out, int64_t r0, int64_t r1, int64_t n_rows, int32
 
 }  // namespace
 
+// gcr_capi.hip: sets the message gcr_last_error returns
+int gcr_internal_fail(int code, const char* msg);
+
 extern "C" int gcr_write_table(const char* path, const char* header, int64_t n_rows, int32_t n_cols,
                                const double* cols, const int32_t* is_int, int32_t n_threads)
 {
-    if (!path || n_rows < 0 || n_cols < 1 || (n_rows > 0 && (!cols || !is_int))) return GCR_ERR_ARG;
+    if (!path || n_rows < 0 || n_cols < 1 || (n_rows > 0 && (!cols || !is_int)))
+        return gcr_internal_fail(GCR_ERR_ARG, "gcr_write_table: bad arguments");
     int T = n_threads > 0 ? n_threads : 1;
     if ((int64_t)T > n_rows / 4096 + 1) T = (int)(n_rows / 4096 + 1);
     std::vector<std::string> parts((size_t)T);
@@ -61,7 +65,7 @@ extern "C" int gcr_write_table(const char* path, const char* header, int64_t n_r
     }
     for (auto& th : pool) th.join();
     FILE* f = std::fopen(path, "wb");
-    if (!f) return GCR_ERR_ARG;
+    if (!f) return gcr_internal_fail(GCR_ERR_ARG, (std::string("cannot open ") + path).c_str());
     bool ok = true;
     if (header && *header) {  // each header line behind "# "
         std::string h = "# ";
@@ -75,5 +79,5 @@ extern "C" int gcr_write_table(const char* path, const char* header, int64_t n_r
     for (const auto& p : parts)
         if (ok && !p.empty()) ok = std::fwrite(p.data(), 1, p.size(), f) == p.size();
     ok = (std::fclose(f) == 0) && ok;
-    return ok ? GCR_OK : GCR_ERR_ARG;
+    return ok ? GCR_OK : gcr_internal_fail(GCR_ERR_ARG, (std::string("cannot write ") + path).c_str());
 }

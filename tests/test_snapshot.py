@@ -72,3 +72,11 @@ def test_native_table_equals_numpy_savetxt(tmp_path):
                                                      len(fmt), table.ctypes.data,
                                                      is_int.ctypes.data, 7))
         assert out.read_bytes() == ref.read_bytes()
+
+
+def test_write_error_names_the_file(tmp_path):
+    import pytest
+    from gcr_amd import _lib
+    with pytest.raises(_lib.GcrError, match="cannot open .*missing"):
+        SN.save_state(str(tmp_path / "missing" / "run"), "beads", 1, 1, 0.1, np.zeros((3, 2)),
+                      np.ones((3, 2)), np.zeros(3, dtype=np.int64))
