@@ -349,8 +349,9 @@ def time_pool(reps, steps, warmup, precision, dim=2, group=None, chunks=None, ba
 def main_features(args):
     """Throughput of each option of the step on the 400k-bead system (mixed
     precision, the tiling the autotuner picks for the plain step): SRK4,
-    FENE springs, walls, the rigid rod, active sliding, two interacting
-    populations, chains with motors, and the growth phase on 40k beads."""
+    FENE springs, walls, the rigid rod, active sliding, 3D chains (100k
+    beads), two interacting populations, chains with motors, and the growth
+    phase on 40k beads."""
     import copy
     from gcr_amd import dropin
     from gcr_amd import engine as E
@@ -379,6 +380,12 @@ def main_features(args):
         w.chunks = dropin.chain_chunks(3, w.chain_id)  # three threads' chunks
         return w
 
+    def chains3d_w():
+        w = W.chains("chains3d", 3, 5000, 20, phi=0.15, cutoff_factor=W.LJ_25_SIGMA, kappa=5.0,
+                     T=0.05, dtmax=5e-4, seed=4)
+        w.phys.skin = 0.3 * float(w.phys.wca[3])
+        return w
+
     # single-population options: (workload, algorithmic HBM bytes per
     # bead-step: x read, x1 written, x1 read, x read, x written, 8 B per
     # coordinate; SRK4 moves positions through twice as many passes)
@@ -389,6 +396,9 @@ def main_features(args):
         "walls (x and y)": (lambda: W.walled(base, 3, 1.25), 80),
         "rigid rod 0.6": (lambda: variant(rod=0.6), 80),
         "active sliding": (sliding_w, 80),
+        # 3D: chains of 20 at volume fraction 0.15 with the same law (120 B
+        # per bead-step: three coordinates)
+        "3D chains, 5000 x 20": (chains3d_w, 120),
     }
 
     def two_pops_set():
@@ -450,7 +460,7 @@ def main_features(args):
     def single_run(mk):
         w = mk()
         chunks = {0: w.chunks} if getattr(w, "chunks", None) is not None else None
-        return time_pool([w], steps, warm, prec, chunks=chunks)
+        return time_pool([w], steps, warm, prec, dim=w.phys.dim, chunks=chunks)
 
     options = {name: (lambda mk=mk: single_run(mk)) for name, (mk, _) in single.items()}
     def group_run(mk):

@@ -18,7 +18,7 @@
 
 using namespace gcr;
 
-static_assert(NL3_MAX * 4 == NL_MAX * 2, "3D lists (int32 slots) share the storage of 2D lists (u16)");
+static_assert(NL3_MAX * 4 == 2 * NL_MAX * 2, "3D lists (int32 slots) take twice the storage of 2D lists (u16)");
 
 namespace {
 
@@ -859,7 +859,7 @@ int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_ce
     // per bead: 10 dim-vectors + noise (2 x 2 dim) of doubles, 5 int pairs, links,
     // cell, slot and two where_active ints; the neighbour list, its length and
     // the re-sort positions; per chain: 4 doubles + 3 ints; per cell: 2 ints
-    const int64_t nl = 2 * NL_MAX + 4 + 8 * dim;
+    const int64_t nl = (dim == 3 ? 4 * NL3_MAX : 2 * NL_MAX) + 4 + 8 * dim;
     return n * (8 * (10 * dim + 4 * dim) + 4 * (5 * 2 + 4 + 2 + 2) + nl) + n_chains * (8 * 4 + 12) +
            (n_cells + 1) * 8;
 }
@@ -877,6 +877,8 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     if (!out) return fail(GCR_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (dim != 2 && dim != 3) return fail(GCR_ERR_ARG, "dim must be 2 or 3");
+    if (dim == 3 && capB >= ((int64_t)1 << NL3_SLOT_BITS))
+        return fail(GCR_ERR_CAPACITY, "3D pools hold fewer than 2^26 beads (list entries pack the slot)");
     if (precision != GCR_FP64 && precision != GCR_MIXED) return fail(GCR_ERR_ARG, "bad precision");
     if (capB <= 0 || capC <= 0 || capCh <= 0 || capR <= 0 || capB > (1ll << 30) || capC > (1ll << 30))
         return fail(GCR_ERR_ARG, "capacities must be positive and below 2^30");
@@ -946,9 +948,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->SCNT, nb));
     A(dalloc(&p->TILES, p->capTile * 4));
     A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
-    // neighbour lists: 2D u16 LDS indices (NL_MAX), 3D int32 slots (NL3_MAX),
-    // 128 B per bead either way
-    A(dalloc(&p->NL, nb * NL_MAX));
+    // neighbour lists: 2D u16 LDS indices (NL_MAX, 128 B per bead), 3D int32
+    // slots (NL3_MAX, 256 B per bead: LJ cutoffs give about 30 neighbours)
+    A(dalloc(&p->NL, nb * NL_MAX * (dim == 3 ? 2 : 1)));
     A(dalloc(&p->NLEN, nb));
     A(dalloc(&p->XR, nb * dim));
     if (rc == GCR_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)

@@ -713,18 +713,15 @@ __device__ __forceinline__ void add_pair_sum(const double* X, const int32_t* MET
 
 // ------------------------------------------------------------ 3D lists
 
-#define NL3_MAX 32  // 3D neighbour-list entries per bead: 8 groups of 4 slots
+#define NL3_MAX 64  // 3D neighbour-list entries per bead: 16 groups of 4 slots
+// A 3D list entry: the partner's slot in the low 26 bits, and above them the
+// stencil offset of its cell, (dx + 1) + 3 (dy + 1) + 9 (dz + 1) -- what
+// pair_sum knows from its loop -- so the list sum needs no cell index.
+#define NL3_SLOT_BITS 26
 
-// Cell offset in {-1, 0, 1} between two cell coordinates of a grid of nc >= 3
-// cells per side (the stencil offset pair_sum visits the partner's cell at).
-__device__ __forceinline__ int cell_offset(int cj, int ci, int nc)
-{
-    const int d = cj - ci;
-    return d > 1 ? d - nc : (d < -1 ? d + nc : d);
-}
-
-// Pair forces of a 3D bead over its list of global slots (groups of 4 at
-// NL[g * stride + s], the last group padded with s itself). Same order and
+// Pair forces of a 3D bead over its list (groups of 4 entries at
+// NL[g * stride + s], the last group padded with s itself): each entry's slot
+// and cell offset give the partner's cell without a division. Same order and
 // the same operations per pair as pair_sum, so either gives the same bits.
 template <typename FT>
 __device__ __forceinline__ void list_pair_sum3(const PoolArgs& a, const RepParams& P, const double* X, int s,
@@ -740,18 +737,21 @@ __device__ __forceinline__ void list_pair_sum3(const PoolArgs& a, const RepParam
     const int ng = (len + 3) >> 2;
     for (int g = 0; g < ng; ++g) {
         const int4 e = G[(int64_t)g * a.nl_stride];
-        const int ts[4] = {e.x, e.y, e.z, e.w};
+        const int es[4] = {e.x, e.y, e.z, e.w};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int t = ts[u];
+            const int t = es[u] & ((1 << NL3_SLOT_BITS) - 1);
+            const int code = (int)((unsigned)es[u] >> NL3_SLOT_BITS);
+            const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
             double xj[3];
             load_vec<3>(X, t, xj);
-            const int cj = a.CELL[t];
-            const int cjc[3] = {cj % nc, (cj / nc) % nc, cj / (nc * nc)};
             FT sep[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
-                sep[i] = cell_sep<FT>((FT)cell_offset(cjc[i], ci[i], nc), k.h, (FT)cell_rel(xj[i], cjc[i], P), ri[i]);
+            for (int i = 0; i < 3; ++i) {
+                int cj = ci[i] + d[i];
+                cj = cj < 0 ? cj + nc : (cj >= nc ? cj - nc : cj);
+                sep[i] = cell_sep<FT>((FT)d[i], k.h, (FT)cell_rel(xj[i], cj, P), ri[i]);
+            }
             if (k.shifted)
                 pair_term_shifted<3, FT>(sep, k, f);
             else
@@ -807,7 +807,8 @@ __global__ void __launch_bounds__(BS) k_nlist3d(PoolArgs a, int j)
     for (int i = 0; i < 3; ++i) ri[i] = (FT)cell_rel(xs[i], ci[i], P);
     int4* NL = reinterpret_cast<int4*>(a.NL);
     int n = 0;
-    int4 buf = make_int4(s, s, s, s);
+    const int pad = s | (13 << NL3_SLOT_BITS);  // the bead itself, offset (0, 0, 0)
+    int4 buf = make_int4(pad, pad, pad, pad);
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = (ci[2] + dz + nc) % nc;
         for (int dy = -1; dy <= 1; ++dy) {
@@ -837,13 +838,14 @@ __global__ void __launch_bounds__(BS) k_nlist3d(PoolArgs a, int j)
                     if (!(ds <= rl2)) continue;
                     if (n < NL3_MAX) {
                         const int sl = n & 3;
-                        buf.x = sl == 0 ? t : buf.x;
-                        buf.y = sl == 1 ? t : buf.y;
-                        buf.z = sl == 2 ? t : buf.z;
-                        buf.w = sl == 3 ? t : buf.w;
+                        const int ent = t | (((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << NL3_SLOT_BITS);
+                        buf.x = sl == 0 ? ent : buf.x;
+                        buf.y = sl == 1 ? ent : buf.y;
+                        buf.z = sl == 2 ? ent : buf.z;
+                        buf.w = sl == 3 ? ent : buf.w;
                         if (sl == 3) {
                             NL[(int64_t)(n >> 2) * a.nl_stride + s] = buf;
-                            buf = make_int4(s, s, s, s);
+                            buf = make_int4(pad, pad, pad, pad);
                         }
                     }
                     ++n;

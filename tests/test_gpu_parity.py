@@ -119,9 +119,10 @@ def cases():
                                     cutoff_factor=W.LJ_25_SIGMA, T=0.01, Dr=0.1, steps=100,
                                     seed=9, skin=3.3)
     # 3D with a skin so wide that most beads have more list candidates than a
-    # 3D list holds: they sum their pairs by the cell scan
+    # 3D list holds (about 90 within the 3.3 cell edge against 64): they sum
+    # their pairs by the cell scan
     out["list_overflow3d"] = W.chains("list_overflow3d", 3, 600, 1, L=10.0, T=0.01, Dr=0.1,
-                                      steps=100, seed=13, skin=1.5)
+                                      steps=100, seed=13, skin=2.1)
     # about 80 beads per cell: a one-cell tile with its halo fits in LDS, a
     # three-cell one does not (see test_tile_shape_does_not_change_results)
     out["dense_cells"] = W.chains("dense_cells", 2, 1300, 1, phi=0.55,
