@@ -414,14 +414,14 @@ int rebuild_tables(gcr_pool* p)
             P.ntile = n;
             P.red_off = P.tile_off;
             P.red_n = P.ntile;
-        } else {
-            P.red_off = P.blk_off;
-            P.red_n = P.nblk;
+        } else {  // k_stage: STAGE_LANES blocks per bead block
+            P.red_off = STAGE_LANES * P.blk_off;
+            P.red_n = STAGE_LANES * P.nblk;
         }
         // k_group_stage writes the maxima of its blocks (GROUP_LANES per bead
-        // block) after the tile and bead-block maxima
+        // block) after the tile and stage-block maxima
         if (P.flags & (GCR_F_GROUP | GCR_F_SLIDE)) {
-            P.red_off = (int32_t)(p->capBlk + p->capTile) + GROUP_LANES * P.blk_off;
+            P.red_off = (int32_t)(STAGE_LANES * p->capBlk + p->capTile) + GROUP_LANES * P.blk_off;
             P.red_n = GROUP_LANES * P.nblk;
         }
     }
@@ -591,11 +591,11 @@ void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEv
             launch_k(k_update<2>, p->nblk, st, ev0, ev1, a, j);
     } else {
         if (which == 0)
-            mixed ? launch_k(k_stage<3, float, 0>, p->nblk, st, ev0, ev1, a, j)
-                  : launch_k(k_stage<3, double, 0>, p->nblk, st, ev0, ev1, a, j);
+            mixed ? launch_k(k_stage<3, float, 0>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j)
+                  : launch_k(k_stage<3, double, 0>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j);
         else if (which == 1)
-            mixed ? launch_k(k_stage<3, float, 1>, p->nblk, st, ev0, ev1, a, j)
-                  : launch_k(k_stage<3, double, 1>, p->nblk, st, ev0, ev1, a, j);
+            mixed ? launch_k(k_stage<3, float, 1>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j)
+                  : launch_k(k_stage<3, double, 1>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j);
         else
             launch_k(k_update<3>, p->nblk, st, ev0, ev1, a, j);
     }
@@ -935,9 +935,10 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CCOUNT, capC + 1));
     A(dalloc(&p->CSTART, capC + 1));
     A(dalloc(&p->CHKSUM, p->capChk));
-    // tile or bead-block maxima of the force stages, then those of
-    // k_group_stage's blocks (GROUP_LANES per bead block)
-    A(dalloc(&p->BMAX, p->capBlk + p->capTile + GROUP_LANES * p->capBlk));
+    // tile or stage-block maxima of the force stages (STAGE_LANES blocks per
+    // bead block in 3D), then those of k_group_stage's blocks (GROUP_LANES per
+    // bead block)
+    A(dalloc(&p->BMAX, STAGE_LANES * p->capBlk + p->capTile + GROUP_LANES * p->capBlk));
     p->capX = (int64_t)capR * GCR_MAX_POPS;
     A(dalloc(&p->XPAIRS, p->capX));
     A(dalloc(&p->GRP, capR));

@@ -24,6 +24,8 @@
 // bead measured slower on 400k beads in two populations (47 against 45 us per
 // stage).
 #define GROUP_LANES 1
+// lanes per bead of the 3D stage kernel (k_stage): they split its pair list
+#define STAGE_LANES 4
 #define ROD_MAX_CPW 21  // chains per wavefront of k_rod at most (chains of 3 beads)
 
 namespace gcr {
@@ -725,7 +727,8 @@ __device__ __forceinline__ void add_pair_sum(const double* X, const int32_t* MET
 // the same operations per pair as pair_sum, so either gives the same bits.
 template <typename FT>
 __device__ __forceinline__ void list_pair_sum3(const PoolArgs& a, const RepParams& P, const double* X, int s,
-                                               const double* xs, int cell, int len, FT* f)
+                                               const double* xs, int cell, int len, FT* f, int first = 0,
+                                               int stride = 1)
 {
     const int nc = P.nc;
     const int ci[3] = {cell % nc, (cell / nc) % nc, cell / (nc * nc)};
@@ -735,7 +738,7 @@ __device__ __forceinline__ void list_pair_sum3(const PoolArgs& a, const RepParam
     for (int i = 0; i < 3; ++i) ri[i] = (FT)cell_rel(xs[i], ci[i], P);
     const int4* G = reinterpret_cast<const int4*>(a.NL) + s;
     const int ng = (len + 3) >> 2;
-    for (int g = 0; g < ng; ++g) {
+    for (int g = first; g < ng; g += stride) {
         const int4 e = G[(int64_t)g * a.nl_stride];
         const int es[4] = {e.x, e.y, e.z, e.w};
 #pragma unroll
@@ -775,6 +778,28 @@ __device__ __forceinline__ void add_pairs(const PoolArgs& a, const RepParams& P,
     } else {
         add_pair_sum<DIM, FT>(X, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
     }
+}
+
+// The same for 3D with STAGE_LANES consecutive lanes per bead: the list's
+// groups split over the lanes and their partial sums added through a
+// butterfly (each add commutes, so every lane ends with the same sum); a
+// bead without a list is summed by its first lane alone, in pair_sum's order.
+template <typename FT>
+__device__ __forceinline__ void add_pairs_lanes(const PoolArgs& a, const RepParams& P, const double* X, int s,
+                                                const double* xs, int2 me, double* acc, int g)
+{
+    const int len = a.NLEN[s];
+    FT f[3] = {(FT)0, (FT)0, (FT)0};
+    if (len >= 0)
+        list_pair_sum3<FT>(a, P, X, s, xs, a.CELL[s], len, f, g, STAGE_LANES);
+    else if (g == 0)
+        pair_sum<3, FT>(X, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, f);
+#pragma unroll
+    for (int m = 1; m < STAGE_LANES; m <<= 1)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) f[i] += __shfl_xor(f[i], m, 64);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) acc[i] += (double)f[i];
 }
 
 // 3D neighbour lists after a re-sort (same run condition): each bead scans
@@ -1038,12 +1063,17 @@ __global__ void __launch_bounds__(BS) k_virtual(PoolArgs a, int j)
 template <int DIM, typename FT, int STAGE>
 __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
 {
-    const int r = a.blk_rep[blockIdx.x];
+    // 3D: STAGE_LANES lanes per bead (the grid has that many blocks per bead
+    // block), which split the bead's pair list; the first one writes
+    constexpr int SL = DIM == 3 ? STAGE_LANES : 1;
+    const int eb = (int)blockIdx.x / SL;
+    const int r = a.blk_rep[eb];
     const RepParams& P = a.P[r];
     RepState* S = a.S + r;
     if (!step_runs(S, j)) return;
     if (STAGE == 1 && !later_stage_applies(P, a.sub)) return;
-    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    const int g = (int)threadIdx.x % SL;
+    const int s = a.blk_slot0[eb] + ((int)blockIdx.x % SL) * (BS / SL) + (int)threadIdx.x / SL;
     const bool valid = s < P.bead_off + P.n;
     const double* Xa = STAGE == 0 ? a.X : a.X1;
     double fsq = 0.0;
@@ -1063,7 +1093,12 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
             load_chain_nb<DIM>(Xa, a.LINK, s, cn);
             bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
         }
-        if (P.flags & GCR_F_WCA) add_pairs<DIM, FT>(a, P, Xa, s, xs, me, acc);
+        if (P.flags & GCR_F_WCA) {
+            if (DIM == 3)
+                add_pairs_lanes<FT>(a, P, Xa, s, xs, me, acc, g);
+            else
+                add_pairs<DIM, FT>(a, P, Xa, s, xs, me, acc);
+        }
         if (P.flags & GCR_F_PROPEL) {
             if (!tang_set) load_vec<DIM>(a.T, s, tang);
             propel<DIM>(a, me.y, tang, acc);
@@ -1073,11 +1108,13 @@ __global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
             load_vec<DIM>(a.X, s, x0);
             add_walls<DIM>(P, x0, acc);
         }
-        if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP)) {  // tangents at this stage, for k_group_stage
-            if (!tang_set) load_vec<DIM>(a.T, s, tang);
-            store_vec<DIM>(a.Tn, s, tang);
+        if (g == 0) {
+            if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP)) {  // tangents at this stage, for k_group_stage
+                if (!tang_set) load_vec<DIM>(a.T, s, tang);
+                store_vec<DIM>(a.Tn, s, tang);
+            }
+            fsq = stage_store<DIM, STAGE>(a, P, S, s, me.x, xs, acc);
         }
-        fsq = stage_store<DIM, STAGE>(a, P, S, s, me.x, xs, acc);
     }
     if (STAGE == 1 && a.sub == 3) {
         // per-block max |F|^2 of the total; k_finalize reduces it per replica
