@@ -27,8 +27,8 @@ thread_local std::string g_err;
 // RepState::frozen values: 0 running, 1 held by the host (replay or
 // diagnostics), 2 marked for restore, FROZEN_FAILED stopped for good
 constexpr int FROZEN_FAILED = 3;
-// after a replay, the replica's next batches start in the safe form (while
-// every running replica of the pool is in that state)
+// after a second replay in a row, the replica's next batches start in the
+// safe form (while every running replica of the pool is in that state)
 constexpr int SAFE_STICKY = 8;
 
 int fail(int code, const std::string& msg)
@@ -129,6 +129,7 @@ struct gcr_pool {
     std::vector<int> live;
     std::vector<int> replays;
     std::vector<int> safe_left;  // per replica: batches still started in the safe form
+    std::vector<int> replayed;   // per replica: its last normal batch was replayed
     std::vector<double*> xi_dev;
     int B = 32, K = 8;
     bool tables_dirty = true;
@@ -784,7 +785,9 @@ int replay(gcr_pool* p)
     if ((rc = pull_states(p))) return rc;
     for (int r : which) {
         p->replays[r] += 1;
-        p->safe_left[r] = SAFE_STICKY;
+        // a second replay in a row: the next batches start in the safe form
+        if (p->replayed[r]) p->safe_left[r] = SAFE_STICKY;
+        p->replayed[r] = 1;
         // the safe batch re-sorts wherever a step left the skin, so this is
         // not expected; if it happens the pair set may have been incomplete,
         // and the replica stops here, marked failed, while the others go on
@@ -900,6 +903,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->live.assign(capR, 0);
     p->replays.assign(capR, 0);
     p->safe_left.assign(capR, 0);
+    p->replayed.assign(capR, 0);
     p->xi_dev.assign(capR, nullptr);
     p->capBlk = capB / BS + capR + 1;
     p->capChk = capC / SCAN_CHUNK + capR + 1;
@@ -1155,6 +1159,7 @@ int gcr_replica_add(gcr_pool* p, const gcr_replica_desc* d, const double* pos, c
     p->live[r] = 1;
     p->replays[r] = 0;
     p->safe_left[r] = 0;
+    p->replayed[r] = 0;
     HIPCHK(hipStreamSynchronize(st));
     int rc = upload_rep(p, r);
     if (rc) return rc;
@@ -1448,8 +1453,13 @@ int gcr_pool_run(gcr_pool* p, int32_t max_batches, int32_t* n_running)
         HIPCHK(hipGraphLaunch(p->g_norm, p->stream));
         if ((rc = pull_states(p))) return rc;
         bool viol = false;
-        for (int r = 0; r < p->capR; ++r)
-            if (p->live[r] && p->S[r].violation) viol = true;
+        for (int r = 0; r < p->capR; ++r) {
+            if (!p->live[r]) continue;
+            if (p->S[r].violation)
+                viol = true;
+            else
+                p->replayed[r] = 0;  // this normal batch went through
+        }
         if (viol && (rc = replay(p))) return rc;
     }
     if (n_running) *n_running = count_running(p);

@@ -24,8 +24,11 @@
 // bead measured slower on 400k beads in two populations (47 against 45 us per
 // stage).
 #define GROUP_LANES 1
-// lanes per bead of the 3D stage kernel (k_stage): they split its pair list
-#define STAGE_LANES 4
+// Lanes per bead of the 3D stage kernel (k_stage), which would split its pair
+// list. One: four took 100k 3D chain beads from 333 to 318 us per step, but
+// the small 3D requests of a serving stream lost more (3.95e9 -> 3.28e9
+// bead-steps/s aggregate).
+#define STAGE_LANES 1
 #define ROD_MAX_CPW 21  // chains per wavefront of k_rod at most (chains of 3 beads)
 
 namespace gcr {
