@@ -76,12 +76,16 @@ def run_gpu(w, precision, steps, warmup, batch, rebuild, kernel_steps, tile=0, t
     n = w.n
     pool = E.Pool(2, precision, bead_capacity=n, cell_capacity=4 * n + 1024,
                   chain_capacity=w.phys.n_chains, replica_capacity=1)
+    # a run shorter than one graph batch gets a batch of its own length, so
+    # the timed region holds no launches of steps that do not run
+    batch = min(batch, steps)
+    rebuild = min(rebuild, batch)
     pool.set_schedule(batch, rebuild)
     pool.set_tile(tile)
     pool.set_tile_threads(threads)
     rid = pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
     tv = w.time_vals.copy()
-    tv[7] = warmup
+    tv[7] = max(warmup, batch)  # captures the batch graphs before the clock starts
     pool.begin(rid, tv)
     pool.run()
     st = pool.status(rid)
@@ -102,15 +106,37 @@ def run_gpu(w, precision, steps, warmup, batch, rebuild, kernel_steps, tile=0, t
     pos = pool.download(rid, pos1=False, tangent=False, forces=False)["pos"]
     pool.close()
     return dict(sec=sec, bead_steps_per_s=n * steps / sec, ms_per_step=1e3 * sec / steps,
-                kernels_ms=kt, replays=st2["replays"], dt=st2["dt"],
-                finite=bool(np.all(np.isfinite(pos))))
+                kernels_ms=kt, replays=st2["replays"], dt=st2["dt"], steps_per_graph=batch,
+                rebuild_every=rebuild, finite=bool(np.all(np.isfinite(pos))))
+
+
+def host_cores():
+    """Threads for the CPU baseline: every core this process may run on
+    (sched_getaffinity), or fewer where OMP_NUM_THREADS states the host's
+    share for this job; with the CPU model and the machine's core count."""
+    granted = len(os.sched_getaffinity(0))
+    threads = granted
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and 0 < int(omp) < granted:
+        threads = int(omp)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(threads=threads, affinity=granted, nproc=os.cpu_count(), cpu_model=model,
+                omp_num_threads=omp or None)
 
 
 def cpu_baseline(w, budget_s):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_lib as O
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    hc = host_cores()
+    threads = hc["threads"]
     sysd = dict(dim=2, n=w.n, n_nearby=w.phys.n_nearby, has_springs=True, has_wca=True,
                 has_propel=True, L=w.phys.L, D=w.phys.D, Dr=w.phys.Dr, radius=w.phys.radius,
                 wca=np.asarray(w.phys.wca), spring=np.asarray(w.phys.spring), v0=w.phys.v0)
@@ -129,6 +155,7 @@ def cpu_baseline(w, budget_s):
         tv[0] = st[0]
         done += 40
     return dict(value=w.n * done / sec, unit="bead-steps/s", cores=threads, kind="port",
+                host=hc,
                 sample=f"{done} SRK2 steps of the full {w.n}-bead system ({sec:.1f} s), C/OpenMP "
                        f"FP64 restatement of the reference algorithm (cell grid + Verlet list "
                        f"rebuilt every 40 steps, Newton-3 inside each thread chunk)")
@@ -200,7 +227,7 @@ def cpu_option(w, budget_s):
     chunks of 40 steps until about budget_s seconds of CPU work."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_lib as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cores()["threads"]
     p = w.phys
     sysd = dict(dim=p.dim, n=w.n, n_nearby=p.n_nearby, has_springs=p.has_springs,
                 has_wca=p.has_wca, has_propel=p.has_propel, L=p.L, D=p.D, Dr=p.Dr,
@@ -234,7 +261,7 @@ def cpu_growth(final, pos, tangent, chain_id, chain_len, rest_length, time_vals,
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_lib as O
     from gcr_amd import grow as G
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cores()["threads"]
     tv = np.array(time_vals, dtype=np.float64).copy()
     tv[3], tv[4] = 1.0, -1.0
     tv[5] = tv[6] = tv[7] = stepskip
@@ -269,7 +296,7 @@ def cpu_groups(reps, laws, fixed, budget_s):
     in chunks of 40 steps until about budget_s seconds."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_lib as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cores()["threads"]
     pops = []
     for k, w in enumerate(reps):
         p = w.phys
@@ -657,7 +684,8 @@ def main():
         "config": {"workload": "2D periodic, 20k chains x 20 beads (400k), phi=0.4, LJ eps=1 "
                                "cutoff 2.5 sigma, kappa=5, chirality U[-0.5,0.5]/chain, v0=1, "
                                "T=0.05, dtmax=5e-4, SRK2",
-                   "beads": w.n, "box": w.phys.L, "steps_per_graph": args.batch,
+                   "beads": w.n, "box": w.phys.L, "steps_per_graph": m["steps_per_graph"],
+                   "rebuild_every": m["rebuild_every"],
                    "cell_list": tuned},
         "fp64": {"value": res["fp64"]["bead_steps_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
                  "kernels_ms": res["fp64"]["kernels_ms"]},

@@ -1569,6 +1569,27 @@ int gcr_replica_order(gcr_pool* p, int32_t rid, double out[4])
     return GCR_OK;
 }
 
+int gcr_replica_cell_list(gcr_pool* p, int32_t rid, int32_t* nc, int32_t* cstart, int32_t* slot)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    const RepParams& P = p->P[rid];
+    if (nc) *nc = P.nc;
+    if (cstart) {
+        HIPCHK(hipMemcpyAsync(cstart, p->CSTART + P.cell_off, sizeof(int32_t) * ((size_t)P.ncell + 1),
+                              hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+        for (int c = 0; c <= P.ncell; ++c) cstart[c] -= P.bead_off;
+    }
+    if (slot) {
+        HIPCHK(hipMemcpyAsync(slot, p->SLOT + P.bead_off, sizeof(int32_t) * (size_t)P.n,
+                              hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+        for (int b = 0; b < P.n; ++b) slot[b] -= P.bead_off;
+    }
+    return GCR_OK;
+}
+
 int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
 {
     if (!p || !ms_out || steps < 1) return fail(GCR_ERR_ARG, "bad arguments");
@@ -1620,7 +1641,21 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
     bool viol = false;
     for (int r = 0; r < p->capR; ++r)
         if (p->live[r] && p->S[r].violation) viol = true;
-    if (viol) return fail(GCR_ERR_STATE, "skin exceeded during kernel timing; results discarded");
+    // put every timed replica back to the snapshot launch_begin took
+    // (positions, directions, bead order, step state) and re-sort it there:
+    // the timing steps leave no trace in later runs
+    for (int r = 0; r < p->capR; ++r)
+        if (p->live[r] && p->S[r].frozen == 0) p->S[r].frozen = 2;
+    if ((rc = push_states(p))) return rc;
+    if (p->dim == 2)
+        k_restore<2><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
+    else
+        k_restore<3><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
+    k_restore_state<<<(p->capR + 63) / 64, 64, 0, p->stream>>>(p->a);
+    launch_rebuild(p, -1);
+    HIPCHK(hipGetLastError());
+    if ((rc = pull_states(p))) return rc;
+    if (viol) return fail(GCR_ERR_STATE, "skin exceeded during kernel timing; timings discarded");
     return GCR_OK;
 }
 

@@ -855,8 +855,10 @@ __global__ void __launch_bounds__(BS) k_nlist3d(PoolArgs a, int j)
                         const int dd = mj.x - me.x;
                         if ((dd < 0 ? -dd : dd) <= P.n_nearby) continue;
                     }
+                    // partners from the same position set as the bead and
+                    // its cells: X1 on the mid-step re-sort, else X
                     double xj[3];
-                    load_vec<3>(a.X, t, xj);
+                    load_vec<3>(Xk, t, xj);
                     FT sep[3];
 #pragma unroll
                     for (int i = 0; i < 3; ++i) sep[i] = cell_sep<FT>(dc[i], h, (FT)cell_rel(xj[i], cjc[i], P), ri[i]);
@@ -1597,16 +1599,11 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
     store_noise<DIM>(a, P, s, S->steps, S->step_global);
 }
 
-// One force evaluation without an update (parity checks of the force path):
-// forces at X (which == 0) or X1 (which == 1) into FT, tangents into Tn.
+// One force evaluation of the bead in slot s without an update: forces at X
+// (which == 0) or X1 (which == 1) into FT, tangents into Tn.
 template <int DIM, typename FT>
-__global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
+__device__ __forceinline__ void eval_bead(const PoolArgs& a, const RepParams& P, int s, int which)
 {
-    const int r = a.blk_rep[blockIdx.x];
-    if (r != rsel) return;
-    const RepParams& P = a.P[r];
-    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
-    if (s >= P.bead_off + P.n) return;
     const double* Xa = which == 0 ? a.X : a.X1;
     double xs[DIM], acc[DIM], tang[DIM];
     load_vec<DIM>(Xa, s, xs);
@@ -1629,6 +1626,18 @@ __global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
     }
     store_vec<DIM>(a.FT, s, acc);
     store_vec<DIM>(a.Tn, s, tang);
+}
+
+// One force evaluation of replica rsel (parity checks of the force path).
+template <int DIM, typename FT>
+__global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
+{
+    const int r = a.blk_rep[blockIdx.x];
+    if (r != rsel) return;
+    const RepParams& P = a.P[r];
+    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
+    if (s >= P.bead_off + P.n) return;
+    eval_bead<DIM, FT>(a, P, s, which);
 }
 
 // ----------------------------------------------------------------- rebuild

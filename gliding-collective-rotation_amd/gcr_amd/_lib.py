@@ -16,7 +16,15 @@ GCR_MIXED = 1
 
 
 class GcrError(RuntimeError):
-    pass
+    """An error code of libgcr (rc: GCR_ERR_ARG -1, GCR_ERR_HIP -2,
+    GCR_ERR_CAPACITY -3, GCR_ERR_STATE -4) with its message."""
+
+    def __init__(self, msg, rc=None):
+        super().__init__(msg)
+        self.rc = rc
+
+
+GCR_ERR_CAPACITY = -3
 
 
 class ReplicaDesc(C.Structure):
@@ -64,6 +72,7 @@ EXPORTS = [
     "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_threads", "gcr_init_chains",
     "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group", "gcr_replica_set_chunks",
     "gcr_replica_where_active", "gcr_write_table",
+    "gcr_replica_cell_list",
 ]
 
 _lib = None
@@ -110,6 +119,7 @@ def load():
     L.gcr_pool_group.argtypes = [vp, vp, i32, vp, vp]
     L.gcr_replica_set_chunks.argtypes = [vp, i32, vp]
     L.gcr_replica_where_active.argtypes = [vp, i32, vp]
+    L.gcr_replica_cell_list.argtypes = [vp, i32, C.POINTER(i32), vp, vp]
     L.gcr_write_table.argtypes = [C.c_char_p, C.c_char_p, i64, i32, vp, vp, i32]
     L.gcr_create.argtypes = [C.POINTER(ReplicaDesc), i32, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]
     L.gcr_upload.argtypes = [vp, vp, vp]
@@ -127,7 +137,7 @@ def load():
 def check(rc):
     if rc != 0:
         msg = load().gcr_last_error().decode(errors="replace")
-        raise GcrError(f"libgcr error {rc}: {msg}")
+        raise GcrError(f"libgcr error {rc}: {msg}", rc)
 
 
 def ptr(a):

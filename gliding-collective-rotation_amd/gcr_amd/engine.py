@@ -233,6 +233,17 @@ class Pool:
         check(self._L.gcr_replica_where_active(self.h, int(rid), ptr(out)))
         return out
 
+    def cell_list(self, rid):
+        """(cells per side, first slot of each cell, slot of each bead) of
+        the replica's last re-sort (gcr_replica_cell_list)."""
+        nc = C.c_int32()
+        check(self._L.gcr_replica_cell_list(self.h, int(rid), C.byref(nc), None, None))
+        ncell = nc.value ** self.dim
+        cstart = np.empty(ncell + 1, dtype=np.int32)
+        slot = np.empty(self.n[rid], dtype=np.int32)
+        check(self._L.gcr_replica_cell_list(self.h, int(rid), C.byref(nc), ptr(cstart), ptr(slot)))
+        return nc.value, cstart, slot
+
     def time_kernels(self, steps):
         out = np.zeros(4)
         check(self._L.gcr_pool_time_kernels(self.h, int(steps),

@@ -64,21 +64,35 @@ def run_batched(ws, steps, precision=E.GCR_MIXED, steps_per_batch=64, rebuild_ev
     return sec, sum(w.n for w in ws) * steps, pool, rids
 
 
-def run_sequential(ws, steps, precision=E.GCR_MIXED, steps_per_batch=64, rebuild_every=8):
-    """Each replica alone in its own pool, one after another (same kernels)."""
+def run_sequential(ws, steps, precision=E.GCR_MIXED, steps_per_batch=64, rebuild_every=8,
+                   warmup=None):
+    """Each replica alone in its own pool, one after another (same kernels).
+    Like run_batched, each pool first runs `warmup` untimed steps (default
+    one graph batch), which captures its batch graphs, so that both sides
+    time steps only."""
+    if warmup is None:
+        warmup = steps_per_batch
     total_sec, total_bs = 0.0, 0
     for w in ws:
         pool = pool_for([w], precision)
-        pool.set_schedule(steps_per_batch, rebuild_every)
-        (r,) = add_all(pool, [w])
-        tv = w.time_vals.copy()
-        tv[7] = steps
-        pool.begin(r, tv)
-        t0 = time.perf_counter()
-        pool.run()
-        total_sec += time.perf_counter() - t0
-        total_bs += w.n * steps
-        pool.close()
+        try:
+            pool.set_schedule(steps_per_batch, rebuild_every)
+            (r,) = add_all(pool, [w])
+            tv = w.time_vals.copy()
+            if warmup:
+                tv[7] = warmup
+                pool.begin(r, tv)
+                pool.run()
+                tv[0] = pool.status(r)["dt"]
+            tv[7] = steps
+            pool.begin(r, tv)
+            t0 = time.perf_counter()
+            pool.run()
+            total_sec += time.perf_counter() - t0
+            assert pool.status(r)["steps"] == steps
+            total_bs += w.n * steps
+        finally:
+            pool.close()
     return total_sec, total_bs
 
 

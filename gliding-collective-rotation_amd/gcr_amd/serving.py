@@ -105,6 +105,7 @@ class _Lane:
         self.active = {}  # rid -> (request, cells)
         self.batches = 0
         self.error = None
+        self.rejected = []  # requests whose input the library refused
 
     def fits(self, req, cells):
         need = np.array([req.n, cells, req.work.phys.n_chains])
@@ -112,12 +113,20 @@ class _Lane:
         return len(self.active) < self.cap_reps and np.all(self.used + need <= cap)
 
     def admit(self, req, cells, now):
+        """True: admitted; False: no contiguous room (fragmentation), retry
+        later. A request the library rejects for another reason (its input)
+        is finished at once as failed, with the message in its result."""
         w = req.work
         try:
             rid = self.pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths,
                                 w.v_tan_angle, w.v0_per_chain)
-        except _lib.GcrError:
-            return False  # fragmented: wait for more room
+        except _lib.GcrError as e:
+            if e.rc == _lib.GCR_ERR_CAPACITY:
+                return False
+            req.admitted = req.finished = now
+            req.result = {"status": {"failed": True, "steps": 0, "error": str(e)}}
+            self.rejected.append(req)
+            return True
         tv = w.time_vals.copy()
         tv[7] = req.steps
         self.pool.begin(rid, tv)
@@ -194,8 +203,14 @@ class Server:
                     continue
                 still.append(r)
             waiting = still
+            for l in self.lanes.values():
+                finished += l.rejected
+                l.rejected = []
             busy = [l for l in self.lanes.values() if l.active]
             if not busy:
+                if waiting:  # nothing runs and nothing fits: it never will
+                    raise RuntimeError(f"requests {[r.rid for r in waiting]} cannot be admitted "
+                                       "into an idle server")
                 if pending and realtime:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
                 continue

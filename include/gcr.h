@@ -190,6 +190,14 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
  * |sum e^{i phi}|/C, nematic order S, director angle, C]
  * (analysis.py:294-328). */
 int gcr_replica_order(gcr_pool* p, int32_t rid, double out[4]);
+/* The cell list of a replica's last re-sort, the counterpart of grid_list
+ * and where_is_bead (updates.py:502-583): *nc = cells per side (edge L/nc,
+ * cell index (z nc + y) nc + x of the bead's position at the re-sort,
+ * int(x nc / L) clamped into [0, nc)); cstart[c] (nc^dim + 1 entries, NULL
+ * skips) = the first slot of cell c; slot[b] (n_beads entries, NULL skips) =
+ * the slot of bead b. Slots are sorted by cell and, inside a cell, by bead
+ * index (a stable sort of the beads by cell). A replica's slots count from 0. */
+int gcr_replica_cell_list(gcr_pool* p, int32_t rid, int32_t* nc, int32_t* cstart, int32_t* slot);
 /* Per-kernel timing: runs `steps` steps of every armed replica with the
  * kernels of the batch graphs launched directly; stage0, stage1 and update
  * record start/end HIP events in their own dispatch (hipExtLaunchKernelGGL)

@@ -384,3 +384,53 @@ def test_mixed_precision_trajectories_of_each_feature(name):
     s.close()
     L = w.phys.L
     assert pos_err(out["pos"], pos_o, L) <= 1e-6 * L
+
+
+def thin_skin_chains(dim, order):
+    """Chains in a box whose cell edge is only 0.004 above the cutoff (L =
+    20.08 in 2D, 12.05 in 3D, cutoff 1: 20 and 12 cells per side), so the
+    half skin (0.002) is below a
+    typical single-step displacement (sqrt(2 D dt) = 0.0045 per axis): most
+    predictors leave it (pred_far) and the batches replay in the safe form,
+    with a re-sort on the predictor positions before the second stage."""
+    if dim == 2:
+        w = W.chains(f"thin{dim}", 2, 60, 10, L=20.08, kappa=4.0, vtan=0.2, T=0.01,
+                     dtmax=1e-3, steps=64, seed=81, skin=0.004)
+    else:
+        w = W.chains(f"thin{dim}", 3, 60, 8, L=12.05, kappa=4.0, T=0.01, dtmax=1e-3,
+                     steps=64, seed=82, skin=0.004)
+    w.phys.srk_order = order
+    return w
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_thin_skin_replays_match_oracle(dim, order):
+    """Replays with pred_far re-sorts on X1 (2D tiles and 3D lists) and the
+    sticky safe batches that follow two replays in a row: the trajectory
+    still matches the oracle. Batches of 8 steps over 64 steps: the first two
+    normal batches replay, the rest start in the safe form, so the replica
+    sees fewer replays than batches."""
+    w = thin_skin_chains(dim, order)
+    rng = np.random.default_rng(12)
+    xi = rng.normal(size=(64, w.n, 2 * dim))
+    tv = w.time_vals.copy()
+    tv[7] = 64
+    pos_o, pos1_o, *_ = oracle_run(w, xi, tv)
+    p = E.Pool(dim, E.GCR_FP64, bead_capacity=w.n, cell_capacity=E.replica_cells(w.phys) + 1,
+               chain_capacity=w.phys.n_chains, replica_capacity=1)
+    try:
+        p.set_schedule(8, 8)
+        rid = p.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle)
+        p.begin(rid, tv, xi=xi)
+        p.run()
+        st = p.status(rid)
+        out = p.download(rid)
+    finally:
+        p.close()
+    print(f"[thin skin] dim={dim} order={order} replays={st['replays']}", flush=True)
+    L = w.phys.L
+    assert st["steps"] == 64 and not st["failed"]
+    assert 2 <= st["replays"] < 8  # replays, then the sticky safe batches
+    assert pos_err(out["pos"], pos_o, L) <= POS_TOL * L
+    assert pos_err(out["pos1"], pos1_o, L) <= POS_TOL * L
