@@ -1764,3 +1764,6 @@ int gcr_eval_forces(gcr_pool* h, double* forces, double* tangent)
 int gcr_destroy(gcr_pool* h) { return gcr_pool_destroy(h); }
 
 }  // extern "C"
+
+// The force evaluation without device allocation (gcr_eval_forces_noalloc).
+#include "gcr_noalloc.hip"

@@ -71,7 +71,7 @@ EXPORTS = [
     "gcr_replica_download", "gcr_replica_eval_forces", "gcr_pool_time_kernels",
     "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_threads", "gcr_init_chains",
     "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group", "gcr_replica_set_chunks",
-    "gcr_replica_where_active", "gcr_write_table",
+    "gcr_replica_where_active", "gcr_write_table", "gcr_eval_forces_noalloc",
     "gcr_replica_cell_list",
 ]
 
@@ -127,6 +127,8 @@ def load():
     L.gcr_download.argtypes = [vp, vp, vp, vp, vp]
     L.gcr_eval_forces.argtypes = [vp, vp, vp]
     L.gcr_destroy.argtypes = [vp]
+    L.gcr_eval_forces_noalloc.argtypes = [C.POINTER(ReplicaDesc), i32, i32, vp, vp, vp, vp, vp,
+                                          vp, vp, vp]
     for name in EXPORTS:
         if name not in ("gcr_last_error", "gcr_replica_bytes", "gcr_replica_cells"):
             getattr(L, name).restype = C.c_int

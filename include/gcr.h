@@ -116,6 +116,29 @@ int gcr_eval_forces(gcr_pool* h, double* forces, double* tangent);
 
 int gcr_destroy(gcr_pool* h);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc):
+ * the force evaluation of gcr_create + gcr_eval_forces without any
+ * allocation. pos, tangent (NULL: zero directions), chain_id, rest_lengths,
+ * v_tan_angle (NULL: zeros), forces and tangent_out (NULL: skipped) are
+ * DEVICE pointers in the layout of gcr_create / gcr_eval_forces. The cell
+ * list and the bead staging live in one fixed device array of the library
+ * (GCR_NOALLOC_SCRATCH_BYTES per device, held until the process ends): at
+ * most GCR_NOALLOC_MAX_BEADS beads and GCR_NOALLOC_MAX_CELLS cells, larger
+ * systems return GCR_ERR_CAPACITY. The work is enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream) and not waited for. Input that
+ * gcr_create would reject on its contents (a position outside [0, L], chain
+ * ids that are not contiguous runs of 0..n_chains-1) gives NaN forces. Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device race. */
+#define GCR_NOALLOC_MAX_BEADS 32768
+#define GCR_NOALLOC_MAX_CELLS 131072
+#define GCR_NOALLOC_SCRATCH_BYTES (16 << 20)
+int gcr_eval_forces_noalloc(const gcr_replica_desc* desc, int32_t dim, int32_t precision,
+                            const double* pos, const double* tangent, const int64_t* chain_id,
+                            const double* rest_lengths, const double* v_tan_angle, double* forces,
+                            double* tangent_out, void* stream);
+
 /* -------------------------------------------------------------- pool API */
 
 /* A pool of replicas of one dimensionality; capacities bound HBM use. */
