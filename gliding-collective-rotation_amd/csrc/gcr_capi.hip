@@ -153,6 +153,8 @@ struct gcr_pool {
     double* BMAX = nullptr;
     int32_t* SCNT = nullptr;  // per slot: active sliding partners of the running stage
     int32_t* TILES = nullptr;
+    int32_t* TT = nullptr;  // tile layout tables (TT_INTS per tile), sized to the tiles in use
+    int64_t capTT = 0;
     uint16_t* NL = nullptr;
     int32_t* NLEN = nullptr;
     double* XR = nullptr;
@@ -474,6 +476,18 @@ int rebuild_tables(gcr_pool* p)
     }
     if (p->ntile)
         HIPCHK(hipMemcpyAsync(p->TILES, tl.data(), tl.size() * 4, hipMemcpyHostToDevice, p->stream));
+    if (p->ntile > p->capTT) {  // the tile layout tables grow with the tiles in use
+        HIPCHK(hipStreamSynchronize(p->stream));
+        if (p->TT) HIPCHK(hipFree(p->TT));
+        p->TT = nullptr;
+        p->capTT = 0;
+        HIPCHK(hipMalloc((void**)&p->TT, sizeof(int32_t) * TT_INTS * (size_t)p->ntile));
+        p->capTT = p->ntile;
+        p->a.TT = p->TT;
+    }
+    // tables of the new tiles are filled by the next re-sort; until then a
+    // tile takes the path that computes its layout in the kernel
+    if (p->ntile) k_fill_i32<<<(p->ntile * TT_INTS + BS - 1) / BS, BS, 0, p->stream>>>(p->TT, p->ntile * TT_INTS, 0);
     HIPCHK(hipMemcpyAsync(p->rep_chk_off, rco.data(), rco.size() * 4, hipMemcpyHostToDevice, p->stream));
     HIPCHK(hipMemcpyAsync(p->dP, p->P.data(), sizeof(RepParams) * p->capR, hipMemcpyHostToDevice,
                           p->stream));
@@ -1691,7 +1705,7 @@ int gcr_pool_destroy(gcr_pool* p)
                     p->XIn, p->X1n,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
-                    p->CHKSUM, p->BMAX, p->TILES, p->NL, p->NLEN, p->XR, p->OBS,
+                    p->CHKSUM, p->BMAX, p->TILES, p->TT, p->NL, p->NLEN, p->XR, p->OBS,
                     p->XPAIRS, p->GRP, p->CHAIN_CHUNK, p->WA, p->WAn, p->MC, p->SCNT};
     for (void* b : bufs)
         if (b) (void)hipFree(b);

@@ -127,6 +127,7 @@ struct PoolArgs {
     double* BMAX;  // [block] max |F0+F1|^2 of the block (stage 1)
     int32_t* SCNT; // [slot] active sliding partners of the running force stage
     const int32_t* TILES;  // [tile][4] (replica, cx0, cy0, 0) for the 2D tile kernels
+    int32_t* TT;           // [tile][TT_INTS] LDS layout of each 2D tile, written at every re-sort
     int32_t ntile;
     // per-slot state
     double* X;     // [slot][dim] positions
@@ -141,8 +142,10 @@ struct PoolArgs {
     int32_t* LINK; // [slot][4] (prev2, prev1, next1, next2) slots, -1 if none
     int32_t* CELL; // [slot] cell (replica-local) the bead was binned into
     int32_t* SLOT; // [orig_global] -> slot
-    // 2D neighbour lists, rebuilt with every cell re-sort
-    uint16_t* NL;  // [group][slot][4] LDS indices (tile order) of the neighbours
+    // neighbour lists, rebuilt with every cell re-sort: 2D [group][slot][4]
+    // u16 LDS indices (tile order); 3D [group][slot][4] int32 slots with
+    // their cell offsets
+    uint16_t* NL;
     int32_t* NLEN; // [slot] list length, -1: no list (pairs from the cell scan)
     double* XR;    // [slot][2] positions at the last re-sort
     int64_t nl_stride;  // slots per group (the pool's bead capacity)

@@ -284,16 +284,23 @@ __device__ __forceinline__ double outside_cell(const double* x, int c, const Rep
 // complete for every pair within rc, and keeps the bead within half a skin
 // of its cell box, which keeps the cell scan complete.
 template <int DIM>
-__device__ __forceinline__ bool moved_too_far(const PoolArgs& a, const RepParams& P, int s, const double* x)
+__device__ __forceinline__ bool displaced_too_far(const RepParams& P, const double* r, const double* x)
 {
-    double r[DIM], d2 = 0.0;
-    load_vec<DIM>(a.XR, s, r);
+    double d2 = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         const double d = wrap1(x[i] - r[i], P.L, P.halfL);
         d2 += d * d;
     }
     return d2 > P.skin_half * P.skin_half;
+}
+
+template <int DIM>
+__device__ __forceinline__ bool moved_too_far(const PoolArgs& a, const RepParams& P, int s, const double* x)
+{
+    double r[DIM];
+    load_vec<DIM>(a.XR, s, r);
+    return displaced_too_far<DIM>(P, r, x);
 }
 
 // ------------------------------------------------------------ bonded forces
@@ -383,8 +390,8 @@ __device__ __forceinline__ BT sgn(BT x)
 //   FP64: the reference's arithmetic, theta = acos(dot), sin = sqrt(1 - dot^2),
 //   skipped when |dot| == 1.
 //   FP32 (mixed): acos of a dot product near 1 loses most digits on nearly
-//   straight chains, so theta = atan2(|ti x tj|, dot) and sin = |ti x tj|,
-//   skipped when the cross product is exactly zero.
+//   straight chains, so theta = atan2(|ti x tj|, dot) (atan2_pos) and
+//   sin = |ti x tj|, skipped when the cross product is exactly zero.
 template <int DIM>
 __device__ __forceinline__ bool bend_phi(const double* ti, const double* tj, double kappa, double theta0,
                                          double& dot, double& phi)
@@ -397,6 +404,31 @@ __device__ __forceinline__ bool bend_phi(const double* ti, const double* tj, dou
     if (theta0 != 0.0) phi += theta0 * sgn<double>(-ti[0] * tj[1] + ti[1] * tj[0]);
     phi *= kappa / sqrt(1.0 - dot * dot);
     return true;
+}
+
+// atan2(y, x) in [0, pi] for y > 0, in FP32: a = min(|x|, y) / max(|x|, y)
+// from the hardware reciprocal, atan(a) = a P(a^2) with a degree-7
+// polynomial fitted to atan on [0, 1] (weighted least squares on Chebyshev
+// nodes; largest error 3e-7, relative error 1.2e-7 near 0), then the octant
+// and the sign of x. About 17 instructions against about 40 for the
+// IEEE-division atan2f.
+__device__ __forceinline__ float atan2_pos(float y, float x)
+{
+    const float ax = fabsf(x);
+    const float lo = fminf(ax, y), hi = fmaxf(ax, y);
+    const float a = lo * __builtin_amdgcn_rcpf(hi);
+    const float u = a * a;
+    float p = -0.004249828401952982f;
+    p = fmaf(p, u, 0.022837471216917038f);
+    p = fmaf(p, u, -0.05779505893588066f);
+    p = fmaf(p, u, 0.09822430461645126f);
+    p = fmaf(p, u, -0.13998103141784668f);
+    p = fmaf(p, u, 0.19968131184577942f);
+    p = fmaf(p, u, -0.3333188593387604f);
+    p = fmaf(p, u, 0.9999998807907104f);
+    float th = a * p;
+    th = y > ax ? 1.5707963267948966f - th : th;
+    return x < 0.0f ? 3.141592653589793f - th : th;
 }
 
 template <int DIM>
@@ -416,7 +448,7 @@ __device__ __forceinline__ bool bend_phi(const float* ti, const float* tj, float
         cn = sqrtf(cx * cx + cy * cy + cz * cz);
     }
     if (cn == 0.0f) return false;
-    phi = atan2f(cn, dot);
+    phi = atan2_pos(cn, dot);
     if (theta0 != 0.0f) phi += theta0 * sgn<float>(-cz);
     phi *= kappa * __builtin_amdgcn_rcpf(cn);
     return true;
@@ -602,17 +634,28 @@ __device__ __forceinline__ FT cell_sep(FT dc, FT h, FT rel_j, FT rel_i)
     return fma(dc, h, rel_j - rel_i);
 }
 
+// 0 < ds <= rc2 (forces.py:122). In FP32 one unsigned compare: for the
+// non-negative ds of a sum of squares the bit patterns order as the values,
+// and ds = +0 wraps to the largest pattern (NaN is excluded in both forms).
+__device__ __forceinline__ bool in_cutoff(double ds, double rc2) { return (ds <= rc2) & (ds > 0.0); }
+__device__ __forceinline__ bool in_cutoff(float ds, float rc2)
+{
+    return __float_as_uint(ds) - 1u < __float_as_uint(rc2);
+}
+
 // Force of one candidate at separation sep (pointing to the partner), added
 // to f. Branch-free: the cutoff and zero-distance mask keep every lane on one
-// path (forces.py:96-118).
+// path (forces.py:96-118); the law is evaluated for every candidate and its
+// value selected away outside the cutoff (at ds = 0 it is not finite, and
+// the select drops it).
 template <int DIM, typename FT>
 __device__ __forceinline__ void pair_term(const FT* sep, const PairConsts<FT>& k, FT* f)
 {
     FT ds = sep[0] * sep[0];
 #pragma unroll
     for (int i = 1; i < DIM; ++i) ds = fma(sep[i], sep[i], ds);
-    const bool use = (ds <= k.rc2) & (ds > (FT)0);
-    const FT cf = PairLaw<FT>::coeff(use ? ds : (FT)1, k.p0, k.p1);
+    const bool use = in_cutoff(ds, k.rc2);
+    const FT cf = PairLaw<FT>::coeff(ds, k.p0, k.p1);
     const FT w = use ? cf : (FT)0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) f[i] = fma(sep[i], w, f[i]);
@@ -930,12 +973,11 @@ __device__ __forceinline__ void propel(const PoolArgs& a, int chain, const doubl
 // previous step's dt and this step's noise xi, which every stage shares
 // (SRK.py:355-388). A rod replica's x1 is replaced (and checked) by k_rod.
 template <int DIM>
-__device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParams& P, RepState* S,
-                                               int s, const double* x, const double* f,
-                                               double resc, int sub)
+__device__ __forceinline__ void virtual_update_in(const PoolArgs& a, const RepParams& P, RepState* S,
+                                                  int s, const double* x, const double* f,
+                                                  double resc, int sub, const double* xi,
+                                                  const double* xr)
 {
-    double xi[DIM];
-    load_vec<DIM>(a.XI, s, xi);
     const double dt = S->dt;
     double dt_eff = dt * resc * 1.0;
     const double s2 = sqrt(2.0 * dt * P.D);
@@ -948,7 +990,20 @@ __device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParam
         x1[i] = pymod(tmp, P.L);
     }
     store_vec<DIM>(a.X1, s, x1);
-    if (!(P.flags & GCR_F_ROD) && moved_too_far<DIM>(a, P, s, x1)) flag_predictor(a, S, sub);
+    if (!(P.flags & GCR_F_ROD) && displaced_too_far<DIM>(P, xr, x1)) flag_predictor(a, S, sub);
+}
+
+// The same, loading the bead's noise and re-sort position (xi: this step's
+// noise, which every stage shares; xr: the position at the last re-sort).
+template <int DIM>
+__device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParams& P, RepState* S,
+                                               int s, const double* x, const double* f,
+                                               double resc, int sub)
+{
+    double xi[DIM], xr[DIM];
+    load_vec<DIM>(a.XI, s, xi);
+    load_vec<DIM>(a.XR, s, xr);
+    virtual_update_in<DIM>(a, P, S, s, x, f, resc, sub, xi, xr);
 }
 
 // WCA walls normal to the axes in P.walls (forces.py:405-435), `wall_zone`

@@ -27,6 +27,17 @@
 #define TILE_MAX_BEADS 1536  // beads of a tile with its halo held in LDS
 #define TILE_MAX_ROWS 16
 #define NL_MAX 64            // neighbour-list entries per bead (16 groups of 4)
+// Layout table of a tile (TT_INTS ints, written by the list builder at every
+// re-sort): [0] 1 when the stage kernels may use it (the tile fits in LDS and
+// its halo rows and columns do not wrap onto themselves), [1] staged beads,
+// [2] owned beads, [3] owned rows; from TT_HALO, per halo row (first slot of
+// its first run, beads in that run, first slot of the run after the x wrap,
+// LDS offset of the row), a halo row being one or two runs of slots; from
+// TT_OWN, per owned row (first slot, beads, LDS index of its first bead).
+#define TT_INTS 128
+#define TT_HALO 4
+#define TT_OWN (TT_HALO + 4 * (TILE_MAX_ROWS + 2))
+static_assert(TT_OWN + 3 * TILE_MAX_ROWS <= TT_INTS, "tile table size");
 
 namespace gcr {
 
@@ -76,17 +87,13 @@ struct FT2 {
 
 // Pair forces of one bead over its list (len entries, groups of 4 LDS
 // indices at NL[g * stride + s]; the last group is padded with the bead's own
-// index, which the zero-distance mask drops). Summed in FT.
+// index, which the zero-distance mask drops). Summed in FT. first: the first
+// group, loaded with the bead's other inputs.
 template <typename FT, bool SHIFTED>
 __device__ __forceinline__ void list_loop(const LPos<FT>* lp, const uint2* G, int64_t stride, int ng,
-                                          const LPos<FT>& me, const PairConsts<FT>& k, FT* f)
+                                          uint2 first, const LPos<FT>& me, const PairConsts<FT>& k, FT* f)
 {
-    uint32_t e0 = 0u, e1 = 0u;
-    if (ng > 0) {
-        const uint2 e = G[0];
-        e0 = e.x;
-        e1 = e.y;
-    }
+    uint32_t e0 = first.x, e1 = first.y;
     for (int g = 0; g < ng; ++g) {
         uint32_t n0 = e0, n1 = e1;
         if (g + 1 < ng) {  // next group in flight during this one
@@ -112,14 +119,14 @@ __device__ __forceinline__ void list_loop(const LPos<FT>* lp, const uint2* G, in
 
 template <typename FT>
 __device__ __forceinline__ FT2<FT> list_pair_sum(const LPos<FT>* lp, const uint2* G, int64_t stride, int len,
-                                                 const LPos<FT> me, const PairConsts<FT>& k)
+                                                 uint2 first, const LPos<FT> me, const PairConsts<FT>& k)
 {
     FT f[2] = {(FT)0, (FT)0};
     const int ng = (len + 3) >> 2;
     if (k.shifted)
-        list_loop<FT, true>(lp, G, stride, ng, me, k, f);
+        list_loop<FT, true>(lp, G, stride, ng, first, me, k, f);
     else
-        list_loop<FT, false>(lp, G, stride, ng, me, k, f);
+        list_loop<FT, false>(lp, G, stride, ng, first, me, k, f);
     return FT2<FT>{f[0], f[1]};
 }
 
@@ -266,6 +273,44 @@ __device__ __forceinline__ void owned_bead_scan(const TileShared& sh, const Tile
     s = sh.gst[c0] + (li - sh.cst[c0]);
 }
 
+// The tile's layout table (TT_INTS ints at TT[blockIdx.x]) from the layout
+// just computed: halo row hy is halo cells hy W .. hy W + W - 1, whose slots
+// are one run, or two where the row's x coordinates wrap round the box (the
+// left halo cell at nc - 1, or the right ones from 0). Block-uniform call.
+__device__ __forceinline__ void write_tile_table(const PoolArgs& a, const RepParams& P, const TileLayout& t,
+                                                 const TileShared& sh)
+{
+    int* T = a.TT + (size_t)blockIdx.x * TT_INTS;
+    const int W = t.W, rows = P.tile_y + 2;
+    const bool fast = t.in_lds && W <= P.nc && rows <= P.nc;
+    if (threadIdx.x == 0) {
+        T[0] = fast ? 1 : 0;
+        T[1] = t.total;
+        T[2] = sh.own_pref[t.own_y];
+        T[3] = t.own_y;
+    }
+    if (!fast) return;
+    const int tid = (int)threadIdx.x;
+    if (tid < rows) {
+        const int c = tid * W, xa = t.cx0 - 1;
+        // halo cells before the wrap: 1 if the left one wraps, nc - xa if the
+        // right ones do, else the whole row
+        const int k = xa < 0 ? 1 : (xa + W > P.nc ? P.nc - xa : W);
+        int* R = T + TT_HALO + 4 * tid;
+        R[0] = sh.gst[c];
+        R[1] = sh.cst[c + k] - sh.cst[c];
+        R[2] = k < W ? sh.gst[c + k] : 0;
+        R[3] = sh.cst[c];
+    }
+    if (tid < t.own_y) {
+        const int c = (tid + 1) * W + 1;  // owned cells 1 .. own_x of halo row tid + 1
+        int* R = T + TT_OWN + 3 * tid;
+        R[0] = sh.gst[c];
+        R[1] = sh.cst[c + t.own_x] - sh.cst[c];
+        R[2] = sh.cst[c];
+    }
+}
+
 // Neighbour lists of the tile's owned beads, after a cell re-sort (same run
 // condition as the re-sort). Each bead scans the 3 x 3 cells around its own
 // in LDS order (row by row; a row's three cells are one LDS range) and keeps the beads within rc + skin that the same-chain rule
@@ -285,6 +330,7 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
     __shared__ int2 lm[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
     const TileLayout t = tile_layout<NT>(a, P, tile, sh);
+    write_tile_table(a, P, t, sh);
     if (t.in_lds) stage_positions<FT, NT>(a, P, a.on_x1 ? a.X1 : a.X, sh, t, lp, lm, lcell);
     __syncthreads();
     const FT rl2 = (FT)P.rl2, h = (FT)P.h;
@@ -344,15 +390,105 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
     }
 }
 
+// The inputs of one owned bead that come from global memory. The head
+// (position, (orig, chain), list length and first group, chain links) is
+// what the pair loop needs and loads in the prologue; the rest (the chain
+// neighbours' positions, and in the lean variant F0 or the noise and the
+// re-sort position) loads when the bead's turn comes, in flight during its
+// pair loop.
+struct BeadHead {
+    int s;
+    double x[2];
+    int2 me;
+    int len;
+    uint2 g0;  // first list group
+    int4 lk;   // chain links (lean variant)
+};
+
+template <bool GEN>
+__device__ __forceinline__ void load_head(const PoolArgs& a, const RepParams& P, const double* Xa, int s,
+                                          BeadHead& b)
+{
+    b.s = s;
+    load_vec<2>(Xa, s, b.x);
+    b.me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+    b.len = -1;
+    b.g0 = make_uint2(0u, 0u);
+    if (P.flags & GCR_F_WCA) {
+        b.len = a.NLEN[s];
+        b.g0 = reinterpret_cast<const uint2*>(a.NL)[s];
+    }
+    b.lk = (!GEN && (P.flags & GCR_F_SPRINGS)) ? *reinterpret_cast<const int4*>(a.LINK + 4 * s)
+                                               : make_int4(-1, -1, -1, -1);
+}
+
+template <int STAGE, bool GEN>
+struct BeadRest {
+    double pre[2];  // lean stage 1: F0 (SRK2 accumulates into it); lean stage 0: XI
+    double xr[2];   // lean stage 0: position at the last re-sort
+    ChainNb<2> cn;  // lean: chain neighbours
+};
+
+// F0 or the noise and the re-sort position (lean variant).
+template <int STAGE, bool GEN>
+__device__ __forceinline__ void load_rest_state(const PoolArgs& a, int s, BeadRest<STAGE, GEN>& b)
+{
+    if (GEN) return;
+    if (STAGE == 1) load_vec<2>(a.F0, s, b.pre);
+    if (STAGE == 0) {
+        load_vec<2>(a.XI, s, b.pre);
+        load_vec<2>(a.XR, s, b.xr);
+    }
+}
+
+// The chain neighbours' positions (lean variant).
+template <int STAGE, bool GEN>
+__device__ __forceinline__ void load_rest(const PoolArgs& a, const double* Xa, const BeadHead& h,
+                                          BeadRest<STAGE, GEN>& b)
+{
+    if (GEN) return;
+    const int4 lk = h.lk;
+    b.cn.lk = lk;
+    if (lk.x >= 0) load_vec<2>(Xa, lk.x, b.cn.p2);
+    if (lk.y >= 0) load_vec<2>(Xa, lk.y, b.cn.p1);
+    if (lk.z >= 0) load_vec<2>(Xa, lk.z, b.cn.n1);
+    if (lk.w >= 0) load_vec<2>(Xa, lk.w, b.cn.n2);
+}
+
+// Owned bead o of a tile from its layout table: (slot, LDS index), or s = -1.
+__device__ __forceinline__ void table_owned(const int* tt, int o, int& s, int& li)
+{
+    s = -1;
+    li = 0;
+    int before = 0;
+    const int own_y = tt[3];
+    for (int ry = 0; ry < own_y; ++ry) {
+        const int* R = tt + TT_OWN + 3 * ry;
+        if (o < before + R[1]) {
+            s = R[0] + (o - before);
+            li = R[2] + (o - before);
+            return;
+        }
+        before += R[1];
+    }
+}
+
 // STAGE 0 / 1 as k_stage (STAGE 1 for SRK stage a.sub); STAGE 2: force
 // evaluation of replica `j` only at the positions X (forces into FT,
 // tangents into Tn; no noise, no update). NT threads per tile (256, 512 or
 // 1024): large tiles keep more waves in flight. GEN = false is the lean
 // variant for pools without SRK4, walls and FENE springs: their code stays
-// out of the kernel, and with it the registers it would hold (the stage-0
-// kernel then stays within 96 VGPRs, five waves per SIMD).
+// out of the kernel, and with it the registers it would hold. Every variant
+// keeps four waves per SIMD (at most 128 VGPRs): with tiles of 256 threads
+// that holds four tiles per CU, and the 400k-bead system's 1024 tiles run
+// at once.
+// Prologue: with the tile's layout table (written at the last re-sort) each
+// thread finds its first two owned beads and its share of the halo straight
+// from the table and issues all their loads at once; without one (a tile
+// whose table is not filled yet, or that the table does not describe) the
+// layout is computed here (tile_layout) and the halo staged after it.
 template <typename FT, int STAGE, int NT, bool GEN>
-__global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
+__global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
     const int r = tile.x;
@@ -365,39 +501,127 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
     }
     if (STAGE == 1 && !later_stage_applies(P, a.sub)) return;
     const double* Xa = STAGE == 1 ? a.X1 : a.X;
+    const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
-    const TileLayout t = tile_layout<NT>(a, P, tile, sh);
-    if (t.in_lds) stage_positions<FT, NT>(a, P, Xa, sh, t, lp, nullptr, lcell);
+    __shared__ int tt[TT_INTS];
+    static_assert(TT_INTS <= NT, "one table entry per thread");
+    if ((int)threadIdx.x < TT_INTS) tt[threadIdx.x] = a.TT[(size_t)blockIdx.x * TT_INTS + threadIdx.x];
+    __syncthreads();
+    const bool fast = tt[0] != 0;
+    constexpr bool early = sizeof(FT) == 4;  // the FP64 kernels have no registers to spare
+    TileLayout t;
+    int n_own;
+    BeadHead h0, h1;
+    int li0 = 0, li1 = 0, c0_unused = 0;
+    h0.s = h1.s = -1;
+    if (fast) {
+        t.in_lds = true;
+        t.W = P.tile_x + 2;
+        n_own = tt[2];
+        const int total = tt[1];
+        if (early && (int)threadIdx.x < n_own) {
+            int s0, s1;
+            table_owned(tt, (int)threadIdx.x, s0, li0);
+            load_head<GEN>(a, P, Xa, s0, h0);
+            if ((int)threadIdx.x + NT < n_own) {
+                table_owned(tt, (int)threadIdx.x + NT, s1, li1);
+                load_head<GEN>(a, P, Xa, s1, h1);
+            }
+        }
+        // the halo: staged bead i lies in the last halo row whose LDS offset
+        // is <= i, in the first run of that row or after its x wrap
+        constexpr int KS = (TILE_MAX_BEADS + NT - 1) / NT;
+        const int rows = P.tile_y + 2;
+        double2 xv[KS];
+        int cv[KS], hv[KS];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            hv[k] = -1;
+            if (i < total) {
+                int hy = 0;
+                for (int rr = 1; rr < rows; ++rr) hy += tt[TT_HALO + 4 * rr + 3] <= i ? 1 : 0;
+                const int* R = tt + TT_HALO + 4 * hy;
+                const int off = i - R[3];
+                const int g = off < R[1] ? R[0] + off : R[2] + (off - R[1]);
+                xv[k] = *reinterpret_cast<const double2*>(Xa + 2 * g);
+                cv[k] = a.CELL[g];
+                hv[k] = hy;
+            }
+        }
+        const int nc = P.nc;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            if (hv[k] < 0) continue;
+            const int i = (int)threadIdx.x + k * NT;
+            const int gy = (tile.z - 1 + hv[k] + nc) % nc;
+            const int gx = cv[k] - gy * nc;  // the cell the bead was binned into
+            // halo column: x - (cx0 - 1), folded into [0, W) across the wrap
+            // (left: the column at nc - 1; right: the columns from 0)
+            int hx = gx - (tile.y - 1);
+            hx += hx < 0 ? nc : (hx >= nc ? -nc : 0);
+            lp[i] = make_lpos<FT>(cell_rel(xv[k].x, gx, P), cell_rel(xv[k].y, gy, P), hx, hv[k]);
+        }
+    } else {
+        t = tile_layout<NT>(a, P, tile, sh);
+        n_own = sh.own_pref[t.own_y];
+        // the first two owned beads of this thread: slots from the layout (the
+        // staged cell table lcell is not filled yet), their heads in flight
+        // while the halo is staged
+        int ry = 0;
+        if (early && (int)threadIdx.x < n_own) {
+            int s0;
+            owned_bead_scan(sh, t, (int)threadIdx.x, ry, li0, c0_unused, s0);
+            load_head<GEN>(a, P, Xa, s0, h0);
+            if ((int)threadIdx.x + NT < n_own) {
+                int ry1 = ry, s1;
+                owned_bead_scan(sh, t, (int)threadIdx.x + NT, ry1, li1, c0_unused, s1);
+                load_head<GEN>(a, P, Xa, s1, h1);
+            }
+        }
+        if (t.in_lds) stage_positions<FT, NT>(a, P, Xa, sh, t, lp, nullptr, lcell);
+    }
     __syncthreads();
 
     const PairConsts<FT> kc = pair_consts<FT>(P);
-    const int n_own = sh.own_pref[t.own_y];
     const uint2* NL = reinterpret_cast<const uint2*>(a.NL);
-    const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
     double fsq = 0.0;
     int ry = 0;
     for (int o = threadIdx.x; o < n_own; o += NT) {
-        int li, c0, s;
-        if (t.in_lds)
-            owned_bead(sh, t, lcell, o, ry, li, c0, s);
-        else
-            owned_bead_scan(sh, t, o, ry, li, c0, s);
-        double xs[2], acc[2], tang[2];
-        load_vec<2>(Xa, s, xs);
-        const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
-        ChainNb<2> cn;
-        // the lean variant keeps the bonded neighbours' loads in flight
-        // during the pair loop; the general one loads them after it, which
-        // holds it at 94 VGPRs (five waves per SIMD instead of four)
-        if (springs && !GEN) load_chain_nb<2>(Xa, a.LINK, s, cn);
+        int li, s;
+        BeadHead bd;
+        if (early && o == (int)threadIdx.x) {  // heads loaded in the prologue
+            li = li0;
+            bd = h0;
+        } else if (early && o == (int)threadIdx.x + NT) {
+            li = li1;
+            bd = h1;
+        } else {
+            int sx, c0;
+            if (fast)
+                table_owned(tt, o, sx, li);
+            else if (t.in_lds)
+                owned_bead(sh, t, lcell, o, ry, li, c0, sx);
+            else
+                owned_bead_scan(sh, t, o, ry, li, c0, sx);
+            load_head<GEN>(a, P, Xa, sx, bd);
+        }
+        s = bd.s;
+        BeadRest<STAGE, GEN> br;
+        load_rest<STAGE, GEN>(a, Xa, bd, br);
+        // FP32 positions: in flight during the pair loop; FP64: after it (the
+        // registers)
+        if (early) load_rest_state<STAGE, GEN>(a, s, br);
+        const double* xs = bd.x;
+        const int2 me = bd.me;
+        double acc[2], tang[2];
         double pa0 = 0.0, pa1 = 0.0;
         if (P.flags & GCR_F_WCA) {
-            const int len = a.NLEN[s];
-            if (len >= 0) {
+            if (bd.len >= 0) {
                 const LPos<FT> mp = lp[li];
-                const FT2<FT> f = list_pair_sum<FT>(lp, NL + s, a.nl_stride, len, mp, kc);
+                const FT2<FT> f = list_pair_sum<FT>(lp, NL + s, a.nl_stride, bd.len, bd.g0, mp, kc);
                 pa0 = (double)f.x;
                 pa1 = (double)f.y;
             } else {
@@ -407,16 +631,27 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
                 pa1 = (double)pg[1];
             }
         }
+        if (!early) load_rest_state<STAGE, GEN>(a, s, br);
         if (STAGE == 1 && !(GEN && (P.flags & GCR_F_SRK4))) {
-            load_vec<2>(a.F0, s, acc);  // SRK2 accumulates into F0
+            // SRK2 accumulates into F0
+            if (GEN) {
+                load_vec<2>(a.F0, s, acc);
+            } else {
+                acc[0] = br.pre[0];
+                acc[1] = br.pre[1];
+            }
         } else {
             acc[0] = 0.0;
             acc[1] = 0.0;
         }
         // bonded terms first, then the pair sum, then propulsion (SRK.py:228-253)
         bool tang_set = false;
-        if (springs && GEN) load_chain_nb<2>(Xa, a.LINK, s, cn);
-        if (springs) bonded<2, FT, GEN>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+        if (springs) {
+            // the general variant loads the bonded neighbours after the pair
+            // loop (its registers)
+            if (GEN) load_chain_nb<2>(Xa, a.LINK, s, br.cn);
+            bonded<2, FT, GEN>(br.cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+        }
         acc[0] += pa0;
         acc[1] += pa1;
         if (STAGE == 2) {
@@ -439,8 +674,13 @@ __global__ void __launch_bounds__(NT) k_stage_tile2d(PoolArgs a, int j)
                 if (!tang_set) load_vec<2>(a.T, s, tang);
                 store_vec<2>(a.Tn, s, tang);
             }
-            const double q = stage_store<2, STAGE, GEN>(a, P, S, s, me.x, xs, acc);
-            fsq = q > fsq ? q : fsq;
+            if (!GEN && STAGE == 0) {  // stage_store's lean stage 0 with the preloaded noise
+                store_vec<2>(a.F0, s, acc);
+                virtual_update_in<2>(a, P, S, s, xs, acc, 1.0, 0, br.pre, br.xr);
+            } else {
+                const double q = stage_store<2, STAGE, GEN>(a, P, S, s, me.x, xs, acc);
+                fsq = q > fsq ? q : fsq;
+            }
         }
     }
     if (STAGE == 1 && a.sub == 3) {
