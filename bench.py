@@ -6,10 +6,14 @@ Workload: 2D periodic box, 20k chains x 20 beads (400k beads), area fraction
 U[-0.5, 0.5] per chain, speed 1, T=0.05, dt=5e-4. One step = one SRK2 step
 (predictor + corrector) of every bead. `value` is the mixed form (FP32 pair
 forces, FP64 accumulation and state); the plain FP64 form is timed beside it.
+The same line carries the ensemble (256 uneven replicas in one pool, 10^4
+steps each, against replicas run one at a time) and the serving stream (48
+Poisson requests through the continuous-batching server, aggregate rate and
+p50/p99 completion latency), and the CPU baseline.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-ensemble] [--no-serving]
 
-Prints one JSON line. Inputs are resident in HBM before the timed region.
+Prints one JSON line. Inputs are resident in HBM before every timed region.
 """
 import argparse
 import json
@@ -60,11 +64,19 @@ def parse():
                          "a plain 1 GiB device copy kernel, the reference point for HBM "
                          "byte counters (FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--seq-sample", type=int, default=16,
+    ap.add_argument("--ens-steps", type=int, default=10000,
+                    help="ensemble: steps of every replica (the configuration's 10^4)")
+    ap.add_argument("--seq-sample", type=int, default=32,
                     help="ensemble: replicas timed one at a time for the sequential rate")
+    ap.add_argument("--seq-steps", type=int, default=512,
+                    help="ensemble: steps of each replica timed one at a time")
     ap.add_argument("--requests", type=int, default=48)
-    ap.add_argument("--steps-scale", type=float, default=0.25,
+    ap.add_argument("--steps-scale", type=float, default=1.0,
                     help="serve: multiplies every request's step count")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="single: leave the ensemble part out of the line")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="single: leave the serving part out of the line")
     ap.add_argument("--budget-gb", type=float, default=16.0)
     ap.add_argument("--only", default="",
                     help="features mode: run only the options whose name contains this")
@@ -161,61 +173,115 @@ def cpu_baseline(w, budget_s):
                        f"rebuilt every 40 steps, Newton-3 inside each thread chunk)")
 
 
-def main_ensemble(args):
-    """256 replicas of uneven size in one pool vs one at a time."""
+def ensemble_part(args):
+    """The 256-replica ensemble (chain counts log-U[50, 2000], lengths U{5..40},
+    chirality U[-0.5, 0.5], v0 U[0.1, 2], phi U[0.1, 0.5], WCA, mixed
+    precision, dt 1e-3) in one pool with its tiling and re-sort period tuned
+    on the pool, `ens_steps` steps each after a warm-up; against a random
+    sample of the replicas run one at a time in their own pools with the same
+    settings, each warmed up (graph capture) before its clock starts."""
+    from gcr_amd import engine as E
     from gcr_amd import ensemble as ENS
     from gcr_amd import workloads as W
     t0 = time.perf_counter()
-    ws = W.uneven_replicas(seed=0, n_replicas=args.replicas, steps=args.steps)
+    ws = W.uneven_replicas(seed=0, n_replicas=args.replicas, steps=args.ens_steps)
     t_init = time.perf_counter() - t0
-    sec, bs, pool, rids = ENS.run_batched(ws, args.steps, steps_per_batch=args.batch,
-                                          rebuild_every=args.rebuild, warmup=args.warmup)
-    replays = sum(pool.status(r)["replays"] for r in rids)
-    pool.close()
+    pool = ENS.pool_for(ws, E.GCR_MIXED)
+    try:
+        rids = ENS.add_all(pool, ws)
+        t0 = time.perf_counter()
+        tuned = ENS.tune_pool(pool, ws, rids, batch=args.batch)
+        t_tune = time.perf_counter() - t0
+        for w, r in zip(ws, rids):  # warm-up: the graphs of the tuned schedule
+            tv = w.time_vals.copy()
+            tv[7] = args.batch
+            pool.begin(r, tv)
+        pool.run()
+        for w, r in zip(ws, rids):
+            tv = w.time_vals.copy()
+            tv[0] = pool.status(r)["dt"]
+            tv[7] = args.ens_steps
+            pool.begin(r, tv)
+        t0 = time.perf_counter()
+        pool.run()
+        sec = time.perf_counter() - t0
+        sts = [pool.status(r) for r in rids]
+        assert all(st["steps"] == args.ens_steps for st in sts)
+        replays = int(sum(st["replays"] for st in sts))
+    finally:
+        pool.close()
+    bs = sum(w.n for w in ws) * args.ens_steps
     rng = np.random.default_rng(1)
-    sample = [ws[i] for i in rng.choice(len(ws), size=min(args.seq_sample, len(ws)),
-                                       replace=False)]
-    seq_steps = max(64, args.steps // 4)
-    ssec, sbs = ENS.run_sequential(sample, seq_steps, steps_per_batch=args.batch,
-                                   rebuild_every=args.rebuild)
-    out = {"metric": "ensemble aggregate bead-steps/sec (256 replicas, 1 GPU)",
-           "value": bs / sec, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32 forces + f64 state", "data": "synthetic",
-           "config": {"workload": "256 replicas: chain counts log-U[50,2000], lengths U{5..40}, "
-                                  "chirality U[-0.5,0.5], v0 U[0.1,2], phi U[0.1,0.5], WCA, "
-                                  "dt=1e-3", **ENS.summary(ws)},
-           "sequential": {"value": sbs / ssec, "replicas_timed": len(sample),
-                          "steps_each": seq_steps},
-           "batched_over_sequential": (bs / sec) / (sbs / ssec),
-           "replays": replays, "host_init_s": t_init}
+    sample = [ws[i] for i in rng.choice(len(ws), size=min(args.seq_sample, len(ws)), replace=False)]
+    ssec, sbs = ENS.run_sequential(sample, args.seq_steps, steps_per_batch=args.batch,
+                                   rebuild_every=tuned["rebuild_every"], tile=tuned["tile"],
+                                   threads=tuned["threads"])
+    value = bs / sec
+    return {"metric": "ensemble aggregate bead-steps/sec (256 uneven replicas, 1 GPU)",
+            "value": value, "unit": "bead-steps/s", "steps": args.ens_steps,
+            "ms_per_step": 1e3 * sec / args.ens_steps, "seconds": sec,
+            "dtype": "f32 forces + f64 state",
+            "config": {"workload": "256 replicas: chain counts log-U[50,2000], lengths U{5..40}, "
+                                   "chirality U[-0.5,0.5], v0 U[0.1,2], phi U[0.1,0.5], WCA, "
+                                   "dt=1e-3, mixed precision", **ENS.summary(ws),
+                       "tuned": tuned, "tuning_s": t_tune},
+            "sequential": {"value": sbs / ssec, "replicas_timed": len(sample),
+                           "steps_each": args.seq_steps,
+                           "how": "each replica alone in its own pool with the tuned settings, "
+                                  "one graph batch of warm-up before its clock starts"},
+            "batched_over_sequential": value / (sbs / ssec),
+            "roofline": {"bound": "hbm", "achieved": 80 * value / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": 80 * value / 1e9 / HBM_PEAK_GBS,
+                         "note": "80 algorithmic bytes per 2D chain bead-step"},
+            "replays": replays, "host_init_s": t_init}
+
+
+def main_ensemble(args):
+    """The ensemble part alone, as one JSON line."""
+    out = ensemble_part(args)
+    out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "data": "synthetic"})
     print(json.dumps(out), flush=True)
 
 
-def main_serve(args):
-    """Poisson request stream through the continuous-batching server."""
+def serving_part(args):
+    """A Poisson stream (20 requests/s) of `requests` simulations through the
+    continuous-batching server under an HBM budget: bead counts log-U[1k,
+    200k], step counts log-U[1e3, 1e5] times steps_scale, ABPs (D_r = 0.1) and
+    chains of 5-40 beads, 2D and 3D, mixed precision."""
     from gcr_amd import serving as SV
     t0 = time.perf_counter()
     reqs = SV.make_stream(seed=0, n_requests=args.requests, scale_steps=args.steps_scale)
     t_init = time.perf_counter() - t0
     srv = SV.Server(hbm_budget_bytes=int(args.budget_gb * (1 << 30)),
-                    steps_per_batch=args.batch, rebuild_every=args.rebuild)
-    done = srv.serve(reqs)
-    srv.close()
+                    steps_per_batch=args.batch, rebuild_every=8)
+    try:
+        done = srv.serve(reqs)
+        graphs = {f"{d}d": lane.pool.graph_stats() for d, lane in srv.lanes.items()}
+    finally:
+        srv.close()
     rep = SV.report(done)
-    offered = sum(r.n * r.steps for r in reqs) / max(r.arrival for r in reqs)
-    out = {"metric": "serving aggregate bead-steps/sec + p50/p99 completion latency (1 GPU)",
-           "value": rep["bead_steps_per_s"], "unit": "bead-steps/s", "n_gpus": 1,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32 forces + f64 state", "data": "synthetic",
-           "config": {"workload": "request stream: Poisson 20 req/s, beads log-U[1k,200k], steps "
-                                  "log-U[1e3,1e5] x steps_scale, ABPs (D_r=0.1) and chains "
-                                  "5-40, 2D and 3D", "requests": len(reqs),
-                      "steps_scale": args.steps_scale, "hbm_budget_gb": args.budget_gb},
-           "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
-           "mean_queue_s": rep["mean_queue_s"], "makespan_s": rep["makespan_s"],
-           "offered_bead_steps_per_s": offered, "host_init_s": t_init}
+    work = sum(r.n * r.steps for r in reqs)
+    offered = work / max(r.arrival for r in reqs)
+    return {"metric": "serving aggregate bead-steps/sec + p50/p99 completion latency (1 GPU)",
+            "value": rep["bead_steps_per_s"], "unit": "bead-steps/s",
+            "dtype": "f32 forces + f64 state",
+            "config": {"workload": "Poisson stream, 20 requests/s; beads log-U[1k,200k]; steps "
+                                   "log-U[1e3,1e5] x steps_scale; ABPs (D_r=0.1) and chains "
+                                   "5-40, 2D and 3D; mixed precision",
+                       "requests": len(reqs), "steps_scale": args.steps_scale,
+                       "hbm_budget_gb": args.budget_gb, "total_bead_steps": work},
+            "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
+            "mean_queue_s": rep["mean_queue_s"], "makespan_s": rep["makespan_s"],
+            "failed": rep["failed"], "offered_bead_steps_per_s": offered,
+            "graphs": graphs, "host_init_s": t_init}
+
+
+def main_serve(args):
+    """The serving part alone, as one JSON line."""
+    out = serving_part(args)
+    out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "data": "synthetic"})
     print(json.dumps(out), flush=True)
 
 
@@ -669,7 +735,8 @@ def main():
     dom = max(("stage0", "stage1", "update"), key=lambda k: kt[k])
     achieved = ALG_BYTES[dom] * w.n / (kt[dom] * 1e-3) / 1e9
     out = {
-        "metric": "bead-steps/sec (400k beads, 1 GPU), mixed vs FP64",
+        "metric": "bead-steps/sec (400k beads, 1 GPU), mixed vs FP64; ensemble agg. + p50/p99 "
+                  "latency",
         "value": m["bead_steps_per_s"],
         "unit": "bead-steps/s",
         "n_gpus": 1,
@@ -699,6 +766,10 @@ def main():
         "device": dev[0],
         "replays": {k: v["replays"] for k, v in res.items()},
     }
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_part(args)
+    if not args.no_serving:
+        out["serving"] = serving_part(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <utility>
 #include <vector>
@@ -134,6 +135,12 @@ struct gcr_pool {
     int B = 32, K = 8;
     bool tables_dirty = true;
     hipGraphExec_t g_norm = nullptr, g_safe = nullptr;
+    // the tables changed since a graph was captured: it is brought up to date
+    // before its next launch (re-captured, and the executable graph updated in
+    // place when the launch sequence kept its shape)
+    bool norm_stale = false, safe_stale = false;
+    int64_t n_capture = 0, n_instantiate = 0, n_update = 0;
+    double graph_seconds = 0.0;  // host time spent capturing, instantiating, updating
     int nblk = 0, nchk = 0;
     hipStream_t stream = nullptr;
     PoolArgs a{};
@@ -497,7 +504,7 @@ int rebuild_tables(gcr_pool* p)
     p->a.nchb = p->nchb;
     p->a.ntile = p->ntile;
     p->tables_dirty = false;
-    drop_graphs(p);
+    p->norm_stale = p->safe_stale = true;
     return GCR_OK;
 }
 
@@ -727,24 +734,70 @@ void launch_batch(gcr_pool* p, int K, bool safe)
     }
 }
 
-int capture(gcr_pool* p, int K, bool safe, hipGraphExec_t* out)
+double now_seconds()
 {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+// Captures the batch graph (K, safe) and makes *exec run it: an existing
+// executable graph is updated in place when the new capture has its shape
+// (replicas admitted or retired change launch sizes and arguments, not the
+// sequence of kernels), else a new one is instantiated.
+int capture(gcr_pool* p, int K, bool safe, hipGraphExec_t* exec)
+{
+    const double t0 = now_seconds();
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     launch_batch(p, K, safe);
     p->a.safe = 0;
     HIPCHK(hipStreamEndCapture(p->stream, &g));
-    HIPCHK(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+    ++p->n_capture;
+    bool updated = false;
+    if (*exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        const hipError_t e = hipGraphExecUpdate(*exec, g, &bad, &res);
+        updated = e == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGetLastError();  // a refused update is not an error of the pool
+            (void)hipGraphExecDestroy(*exec);
+            *exec = nullptr;
+        }
+    }
+    if (updated) {
+        ++p->n_update;
+    } else {
+        const hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(GCR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        ++p->n_instantiate;
+    }
     HIPCHK(hipGraphDestroy(g));
+    p->graph_seconds += now_seconds() - t0;
     return GCR_OK;
 }
 
+// The normal batch graph, up to date with the tables.
 int ensure_ready(gcr_pool* p)
 {
     int rc;
     if (p->tables_dirty && (rc = rebuild_tables(p)) != GCR_OK) return rc;
-    if (!p->g_norm && (rc = capture(p, p->K, false, &p->g_norm)) != GCR_OK) return rc;
-    if (!p->g_safe && (rc = capture(p, 1, true, &p->g_safe)) != GCR_OK) return rc;
+    if ((!p->g_norm || p->norm_stale) && (rc = capture(p, p->K, false, &p->g_norm)) != GCR_OK) return rc;
+    p->norm_stale = false;
+    return GCR_OK;
+}
+
+// The safe batch graph, captured when a replay or a safe batch first needs it.
+int ensure_safe(gcr_pool* p)
+{
+    int rc;
+    if (p->tables_dirty && (rc = rebuild_tables(p)) != GCR_OK) return rc;
+    if ((!p->g_safe || p->safe_stale) && (rc = capture(p, 1, true, &p->g_safe)) != GCR_OK) return rc;
+    p->safe_stale = false;
     return GCR_OK;
 }
 
@@ -788,8 +841,9 @@ int replay(gcr_pool* p)
             p->S[r].frozen = 1;
         }
     }
-    int rc = push_states(p);
+    int rc = ensure_safe(p);
     if (rc) return rc;
+    if ((rc = push_states(p))) return rc;
     if (p->dim == 2)
         k_restore<2><<<p->nblk, BS, 0, p->stream>>>(p->a, 1);
     else
@@ -1450,6 +1504,7 @@ int gcr_pool_run(gcr_pool* p, int32_t max_batches, int32_t* n_running)
                 if (p->safe_left[r] > 0) ++nsafe;
             }
         if (nrun > 0 && nsafe == nrun) {
+            if ((rc = ensure_safe(p))) return rc;
             HIPCHK(hipGraphLaunch(p->g_safe, p->stream));
             if ((rc = pull_states(p))) return rc;
             bool failed = false;
@@ -1670,6 +1725,16 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4])
     HIPCHK(hipGetLastError());
     if ((rc = pull_states(p))) return rc;
     if (viol) return fail(GCR_ERR_STATE, "skin exceeded during kernel timing; timings discarded");
+    return GCR_OK;
+}
+
+int gcr_pool_graph_stats(gcr_pool* p, double out[4])
+{
+    if (!p || !out) return fail(GCR_ERR_ARG, "null argument");
+    out[0] = (double)p->n_capture;
+    out[1] = (double)p->n_instantiate;
+    out[2] = (double)p->n_update;
+    out[3] = p->graph_seconds;
     return GCR_OK;
 }
 

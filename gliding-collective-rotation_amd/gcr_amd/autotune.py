@@ -41,12 +41,21 @@ def _arm(pool, rid, w, steps):
     pool.begin(rid, tv)
 
 
-def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512, 1024), steps=32):
+def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512, 1024), steps=32,
+                warm=256):
     """[(tile, threads, stage0 + stage1 ms)] for every candidate that ran
-    without exceeding the skin, fastest first."""
+    without exceeding the skin, fastest first. The candidates are timed on
+    the state `warm` steps after the initial one (chains of attractive beads
+    cluster within the first hundreds of steps, which lengthens the lists of
+    dense tiles); each timing restores that state, so every candidate sees
+    the same one."""
     pool, rid = _pool_for(w, precision, skin)
     out = []
     try:
+        if warm > 0:
+            pool.set_schedule(64, 8)
+            _arm(pool, rid, w, warm)
+            pool.run()
         for t in tiles:
             for nt in threads:
                 pool.set_tile(t)

@@ -233,6 +233,14 @@ class Pool:
         check(self._L.gcr_replica_where_active(self.h, int(rid), ptr(out)))
         return out
 
+    def graph_stats(self):
+        """Batch-graph work so far: captures, instantiations, in-place
+        updates and the host seconds they took (gcr_pool_graph_stats)."""
+        out = np.zeros(4)
+        check(self._L.gcr_pool_graph_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(captures=int(out[0]), instantiations=int(out[1]), updates=int(out[2]),
+                    seconds=float(out[3]))
+
     def cell_list(self, rid):
         """(cells per side, first slot of each cell, slot of each bead) of
         the replica's last re-sort (gcr_replica_cell_list)."""

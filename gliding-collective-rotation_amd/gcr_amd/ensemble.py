@@ -65,7 +65,7 @@ def run_batched(ws, steps, precision=E.GCR_MIXED, steps_per_batch=64, rebuild_ev
 
 
 def run_sequential(ws, steps, precision=E.GCR_MIXED, steps_per_batch=64, rebuild_every=8,
-                   warmup=None):
+                   warmup=None, tile=0, threads=0):
     """Each replica alone in its own pool, one after another (same kernels).
     Like run_batched, each pool first runs `warmup` untimed steps (default
     one graph batch), which captures its batch graphs, so that both sides
@@ -77,6 +77,8 @@ def run_sequential(ws, steps, precision=E.GCR_MIXED, steps_per_batch=64, rebuild
         pool = pool_for([w], precision)
         try:
             pool.set_schedule(steps_per_batch, rebuild_every)
+            pool.set_tile(tile)
+            pool.set_tile_threads(threads)
             (r,) = add_all(pool, [w])
             tv = w.time_vals.copy()
             if warmup:
@@ -100,3 +102,50 @@ def summary(ws):
     n = np.array([w.n for w in ws])
     return dict(replicas=len(ws), beads=int(n.sum()), min_beads=int(n.min()),
                 max_beads=int(n.max()), mean_beads=float(n.mean()))
+
+
+def tune_pool(pool, ws, rids, tiles=(0, 4, 6, 8), threads=(256, 512), rebuilds=(8, 16, 32),
+              batch=64, warm=128, kernel_steps=16, schedule_steps=256):
+    """Autotunes a batched pool in place: the tile edge and threads per tile
+    of its 2D force kernels (by the force-stage durations of every replica
+    together, gcr_pool_time_kernels, on the state `warm` steps in; no result
+    depends on them), then the re-sort period (by the wall time of graph
+    batches, replays included). Returns the settings and the trials; the
+    replicas are left armed for nothing (begin them again to run)."""
+    def arm(steps):
+        for w, r in zip(ws, rids):
+            tv = w.time_vals.copy()
+            tv[4] = -1.0
+            tv[7] = steps
+            pool.begin(r, tv)
+
+    pool.set_schedule(batch, 8)
+    arm(warm)
+    pool.run()
+    tiling = []
+    for t in (tiles if pool.dim == 2 else (0,)):
+        for nt in (threads if pool.dim == 2 else (0,)):
+            pool.set_tile(t)
+            pool.set_tile_threads(nt)
+            arm(1 << 30)
+            pool.time_kernels(2)  # tables, warm-up
+            kt = pool.time_kernels(kernel_steps)
+            tiling.append((t, nt, kt["stage0"] + kt["stage1"]))
+    tiling.sort(key=lambda x: x[2])
+    tile, nt = tiling[0][0], tiling[0][1]
+    pool.set_tile(tile)
+    pool.set_tile_threads(nt)
+    sched = []
+    for k in rebuilds:
+        pool.set_schedule(batch, k)
+        arm(batch)
+        pool.run()  # captures
+        arm(schedule_steps)
+        t0 = time.perf_counter()
+        pool.run()
+        sched.append((k, 1e3 * (time.perf_counter() - t0) / schedule_steps))
+    sched.sort(key=lambda x: x[1])
+    pool.set_schedule(batch, sched[0][0])
+    return dict(tile=tile, threads=nt, rebuild_every=sched[0][0],
+                tiling_trials=[[t, n, round(1e3 * m, 2)] for t, n, m in tiling],
+                schedule_trials=[[k, round(m, 4)] for k, m in sched])

@@ -227,6 +227,10 @@ int gcr_replica_cell_list(gcr_pool* p, int32_t rid, int32_t* nc, int32_t* cstart
  * on the pool's stream. ms_out = average duration of [stage0, stage1, update,
  * cell re-sort (all its kernels, per re-sort)]. */
 int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4]);
+/* Batch graphs of the pool so far (no counterpart in the reference; the
+ * cost of admitting and retiring replicas): out = [captures, instantiations,
+ * in-place updates of an executable graph, host seconds spent on them]. */
+int gcr_pool_graph_stats(gcr_pool* p, double out[4]);
 /* Tile edge (cells) of the 2D LDS force kernel; 0 picks it per replica from
  * the cell occupancy (about 512 owned beads per workgroup, the tile with its
  * halo within 70% of the LDS capacity at the mean occupancy). At most 16. */

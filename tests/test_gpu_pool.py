@@ -97,3 +97,23 @@ def test_device_order_parameters_match_host(kind):
     assert abs(dev["polar"] - OB.polar_order(phi)) < 1e-12
     assert abs(dev["nematic"] - S) < 1e-12
     assert abs(dev["director"] - d) < 1e-9
+
+
+def test_serving_updates_graphs_in_place():
+    """Admitting and retiring requests changes launch sizes, not the kernel
+    sequence: after the first capture, the normal batch graph of each lane is
+    updated in place, and the safe graph is captured only for a replay."""
+    reqs = small_stream()
+    srv = SV.Server(hbm_budget_bytes=64 << 20, steps_per_batch=32, rebuild_every=8,
+                    max_replicas=8)
+    done = srv.serve(reqs, keep_results=True, realtime=False)
+    stats = {d: lane.pool.graph_stats() for d, lane in srv.lanes.items()}
+    replays = sum(r.result["status"]["replays"] for r in done)
+    srv.close()
+    print("graph stats", stats, "replays", replays, flush=True)
+    for d, st in stats.items():
+        assert st["captures"] >= 2, (d, st)
+        assert st["updates"] >= 1, (d, st)
+        assert st["instantiations"] + st["updates"] == st["captures"]
+        if replays == 0:
+            assert st["instantiations"] <= 2, (d, st)
