@@ -132,6 +132,8 @@ struct gcr_pool {
     std::vector<int> safe_left;  // per replica: batches still started in the safe form
     std::vector<int> replayed;   // per replica: its last normal batch was replayed
     std::vector<double*> xi_dev;
+    std::vector<double*> ser_dev;  // per replica: its observables series (GCR_SERIES_MAX rows)
+    std::vector<int> ser_rows;
     int B = 32, K = 8;
     bool tables_dirty = true;
     hipGraphExec_t g_norm = nullptr, g_safe = nullptr;
@@ -165,7 +167,8 @@ struct gcr_pool {
     uint16_t* NL = nullptr;
     int32_t* NLEN = nullptr;
     double* XR = nullptr;
-    double* OBS = nullptr;  // order-parameter partial sums (4 per chain block + 4)
+    double* OBS = nullptr;  // order-parameter partial sums (5 per chain block + 5)
+    double *COMW = nullptr, *COMU = nullptr;  // per chain: last centre, unwrapped displacement
     int64_t capChk = 0, capBlk = 0, capTile = 0, capChb = 0;
     int ntile = 0;
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
@@ -973,6 +976,8 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->safe_left.assign(capR, 0);
     p->replayed.assign(capR, 0);
     p->xi_dev.assign(capR, nullptr);
+    p->ser_dev.assign(capR, nullptr);
+    p->ser_rows.assign(capR, 0);
     p->capBlk = capB / BS + capR + 1;
     p->capChk = capC / SCAN_CHUNK + capR + 1;
     p->capTile = dim == 2 ? capC + (int64_t)capR + 1 : 1;  // tiles of one cell at worst
@@ -1020,7 +1025,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->MC, nb));
     A(dalloc(&p->SCNT, nb));
     A(dalloc(&p->TILES, p->capTile * 4));
-    A(dalloc(&p->OBS, 4 * (capCh / BS + 2)));
+    A(dalloc(&p->OBS, 5 * (capCh / BS + 2)));
+    A(dalloc(&p->COMW, 3 * capCh));
+    A(dalloc(&p->COMU, 3 * capCh));
     // neighbour lists: 2D u16 LDS indices (NL_MAX, 128 B per bead), 3D int32
     // slots (NL3_MAX, 256 B per bead: LJ cutoffs give about 30 neighbours)
     A(dalloc(&p->NL, nb * NL_MAX * (dim == 3 ? 2 : 1)));
@@ -1339,6 +1346,11 @@ int gcr_replica_remove(gcr_pool* p, int32_t rid)
         (void)hipFree(p->xi_dev[rid]);
         p->xi_dev[rid] = nullptr;
     }
+    if (p->ser_dev[rid]) {
+        (void)hipFree(p->ser_dev[rid]);
+        p->ser_dev[rid] = nullptr;
+    }
+    p->ser_rows[rid] = 0;
     p->live[rid] = 0;
     p->S[rid] = RepState{};
     p->S[rid].frozen = 1;
@@ -1638,6 +1650,55 @@ int gcr_replica_order(gcr_pool* p, int32_t rid, double out[4])
     return GCR_OK;
 }
 
+int gcr_replica_sample(gcr_pool* p, int32_t rid, int32_t reset)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    if (p->tables_dirty && (rc = rebuild_tables(p))) return rc;
+    if (!p->ser_dev[rid]) {
+        HIPCHK(hipMalloc((void**)&p->ser_dev[rid], sizeof(double) * GCR_SERIES_COLS * GCR_SERIES_MAX));
+        p->ser_rows[rid] = 0;
+        reset = 1;  // no origin yet
+    }
+    if (reset) p->ser_rows[rid] = 0;
+    if (p->ser_rows[rid] >= GCR_SERIES_MAX) return fail(GCR_ERR_CAPACITY, "observables series full");
+    const RepParams& P = p->P[rid];
+    const double* tang = p->T;
+    if (P.flags & GCR_F_ORIENT) {
+        // tangents at x1 from the bonds, as the download returns them
+        if (p->dim == 2)
+            k_tangents<2><<<p->nblk, BS, 0, p->stream>>>(p->a, rid);
+        else
+            k_tangents<3><<<p->nblk, BS, 0, p->stream>>>(p->a, rid);
+        tang = p->Tn;
+    }
+    const int nblk = (P.n_chains + BS - 1) / BS;
+    if (p->dim == 2)
+        k_chain_sample<2><<<nblk, BS, 0, p->stream>>>(p->a, rid, tang, reset ? 1 : 0, p->COMW, p->COMU, p->OBS);
+    else
+        k_chain_sample<3><<<nblk, BS, 0, p->stream>>>(p->a, rid, tang, reset ? 1 : 0, p->COMW, p->COMU, p->OBS);
+    k_sample_final<<<1, 64, 0, p->stream>>>(p->a, rid, p->OBS, nblk,
+                                            p->ser_dev[rid] + (size_t)GCR_SERIES_COLS * p->ser_rows[rid]);
+    HIPCHK(hipGetLastError());
+    p->ser_rows[rid] += 1;
+    return GCR_OK;
+}
+
+int gcr_replica_series(gcr_pool* p, int32_t rid, double* out, int32_t max_rows, int32_t* rows)
+{
+    int rc = check_rid(p, rid);
+    if (rc) return rc;
+    const int n = p->ser_rows[rid];
+    if (rows) *rows = n;
+    if (out && n > 0 && max_rows > 0) {
+        const int m = n < max_rows ? n : max_rows;
+        HIPCHK(hipMemcpyAsync(out, p->ser_dev[rid], sizeof(double) * GCR_SERIES_COLS * (size_t)m,
+                              hipMemcpyDeviceToHost, p->stream));
+    }
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return GCR_OK;
+}
+
 int gcr_replica_cell_list(gcr_pool* p, int32_t rid, int32_t* nc, int32_t* cstart, int32_t* slot)
 {
     int rc = check_rid(p, rid);
@@ -1764,13 +1825,15 @@ int gcr_pool_destroy(gcr_pool* p)
     drop_graphs(p);
     for (double* q : p->xi_dev)
         if (q) (void)hipFree(q);
+    for (double* q : p->ser_dev)
+        if (q) (void)hipFree(q);
     void* bufs[] = {p->blk_rep, p->blk_slot0, p->chk_rep, p->chk_cell0, p->rep_chk_off, p->chb_rep,
                     p->chb_chain0, p->dP, p->dS,
                     p->dSs, p->X, p->X1, p->F0, p->FT, p->XI, p->T, p->Xn, p->Tn, p->Xs, p->Ts,
                     p->XIn, p->X1n,
                     p->META, p->LINK, p->CELL, p->SLOT, p->KEYOFF, p->PAIRS, p->METAn, p->METAs,
                     p->CHAIN_A, p->CHAIN_CS, p->CHAIN_FIRST, p->CHAIN_LEN, p->CCOUNT, p->CSTART,
-                    p->CHKSUM, p->BMAX, p->TILES, p->TT, p->NL, p->NLEN, p->XR, p->OBS,
+                    p->CHKSUM, p->BMAX, p->TILES, p->TT, p->NL, p->NLEN, p->XR, p->OBS, p->COMW, p->COMU,
                     p->XPAIRS, p->GRP, p->CHAIN_CHUNK, p->WA, p->WAn, p->MC, p->SCNT};
     for (void* b : bufs)
         if (b) (void)hipFree(b);

@@ -104,6 +104,8 @@ struct RepState {
                           // left the skin
     int32_t fixed;        // a fixed population of a group: never moves (pop.fixed)
     int32_t sort_due;     // safe batch: the last update left the skin, re-sort first
+    int32_t pad;
+    double t_total;       // simulated time since the replica was added (sum of dt_used)
 };
 
 struct PoolArgs {

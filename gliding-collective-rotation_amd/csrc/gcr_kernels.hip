@@ -1202,6 +1202,7 @@ __device__ __forceinline__ void finish_step(const RepParams& P, RepState* S, dou
     S->dt = dt;
     S->dt_used = dt_used;
     S->elapsed += dt_used;
+    S->t_total += dt_used;
     S->maxm = m;
     S->steps += 1;
     S->step_global += 1;
@@ -1590,6 +1591,7 @@ __global__ void k_group_dt(PoolArgs a, int j)
     S->dt = dt;
     S->dt_used = dt_used;
     S->elapsed += dt_used;
+    S->t_total += dt_used;
     S->maxm = m;
     S->steps += 1;
     S->step_global += 1;
@@ -2358,6 +2360,100 @@ __global__ void __launch_bounds__(BS) k_chain_order(PoolArgs a, int r, const dou
         __syncthreads();
     }
     if (threadIdx.x < 4) OBS[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One sample of the collective observables of replica r, per chain: the
+// order terms of k_chain_order, and the chain centre of mass -- the mean of
+// the chain's beads unwrapped bond by bond from its first bead (minimum
+// image), folded into the box -- whose displacement since the last sample
+// (minimum image: a centre moves less than half a box between samples) is
+// added to its unwrapped displacement U since the origin; reset makes the
+// current centres the origin. Per block, the sums (cos phi, sin phi, Q11,
+// Q12, |U|^2) go to PART[5 * block + k] in a fixed order.
+template <int DIM>
+__global__ void __launch_bounds__(BS) k_chain_sample(PoolArgs a, int r, const double* tang, int reset,
+                                                     double* COMW, double* COMU, double* PART)
+{
+    const RepParams& P = a.P[r];
+    const int c = blockIdx.x * BS + threadIdx.x;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (c < P.n_chains) {
+        const int chain = P.chain_off + c;
+        const int first = a.CHAIN_FIRST[chain], len = a.CHAIN_LEN[chain];
+        double sx = 0.0, sy = 0.0;
+        double prev[DIM], acc[DIM];
+        const int s0 = a.SLOT[first];
+        load_vec<DIM>(a.X, s0, prev);
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) acc[i] = prev[i];
+        for (int k = 0; k < len; ++k) {
+            const int s = a.SLOT[first + k];
+            sx += tang[(size_t)s * DIM];
+            sy += tang[(size_t)s * DIM + 1];
+            if (k > 0) {
+                double x[DIM];
+                load_vec<DIM>(a.X, s, x);
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    prev[i] += wrap1(x[i] - prev[i], P.L, P.halfL);
+                    acc[i] += prev[i];
+                }
+            }
+        }
+        const double phi = atan2(sy, sx);
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        v[0] = cs;
+        v[1] = sn;
+        v[2] = 2.0 * cs * cs - 1.0;
+        v[3] = 2.0 * cs * sn;
+        double* W = COMW + (size_t)chain * 3;
+        double* U = COMU + (size_t)chain * 3;
+        double u2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            const double com = pymod(acc[i] / (double)len, P.L);
+            double u = 0.0;
+            if (!reset) u = U[i] + wrap1(com - W[i], P.L, P.halfL);
+            W[i] = com;
+            U[i] = u;
+            u2 += u * u;
+        }
+        v[4] = u2;
+    }
+    __shared__ double red[5][BS];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) PART[5 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The sample row of replica r from the block sums (added in block order):
+// [time since the replica was added, polar order |sum e^{i phi}| / C,
+// nematic order |(Q11, Q12)| / C, director arctan((|Q| - Q11) / Q12) (0 when
+// Q12 = 0), mean |U|^2 over the chains, steps since the replica was added]
+// (analysis.py:294-328). One thread.
+__global__ void k_sample_final(PoolArgs a, int r, const double* PART, int nblk, double* row)
+{
+    if (threadIdx.x != 0) return;
+    double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < nblk; ++b)
+        for (int k = 0; k < 5; ++k) t[k] += PART[5 * b + k];
+    const double C = (double)a.P[r].n_chains;
+    const double qf = hypot(t[2], t[3]);
+    row[0] = a.S[r].t_total;
+    row[1] = hypot(t[0], t[1]) / C;
+    row[2] = qf / C;
+    row[3] = t[3] != 0.0 ? atan((qf - t[2]) / t[3]) : 0.0;
+    row[4] = t[4] / C;
+    row[5] = (double)a.S[r].step_global;
 }
 
 // Sums of the per-block terms, in block order, into OBS[4 * nblk + k].

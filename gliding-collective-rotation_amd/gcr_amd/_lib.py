@@ -72,7 +72,7 @@ EXPORTS = [
     "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_threads", "gcr_init_chains",
     "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group", "gcr_replica_set_chunks",
     "gcr_replica_where_active", "gcr_write_table", "gcr_eval_forces_noalloc",
-    "gcr_replica_cell_list", "gcr_pool_graph_stats",
+    "gcr_replica_cell_list", "gcr_pool_graph_stats", "gcr_replica_sample", "gcr_replica_series",
 ]
 
 _lib = None
@@ -120,6 +120,8 @@ def load():
     L.gcr_replica_set_chunks.argtypes = [vp, i32, vp]
     L.gcr_replica_where_active.argtypes = [vp, i32, vp]
     L.gcr_pool_graph_stats.argtypes = [vp, dp]
+    L.gcr_replica_sample.argtypes = [vp, i32, i32]
+    L.gcr_replica_series.argtypes = [vp, i32, vp, i32, C.POINTER(i32)]
     L.gcr_replica_cell_list.argtypes = [vp, i32, C.POINTER(i32), vp, vp]
     L.gcr_write_table.argtypes = [C.c_char_p, C.c_char_p, i64, i32, vp, vp, i32]
     L.gcr_create.argtypes = [C.POINTER(ReplicaDesc), i32, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]

@@ -233,6 +233,22 @@ class Pool:
         check(self._L.gcr_replica_where_active(self.h, int(rid), ptr(out)))
         return out
 
+    def sample(self, rid, reset=False):
+        """Appends one row of the replica's observables series on the device
+        (gcr_replica_sample): time, polar order, nematic order, director
+        angle, chain-centre MSD since the origin, steps."""
+        check(self._L.gcr_replica_sample(self.h, int(rid), 1 if reset else 0))
+
+    def series(self, rid):
+        """The replica's observables series as a dict of arrays."""
+        n = C.c_int32()
+        check(self._L.gcr_replica_series(self.h, int(rid), None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 6))
+        check(self._L.gcr_replica_series(self.h, int(rid), ptr(out), n.value, C.byref(n)))
+        out = out[: n.value]
+        return dict(t=out[:, 0], polar=out[:, 1], nematic=out[:, 2], director=out[:, 3],
+                    msd=out[:, 4], step=out[:, 5].astype(np.int64))
+
     def graph_stats(self):
         """Batch-graph work so far: captures, instantiations, in-place
         updates and the host seconds they took (gcr_pool_graph_stats)."""

@@ -213,6 +213,23 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
  * |sum e^{i phi}|/C, nematic order S, director angle, C]
  * (analysis.py:294-328). */
 int gcr_replica_order(gcr_pool* p, int32_t rid, double out[4]);
+/* The collective observables of a replica as a time series kept on the
+ * device (no host round trip per sample). gcr_replica_sample appends one row
+ * [simulated time since the replica was added, polar order, nematic order,
+ * director angle -- analysis.py:294-328 over the chains' mean directions of
+ * output_methods.py:492-510, as gcr_replica_order --, mean square
+ * displacement of the chain centres of mass since the origin, steps since
+ * the replica was added]; reset != 0 empties the series and makes the current
+ * centres the origin (also on the first call). A chain's centre is the mean
+ * of its beads unwrapped bond by bond from its first bead; its displacement is
+ * followed sample to sample by the minimum image, so samples must come before
+ * a centre has moved half a box. At most GCR_SERIES_MAX rows (GCR_ERR_CAPACITY
+ * beyond). gcr_replica_series copies min(rows, max_rows) rows of
+ * GCR_SERIES_COLS doubles and sets *rows to the rows held. */
+#define GCR_SERIES_COLS 6
+#define GCR_SERIES_MAX 8192
+int gcr_replica_sample(gcr_pool* p, int32_t rid, int32_t reset);
+int gcr_replica_series(gcr_pool* p, int32_t rid, double* out, int32_t max_rows, int32_t* rows);
 /* The cell list of a replica's last re-sort, the counterpart of grid_list
  * and where_is_bead (updates.py:502-583): *nc = cells per side (edge L/nc,
  * cell index (z nc + y) nc + x of the bead's position at the re-sort,
