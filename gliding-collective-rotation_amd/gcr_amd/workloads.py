@@ -146,3 +146,32 @@ def uneven_replicas(seed=0, n_replicas=256, steps=10000):
         out.append(chains(f"rep{i}", 2, nc, ln, phi=phi, vtan=vt, v0=v0, T=1e-3, dtmax=1e-3,
                           steps=steps, seed=seed * 100003 + i))
     return out
+
+
+def aligned_rods(w, seed=0):
+    """w with an aligned nematic start: every chain a straight rod along +x or
+    -x (probability 1/2 each), on a lattice of rod-sized cells filled in random
+    order, so that the nematic order is 1 and the polar order near 0 at t = 0.
+    Cells are at least one rod length + 1.2 wide and 1.05 high, so no two
+    beads of different chains start closer than the WCA range r = 1."""
+    rng = np.random.default_rng(seed)
+    L, C = w.phys.L, w.phys.n_chains
+    m = w.n // C
+    a = float(w.rest_lengths[0])
+    nx = int(L // ((m - 1) * a + 1.2))
+    ny = -(-C // nx)
+    dy = L / ny
+    if w.phys.dim != 2 or nx < 1 or dy <= 1.05:
+        raise ValueError("aligned_rods: 2D box too small for the rods")
+    pos = np.zeros((w.n, 2))
+    tang = np.zeros((w.n, 2))
+    k = np.arange(m)
+    for c, cell in enumerate(rng.permutation(nx * ny)[:C]):
+        sgn = 1.0 if rng.random() < 0.5 else -1.0
+        x0 = (cell % nx) * (L / nx) + 0.1
+        sl = slice(c * m, (c + 1) * m)
+        pos[sl, 0] = np.mod(x0 + (k if sgn > 0 else m - 1 - k) * a, L)
+        pos[sl, 1] = (cell // nx + 0.5) * dy
+        tang[sl, 0] = sgn
+    w.pos, w.tangent = pos, tang
+    return w
