@@ -17,7 +17,7 @@ scalar v_tan_angle (a per-population constant reduces to the reference).
 import numpy as np
 
 from . import config
-from .engine import init_chains
+from gcr_amd.engine import init_chains
 
 
 def make_WCA_params(repel_d_max, WCA_epsilon, WCA_shift, LJ_cutoff_factor):

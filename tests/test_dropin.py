@@ -6,7 +6,8 @@ import pytest
 
 import gcr_oracle_np as N
 import oracle_lib as O
-from gcr_amd import config, defaults, dropin, populations
+from gcr_amd import dropin
+from refmirror import config, defaults, populations
 from gcr_amd.engine import init_chains
 
 
