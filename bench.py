@@ -405,6 +405,8 @@ def time_pool(reps, steps, warmup, precision, dim=2, group=None, chunks=None, ba
     pool = E.Pool(dim, precision, bead_capacity=nb, cell_capacity=cells + len(reps),
                   chain_capacity=sum(w.phys.n_chains for w in reps),
                   replica_capacity=len(reps))
+    if dim == 3:  # the 2D tile settings do not carry over: 3D tiles are picked per replica
+        tile, threads = 0, 0
     try:
         pool.set_schedule(batch, rebuild)
         pool.set_tile(tile)

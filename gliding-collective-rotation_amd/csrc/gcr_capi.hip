@@ -3,6 +3,7 @@
 // exceeded, and the extern "C" entry points declared in include/gcr.h.
 #include "gcr_kernels.hip"
 #include "gcr_tile2d.hip"
+#include "gcr_tile3d.hip"
 
 #include <hip/hip_ext.h>
 
@@ -18,8 +19,6 @@
 #include "gcr.h"
 
 using namespace gcr;
-
-static_assert(NL3_MAX * 4 == 2 * NL_MAX * 2, "3D lists (int32 slots) take twice the storage of 2D lists (u16)");
 
 namespace {
 
@@ -245,6 +244,20 @@ int pick_tile(const RepParams& P, int forced)
     return std::min(side, P.nc);
 }
 
+// Tile side (cells) of the 3D LDS force kernel: the largest cube whose tile
+// with its halo holds at most 70% of T3_MAX_BEADS at the mean occupancy
+// (a 3D halo is 2-4x the owned cells, so tiles stay small), at most
+// T3_MAX_SIDE and the grid; a forced side is clamped the same way.
+int pick_tile3(const RepParams& P, int forced)
+{
+    const double occ = std::max((double)P.n / (double)P.ncell, 1e-3);
+    int side = forced > 0 ? forced : T3_MAX_SIDE;
+    side = std::max(1, std::min(side, T3_MAX_SIDE));
+    if (forced <= 0)
+        while (side > 1 && (side + 2) * (side + 2) * (side + 2) * occ > 0.7 * T3_MAX_BEADS) --side;
+    return std::min(side, P.nc);
+}
+
 // The dispatcher hands a launch's workgroups to the 8 XCDs of the MI355X
 // round-robin by workgroup index, and each XCD has its own L2. order[b] is
 // the tile (index into a row-major list of n tiles) that workgroup
@@ -427,19 +440,39 @@ int rebuild_tables(gcr_pool* p)
             P.ntile = n;
             P.red_off = P.tile_off;
             P.red_n = P.ntile;
-        } else {  // k_stage: STAGE_LANES blocks per bead block
-            P.red_off = STAGE_LANES * P.blk_off;
-            P.red_n = STAGE_LANES * P.nblk;
+        } else {  // cubic tiles (replica, cx0, cy0, cz0), XCD bands of (x, y, z) order
+            const int side = pick_tile3(P, p->tile_side);
+            P.tile_x = P.tile_y = side;
+            P.tile_off = (int32_t)(tl.size() / 4);
+            std::vector<int32_t> order3;
+            for (int z = 0; z < P.nc; z += side)
+                for (int y = 0; y < P.nc; y += side)
+                    for (int x = 0; x < P.nc; x += side) {
+                        order3.push_back(x);
+                        order3.push_back(y);
+                        order3.push_back(z);
+                    }
+            const int n = (int)order3.size() / 3;
+            const std::vector<int> order = xcd_order(P.tile_off, n);
+            for (int b = 0; b < n; ++b) {
+                tl.push_back(r);
+                tl.push_back(order3[3 * order[b]]);
+                tl.push_back(order3[3 * order[b] + 1]);
+                tl.push_back(order3[3 * order[b] + 2]);
+            }
+            P.ntile = n;
+            P.red_off = P.tile_off;
+            P.red_n = P.ntile;
         }
         // k_group_stage writes the maxima of its blocks (GROUP_LANES per bead
         // block) after the tile and stage-block maxima
         if (P.flags & (GCR_F_GROUP | GCR_F_SLIDE)) {
-            P.red_off = (int32_t)(STAGE_LANES * p->capBlk + p->capTile) + GROUP_LANES * P.blk_off;
+            P.red_off = (int32_t)p->capTile + GROUP_LANES * P.blk_off;
             P.red_n = GROUP_LANES * P.nblk;
         }
     }
     rco[p->capR] = (int32_t)cr.size();
-    if (p->dim == 2) {
+    {
         // about one owned bead per thread: the per-bead chain of dependent
         // loads is long, and only more waves per CU hide it
         int64_t beads = 0;
@@ -447,7 +480,16 @@ int rebuild_tables(gcr_pool* p)
             if (p->live[r]) beads += p->P[r].n;
         const int64_t nt = (int64_t)tl.size() / 4;
         const int64_t per = nt ? beads / nt : 0;
-        p->nt_launch = p->tile_threads ? p->tile_threads : (per >= 768 ? 1024 : per >= 384 ? 512 : 256);
+        if (p->tile_threads)
+            p->nt_launch = p->tile_threads;
+        else if (p->dim == 2)
+            p->nt_launch = per >= 768 ? 1024 : per >= 384 ? 512 : 256;
+        else  // 3D: the halo is 3-5x the owned beads, and staging it is the
+              // longer part of the prologue; FP64 records (40 B) leave room
+              // for two tiles per CU, whose 512 threads would then idle
+              // (100k 3D chain beads: FP32 25 vs 37 us per stage at 512 vs
+              // 256 threads, FP64 41 vs 46 us at 256 vs 512)
+            p->nt_launch = p->precision == GCR_MIXED ? 512 : 256;
     }
     if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk ||
         (int64_t)tl.size() / 4 > p->capTile || (int64_t)hr.size() > p->capChb)
@@ -552,10 +594,24 @@ void launch_rebuild(gcr_pool* p, int j, bool on_x1 = false, int sub = 0)
     } else {
         k_gather<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<3><<<p->nblk, BS, 0, st>>>(a, 1, j);
-        if (p->precision == GCR_MIXED)
-            k_nlist3d<float><<<p->nblk, BS, 0, st>>>(a, j);
-        else
-            k_nlist3d<double><<<p->nblk, BS, 0, st>>>(a, j);
+        if (p->ntile) {
+            const int nt = p->nt_launch;
+            if (p->precision == GCR_MIXED) {
+                if (nt == 1024)
+                    k_nlist_tile3d<float, 1024><<<p->ntile, 1024, 0, st>>>(a, j);
+                else if (nt == 512)
+                    k_nlist_tile3d<float, 512><<<p->ntile, 512, 0, st>>>(a, j);
+                else
+                    k_nlist_tile3d<float, 256><<<p->ntile, 256, 0, st>>>(a, j);
+            } else {
+                if (nt == 1024)
+                    k_nlist_tile3d<double, 1024><<<p->ntile, 1024, 0, st>>>(a, j);
+                else if (nt == 512)
+                    k_nlist_tile3d<double, 512><<<p->ntile, 512, 0, st>>>(a, j);
+                else
+                    k_nlist_tile3d<double, 256><<<p->ntile, 256, 0, st>>>(a, j);
+            }
+        }
     }
 }
 
@@ -595,6 +651,18 @@ void launch_tile(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent
         launch_tile_v<FT, STAGE, false>(p, a, j, ev0, ev1);
 }
 
+// 3D tile kernels at the pool's threads per tile.
+template <typename FT, int STAGE>
+void launch_tile3(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (p->nt_launch == 1024)
+        launch_k(k_stage_tile3d<FT, STAGE, 1024>, p->ntile, p->stream, ev0, ev1, a, j, 1024);
+    else if (p->nt_launch == 512)
+        launch_k(k_stage_tile3d<FT, STAGE, 512>, p->ntile, p->stream, ev0, ev1, a, j, 512);
+    else
+        launch_k(k_stage_tile3d<FT, STAGE, 256>, p->ntile, p->stream, ev0, ev1, a, j, 256);
+}
+
 // One launch of a step: which = 0 the first force stage, 1 a later force
 // stage (SRK stage `sub`: 1 and 2 exist in SRK4 only, 3 is the last), 2 the
 // update, 3 the max-force reduction and dt.
@@ -616,11 +684,9 @@ void launch_stage(gcr_pool* p, int which, int j, hipEvent_t ev0 = nullptr, hipEv
             launch_k(k_update<2>, p->nblk, st, ev0, ev1, a, j);
     } else {
         if (which == 0)
-            mixed ? launch_k(k_stage<3, float, 0>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j)
-                  : launch_k(k_stage<3, double, 0>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j);
+            mixed ? launch_tile3<float, 0>(p, a, j, ev0, ev1) : launch_tile3<double, 0>(p, a, j, ev0, ev1);
         else if (which == 1)
-            mixed ? launch_k(k_stage<3, float, 1>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j)
-                  : launch_k(k_stage<3, double, 1>, p->nblk * STAGE_LANES, st, ev0, ev1, a, j);
+            mixed ? launch_tile3<float, 1>(p, a, j, ev0, ev1) : launch_tile3<double, 1>(p, a, j, ev0, ev1);
         else
             launch_k(k_update<3>, p->nblk, st, ev0, ev1, a, j);
     }
@@ -933,7 +999,7 @@ int64_t gcr_replica_bytes(int32_t dim, int64_t n, int64_t n_chains, int64_t n_ce
     // per bead: 10 dim-vectors + noise (2 x 2 dim) of doubles, 5 int pairs, links,
     // cell, slot and two where_active ints; the neighbour list, its length and
     // the re-sort positions; per chain: 4 doubles + 3 ints; per cell: 2 ints
-    const int64_t nl = (dim == 3 ? 4 * NL3_MAX : 2 * NL_MAX) + 4 + 8 * dim;
+    const int64_t nl = 2 * (dim == 3 ? NL3_MAX : NL_MAX) + 4 + 8 * dim;
     return n * (8 * (10 * dim + 4 * dim) + 4 * (5 * 2 + 4 + 2 + 2) + nl) + n_chains * (8 * 4 + 12) +
            (n_cells + 1) * 8;
 }
@@ -951,8 +1017,6 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     if (!out) return fail(GCR_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (dim != 2 && dim != 3) return fail(GCR_ERR_ARG, "dim must be 2 or 3");
-    if (dim == 3 && capB >= ((int64_t)1 << NL3_SLOT_BITS))
-        return fail(GCR_ERR_CAPACITY, "3D pools hold fewer than 2^26 beads (list entries pack the slot)");
     if (precision != GCR_FP64 && precision != GCR_MIXED) return fail(GCR_ERR_ARG, "bad precision");
     if (capB <= 0 || capC <= 0 || capCh <= 0 || capR <= 0 || capB > (1ll << 30) || capC > (1ll << 30))
         return fail(GCR_ERR_ARG, "capacities must be positive and below 2^30");
@@ -980,7 +1044,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->ser_rows.assign(capR, 0);
     p->capBlk = capB / BS + capR + 1;
     p->capChk = capC / SCAN_CHUNK + capR + 1;
-    p->capTile = dim == 2 ? capC + (int64_t)capR + 1 : 1;  // tiles of one cell at worst
+    p->capTile = capC + (int64_t)capR + 1;  // tiles of one cell at worst
     p->capChb = capCh / BS + capR + 1;
     int rc = GCR_OK;
     auto A = [&](int r) {
@@ -1012,10 +1076,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->CCOUNT, capC + 1));
     A(dalloc(&p->CSTART, capC + 1));
     A(dalloc(&p->CHKSUM, p->capChk));
-    // tile or stage-block maxima of the force stages (STAGE_LANES blocks per
-    // bead block in 3D), then those of k_group_stage's blocks (GROUP_LANES per
-    // bead block)
-    A(dalloc(&p->BMAX, STAGE_LANES * p->capBlk + p->capTile + GROUP_LANES * p->capBlk));
+    // tile maxima of the force stages, then those of k_group_stage's blocks
+    // (GROUP_LANES per bead block)
+    A(dalloc(&p->BMAX, p->capTile + GROUP_LANES * p->capBlk));
     p->capX = (int64_t)capR * GCR_MAX_POPS;
     A(dalloc(&p->XPAIRS, p->capX));
     A(dalloc(&p->GRP, capR));
@@ -1028,9 +1091,9 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     A(dalloc(&p->OBS, 5 * (capCh / BS + 2)));
     A(dalloc(&p->COMW, 3 * capCh));
     A(dalloc(&p->COMU, 3 * capCh));
-    // neighbour lists: 2D u16 LDS indices (NL_MAX, 128 B per bead), 3D int32
-    // slots (NL3_MAX, 256 B per bead: LJ cutoffs give about 30 neighbours)
-    A(dalloc(&p->NL, nb * NL_MAX * (dim == 3 ? 2 : 1)));
+    // neighbour lists: u16 LDS indices (NL_MAX, 128 B per bead in 2D;
+    // NL3_MAX, 256 B per bead in 3D)
+    A(dalloc(&p->NL, nb * (dim == 3 ? NL3_MAX : NL_MAX)));
     A(dalloc(&p->NLEN, nb));
     A(dalloc(&p->XR, nb * dim));
     if (rc == GCR_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
@@ -1101,7 +1164,6 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     a.WA = p->WA;
     a.WAn = p->WAn;
     a.MC = p->MC;
-    a.red_base = dim == 2 ? (int32_t)p->capTile : 0;
     *out = p;
     return GCR_OK;
 }
@@ -1605,8 +1667,8 @@ int gcr_replica_eval_forces(gcr_pool* p, int32_t rid, double* forces, double* ta
         mixed ? launch_tile<float, 2>(p, p->a, rid, nullptr, nullptr)
               : launch_tile<double, 2>(p, p->a, rid, nullptr, nullptr);
     else
-        mixed ? k_eval<3, float><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid)
-              : k_eval<3, double><<<p->nblk, BS, 0, p->stream>>>(p->a, 0, rid);
+        mixed ? launch_tile3<float, 2>(p, p->a, rid, nullptr, nullptr)
+              : launch_tile3<double, 2>(p, p->a, rid, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(p->Ts, p->Tn, sizeof(double) * (size_t)p->capB * p->dim,
                           hipMemcpyDeviceToDevice, p->stream));

@@ -54,7 +54,7 @@ struct RepParams {
     int32_t blk_off;    // first bead block
     int32_t red_off;    // first block of the stage-1 max reduction
     int32_t red_n;      // blocks of the stage-1 max reduction
-    int32_t tile_x, tile_y;  // 2D cell tile of the LDS force kernel
+    int32_t tile_x, tile_y;  // cell tile of the LDS force kernels (3D: cubes of tile_x)
     int32_t tile_off, ntile; // first tile, tiles of this replica
     double L, halfL;
     double h, inv_h;    // cell edge and its inverse
@@ -128,7 +128,7 @@ struct PoolArgs {
     RepState* S;
     double* BMAX;  // [block] max |F0+F1|^2 of the block (stage 1)
     int32_t* SCNT; // [slot] active sliding partners of the running force stage
-    const int32_t* TILES;  // [tile][4] (replica, cx0, cy0, 0) for the 2D tile kernels
+    const int32_t* TILES;  // [tile][4] (replica, cx0, cy0, 0) in 2D, (replica, cx0, cy0, cz0) in 3D
     int32_t* TT;           // [tile][TT_INTS] LDS layout of each 2D tile, written at every re-sort
     int32_t ntile;
     // per-slot state
@@ -144,12 +144,11 @@ struct PoolArgs {
     int32_t* LINK; // [slot][4] (prev2, prev1, next1, next2) slots, -1 if none
     int32_t* CELL; // [slot] cell (replica-local) the bead was binned into
     int32_t* SLOT; // [orig_global] -> slot
-    // neighbour lists, rebuilt with every cell re-sort: 2D [group][slot][4]
-    // u16 LDS indices (tile order); 3D [group][slot][4] int32 slots with
-    // their cell offsets
+    // neighbour lists, rebuilt with every cell re-sort: [group][slot][4] u16
+    // LDS indices in the order the bead's tile stages its halo
     uint16_t* NL;
     int32_t* NLEN; // [slot] list length, -1: no list (pairs from the cell scan)
-    double* XR;    // [slot][2] positions at the last re-sort
+    double* XR;    // [slot][dim] positions at the last re-sort
     int64_t nl_stride;  // slots per group (the pool's bead capacity)
     // per-chain constants
     const double* CHAIN_A;   // rest length
@@ -190,7 +189,6 @@ struct PoolArgs {
     // populations acting on each other
     const XPair* XPAIRS;
     const int32_t* GRP;
-    int32_t red_base;  // first BMAX entry of the bead-block reductions of 2D group members
     int32_t safe;
     int32_t on_x1;
     // SRK stage of this launch: 0 the first force evaluation (at X), 1 and 2

@@ -1,13 +1,15 @@
 // gcr_kernels.hip -- CDNA4 (gfx950) kernels of the Brownian-dynamics hot path.
 //
-// One step of SRK2 (SRK.py:165-396) is three launches over all replicas of a
-// pool:
-//   k_stage<0>  forces at x   (springs, pair, propulsion) -> x1, F0, noise
-//   k_stage<1>  forces at x1, F = F0 + F1, per-block max |F|^2
+// One step of SRK2 (SRK.py:165-396) is four launches over all replicas of a
+// pool (the force stages are the LDS tile kernels of gcr_tile2d.hip and
+// gcr_tile3d.hip):
+//   stage 0     forces at x   (springs, pair, propulsion) -> x1, F0
+//   stage 1     forces at x1, F = F0 + F1, per-tile max |F|^2
 //   k_finalize  per-replica max force -> dt, dt_used (SRK.py:323-353)
-//   k_update    x <- x + dt_used/(2r) F + sqrt(2 D dt_used) xi, rot. diffusion
+//   k_update    x <- x + dt_used/(2r) F + sqrt(2 D dt_used) xi, rot. diffusion,
+//               the next step's noise
 // Every `rebuild_every` steps the beads are re-sorted by cell (counting sort):
-//   k_bin, k_scan_{chunks,reps,final}, k_scatter, k_gather, k_links
+//   k_bin, k_scan_{chunks,reps,final}, k_scatter, k_gather, k_links, lists
 // Force kernels write one output per bead and use no atomics on forces, so a
 // replica's result does not depend on scheduling or on other replicas.
 #include <hip/hip_runtime.h>
@@ -24,11 +26,6 @@
 // bead measured slower on 400k beads in two populations (47 against 45 us per
 // stage).
 #define GROUP_LANES 1
-// Lanes per bead of the 3D stage kernel (k_stage), which would split its pair
-// list. One: four took 100k 3D chain beads from 333 to 318 us per step, but
-// the small 3D requests of a serving stream lost more (3.95e9 -> 3.28e9
-// bead-steps/s aggregate).
-#define STAGE_LANES 1
 #define ROD_MAX_CPW 21  // chains per wavefront of k_rod at most (chains of 3 beads)
 
 namespace gcr {
@@ -759,179 +756,13 @@ __device__ __forceinline__ void add_pair_sum(const double* X, const int32_t* MET
     for (int i = 0; i < DIM; ++i) acc[i] += (double)f[i];
 }
 
-// ------------------------------------------------------------ 3D lists
-
-#define NL3_MAX 64  // 3D neighbour-list entries per bead: 16 groups of 4 slots
-// A 3D list entry: the partner's slot in the low 26 bits, and above them the
-// stencil offset of its cell, (dx + 1) + 3 (dy + 1) + 9 (dz + 1) -- what
-// pair_sum knows from its loop -- so the list sum needs no cell index.
-#define NL3_SLOT_BITS 26
-
-// Pair forces of a 3D bead over its list (groups of 4 entries at
-// NL[g * stride + s], the last group padded with s itself): each entry's slot
-// and cell offset give the partner's cell without a division. Same order and
-// the same operations per pair as pair_sum, so either gives the same bits.
-template <typename FT>
-__device__ __forceinline__ void list_pair_sum3(const PoolArgs& a, const RepParams& P, const double* X, int s,
-                                               const double* xs, int cell, int len, FT* f, int first = 0,
-                                               int stride = 1)
-{
-    const int nc = P.nc;
-    const int ci[3] = {cell % nc, (cell / nc) % nc, cell / (nc * nc)};
-    const PairConsts<FT> k = pair_consts<FT>(P);
-    FT ri[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ri[i] = (FT)cell_rel(xs[i], ci[i], P);
-    const int4* G = reinterpret_cast<const int4*>(a.NL) + s;
-    const int ng = (len + 3) >> 2;
-    for (int g = first; g < ng; g += stride) {
-        const int4 e = G[(int64_t)g * a.nl_stride];
-        const int es[4] = {e.x, e.y, e.z, e.w};
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int t = es[u] & ((1 << NL3_SLOT_BITS) - 1);
-            const int code = (int)((unsigned)es[u] >> NL3_SLOT_BITS);
-            const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
-            double xj[3];
-            load_vec<3>(X, t, xj);
-            FT sep[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                int cj = ci[i] + d[i];
-                cj = cj < 0 ? cj + nc : (cj >= nc ? cj - nc : cj);
-                sep[i] = cell_sep<FT>((FT)d[i], k.h, (FT)cell_rel(xj[i], cj, P), ri[i]);
-            }
-            if (k.shifted)
-                pair_term_shifted<3, FT>(sep, k, f);
-            else
-                pair_term<3, FT>(sep, k, f);
-        }
-    }
-}
-
-// Pair forces of a bead, added to its FP64 force: from its list when it has
-// one (3D), else by the cell scan.
+// Pair forces of a bead from the cell scan, added to its FP64 force (the
+// force evaluations outside the tile kernels).
 template <int DIM, typename FT>
 __device__ __forceinline__ void add_pairs(const PoolArgs& a, const RepParams& P, const double* X, int s,
                                           const double* xs, int2 me, double* acc)
 {
-    const int len = DIM == 3 ? a.NLEN[s] : -1;
-    if (DIM == 3 && len >= 0) {
-        FT f[3] = {(FT)0, (FT)0, (FT)0};
-        list_pair_sum3<FT>(a, P, X, s, xs, a.CELL[s], len, f);
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) acc[i] += (double)f[i];
-    } else {
-        add_pair_sum<DIM, FT>(X, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
-    }
-}
-
-// The same for 3D with STAGE_LANES consecutive lanes per bead: the list's
-// groups split over the lanes and their partial sums added through a
-// butterfly (each add commutes, so every lane ends with the same sum); a
-// bead without a list is summed by its first lane alone, in pair_sum's order.
-template <typename FT>
-__device__ __forceinline__ void add_pairs_lanes(const PoolArgs& a, const RepParams& P, const double* X, int s,
-                                                const double* xs, int2 me, double* acc, int g)
-{
-    const int len = a.NLEN[s];
-    FT f[3] = {(FT)0, (FT)0, (FT)0};
-    if (len >= 0)
-        list_pair_sum3<FT>(a, P, X, s, xs, a.CELL[s], len, f, g, STAGE_LANES);
-    else if (g == 0)
-        pair_sum<3, FT>(X, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, f);
-#pragma unroll
-    for (int m = 1; m < STAGE_LANES; m <<= 1)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) f[i] += __shfl_xor(f[i], m, 64);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) acc[i] += (double)f[i];
-}
-
-// 3D neighbour lists after a re-sort (same run condition): each bead scans
-// its 27 cells in pair_sum's order and keeps the slots within h that the
-// same-chain rule does not exclude. NLEN = -1: no list (more than NL3_MAX
-// entries, or fewer than 3 cells per side).
-template <typename FT>
-__global__ void __launch_bounds__(BS) k_nlist3d(PoolArgs a, int j)
-{
-    const int r = a.blk_rep[blockIdx.x];
-    const RepParams& P = a.P[r];
-    if (!rebuild_runs(a, r, j)) return;
-    if (!(P.flags & GCR_F_WCA)) return;
-    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
-    if (s >= P.bead_off + P.n) return;
-    const int nc = P.nc;
-    if (nc < 3) {
-        a.NLEN[s] = -1;
-        return;
-    }
-    const double* Xk = a.on_x1 ? a.X1 : a.X;
-    double xs[3];
-    load_vec<3>(Xk, s, xs);
-    const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
-    const int cell = a.CELL[s];
-    const int ci[3] = {cell % nc, (cell / nc) % nc, cell / (nc * nc)};
-    const FT h = (FT)P.h, rl2 = (FT)P.rl2;
-    FT ri[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ri[i] = (FT)cell_rel(xs[i], ci[i], P);
-    int4* NL = reinterpret_cast<int4*>(a.NL);
-    int n = 0;
-    const int pad = s | (13 << NL3_SLOT_BITS);  // the bead itself, offset (0, 0, 0)
-    int4 buf = make_int4(pad, pad, pad, pad);
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = (ci[2] + dz + nc) % nc;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = (ci[1] + dy + nc) % nc;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int x = (ci[0] + dx + nc) % nc;
-                const int cjc[3] = {x, y, z};
-                const FT dc[3] = {(FT)dx, (FT)dy, (FT)dz};
-                const int c2 = P.cell_off + (z * nc + y) * nc + x;
-                const int t1 = a.CSTART[c2 + 1];
-                for (int t = a.CSTART[c2]; t < t1; ++t) {
-                    if (t == s) continue;
-                    const int2 mj = *reinterpret_cast<const int2*>(a.META + 2 * t);
-                    if (mj.y == me.y) {
-                        if (P.n_nearby < 0) continue;
-                        const int dd = mj.x - me.x;
-                        if ((dd < 0 ? -dd : dd) <= P.n_nearby) continue;
-                    }
-                    // partners from the same position set as the bead and
-                    // its cells: X1 on the mid-step re-sort, else X
-                    double xj[3];
-                    load_vec<3>(Xk, t, xj);
-                    FT sep[3];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) sep[i] = cell_sep<FT>(dc[i], h, (FT)cell_rel(xj[i], cjc[i], P), ri[i]);
-                    FT ds = sep[0] * sep[0];
-                    ds = fma(sep[1], sep[1], ds);
-                    ds = fma(sep[2], sep[2], ds);
-                    if (!(ds <= rl2)) continue;
-                    if (n < NL3_MAX) {
-                        const int sl = n & 3;
-                        const int ent = t | (((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << NL3_SLOT_BITS);
-                        buf.x = sl == 0 ? ent : buf.x;
-                        buf.y = sl == 1 ? ent : buf.y;
-                        buf.z = sl == 2 ? ent : buf.z;
-                        buf.w = sl == 3 ? ent : buf.w;
-                        if (sl == 3) {
-                            NL[(int64_t)(n >> 2) * a.nl_stride + s] = buf;
-                            buf = make_int4(pad, pad, pad, pad);
-                        }
-                    }
-                    ++n;
-                }
-            }
-        }
-    }
-    if (n <= NL3_MAX) {
-        if (n & 3) NL[(int64_t)(n >> 2) * a.nl_stride + s] = buf;
-        a.NLEN[s] = n;
-    } else {
-        a.NLEN[s] = -1;
-    }
+    add_pair_sum<DIM, FT>(X, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, acc);
 }
 
 // ------------------------------------------------------------------ stages
@@ -1114,73 +945,6 @@ __global__ void __launch_bounds__(BS) k_virtual(PoolArgs a, int j)
     load_vec<DIM>(a.X, s, x);
     load_vec<DIM>(a.F0, s, f);
     virtual_update<DIM>(a, P, S, s, x, f, virtual_rescale(P, a.sub), a.sub);
-}
-
-// Stage kernels over bead blocks (any dimension; the 2D production path is
-// k_stage_tile2d). STAGE 0: forces at x -> x1, F0, noise. STAGE 1: forces at
-// x1 added to F0 (the reference accumulates into the same array,
-// SRK.py:153-160) and the block's max |F|^2 for k_finalize.
-template <int DIM, typename FT, int STAGE>
-__global__ void __launch_bounds__(BS) k_stage(PoolArgs a, int j)
-{
-    // 3D: STAGE_LANES lanes per bead (the grid has that many blocks per bead
-    // block), which split the bead's pair list; the first one writes
-    constexpr int SL = DIM == 3 ? STAGE_LANES : 1;
-    const int eb = (int)blockIdx.x / SL;
-    const int r = a.blk_rep[eb];
-    const RepParams& P = a.P[r];
-    RepState* S = a.S + r;
-    if (!step_runs(S, j)) return;
-    if (STAGE == 1 && !later_stage_applies(P, a.sub)) return;
-    const int g = (int)threadIdx.x % SL;
-    const int s = a.blk_slot0[eb] + ((int)blockIdx.x % SL) * (BS / SL) + (int)threadIdx.x / SL;
-    const bool valid = s < P.bead_off + P.n;
-    const double* Xa = STAGE == 0 ? a.X : a.X1;
-    double fsq = 0.0;
-    if (valid) {
-        double xs[DIM], acc[DIM], tang[DIM];
-        load_vec<DIM>(Xa, s, xs);
-        if (STAGE == 1 && !(P.flags & GCR_F_SRK4)) {
-            load_vec<DIM>(a.F0, s, acc);  // SRK2 accumulates into F0
-        } else {
-#pragma unroll
-            for (int i = 0; i < DIM; ++i) acc[i] = 0.0;
-        }
-        int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
-        bool tang_set = false;
-        if (P.flags & GCR_F_SPRINGS) {
-            ChainNb<DIM> cn;
-            load_chain_nb<DIM>(Xa, a.LINK, s, cn);
-            bonded<DIM, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
-        }
-        if (P.flags & GCR_F_WCA) {
-            if (DIM == 3)
-                add_pairs_lanes<FT>(a, P, Xa, s, xs, me, acc, g);
-            else
-                add_pairs<DIM, FT>(a, P, Xa, s, xs, me, acc);
-        }
-        if (P.flags & GCR_F_PROPEL) {
-            if (!tang_set) load_vec<DIM>(a.T, s, tang);
-            propel<DIM>(a, me.y, tang, acc);
-        }
-        if (P.flags & GCR_F_WALLS) {
-            double x0[DIM];
-            load_vec<DIM>(a.X, s, x0);
-            add_walls<DIM>(P, x0, acc);
-        }
-        if (g == 0) {
-            if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP)) {  // tangents at this stage, for k_group_stage
-                if (!tang_set) load_vec<DIM>(a.T, s, tang);
-                store_vec<DIM>(a.Tn, s, tang);
-            }
-            fsq = stage_store<DIM, STAGE>(a, P, S, s, me.x, xs, acc);
-        }
-    }
-    if (STAGE == 1 && a.sub == 3) {
-        // per-block max |F|^2 of the total; k_finalize reduces it per replica
-        const double m = block_max(fsq);
-        if (threadIdx.x == 0) a.BMAX[blockIdx.x] = m;
-    }
 }
 
 // Step bookkeeping of a replica from the largest |F|^2 over its blocks
@@ -1683,18 +1447,6 @@ __device__ __forceinline__ void eval_bead(const PoolArgs& a, const RepParams& P,
     }
     store_vec<DIM>(a.FT, s, acc);
     store_vec<DIM>(a.Tn, s, tang);
-}
-
-// One force evaluation of replica rsel (parity checks of the force path).
-template <int DIM, typename FT>
-__global__ void __launch_bounds__(BS) k_eval(PoolArgs a, int which, int rsel)
-{
-    const int r = a.blk_rep[blockIdx.x];
-    if (r != rsel) return;
-    const RepParams& P = a.P[r];
-    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
-    if (s >= P.bead_off + P.n) return;
-    eval_bead<DIM, FT>(a, P, s, which);
 }
 
 // ----------------------------------------------------------------- rebuild

@@ -19,7 +19,7 @@
 // cell, so a separation is (dcell * h) + (dx), accurate to the ulp of a cell
 // edge, and identical to the one the global cell scan forms.
 // Bonded terms, propulsion and the SRK2 stage update are the same device
-// functions as k_stage; outputs are one write per bead. A bead whose list
+// functions as the 3D tile kernels; outputs are one write per bead. A bead whose list
 // overflowed, and every bead of a tile that does not fit in LDS (or of a box
 // of fewer than 3 cells per side), sums its pairs from global memory with
 // pair_sum instead.
@@ -473,7 +473,7 @@ __device__ __forceinline__ void table_owned(const int* tt, int o, int& s, int& l
     }
 }
 
-// STAGE 0 / 1 as k_stage (STAGE 1 for SRK stage a.sub); STAGE 2: force
+// STAGE 0 / 1: the SRK force stages (STAGE 1 for SRK stage a.sub); STAGE 2: force
 // evaluation of replica `j` only at the positions X (forces into FT,
 // tangents into Tn; no noise, no update). NT threads per tile (256, 512 or
 // 1024): large tiles keep more waves in flight. GEN = false is the lean

@@ -119,10 +119,16 @@ def cases():
                                     cutoff_factor=W.LJ_25_SIGMA, T=0.01, Dr=0.1, steps=100,
                                     seed=9, skin=3.3)
     # 3D with a skin so wide that most beads have more list candidates than a
-    # 3D list holds (about 90 within the 3.3 cell edge against 64): they sum
-    # their pairs by the cell scan
-    out["list_overflow3d"] = W.chains("list_overflow3d", 3, 600, 1, L=10.0, T=0.01, Dr=0.1,
-                                      steps=100, seed=13, skin=2.1)
+    # 3D list holds (about 135 within rc + skin = 3.9 against 128): they sum
+    # their pairs by the cell scan, the others from LDS (one-cell tiles fit)
+    out["list_overflow3d"] = W.chains("list_overflow3d", 3, 1200, 1, L=13.0, T=0.01, Dr=0.1,
+                                      steps=100, seed=13, skin=2.9)
+    # 3D chains with the 400k-bead system's LJ law at volume fraction 0.15:
+    # about 7 beads per cell, so a 4-cell 3D tile with its halo (216 cells)
+    # overflows LDS and its beads take the cell scan, while 2- and 3-cell
+    # tiles fit (see test_tile_shape_does_not_change_results)
+    out["lj3d"] = W.chains("lj3d", 3, 400, 10, phi=0.15, cutoff_factor=W.LJ_25_SIGMA, kappa=5.0,
+                           T=0.05, dtmax=5e-4, steps=100, seed=14, skin=0.67)
     # about 80 beads per cell: a one-cell tile with its halo fits in LDS, a
     # three-cell one does not (see test_tile_shape_does_not_change_results)
     out["dense_cells"] = W.chains("dense_cells", 2, 1300, 1, phi=0.55,
@@ -323,21 +329,25 @@ def test_alone_vs_mixed_pool_bit_identical():
 @pytest.mark.parametrize("precision", [E.GCR_FP64, E.GCR_MIXED])
 @pytest.mark.parametrize("name,sides", [("chains_lj_cut", ((1, 256), (2, 512), (5, 1024), (16, 512))),
                                         ("dense_cells", ((1, 256), (3, 512))),
-                                        ("list_overflow", ((1, 512), (4, 256)))])
+                                        ("list_overflow", ((1, 512), (4, 256))),
+                                        ("chains3d", ((1, 256), (2, 512), (6, 1024))),
+                                        ("lj3d", ((2, 256), (3, 512), (4, 256))),
+                                        ("list_overflow3d", ((1, 256), (3, 512)))])
 def test_tile_shape_does_not_change_results(name, sides, precision):
     """Separations are formed from cell-relative positions in every pair path,
     and lists keep the cell-scan order, so the tile edge and the threads per
     tile (and with them whether
     a bead's pairs come from its LDS list or from the global cell scan: in
-    dense_cells a 3-cell tile does not fit in LDS) leaves every bit of the
-    trajectory unchanged."""
+    dense_cells a 3-cell tile does not fit in LDS, in lj3d a 4-cell 3D tile)
+    leaves every bit of the trajectory unchanged."""
     w = CASES[name]
     print(f"[tile shapes] {name} sides={sides} precision={precision}", flush=True)
     tv = w.time_vals.copy()
     tv[7] = 50
     outs = []
     for side, threads in sides:
-        p = E.Pool(2, precision, bead_capacity=w.n, cell_capacity=4 * w.n + 1024,
+        p = E.Pool(w.phys.dim, precision, bead_capacity=w.n,
+                   cell_capacity=E.replica_cells(w.phys) + 1,
                    chain_capacity=w.phys.n_chains, replica_capacity=1)
         p.set_schedule(16, 8)
         p.set_tile(side)
