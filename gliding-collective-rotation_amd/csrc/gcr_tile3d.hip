@@ -71,26 +71,40 @@ __device__ __forceinline__ void separation3(const LPos3<FT>& q, const LPos3<FT>&
 
 // Pair forces of one bead over its list of LDS indices (groups of 4 u16 at
 // G[g * stride], the last group padded with the bead's own index, which the
-// zero-distance mask drops), summed in FT.
+// zero-distance mask drops), summed in FT. The groups are read from global
+// memory four at a time, the next four in flight while the current ones are
+// summed: a 3D launch keeps only two to four waves per SIMD, too few to hide
+// one load's latency behind another wave's pairs.
 template <typename FT>
 __device__ __forceinline__ void list3_pair_sum(const LPos3<FT>* lp, const uint2* G, int64_t stride, int len,
                                                const LPos3<FT>& me, const PairConsts<FT>& k, FT* f)
 {
     const int ng = (len + 3) >> 2;
-    uint2 e = ng > 0 ? G[0] : make_uint2(0u, 0u);
-    for (int g = 0; g < ng; ++g) {
-        const uint2 n = g + 1 < ng ? G[(int64_t)(g + 1) * stride] : e;  // next group in flight
-        const LPos3<FT> q[4] = {lp[e.x & 0xffffu], lp[e.x >> 16], lp[e.y & 0xffffu], lp[e.y >> 16]};
+    const uint2 none = make_uint2(0u, 0u);
+    uint2 e[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            FT sep[3];
-            separation3<FT>(q[u], me, k.h, sep);
-            if (k.shifted)
-                pair_term_shifted<3, FT>(sep, k, f);
-            else
-                pair_term<3, FT>(sep, k, f);
+    for (int u = 0; u < 4; ++u) e[u] = u < ng ? G[(int64_t)u * stride] : none;
+    for (int g0 = 0; g0 < ng; g0 += 4) {
+        uint2 n[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) n[u] = g0 + 4 + u < ng ? G[(int64_t)(g0 + 4 + u) * stride] : none;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            if (g0 + v >= ng) break;
+            const uint2 ev = e[v];
+            const LPos3<FT> q[4] = {lp[ev.x & 0xffffu], lp[ev.x >> 16], lp[ev.y & 0xffffu], lp[ev.y >> 16]};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                FT sep[3];
+                separation3<FT>(q[u], me, k.h, sep);
+                if (k.shifted)
+                    pair_term_shifted<3, FT>(sep, k, f);
+                else
+                    pair_term<3, FT>(sep, k, f);
+            }
         }
-        e = n;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = n[u];
     }
 }
 
