@@ -80,6 +80,10 @@ def parse():
     ap.add_argument("--no-serving", action="store_true",
                     help="single: leave the serving part out of the line")
     ap.add_argument("--budget-gb", type=float, default=16.0)
+    ap.add_argument("--rate", type=float, default=20.0,
+                    help="serving: mean request arrivals per second (Poisson)")
+    ap.add_argument("--serve-rebuild", type=int, default=16,
+                    help="serving: steps between cell re-sorts of the server's pools")
     ap.add_argument("--only", default="",
                     help="features mode: run only the options whose name contains this")
     return ap.parse_args()
@@ -247,16 +251,20 @@ def main_ensemble(args):
 
 
 def serving_part(args):
-    """A Poisson stream (20 requests/s) of `requests` simulations through the
-    continuous-batching server under an HBM budget: bead counts log-U[1k,
-    200k], step counts log-U[1e3, 1e5] times steps_scale, ABPs (D_r = 0.1) and
-    chains of 5-40 beads, 2D and 3D, mixed precision."""
+    """A Poisson stream (`rate` requests/s, 20 by default) of `requests`
+    simulations through the continuous-batching server under an HBM budget:
+    bead counts log-U[1k, 200k], step counts log-U[1e3, 1e5] times
+    steps_scale, ABPs (D_r = 0.1) and chains of 5-40 beads, 2D and 3D, mixed
+    precision. At 20 requests/s the stream offers about ten times the work
+    one GPU does, so the latencies are those of a growing queue; `offered`
+    against `value` says how loaded the server was."""
     from gcr_amd import serving as SV
     t0 = time.perf_counter()
-    reqs = SV.make_stream(seed=0, n_requests=args.requests, scale_steps=args.steps_scale)
+    reqs = SV.make_stream(seed=0, n_requests=args.requests, rate=args.rate,
+                          scale_steps=args.steps_scale)
     t_init = time.perf_counter() - t0
     srv = SV.Server(hbm_budget_bytes=int(args.budget_gb * (1 << 30)),
-                    steps_per_batch=args.batch, rebuild_every=8)
+                    steps_per_batch=args.batch, rebuild_every=args.serve_rebuild)
     try:
         done = srv.serve(reqs)
         graphs = {f"{d}d": lane.pool.graph_stats() for d, lane in srv.lanes.items()}
@@ -268,10 +276,11 @@ def serving_part(args):
     return {"metric": "serving aggregate bead-steps/sec + p50/p99 completion latency (1 GPU)",
             "value": rep["bead_steps_per_s"], "unit": "bead-steps/s",
             "dtype": "f32 forces + f64 state",
-            "config": {"workload": "Poisson stream, 20 requests/s; beads log-U[1k,200k]; steps "
-                                   "log-U[1e3,1e5] x steps_scale; ABPs (D_r=0.1) and chains "
+            "config": {"workload": f"Poisson stream, {args.rate:g} requests/s; beads log-U[1k,200k]; "
+                                   "steps log-U[1e3,1e5] x steps_scale; ABPs (D_r=0.1) and chains "
                                    "5-40, 2D and 3D; mixed precision",
-                       "requests": len(reqs), "steps_scale": args.steps_scale,
+                       "requests": len(reqs), "rate_per_s": args.rate,
+                       "steps_scale": args.steps_scale, "rebuild_every": args.serve_rebuild,
                        "hbm_budget_gb": args.budget_gb, "total_bead_steps": work},
             "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
             "mean_queue_s": rep["mean_queue_s"], "makespan_s": rep["makespan_s"],
