@@ -248,12 +248,16 @@ int gcr_pool_time_kernels(gcr_pool* p, int32_t steps, double ms_out[4]);
  * cost of admitting and retiring replicas): out = [captures, instantiations,
  * in-place updates of an executable graph, host seconds spent on them]. */
 int gcr_pool_graph_stats(gcr_pool* p, double out[4]);
-/* Tile edge (cells) of the 2D LDS force kernel; 0 picks it per replica from
- * the cell occupancy (about 512 owned beads per workgroup, the tile with its
- * halo within 70% of the LDS capacity at the mean occupancy). At most 16. */
+/* Tile edge (cells) of the LDS force kernels; 0 picks it per replica from
+ * the cell occupancy. 2D: square tiles of about 512 owned beads, the tile
+ * with its halo within 70% of the LDS capacity at the mean occupancy; at
+ * most 16. 3D: cubes, the largest whose halo holds at most 70% of the 3D
+ * tile's LDS capacity at the mean occupancy; a side set here is clamped to 6
+ * and to the grid. No result bit depends on the tiling. */
 int gcr_pool_set_tile(gcr_pool* p, int32_t side);
-/* Threads per tile workgroup of the 2D force kernels: 256, 512, 1024, or 0 to
- * pick about one owned bead per thread. Neither this nor the tile
+/* Threads per tile workgroup of the force kernels: 256, 512, 1024, or 0 to
+ * pick about one owned bead per thread in 2D, and 512 (FP32) or 256 (FP64)
+ * threads per 3D tile. Neither this nor the tile
  * edge changes any result bit: the pair sums keep one order and one
  * arithmetic whatever the tiling. */
 int gcr_pool_set_tile_threads(gcr_pool* p, int32_t threads);
