@@ -223,6 +223,13 @@ def ensemble_part(args):
                                    rebuild_every=tuned["rebuild_every"], tile=tuned["tile"],
                                    threads=tuned["threads"])
     value = bs / sec
+    # what one persistent launch per batch would pay instead of the graph's
+    # back-to-back launches: a grid-wide barrier at each boundary between the
+    # step's kernels (stage 0 | stage 1 | max force and dt | update: 3 per
+    # step), at the residency of the stage kernels (four 256-thread tiles
+    # per CU)
+    from gcr_amd import _lib
+    bar_us, bar_blocks = _lib.grid_barrier_us(4, 200)
     return {"metric": "ensemble aggregate bead-steps/sec (256 uneven replicas, 1 GPU)",
             "value": value, "unit": "bead-steps/s", "steps": args.ens_steps,
             "ms_per_step": 1e3 * sec / args.ens_steps, "seconds": sec,
@@ -239,6 +246,13 @@ def ensemble_part(args):
             "roofline": {"bound": "hbm", "achieved": 80 * value / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": 80 * value / 1e9 / HBM_PEAK_GBS,
                          "note": "80 algorithmic bytes per 2D chain bead-step"},
+            "persistent_launch": {"grid_barrier_us": bar_us, "workgroups": bar_blocks,
+                                  "barriers_per_step": 3,
+                                  "barrier_us_per_step": 3 * bar_us,
+                                  "graph_ms_per_step": 1e3 * sec / args.ens_steps,
+                                  "note": "a persistent kernel would replace the graph's "
+                                          "back-to-back launches (no gap between them in the "
+                                          "kernel trace) by these barriers"},
             "replays": replays, "host_init_s": t_init}
 
 

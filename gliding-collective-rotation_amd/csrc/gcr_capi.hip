@@ -5,6 +5,7 @@
 #include "gcr_tile2d.hip"
 #include "gcr_tile3d.hip"
 
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_ext.h>
 
 #include <algorithm>
@@ -979,6 +980,58 @@ int upload_perm(gcr_pool* p, int rid, const double* host, double* dst)
 extern "C" {
 
 const char* gcr_last_error(void) { return g_err.c_str(); }
+
+// Grid-wide barriers of a cooperative launch, `rounds` of them back to back.
+__global__ void __launch_bounds__(BS) k_grid_barrier_probe(int rounds, int* sink)
+{
+    cooperative_groups::grid_group g = cooperative_groups::this_grid();
+    for (int i = 0; i < rounds; ++i) g.sync();
+    if (blockIdx.x == 0 && threadIdx.x == 0) sink[0] = rounds;
+}
+
+int gcr_grid_barrier_us(int32_t blocks_per_cu, int32_t rounds, double out[2])
+{
+    if (!out || blocks_per_cu < 1 || rounds < 1) return fail(GCR_ERR_ARG, "need blocks_per_cu >= 1, rounds >= 1");
+    int dev = 0;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (!prop.cooperativeLaunch) return fail(GCR_ERR_STATE, "the device has no cooperative launch");
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_grid_barrier_probe, BS, 0));
+    const int blocks = std::min(blocks_per_cu, per_cu) * prop.multiProcessorCount;
+    int* sink = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GCR_OK;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && rc == GCR_OK) rc = fail(GCR_ERR_HIP, hipGetErrorString(e));
+        return rc == GCR_OK;
+    };
+    float ms[2] = {0.f, 0.f};
+    if (ok(hipMalloc((void**)&sink, sizeof(int))) && ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
+        ok(hipEventCreate(&e0)) && ok(hipEventCreate(&e1))) {
+        // a warm-up launch, then the launch without barriers and with `rounds`
+        const int counts[3] = {rounds, 0, rounds};
+        for (int k = 0; k < 3 && rc == GCR_OK; ++k) {
+            int n = counts[k];
+            void* args[] = {&n, &sink};
+            if (ok(hipEventRecord(e0, st)) &&
+                ok(hipLaunchCooperativeKernel((const void*)k_grid_barrier_probe, dim3(blocks), dim3(BS), args, 0,
+                                              st)) &&
+                ok(hipEventRecord(e1, st)) && ok(hipEventSynchronize(e1)) && k > 0)
+                ok(hipEventElapsedTime(&ms[k - 1], e0, e1));
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (sink) (void)hipFree(sink);
+    if (rc) return rc;
+    out[0] = 1e3 * (double)(ms[1] - ms[0]) / rounds;
+    out[1] = (double)blocks;
+    return GCR_OK;
+}
 
 int gcr_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_bytes)
 {

@@ -64,7 +64,7 @@ class ReplicaDesc(C.Structure):
 
 
 EXPORTS = [
-    "gcr_last_error", "gcr_device_info", "gcr_create", "gcr_upload", "gcr_run", "gcr_download",
+    "gcr_last_error", "gcr_device_info", "gcr_grid_barrier_us", "gcr_create", "gcr_upload", "gcr_run", "gcr_download",
     "gcr_eval_forces", "gcr_destroy", "gcr_pool_create", "gcr_pool_set_schedule",
     "gcr_replica_bytes", "gcr_replica_cells", "gcr_replica_add", "gcr_replica_remove",
     "gcr_replica_upload", "gcr_replica_begin", "gcr_pool_run", "gcr_replica_status",
@@ -95,6 +95,7 @@ def load():
     L.gcr_last_error.restype = C.c_char_p
     L.gcr_last_error.argtypes = []
     L.gcr_device_info.argtypes = [C.c_char_p, i32, C.POINTER(i32), C.POINTER(i64)]
+    L.gcr_grid_barrier_us.argtypes = [i32, i32, C.POINTER(C.c_double)]
     L.gcr_pool_create.argtypes = [i32, i32, i64, i64, i64, i32, C.POINTER(vp)]
     L.gcr_pool_set_schedule.argtypes = [vp, i32, i32]
     L.gcr_replica_bytes.restype = i64
@@ -164,3 +165,13 @@ def device_info():
     mem = C.c_int64()
     check(L.gcr_device_info(name, 256, C.byref(ncu), C.byref(mem)))
     return name.value.decode(), ncu.value, mem.value
+
+
+def grid_barrier_us(blocks_per_cu=4, rounds=200):
+    """Microseconds per grid-wide barrier of a cooperative launch with
+    blocks_per_cu workgroups of 256 threads per CU (gcr_grid_barrier_us), and
+    the workgroups it used."""
+    L = load()
+    out = (C.c_double * 2)()
+    check(L.gcr_grid_barrier_us(int(blocks_per_cu), int(rounds), out))
+    return float(out[0]), int(out[1])

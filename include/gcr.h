@@ -83,6 +83,13 @@ typedef struct gcr_replica_desc {
 const char* gcr_last_error(void);
 /* Device name, CU count and HBM bytes of the current device. */
 int gcr_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_bytes);
+/* No counterpart in the reference: the cost of one grid-wide barrier on this
+ * device, measured with a cooperative launch of blocks_per_cu workgroups of
+ * 256 threads per CU (fewer if they cannot all be resident) running `rounds`
+ * barriers back to back: out = [microseconds per barrier, workgroups]. The
+ * price a persistent step kernel would pay at each boundary between its
+ * stages, against the batch graphs' back-to-back launches. */
+int gcr_grid_barrier_us(int32_t blocks_per_cu, int32_t rounds, double out[2]);
 
 /* ------------------------------------------------------ single system API */
 
