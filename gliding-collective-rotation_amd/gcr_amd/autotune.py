@@ -27,7 +27,8 @@ def _pool_for(w, precision, skin):
     phys = copy.copy(w.phys)
     phys.skin = skin
     n = w.n
-    pool = E.Pool(phys.dim, precision, bead_capacity=n, cell_capacity=4 * n + 1024,
+    cells = max(4 * n + 1024, E.replica_cells(phys) + 1)
+    pool = E.Pool(phys.dim, precision, bead_capacity=n, cell_capacity=cells,
                   chain_capacity=phys.n_chains, replica_capacity=1)
     rid = pool.add(phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
                    getattr(w, "v0_per_chain", None))
@@ -93,17 +94,19 @@ def time_schedule(w, precision, skin, rebuild_every, tile, threads, steps=256, b
 
 
 def tune(w, precision=E.GCR_MIXED, skin_fracs=(0.1, 0.15, 0.2, 0.3), rebuilds=(8, 16, 32),
-         tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512, 1024), steps=256):
+         tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512, 1024), steps=256, tiles3=(0, 2, 3, 4),
+         threads3=(256, 512)):
     """Best (skin, rebuild_every, tile, threads) for workload w.
 
     The tiling is tuned first at the default skin (0.3 cutoff), then every
     (skin, period) pair is timed with that tiling, and the tiling is re-tuned
-    at the chosen skin. skin_fracs are fractions of the pair cutoff."""
+    at the chosen skin. skin_fracs are fractions of the pair cutoff; tiles
+    and threads are the 2D candidates, tiles3 and threads3 the 3D ones (cube
+    sides; 0 is the automatic choice)."""
     rc = float(w.phys.wca[3])
-    dim = w.phys.dim
-    tiling = [(0, 0, 0.0)]
-    if dim == 2:
-        tiling = tune_tiling(w, precision, 0.3 * rc, tiles, threads)
+    if w.phys.dim == 3:
+        tiles, threads = tiles3, threads3
+    tiling = tune_tiling(w, precision, 0.3 * rc, tiles, threads)
     tile, nt = tiling[0][0], tiling[0][1]
     trials = []
     for f in skin_fracs:
@@ -112,9 +115,8 @@ def tune(w, precision=E.GCR_MIXED, skin_fracs=(0.1, 0.15, 0.2, 0.3), rebuilds=(8
             if not failed:
                 trials.append(dict(skin=f * rc, rebuild_every=k, ms_per_step=ms, replays=replays))
     best = min(trials, key=lambda r: r["ms_per_step"])
-    if dim == 2:
-        tiling = tune_tiling(w, precision, best["skin"], tiles, threads)
-        tile, nt = tiling[0][0], tiling[0][1]
+    tiling = tune_tiling(w, precision, best["skin"], tiles, threads)
+    tile, nt = tiling[0][0], tiling[0][1]
     return dict(skin=best["skin"], rebuild_every=best["rebuild_every"], tile=tile, threads=nt,
                 ms_per_step=best["ms_per_step"], schedule_trials=trials,
                 tiling_trials=[dict(tile=t, threads=n, stages_ms=m) for t, n, m in tiling])

@@ -70,6 +70,17 @@ def test_autotune_picks_a_measured_setting():
     assert r["ms_per_step"] == min(t["ms_per_step"] for t in r["schedule_trials"])
 
 
+def test_autotune_tunes_3d_tiles():
+    from gcr_amd import autotune as AT
+    w = W.chains("tune3d", 3, 400, 10, phi=0.15, cutoff_factor=W.LJ_25_SIGMA, kappa=5.0,
+                 T=0.05, dtmax=5e-4, steps=100, seed=6)
+    r = AT.tune(w, E.GCR_MIXED, skin_fracs=(0.3,), rebuilds=(16,), tiles3=(0, 2, 3),
+                threads3=(256, 512), steps=64)
+    print("autotune 3D:", {k: r[k] for k in ("tile", "threads", "ms_per_step")}, flush=True)
+    assert r["tile"] in (0, 2, 3) and r["threads"] in (256, 512)
+    assert len(r["tiling_trials"]) == 6
+
+
 @pytest.mark.parametrize("kind", ["chains2d", "abp2d", "chains3d"])
 def test_device_order_parameters_match_host(kind):
     """gcr_replica_order against the host observables on the downloaded
