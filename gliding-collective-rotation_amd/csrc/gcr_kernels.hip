@@ -129,24 +129,27 @@ __device__ __forceinline__ void bead_noise(uint64_t seed, uint32_t orig, int64_t
 // Noise of bead `orig_global` for the replica's step with call-local index
 // `steps` and global index `step_global`: drawn in-kernel (keyed by the
 // original bead index, so it does not depend on the slot layout), or read
-// from the injected array (zeros past its end: that step never runs).
+// from the injected array (zeros past its end: that step never runs). The
+// draw runs in both cases and the injected values replace it, so that no
+// branch holding loads precedes the draw's arithmetic (the waits on memory
+// after such a join would wait for every load in flight).
 template <int DIM>
 __device__ __forceinline__ void draw_noise(const RepParams& P, int orig_global, int64_t steps,
                                            int64_t step_global, double* xi)
 {
     const int orig_local = orig_global - P.bead_off;
+    bead_noise<DIM>(P.seed, (uint32_t)orig_local, step_global, (P.flags & GCR_F_ROTDIFF) != 0, xi);
     if (P.flags & GCR_F_INJECT) {
         if (steps < P.xi_steps) {
-            const double* src = P.xi_inject + ((size_t)steps * P.n + orig_local) * (2 * DIM);
+            // a global-memory load (a flat one would count against both waits)
+            typedef const __attribute__((address_space(1))) double* gptr;
+            const gptr src = (gptr)(P.xi_inject + ((size_t)steps * P.n + orig_local) * (2 * DIM));
 #pragma unroll
             for (int i = 0; i < 2 * DIM; ++i) xi[i] = src[i];
         } else {
 #pragma unroll
             for (int i = 0; i < 2 * DIM; ++i) xi[i] = 0.0;
         }
-    } else {
-        bead_noise<DIM>(P.seed, (uint32_t)orig_local, step_global, (P.flags & GCR_F_ROTDIFF) != 0,
-                        xi);
     }
 }
 
@@ -807,9 +810,8 @@ template <int DIM>
 __device__ __forceinline__ void virtual_update_in(const PoolArgs& a, const RepParams& P, RepState* S,
                                                   int s, const double* x, const double* f,
                                                   double resc, int sub, const double* xi,
-                                                  const double* xr)
+                                                  const double* xr, double dt)
 {
-    const double dt = S->dt;
     double dt_eff = dt * resc * 1.0;
     const double s2 = sqrt(2.0 * dt * P.D);
     dt_eff /= P.radius;
@@ -834,7 +836,7 @@ __device__ __forceinline__ void virtual_update(const PoolArgs& a, const RepParam
     double xi[DIM], xr[DIM];
     load_vec<DIM>(a.XI, s, xi);
     load_vec<DIM>(a.XR, s, xr);
-    virtual_update_in<DIM>(a, P, S, s, x, f, resc, sub, xi, xr);
+    virtual_update_in<DIM>(a, P, S, s, x, f, resc, sub, xi, xr, S->dt);
 }
 
 // WCA walls normal to the axes in P.walls (forces.py:405-435), `wall_zone`
@@ -1362,25 +1364,44 @@ __global__ void k_group_dt(PoolArgs a, int j)
 }
 
 // Corrector update (SRK.py:355-388) and rotational diffusion (SRK.py:398-423).
+// The replica's parameters and step state are read before the first store
+// (after one the compiler could not keep them in scalar registers: every use
+// would be a vector load waiting on memory), and every input of the bead is
+// in flight at once, (orig, chain) first, so that the next step's noise is
+// drawn while the rest arrives.
 template <int DIM>
 __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
-    const RepParams& P = a.P[r];
+    const RepParams P = a.P[r];
     const RepState* S = a.S + r;
     if (S->frozen || S->steps != S->batch_base + j + 1) return;
     const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (s >= P.bead_off + P.n) return;
     const double dt_used = S->dt_used;
-    double dt_eff = dt_used * force_prefactor(P);
-    const double s2 = sqrt(2.0 * dt_used * P.D);
-    dt_eff /= P.radius;
-    double x[DIM], f[DIM], xi[2 * DIM];
-    const bool still = (P.flags & GCR_F_MSTILL) && a.WA[s] >= 0;  // a bound motor
-    if (!(P.flags & GCR_F_ROD) && !still) {  // rod replicas: the corrector ran in k_rod
-        load_vec<DIM>(a.X, s, x);
-        load_vec<DIM>(a.FT, s, f);
-        load_vec<DIM>(a.XI, s, xi);
+    const int64_t steps = S->steps, step_global = S->step_global;
+    const bool rot = (P.flags & GCR_F_ROTDIFF) != 0;
+    // rod replicas: the corrector ran in k_rod; a bound motor stays put
+    const bool moves = !(P.flags & GCR_F_ROD) && !((P.flags & GCR_F_MSTILL) && a.WA[s] >= 0);
+    // loaded whether or not the bead moves (in bounds either way), so that
+    // no branch joins behind them and the noise waits for (orig, chain) only
+    const int orig = a.META[2 * s];
+    double x[DIM], f[DIM], xi[2 * DIM], xr[DIM], t[DIM];
+    load_vec<DIM>(a.X, s, x);
+    load_vec<DIM>(a.FT, s, f);
+    load_vec<DIM>(a.XI, s, xi);
+    load_vec<DIM>(a.XR, s, xr);
+    // the next step's noise (finalize already advanced the step counters)
+    double xin[2 * DIM];
+    draw_noise<DIM>(P, orig, steps, step_global, xin);
+    if (rot) {
+        load_vec<DIM>(a.T, s, t);
+        load_vec<DIM>(a.XIR, s, xi + DIM);
+    }
+    if (moves) {
+        double dt_eff = dt_used * force_prefactor(P);
+        const double s2 = sqrt(2.0 * dt_used * P.D);
+        dt_eff /= P.radius;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
             double tmp = x[i] + dt_eff * f[i];
@@ -1389,15 +1410,13 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
         }
         store_vec<DIM>(a.X, s, x);
         // outside the safe batch a replay, inside it a re-sort before the next step
-        if (moved_too_far<DIM>(a, P, s, x)) atomicOr(a.safe ? &a.S[r].sort_due : &a.S[r].violation, 1);
+        if (displaced_too_far<DIM>(P, xr, x)) atomicOr(a.safe ? &a.S[r].sort_due : &a.S[r].violation, 1);
     }
     // the step is over: the next mid-step re-sort waits for a new predictor
     if (threadIdx.x == 0 && blockIdx.x == P.blk_off) a.S[r].pred_far = 0;
-    if (P.flags & GCR_F_ROTDIFF) {
+    if (rot) {
         const double sr = sqrt(2.0 * dt_used * P.Dr) * 3.141592653589793;
-        double t[DIM], nr[DIM], proj = 0.0;
-        load_vec<DIM>(a.T, s, t);
-        load_vec<DIM>(a.XIR, s, xi + DIM);
+        double nr[DIM], proj = 0.0;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) nr[i] = xi[DIM + i];
 #pragma unroll
@@ -1415,9 +1434,8 @@ __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
         for (int i = 0; i < DIM; ++i) t[i] /= mag;
         store_vec<DIM>(a.T, s, t);
     }
-    // the next step's noise (finalize already advanced the step counters),
-    // drawn here where the ALUs would otherwise wait on memory
-    store_noise<DIM>(a, P, s, S->steps, S->step_global);
+    store_vec<DIM>(a.XI, s, xin);
+    if (rot) store_vec<DIM>(a.XIR, s, xin + DIM);
 }
 
 // One force evaluation of the bead in slot s without an update: forces at X

@@ -492,23 +492,37 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
     const int r = tile.x;
-    const RepParams& P = a.P[r];
     RepState* S = a.S + r;
     if (STAGE == 2) {
         if (r != j) return;
     } else if (!step_runs(S, j)) {
         return;
     }
-    if (STAGE == 1 && !later_stage_applies(P, a.sub)) return;
-    const double* Xa = STAGE == 1 ? a.X1 : a.X;
-    const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
+    if (STAGE == 1 && !later_stage_applies(a.P[r], a.sub)) return;
+    // The replica's parameters and the predictor's dt live in LDS for the
+    // whole kernel: read from global memory after the kernel's first store,
+    // which could alias them, each would be a vector load waiting on memory.
+    constexpr int PW = (int)(sizeof(RepParams) / 4);
+    static_assert(sizeof(RepParams) % 4 == 0 && TT_INTS + PW + 1 <= NT, "one word per thread");
+    __shared__ RepParams Ps;
+    __shared__ double dt_s;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
     __shared__ int tt[TT_INTS];
-    static_assert(TT_INTS <= NT, "one table entry per thread");
-    if ((int)threadIdx.x < TT_INTS) tt[threadIdx.x] = a.TT[(size_t)blockIdx.x * TT_INTS + threadIdx.x];
+    {
+        const int tid = (int)threadIdx.x;
+        if (tid < TT_INTS)
+            tt[tid] = a.TT[(size_t)blockIdx.x * TT_INTS + tid];
+        else if (tid < TT_INTS + PW)
+            reinterpret_cast<uint32_t*>(&Ps)[tid - TT_INTS] = reinterpret_cast<const uint32_t*>(a.P + r)[tid - TT_INTS];
+        else if (tid == TT_INTS + PW)
+            dt_s = S->dt;
+    }
     __syncthreads();
+    const RepParams& P = Ps;
+    const double* Xa = STAGE == 1 ? a.X1 : a.X;
+    const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
     const bool fast = tt[0] != 0;
     constexpr bool early = sizeof(FT) == 4;  // the FP64 kernels have no registers to spare
     TileLayout t;
@@ -590,6 +604,9 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
     double fsq = 0.0;
     int ry = 0;
     for (int o = threadIdx.x; o < n_own; o += NT) {
+        // each bead reads the parameters it needs from LDS again rather than
+        // holding them in registers across the loop
+        asm volatile("" ::: "memory");
         int li, s;
         BeadHead bd;
         if (early && o == (int)threadIdx.x) {  // heads loaded in the prologue
@@ -676,7 +693,7 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
             }
             if (!GEN && STAGE == 0) {  // stage_store's lean stage 0 with the preloaded noise
                 store_vec<2>(a.F0, s, acc);
-                virtual_update_in<2>(a, P, S, s, xs, acc, 1.0, 0, br.pre, br.xr);
+                virtual_update_in<2>(a, P, S, s, xs, acc, 1.0, 0, br.pre, br.xr, dt_s);
             } else {
                 const double q = stage_store<2, STAGE, GEN>(a, P, S, s, me.x, xs, acc);
                 fsq = q > fsq ? q : fsq;
