@@ -1630,44 +1630,43 @@ __global__ void __launch_bounds__(BS) k_gather(PoolArgs a, int with_t, int j)
     if (!rebuild_runs(a, r, j)) return;
     const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (ns >= P.bead_off + P.n) return;
+    // everything this kernel reads, loaded before its first store (a store
+    // could alias the replica's parameters, and every later read of them
+    // would wait on memory)
+    const uint32_t flags = P.flags;
+    const bool rot = (flags & GCR_F_ROTDIFF) != 0, srk4 = (flags & GCR_F_SRK4) != 0;
+    const bool wa = a.on_x1 && (flags & (GCR_F_SLIDE | GCR_F_GROUP | GCR_F_MOTOR));
     const int2 pr = *reinterpret_cast<const int2*>(a.PAIRS + 2 * ns);
     const int so = pr.y;
+    double x[DIM], t[DIM], xi[DIM], xir[DIM], x1[DIM], fl[DIM];
+    load_vec<DIM>(a.X, so, x);
+    if (with_t) load_vec<DIM>(a.T, so, t);
+    const int2 meta = *reinterpret_cast<const int2*>(a.META + 2 * so);
+    // the pending noise and the last predictor travel with the bead
+    load_vec<DIM>(a.XI, so, xi);
+    if (rot) load_vec<DIM>(a.XIR, so, xir);
+    load_vec<DIM>(a.X1, so, x1);
+    // mid-step: one force array is live and the other free -- SRK2's stage-0
+    // forces (FT is written by the last stage), SRK4's running total (F0
+    // holds the forces of a stage already applied); the live one travels
+    // through the free one
+    if (a.on_x1) load_vec<DIM>(srk4 ? a.FT : a.F0, so, fl);
+    const int wav = wa ? a.WA[so] : 0;  // where_active so far, or a motor's bound bead
     const int key = a.KEYOFF[2 * so];
     const int t0 = a.CSTART[P.cell_off + key], t1 = a.CSTART[P.cell_off + key + 1];
     int rank = 0;
-    for (int t = t0; t < t1; ++t) rank += a.PAIRS[2 * t] < pr.x;
+    for (int q = t0; q < t1; ++q) rank += a.PAIRS[2 * q] < pr.x;
     const int d = t0 + rank;
-    double x[DIM];
-    load_vec<DIM>(a.X, so, x);
     store_vec<DIM>(a.Xn, d, x);
-    if (with_t) {
-        double t[DIM];
-        load_vec<DIM>(a.T, so, t);
-        store_vec<DIM>(a.Tn, d, t);
-    }
-    *reinterpret_cast<int2*>(a.METAn + 2 * d) = *reinterpret_cast<const int2*>(a.META + 2 * so);
+    if (with_t) store_vec<DIM>(a.Tn, d, t);
+    *reinterpret_cast<int2*>(a.METAn + 2 * d) = meta;
     a.CELL[d] = key;
     a.SLOT[pr.x] = d;
-    // the pending noise and the last predictor travel with the bead
-    load_vec<DIM>(a.XI, so, x);
-    store_vec<DIM>(a.XIn, d, x);
-    if (P.flags & GCR_F_ROTDIFF) {
-        load_vec<DIM>(a.XIR, so, x);
-        store_vec<DIM>(a.XIRn, d, x);
-    }
-    load_vec<DIM>(a.X1, so, x);
-    store_vec<DIM>(a.X1n, d, x);
-    if (a.on_x1) {
-        // mid-step: one force array is live and the other free -- SRK2's
-        // stage-0 forces (FT is written by the last stage), SRK4's running
-        // total (F0 holds the forces of a stage already applied); the live
-        // one travels through the free one
-        const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
-        load_vec<DIM>(srk4 ? a.FT : a.F0, so, x);
-        store_vec<DIM>(srk4 ? a.F0 : a.FT, d, x);
-        if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP | GCR_F_MOTOR))
-            a.WAn[d] = a.WA[so];  // where_active so far, or a motor's bound bead
-    }
+    store_vec<DIM>(a.XIn, d, xi);
+    if (rot) store_vec<DIM>(a.XIRn, d, xir);
+    store_vec<DIM>(a.X1n, d, x1);
+    if (a.on_x1) store_vec<DIM>(srk4 ? a.F0 : a.FT, d, fl);
+    if (wa) a.WAn[d] = wav;
 }
 
 template <int DIM>
@@ -1678,39 +1677,38 @@ __global__ void __launch_bounds__(BS) k_links(PoolArgs a, int with_t, int j)
     if (!rebuild_runs(a, r, j)) return;
     const int ns = a.blk_slot0[blockIdx.x] + threadIdx.x;
     if (ns >= P.bead_off + P.n) return;
-    int2 me = *reinterpret_cast<const int2*>(a.METAn + 2 * ns);
-    int first = a.CHAIN_FIRST[me.y], len = a.CHAIN_LEN[me.y];
-    int kk = me.x - first;
+    // every read before the first store (as in k_gather)
+    const uint32_t flags = P.flags;
+    const bool rot = (flags & GCR_F_ROTDIFF) != 0, srk4 = (flags & GCR_F_SRK4) != 0;
+    const bool wa = a.on_x1 && (flags & (GCR_F_SLIDE | GCR_F_GROUP | GCR_F_MOTOR));
+    const int2 me = *reinterpret_cast<const int2*>(a.METAn + 2 * ns);
+    double x[DIM], t[DIM], xi[DIM], xir[DIM], x1[DIM], fl[DIM];
+    load_vec<DIM>(a.Xn, ns, x);
+    if (with_t) load_vec<DIM>(a.Tn, ns, t);
+    load_vec<DIM>(a.XIn, ns, xi);
+    if (rot) load_vec<DIM>(a.XIRn, ns, xir);
+    load_vec<DIM>(a.X1n, ns, x1);
+    if (a.on_x1) load_vec<DIM>(srk4 ? a.F0 : a.FT, ns, fl);
+    const int wav = wa ? a.WAn[ns] : 0;
+    const int first = a.CHAIN_FIRST[me.y], len = a.CHAIN_LEN[me.y];
+    const int kk = me.x - first;
     int4 lk;
     lk.x = kk >= 2 ? a.SLOT[me.x - 2] : -1;
     lk.y = kk >= 1 ? a.SLOT[me.x - 1] : -1;
     lk.z = kk + 1 < len ? a.SLOT[me.x + 1] : -1;
     lk.w = kk + 2 < len ? a.SLOT[me.x + 2] : -1;
     *reinterpret_cast<int4*>(a.LINK + 4 * ns) = lk;
-    double x[DIM];
-    load_vec<DIM>(a.Xn, ns, x);
     store_vec<DIM>(a.X, ns, x);
     if (!a.on_x1) store_vec<DIM>(a.XR, ns, x);
-    if (with_t) {
-        double t[DIM];
-        load_vec<DIM>(a.Tn, ns, t);
-        store_vec<DIM>(a.T, ns, t);
-    }
+    if (with_t) store_vec<DIM>(a.T, ns, t);
     *reinterpret_cast<int2*>(a.META + 2 * ns) = me;
-    load_vec<DIM>(a.XIn, ns, x);
-    store_vec<DIM>(a.XI, ns, x);
-    if (P.flags & GCR_F_ROTDIFF) {
-        load_vec<DIM>(a.XIRn, ns, x);
-        store_vec<DIM>(a.XIR, ns, x);
-    }
-    load_vec<DIM>(a.X1n, ns, x);
-    store_vec<DIM>(a.X1, ns, x);
+    store_vec<DIM>(a.XI, ns, xi);
+    if (rot) store_vec<DIM>(a.XIR, ns, xir);
+    store_vec<DIM>(a.X1, ns, x1);
     if (a.on_x1) {
-        store_vec<DIM>(a.XR, ns, x);
-        const bool srk4 = (P.flags & GCR_F_SRK4) != 0;
-        load_vec<DIM>(srk4 ? a.F0 : a.FT, ns, x);
-        store_vec<DIM>(srk4 ? a.FT : a.F0, ns, x);
-        if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP | GCR_F_MOTOR)) a.WA[ns] = a.WAn[ns];
+        store_vec<DIM>(a.XR, ns, x1);
+        store_vec<DIM>(srk4 ? a.FT : a.F0, ns, fl);
+        if (wa) a.WA[ns] = wav;
     }
 }
 

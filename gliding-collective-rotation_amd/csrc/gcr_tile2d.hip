@@ -322,13 +322,21 @@ template <typename FT, int NT>
 __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
-    const RepParams& P = a.P[tile.x];
     if (!rebuild_runs(a, tile.x, j)) return;
-    if (!(P.flags & GCR_F_WCA)) return;
+    if (!(a.P[tile.x].flags & GCR_F_WCA)) return;
+    // the replica's parameters in LDS (reads of global memory after the
+    // table's stores would each wait on memory)
+    constexpr int PW = (int)(sizeof(RepParams) / 4);
+    static_assert(PW <= NT, "one word per thread");
+    __shared__ RepParams Ps;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ int2 lm[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
+    if ((int)threadIdx.x < PW)
+        reinterpret_cast<uint32_t*>(&Ps)[threadIdx.x] = reinterpret_cast<const uint32_t*>(a.P + tile.x)[threadIdx.x];
+    __syncthreads();
+    const RepParams& P = Ps;
     const TileLayout t = tile_layout<NT>(a, P, tile, sh);
     write_tile_table(a, P, t, sh);
     if (t.in_lds) stage_positions<FT, NT>(a, P, a.on_x1 ? a.X1 : a.X, sh, t, lp, lm, lcell);
@@ -633,6 +641,7 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
         if (early) load_rest_state<STAGE, GEN>(a, s, br);
         const double* xs = bd.x;
         const int2 me = bd.me;
+
         double acc[2], tang[2];
         double pa0 = 0.0, pa1 = 0.0;
         if (P.flags & GCR_F_WCA) {

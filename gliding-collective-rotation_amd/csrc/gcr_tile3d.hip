@@ -255,9 +255,17 @@ template <typename FT, int NT>
 __global__ void __launch_bounds__(NT) k_nlist_tile3d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
-    const RepParams& P = a.P[tile.x];
     if (!rebuild_runs(a, tile.x, j)) return;
-    if (!(P.flags & GCR_F_WCA)) return;
+    if (!(a.P[tile.x].flags & GCR_F_WCA)) return;
+    // the replica's parameters in LDS: read from global memory after the
+    // kernel's first store, each would be a load waiting on memory
+    constexpr int PW = (int)(sizeof(RepParams) / 4);
+    static_assert(PW <= NT, "one word per thread");
+    __shared__ RepParams Ps;
+    if ((int)threadIdx.x < PW)
+        reinterpret_cast<uint32_t*>(&Ps)[threadIdx.x] = reinterpret_cast<const uint32_t*>(a.P + tile.x)[threadIdx.x];
+    __syncthreads();
+    const RepParams& P = Ps;
     __shared__ T3Shared sh;
     __shared__ LPos3<FT> lp[T3_MAX_BEADS];
     __shared__ int2 lm[T3_MAX_BEADS];
