@@ -162,7 +162,7 @@ struct gcr_pool {
     double* BMAX = nullptr;
     int32_t* SCNT = nullptr;  // per slot: active sliding partners of the running stage
     int32_t* TILES = nullptr;
-    int32_t* TT = nullptr;  // tile layout tables (TT_INTS per tile), sized to the tiles in use
+    int32_t* TT = nullptr;  // tile layout tables (TT_INTS / TT3_INTS per tile), sized to the tiles in use
     int64_t capTT = 0;
     uint16_t* NL = nullptr;
     int32_t* NLEN = nullptr;
@@ -265,6 +265,9 @@ int pick_tile3(const RepParams& P, int forced)
 // first + b runs: each XCD gets one contiguous run of tiles (a band of
 // rows), so neighbouring tiles, which read each other's cells as halo, mostly
 // share an L2.
+// Ints of a tile's layout table.
+int tt_ints(int dim) { return dim == 2 ? TT_INTS : TT3_INTS; }
+
 std::vector<int> xcd_order(int first, int n)
 {
     constexpr int NXCD = 8;
@@ -534,13 +537,16 @@ int rebuild_tables(gcr_pool* p)
         if (p->TT) HIPCHK(hipFree(p->TT));
         p->TT = nullptr;
         p->capTT = 0;
-        HIPCHK(hipMalloc((void**)&p->TT, sizeof(int32_t) * TT_INTS * (size_t)p->ntile));
+        HIPCHK(hipMalloc((void**)&p->TT, sizeof(int32_t) * tt_ints(p->dim) * (size_t)p->ntile));
         p->capTT = p->ntile;
         p->a.TT = p->TT;
     }
     // tables of the new tiles are filled by the next re-sort; until then a
     // tile takes the path that computes its layout in the kernel
-    if (p->ntile) k_fill_i32<<<(p->ntile * TT_INTS + BS - 1) / BS, BS, 0, p->stream>>>(p->TT, p->ntile * TT_INTS, 0);
+    if (p->ntile) {
+        const int n = p->ntile * tt_ints(p->dim);
+        k_fill_i32<<<(n + BS - 1) / BS, BS, 0, p->stream>>>(p->TT, n, 0);
+    }
     HIPCHK(hipMemcpyAsync(p->rep_chk_off, rco.data(), rco.size() * 4, hipMemcpyHostToDevice, p->stream));
     HIPCHK(hipMemcpyAsync(p->dP, p->P.data(), sizeof(RepParams) * p->capR, hipMemcpyHostToDevice,
                           p->stream));

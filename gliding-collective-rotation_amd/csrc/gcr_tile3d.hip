@@ -29,6 +29,18 @@
 // list sums its pairs by the slow global cell scan, which sets the launch's
 // length
 #define NL3_MAX 128
+// Layout table of a 3D tile (TT3_INTS ints at TT[tile * TT3_INTS], written by
+// the list builder at every re-sort): [0] 1 when the stage kernels may use it
+// (the tile fits in LDS and no halo axis wraps onto itself), [1] staged
+// beads, [2] owned beads, [3] owned rows; from TT3_HALO, per halo row
+// hz W + hy: (first slot of its first run, beads in that run, first slot of
+// the run after the x wrap, LDS offset of the row), a halo row being one or
+// two runs of slots; from TT3_OWN, per owned row oz own_y + oy: (first slot,
+// beads, LDS index of its first bead).
+#define TT3_INTS 384
+#define TT3_HALO 8
+#define TT3_OWN (TT3_HALO + 4 * (T3_MAX_SIDE + 2) * (T3_MAX_SIDE + 2))
+static_assert(TT3_OWN + 3 * T3_MAX_ROWS <= TT3_INTS, "3D tile table size");
 
 namespace gcr {
 
@@ -248,6 +260,60 @@ __device__ __forceinline__ void owned_bead3(const T3Shared& sh, const T3Layout& 
     s = sh.gst[c0] + (li - sh.cst[c0]);
 }
 
+// The tile's layout table from the layout just computed. Block-uniform call.
+__device__ __forceinline__ void write_tile_table3(const PoolArgs& a, const RepParams& P, const T3Layout& t,
+                                                  const T3Shared& sh)
+{
+    int* T = a.TT + (size_t)blockIdx.x * TT3_INTS;
+    const int W = t.W, rows = W * W, orows = t.own_y * t.own_z;
+    const bool fast = t.in_lds && W <= P.nc;
+    if (threadIdx.x == 0) {
+        T[0] = fast ? 1 : 0;
+        T[1] = t.total;
+        T[2] = sh.own_pref[orows];
+        T[3] = orows;
+    }
+    if (!fast) return;
+    // halo cells before the x wrap: 1 if the left one wraps, nc - xa if the
+    // right ones do, else the whole row
+    const int xa = t.cx0 - 1;
+    const int k = xa < 0 ? 1 : (xa + W > P.nc ? P.nc - xa : W);
+    for (int hr = threadIdx.x; hr < rows; hr += blockDim.x) {
+        const int c = hr * W;  // the row's first halo cell
+        int* R = T + TT3_HALO + 4 * hr;
+        R[0] = sh.gst[c];
+        R[1] = sh.cst[c + k] - sh.cst[c];
+        R[2] = k < W ? sh.gst[c + k] : 0;
+        R[3] = sh.cst[c];
+    }
+    for (int ro = threadIdx.x; ro < orows; ro += blockDim.x) {
+        const int oz = ro / t.own_y, oy = ro - oz * t.own_y;
+        const int c = ((oz + 1) * W + oy + 1) * W + 1;  // owned cells 1 .. own_x of halo row (oy+1, oz+1)
+        int* R = T + TT3_OWN + 3 * ro;
+        R[0] = sh.gst[c];
+        R[1] = sh.cst[c + t.own_x] - sh.cst[c];
+        R[2] = sh.cst[c];
+    }
+}
+
+// Owned bead o of a tile from its layout table: (slot, LDS index), or s = -1.
+__device__ __forceinline__ void table_owned3(const int* tt, int o, int& s, int& li)
+{
+    s = -1;
+    li = 0;
+    int before = 0;
+    const int orows = tt[3];
+    for (int ro = 0; ro < orows; ++ro) {
+        const int* R = tt + TT3_OWN + 3 * ro;
+        if (o < before + R[1]) {
+            s = R[0] + (o - before);
+            li = R[2] + (o - before);
+            return;
+        }
+        before += R[1];
+    }
+}
+
 // Neighbour lists of the tile's owned beads after a cell re-sort (same run
 // condition as the re-sort), in the cell scan's order; NLEN = -1 marks a
 // bead without a list (more than NL3_MAX entries, or a tile outside LDS).
@@ -271,6 +337,7 @@ __global__ void __launch_bounds__(NT) k_nlist_tile3d(PoolArgs a, int j)
     __shared__ int2 lm[T3_MAX_BEADS];
     __shared__ uint16_t lcell[T3_MAX_BEADS];
     const T3Layout t = tile3_layout<NT>(a, P, tile, sh);
+    write_tile_table3(a, P, t, sh);
     if (t.in_lds) stage3<FT, NT>(a, P, a.on_x1 ? a.X1 : a.X, sh, t, lp, lm, lcell);
     __syncthreads();
     const FT rl2 = (FT)P.rl2, h = (FT)P.h;
@@ -333,63 +400,193 @@ __global__ void __launch_bounds__(NT) k_nlist_tile3d(PoolArgs a, int j)
     }
 }
 
+// The inputs of one owned bead that come from global memory and that the
+// pair loop and the bonded terms need: position, (orig, chain), list length
+// and chain links.
+struct Head3 {
+    int s;
+    double x[3];
+    int2 me;
+    int len;
+    int4 lk;
+};
+
+__device__ __forceinline__ void load_head3(const PoolArgs& a, const RepParams& P, const double* Xa, int s,
+                                           bool listed, Head3& b)
+{
+    b.s = s;
+    load_vec<3>(Xa, s, b.x);
+    b.me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+    b.len = (listed && (P.flags & GCR_F_WCA)) ? a.NLEN[s] : -1;
+    b.lk = (P.flags & GCR_F_SPRINGS) ? *reinterpret_cast<const int4*>(a.LINK + 4 * s) : make_int4(-1, -1, -1, -1);
+}
+
 // The 3D force stages: STAGE 0 / 1 as the 2D tile kernels (STAGE 1 for SRK
 // stage a.sub), STAGE 2 a force evaluation of replica j only at X (forces
 // into FT, tangents into Tn). Per bead: the bonded terms, the pair sum, the
 // propulsion and walls, then stage_store -- every feature of the step.
+// Prologue: with the tile's layout table (written at the last re-sort) each
+// thread finds its first owned bead and its share of the halo from the table
+// and issues their loads at once; without one the layout is computed here
+// (tile3_layout) and the halo staged after it. In the bead loop the chain
+// neighbours' positions are in flight during the pair loop, and the pair sum
+// (in FT) is added after the bonded terms, as the reference orders them.
 template <typename FT, int STAGE, int NT>
-__global__ void __launch_bounds__(NT) k_stage_tile3d(PoolArgs a, int j)
+__global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
     const int r = tile.x;
-    const RepParams& P = a.P[r];
     RepState* S = a.S + r;
     if (STAGE == 2) {
         if (r != j) return;
     } else if (!step_runs(S, j)) {
         return;
     }
-    if (STAGE == 1 && !later_stage_applies(P, a.sub)) return;
-    const double* Xa = STAGE == 1 ? a.X1 : a.X;
+    if (STAGE == 1 && !later_stage_applies(a.P[r], a.sub)) return;
+    // the tile's layout table and the replica's parameters in LDS (the
+    // parameters: read from global memory after the kernel's first store,
+    // each would be a load waiting on memory)
+    constexpr int PW = (int)(sizeof(RepParams) / 4);
+    __shared__ RepParams Ps;
+    __shared__ int tt[TT3_INTS];
     __shared__ T3Shared sh;
     __shared__ LPos3<FT> lp[T3_MAX_BEADS];
     __shared__ uint16_t lcell[T3_MAX_BEADS];
-    const T3Layout t = tile3_layout<NT>(a, P, tile, sh);
-    if (t.in_lds) stage3<FT, NT>(a, P, Xa, sh, t, lp, nullptr, lcell);
+    for (int i = threadIdx.x; i < TT3_INTS + PW; i += NT) {
+        if (i < TT3_INTS)
+            tt[i] = a.TT[(size_t)blockIdx.x * TT3_INTS + i];
+        else
+            reinterpret_cast<uint32_t*>(&Ps)[i - TT3_INTS] = reinterpret_cast<const uint32_t*>(a.P + r)[i - TT3_INTS];
+    }
+    __syncthreads();
+    const RepParams& P = Ps;
+    const double* Xa = STAGE == 1 ? a.X1 : a.X;
+    const bool fast = tt[0] != 0;
+    constexpr bool early = sizeof(FT) == 4;  // the FP64 kernels hold more registers
+    T3Layout t;
+    int n_own, li0 = 0;
+    Head3 h0;
+    h0.s = -1;
+    if (fast) {
+        t.in_lds = true;
+        n_own = tt[2];
+        const int total = tt[1];
+        const int W = P.tile_x + 2, rows = W * W, nc = P.nc;
+        if ((int)threadIdx.x < n_own) {
+            int s0;
+            table_owned3(tt, (int)threadIdx.x, s0, li0);
+            load_head3(a, P, Xa, s0, true, h0);
+        }
+        // the halo: staged bead i lies in the last halo row whose LDS offset
+        // is <= i, in the first run of that row or after its x wrap
+        constexpr int KS = (T3_MAX_BEADS + NT - 1) / NT;
+        double xv[KS][3];
+        int cv[KS], rv[KS];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            rv[k] = -1;
+            if (i < total) {
+                int lo = 0, hi = rows;  // row offset[lo] <= i < row offset[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (tt[TT3_HALO + 4 * mid + 3] <= i)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                const int* R = tt + TT3_HALO + 4 * lo;
+                const int off = i - R[3];
+                const int g = off < R[1] ? R[0] + off : R[2] + (off - R[1]);
+                load_vec<3>(Xa, g, xv[k]);
+                cv[k] = a.CELL[g];
+                rv[k] = lo;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            if (rv[k] < 0) continue;
+            const int i = (int)threadIdx.x + k * NT;
+            const int hz = rv[k] / W, hy = rv[k] - hz * W;
+            const int gz = (tile.w - 1 + hz + nc) % nc;
+            const int gy = (tile.z - 1 + hy + nc) % nc;
+            const int gx = cv[k] - (gz * nc + gy) * nc;  // the cell the bead was binned into
+            // halo column: x - (cx0 - 1), folded into [0, W) across the wrap
+            int hx = gx - (tile.y - 1);
+            hx += hx < 0 ? nc : (hx >= nc ? -nc : 0);
+            double rel[3];
+            rel[0] = cell_rel(xv[k][0], gx, P);
+            rel[1] = cell_rel(xv[k][1], gy, P);
+            rel[2] = cell_rel(xv[k][2], gz, P);
+            lp[i] = make_lpos3<FT>(rel, hx, hy, hz);
+        }
+    } else {
+        t = tile3_layout<NT>(a, P, tile, sh);
+        n_own = sh.own_pref[t.own_y * t.own_z];
+        if (t.in_lds) stage3<FT, NT>(a, P, Xa, sh, t, lp, nullptr, lcell);
+    }
     __syncthreads();
     const PairConsts<FT> kc = pair_consts<FT>(P);
     const uint2* NL = reinterpret_cast<const uint2*>(a.NL);
-    const int n_own = sh.own_pref[t.own_y * t.own_z];
     double fsq = 0.0;
     int ro = 0;
     for (int o = threadIdx.x; o < n_own; o += NT) {
-        int li, c0, s;
-        owned_bead3(sh, t, lcell, t.in_lds, o, ro, li, c0, s);
-        double xs[3], acc[3], tang[3];
-        load_vec<3>(Xa, s, xs);
-        if (STAGE == 1 && !(P.flags & GCR_F_SRK4)) {
-            load_vec<3>(a.F0, s, acc);  // SRK2 accumulates into F0
+        // each bead reads the parameters it needs from LDS again rather than
+        // holding them in registers across the loop
+        asm volatile("" ::: "memory");
+        int li;
+        Head3 hd;
+        if (fast && o == (int)threadIdx.x) {  // loaded in the prologue
+            li = li0;
+            hd = h0;
         } else {
+            int sx;
+            if (fast) {
+                table_owned3(tt, o, sx, li);
+            } else {
+                int c0;
+                owned_bead3(sh, t, lcell, t.in_lds, o, ro, li, c0, sx);
+            }
+            load_head3(a, P, Xa, sx, t.in_lds, hd);
+        }
+        const int s = hd.s;
+        const int2 me = hd.me;
+        const double* xs = hd.x;
+        double acc[3], tang[3];
+        // SRK2 accumulates into F0: FP32 in flight during the pair loop
+        const bool from_f0 = STAGE == 1 && !(P.flags & GCR_F_SRK4);
+        if (early && from_f0) load_vec<3>(a.F0, s, acc);
+        // the chain neighbours' positions: FP32, in flight during the pair
+        // loop; FP64, after it (the registers)
+        ChainNb<3> cn;
+        cn.lk = hd.lk;
+        auto load_nb = [&]() {
+            if (cn.lk.x >= 0) load_vec<3>(Xa, cn.lk.x, cn.p2);
+            if (cn.lk.y >= 0) load_vec<3>(Xa, cn.lk.y, cn.p1);
+            if (cn.lk.z >= 0) load_vec<3>(Xa, cn.lk.z, cn.n1);
+            if (cn.lk.w >= 0) load_vec<3>(Xa, cn.lk.w, cn.n2);
+        };
+        if (early) load_nb();
+        FT f[3] = {(FT)0, (FT)0, (FT)0};
+        if (P.flags & GCR_F_WCA) {
+            if (hd.len >= 0)
+                list3_pair_sum<FT>(lp, NL + s, a.nl_stride, hd.len, lp[li], kc, f);
+            else
+                pair_sum<3, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, f);
+        }
+        if (!early) {
+            load_nb();
+            if (from_f0) load_vec<3>(a.F0, s, acc);
+        }
+        if (!from_f0) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) acc[i] = 0.0;
         }
-        const int2 me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+        // bonded terms first, then the pair sum, then propulsion (SRK.py:228-253)
         bool tang_set = false;
-        if (P.flags & GCR_F_SPRINGS) {
-            ChainNb<3> cn;
-            load_chain_nb<3>(Xa, a.LINK, s, cn);
-            bonded<3, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
-        }
-        if (P.flags & GCR_F_WCA) {
-            FT f[3] = {(FT)0, (FT)0, (FT)0};
-            const int len = t.in_lds ? a.NLEN[s] : -1;
-            if (len >= 0)
-                list3_pair_sum<FT>(lp, NL + s, a.nl_stride, len, lp[li], kc, f);
-            else
-                pair_sum<3, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, f);
+        if (P.flags & GCR_F_SPRINGS) bonded<3, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) acc[i] += (double)f[i];
-        }
+        for (int i = 0; i < 3; ++i) acc[i] += (double)f[i];
         if (STAGE == 2) {
             if (!tang_set) load_vec<3>(a.T, s, tang);
             if (P.flags & GCR_F_PROPEL) propel<3>(a, me.y, tang, acc);
