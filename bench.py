@@ -32,12 +32,12 @@ HBM_PEAK_GBS = 8000.0
 # bead-step in total
 ALG_BYTES = {"stage0": 32, "stage1": 16, "update": 32}
 # HBM bytes per bead per launch of the mixed kernels on this workload, from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (tile edge 8, 512 threads per
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (tile edge 8, 256 threads per
 # tile, skin 0.3 cutoff, re-sort every 32 steps), each counter scaled by its
 # reading for a plain 1 GiB copy kernel (`--mode copy`: reads x 2.0, writes x
 # 1.0); profiles/stage_kernels_400k_beads.md holds the passes and commands.
 # Measured again whenever these kernels change.
-TRAFFIC_BYTES = {"stage0": 169.0, "stage1": 136.0, "update": 104.8}
+TRAFFIC_BYTES = {"stage0": 172.0, "stage1": 137.0, "update": 105.0}
 
 
 def parse():
