@@ -174,6 +174,7 @@ struct gcr_pool {
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
     int tile_threads = 0;  // threads per tile workgroup: 256, 512, 1024, 0 = from the tile sizes
     int nt_launch = BS;    // resolved by rebuild_tables
+    int nt_list = BS;      // threads per tile of the list builder (rebuild_tables)
     bool has_srk4 = false; // a live replica integrates with SRK4: the step has 4 force stages
     bool tile_gen = false; // a live replica needs the general tile kernels (SRK4, walls, FENE)
     // populations acting on each other: replicas in population order, the law
@@ -494,6 +495,9 @@ int rebuild_tables(gcr_pool* p)
               // (100k 3D chain beads: FP32 25 vs 37 us per stage at 512 vs
               // 256 threads, FP64 41 vs 46 us at 256 vs 512)
             p->nt_launch = p->precision == GCR_MIXED ? 512 : 256;
+        // the 2D list builder: one owned bead per thread whatever the force
+        // stages run at (each bead's 3 x 3 scan is a long chain of LDS reads)
+        p->nt_list = p->dim == 2 ? (per >= 768 ? 1024 : per >= 384 ? 512 : 256) : p->nt_launch;
     }
     if ((int64_t)br.size() > p->capBlk || (int64_t)cr.size() > p->capChk ||
         (int64_t)tl.size() / 4 > p->capTile || (int64_t)hr.size() > p->capChb)
@@ -581,7 +585,7 @@ void launch_rebuild(gcr_pool* p, int j, bool on_x1 = false, int sub = 0)
         k_gather<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         k_links<2><<<p->nblk, BS, 0, st>>>(a, 1, j);
         if (p->ntile) {
-            const int nt = p->nt_launch;
+            const int nt = p->nt_list;
             if (p->precision == GCR_MIXED) {
                 if (nt == 1024)
                     k_nlist_tile2d<float, 1024><<<p->ntile, 1024, 0, st>>>(a, j);
