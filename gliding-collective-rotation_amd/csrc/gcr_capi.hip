@@ -559,6 +559,7 @@ int rebuild_tables(gcr_pool* p)
     p->a.nchk = p->nchk;
     p->a.nchb = p->nchb;
     p->a.ntile = p->ntile;
+    p->a.nt_stage = p->nt_launch;
     p->tables_dirty = false;
     p->norm_stale = p->safe_stale = true;
     return GCR_OK;

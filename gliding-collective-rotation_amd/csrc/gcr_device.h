@@ -195,4 +195,7 @@ struct PoolArgs {
     // the inner SRK4 evaluations, 3 the last (at X1); the re-sort on x1
     // keys on the virtual positions written by stage `sub`
     int32_t sub;
+    // threads per tile of the 2D force stages (the list builder orders each
+    // tile's owned beads for that many)
+    int32_t nt_stage;
 };

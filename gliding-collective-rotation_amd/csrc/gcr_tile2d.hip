@@ -28,16 +28,24 @@
 #define TILE_MAX_ROWS 16
 #define NL_MAX 64            // neighbour-list entries per bead (16 groups of 4)
 // Layout table of a tile (TT_INTS ints, written by the list builder at every
-// re-sort): [0] 1 when the stage kernels may use it (the tile fits in LDS and
-// its halo rows and columns do not wrap onto themselves), [1] staged beads,
-// [2] owned beads, [3] owned rows; from TT_HALO, per halo row (first slot of
-// its first run, beads in that run, first slot of the run after the x wrap,
-// LDS offset of the row), a halo row being one or two runs of slots; from
-// TT_OWN, per owned row (first slot, beads, LDS index of its first bead).
-#define TT_INTS 128
+// re-sort): [0] bit 0 set when the stage kernels may use it (the tile fits in
+// LDS and its halo rows and columns do not wrap onto themselves), bit 1 set
+// when the processing order below is filled, [1] staged beads, [2] owned
+// beads, [3] owned rows; from TT_HALO, per halo row (first slot of its first
+// run, beads in that run, first slot of the run after the x wrap, LDS offset
+// of the row), a halo row being one or two runs of slots; from TT_OWN, per
+// owned row (first slot, beads, LDS index of its first bead); from TT_PERM,
+// as u16, the owned bead each pass of the stage kernels' bead loop takes
+// (TT_PERM_MAX beads at most): the beads sorted by list length, longest
+// first, every other pass of NT threads reversed, so that the lanes of a
+// wave walk lists of about one length and every wave gets a long and a
+// short one.
 #define TT_HALO 4
 #define TT_OWN (TT_HALO + 4 * (TILE_MAX_ROWS + 2))
-static_assert(TT_OWN + 3 * TILE_MAX_ROWS <= TT_INTS, "tile table size");
+#define TT_PERM 128
+#define TT_PERM_MAX 1024
+#define TT_INTS (TT_PERM + TT_PERM_MAX / 2)
+static_assert(TT_OWN + 3 * TILE_MAX_ROWS <= TT_PERM, "tile table size");
 
 namespace gcr {
 
@@ -277,7 +285,7 @@ __device__ __forceinline__ void owned_bead_scan(const TileShared& sh, const Tile
 // just computed: halo row hy is halo cells hy W .. hy W + W - 1, whose slots
 // are one run, or two where the row's x coordinates wrap round the box (the
 // left halo cell at nc - 1, or the right ones from 0). Block-uniform call.
-__device__ __forceinline__ void write_tile_table(const PoolArgs& a, const RepParams& P, const TileLayout& t,
+__device__ __forceinline__ bool write_tile_table(const PoolArgs& a, const RepParams& P, const TileLayout& t,
                                                  const TileShared& sh)
 {
     int* T = a.TT + (size_t)blockIdx.x * TT_INTS;
@@ -289,7 +297,7 @@ __device__ __forceinline__ void write_tile_table(const PoolArgs& a, const RepPar
         T[2] = sh.own_pref[t.own_y];
         T[3] = t.own_y;
     }
-    if (!fast) return;
+    if (!fast) return false;
     const int tid = (int)threadIdx.x;
     if (tid < rows) {
         const int c = tid * W, xa = t.cx0 - 1;
@@ -309,6 +317,7 @@ __device__ __forceinline__ void write_tile_table(const PoolArgs& a, const RepPar
         R[1] = sh.cst[c + t.own_x] - sh.cst[c];
         R[2] = sh.cst[c];
     }
+    return true;
 }
 
 // Neighbour lists of the tile's owned beads, after a cell re-sort (same run
@@ -333,12 +342,15 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ int2 lm[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
+    __shared__ uint8_t okey[TT_PERM_MAX];     // list groups of each owned bead (17: no list)
+    __shared__ uint16_t osort[TT_PERM_MAX];   // owned beads by list length, longest first
+    __shared__ int obin[NL_MAX / 4 + 2];
     if ((int)threadIdx.x < PW)
         reinterpret_cast<uint32_t*>(&Ps)[threadIdx.x] = reinterpret_cast<const uint32_t*>(a.P + tile.x)[threadIdx.x];
     __syncthreads();
     const RepParams& P = Ps;
     const TileLayout t = tile_layout<NT>(a, P, tile, sh);
-    write_tile_table(a, P, t, sh);
+    const bool write_ok = write_tile_table(a, P, t, sh);
     if (t.in_lds) stage_positions<FT, NT>(a, P, a.on_x1 ? a.X1 : a.X, sh, t, lp, lm, lcell);
     __syncthreads();
     const FT rl2 = (FT)P.rl2, h = (FT)P.h;
@@ -395,7 +407,38 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
         } else {
             a.NLEN[s] = -1;
         }
+        if (o < TT_PERM_MAX) okey[o] = (uint8_t)(n <= NL_MAX ? (n + 3) >> 2 : NL_MAX / 4 + 1);
     }
+    // The processing order of the stage kernels' bead loop (a.nt_stage
+    // threads per tile): a counting sort of the owned beads by list groups,
+    // longest first (which bead of a key comes first does not matter: the
+    // order decides only which thread computes a bead, not its result).
+    if (!write_ok || n_own > TT_PERM_MAX || a.nt_stage <= 0) return;
+    constexpr int NK = NL_MAX / 4 + 2;
+    for (int k = threadIdx.x; k < NK; k += NT) obin[k] = 0;
+    __syncthreads();
+    for (int o = threadIdx.x; o < n_own; o += NT) atomicAdd(&obin[okey[o]], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = NK - 1; k >= 0; --k) {
+            const int c = obin[k];
+            obin[k] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < n_own; o += NT) osort[atomicAdd(&obin[okey[o]], 1)] = (uint16_t)o;
+    __syncthreads();
+    // pass k of the stage loop takes owned positions [k NTS, k NTS + m_k):
+    // in order on even passes, reversed on odd ones
+    const int NTS = a.nt_stage;
+    uint16_t* perm = reinterpret_cast<uint16_t*>(a.TT + (size_t)blockIdx.x * TT_INTS + TT_PERM);
+    for (int o = threadIdx.x; o < n_own; o += NT) {
+        const int k = o / NTS, tk = o - k * NTS, mk = min(NTS, n_own - k * NTS);
+        perm[o] = osort[(k & 1) ? k * NTS + (mk - 1 - tk) : o];
+    }
+    if (threadIdx.x == 0 && write_ok) a.TT[(size_t)blockIdx.x * TT_INTS] = 3;
 }
 
 // The inputs of one owned bead that come from global memory. The head
@@ -511,27 +554,29 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
     // whole kernel: read from global memory after the kernel's first store,
     // which could alias them, each would be a vector load waiting on memory.
     constexpr int PW = (int)(sizeof(RepParams) / 4);
-    static_assert(sizeof(RepParams) % 4 == 0 && TT_INTS + PW + 1 <= NT, "one word per thread");
+    static_assert(sizeof(RepParams) % 4 == 0, "parameters copied by words");
     __shared__ RepParams Ps;
     __shared__ double dt_s;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
     __shared__ int tt[TT_INTS];
-    {
-        const int tid = (int)threadIdx.x;
-        if (tid < TT_INTS)
-            tt[tid] = a.TT[(size_t)blockIdx.x * TT_INTS + tid];
-        else if (tid < TT_INTS + PW)
-            reinterpret_cast<uint32_t*>(&Ps)[tid - TT_INTS] = reinterpret_cast<const uint32_t*>(a.P + r)[tid - TT_INTS];
-        else if (tid == TT_INTS + PW)
+    for (int i = threadIdx.x; i <= TT_INTS + PW; i += NT) {
+        if (i < TT_INTS)
+            tt[i] = a.TT[(size_t)blockIdx.x * TT_INTS + i];
+        else if (i < TT_INTS + PW)
+            reinterpret_cast<uint32_t*>(&Ps)[i - TT_INTS] = reinterpret_cast<const uint32_t*>(a.P + r)[i - TT_INTS];
+        else
             dt_s = S->dt;
     }
     __syncthreads();
     const RepParams& P = Ps;
     const double* Xa = STAGE == 1 ? a.X1 : a.X;
     const bool springs = (P.flags & GCR_F_SPRINGS) != 0;
-    const bool fast = tt[0] != 0;
+    const bool fast = (tt[0] & 1) != 0;
+    // the owned bead a position of the bead loop takes (the list builder's
+    // order when the table holds one)
+    const uint16_t* perm = (tt[0] & 2) ? reinterpret_cast<const uint16_t*>(tt + TT_PERM) : nullptr;
     constexpr bool early = sizeof(FT) == 4;  // the FP64 kernels have no registers to spare
     TileLayout t;
     int n_own;
@@ -545,10 +590,11 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
         const int total = tt[1];
         if (early && (int)threadIdx.x < n_own) {
             int s0, s1;
-            table_owned(tt, (int)threadIdx.x, s0, li0);
+            const int o0 = (int)threadIdx.x, o1 = o0 + NT;
+            table_owned(tt, perm ? (int)perm[o0] : o0, s0, li0);
             load_head<GEN>(a, P, Xa, s0, h0);
-            if ((int)threadIdx.x + NT < n_own) {
-                table_owned(tt, (int)threadIdx.x + NT, s1, li1);
+            if (o1 < n_own) {
+                table_owned(tt, perm ? (int)perm[o1] : o1, s1, li1);
                 load_head<GEN>(a, P, Xa, s1, h1);
             }
         }
@@ -626,7 +672,7 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
         } else {
             int sx, c0;
             if (fast)
-                table_owned(tt, o, sx, li);
+                table_owned(tt, perm ? (int)perm[o] : o, sx, li);
             else if (t.in_lds)
                 owned_bead(sh, t, lcell, o, ry, li, c0, sx);
             else
