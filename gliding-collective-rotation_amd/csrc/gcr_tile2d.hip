@@ -456,13 +456,21 @@ struct BeadHead {
     int4 lk;   // chain links (lean variant)
 };
 
+// The bead's position and (orig, chain): not needed before the pair loop
+// ends, so the prologue leaves them out of its burst of loads and the bead's
+// turn issues them.
+__device__ __forceinline__ void load_head_pos(const PoolArgs& a, const double* Xa, BeadHead& b)
+{
+    load_vec<2>(Xa, b.s, b.x);
+    b.me = *reinterpret_cast<const int2*>(a.META + 2 * b.s);
+}
+
 template <bool GEN>
 __device__ __forceinline__ void load_head(const PoolArgs& a, const RepParams& P, const double* Xa, int s,
-                                          BeadHead& b)
+                                          BeadHead& b, bool pos = true)
 {
     b.s = s;
-    load_vec<2>(Xa, s, b.x);
-    b.me = *reinterpret_cast<const int2*>(a.META + 2 * s);
+    if (pos) load_head_pos(a, Xa, b);
     b.len = -1;
     b.g0 = make_uint2(0u, 0u);
     if (P.flags & GCR_F_WCA) {
@@ -592,10 +600,10 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
             int s0, s1;
             const int o0 = (int)threadIdx.x, o1 = o0 + NT;
             table_owned(tt, perm ? (int)perm[o0] : o0, s0, li0);
-            load_head<GEN>(a, P, Xa, s0, h0);
+            load_head<GEN>(a, P, Xa, s0, h0, false);
             if (o1 < n_own) {
                 table_owned(tt, perm ? (int)perm[o1] : o1, s1, li1);
-                load_head<GEN>(a, P, Xa, s1, h1);
+                load_head<GEN>(a, P, Xa, s1, h1, false);
             }
         }
         // the halo: staged bead i lies in the last halo row whose LDS offset
@@ -642,11 +650,11 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
         if (early && (int)threadIdx.x < n_own) {
             int s0;
             owned_bead_scan(sh, t, (int)threadIdx.x, ry, li0, c0_unused, s0);
-            load_head<GEN>(a, P, Xa, s0, h0);
+            load_head<GEN>(a, P, Xa, s0, h0, false);
             if ((int)threadIdx.x + NT < n_own) {
                 int ry1 = ry, s1;
                 owned_bead_scan(sh, t, (int)threadIdx.x + NT, ry1, li1, c0_unused, s1);
-                load_head<GEN>(a, P, Xa, s1, h1);
+                load_head<GEN>(a, P, Xa, s1, h1, false);
             }
         }
         if (t.in_lds) stage_positions<FT, NT>(a, P, Xa, sh, t, lp, nullptr, lcell);
@@ -666,9 +674,11 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
         if (early && o == (int)threadIdx.x) {  // heads loaded in the prologue
             li = li0;
             bd = h0;
+            load_head_pos(a, Xa, bd);
         } else if (early && o == (int)threadIdx.x + NT) {
             li = li1;
             bd = h1;
+            load_head_pos(a, Xa, bd);
         } else {
             int sx, c0;
             if (fast)
