@@ -94,9 +94,10 @@ def run_gpu(w, precision, steps, warmup, batch, rebuild, kernel_steps, tile=0, t
     n = w.n
     pool = E.Pool(2, precision, bead_capacity=n, cell_capacity=4 * n + 1024,
                   chain_capacity=w.phys.n_chains, replica_capacity=1)
-    # a run shorter than one graph batch gets a batch of its own length, so
-    # the timed region holds no launches of steps that do not run
-    batch = min(batch, steps)
+    # a batch starts with a re-sort: a longer re-sort period needs batches of
+    # that period; a run shorter than one graph batch gets a batch of its own
+    # length, so the timed region holds no launches of steps that do not run
+    batch = min(max(batch, rebuild), steps)
     rebuild = min(rebuild, batch)
     pool.set_schedule(batch, rebuild)
     pool.set_tile(tile)
@@ -753,6 +754,9 @@ def main():
             cfg["tuning_s"] = time.perf_counter() - t0
             cfg["tiling_top"] = [[r["tile"], r["threads"], round(1e3 * r["stages_ms"], 1)]
                                  for r in t["tiling_trials"][:4]]
+            cfg["schedule_top"] = [[round(r["skin"], 3), r["rebuild_every"], round(1e3 * r["ms_per_step"], 1),
+                                    r["replays"]]
+                                   for r in sorted(t["schedule_trials"], key=lambda r: r["ms_per_step"])[:5]]
         tuned[name] = cfg
         w.phys.skin = cfg["skin"]
         res[name] = run_gpu(w, prec, args.steps, args.warmup, args.batch, cfg["rebuild_every"],

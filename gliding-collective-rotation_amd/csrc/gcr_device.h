@@ -128,7 +128,9 @@ struct PoolArgs {
     RepState* S;
     double* BMAX;  // [block] max |F0+F1|^2 of the block (stage 1)
     int32_t* SCNT; // [slot] active sliding partners of the running force stage
-    const int32_t* TILES;  // [tile][4] (replica, cx0, cy0, 0) in 2D, (replica, cx0, cy0, cz0) in 3D
+    int32_t* TILES;        // [tile][4] (replica, cx0, cy0, ints of the tile's layout table to read:
+                           // set by the list builder, 0 = the table's first TT_PERM) in 2D,
+                           // (replica, cx0, cy0, cz0) in 3D
     int32_t* TT;           // [tile][TT_INTS] (3D: TT3_INTS) LDS layout of each tile, written at every re-sort
     int32_t ntile;
     // per-slot state

@@ -412,12 +412,22 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
     // The processing order of the stage kernels' bead loop (a.nt_stage
     // threads per tile): a counting sort of the owned beads by list groups,
     // longest first (which bead of a key comes first does not matter: the
-    // order decides only which thread computes a bead, not its result).
+    // order decides only which thread computes a bead, not its result). Only
+    // for lists of three groups or more on average: the sorted order spreads
+    // a wave's beads over the tile, so its loads of lists and positions touch
+    // more cache lines, which short lists do not win back.
+    if (threadIdx.x == 0) a.TILES[4 * (size_t)blockIdx.x + 3] = 0;  // the table without an order
     if (!write_ok || n_own > TT_PERM_MAX || a.nt_stage <= 0) return;
     constexpr int NK = NL_MAX / 4 + 2;
     for (int k = threadIdx.x; k < NK; k += NT) obin[k] = 0;
     __syncthreads();
     for (int o = threadIdx.x; o < n_own; o += NT) atomicAdd(&obin[okey[o]], 1);
+    __syncthreads();
+    {
+        int groups = 0;
+        for (int k = 1; k < NK; ++k) groups += k * obin[k];
+        if (groups < 3 * n_own) return;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int acc = 0;
@@ -438,7 +448,10 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
         const int k = o / NTS, tk = o - k * NTS, mk = min(NTS, n_own - k * NTS);
         perm[o] = osort[(k & 1) ? k * NTS + (mk - 1 - tk) : o];
     }
-    if (threadIdx.x == 0 && write_ok) a.TT[(size_t)blockIdx.x * TT_INTS] = 3;
+    if (threadIdx.x == 0) {
+        a.TT[(size_t)blockIdx.x * TT_INTS] = 3;
+        a.TILES[4 * (size_t)blockIdx.x + 3] = TT_PERM + (n_own + 1) / 2;
+    }
 }
 
 // The inputs of one owned bead that come from global memory. The head
@@ -456,21 +469,13 @@ struct BeadHead {
     int4 lk;   // chain links (lean variant)
 };
 
-// The bead's position and (orig, chain): not needed before the pair loop
-// ends, so the prologue leaves them out of its burst of loads and the bead's
-// turn issues them.
-__device__ __forceinline__ void load_head_pos(const PoolArgs& a, const double* Xa, BeadHead& b)
-{
-    load_vec<2>(Xa, b.s, b.x);
-    b.me = *reinterpret_cast<const int2*>(a.META + 2 * b.s);
-}
-
 template <bool GEN>
 __device__ __forceinline__ void load_head(const PoolArgs& a, const RepParams& P, const double* Xa, int s,
-                                          BeadHead& b, bool pos = true)
+                                          BeadHead& b)
 {
     b.s = s;
-    if (pos) load_head_pos(a, Xa, b);
+    load_vec<2>(Xa, s, b.x);
+    b.me = *reinterpret_cast<const int2*>(a.META + 2 * s);
     b.len = -1;
     b.g0 = make_uint2(0u, 0u);
     if (P.flags & GCR_F_WCA) {
@@ -569,11 +574,14 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
     __shared__ int tt[TT_INTS];
-    for (int i = threadIdx.x; i <= TT_INTS + PW; i += NT) {
-        if (i < TT_INTS)
+    // the table's head and rows, and the bead order where the list builder
+    // wrote one (tile.w ints in all)
+    const int tlen = tile.w > TT_PERM ? (tile.w < TT_INTS ? tile.w : TT_INTS) : TT_PERM;
+    for (int i = threadIdx.x; i <= tlen + PW; i += NT) {
+        if (i < tlen)
             tt[i] = a.TT[(size_t)blockIdx.x * TT_INTS + i];
-        else if (i < TT_INTS + PW)
-            reinterpret_cast<uint32_t*>(&Ps)[i - TT_INTS] = reinterpret_cast<const uint32_t*>(a.P + r)[i - TT_INTS];
+        else if (i < tlen + PW)
+            reinterpret_cast<uint32_t*>(&Ps)[i - tlen] = reinterpret_cast<const uint32_t*>(a.P + r)[i - tlen];
         else
             dt_s = S->dt;
     }
@@ -584,7 +592,7 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
     const bool fast = (tt[0] & 1) != 0;
     // the owned bead a position of the bead loop takes (the list builder's
     // order when the table holds one)
-    const uint16_t* perm = (tt[0] & 2) ? reinterpret_cast<const uint16_t*>(tt + TT_PERM) : nullptr;
+    const uint16_t* perm = ((tt[0] & 2) && tlen > TT_PERM) ? reinterpret_cast<const uint16_t*>(tt + TT_PERM) : nullptr;
     constexpr bool early = sizeof(FT) == 4;  // the FP64 kernels have no registers to spare
     TileLayout t;
     int n_own;
@@ -600,10 +608,10 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
             int s0, s1;
             const int o0 = (int)threadIdx.x, o1 = o0 + NT;
             table_owned(tt, perm ? (int)perm[o0] : o0, s0, li0);
-            load_head<GEN>(a, P, Xa, s0, h0, false);
+            load_head<GEN>(a, P, Xa, s0, h0);
             if (o1 < n_own) {
                 table_owned(tt, perm ? (int)perm[o1] : o1, s1, li1);
-                load_head<GEN>(a, P, Xa, s1, h1, false);
+                load_head<GEN>(a, P, Xa, s1, h1);
             }
         }
         // the halo: staged bead i lies in the last halo row whose LDS offset
@@ -650,11 +658,11 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
         if (early && (int)threadIdx.x < n_own) {
             int s0;
             owned_bead_scan(sh, t, (int)threadIdx.x, ry, li0, c0_unused, s0);
-            load_head<GEN>(a, P, Xa, s0, h0, false);
+            load_head<GEN>(a, P, Xa, s0, h0);
             if ((int)threadIdx.x + NT < n_own) {
                 int ry1 = ry, s1;
                 owned_bead_scan(sh, t, (int)threadIdx.x + NT, ry1, li1, c0_unused, s1);
-                load_head<GEN>(a, P, Xa, s1, h1, false);
+                load_head<GEN>(a, P, Xa, s1, h1);
             }
         }
         if (t.in_lds) stage_positions<FT, NT>(a, P, Xa, sh, t, lp, nullptr, lcell);
@@ -674,11 +682,9 @@ __global__ void __launch_bounds__(NT, 4) k_stage_tile2d(PoolArgs a, int j)
         if (early && o == (int)threadIdx.x) {  // heads loaded in the prologue
             li = li0;
             bd = h0;
-            load_head_pos(a, Xa, bd);
         } else if (early && o == (int)threadIdx.x + NT) {
             li = li1;
             bd = h1;
-            load_head_pos(a, Xa, bd);
         } else {
             int sx, c0;
             if (fast)

@@ -75,7 +75,11 @@ def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 51
 
 def time_schedule(w, precision, skin, rebuild_every, tile, threads, steps=256, batch=64):
     """Wall-clock ms per step of graph batches (replays included) and the
-    number of replays, after one untimed batch (graph capture)."""
+    number of replays, after one untimed batch (graph capture). A batch
+    starts with a re-sort, so a period longer than `batch` runs in batches
+    of the period."""
+    batch = max(batch, rebuild_every)
+    steps = max(steps, 2 * batch)
     pool, rid = _pool_for(w, precision, skin)
     try:
         pool.set_schedule(batch, rebuild_every)
@@ -93,7 +97,7 @@ def time_schedule(w, precision, skin, rebuild_every, tile, threads, steps=256, b
     return 1e3 * sec / steps, st["replays"], st["failed"]
 
 
-def tune(w, precision=E.GCR_MIXED, skin_fracs=(0.1, 0.15, 0.2, 0.3), rebuilds=(8, 16, 32),
+def tune(w, precision=E.GCR_MIXED, skin_fracs=(0.1, 0.15, 0.2, 0.3, 0.4), rebuilds=(8, 16, 32, 64, 128),
          tiles=(0, 4, 6, 8, 10, 12), threads=(256, 512, 1024), steps=256, tiles3=(0, 2, 3, 4),
          threads3=(256, 512)):
     """Best (skin, rebuild_every, tile, threads) for workload w.
@@ -102,7 +106,9 @@ def tune(w, precision=E.GCR_MIXED, skin_fracs=(0.1, 0.15, 0.2, 0.3), rebuilds=(8
     (skin, period) pair is timed with that tiling, and the tiling is re-tuned
     at the chosen skin. skin_fracs are fractions of the pair cutoff; tiles
     and threads are the 2D candidates, tiles3 and threads3 the 3D ones (cube
-    sides; 0 is the automatic choice)."""
+    sides; 0 is the automatic choice). A re-sort period longer than the
+    caller's graph batch needs batches of that period (a batch starts with a
+    re-sort)."""
     rc = float(w.phys.wca[3])
     if w.phys.dim == 3:
         tiles, threads = tiles3, threads3
