@@ -5,6 +5,7 @@ Each launch of a step phase covers every replica's beads; per-replica step
 state (dt, step count, max force) lives on the device, so replicas of any
 size and parameters share launches without synchronising with each other.
 """
+import copy
 import time
 
 import numpy as np
@@ -104,7 +105,7 @@ def summary(ws):
                 max_beads=int(n.max()), mean_beads=float(n.mean()))
 
 
-def tune_pool(pool, ws, rids, tiles=(0, 4, 6, 8), threads=(256, 512), rebuilds=(8, 16, 32),
+def tune_pool(pool, ws, rids, tiles=(0, 4, 6, 8), threads=(256, 512), rebuilds=(8, 16, 32, 64, 128),
               batch=64, warm=128, kernel_steps=16, schedule_steps=256):
     """Autotunes a batched pool in place: the tile edge and threads per tile
     of its 2D force kernels (by the force-stage durations of every replica
@@ -119,7 +120,7 @@ def tune_pool(pool, ws, rids, tiles=(0, 4, 6, 8), threads=(256, 512), rebuilds=(
             tv[7] = steps
             pool.begin(r, tv)
 
-    pool.set_schedule(batch, 8)
+    pool.set_schedule(batch, min(8, batch))
     arm(warm)
     pool.run()
     tiling = []
@@ -137,15 +138,72 @@ def tune_pool(pool, ws, rids, tiles=(0, 4, 6, 8), threads=(256, 512), rebuilds=(
     pool.set_tile_threads(nt)
     sched = []
     for k in rebuilds:
-        pool.set_schedule(batch, k)
-        arm(batch)
+        # a batch starts with a re-sort: a longer period runs in batches of
+        # the period
+        kb = max(batch, k)
+        pool.set_schedule(kb, k)
+        arm(kb)
         pool.run()  # captures
-        arm(schedule_steps)
+        n = max(schedule_steps, 2 * kb)
+        arm(n)
         t0 = time.perf_counter()
         pool.run()
-        sched.append((k, 1e3 * (time.perf_counter() - t0) / schedule_steps))
+        sched.append((k, 1e3 * (time.perf_counter() - t0) / n))
     sched.sort(key=lambda x: x[1])
-    pool.set_schedule(batch, sched[0][0])
-    return dict(tile=tile, threads=nt, rebuild_every=sched[0][0],
+    best = sched[0][0]
+    pool.set_schedule(max(batch, best), best)
+    return dict(tile=tile, threads=nt, rebuild_every=best, steps_per_batch=max(batch, best),
                 tiling_trials=[[t, n, round(1e3 * m, 2)] for t, n, m in tiling],
                 schedule_trials=[[k, round(m, 4)] for k, m in sched])
+
+
+def with_skin(ws, frac):
+    """Copies of the workloads whose cell-list skin is `frac` times each
+    replica's pair cutoff (positions and the rest shared)."""
+    out = []
+    for w in ws:
+        c = copy.copy(w)
+        c.phys = copy.copy(w.phys)
+        c.phys.skin = frac * float(w.phys.wca[3])
+        out.append(c)
+    return out
+
+
+def tune_skin(ws, precision=E.GCR_MIXED, fracs=(0.3, 0.6, 1.0, 1.4), steps=512, batch=64, **kw):
+    """The cell-list skin of a batched pool, as a fraction of each replica's
+    pair cutoff: for each candidate a pool of all replicas with that skin,
+    tuned by tune_pool, then timed over `steps` steps of graph batches
+    (replays included). The skin sets the cell grid, so each candidate is a
+    pool of its own; results move at the rounding level only (another grid
+    sums the same pairs in another order). Returns (best fraction, the
+    tune_pool settings there, trials [fraction, ms per step, replays])."""
+    trials, best = [], None
+    for f in fracs:
+        wf = with_skin(ws, f)
+        pool = pool_for(wf, precision)
+        try:
+            rids = add_all(pool, wf)
+            tuned = tune_pool(pool, wf, rids, batch=batch, **kw)
+            kb = tuned["steps_per_batch"]
+            for w, r in zip(wf, rids):
+                tv = w.time_vals.copy()
+                tv[4] = -1.0
+                tv[7] = kb
+                pool.begin(r, tv)
+            pool.run()  # the tuned schedule's graphs
+            replays0 = sum(pool.status(r)["replays"] for r in rids)
+            for w, r in zip(wf, rids):
+                tv = w.time_vals.copy()
+                tv[4] = -1.0
+                tv[7] = steps
+                pool.begin(r, tv)
+            t0 = time.perf_counter()
+            pool.run()
+            ms = 1e3 * (time.perf_counter() - t0) / steps
+            replays = sum(pool.status(r)["replays"] for r in rids) - replays0
+        finally:
+            pool.close()
+        trials.append([f, round(ms, 4), int(replays)])
+        if best is None or ms < best[1]:
+            best = (f, ms, tuned)
+    return best[0], best[2], trials

@@ -12,6 +12,7 @@ device. Results do not depend on what else is running: noise is keyed by
 the cell-list schedule follows each replica's own step count.
 """
 import ctypes as C
+import copy
 import threading
 import time
 from dataclasses import dataclass, field
@@ -161,8 +162,13 @@ class _Lane:
 
 
 class Server:
+    """skin_frac: the cell-list skin the server gives every request, as a
+    fraction of the request's pair cutoff (None: the request's own). The
+    skin is an engine setting: results move at the rounding level only."""
+
     def __init__(self, hbm_budget_bytes=8 << 30, precision=E.GCR_MIXED, steps_per_batch=64,
-                 rebuild_every=8, max_replicas=256, split_2d=0.5):
+                 rebuild_every=8, max_replicas=256, split_2d=0.5, skin_frac=None):
+        self.skin_frac = skin_frac
         self.lanes = {
             2: _Lane(2, int(hbm_budget_bytes * split_2d), precision, steps_per_batch,
                      rebuild_every, max_replicas),
@@ -178,6 +184,12 @@ class Server:
         """Run the stream. With realtime, requests become visible at their
         arrival time; otherwise all are visible at once (offline batch)."""
         pending = sorted(requests, key=lambda r: r.arrival)
+        if self.skin_frac is not None:
+            for r in pending:
+                w = copy.copy(r.work)
+                w.phys = copy.copy(r.work.phys)
+                w.phys.skin = self.skin_frac * float(w.phys.wca[3])
+                r.work = w
         for r in pending:
             if not self.lanes[r.dim].fits(r, 0) and r.n > self.lanes[r.dim].cap_beads:
                 raise ValueError(f"request {r.rid} exceeds the HBM budget of its pool")
