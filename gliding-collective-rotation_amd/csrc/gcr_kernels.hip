@@ -154,30 +154,39 @@ __device__ __forceinline__ void draw_noise(const RepParams& P, int orig_global, 
 }
 
 // Per-block running condition of a replica for local step j of a batch.
+// Every field is read whatever the outcome and the tests are combined without
+// branches, so that the reads issue together: at the start of a kernel the
+// replica's state is not in the scalar cache, and a test per read would wait
+// on memory once per cache line, in sequence.
 __device__ __forceinline__ bool step_runs(const RepState* S, int j)
 {
-    if (S->frozen || S->fixed) return false;
-    if (S->steps != S->batch_base + j) return false;
-    bool a = (S->max_steps < 0) || (S->steps < S->max_steps);
-    bool b = (S->mdu < 0) || (S->elapsed < S->mdu);
-    return a && b;
+    const int32_t frozen = S->frozen, fixed = S->fixed;
+    const int64_t steps = S->steps, base = S->batch_base, max_steps = S->max_steps;
+    const double mdu = S->mdu, elapsed = S->elapsed;
+    const bool idle = (frozen | fixed) != 0;
+    const bool due = steps == base + j;
+    const bool a = (max_steps < 0) | (steps < max_steps);
+    const bool b = (mdu < 0) | (elapsed < mdu);
+    return !idle & due & a & b;
 }
 
 // A cell re-sort placed before local step j runs only when that step runs
 // (j < 0: unconditional, for set-up and diagnostics), so the arrays of the
 // last executed step (x1, forces) keep the layout they were computed in. The
 // mid-step re-sort on x1 runs only for a replica whose predictor of this
-// step left the skin.
+// step left the skin. All reads issue together, as in step_runs.
 __device__ __forceinline__ bool rebuild_runs(const PoolArgs& a, int r, int j)
 {
     const RepState* S = a.S + r;
-    if (S->frozen) return false;
-    if (S->fixed) return j < 0 && !a.on_x1;  // a fixed population is sorted once
-    if (a.on_x1) return step_runs(S, j) && ((S->pred_far >> a.sub) & 1);
+    const int32_t frozen = S->frozen, fixed = S->fixed, pred_far = S->pred_far, sort_due = S->sort_due;
+    const bool runs = step_runs(S, j);
+    if (frozen) return false;
+    if (fixed) return j < 0 && !a.on_x1;  // a fixed population is sorted once
+    if (a.on_x1) return runs && ((pred_far >> a.sub) & 1);
     // the safe batch re-sorts at its start and then wherever an update left
     // the skin (its lists are otherwise still complete)
-    if (a.safe && j > 0) return step_runs(S, j) && S->sort_due;
-    return j < 0 || step_runs(S, j);
+    if (a.safe && j > 0) return runs && sort_due;
+    return j < 0 || runs;
 }
 
 // A virtual position outside the skin: outside the safe batch the batch is
@@ -205,9 +214,9 @@ __device__ __forceinline__ double virtual_rescale(const RepParams& P, int sub)
 
 // Whether a launch of the later force stages (at X1) for SRK stage `sub`
 // has work for this replica: SRK4 evaluates at stages 1, 2 and 3, SRK2 at 3.
-__device__ __forceinline__ bool later_stage_applies(const RepParams& P, int sub)
+__device__ __forceinline__ bool later_stage_applies(uint32_t flags, int sub)
 {
-    return (P.flags & GCR_F_SRK4) ? sub >= 1 : sub == 3;
+    return (flags & GCR_F_SRK4) ? sub >= 1 : sub == 3;
 }
 
 template <int DIM>
@@ -980,11 +989,14 @@ __global__ void __launch_bounds__(BS) k_finalize(PoolArgs a, int j)
 {
     const int r = blockIdx.x;
     RepState* S = a.S + r;
-    if (!step_runs(S, j)) return;
     const RepParams& P = a.P[r];
+    // the reduction's extent is read with the run test's reads
+    const int red_n = P.red_n, red_off = P.red_off;
+    asm volatile("" ::"s"(red_n), "s"(red_off));
+    if (!step_runs(S, j)) return;
     double m2 = 0.0;
-    for (int i = threadIdx.x; i < P.red_n; i += BS) {
-        double v = a.BMAX[P.red_off + i];
+    for (int i = threadIdx.x; i < red_n; i += BS) {
+        double v = a.BMAX[red_off + i];
         m2 = v > m2 ? v : m2;
     }
     m2 = block_max(m2);
@@ -1368,18 +1380,21 @@ __global__ void k_group_dt(PoolArgs a, int j)
 // (after one the compiler could not keep them in scalar registers: every use
 // would be a vector load waiting on memory), and every input of the bead is
 // in flight at once, (orig, chain) first, so that the next step's noise is
-// drawn while the rest arrives.
+// drawn while the rest arrives. Before that, the block's table entries, then
+// the replica's state and extent, are each read in one go (one wait on
+// memory each, rather than one per test).
 template <int DIM>
 __global__ void __launch_bounds__(BS) k_update(PoolArgs a, int j)
 {
     const int r = a.blk_rep[blockIdx.x];
+    const int slot0 = a.blk_slot0[blockIdx.x];
     const RepParams P = a.P[r];
     const RepState* S = a.S + r;
-    if (S->frozen || S->steps != S->batch_base + j + 1) return;
-    const int s = a.blk_slot0[blockIdx.x] + threadIdx.x;
-    if (s >= P.bead_off + P.n) return;
+    const int32_t frozen = S->frozen;
+    const int64_t steps = S->steps, base = S->batch_base, step_global = S->step_global;
     const double dt_used = S->dt_used;
-    const int64_t steps = S->steps, step_global = S->step_global;
+    const int s = slot0 + threadIdx.x;
+    if ((frozen != 0) | (steps != base + j + 1) | (s >= P.bead_off + P.n)) return;
     const bool rot = (P.flags & GCR_F_ROTDIFF) != 0;
     // rod replicas: the corrector ran in k_rod; a bound motor stays put
     const bool moves = !(P.flags & GCR_F_ROD) && !((P.flags & GCR_F_MSTILL) && a.WA[s] >= 0);

@@ -437,12 +437,13 @@ __global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(Po
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
     const int r = tile.x;
     RepState* S = a.S + r;
-    if (STAGE == 2) {
-        if (r != j) return;
-    } else if (!step_runs(S, j)) {
-        return;
+    {
+        // the run test's reads (state, and the stage's SRK order) issue together
+        const uint32_t flags = a.P[r].flags;
+        const bool runs = STAGE == 2 ? r == j : step_runs(S, j);
+        const bool stage = STAGE != 1 || later_stage_applies(flags, a.sub);
+        if (!(runs & stage)) return;
     }
-    if (STAGE == 1 && !later_stage_applies(a.P[r], a.sub)) return;
     // the tile's layout table and the replica's parameters in LDS (the
     // parameters: read from global memory after the kernel's first store,
     // each would be a load waiting on memory)
