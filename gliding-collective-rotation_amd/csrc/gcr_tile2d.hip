@@ -221,7 +221,7 @@ __device__ __forceinline__ int staged_cell(const TileShared& sh, const TileLayou
     return lo;
 }
 
-// Stage the halo's positions (and, for the list builder, (orig, chain)), and
+// Stage the halo's positions (and, for the list builder, the original index), and
 // the halo cell of every staged bead. FP32 positions: each thread first marks
 // the LDS range of its cells and every staged bead reads its cell from there
 // (one LDS read instead of a search). FP64 positions: the search, which keeps
@@ -230,7 +230,7 @@ __device__ __forceinline__ int staged_cell(const TileShared& sh, const TileLayou
 template <typename FT, int NT>
 __device__ __forceinline__ void stage_positions(const PoolArgs& a, const RepParams& P, const double* Xa,
                                                 const TileShared& sh, const TileLayout& t, LPos<FT>* lp,
-                                                int2* lm, uint16_t* lcell)
+                                                int* lm, uint16_t* lcell)
 {
     const int nc = P.nc;
     constexpr bool mark = sizeof(FT) == 4;
@@ -250,7 +250,7 @@ __device__ __forceinline__ void stage_positions(const PoolArgs& a, const RepPara
         const int gy = (t.cy0 - 1 + hy + nc) % nc;
         const double2 x = *reinterpret_cast<const double2*>(Xa + 2 * g);
         lp[i] = make_lpos<FT>(cell_rel(x.x, gx, P), cell_rel(x.y, gy, P), hx, hy);
-        if (lm) lm[i] = *reinterpret_cast<const int2*>(a.META + 2 * g);
+        if (lm) lm[i] = a.META[2 * g];
     }
 }
 
@@ -340,7 +340,11 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
     __shared__ RepParams Ps;
     __shared__ TileShared sh;
     __shared__ LPos<FT> lp[TILE_MAX_BEADS];
-    __shared__ int2 lm[TILE_MAX_BEADS];
+    // the staged beads' original indices (the chain rule needs no more: a
+    // chain's beads are one run of original indices, so a bead knows its
+    // chain's run; keeping the chain per staged bead as well would hold the
+    // FP32 kernel to three tiles per CU)
+    __shared__ int lm[TILE_MAX_BEADS];
     __shared__ uint16_t lcell[TILE_MAX_BEADS];
     __shared__ uint8_t okey[TT_PERM_MAX];     // list groups of each owned bead (17: no list)
     __shared__ uint16_t osort[TT_PERM_MAX];   // owned beads by list length, longest first
@@ -367,9 +371,13 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
         }
         owned_bead(sh, t, lcell, o, ry, li, c0, s);
         const LPos<FT> me = lp[li];
-        const int2 mm = lm[li];
+        const int2 mm = *reinterpret_cast<const int2*>(a.META + 2 * s);
+        const int cf = a.CHAIN_FIRST[mm.y], cl = a.CHAIN_LEN[mm.y];
         int n = 0;
-        uint32_t w0 = 0u, w1 = 0u;  // the group being filled
+        // the group being filled, as one 64-bit word into which each entry
+        // shifts from the top: after four the first sits in the low half of
+        // w0 and the fourth in the high half of w1
+        uint32_t w0 = 0u, w1 = 0u;
         // the three cells of a halo row are consecutive in LDS: one range per row
         for (int dy = -1; dy <= 1; ++dy) {
             const int c = c0 + dy * t.W;
@@ -377,29 +385,25 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
             for (int q = sh.cst[c - 1]; q < q1; ++q) {
                 FT sep[2];
                 separation<FT>(lp[q], me, h, sep);
-                const int2 m = lm[q];
-                const int d = m.x - mm.x;
-                const bool excl = (m.y == mm.y) & ((nn < 0) | ((d < 0 ? -d : d) <= nn));
+                const int oq = lm[q];
+                const int d = oq - mm.x;
+                const bool same_chain = (uint32_t)(oq - cf) < (uint32_t)cl;
+                const bool excl = same_chain & ((nn < 0) | ((d < 0 ? -d : d) <= nn));
                 const bool ok = (q != li) & (fma(sep[1], sep[1], sep[0] * sep[0]) <= rl2) & !excl;
                 const bool put = ok & (n < NL_MAX);
-                const uint32_t v = put ? (uint32_t)q << (16 * (n & 1)) : 0u;
-                w0 |= (n & 2) ? 0u : v;
-                w1 |= (n & 2) ? v : 0u;
-                if (put & ((n & 3) == 3)) {
-                    NL[(int64_t)(n >> 2) * a.nl_stride + s] = make_uint2(w0, w1);
-                    w0 = w1 = 0u;
-                }
+                const uint32_t s0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+                const uint32_t s1 = __builtin_amdgcn_alignbit((uint32_t)q, w1, 16);
+                w0 = put ? s0 : w0;
+                w1 = put ? s1 : w1;
+                if (put & ((n & 3) == 3)) NL[(int64_t)(n >> 2) * a.nl_stride + s] = make_uint2(w0, w1);
                 n += ok ? 1 : 0;
             }
         }
         if (n <= NL_MAX) {
             if (n & 3) {  // pad the last group with the bead's own index
                 for (int sl = n & 3; sl < 4; ++sl) {
-                    const uint32_t v = (uint32_t)li << (16 * (sl & 1));
-                    if (sl < 2)
-                        w0 |= v;
-                    else
-                        w1 |= v;
+                    w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+                    w1 = __builtin_amdgcn_alignbit((uint32_t)li, w1, 16);
                 }
                 NL[(int64_t)(n >> 2) * a.nl_stride + s] = make_uint2(w0, w1);
             }
