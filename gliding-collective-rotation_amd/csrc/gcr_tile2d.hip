@@ -373,6 +373,16 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
         const LPos<FT> me = lp[li];
         const int2 mm = *reinterpret_cast<const int2*>(a.META + 2 * s);
         const int cf = a.CHAIN_FIRST[mm.y], cl = a.CHAIN_LEN[mm.y];
+        // the original indices the list leaves out, one run: the chain's
+        // beads within n_nearby of this one (itself included), or the whole
+        // chain for n_nearby < 0 (updates.py:645-650)
+        int ex_lo = cf, ex_hi = cf + cl;
+        if (nn >= 0) {
+            const int w = nn < cl ? nn : cl;
+            ex_lo = max(cf, mm.x - w);
+            ex_hi = min(cf + cl, mm.x + w + 1);
+        }
+        const uint32_t ex_n = (uint32_t)(ex_hi - ex_lo);
         int n = 0;
         // the group being filled, as one 64-bit word into which each entry
         // shifts from the top: after four the first sits in the low half of
@@ -385,11 +395,8 @@ __global__ void __launch_bounds__(NT) k_nlist_tile2d(PoolArgs a, int j)
             for (int q = sh.cst[c - 1]; q < q1; ++q) {
                 FT sep[2];
                 separation<FT>(lp[q], me, h, sep);
-                const int oq = lm[q];
-                const int d = oq - mm.x;
-                const bool same_chain = (uint32_t)(oq - cf) < (uint32_t)cl;
-                const bool excl = same_chain & ((nn < 0) | ((d < 0 ? -d : d) <= nn));
-                const bool ok = (q != li) & (fma(sep[1], sep[1], sep[0] * sep[0]) <= rl2) & !excl;
+                const bool excl = (uint32_t)(lm[q] - ex_lo) < ex_n;
+                const bool ok = (fma(sep[1], sep[1], sep[0] * sep[0]) <= rl2) & !excl;
                 const bool put = ok & (n < NL_MAX);
                 const uint32_t s0 = __builtin_amdgcn_alignbit(w1, w0, 16);
                 const uint32_t s1 = __builtin_amdgcn_alignbit((uint32_t)q, w1, 16);
