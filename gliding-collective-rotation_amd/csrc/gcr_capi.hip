@@ -126,7 +126,11 @@ struct gcr_pool {
     int capR = 0;
     Segs segB, segC, segCh;
     std::vector<RepParams> P;
-    std::vector<RepState> S;
+    // host copy of the replicas' step states, in page-locked memory: it goes
+    // to and from the device around every batch, and a copy from pageable
+    // memory is staged through a bounce buffer (about 20 us for a few
+    // hundred bytes)
+    RepState* S = nullptr;
     std::vector<int> live;
     std::vector<int> replays;
     std::vector<int> safe_left;  // per replica: batches still started in the safe form
@@ -215,7 +219,7 @@ int upload_rep(gcr_pool* p, int r)
 
 int pull_states(gcr_pool* p)
 {
-    HIPCHK(hipMemcpyAsync(p->S.data(), p->dS, sizeof(RepState) * p->capR, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(p->S, p->dS, sizeof(RepState) * p->capR, hipMemcpyDeviceToHost,
                           p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     return GCR_OK;
@@ -223,7 +227,7 @@ int pull_states(gcr_pool* p)
 
 int push_states(gcr_pool* p)
 {
-    HIPCHK(hipMemcpyAsync(p->dS, p->S.data(), sizeof(RepState) * p->capR, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(p->dS, p->S, sizeof(RepState) * p->capR, hipMemcpyHostToDevice,
                           p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     return GCR_OK;
@@ -1097,8 +1101,15 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     p->segC.init(capC);
     p->segCh.init(capCh);
     p->P.assign(capR, RepParams{});
-    p->S.assign(capR, RepState{});
-    for (auto& s : p->S) s.frozen = 1;
+    if (hipHostMalloc((void**)&p->S, sizeof(RepState) * capR, hipHostMallocDefault) != hipSuccess) {
+        p->S = nullptr;
+        gcr_pool_destroy(p);
+        return fail(GCR_ERR_HIP, "page-locked host memory for the replica states");
+    }
+    for (int r = 0; r < capR; ++r) {
+        p->S[r] = RepState{};
+        p->S[r].frozen = 1;
+    }
     p->live.assign(capR, 0);
     p->replays.assign(capR, 0);
     p->safe_left.assign(capR, 0);
@@ -1168,7 +1179,7 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
         rc = fail(GCR_ERR_HIP, "memset failed");
     if (rc == GCR_OK && hipMemset(p->FT, 0, sizeof(double) * nb * dim) != hipSuccess)
         rc = fail(GCR_ERR_HIP, "memset failed");
-    if (rc == GCR_OK && hipMemcpy(p->dS, p->S.data(), sizeof(RepState) * capR, hipMemcpyHostToDevice) != hipSuccess)
+    if (rc == GCR_OK && hipMemcpy(p->dS, p->S, sizeof(RepState) * capR, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(GCR_ERR_HIP, "state upload failed");
     if (rc != GCR_OK) {
         gcr_pool_destroy(p);
@@ -1964,6 +1975,7 @@ int gcr_pool_destroy(gcr_pool* p)
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->S) (void)hipHostFree(p->S);
     delete p;
     return GCR_OK;
 }
