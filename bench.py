@@ -307,6 +307,8 @@ def serving_part(args):
     try:
         done = srv.serve(reqs)
         graphs = {f"{d}d": lane.pool.graph_stats() for d, lane in srv.lanes.items()}
+        lanes = {f"{d}d": {"batches": lane.batches, "busy_s": lane.busy_s}
+                 for d, lane in srv.lanes.items()}
     finally:
         srv.close()
     rep = SV.report(done)
@@ -325,7 +327,7 @@ def serving_part(args):
             "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
             "mean_queue_s": rep["mean_queue_s"], "makespan_s": rep["makespan_s"],
             "failed": rep["failed"], "offered_bead_steps_per_s": offered,
-            "graphs": graphs, "host_init_s": t_init}
+            "graphs": graphs, "lanes": lanes, "host_init_s": t_init}
 
 
 def main_serve(args):

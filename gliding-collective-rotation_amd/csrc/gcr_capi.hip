@@ -667,16 +667,26 @@ void launch_tile(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent
         launch_tile_v<FT, STAGE, false>(p, a, j, ev0, ev1);
 }
 
-// 3D tile kernels at the pool's threads per tile.
+// 3D tile kernels at the pool's threads per tile, in their lean variant
+// unless a replica needs the general one.
+template <typename FT, int STAGE, bool GEN>
+void launch_tile3_v(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (p->nt_launch == 1024)
+        launch_k(k_stage_tile3d<FT, STAGE, 1024, GEN>, p->ntile, p->stream, ev0, ev1, a, j, 1024);
+    else if (p->nt_launch == 512)
+        launch_k(k_stage_tile3d<FT, STAGE, 512, GEN>, p->ntile, p->stream, ev0, ev1, a, j, 512);
+    else
+        launch_k(k_stage_tile3d<FT, STAGE, 256, GEN>, p->ntile, p->stream, ev0, ev1, a, j, 256);
+}
+
 template <typename FT, int STAGE>
 void launch_tile3(gcr_pool* p, const PoolArgs& a, int j, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (p->nt_launch == 1024)
-        launch_k(k_stage_tile3d<FT, STAGE, 1024>, p->ntile, p->stream, ev0, ev1, a, j, 1024);
-    else if (p->nt_launch == 512)
-        launch_k(k_stage_tile3d<FT, STAGE, 512>, p->ntile, p->stream, ev0, ev1, a, j, 512);
+    if (p->tile_gen)
+        launch_tile3_v<FT, STAGE, true>(p, a, j, ev0, ev1);
     else
-        launch_k(k_stage_tile3d<FT, STAGE, 256>, p->ntile, p->stream, ev0, ev1, a, j, 256);
+        launch_tile3_v<FT, STAGE, false>(p, a, j, ev0, ev1);
 }
 
 // One launch of a step: which = 0 the first force stage, 1 a later force

@@ -431,7 +431,11 @@ __device__ __forceinline__ void load_head3(const PoolArgs& a, const RepParams& P
 // (tile3_layout) and the halo staged after it. In the bead loop the chain
 // neighbours' positions are in flight during the pair loop, and the pair sum
 // (in FT) is added after the bonded terms, as the reference orders them.
-template <typename FT, int STAGE, int NT>
+// GEN = false is the lean variant for pools without SRK4, walls, FENE
+// springs, sliding and interacting populations: their code stays out of the
+// kernel, and with it the registers it would hold (the general FP32 variant
+// spills at 128 VGPRs).
+template <typename FT, int STAGE, int NT, bool GEN>
 __global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(PoolArgs a, int j)
 {
     const int4 tile = reinterpret_cast<const int4*>(a.TILES)[blockIdx.x];
@@ -555,7 +559,7 @@ __global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(Po
         const double* xs = hd.x;
         double acc[3], tang[3];
         // SRK2 accumulates into F0: FP32 in flight during the pair loop
-        const bool from_f0 = STAGE == 1 && !(P.flags & GCR_F_SRK4);
+        const bool from_f0 = STAGE == 1 && !(GEN && (P.flags & GCR_F_SRK4));
         if (early && from_f0) load_vec<3>(a.F0, s, acc);
         // the chain neighbours' positions: FP32, in flight during the pair
         // loop; FP64, after it (the registers)
@@ -585,13 +589,13 @@ __global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(Po
         }
         // bonded terms first, then the pair sum, then propulsion (SRK.py:228-253)
         bool tang_set = false;
-        if (P.flags & GCR_F_SPRINGS) bonded<3, FT>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
+        if (P.flags & GCR_F_SPRINGS) bonded<3, FT, GEN>(cn, xs, a.CHAIN_A[me.y], P, acc, tang, tang_set);
 #pragma unroll
         for (int i = 0; i < 3; ++i) acc[i] += (double)f[i];
         if (STAGE == 2) {
             if (!tang_set) load_vec<3>(a.T, s, tang);
             if (P.flags & GCR_F_PROPEL) propel<3>(a, me.y, tang, acc);
-            if (P.flags & GCR_F_WALLS) add_walls<3>(P, xs, acc);
+            if (GEN && (P.flags & GCR_F_WALLS)) add_walls<3>(P, xs, acc);
             store_vec<3>(a.FT, s, acc);
             store_vec<3>(a.Tn, s, tang);
             continue;
@@ -600,12 +604,12 @@ __global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(Po
             if (!tang_set) load_vec<3>(a.T, s, tang);
             propel<3>(a, me.y, tang, acc);
         }
-        if (P.flags & GCR_F_WALLS) {  // at the step's start position in every stage
+        if (GEN && (P.flags & GCR_F_WALLS)) {  // at the step's start position in every stage
             double x0[3];
             load_vec<3>(a.X, s, x0);
             add_walls<3>(P, x0, acc);
         }
-        if (P.flags & (GCR_F_SLIDE | GCR_F_GROUP)) {  // tangents at this stage, for k_group_stage
+        if (GEN && (P.flags & (GCR_F_SLIDE | GCR_F_GROUP))) {  // tangents at this stage, for k_group_stage
             if (!tang_set) load_vec<3>(a.T, s, tang);
             store_vec<3>(a.Tn, s, tang);
         }

@@ -6,8 +6,9 @@ the HBM budget (beads, cells, chains, replica slots reserved at start-up),
 keeps its state resident on the device, advances every admitted simulation
 in the same launches batch after batch, and retires a simulation the batch
 it completes, freeing its room for the next arrivals. 2D and 3D requests go
-to two pools driven from two host threads, so their launches overlap on the
-device. Results do not depend on what else is running: noise is keyed by
+to two pools, each driven by its own host thread at its own pace (admit,
+one graph batch, retire), so their launches overlap on the device and
+neither pool waits for the other's batches. Results do not depend on what else is running: noise is keyed by
 (request seed, bead, step), reductions are per replica and order-free, and
 the cell-list schedule follows each replica's own step count.
 """
@@ -105,7 +106,7 @@ class _Lane:
         self.used = np.zeros(3, dtype=np.int64)  # beads, cells, chains
         self.active = {}  # rid -> (request, cells)
         self.batches = 0
-        self.error = None
+        self.busy_s = 0.0  # host time inside the pool's graph batches
         self.rejected = []  # requests whose input the library refused
 
     def fits(self, req, cells):
@@ -135,15 +136,6 @@ class _Lane:
         self.active[rid] = (req, cells)
         req.admitted = now
         return True
-
-    def step_batch(self):
-        self.error = None
-        try:
-            if self.active:
-                self.pool.run(max_batches=1)
-                self.batches += 1
-        except Exception as e:  # re-raised by the serving loop
-            self.error = e
 
     def retire(self, now, keep_results):
         done = []
@@ -194,50 +186,65 @@ class Server:
             if not self.lanes[r.dim].fits(r, 0) and r.n > self.lanes[r.dim].cap_beads:
                 raise ValueError(f"request {r.rid} exceeds the HBM budget of its pool")
         cells = {r.rid: replica_cells(r.work) for r in pending}
-        waiting = []
         finished = []
-        t0 = time.perf_counter()
+        errors = []
         lock = threading.Lock()
+        t0 = time.perf_counter()
 
         def now():
             return time.perf_counter() - t0
 
-        while pending or waiting or any(l.active for l in self.lanes.values()):
-            t = now()
-            while pending and (not realtime or pending[0].arrival <= t):
-                waiting.append(pending.pop(0))
-            # admission in arrival order; a request that does not fit waits,
-            # smaller later ones may pass it (backfill)
-            still = []
-            for r in waiting:
-                lane = self.lanes[r.dim]
-                if lane.fits(r, cells[r.rid]) and lane.admit(r, cells[r.rid], now()):
-                    continue
-                still.append(r)
-            waiting = still
-            for l in self.lanes.values():
-                finished += l.rejected
-                l.rejected = []
-            busy = [l for l in self.lanes.values() if l.active]
-            if not busy:
-                if waiting:  # nothing runs and nothing fits: it never will
+        def drive(lane, queue):
+            """One lane's loop: admit what has arrived and fits, advance the
+            lane's pool by one graph batch, retire what completed. Each lane
+            keeps its own pace: a batch of the 2D pool never waits for one of
+            the 3D pool, and their launches overlap on the device."""
+            waiting = []
+            while queue or waiting or lane.active:
+                t = now()
+                while queue and (not realtime or queue[0].arrival <= t):
+                    waiting.append(queue.pop(0))
+                # admission in arrival order; a request that does not fit
+                # waits, smaller later ones may pass it (backfill)
+                still = []
+                for r in waiting:
+                    if lane.fits(r, cells[r.rid]) and lane.admit(r, cells[r.rid], now()):
+                        continue
+                    still.append(r)
+                waiting = still
+                done, lane.rejected = lane.rejected, []
+                if lane.active:
+                    tb = time.perf_counter()
+                    lane.pool.run(max_batches=1)
+                    lane.busy_s += time.perf_counter() - tb
+                    lane.batches += 1
+                    done += lane.retire(now(), keep_results)
+                elif waiting:  # nothing runs and nothing fits: it never will
                     raise RuntimeError(f"requests {[r.rid for r in waiting]} cannot be admitted "
                                        "into an idle server")
-                if pending and realtime:
-                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
-                continue
-            threads = [threading.Thread(target=l.step_batch) for l in busy]
-            for th in threads:
-                th.start()
-            for th in threads:
-                th.join()
-            for l in busy:
-                if l.error is not None:
-                    raise l.error
-            t = now()
-            for l in busy:
+                elif queue and realtime:
+                    time.sleep(max(0.0, min(0.01, queue[0].arrival - now())))
                 with lock:
-                    finished += l.retire(t, keep_results)
+                    finished.extend(done)
+
+        def guarded(lane, queue):
+            try:
+                drive(lane, queue)
+            except BaseException as e:  # re-raised by serve
+                with lock:
+                    errors.append(e)
+
+        threads = []
+        for dim, lane in self.lanes.items():
+            queue = [r for r in pending if r.dim == dim]
+            if queue:
+                threads.append(threading.Thread(target=guarded, args=(lane, queue)))
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        if errors:
+            raise errors[0]
         return finished
 
 
