@@ -189,6 +189,7 @@ class Server:
         finished = []
         errors = []
         lock = threading.Lock()
+        stop = threading.Event()  # set when a pool fails: the others end too
         t0 = time.perf_counter()
 
         def now():
@@ -200,7 +201,7 @@ class Server:
             keeps its own pace: a batch of the 2D pool never waits for one of
             the 3D pool, and their launches overlap on the device."""
             waiting = []
-            while queue or waiting or lane.active:
+            while (queue or waiting or lane.active) and not stop.is_set():
                 t = now()
                 while queue and (not realtime or queue[0].arrival <= t):
                     waiting.append(queue.pop(0))
@@ -233,6 +234,7 @@ class Server:
             except BaseException as e:  # re-raised by serve
                 with lock:
                     errors.append(e)
+                stop.set()
 
         threads = []
         for dim, lane in self.lanes.items():
