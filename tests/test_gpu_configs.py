@@ -2,6 +2,8 @@
 oracle's Verlet-list path (pair_mode 1, pinned against the all-pairs path in
 tests/test_oracle.py) on the host's cores.
 
+- 28 beads (4 chains x 7 in a 20 x 20 box): the full 10^4 steps in FP64
+  with in-kernel Philox noise within 1e-10 L.
 - 20k beads (1000 chains x 20, phi 0.3, WCA, kappa 10, chirality 0.2): 100
   SRK2 steps in FP64 with in-kernel Philox noise, which the oracle reproduces
   (orc_bead_noise), within 1e-10 L; the mixed form's error beside it.
@@ -81,6 +83,29 @@ def run_single(w, precision, steps, skin=None, tile=0, threads=0, batch=64, rebu
         return p.status(rid), p.download(rid, tangent=False, forces=False)
     finally:
         p.close()
+
+
+def test_chains_4x7_fp64_full_length():
+    """The CPU-runnable workload (4 flexible chains of 7 beads in a 20 x 20
+    box, WCA, springs, tangent propulsion, T = 0.01, dt 1e-3) for its full
+    10^4 steps in FP64 with the in-kernel noise, against the oracle: within
+    1e-10 L after 100, 1000 and 10^4 steps (the error is printed at each;
+    about 4e-11 after 10^4 steps, 2e-9 being the bar)."""
+    w = W.chains_4x7(seed=1, steps=10000)
+    L = w.phys.L
+    errs = {}
+    for steps in (100, 1000, 10000):
+        pos_o, pos1_o, _, _, st_o = oracle(w, steps)
+        st, out = run_single(w, E.GCR_FP64, steps)
+        assert st["steps"] == int(st_o[2]) == steps
+        errs[steps] = err(out["pos"], pos_o, L)
+        if steps == 100:
+            assert st["elapsed"] == pytest.approx(st_o[1], rel=1e-10)
+            assert err(out["pos1"], pos1_o, L) <= POS_TOL * L
+    print(f"[4 x 7 chains] fp64 vs oracle: " +
+          ", ".join(f"{k} steps {v:.3e}" for k, v in errs.items()) + f" (L = {L:g})", flush=True)
+    for steps, e in errs.items():
+        assert e <= POS_TOL * L, (steps, e)
 
 
 def test_chains_1000x20_fp64_100_steps():
