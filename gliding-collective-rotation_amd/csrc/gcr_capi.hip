@@ -176,6 +176,7 @@ struct gcr_pool {
     int64_t capChk = 0, capBlk = 0, capTile = 0, capChb = 0;
     int ntile = 0;
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
+    int n_cu = 256;        // compute units of the device (3D tile sides)
     int tile_threads = 0;  // threads per tile workgroup: 256, 512, 1024, 0 = from the tile sizes
     int nt_launch = BS;    // resolved by rebuild_tables
     int nt_list = BS;      // threads per tile of the list builder (rebuild_tables)
@@ -252,16 +253,38 @@ int pick_tile(const RepParams& P, int forced)
 
 // Tile side (cells) of the 3D LDS force kernel: the largest cube whose tile
 // with its halo holds at most 70% of T3_MAX_BEADS at the mean occupancy
-// (a 3D halo is 2-4x the owned cells, so tiles stay small), at most
-// T3_MAX_SIDE and the grid; a forced side is clamped the same way.
-int pick_tile3(const RepParams& P, int forced)
+// (a 3D halo is 2-4x the owned cells, so tiles stay small), at most `cap`
+// and the grid; a forced side is clamped to T3_MAX_SIDE and the grid.
+int pick_tile3(const RepParams& P, int forced, int cap = T3_MAX_SIDE)
 {
     const double occ = std::max((double)P.n / (double)P.ncell, 1e-3);
-    int side = forced > 0 ? forced : T3_MAX_SIDE;
+    int side = forced > 0 ? forced : cap;
     side = std::max(1, std::min(side, T3_MAX_SIDE));
     if (forced <= 0)
         while (side > 1 && (side + 2) * (side + 2) * (side + 2) * occ > 0.7 * T3_MAX_BEADS) --side;
     return std::min(side, P.nc);
+}
+
+// The largest side cap (8, 7 or 6) at which the pool's 3D tiles still number
+// at least two per compute unit: cubes larger than 6 cells stage less halo
+// per owned bead, but a launch of fewer tiles than that leaves compute units
+// with one tile or none (106k 3D chain beads: 45 us per step at 6 cells, 512
+// tiles; 64 us at 7 cells, 343 tiles; 197k beads: 74 us at 6 cells, 1000
+// tiles; 62 us at 7 cells, 512 tiles). 6 when no cap reaches that.
+int pick_cap3(const gcr_pool* p)
+{
+    for (int cap = T3_MAX_SIDE; cap > 6; --cap) {
+        int64_t tiles = 0;
+        for (int r = 0; r < p->capR; ++r) {
+            if (!p->live[r]) continue;
+            const RepParams& P = p->P[r];
+            const int side = pick_tile3(P, 0, cap);
+            const int64_t per = (P.nc + side - 1) / side;
+            tiles += per * per * per;
+        }
+        if (tiles >= 2 * (int64_t)p->n_cu) return cap;
+    }
+    return 6;
 }
 
 // The dispatcher hands a launch's workgroups to the 8 XCDs of the MI355X
@@ -406,6 +429,7 @@ int rebuild_tables(gcr_pool* p)
         }
     }
     if ((int64_t)xp.size() > p->capX) return fail(GCR_ERR_CAPACITY, "population law table full");
+    const int cap3 = p->dim == 3 ? pick_cap3(p) : 0;
     for (int r = 0; r < p->capR; ++r) {
         rco[r] = (int32_t)cr.size();
         RepParams& P = p->P[r];
@@ -450,7 +474,7 @@ int rebuild_tables(gcr_pool* p)
             P.red_off = P.tile_off;
             P.red_n = P.ntile;
         } else {  // cubic tiles (replica, cx0, cy0, cz0), XCD bands of (x, y, z) order
-            const int side = pick_tile3(P, p->tile_side);
+            const int side = pick_tile3(P, p->tile_side, cap3);
             P.tile_x = P.tile_y = side;
             P.tile_off = (int32_t)(tl.size() / 4);
             std::vector<int32_t> order3;
@@ -1101,6 +1125,12 @@ int gcr_pool_create(int32_t dim, int32_t precision, int64_t capB, int64_t capC, 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GCR_ERR_HIP, "no HIP device");
     gcr_pool* p = new gcr_pool();
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            p->n_cu = std::max(1, prop.multiProcessorCount);
+    }
     p->dim = dim;
     p->precision = precision;
     p->capB = capB;

@@ -20,9 +20,9 @@
 // different tiling. A bead whose list overflowed, and every bead of a tile
 // that does not fit in LDS (or of a box of fewer than 3 cells per side), sums
 // its pairs by the cell scan from global memory.
-#define T3_MAX_SIDE 6
-#define T3_MAX_HALO 512    // (side + 2)^3
-#define T3_MAX_ROWS 36     // owned (y, z) rows of a tile
+#define T3_MAX_SIDE 8
+#define T3_MAX_HALO 1000   // (side + 2)^3
+#define T3_MAX_ROWS 64     // owned (y, z) rows of a tile
 #define T3_MAX_BEADS 1536  // beads of a tile with its halo held in LDS
 // neighbour-list entries per 3D bead (32 groups of 4): a bead inside an LJ
 // cluster has about 90 beads within rc + skin, and one that overflows its
@@ -37,7 +37,7 @@
 // the run after the x wrap, LDS offset of the row), a halo row being one or
 // two runs of slots; from TT3_OWN, per owned row oz own_y + oy: (first slot,
 // beads, LDS index of its first bead).
-#define TT3_INTS 384
+#define TT3_INTS 640
 #define TT3_HALO 8
 #define TT3_OWN (TT3_HALO + 4 * (T3_MAX_SIDE + 2) * (T3_MAX_SIDE + 2))
 static_assert(TT3_OWN + 3 * T3_MAX_ROWS <= TT3_INTS, "3D tile table size");

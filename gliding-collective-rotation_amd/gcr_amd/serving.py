@@ -88,6 +88,11 @@ def replica_cells(w):
     return int(lib.gcr_replica_cells(w.phys.dim, C.byref(d))) + 1
 
 
+# most graph batches one call of the serving loop runs before it looks at
+# arrivals, errors and completions again
+MAX_BATCHES_PER_RUN = 32
+
+
 class _Lane:
     """One pool (one dimensionality) and its bookkeeping."""
 
@@ -103,10 +108,13 @@ class _Lane:
         self.pool = E.Pool(dim, precision, self.cap_beads, self.cap_cells, self.cap_chains,
                            max_replicas)
         self.pool.set_schedule(steps_per_batch, rebuild_every)
+        self.steps_per_batch = steps_per_batch
         self.used = np.zeros(3, dtype=np.int64)  # beads, cells, chains
         self.active = {}  # rid -> (request, cells)
         self.batches = 0
         self.busy_s = 0.0  # host time inside the pool's graph batches
+        self.batch_s = 0.0  # host time of the last graph batch
+        self.min_left = None  # steps left of the active request nearest its end (None: unknown)
         self.rejected = []  # requests whose input the library refused
 
     def fits(self, req, cells):
@@ -134,14 +142,21 @@ class _Lane:
         self.pool.begin(rid, tv)
         self.used += [req.n, cells, w.phys.n_chains]
         self.active[rid] = (req, cells)
+        if self.min_left is not None:
+            self.min_left = min(self.min_left, req.steps)
         req.admitted = now
         return True
 
     def retire(self, now, keep_results):
+        """Retires the requests that completed (or failed); records the steps
+        left of the nearest-to-complete request still running."""
         done = []
+        left = []
         for rid, (req, cells) in list(self.active.items()):
             st = self.pool.status(rid)
-            if not st["running"]:
+            if st["running"]:
+                left.append(req.steps - st["steps"])
+            else:
                 if keep_results and not st["failed"]:
                     req.result = self.pool.download(rid, pos1=False, forces=False)
                 req.result["status"] = st
@@ -150,7 +165,24 @@ class _Lane:
                 self.used -= [req.n, cells, req.work.phys.n_chains]
                 del self.active[rid]
                 done.append(req)
+        self.min_left = min(left) if left else None
         return done
+
+    def batches_ahead(self, queue, now, realtime):
+        """Graph batches the next run may take without delaying a retirement
+        or an admission: up to the batch in which the nearest-to-complete
+        request ends, and, while requests are still to arrive, not past the
+        next arrival (at the last batch's duration); one when either is
+        unknown. Fewer returns to the host between batches leave fewer gaps
+        in the pool's launches."""
+        if self.min_left is None or self.min_left <= 0:
+            return 1
+        k = -(-self.min_left // self.steps_per_batch)
+        if queue and realtime:
+            if self.batch_s <= 0.0:
+                return 1
+            k = min(k, int((queue[0].arrival - now) / self.batch_s))
+        return max(1, min(k, MAX_BATCHES_PER_RUN))
 
 
 class Server:
@@ -197,9 +229,11 @@ class Server:
 
         def drive(lane, queue):
             """One lane's loop: admit what has arrived and fits, advance the
-            lane's pool by one graph batch, retire what completed. Each lane
-            keeps its own pace: a batch of the 2D pool never waits for one of
-            the 3D pool, and their launches overlap on the device."""
+            lane's pool by as many graph batches as it can take before the
+            next completion or arrival (batches_ahead), retire what
+            completed. Each lane keeps its own pace: a batch of the 2D pool
+            never waits for one of the 3D pool, and their launches overlap on
+            the device."""
             waiting = []
             while (queue or waiting or lane.active) and not stop.is_set():
                 t = now()
@@ -215,10 +249,13 @@ class Server:
                 waiting = still
                 done, lane.rejected = lane.rejected, []
                 if lane.active:
+                    k = lane.batches_ahead(queue, now(), realtime)
                     tb = time.perf_counter()
-                    lane.pool.run(max_batches=1)
-                    lane.busy_s += time.perf_counter() - tb
-                    lane.batches += 1
+                    lane.pool.run(max_batches=k)
+                    run_s = time.perf_counter() - tb
+                    lane.busy_s += run_s
+                    lane.batch_s = run_s / k
+                    lane.batches += k
                     done += lane.retire(now(), keep_results)
                 elif waiting:  # nothing runs and nothing fits: it never will
                     raise RuntimeError(f"requests {[r.rid for r in waiting]} cannot be admitted "

@@ -259,8 +259,10 @@ int gcr_pool_graph_stats(gcr_pool* p, double out[4]);
  * the cell occupancy. 2D: square tiles of about 512 owned beads, the tile
  * with its halo within 70% of the LDS capacity at the mean occupancy; at
  * most 16. 3D: cubes, the largest whose halo holds at most 70% of the 3D
- * tile's LDS capacity at the mean occupancy; a side set here is clamped to 6
- * and to the grid. No result bit depends on the tiling. */
+ * tile's LDS capacity at the mean occupancy, at most 8 cells while the
+ * pool's tiles still number two per compute unit (else at most 6); a side
+ * set here is clamped to 8 and to the grid. No result bit depends on the
+ * tiling. */
 int gcr_pool_set_tile(gcr_pool* p, int32_t side);
 /* Threads per tile workgroup of the force kernels: 256, 512, 1024, or 0 to
  * pick about one owned bead per thread in 2D, and 512 (FP32) or 256 (FP64)

@@ -18,29 +18,39 @@ class FakePool:
         self.lane, self.delay, self.fail_at = lane, delay, fail_at
 
     def run(self, max_batches=0):
-        assert max_batches == 1
-        time.sleep(self.delay)
-        if self.fail_at is not None and self.lane.batches == self.fail_at:
-            raise RuntimeError(f"pool of dim {self.lane.dim} failed")
-        for rid in self.lane.active:
-            self.lane.left[rid] -= STEPS_PER_BATCH
+        assert max_batches >= 1
+        self.calls += 1
+        for b in range(max_batches):
+            if not any(v > 0 for v in self.lane.left.values()):
+                break
+            time.sleep(self.delay)
+            if self.fail_at is not None and self.done == self.fail_at:
+                raise RuntimeError(f"pool of dim {self.lane.dim} failed")
+            self.done += 1
+            for rid in self.lane.active:
+                self.lane.left[rid] -= STEPS_PER_BATCH
 
 
-class FakeLane:
-    """The interface Server.serve drives (fits, admit, retire, pool.run)."""
+class FakeLane(SV._Lane):
+    """The interface Server.serve drives (fits, admit, retire, pool.run, and
+    _Lane's batches_ahead)."""
 
     def __init__(self, dim, cap_beads, delay=0.0, fail_at=None, bad_n=None):
         self.dim = dim
         self.cap_beads = cap_beads
+        self.steps_per_batch = STEPS_PER_BATCH
         self.used = 0
         self.active = {}
         self.left = {}
         self.rejected = []
         self.batches = 0
         self.busy_s = 0.0
+        self.batch_s = 0.0
+        self.min_left = None
         self.bad_n = bad_n
         self.peak = 0
         self.pool = FakePool(self, delay, fail_at)
+        self.pool.calls = self.pool.done = 0
         self.thread = set()
 
     def fits(self, req, cells):
@@ -57,6 +67,8 @@ class FakeLane:
         self.left[req.rid] = req.steps
         self.used += req.n
         self.peak = max(self.peak, self.used)
+        if self.min_left is not None:
+            self.min_left = min(self.min_left, req.steps)
         req.admitted = now
         return True
 
@@ -68,7 +80,9 @@ class FakeLane:
                 req.result = {"status": {"failed": False, "steps": req.steps}}
                 self.used -= req.n
                 del self.active[rid]
+                del self.left[rid]
                 done.append(req)
+        self.min_left = min(self.left.values()) if self.left else None
         return done
 
 
@@ -113,7 +127,7 @@ def test_pools_advance_at_their_own_pace():
     reqs = [request(0, 3, 1000, 64 * 10)] + [request(1 + i, 2, 500, 64 * 40) for i in range(3)]
     done = server(lanes).serve(reqs, realtime=False)
     fin = {r.rid: r.finished for r in done}
-    assert lanes[2].batches == 40 and lanes[3].batches == 10
+    assert lanes[2].pool.done == 40 and lanes[3].pool.done == 10
     # 40 batches of the 2D pool at 1 ms end well before 10 of the 3D pool at 50 ms;
     # stepped together they would take 40 x 50 ms
     assert max(fin[i] for i in (1, 2, 3)) < 0.5 * fin[0]
@@ -136,9 +150,9 @@ def test_error_in_one_pool_ends_the_stream():
     t0 = time.perf_counter()
     with pytest.raises(RuntimeError, match="pool of dim 3 failed"):
         server(lanes).serve(reqs, realtime=False)
-    # the 2D pool's thread stops at its next batch boundary instead of
+    # the 2D pool's thread stops at its next return to the loop instead of
     # running its 1000 batches
-    assert lanes[2].batches < 100
+    assert lanes[2].pool.done < 200
     assert time.perf_counter() - t0 < 5
 
 
@@ -156,3 +170,30 @@ def test_arrivals_are_respected_in_realtime():
     for r in done:
         assert r.admitted >= r.arrival - 1e-3
     assert {r.rid for r in done} == {0, 1, 2}
+
+
+def test_several_batches_per_run_until_the_next_completion():
+    # requests of 3, 5 and 8 batches admitted together: the loop runs 3
+    # batches, retires one, runs 2, retires one, runs 3 -- no return to the
+    # host between batches that complete nothing
+    lanes = {2: FakeLane(2, 10**6), 3: FakeLane(3, 10**6)}
+    reqs = [request(0, 2, 100, 64 * 3), request(1, 2, 100, 64 * 5 - 10), request(2, 2, 100, 64 * 8)]
+    done = server(lanes).serve(reqs, realtime=False)
+    assert sorted(r.rid for r in done) == [0, 1, 2]
+    assert lanes[2].pool.done == 8
+    # the first call runs one batch (nothing known yet), then 2, 2 and 3
+    assert lanes[2].pool.calls == 4
+
+
+def test_batches_ahead_stops_at_the_next_arrival():
+    lane = FakeLane(2, 10**6)
+    lane.min_left = 64 * 100
+    assert lane.batches_ahead([], 0.0, realtime=True) == SV.MAX_BATCHES_PER_RUN
+    assert lane.batches_ahead([request(0, 2, 10, 64, arrival=5.0)], 0.0, realtime=True) == 1
+    lane.batch_s = 0.1
+    assert lane.batches_ahead([request(0, 2, 10, 64, arrival=1.05)], 0.0, realtime=True) == 10
+    assert lane.batches_ahead([request(0, 2, 10, 64, arrival=1.05)], 0.0, realtime=False) == 32
+    lane.min_left = 64 * 3 - 1
+    assert lane.batches_ahead([], 0.0, realtime=True) == 3
+    lane.min_left = None
+    assert lane.batches_ahead([], 0.0, realtime=True) == 1
