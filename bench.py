@@ -58,13 +58,16 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline budget: steps are added in chunks of 40 until this much time")
     ap.add_argument("--kernel-steps", type=int, default=32)
-    ap.add_argument("--mode", choices=("single", "ensemble", "serve", "copy", "features",
-                                       "observables"),
+    ap.add_argument("--mode", choices=("single", "ensemble", "serve", "request", "copy",
+                                       "features", "observables"),
                     default="single",
                     help="single: 400k-bead system (the headline line); ensemble: 256 "
-                         "replicas batched vs one at a time; serve: request stream; copy: "
+                         "replicas batched vs one at a time; serve: request stream; request: "
+                         "requests of the serving stream (--ids) each alone in a pool; copy: "
                          "a plain 1 GiB device copy kernel, the reference point for HBM "
                          "byte counters (FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--ids", default="4,29",
+                    help="request mode: indices of the serving stream's requests to time")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=10000,
                     help="ensemble: steps of every replica (the configuration's 10^4)")
@@ -748,6 +751,56 @@ def main_copy():
                       "bytes_written_per_launch": 8 * n, "launches": 3}), flush=True)
 
 
+def main_request(args):
+    """Requests of the serving stream (seed 0, --requests, --rate), each alone
+    in a pool at the server's settings (skin --serve-skin of its cutoff,
+    re-sort every --serve-rebuild steps, batches of --batch): 128 warm-up
+    steps, --steps timed steps, then the per-kernel durations over
+    --kernel-steps steps. One JSON line per request."""
+    import copy
+    from gcr_amd import engine as E
+    from gcr_amd import serving as SV
+    reqs = SV.make_stream(seed=0, n_requests=args.requests, rate=args.rate)
+    for i in (int(x) for x in args.ids.split(",")):
+        r = reqs[i]
+        w = copy.copy(r.work)
+        w.phys = copy.copy(r.work.phys)
+        if args.serve_skin > 0:
+            w.phys.skin = args.serve_skin * float(w.phys.wca[3])
+        pool = E.Pool(w.phys.dim, E.GCR_MIXED, w.n, E.replica_cells(w.phys) + 1,
+                      w.phys.n_chains, 1)
+        try:
+            batch = max(args.batch, args.serve_rebuild)
+            pool.set_schedule(batch, args.serve_rebuild)
+            rid = pool.add(w.phys, w.pos, w.tangent, w.chain_id, w.rest_lengths, w.v_tan_angle,
+                           w.v0_per_chain)
+            tv = w.time_vals.copy()
+            tv[7] = max(128, batch)
+            pool.begin(rid, tv)
+            pool.run()
+            tv[0] = pool.status(rid)["dt"]
+            tv[7] = args.steps
+            pool.begin(rid, tv)
+            t0 = time.perf_counter()
+            pool.run()
+            sec = time.perf_counter() - t0
+            st = pool.status(rid)
+            assert st["steps"] == args.steps, st
+            tv[0] = st["dt"]
+            tv[7] = args.kernel_steps
+            pool.begin(rid, tv)
+            kt = pool.time_kernels(args.kernel_steps)
+        finally:
+            pool.close()
+        print(json.dumps({"mode": "request", "request": i, "dim": r.dim, "beads": r.n,
+                          "chain_length": r.n // max(1, w.phys.n_chains), "steps": args.steps,
+                          "bead_steps_per_s": r.n * args.steps / sec,
+                          "ms_per_step": 1e3 * sec / args.steps,
+                          "kernels_ms": {k: float(v) for k, v in kt.items()},
+                          "request_steps": r.steps, "alone_s": r.steps * sec / args.steps}),
+              flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -759,6 +812,8 @@ def main():
         return main_ensemble(args)
     if args.mode == "serve":
         return main_serve(args)
+    if args.mode == "request":
+        return main_request(args)
     if args.mode == "features":
         return main_features(args)
     if args.mode == "observables":
