@@ -129,6 +129,11 @@ def cases():
     # tiles fit (see test_tile_shape_does_not_change_results)
     out["lj3d"] = W.chains("lj3d", 3, 400, 10, phi=0.15, cutoff_factor=W.LJ_25_SIGMA, kappa=5.0,
                            T=0.05, dtmax=5e-4, steps=100, seed=14, skin=0.67)
+    # sparse 3D chains (WCA, skin 0.6, about 0.8 beads per 1.6-cell): 16 cells
+    # per side, so cubes of 7 and 8 cells (and their 9^3 and 10^3 halos) fit in
+    # LDS without a halo axis wrapping onto itself
+    out["sparse3d"] = W.chains("sparse3d", 3, 320, 10, L=25.6, kappa=4.0, vtan=0.1, T=0.01,
+                               steps=100, seed=15, skin=0.6)
     # about 80 beads per cell: a one-cell tile with its halo fits in LDS, a
     # three-cell one does not (see test_tile_shape_does_not_change_results)
     out["dense_cells"] = W.chains("dense_cells", 2, 1300, 1, phi=0.55,
@@ -332,7 +337,8 @@ def test_alone_vs_mixed_pool_bit_identical():
                                         ("list_overflow", ((1, 512), (4, 256))),
                                         ("chains3d", ((1, 256), (2, 512), (6, 1024))),
                                         ("lj3d", ((2, 256), (3, 512), (4, 256))),
-                                        ("list_overflow3d", ((1, 256), (3, 512)))])
+                                        ("list_overflow3d", ((1, 256), (3, 512))),
+                                        ("sparse3d", ((8, 512), (7, 256), (5, 512), (3, 1024)))])
 def test_tile_shape_does_not_change_results(name, sides, precision):
     """Separations are formed from cell-relative positions in every pair path,
     and lists keep the cell-scan order, so the tile edge and the threads per
