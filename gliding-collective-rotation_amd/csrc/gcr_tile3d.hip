@@ -558,9 +558,10 @@ __global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(Po
         const int2 me = hd.me;
         const double* xs = hd.x;
         double acc[3], tang[3];
-        // SRK2 accumulates into F0: FP32 in flight during the pair loop
+        // SRK2 accumulates into F0: loaded after the pair loop, in flight
+        // during the bonded terms (held across the pair loop, its registers
+        // made the FP32 stage spill)
         const bool from_f0 = STAGE == 1 && !(GEN && (P.flags & GCR_F_SRK4));
-        if (early && from_f0) load_vec<3>(a.F0, s, acc);
         // the chain neighbours' positions: FP32, in flight during the pair
         // loop; FP64, after it (the registers)
         ChainNb<3> cn;
@@ -579,10 +580,8 @@ __global__ void __launch_bounds__(NT, sizeof(FT) == 4 ? 4 : 2) k_stage_tile3d(Po
             else
                 pair_sum<3, FT>(Xa, a.META, a.CSTART, xs, me.x, me.y, a.CELL[s], P, f);
         }
-        if (!early) {
-            load_nb();
-            if (from_f0) load_vec<3>(a.F0, s, acc);
-        }
+        if (!early) load_nb();
+        if (from_f0) load_vec<3>(a.F0, s, acc);
         if (!from_f0) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) acc[i] = 0.0;
