@@ -176,6 +176,7 @@ struct gcr_pool {
     int64_t capChk = 0, capBlk = 0, capTile = 0, capChb = 0;
     int ntile = 0;
     int tile_side = 0;     // 0: chosen per replica from the cell occupancy
+    int tile_w = 0, tile_h = 0;  // 2D: a rectangular tile (cells in x, y); 0: tile_side rules
     int n_cu = 256;        // compute units of the device (3D tile sides)
     int tile_threads = 0;  // threads per tile workgroup: 256, 512, 1024, 0 = from the tile sizes
     int nt_launch = BS;    // resolved by rebuild_tables
@@ -453,12 +454,17 @@ int rebuild_tables(gcr_pool* p)
                 hc.push_back(P.chain_off + c);
             }
         if (p->dim == 2) {
+            // a square tile unless the pool sets a rectangle (rows of more
+            // cells in x are longer runs of slots in the halo's loads)
             const int side = pick_tile(P, p->tile_side);
-            P.tile_x = P.tile_y = side;
+            const int tx = p->tile_w > 0 ? std::min(p->tile_w, P.nc) : side;
+            const int ty = p->tile_h > 0 ? std::min(p->tile_h, P.nc) : side;
+            P.tile_x = tx;
+            P.tile_y = ty;
             P.tile_off = (int32_t)(tl.size() / 4);
             std::vector<int32_t> rowmajor;  // (x, y) of the tiles, row by row
-            for (int y = 0; y < P.nc; y += side)
-                for (int x = 0; x < P.nc; x += side) {
+            for (int y = 0; y < P.nc; y += ty)
+                for (int x = 0; x < P.nc; x += tx) {
                     rowmajor.push_back(x);
                     rowmajor.push_back(y);
                 }
@@ -1981,6 +1987,21 @@ int gcr_pool_set_tile(gcr_pool* p, int32_t side)
     if (!p) return fail(GCR_ERR_ARG, "null pool");
     if (side < 0 || side > TILE_MAX_ROWS) return fail(GCR_ERR_ARG, "tile side must be in [0, 16]");
     p->tile_side = side;
+    p->tile_w = p->tile_h = 0;
+    p->tables_dirty = true;
+    return GCR_OK;
+}
+
+int gcr_pool_set_tile_shape(gcr_pool* p, int32_t tile_x, int32_t tile_y)
+{
+    if (!p) return fail(GCR_ERR_ARG, "null pool");
+    if (p->dim != 2) return fail(GCR_ERR_ARG, "rectangular tiles are 2D");
+    const bool clear = tile_x == 0 && tile_y == 0;
+    if (!clear && (tile_x < 1 || tile_x > TILE_MAX_ROWS || tile_y < 1 || tile_y > TILE_MAX_ROWS))
+        return fail(GCR_ERR_ARG, "tile edges must be in [1, 16] (or both 0)");
+    p->tile_w = tile_x;
+    p->tile_h = tile_y;
+    if (!clear) p->tile_side = 0;
     p->tables_dirty = true;
     return GCR_OK;
 }

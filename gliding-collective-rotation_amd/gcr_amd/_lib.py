@@ -69,7 +69,8 @@ EXPORTS = [
     "gcr_replica_bytes", "gcr_replica_cells", "gcr_replica_add", "gcr_replica_remove",
     "gcr_replica_upload", "gcr_replica_begin", "gcr_pool_run", "gcr_replica_status",
     "gcr_replica_download", "gcr_replica_eval_forces", "gcr_pool_time_kernels",
-    "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_threads", "gcr_init_chains",
+    "gcr_pool_destroy", "gcr_pool_set_tile", "gcr_pool_set_tile_shape", "gcr_pool_set_tile_threads",
+    "gcr_init_chains",
     "gcr_replica_order", "gcr_replica_set_physics", "gcr_pool_group", "gcr_replica_set_chunks",
     "gcr_replica_where_active", "gcr_write_table", "gcr_eval_forces_noalloc",
     "gcr_replica_cell_list", "gcr_pool_graph_stats", "gcr_replica_sample", "gcr_replica_series",
@@ -114,6 +115,7 @@ def load():
     L.gcr_pool_time_kernels.argtypes = [vp, i32, dp]
     L.gcr_pool_destroy.argtypes = [vp]
     L.gcr_pool_set_tile.argtypes = [vp, i32]
+    L.gcr_pool_set_tile_shape.argtypes = [vp, i32, i32]
     L.gcr_pool_set_tile_threads.argtypes = [vp, i32]
     L.gcr_replica_order.argtypes = [vp, i32, dp]
     L.gcr_replica_set_physics.argtypes = [vp, i32, C.POINTER(ReplicaDesc), vp, vp]

@@ -49,9 +49,23 @@ def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 51
     the state `warm` steps after the initial one (chains of attractive beads
     cluster within the first hundreds of steps, which lengthens the lists of
     dense tiles); each timing restores that state, so every candidate sees
-    the same one."""
+    the same one. In 2D, after the squares, rectangles of about the best
+    square's area and more cells in x (tile = (x, y)) at its thread count:
+    a halo row of more cells is a longer run of slots to load."""
     pool, rid = _pool_for(w, precision, skin)
     out = []
+
+    def trial(t, nt):
+        pool.set_tile(t)
+        pool.set_tile_threads(nt)
+        _arm(pool, rid, w, 1 << 30)
+        try:
+            pool.time_kernels(4)  # builds tables and warms up
+            kt = pool.time_kernels(steps)
+        except GcrError:
+            return
+        out.append((t, nt, kt["stage0"] + kt["stage1"]))
+
     try:
         if warm > 0:
             pool.set_schedule(64, 8)
@@ -59,18 +73,25 @@ def tune_tiling(w, precision, skin, tiles=(0, 4, 6, 8, 10, 12), threads=(256, 51
             pool.run()
         for t in tiles:
             for nt in threads:
-                pool.set_tile(t)
-                pool.set_tile_threads(nt)
-                _arm(pool, rid, w, 1 << 30)
-                try:
-                    pool.time_kernels(4)  # builds tables and warms up
-                    kt = pool.time_kernels(steps)
-                except GcrError:
-                    continue
-                out.append((t, nt, kt["stage0"] + kt["stage1"]))
+                trial(t, nt)
+        best = min(out, key=lambda r: r[2]) if out else None
+        if w.phys.dim == 2 and best is not None and isinstance(best[0], int) and best[0] > 0:
+            for shape in rectangles(best[0]):
+                trial(shape, best[1])
     finally:
         pool.close()
     return sorted(out, key=lambda r: r[2])
+
+
+def rectangles(s, max_edge=16):
+    """(x, y) tiles of about s x s cells with 1 to 3 more cells in x, or
+    twice as many: y = floor or round of s^2 / x."""
+    shapes = []
+    for x in (s + 1, s + 2, s + 3, 2 * s):
+        for y in (int(s * s / x), int(round(s * s / x))):
+            if 1 <= y < x <= max_edge and (x, y) not in shapes:
+                shapes.append((x, y))
+    return shapes
 
 
 def time_schedule(w, precision, skin, rebuild_every, tile, threads, steps=256, batch=64):

@@ -120,8 +120,12 @@ class Pool:
         check(self._L.gcr_pool_set_schedule(self.h, int(steps_per_batch), int(rebuild_every)))
 
     def set_tile(self, side):
-        """Tile edge of the 2D LDS force kernel (0: automatic)."""
-        check(self._L.gcr_pool_set_tile(self.h, int(side)))
+        """Tile edge of the LDS force kernels (0: automatic), or for 2D a
+        (cells in x, cells in y) rectangle."""
+        if isinstance(side, (tuple, list)):
+            check(self._L.gcr_pool_set_tile_shape(self.h, int(side[0]), int(side[1])))
+        else:
+            check(self._L.gcr_pool_set_tile(self.h, int(side)))
 
     def set_tile_threads(self, threads):
         """Threads per tile workgroup of the 2D force kernels (0, 256, 512 or 1024)."""

@@ -264,6 +264,12 @@ int gcr_pool_graph_stats(gcr_pool* p, double out[4]);
  * set here is clamped to 8 and to the grid. No result bit depends on the
  * tiling. */
 int gcr_pool_set_tile(gcr_pool* p, int32_t side);
+/* 2D only: tiles of tile_x x tile_y cells (each in [1, 16]) for every
+ * replica instead of squares; 0, 0 returns to gcr_pool_set_tile's rule.
+ * Setting a side with gcr_pool_set_tile clears the rectangle. Like the side,
+ * the same grid constants of config.py:147-200 as they bear on speed, with
+ * no effect on any result bit. */
+int gcr_pool_set_tile_shape(gcr_pool* p, int32_t tile_x, int32_t tile_y);
 /* Threads per tile workgroup of the force kernels: 256, 512, 1024, or 0 to
  * pick about one owned bead per thread in 2D, and 512 (FP32) or 256 (FP64)
  * threads per 3D tile. Neither this nor the tile

@@ -332,7 +332,8 @@ def test_alone_vs_mixed_pool_bit_identical():
 
 
 @pytest.mark.parametrize("precision", [E.GCR_FP64, E.GCR_MIXED])
-@pytest.mark.parametrize("name,sides", [("chains_lj_cut", ((1, 256), (2, 512), (5, 1024), (16, 512))),
+@pytest.mark.parametrize("name,sides", [("chains_lj_cut", ((1, 256), (2, 512), (5, 1024), (16, 512),
+                                                          ((9, 7), 256), ((16, 4), 512))),
                                         ("dense_cells", ((1, 256), (3, 512))),
                                         ("list_overflow", ((1, 512), (4, 256))),
                                         ("chains3d", ((1, 256), (2, 512), (6, 1024))),

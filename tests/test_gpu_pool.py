@@ -63,10 +63,15 @@ def test_autotune_picks_a_measured_setting():
                 threads=(256, 512), steps=64)
     print("autotune:", {k: r[k] for k in ("skin", "rebuild_every", "tile", "threads",
                                            "ms_per_step")}, flush=True)
-    assert r["threads"] in (256, 512) and r["tile"] in (0, 2, 4)
+    # the squares, then rectangles of about the best explicit square's area
+    shapes = [t for t in (0, 2, 4)] + [x for t in (2, 4) for x in AT.rectangles(t)]
+    assert r["threads"] in (256, 512) and r["tile"] in shapes
     assert r["rebuild_every"] in (8, 16)
     assert any(abs(r["skin"] - f * w.phys.wca[3]) < 1e-12 for f in (0.2, 0.3))
-    assert len(r["tiling_trials"]) == 6
+    squares = [t for t in r["tiling_trials"] if not isinstance(t["tile"], tuple)]
+    assert len(squares) == 6
+    assert all(t["tile"] in shapes for t in r["tiling_trials"])
+    assert r["tiling_trials"][0]["tile"] == r["tile"]
     assert r["ms_per_step"] == min(t["ms_per_step"] for t in r["schedule_trials"])
 
 
