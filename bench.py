@@ -32,12 +32,20 @@ HBM_PEAK_GBS = 8000.0
 # bead-step in total
 ALG_BYTES = {"stage0": 32, "stage1": 16, "update": 32}
 # HBM bytes per bead per launch of the mixed kernels on this workload, from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (tile edge 8, 256 threads per
-# tile, skin 0.3 cutoff, re-sort every 32 steps), each counter scaled by its
-# reading for a plain 1 GiB copy kernel (`--mode copy`: reads x 2.0, writes x
-# 1.0); profiles/stage_kernels_400k_beads.md holds the passes and commands.
-# Measured again whenever these kernels change.
-TRAFFIC_BYTES = {"stage0": 209.0, "stage1": 144.0, "update": 105.0}
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (16 x 4-cell tiles of 256 threads,
+# the autotuner's pick, skin 0.3 cutoff, re-sort every 32 steps), each
+# counter scaled by its reading for a plain 1 GiB copy kernel (`--mode copy`:
+# reads x 2.0, writes x 1.0); profiles/stage_kernels_400k_beads.md holds the
+# passes and commands. Measured again whenever these kernels change.
+TRAFFIC_BYTES = {"stage0": 208.0, "stage1": 144.0, "update": 105.0}
+
+
+def parse_tile(text):
+    """'8' -> 8, '16x4' -> (16, 4)."""
+    if "x" in text:
+        x, y = text.split("x")
+        return (int(x), int(y))
+    return int(text)
 
 
 def parse():
@@ -48,7 +56,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=64, help="steps per captured graph")
     ap.add_argument("--rebuild", type=int, default=8, help="cell-list rebuild period")
     ap.add_argument("--skin", type=float, default=0.0)
-    ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0: automatic)")
+    ap.add_argument("--tile", default="0",
+                    help="tile edge in cells, or XxY for a 2D rectangle (0: automatic)")
     ap.add_argument("--threads", type=int, default=0, help="threads per tile (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="single: take skin / rebuild / tile / threads from the flags instead of "
@@ -827,7 +836,7 @@ def main():
     for name, prec in (("fp64", E.GCR_FP64), ("mixed", E.GCR_MIXED)):
         if args.no_autotune:
             cfg = dict(skin=args.skin if args.skin > 0 else 0.3 * float(w.phys.wca[3]),
-                       rebuild_every=args.rebuild, tile=args.tile, threads=args.threads)
+                       rebuild_every=args.rebuild, tile=parse_tile(args.tile), threads=args.threads)
         else:
             from gcr_amd import autotune as AT
             t0 = time.perf_counter()
